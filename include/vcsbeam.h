@@ -1,0 +1,716 @@
+/*
+ * vcsbeam.h -- C-ABI of the MI355X-native voltage-beamforming path.
+ *
+ * Drop-in for the part of VCSBeam's public header (include/vcsbeam.h.in in
+ * MWA-vcsbeam) that the voltage-beamforming path uses: the context, forward
+ * fine PFB, tied-array / incoherent beamformer, inverse PFB and VDIF
+ * requantisation entry points keep their names, argument meaning and
+ * "exit on error" behaviour.  Citations are file:line in the reference tree.
+ *
+ * Differences from the reference header, all deliberate:
+ *  - no CUDA/HIP, mwalib, PAL, psrfits, vdifio, hyperbeam or MPI headers are
+ *    included; gpuDoubleComplex / gpuFloatComplex are plain structs with the
+ *    layout of hipDoubleComplex / hipFloatComplex;
+ *  - the mwalib metadata types are replaced by minimal structs carrying only
+ *    the fields the path reads (same field names);
+ *  - device buffers behind the context (d_v, d_S, d_e, ...) use the layouts
+ *    documented at each field; host buffers keep the reference layouts;
+ *  - MI355X extensions (precision MFMA beamformer, device FP64 Jones/phase,
+ *    device-resident inverse PFB, batched launch, beam surrogate) are
+ *    appended at the end of the header and of vcsbeam_context.
+ */
+#ifndef VCSBEAM_H
+#define VCSBEAM_H
+
+#include <math.h>
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+typedef struct { double x, y; } gpuDoubleComplex; /* == hipDoubleComplex */
+typedef struct { float x, y; } gpuFloatComplex;   /* == hipFloatComplex */
+typedef struct { int8_t x, y; } vm_char2;         /* == HIP char2 */
+typedef void *gpuStream_t;                        /* hipStream_t */
+typedef void *gpufftHandle;                       /* opaque plan of the in-house FFT */
+
+#define ERROR_MESSAGE_LEN 1024
+#define MAX_COMMAND_LENGTH 1024
+
+#define SPEED_OF_LIGHT_IN_VACUUM_M_PER_S 299792458.0
+#define MWA_LATITUDE_RADIANS (-0.4660608448386394)   /* -26.703319 deg */
+#define MWA_LONGITUDE_RADIANS (2.0362898668561042)   /* 116.67081524 deg */
+#define MWA_ALTITUDE_METRES 377.827
+
+#define PIBY2 (0.5 * M_PI)
+#define D2R (M_PI / 180.0)
+#define R2D (180.0 / M_PI)
+
+/* ---------------------------------------- mwalib replacement (minimal) -- */
+/* Field names follow mwalib's MetafitsMetadata / VoltageMetadata / Rfinput /
+ * CoarseChannel; only the fields the beamforming path reads are present. */
+
+typedef enum MWAVersion {
+    CorrOldLegacy = 1,
+    CorrLegacy = 2,
+    CorrMWAXv2 = 3,
+    VCSLegacyRecombined = 4,
+    VCSMWAXv2 = 5
+} MWAVersion;
+
+typedef struct Rfinput {
+    uint32_t input;
+    uint32_t ant;
+    uint32_t tile_id;
+    char *tile_name;
+    char *pol;                  /* "X" or "Y" */
+    double electrical_length_m;
+    double north_m;
+    double east_m;
+    double height_m;
+    uint32_t vcs_order;
+    uint32_t subfile_order;
+    bool flagged;
+} Rfinput;
+
+typedef struct CoarseChannel {
+    uint32_t corr_chan_number;
+    uint32_t rec_chan_number;
+    uint32_t gpubox_number;
+    uint32_t chan_width_hz;
+    uint32_t chan_start_hz;
+    uint32_t chan_centre_hz;
+    uint32_t chan_end_hz;
+} CoarseChannel;
+
+typedef struct MetafitsMetadata {
+    uint32_t obs_id;
+    MWAVersion mwa_version;
+    double sched_start_mjd;
+    uint64_t sched_start_gps_time_ms;
+    size_t num_ants;
+    size_t num_rf_inputs;
+    Rfinput *rf_inputs;
+    size_t num_ant_pols;
+    size_t num_visibility_pols;
+    size_t num_metafits_coarse_chans;
+    CoarseChannel *metafits_coarse_chans;
+    uint32_t coarse_chan_width_hz;
+    size_t num_volt_fine_chans_per_coarse;
+    double ra_tile_pointing_deg;
+    double dec_tile_pointing_deg;
+    char *project_id;
+} MetafitsMetadata;
+
+typedef struct VoltageMetadata {
+    MWAVersion mwa_version;
+    uint64_t num_samples_per_voltage_block;
+    uint64_t num_voltage_blocks_per_second;
+    uint64_t voltage_block_size_bytes;
+} VoltageMetadata;
+
+typedef struct MetafitsContext MetafitsContext; /* opaque (mwalib) */
+typedef struct VoltageContext VoltageContext;   /* opaque (mwalib) */
+
+/* ------------------------------------------------------------ performance */
+/* include/vcsbeam.h.in:100-155 */
+#define PERFORMANCE_MAX_NUM_STOPWATCHES 16
+#define PERFORMANCE_MAX_START_STOP 4096
+#define PERFORMANCE_NO_STOPWATCH_FOUND -1
+#define PERFORMANCE_NO_MPI -1
+
+typedef struct logger_stopwatch_t {
+    char *name;
+    char *description;
+    double values[PERFORMANCE_MAX_START_STOP];
+    int nstart_stops;
+    bool running;
+    double total, total_sq;
+} logger_stopwatch;
+
+typedef struct logger_t {
+    double begintime;
+    FILE *fout;
+    logger_stopwatch stopwatches[PERFORMANCE_MAX_NUM_STOPWATCHES];
+    int nstopwatches;
+    int world_rank;
+} logger;
+
+logger *create_logger(FILE *fout, int world_rank);
+void destroy_logger(logger *log);
+void logger_add_stopwatch(logger *log, const char *stopwatch_name, const char *description);
+void logger_start_stopwatch(logger *log, const char *stopwatch_name, bool print_description);
+void logger_stop_stopwatch(logger *log, const char *stopwatch_name);
+void logger_timed_message(logger *log, const char *message);
+void logger_message(logger *log, const char *message);
+void logger_stopwatch_report_stats(logger *log, const char *stopwatch_name);
+void logger_report_all_stats(logger *log);
+
+/* ------------------------------------------------------------ calibration */
+/* include/vcsbeam.h.in:185-202 (loaders are outside the path; D is supplied
+ * directly in vm->D) */
+typedef struct calibration_t {
+    char *metafits;
+    char *caldir;
+    int cal_type;
+    char *ref_ant;
+    bool keep_cross_terms;
+    double phase_offset;
+    double phase_slope;
+    bool use_bandpass;
+    bool custom_pq_correction;
+    char *flags_file;
+    int nflags;
+    char **flagged_tilenames;
+} calibration;
+
+/* ------------------------------------------------------------------ jones */
+#define NSTOKES 4
+
+/* Index macros, identical to include/vcsbeam.h.in:234-268 */
+#define v_IDX(s, c, i, nc, ni) ((s) * (ni) * (nc) + (c) * (ni) + (i))
+#define VSPVB 64000
+#define vMWAX_IDX(s, i, ni) ((((s) / VSPVB) * (ni) + (i)) * VSPVB + ((s) % VSPVB))
+#define D_IDX(a, c, p1, p2, nc, npol) \
+    ((a) * (npol) * (npol) * (nc) + (c) * (npol) * (npol) + (p1) * (npol) + (p2))
+#define J_IDX(p, a, c, p1, p2, nant, nc, npol)                                   \
+    ((p) * (nant) * (npol) * (npol) * (nc) + (a) * (npol) * (npol) * (nc) +     \
+     (c) * (npol) * (npol) + (p1) * (npol) + (p2))
+#define Jv_IDX(p, s, c, a, ns, nc, na) \
+    ((p) * (na) * (nc) * (ns) + (s) * (na) * (nc) + (c) * (na) + (a))
+#define B_IDX(p, s, c, pol, ns, nc, npol) \
+    ((p) * (npol) * (nc) * (ns) + (s) * (npol) * (nc) + (c) * (npol) + (pol))
+#define C_IDX(p, s, st, c, ns, nst, nc) \
+    ((p) * ((nc) * (nst) * (ns)) + (s) * ((nc) * (nst)) + (st) * (nc) + (c))
+#define I_IDX(s, c, nc) ((s) * (nc) + (c))
+#define PHI_IDX(p, a, c, na, nc) ((p) * (nc) * (na) + (a) * (nc) + (c))
+#define PB_IDX(p, a, pol, na, npol) ((p) * (na) * (npol) + (a) * (npol) + (pol))
+
+/* 2x2 complex matrix operations, src/jones.c:201-578 */
+void cp2x2(gpuDoubleComplex *Min, gpuDoubleComplex *Mout);
+void inv2x2(gpuDoubleComplex *Min, gpuDoubleComplex *Mout);
+void inv2x2d(double *Min, double *Mout);
+void inv2x2S(gpuDoubleComplex *Min, gpuDoubleComplex *Mout);
+void mult2x2d(gpuDoubleComplex *M1, gpuDoubleComplex *M2, gpuDoubleComplex *Mout);
+void mult2x2d_RxC(double *M1, gpuDoubleComplex *M2, gpuDoubleComplex *Mout);
+void mult2x2d_CxR(gpuDoubleComplex *M1, double *M2, gpuDoubleComplex *Mout);
+void conj2x2(gpuDoubleComplex *M, gpuDoubleComplex *Mout);
+double norm2x2(gpuDoubleComplex *M, gpuDoubleComplex *Mout);
+void reverse2x2(gpuDoubleComplex *M, gpuDoubleComplex *Mout);
+void swaprows2x2(gpuDoubleComplex *M, gpuDoubleComplex *Mout);
+void swapcols2x2(gpuDoubleComplex *M, gpuDoubleComplex *Mout);
+bool is2x2zero(gpuDoubleComplex *M);
+void calc_hermitian(gpuDoubleComplex *M, gpuDoubleComplex *H);
+void calc_coherency_matrix(gpuDoubleComplex *M, gpuDoubleComplex *c);
+void fprintf_complex_matrix(FILE *fout, gpuDoubleComplex *M);
+
+/* ----------------------------------------------------------------- filter */
+typedef enum filter_type_t { ANALYSIS_FILTER, SYNTHESIS_FILTER } filter_type;
+
+typedef struct pfb_filter_t {
+    double *coeffs;
+    int ncoeffs;
+    int ntaps;
+    int nchans;
+    gpuDoubleComplex *twiddles;
+    filter_type type;
+} pfb_filter;
+
+gpuDoubleComplex *roots_of_unity(int N);
+
+/* --------------------------------------------------------------- metadata */
+typedef enum vcsbeam_datatype_t {
+    VM_INT4,  /* legacy 4+4-bit [s][c][i] (reference) */
+    VM_DBL,   /* complex double [s][c][i] (reference) */
+    VM_FLT    /* MI355X: complex float [c][i][s], i in VCS order */
+} vcsbeam_datatype;
+
+typedef enum vm_error_t {
+    VM_SUCCESS,
+    VM_END_OF_DATA,
+    VM_READ_BUFFER_NOT_SET,
+    VM_READ_BUFFER_LOCKED
+} vm_error;
+
+typedef struct geometric_delays_t {
+    gpuDoubleComplex *phi;   /* host [p][a][c] (PHI_IDX) */
+    gpuDoubleComplex *d_phi; /* device, same layout */
+    uintptr_t npointings;
+    uintptr_t nant;
+    uintptr_t nchan;
+    double *chan_freqs_hz;
+    MetafitsMetadata *obs_metadata;
+} geometric_delays;
+
+typedef struct primary_beam_t {
+    gpuDoubleComplex *B;     /* host [p][a][2][2] (PB_IDX with npol=4) */
+    void *beam;              /* FEEBeam* in the reference (hyperbeam) */
+    uint32_t **delays;
+    double **amps;
+    uintptr_t npointings;
+    uintptr_t nant;
+    uintptr_t npol;
+    uint32_t freq_hz;
+    MetafitsMetadata *obs_metadata;
+} primary_beam;
+
+typedef struct beam_geom_t {
+    double mean_ra;
+    double mean_dec;
+    double az;
+    double el;
+    double lmst;
+    double fracmjd;
+    double intmjd;
+    double unit_N;
+    double unit_E;
+    double unit_H;
+} beam_geom;
+
+/* Minimal stand-in for the PSRFITS splice struct: only the per-subint arrays
+ * that vmSendSToFits fills (src/form_beam.cpp:666-671). */
+typedef struct vm_psrfits_sub_t {
+    float *dat_offsets;
+    float *dat_scales;
+    uint8_t *data;
+    int bytes_per_subint;
+} vm_psrfits_sub;
+typedef struct vm_psrfits_t { vm_psrfits_sub sub; } vm_psrfits;
+typedef struct mpi_psrfits_t {
+    vm_psrfits coarse_chan_pf;
+    int writer_id;
+} mpi_psrfits;
+
+typedef struct host_buffer_t {
+    void *buffer;
+    size_t buffer_size;
+    void *read_ptr;
+    size_t read_size;
+    void *copy_from_ptr;
+    void *copy_to_ptr;
+    size_t copy_size;
+    bool locked;
+} host_buffer;
+
+typedef struct device_buffer_t {
+    void *buffer;
+    size_t buffer_size;
+} device_buffer;
+
+/* ------------------------------------------------------------ forward PFB */
+/* include/vcsbeam.h.in:449-522 */
+#define CLIP(x, max) ((x) < -(max) ? -(max) : (x) > (max)-1 ? (max)-1 : (x))
+#define INT_TO_UINT8(x) ((x) < 0 ? CLIP(x, 8) + 0x10 : CLIP(x, 8))
+#define DEMOTE(x) (INT_TO_UINT8((int)round(x)))
+#define PACK_NIBBLES(r, i) ((DEMOTE(r) << 4) + DEMOTE(i))
+
+typedef enum pfb_flags_t {
+    PFB_MALLOC_HOST_INPUT = 0x01,
+    PFB_MALLOC_HOST_OUTPUT = 0x02,
+    PFB_MALLOC_DEVICE_INPUT = 0x04,
+    PFB_MALLOC_DEVICE_OUTPUT = 0x08,
+    PFB_MALLOC_ALL = 0x0F,
+    PFB_TYPE_MASK = 0xF0,
+    PFB_COMPLEX_INT4 = 0x10,
+    PFB_COMPLEX_FLOAT64 = 0x20,
+    PFB_COMPLEX_FLOAT32 = 0x40, /* MI355X: device output VM_FLT [c][i][s] */
+    PFB_IMAG_PART_FIRST = 0x100,
+    PFB_EMULATE_FPGA = 0x200,
+    PFB_STAGED = 0x400,         /* MI355X: run WOLA/round/FFT/pack as the
+                                   reference's four separate passes */
+    PFB_SMART = 0x21D,
+    PFB_FULL_PRECISION = 0x2D
+} pfb_flags;
+
+typedef struct forward_pfb_t {
+    vm_char2 *d_htr_data;
+    void *vcs_data;              /* host output, reference layout/format */
+    void *d_vcs_data;            /* device output: VM_FLT [c][i][s] complex
+                                    float, or VM_INT4 legacy [s][c][i] */
+    size_t d_htr_size;
+    size_t htr_stride;
+    size_t vcs_size;
+    size_t d_vcs_size;
+    size_t vcs_stride;
+    size_t char2s_per_second;
+    size_t bytes_per_block;
+    size_t chunks_per_second;
+    size_t chunk;
+    int ninputs_per_cufft_batch;
+    int cufft_batch_size;
+    gpuFloatComplex *d_weighted_overlap_add; /* staged mode: int2 / float2
+                                                [m][i][n] as in the reference */
+    size_t weighted_overlap_add_size;
+    int *filter_coeffs;
+    int *d_filter_coeffs;
+    int nspectra;
+    int nspectra_per_chunk;
+    int M;
+    int K;
+    int I;
+    int P;
+    int *i_output_idx;
+    int *d_i_output_idx;
+    pfb_flags flags;
+    gpufftHandle plan;
+    /* MI355X additions */
+    void *d_flt_out;             /* complex float [c][i][s] fused-kernel output
+                                    when d_vcs_data holds another format */
+    int filter_fits_int16;       /* 1 -> packed dot2 WOLA path */
+} forward_pfb;
+
+/* ---------------------------------------------------------------- context */
+typedef struct vm_device_state vm_device_state; /* library-private */
+
+/* Reader callback replacing mwalib_voltage_context_read_second: fill
+ * `nbytes` bytes at `dst` with one second of voltages for gps_second and
+ * coarse channel coarse_chan_idx; return 0 on success. */
+typedef int (*vm_read_second_fn)(void *user, uint64_t gps_second,
+                                 int coarse_chan_idx, void *dst, size_t nbytes);
+
+typedef enum vm_beamform_form_t {
+    VM_BF_FP32 = 0,          /* plain FP32 drop-in (default) */
+    VM_BF_MFMA_FP16 = 1,     /* FP16 voltages, split hi+lo FP16 weights, MFMA */
+    VM_BF_MFMA_BF16 = 2      /* BF16 voltages, split hi+lo BF16 weights, MFMA */
+} vm_beamform_form;
+
+typedef struct vcsbeam_context_t {
+    bool use_mpi;
+    int mpi_size;
+    int mpi_rank;
+    int ncoarse_chans;
+    int coarse_chan_idx;
+    int writer;
+
+    MetafitsContext *obs_context;
+    MetafitsMetadata *obs_metadata;
+    VoltageContext *vcs_context;
+    VoltageMetadata *vcs_metadata;
+    MetafitsContext *cal_context;
+    MetafitsMetadata *cal_metadata;
+    MetafitsContext *obs_context_legacy;
+    MetafitsMetadata *obs_metadata_legacy;
+
+    char *datadir;
+    char **filenames;
+    int nfiles;
+    unsigned int nfiletimes;
+    unsigned int seconds_per_file;
+
+    calibration cal;
+
+    double *ras_hours;
+    double *decs_degs;
+
+    pfb_filter *analysis_filter;
+    pfb_filter *synth_filter;
+
+    forward_pfb *fpfb;
+
+    host_buffer *v;
+    void *d_v;                 /* device voltages (see vcsbeam_datatype) */
+    uintptr_t v_size_bytes;
+    uintptr_t d_v_size_bytes;
+
+    void *S, *d_S;             /* host S: [p][s][st][c] float (C_IDX);
+                                  device d_S: [p][st][c][s] float */
+    uintptr_t S_size_bytes;
+    uintptr_t d_S_size_bytes;
+
+    gpuDoubleComplex *e, *d_e; /* host e: [p][s][c][pol] complex double;
+                                  device d_e: [p][c][s][pol] complex float */
+    uintptr_t e_size_bytes;
+    uintptr_t d_e_size_bytes;
+
+    gpuDoubleComplex *J, *d_J; /* [p][a][c][2][2] complex double, both */
+    uintptr_t J_size_bytes;
+    uintptr_t d_J_size_bytes;
+
+    gpuDoubleComplex *Jv_P, *d_Jv_P; /* unused: J^-1 v is fused into the */
+    gpuDoubleComplex *Jv_Q, *d_Jv_Q; /* beamformer (kept for layout)      */
+    uintptr_t Jv_size_bytes;
+    uintptr_t d_Jv_size_bytes;
+
+    gpuDoubleComplex *D, *d_D; /* [a][c][2][2] complex double, both */
+    uintptr_t D_size_bytes;
+    uintptr_t d_D_size_bytes;
+
+    uint32_t *polP_idxs, *d_polP_idxs;
+    uint32_t *polQ_idxs, *d_polQ_idxs;
+    uintptr_t pol_idxs_size_bytes;
+    uintptr_t d_pol_idxs_size_bytes;
+
+    geometric_delays gdelays;
+    primary_beam pb;
+
+    unsigned int npointing;
+
+    uintptr_t max_gpu_mem_bytes;
+    uint32_t chunks_per_second;
+
+    int num_coarse_chans_to_process;
+    int *coarse_chan_idxs_to_process;
+    int *cal_coarse_chan_idxs_to_process;
+
+    int num_gps_seconds_to_process;
+    uint32_t *gps_seconds_to_process;
+
+    bool output_fine_channels;
+    bool output_coarse_channels;
+
+    size_t current_gps_idx;
+
+    bool do_forward_pfb;
+    bool do_inverse_pfb;
+
+    int sample_rate;
+    int fine_sample_rate;
+    int bytes_per_second;
+    int num_not_flagged;
+    int nchan;
+    int nfine_chan;
+
+    int out_nstokes;
+
+    uint8_t vhdr[32];          /* vdif_header (32-byte VDIF frame header) */
+    struct vdifinfo *vf;
+
+    float *d_offsets, *offsets;
+    float *d_scales, *scales;
+    uint8_t *d_Cscaled, *Cscaled;
+
+    size_t offsets_size;
+    size_t scales_size;
+    size_t Cscaled_size;
+
+    int chunk_to_load;
+    vcsbeam_datatype datatype;
+
+    gpuStream_t *streams;
+
+    logger *log;
+    char log_message[MAX_COMMAND_LENGTH];
+    char error_message[ERROR_MESSAGE_LEN];
+
+    /* ---- MI355X extensions ---- */
+    vm_device_state *dev;        /* device-side state owned by the library */
+    vm_beamform_form beamform_form;
+    vm_read_second_fn read_second;
+    void *read_second_user;
+    float *bf_err;               /* optional host [p][st] norm-wise errors
+                                    filled by vmBeamformErrorReport */
+} vcsbeam_context;
+
+#define NO_ANTENNA_FOUND -1
+
+/* -------------------------------------------- context and buffer set-up -- */
+/* include/vcsbeam.h.in:667-856 */
+void vmCheckError(vm_error err);
+vcsbeam_context *vmInit(bool use_mpi);
+void destroy_vcsbeam_context(vcsbeam_context *vm);
+void vmSetOutputChannelisation(vcsbeam_context *vm, bool out_fine, bool out_coarse);
+
+void vmMallocVHost(vcsbeam_context *vm);
+void vmMallocVDevice(vcsbeam_context *vm);
+void vmFreeVHost(vcsbeam_context *vm);
+void vmFreeVDevice(vcsbeam_context *vm);
+void vmMallocJVHost(vcsbeam_context *vm);
+void vmMallocJVDevice(vcsbeam_context *vm);
+void vmFreeJVHost(vcsbeam_context *vm);
+void vmFreeJVDevice(vcsbeam_context *vm);
+void vmMallocEHost(vcsbeam_context *vm);
+void vmMallocEDevice(vcsbeam_context *vm);
+void vmFreeEHost(vcsbeam_context *vm);
+void vmFreeEDevice(vcsbeam_context *vm);
+void vmMallocSHost(vcsbeam_context *vm);
+void vmMallocSDevice(vcsbeam_context *vm);
+void vmFreeSHost(vcsbeam_context *vm);
+void vmFreeSDevice(vcsbeam_context *vm);
+void vmMallocJHost(vcsbeam_context *vm);
+void vmMallocJDevice(vcsbeam_context *vm);
+void vmFreeJHost(vcsbeam_context *vm);
+void vmFreeJDevice(vcsbeam_context *vm);
+void vmMallocDHost(vcsbeam_context *vm);
+void vmMallocDDevice(vcsbeam_context *vm);
+void vmFreeDHost(vcsbeam_context *vm);
+void vmFreeDDevice(vcsbeam_context *vm);
+void vmMallocPQIdxsHost(vcsbeam_context *vm);
+void vmMallocPQIdxsDevice(vcsbeam_context *vm);
+void vmFreePQIdxsHost(vcsbeam_context *vm);
+void vmFreePQIdxsDevice(vcsbeam_context *vm);
+
+void vmSetMaxGPUMem(vcsbeam_context *vm, int nchunks);
+void vmPushChunk(vcsbeam_context *vm);
+void vmPushJ(vcsbeam_context *vm);
+void vmCreatePrimaryBeam(vcsbeam_context *vm);
+void vmCreateGeometricDelays(vcsbeam_context *vm);
+void vmCreateCudaStreams(vcsbeam_context *vm);
+void vmDestroyCudaStreams(vcsbeam_context *vm);
+void vmCreateStatistics(vcsbeam_context *vm, mpi_psrfits *mpfs);
+void vmDestroyStatistics(vcsbeam_context *vm);
+void vmSetPolIdxLists(vcsbeam_context *vm);
+void vmCalcJ(vcsbeam_context *vm);
+void vmSetNumPointings(vcsbeam_context *vm, unsigned int npointings);
+void vmSetNumNotFlaggedRFInputs(vcsbeam_context *vm);
+
+void vmLoadFilter(vcsbeam_context *vm, char *filtername, filter_type type, int nchans);
+void free_pfb_filter(pfb_filter *filter);
+void vmScaleFilterCoeffs(vcsbeam_context *vm, filter_type type, double scale_factor);
+
+host_buffer *vmInitReadBuffer(size_t read_size, size_t margin_size);
+vm_error vmReadBufferCopyMargin(host_buffer *rb);
+void vmFreeReadBuffer(host_buffer *rb);
+vm_error vmReadNextSecond(vcsbeam_context *vm);
+void vmReportPerformanceStats(vcsbeam_context *vm);
+
+/* ------------------------------------------------------------- geometry */
+/* include/vcsbeam.h.in:878-922 */
+void vmCalcPhi(vcsbeam_context *vm, beam_geom *beam_geom_vals);
+void calc_geometric_delays(beam_geom *beam_geom_vals, uint32_t freq_hz,
+                           MetafitsMetadata *obs_metadata, gpuDoubleComplex *phi);
+void free_geometric_delays(geometric_delays *gdelays);
+void vmPushPhi(vcsbeam_context *vm);
+double calc_array_factor(MetafitsMetadata *obs_metadata, uint32_t freq_hz,
+                         beam_geom *bg1, beam_geom *bg2);
+
+/* ------------------------------------------------------------ forward PFB */
+/* include/vcsbeam.h.in:924-935 */
+void vmInitForwardPFB(vcsbeam_context *vm, int M, pfb_flags flags);
+void vmUploadForwardPFBChunk(vcsbeam_context *vm);
+void vmWOLAChunk(vcsbeam_context *vm);
+void vmFPGARoundingChunk(vcsbeam_context *vm);
+void vmFFTChunk(vcsbeam_context *vm);
+void vmPackChunk(vcsbeam_context *vm);
+void vmDownloadForwardPFBChunk(vcsbeam_context *vm);
+void vmExecuteForwardPFB(vcsbeam_context *vm);
+void vmWritePFBOutputToFile(vcsbeam_context *vm);
+void vmFreeForwardPFB(forward_pfb *fpfb);
+
+/* ------------------------------------------------------------ inverse PFB */
+/* include/vcsbeam.h.in:952-979 */
+struct gpu_ipfb_arrays {
+    int ntaps;
+    int in_size;
+    int ft_size;
+    int out_size;
+    float *in_real, *in_imag;   /* unused by this implementation */
+    float *ft_real, *ft_imag;   /* host copies of the filter (polyphase) */
+    float *d_in_real, *d_in_imag; /* d_in_real: device complex staging */
+    float *d_ft_real, *d_ft_imag; /* d_ft_real: device polyphase filter */
+    float *d_out;
+};
+
+void cu_invert_pfb(gpuDoubleComplex *data_buffer_fine, int file_no, int npointing,
+                   int nsamples, int nchan, int npol, int sizeof_buffer,
+                   struct gpu_ipfb_arrays *g, float *data_buffer_uvdif);
+void cu_load_ipfb_filter(pfb_filter *filter, struct gpu_ipfb_arrays *g);
+void malloc_ipfb(struct gpu_ipfb_arrays *g, pfb_filter *filter, int nsamples,
+                 int npol, int npointing);
+void free_ipfb(struct gpu_ipfb_arrays *g);
+
+/* -------------------------------------------------------------- form beam */
+/* include/vcsbeam.h.in:997-1048 */
+#define REAL_NIBBLE_TO_UINT8(X) (((X) >> 4) & 0xf)
+#define IMAG_NIBBLE_TO_UINT8(X) ((X)&0xf)
+#define UINT8_TO_INT(X) ((X) >= 0x8 ? (signed int)(X)-0x10 : (signed int)(X))
+
+void cu_form_incoh_beam(uint8_t *data, uint8_t *d_data, size_t data_size,
+                        float *d_incoh, unsigned int nsample, int nchan, int ninput,
+                        float *offsets, float *d_offsets, float *scales,
+                        float *d_scales, uint8_t *Iscaled, uint8_t *d_Iscaled,
+                        size_t Iscaled_size);
+void vmApplyJChunk(vcsbeam_context *vm);
+void vmBeamformChunk(vcsbeam_context *vm);
+void vmBeamformSecond(vcsbeam_context *vm);
+void vmPullE(vcsbeam_context *vm);
+void vmPullS(vcsbeam_context *vm);
+void prepare_data_buffer_fine(gpuDoubleComplex *data_buffer_fine, vcsbeam_context *vm,
+                              uintptr_t timestep_idx);
+void vmSendSToFits(vcsbeam_context *vm, mpi_psrfits *mpfs);
+float *create_pinned_data_buffer(size_t size);
+void vmPushPolIdxLists(vcsbeam_context *vm);
+gpuDoubleComplex *create_data_buffer_fine(int npointing, int nsamples, int nchan, int npol);
+void allocate_input_output_arrays(void **data, void **d_data, size_t size);
+void free_input_output_arrays(void *data, void *d_data);
+
+/* ------------------------------------------------------------------- VDIF */
+/* src/beam_vdif.c:29-38, 254-260 */
+void float2int8_trunc(float *f, int n, float min, float max, int8_t *i);
+void to_offset_binary(int8_t *i, int n);
+
+/* ======================================================================
+ * MI355X extensions -- no counterpart in the reference header.
+ * ====================================================================== */
+
+/* Bind metadata supplied by the caller (replaces vmLoadObsMetafits +
+ * vmBindObsData's mwalib glue, src/metadata.c:166-233): sets the shorthand
+ * sizes and allocates the read buffer exactly as vmBindObsData does. */
+void vmBindObsMetadata(vcsbeam_context *vm, MetafitsMetadata *obs_metadata,
+                       VoltageMetadata *vcs_metadata, int coarse_chan_idx,
+                       uint32_t first_gps_second, int num_gps_seconds);
+
+/* vmReadNextSecond with the second taken from memory instead of a file. */
+vm_error vmReadNextSecondFrom(vcsbeam_context *vm, const void *src, size_t nbytes);
+
+/* In-memory filter (replaces reading RUNTIME_DIR/<name>.dat). */
+void vmSetFilter(vcsbeam_context *vm, const double *coeffs, int ncoeffs,
+                 filter_type type, int nchans);
+
+/* Device FP64: phi from tile positions and pointing unit vectors, with
+ * argument reduction in cycles; result in gdelays.d_phi (and not on host). */
+void vmCalcPhiDevice(vcsbeam_context *vm, beam_geom *beam_geom_vals);
+/* Device FP64: J^-1 = F (D B)^-1 from d_D and pb.B; result in d_J. */
+void vmCalcJDevice(vcsbeam_context *vm);
+/* Upload vm->D to d_D. */
+void vmPushD(vcsbeam_context *vm);
+
+/* Precision MFMA beamformer for one chunk (vm->beamform_form selects FP16 or
+ * BF16); vmBeamformChunk dispatches here when beamform_form != VM_BF_FP32. */
+void vmBeamformChunkMFMA(vcsbeam_context *vm);
+
+/* Device-resident inverse PFB of pointing 0 straight from d_e (no host
+ * round trip); output complex float [n][pol] on the device, copied to
+ * data_buffer_vdif when it is not NULL. */
+void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g,
+                       int file_no, float *data_buffer_vdif);
+
+/* Device VDIF requantisation of n floats (float2int8_trunc + offset binary). */
+void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n,
+                         gpuStream_t stream);
+
+/* Synchronise the context's device work. */
+void vmSynchronize(vcsbeam_context *vm);
+/* The HIP stream all context kernels are issued on. */
+gpuStream_t vmGetStream(vcsbeam_context *vm);
+
+/* Device memory, copies and event timing for callers without a GPU runtime
+ * binding of their own (exit on error like every other entry point). */
+void *vmDeviceMalloc(size_t bytes);
+void vmDeviceFree(void *p);
+void vmMemcpyHtoD(void *dst, const void *src, size_t bytes);
+void vmMemcpyDtoH(void *dst, const void *src, size_t bytes);
+void vmMemsetDevice(void *dst, int value, size_t bytes);
+void vmDeviceSynchronize(void);
+int vmDeviceCount(void);
+void *vmEventCreate(void);
+void vmEventRecord(void *ev, gpuStream_t stream);
+float vmEventElapsedMs(void *start, void *stop);
+void vmEventDestroy(void *ev);
+
+/* Library build/version string. */
+const char *vmLibraryVersion(void);
+
+/* Struct layout probe for foreign-function mirrors: writes sizeof() and the
+ * offsets of the fields listed in the implementation into out[0..n). */
+int vmStructLayout(const char *name, size_t *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VCSBEAM_H */

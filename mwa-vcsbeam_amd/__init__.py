@@ -1,0 +1,385 @@
+"""ctypes binding of the MI355X voltage-beamforming C-ABI (``include/vcsbeam.h``).
+
+This module is plumbing for the test suite and the benchmark: it loads the
+in-tree ``_lib/libvcsbeam_amd.so`` (HIP kernels + host C), mirrors the C
+structs the callers touch, and declares the exported entry points.  It never
+falls back to another implementation: if the library is missing it raises.
+
+The directory name contains a hyphen, so load it with :func:`importlib`::
+
+    spec = importlib.util.spec_from_file_location("vcsbeam_amd", ".../mwa-vcsbeam_amd/__init__.py")
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "_lib")
+LIBPATH = os.path.join(LIBDIR, "libvcsbeam_amd.so")
+SYNTHPATH = os.path.join(LIBDIR, "libvmsynth.so")
+RUNTIME_DIR = os.path.join(HERE, "pfb_filter")
+
+# ------------------------------------------------------------------ enums --
+VCSLegacyRecombined = 4
+VCSMWAXv2 = 5
+VM_INT4, VM_DBL, VM_FLT = 0, 1, 2
+VM_SUCCESS, VM_END_OF_DATA, VM_READ_BUFFER_NOT_SET, VM_READ_BUFFER_LOCKED = 0, 1, 2, 3
+ANALYSIS_FILTER, SYNTHESIS_FILTER = 0, 1
+PFB_MALLOC_HOST_INPUT = 0x01
+PFB_MALLOC_HOST_OUTPUT = 0x02
+PFB_MALLOC_DEVICE_INPUT = 0x04
+PFB_MALLOC_DEVICE_OUTPUT = 0x08
+PFB_COMPLEX_INT4 = 0x10
+PFB_COMPLEX_FLOAT64 = 0x20
+PFB_COMPLEX_FLOAT32 = 0x40
+PFB_IMAG_PART_FIRST = 0x100
+PFB_EMULATE_FPGA = 0x200
+PFB_STAGED = 0x400
+PFB_SMART = 0x21D
+PFB_FULL_PRECISION = 0x2D
+VM_BF_FP32, VM_BF_MFMA_FP16, VM_BF_MFMA_BF16 = 0, 1, 2
+MWA_ALTITUDE_METRES = 377.827
+SPEED_OF_LIGHT = 299792458.0
+
+
+# ---------------------------------------------------------------- structs --
+class Rfinput(C.Structure):
+    _fields_ = [("input", C.c_uint32), ("ant", C.c_uint32), ("tile_id", C.c_uint32),
+                ("tile_name", C.c_char_p), ("pol", C.c_char_p),
+                ("electrical_length_m", C.c_double), ("north_m", C.c_double),
+                ("east_m", C.c_double), ("height_m", C.c_double),
+                ("vcs_order", C.c_uint32), ("subfile_order", C.c_uint32), ("flagged", C.c_bool)]
+
+
+class CoarseChannel(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("corr_chan_number", "rec_chan_number", "gpubox_number",
+                                          "chan_width_hz", "chan_start_hz", "chan_centre_hz",
+                                          "chan_end_hz")]
+
+
+class MetafitsMetadata(C.Structure):
+    _fields_ = [("obs_id", C.c_uint32), ("mwa_version", C.c_int), ("sched_start_mjd", C.c_double),
+                ("sched_start_gps_time_ms", C.c_uint64), ("num_ants", C.c_size_t),
+                ("num_rf_inputs", C.c_size_t), ("rf_inputs", C.POINTER(Rfinput)),
+                ("num_ant_pols", C.c_size_t), ("num_visibility_pols", C.c_size_t),
+                ("num_metafits_coarse_chans", C.c_size_t),
+                ("metafits_coarse_chans", C.POINTER(CoarseChannel)),
+                ("coarse_chan_width_hz", C.c_uint32), ("num_volt_fine_chans_per_coarse", C.c_size_t),
+                ("ra_tile_pointing_deg", C.c_double), ("dec_tile_pointing_deg", C.c_double),
+                ("project_id", C.c_char_p)]
+
+
+class VoltageMetadata(C.Structure):
+    _fields_ = [("mwa_version", C.c_int), ("num_samples_per_voltage_block", C.c_uint64),
+                ("num_voltage_blocks_per_second", C.c_uint64),
+                ("voltage_block_size_bytes", C.c_uint64)]
+
+
+class Calibration(C.Structure):
+    _fields_ = [("metafits", C.c_void_p), ("caldir", C.c_void_p), ("cal_type", C.c_int),
+                ("ref_ant", C.c_void_p), ("keep_cross_terms", C.c_bool),
+                ("phase_offset", C.c_double), ("phase_slope", C.c_double),
+                ("use_bandpass", C.c_bool), ("custom_pq_correction", C.c_bool),
+                ("flags_file", C.c_void_p), ("nflags", C.c_int), ("flagged_tilenames", C.c_void_p)]
+
+
+class PfbFilter(C.Structure):
+    _fields_ = [("coeffs", C.POINTER(C.c_double)), ("ncoeffs", C.c_int), ("ntaps", C.c_int),
+                ("nchans", C.c_int), ("twiddles", C.c_void_p), ("type", C.c_int)]
+
+
+class GeometricDelays(C.Structure):
+    _fields_ = [("phi", C.c_void_p), ("d_phi", C.c_void_p), ("npointings", C.c_size_t),
+                ("nant", C.c_size_t), ("nchan", C.c_size_t),
+                ("chan_freqs_hz", C.POINTER(C.c_double)), ("obs_metadata", C.c_void_p)]
+
+
+class PrimaryBeam(C.Structure):
+    _fields_ = [("B", C.c_void_p), ("beam", C.c_void_p), ("delays", C.c_void_p),
+                ("amps", C.c_void_p), ("npointings", C.c_size_t), ("nant", C.c_size_t),
+                ("npol", C.c_size_t), ("freq_hz", C.c_uint32), ("obs_metadata", C.c_void_p)]
+
+
+class BeamGeom(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("mean_ra", "mean_dec", "az", "el", "lmst", "fracmjd",
+                                          "intmjd", "unit_N", "unit_E", "unit_H")]
+
+
+class HostBuffer(C.Structure):
+    _fields_ = [("buffer", C.c_void_p), ("buffer_size", C.c_size_t), ("read_ptr", C.c_void_p),
+                ("read_size", C.c_size_t), ("copy_from_ptr", C.c_void_p),
+                ("copy_to_ptr", C.c_void_p), ("copy_size", C.c_size_t), ("locked", C.c_bool)]
+
+
+class ForwardPfb(C.Structure):
+    _fields_ = [("d_htr_data", C.c_void_p), ("vcs_data", C.c_void_p), ("d_vcs_data", C.c_void_p),
+                ("d_htr_size", C.c_size_t), ("htr_stride", C.c_size_t), ("vcs_size", C.c_size_t),
+                ("d_vcs_size", C.c_size_t), ("vcs_stride", C.c_size_t),
+                ("char2s_per_second", C.c_size_t), ("bytes_per_block", C.c_size_t),
+                ("chunks_per_second", C.c_size_t), ("chunk", C.c_size_t),
+                ("ninputs_per_cufft_batch", C.c_int), ("cufft_batch_size", C.c_int),
+                ("d_weighted_overlap_add", C.c_void_p), ("weighted_overlap_add_size", C.c_size_t),
+                ("filter_coeffs", C.c_void_p), ("d_filter_coeffs", C.c_void_p),
+                ("nspectra", C.c_int), ("nspectra_per_chunk", C.c_int), ("M", C.c_int),
+                ("K", C.c_int), ("I", C.c_int), ("P", C.c_int), ("i_output_idx", C.c_void_p),
+                ("d_i_output_idx", C.c_void_p), ("flags", C.c_int), ("plan", C.c_void_p),
+                ("d_flt_out", C.c_void_p), ("filter_fits_int16", C.c_int)]
+
+
+class PsrfitsSub(C.Structure):
+    _fields_ = [("dat_offsets", C.c_void_p), ("dat_scales", C.c_void_p), ("data", C.c_void_p),
+                ("bytes_per_subint", C.c_int)]
+
+
+class MpiPsrfits(C.Structure):
+    # mpi_psrfits { vm_psrfits coarse_chan_pf { vm_psrfits_sub sub }; int writer_id }
+    _fields_ = [("sub", PsrfitsSub), ("writer_id", C.c_int)]
+
+
+class GpuIpfbArrays(C.Structure):
+    _fields_ = [("ntaps", C.c_int), ("in_size", C.c_int), ("ft_size", C.c_int),
+                ("out_size", C.c_int)] + [(n, C.c_void_p) for n in (
+                    "in_real", "in_imag", "ft_real", "ft_imag", "d_in_real", "d_in_imag",
+                    "d_ft_real", "d_ft_imag", "d_out")]
+
+
+_P = C.c_void_p
+_SZ = C.c_size_t
+
+
+class VcsbeamContext(C.Structure):
+    _fields_ = [
+        ("use_mpi", C.c_bool), ("mpi_size", C.c_int), ("mpi_rank", C.c_int),
+        ("ncoarse_chans", C.c_int), ("coarse_chan_idx", C.c_int), ("writer", C.c_int),
+        ("obs_context", _P), ("obs_metadata", C.POINTER(MetafitsMetadata)), ("vcs_context", _P),
+        ("vcs_metadata", C.POINTER(VoltageMetadata)), ("cal_context", _P), ("cal_metadata", _P),
+        ("obs_context_legacy", _P), ("obs_metadata_legacy", _P),
+        ("datadir", _P), ("filenames", _P), ("nfiles", C.c_int), ("nfiletimes", C.c_uint),
+        ("seconds_per_file", C.c_uint),
+        ("cal", Calibration),
+        ("ras_hours", _P), ("decs_degs", _P),
+        ("analysis_filter", C.POINTER(PfbFilter)), ("synth_filter", C.POINTER(PfbFilter)),
+        ("fpfb", C.POINTER(ForwardPfb)),
+        ("v", C.POINTER(HostBuffer)), ("d_v", _P), ("v_size_bytes", _SZ), ("d_v_size_bytes", _SZ),
+        ("S", _P), ("d_S", _P), ("S_size_bytes", _SZ), ("d_S_size_bytes", _SZ),
+        ("e", _P), ("d_e", _P), ("e_size_bytes", _SZ), ("d_e_size_bytes", _SZ),
+        ("J", _P), ("d_J", _P), ("J_size_bytes", _SZ), ("d_J_size_bytes", _SZ),
+        ("Jv_P", _P), ("d_Jv_P", _P), ("Jv_Q", _P), ("d_Jv_Q", _P), ("Jv_size_bytes", _SZ),
+        ("d_Jv_size_bytes", _SZ),
+        ("D", _P), ("d_D", _P), ("D_size_bytes", _SZ), ("d_D_size_bytes", _SZ),
+        ("polP_idxs", _P), ("d_polP_idxs", _P), ("polQ_idxs", _P), ("d_polQ_idxs", _P),
+        ("pol_idxs_size_bytes", _SZ), ("d_pol_idxs_size_bytes", _SZ),
+        ("gdelays", GeometricDelays), ("pb", PrimaryBeam),
+        ("npointing", C.c_uint),
+        ("max_gpu_mem_bytes", _SZ), ("chunks_per_second", C.c_uint32),
+        ("num_coarse_chans_to_process", C.c_int), ("coarse_chan_idxs_to_process", _P),
+        ("cal_coarse_chan_idxs_to_process", _P),
+        ("num_gps_seconds_to_process", C.c_int), ("gps_seconds_to_process", _P),
+        ("output_fine_channels", C.c_bool), ("output_coarse_channels", C.c_bool),
+        ("current_gps_idx", _SZ),
+        ("do_forward_pfb", C.c_bool), ("do_inverse_pfb", C.c_bool),
+        ("sample_rate", C.c_int), ("fine_sample_rate", C.c_int), ("bytes_per_second", C.c_int),
+        ("num_not_flagged", C.c_int), ("nchan", C.c_int), ("nfine_chan", C.c_int),
+        ("out_nstokes", C.c_int),
+        ("vhdr", C.c_uint8 * 32), ("vf", _P),
+        ("d_offsets", _P), ("offsets", _P), ("d_scales", _P), ("scales", _P),
+        ("d_Cscaled", _P), ("Cscaled", _P),
+        ("offsets_size", _SZ), ("scales_size", _SZ), ("Cscaled_size", _SZ),
+        ("chunk_to_load", C.c_int), ("datatype", C.c_int),
+        ("streams", _P), ("log", _P),
+        ("log_message", C.c_char * 1024), ("error_message", C.c_char * 1024),
+        ("dev", _P), ("beamform_form", C.c_int), ("read_second", _P), ("read_second_user", _P),
+        ("bf_err", _P),
+    ]
+
+
+# Field lists matching vmStructLayout() in csrc/vm_host.c
+LAYOUT_FIELDS = {
+    "vcsbeam_context": (VcsbeamContext, ["obs_metadata", "cal", "fpfb", "v", "d_v", "S", "e", "J",
+                                         "D", "polP_idxs", "gdelays", "pb", "npointing",
+                                         "chunks_per_second", "gps_seconds_to_process",
+                                         "do_inverse_pfb", "sample_rate", "out_nstokes", "vhdr",
+                                         "d_offsets", "offsets_size", "chunk_to_load", "streams",
+                                         "log_message", "dev", "beamform_form", "bf_err"]),
+    "forward_pfb": (ForwardPfb, ["vcs_data", "d_vcs_data", "vcs_size", "d_weighted_overlap_add",
+                                 "nspectra", "P", "flags", "d_flt_out", "filter_fits_int16"]),
+    "MetafitsMetadata": (MetafitsMetadata, ["sched_start_mjd", "rf_inputs",
+                                            "metafits_coarse_chans", "coarse_chan_width_hz",
+                                            "project_id"]),
+    "Rfinput": (Rfinput, ["pol", "electrical_length_m", "vcs_order", "flagged"]),
+    "gpu_ipfb_arrays": (GpuIpfbArrays, ["in_real", "d_out"]),
+}
+
+_CTX = C.POINTER(VcsbeamContext)
+
+# name: (restype, argtypes)
+_SIGS = {
+    "vmInit": (_CTX, [C.c_bool]),
+    "destroy_vcsbeam_context": (None, [_CTX]),
+    "vmBindObsMetadata": (None, [_CTX, C.POINTER(MetafitsMetadata), C.POINTER(VoltageMetadata),
+                                 C.c_int, C.c_uint32, C.c_int]),
+    "vmReadNextSecondFrom": (C.c_int, [_CTX, C.c_void_p, C.c_size_t]),
+    "vmReadNextSecond": (C.c_int, [_CTX]),
+    "vmReadBufferCopyMargin": (C.c_int, [C.POINTER(HostBuffer)]),
+    "vmSetFilter": (None, [_CTX, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int]),
+    "vmLoadFilter": (None, [_CTX, C.c_char_p, C.c_int, C.c_int]),
+    "vmScaleFilterCoeffs": (None, [_CTX, C.c_int, C.c_double]),
+    "vmInitForwardPFB": (None, [_CTX, C.c_int, C.c_int]),
+    "vmSetMaxGPUMem": (None, [_CTX, C.c_int]),
+    "vmSetNumPointings": (None, [_CTX, C.c_uint]),
+    "vmSetPolIdxLists": (None, [_CTX]),
+    "vmPushPolIdxLists": (None, [_CTX]),
+    "vmCalcJ": (None, [_CTX]),
+    "vmCalcPhi": (None, [_CTX, C.POINTER(BeamGeom)]),
+    "vmCalcPhiDevice": (None, [_CTX, C.POINTER(BeamGeom)]),
+    "vmCalcJDevice": (None, [_CTX]),
+    "vmPushD": (None, [_CTX]),
+    "vmPushJ": (None, [_CTX]),
+    "vmPushPhi": (None, [_CTX]),
+    "vmCreateGeometricDelays": (None, [_CTX]),
+    "vmCreatePrimaryBeam": (None, [_CTX]),
+    "vmCreateCudaStreams": (None, [_CTX]),
+    "vmDestroyCudaStreams": (None, [_CTX]),
+    "vmCreateStatistics": (None, [_CTX, C.c_void_p]),
+    "vmDestroyStatistics": (None, [_CTX]),
+    "vmBeamformSecond": (None, [_CTX]),
+    "vmBeamformChunk": (None, [_CTX]),
+    "vmApplyJChunk": (None, [_CTX]),
+    "vmBeamformChunkMFMA": (None, [_CTX]),
+    "vmPullE": (None, [_CTX]),
+    "vmPullS": (None, [_CTX]),
+    "vmSendSToFits": (None, [_CTX, C.c_void_p]),
+    "vmExecuteForwardPFB": (None, [_CTX]),
+    "vmUploadForwardPFBChunk": (None, [_CTX]),
+    "vmWOLAChunk": (None, [_CTX]),
+    "vmFPGARoundingChunk": (None, [_CTX]),
+    "vmFFTChunk": (None, [_CTX]),
+    "vmPackChunk": (None, [_CTX]),
+    "vmDownloadForwardPFBChunk": (None, [_CTX]),
+    "vmSynchronize": (None, [_CTX]),
+    "vmGetStream": (C.c_void_p, [_CTX]),
+    "cu_form_incoh_beam": (None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_uint,
+                                  C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "allocate_input_output_arrays": (None, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                            C.c_size_t]),
+    "free_input_output_arrays": (None, [C.c_void_p, C.c_void_p]),
+    "malloc_ipfb": (None, [C.POINTER(GpuIpfbArrays), C.POINTER(PfbFilter), C.c_int, C.c_int,
+                           C.c_int]),
+    "free_ipfb": (None, [C.POINTER(GpuIpfbArrays)]),
+    "cu_load_ipfb_filter": (None, [C.POINTER(PfbFilter), C.POINTER(GpuIpfbArrays)]),
+    "cu_invert_pfb": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                             C.POINTER(GpuIpfbArrays), C.c_void_p]),
+    "vmInvertPFBDevice": (None, [_CTX, C.POINTER(GpuIpfbArrays), C.c_int, C.c_void_p]),
+    "vmRequantVDIFDevice": (None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "create_data_buffer_fine": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "prepare_data_buffer_fine": (None, [C.c_void_p, _CTX, C.c_size_t]),
+    "float2int8_trunc": (None, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_void_p]),
+    "to_offset_binary": (None, [C.c_void_p, C.c_int]),
+    "calc_array_factor": (C.c_double, [C.POINTER(MetafitsMetadata), C.c_uint32,
+                                       C.POINTER(BeamGeom), C.POINTER(BeamGeom)]),
+    "vmStructLayout": (C.c_int, [C.c_char_p, C.POINTER(C.c_size_t), C.c_int]),
+    "vmDeviceMalloc": (C.c_void_p, [C.c_size_t]),
+    "vmDeviceFree": (None, [C.c_void_p]),
+    "vmMemcpyHtoD": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "vmMemcpyDtoH": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "vmMemsetDevice": (None, [C.c_void_p, C.c_int, C.c_size_t]),
+    "vmDeviceSynchronize": (None, []),
+    "vmDeviceCount": (C.c_int, []),
+    "vmEventCreate": (C.c_void_p, []),
+    "vmEventRecord": (None, [C.c_void_p, C.c_void_p]),
+    "vmEventElapsedMs": (C.c_float, [C.c_void_p, C.c_void_p]),
+    "vmEventDestroy": (None, [C.c_void_p]),
+    "vmLibraryVersion": (C.c_char_p, []),
+}
+
+for _n in ("vmMallocVDevice", "vmMallocJVDevice", "vmMallocEHost", "vmMallocEDevice",
+           "vmMallocSHost", "vmMallocSDevice", "vmMallocJHost", "vmMallocJDevice", "vmMallocDHost",
+           "vmMallocDDevice", "vmMallocPQIdxsHost", "vmMallocPQIdxsDevice", "vmFreeVDevice",
+           "vmFreeJVDevice", "vmFreeEHost", "vmFreeEDevice", "vmFreeSHost", "vmFreeSDevice",
+           "vmFreeJHost", "vmFreeJDevice", "vmFreeDHost", "vmFreeDDevice", "vmFreePQIdxsHost",
+           "vmFreePQIdxsDevice", "vmMallocJVHost", "vmFreeJVHost", "vmMallocVHost",
+           "vmFreeVHost"):
+    _SIGS[_n] = (None, [_CTX])
+
+_lib = None
+_synth = None
+
+
+def load():
+    """Load the C-ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} not built: run python mwa-vcsbeam_amd/build.py")
+    os.environ.setdefault("VCSBEAM_RUNTIME_DIR", RUNTIME_DIR)
+    lib = C.CDLL(LIBPATH, mode=C.RTLD_GLOBAL)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def synth():
+    """Load the synthetic-input generator library."""
+    global _synth
+    if _synth is not None:
+        return _synth
+    if not os.path.exists(SYNTHPATH):
+        raise RuntimeError(f"{SYNTHPATH} not built: run python mwa-vcsbeam_amd/build.py")
+    s = C.CDLL(SYNTHPATH)
+    s.vmsynth_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, C.c_void_p]
+    s.vmsynth_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, C.c_void_p]
+    s.vmsynth_mwax_voltages.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int,
+                                        C.c_double, C.c_int, C.c_void_p]
+    s.vmsynth_legacy_int4.argtypes = [C.c_uint64, C.c_size_t, C.c_double, C.c_void_p]
+    s.vmsynth_kaiser_filter.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.c_void_p]
+    for f in ("vmsynth_uniform", "vmsynth_normal", "vmsynth_mwax_voltages",
+              "vmsynth_legacy_int4", "vmsynth_kaiser_filter"):
+        getattr(s, f).restype = None
+    _synth = s
+    return s
+
+
+class DeviceArray:
+    """A device allocation owned by Python (freed on garbage collection)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = load().vmDeviceMalloc(self.nbytes)
+
+    @classmethod
+    def from_numpy(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        load().vmMemcpyHtoD(d.ptr, a.ctypes.data, a.nbytes)
+        return d
+
+    def to_numpy(self, dtype, shape):
+        import numpy as np
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        load().vmMemcpyDtoH(out.ctypes.data, self.ptr, out.nbytes)
+        return out
+
+    def zero(self):
+        load().vmMemsetDevice(self.ptr, 0, self.nbytes)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            load().vmDeviceFree(self.ptr)
+            self.ptr = None
+
+
+def struct_layout(name):
+    """(C sizes/offsets from the library, ctypes sizes/offsets) for a struct."""
+    lib = load()
+    buf = (C.c_size_t * 64)()
+    n = lib.vmStructLayout(name.encode(), buf, 64)
+    cls, fields = LAYOUT_FIELDS[name]
+    mine = [C.sizeof(cls)] + [getattr(cls, f).offset for f in fields]
+    return list(buf[:n]), mine

@@ -1,0 +1,290 @@
+"""Host-side drivers mirroring the reference applications' call sequences.
+
+Each function drives the C-ABI exactly in the order the corresponding
+MWA-vcsbeam application does (minus command-line parsing and file I/O):
+
+* :func:`tied_array_beam` -- app/make_mwa_tied_array_beam.c:75-385
+* :func:`incoherent_beam` -- app/make_mwa_incoh_beam.c:53-189
+* :func:`fine_pfb_offline` -- app/fine_pfb_offline.c:38-94
+
+Inputs come from a :class:`workloads.Observation`; outputs are returned as
+numpy arrays in the reference host layouts.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import (ANALYSIS_FILTER, SYNTHESIS_FILTER, BeamGeom, GpuIpfbArrays, MpiPsrfits,
+               PFB_FULL_PRECISION, PFB_MALLOC_HOST_OUTPUT, PFB_SMART, VM_BF_FP32, VM_SUCCESS, load)
+from .workloads import unit_vectors
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _beam_geoms(az, za):
+    u = unit_vectors(az, za)
+    bg = (BeamGeom * len(u))()
+    for p in range(len(u)):
+        bg[p].az = float(az[p])
+        bg[p].el = float(np.pi / 2 - za[p])
+        bg[p].unit_E, bg[p].unit_N, bg[p].unit_H = (float(x) for x in u[p])
+    return bg
+
+
+def tied_array_beam(obs, az, za, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, nchunks=1,
+                    form=VM_BF_FP32, nseconds=1, device_geometry=False, want_e=True,
+                    ipfb=False, synth_filter=None, ipfb_device=False, return_vm=False):
+    """Tied-array beamforming of `nseconds` seconds of `obs` towards the
+    directions (az, za) [radians].  Returns dict with per-second lists:
+    'S' [p][s][st][c] float32, 'e' [p][s][c][pol] complex128 (if want_e),
+    'vdif' float32 [n][pol][2] (if ipfb)."""
+    lib = load()
+    az = np.atleast_1d(np.asarray(az, dtype=np.float64))
+    za = np.atleast_1d(np.asarray(za, dtype=np.float64))
+    npointing = len(az)
+    vm = lib.vmInit(False)
+    v = vm.contents
+    lib.vmBindObsMetadata(vm, C.byref(obs.md), C.byref(obs.vmd), 0, obs.md.obs_id, nseconds)
+    v.out_nstokes = nstokes
+    v.chunks_per_second = nchunks
+    v.beamform_form = form
+    coeffs = np.ascontiguousarray(obs.filter, dtype=np.float64)
+    if v.do_forward_pfb:
+        lib.vmSetFilter(vm, coeffs.ctypes.data_as(C.POINTER(C.c_double)), len(coeffs),
+                        ANALYSIS_FILTER, 128)
+        lib.vmInitForwardPFB(vm, 128, pfb_flags)
+    lib.vmSetNumPointings(vm, npointing)
+    bg = _beam_geoms(az, za)
+    nchans, npols, nsamples = v.nfine_chan, 2, v.fine_sample_rate
+    if ipfb:
+        v.do_inverse_pfb = True
+        sf = np.ascontiguousarray(synth_filter if synth_filter is not None else coeffs / coeffs.max(),
+                                  dtype=np.float64)
+        lib.vmSetFilter(vm, sf.ctypes.data_as(C.POINTER(C.c_double)), len(sf), SYNTHESIS_FILTER,
+                        nchans)
+        lib.vmScaleFilterCoeffs(vm, SYNTHESIS_FILTER, 15.0 / 7.2)
+        data_buffer_fine = lib.create_data_buffer_fine(npointing, 2 * nsamples, nchans, npols)
+    lib.vmSetMaxGPUMem(vm, nchunks)
+    for f in ("vmMallocVDevice", "vmMallocJVDevice", "vmMallocEHost", "vmMallocEDevice",
+              "vmMallocSHost", "vmMallocSDevice", "vmMallocJHost", "vmMallocJDevice",
+              "vmMallocDHost", "vmMallocDDevice", "vmMallocPQIdxsHost", "vmMallocPQIdxsDevice"):
+        getattr(lib, f)(vm)
+    lib.vmSetPolIdxLists(vm)
+    lib.vmPushPolIdxLists(vm)
+    gi = GpuIpfbArrays()
+    vdif = None
+    if ipfb:
+        vdif = np.zeros(nsamples * nchans * npols * npointing * 2, dtype=np.float32)
+        lib.malloc_ipfb(C.byref(gi), v.synth_filter, nsamples, npols, npointing)
+        lib.cu_load_ipfb_filter(v.synth_filter, C.byref(gi))
+    # calibration D (flagged tiles zeroed), primary beam B = identity
+    D = np.ascontiguousarray(obs.calibration(nchans))
+    C.memmove(v.D, D.ctypes.data, D.nbytes)
+    lib.vmCreatePrimaryBeam(vm)
+    lib.vmCreateGeometricDelays(vm)
+    lib.vmCreateCudaStreams(vm)
+    if device_geometry:
+        lib.vmPushD(vm)
+    out = {"S": [], "e": [], "vdif": []}
+    for t in range(nseconds):
+        sec = obs.second(t) if not obs.legacy else obs.legacy_second(t)
+        err = lib.vmReadNextSecondFrom(vm, _ptr(sec), sec.nbytes)
+        if err != VM_SUCCESS:
+            raise RuntimeError(f"vmReadNextSecondFrom: {err}")
+        if device_geometry:
+            lib.vmCalcPhiDevice(vm, bg)
+            lib.vmCalcJDevice(vm)
+        else:
+            lib.vmCalcPhi(vm, bg)
+            lib.vmCalcJ(vm)
+            lib.vmPushPhi(vm)
+            lib.vmPushJ(vm)
+        lib.vmBeamformSecond(vm)
+        if ipfb:
+            if ipfb_device:
+                lib.vmInvertPFBDevice(vm, C.byref(gi), t, _ptr(vdif))
+            else:
+                lib.vmPullE(vm)
+                lib.prepare_data_buffer_fine(data_buffer_fine, vm, t)
+                lib.cu_invert_pfb(data_buffer_fine, t, npointing, nsamples, nchans, npols, 0,
+                                  C.byref(gi), _ptr(vdif))
+            out["vdif"].append(vdif.copy())
+        lib.vmPullS(vm)
+        S = np.ctypeslib.as_array(C.cast(v.S, C.POINTER(C.c_float)),
+                                  shape=(npointing, nsamples, nstokes, nchans)).copy()
+        out["S"].append(S)
+        if want_e:
+            if not ipfb or ipfb_device:
+                lib.vmPullE(vm)
+            e = np.ctypeslib.as_array(C.cast(v.e, C.POINTER(C.c_double)),
+                                      shape=(npointing, nsamples, nchans, 2, 2)).copy()
+            out["e"].append(e[..., 0] + 1j * e[..., 1])
+    if return_vm:
+        out["vm"] = vm
+        return out
+    if ipfb:
+        lib.free_ipfb(C.byref(gi))
+    lib.vmDestroyCudaStreams(vm)
+    for f in ("vmFreeVDevice", "vmFreeJVDevice", "vmFreeEHost", "vmFreeEDevice", "vmFreeSHost",
+              "vmFreeSDevice", "vmFreeJDevice", "vmFreeJHost", "vmFreeDHost", "vmFreeDDevice",
+              "vmFreePQIdxsDevice", "vmFreePQIdxsHost"):
+        getattr(lib, f)(vm)
+    lib.destroy_vcsbeam_context(vm)
+    return out
+
+
+def fine_pfb_offline(obs, *, nseconds=1, nchunks=1, pfb_flags=PFB_SMART | PFB_MALLOC_HOST_OUTPUT):
+    """Forward PFB of `nseconds` seconds to the legacy recombined format.
+    Returns a list of per-second host outputs (uint8 [s][c][i] for SMART,
+    complex128 [s][c][i] for FULL_PRECISION + host output)."""
+    lib = load()
+    vm = lib.vmInit(False)
+    v = vm.contents
+    lib.vmBindObsMetadata(vm, C.byref(obs.md), C.byref(obs.vmd), 0, obs.md.obs_id, nseconds)
+    coeffs = np.ascontiguousarray(obs.filter, dtype=np.float64)
+    lib.vmSetFilter(vm, coeffs.ctypes.data_as(C.POINTER(C.c_double)), len(coeffs),
+                    ANALYSIS_FILTER, 128)
+    v.chunks_per_second = nchunks
+    lib.vmInitForwardPFB(vm, 128, pfb_flags)
+    fp = v.fpfb.contents
+    outs = []
+    for t in range(nseconds):
+        sec = obs.second(t)
+        if lib.vmReadNextSecondFrom(vm, _ptr(sec), sec.nbytes) != VM_SUCCESS:
+            raise RuntimeError("read failed")
+        lib.vmExecuteForwardPFB(vm)
+        v.v.contents.locked = False
+        raw = np.ctypeslib.as_array(C.cast(fp.vcs_data, C.POINTER(C.c_uint8)),
+                                    shape=(fp.vcs_size,)).copy()
+        if pfb_flags & 0x10:
+            outs.append(raw.reshape(fp.nspectra, fp.K, fp.I))
+        elif pfb_flags & 0x40:
+            # device layout [c][i][s] per chunk, chunks concatenated
+            nch = fp.nspectra // fp.nspectra_per_chunk
+            a = raw.view(np.complex64).reshape(nch, fp.K, fp.I, fp.nspectra_per_chunk)
+            outs.append(np.ascontiguousarray(a.transpose(1, 2, 0, 3)).reshape(fp.K, fp.I,
+                                                                              fp.nspectra))
+        else:
+            outs.append(raw.view(np.complex128).reshape(fp.nspectra, fp.K, fp.I))
+    lib.destroy_vcsbeam_context(vm)
+    return outs
+
+
+def incoherent_beam(data, nsample, nchan, ninput):
+    """cu_form_incoh_beam on one second of legacy 4+4-bit data [s][c][i].
+    Returns (incoh float [s][c] before scaling, offsets, scales, Iscaled)."""
+    from . import DeviceArray
+    lib = load()
+    size = nsample * nchan * ninput
+    h, d = C.c_void_p(), C.c_void_p()
+    lib.allocate_input_output_arrays(C.byref(h), C.byref(d), size)
+    C.memmove(h, data.ctypes.data, size)
+    d_incoh = DeviceArray(nsample * nchan * 4)
+    d_off = DeviceArray(nchan * 4)
+    d_scl = DeviceArray(nchan * 4)
+    d_Is = DeviceArray(nsample * nchan)
+    off = np.zeros(nchan, np.float32)
+    scl = np.zeros(nchan, np.float32)
+    Is = np.zeros(nsample * nchan, np.uint8)
+    lib.cu_form_incoh_beam(h, d, size, d_incoh.ptr, nsample, nchan, ninput, _ptr(off), d_off.ptr,
+                           _ptr(scl), d_scl.ptr, _ptr(Is), d_Is.ptr, Is.nbytes)
+    incoh = d_incoh.to_numpy(np.float32, (nsample, nchan))
+    lib.free_input_output_arrays(h, d)
+    return incoh, off, scl, Is.reshape(nsample, nchan)
+
+
+class TiedArraySession:
+    """A tied-array beamforming context held open across steps.
+
+    Set-up follows app/make_mwa_tied_array_beam.c:85-240; the per-second
+    device work is split into :meth:`pfb` (forward PFB of the resident
+    second), :meth:`geometry` (FP64 phases and Jones inverses on the GPU) and
+    :meth:`beamform` (weights + beamformer), so that each can be timed.
+    d_e is only allocated when `want_e` (the reference always writes e; the
+    many-beam Stokes workloads do not need it).
+    """
+
+    def __init__(self, obs, az, za, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, form=VM_BF_FP32,
+                 want_e=False):
+        lib = self.lib = load()
+        self.obs = obs
+        az = np.atleast_1d(np.asarray(az, dtype=np.float64))
+        za = np.atleast_1d(np.asarray(za, dtype=np.float64))
+        self.npointing = len(az)
+        vm = self.vm = lib.vmInit(False)
+        v = self.v = vm.contents
+        lib.vmBindObsMetadata(vm, C.byref(obs.md), C.byref(obs.vmd), 0, obs.md.obs_id, 1 << 20)
+        v.out_nstokes = nstokes
+        v.chunks_per_second = 1
+        v.beamform_form = form
+        self.coeffs = np.ascontiguousarray(obs.filter, dtype=np.float64)
+        lib.vmSetFilter(vm, self.coeffs.ctypes.data_as(C.POINTER(C.c_double)), len(self.coeffs),
+                        ANALYSIS_FILTER, 128)
+        lib.vmInitForwardPFB(vm, 128, pfb_flags)
+        lib.vmSetNumPointings(vm, self.npointing)
+        self.bg = _beam_geoms(az, za)
+        lib.vmSetMaxGPUMem(vm, 1)
+        names = ["vmMallocVDevice", "vmMallocSHost", "vmMallocSDevice", "vmMallocJHost",
+                 "vmMallocJDevice", "vmMallocDHost", "vmMallocDDevice", "vmMallocPQIdxsHost",
+                 "vmMallocPQIdxsDevice"]
+        if want_e:
+            names += ["vmMallocEHost", "vmMallocEDevice"]
+        for f in names:
+            getattr(lib, f)(vm)
+        lib.vmSetPolIdxLists(vm)
+        lib.vmPushPolIdxLists(vm)
+        D = np.ascontiguousarray(obs.calibration(v.nfine_chan))
+        C.memmove(v.D, D.ctypes.data, D.nbytes)
+        lib.vmCreatePrimaryBeam(vm)
+        lib.vmCreateGeometricDelays(vm)
+        lib.vmPushD(vm)
+        self.stream = lib.vmGetStream(vm)
+
+    def read(self, t):
+        sec = self.obs.second(t)
+        self.v.v.contents.locked = False
+        err = self.lib.vmReadNextSecondFrom(self.vm, _ptr(sec), sec.nbytes)
+        if err != VM_SUCCESS:
+            raise RuntimeError(f"vmReadNextSecondFrom: {err}")
+        self.lib.vmUploadForwardPFBChunk(self.vm)
+
+    def pfb(self):
+        for f in ("vmWOLAChunk", "vmFPGARoundingChunk", "vmFFTChunk", "vmPackChunk"):
+            getattr(self.lib, f)(self.vm)
+
+    def geometry(self):
+        self.lib.vmCalcPhiDevice(self.vm, self.bg)
+        self.lib.vmCalcJDevice(self.vm)
+
+    def beamform(self):
+        self.lib.vmApplyJChunk(self.vm)
+        self.lib.vmBeamformChunk(self.vm)
+
+    def stokes(self):
+        """Pull S to the host layout [p][s][st][c]."""
+        self.lib.vmPullS(self.vm)
+        v = self.v
+        return np.ctypeslib.as_array(C.cast(v.S, C.POINTER(C.c_float)),
+                                     shape=(self.npointing, v.fine_sample_rate, v.out_nstokes,
+                                            v.nfine_chan)).copy()
+
+    def voltages(self):
+        """The device voltages d_v (VM_FLT [c][i][s]) as complex64."""
+        v = self.v
+        n = v.nfine_chan * self.obs.ninput * v.fine_sample_rate
+        out = np.empty(n, np.complex64)
+        self.lib.vmMemcpyDtoH(out.ctypes.data, v.d_v, out.nbytes)
+        return out.reshape(v.nfine_chan, self.obs.ninput, v.fine_sample_rate)
+
+    def close(self):
+        lib, vm = self.lib, self.vm
+        for f in ("vmFreeVDevice", "vmFreeSHost", "vmFreeSDevice", "vmFreeJDevice", "vmFreeJHost",
+                  "vmFreeDHost", "vmFreeDDevice", "vmFreePQIdxsDevice", "vmFreePQIdxsHost"):
+            getattr(lib, f)(vm)
+        if self.v.e:
+            lib.vmFreeEHost(vm)
+            lib.vmFreeEDevice(vm)
+        lib.destroy_vcsbeam_context(vm)
+        self.vm = None

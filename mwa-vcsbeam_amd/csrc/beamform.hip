@@ -1,0 +1,554 @@
+// beamform.hip -- tied-array beamformer (plain FP32 form), incoherent beam,
+// Stokes renormalisation and the host-layout pulls.
+//
+// Reference behaviour: MWA-vcsbeam src/form_beam.cpp:60-87 (incoh_beam),
+// 122-204 (vmApplyJ_kernel), 239-371 (vmBeamform_kernel), 408-449
+// (renormalise_channels_kernel), 463-804 (host).
+//
+// MI355X design:
+//  * J^-1 v and phi * (J^-1 v) are fused: vmApplyJChunk folds the delay phase
+//    into the Jones inverse once per second, W[p][c][a] = phi[p][a][c] J[p][a][c]
+//    (products in FP64, stored as complex float); vmBeamformChunk then does
+//    e = sum_a W v, the auto-correlation sums and Stokes in one pass, with no
+//    Jv_Q / Jv_P intermediates (5.2 GB per beam-second in the reference).
+//  * One lane per (channel, sample), PB beams per lane; the antenna loop reads
+//    weights with wave-uniform (scalar) loads and voltages with coalesced loads
+//    along the sample axis of the VM_FLT layout.
+//  * Device outputs use sample-innermost layouts (d_S [p][st][c][s], d_e
+//    [p][c][s][pol]) so stores are coalesced; vmPullS / vmPullE transpose to the
+//    reference host layouts.
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "vm_internal.h"
+
+#define HIPCHECK(x)                                                               \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) VM_FATAL("HIP: %s (%s)", hipGetErrorString(e_), #x); \
+    } while (0)
+
+struct cf { float x, y; };
+
+__device__ __forceinline__ void unpack4(uint8_t b, float &re, float &im)
+{
+    // include/vcsbeam.h.in:997-1003: high nibble real, low nibble imaginary
+    int r = (b >> 4) & 0xf, i = b & 0xf;
+    re = (float)(r >= 8 ? r - 16 : r);
+    im = (float)(i >= 8 ? i - 16 : i);
+}
+
+// ------------------------------------------------------------ weights -----
+// W[p][c][a][q] = phi[p][a][c] * J[p][a][c][q] (FP64 product, float result)
+__global__ void k_build_weights(const double2 *__restrict__ J, const double2 *__restrict__ phi,
+                                int npointing, int nant, int nchan, cf *__restrict__ W)
+{
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t total = (size_t)npointing * nant * nchan;
+    if (j >= total) return;
+    int c = (int)(j % nchan);
+    int a = (int)((j / nchan) % nant);
+    int p = (int)(j / ((size_t)nchan * nant));
+    double2 f = phi[j]; // PHI_IDX(p,a,c) == j
+    const double2 *Jm = J + j * 4; // J_IDX(p,a,c,0,0)
+    cf *out = W + (((size_t)p * nchan + c) * nant + a) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double2 m = Jm[q];
+        out[q] = cf{(float)(f.x * m.x - f.y * m.y), (float)(f.x * m.y + f.y * m.x)};
+    }
+}
+
+// ------------------------------------------------------ FP32 beamformer ----
+// grid (ceil(ns/256), nchan, ceil(npointing/PB)), block 256.
+// VM_FLT voltages v[c][i][s] (complex float) or VM_INT4 legacy [s][c][i].
+template <int PB, bool INT4>
+__global__ __launch_bounds__(256) void k_beamform_fp32(
+    const void *__restrict__ v, int ns, int nchan, int ni, const cf *__restrict__ W,
+    const uint32_t *__restrict__ polQ, const uint32_t *__restrict__ polP, int nant,
+    int npointing, float invw, int nstokes, int soffset, int ns_total, float *__restrict__ S,
+    cf *__restrict__ e)
+{
+    extern __shared__ uint8_t lds[];
+    const int t = threadIdx.x;
+    const int s = blockIdx.x * 256 + t;
+    const int c = blockIdx.y;
+    const int p0 = blockIdx.z * PB;
+    const bool valid = s < ns;
+
+    if (INT4) {
+        // stage this block's 256 samples x ni bytes, coalesced per (s, c) row
+        const uint8_t *vb = (const uint8_t *)v;
+        const int s0 = blockIdx.x * 256;
+        const int nrow = min(256, ns - s0);
+        const int dw = ni / 4;
+        for (int j = t; j < nrow * dw; j += 256) {
+            int r = j / dw, q = j % dw;
+            ((uint32_t *)lds)[r * dw + q] =
+                ((const uint32_t *)(vb + ((size_t)(s0 + r) * nchan + c) * ni))[q];
+        }
+        __syncthreads();
+    }
+
+    float ex[PB][2], ey[PB][2], nxx[PB], nyy[PB], nxy[PB][2];
+#pragma unroll
+    for (int b = 0; b < PB; b++) {
+        ex[b][0] = ex[b][1] = ey[b][0] = ey[b][1] = 0.f;
+        nxx[b] = nyy[b] = nxy[b][0] = nxy[b][1] = 0.f;
+    }
+    const cf *vf = (const cf *)v + (size_t)c * ni * ns;
+    for (int a = 0; a < nant; a++) {
+        const uint32_t iq = polQ[a], ip = polP[a];
+        float qr = 0.f, qi = 0.f, pr = 0.f, pi = 0.f;
+        if (valid) {
+            if (INT4) {
+                unpack4(lds[t * ni + iq], qr, qi);
+                unpack4(lds[t * ni + ip], pr, pi);
+            } else {
+                cf q = vf[(size_t)iq * ns + s], pp = vf[(size_t)ip * ns + s];
+                qr = q.x; qi = q.y; pr = pp.x; pi = pp.y;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < PB; b++) {
+            int p = min(p0 + b, npointing - 1);
+            const cf *w = W + (((size_t)p * nchan + c) * nant + a) * 4;
+            cf w00 = w[0], w01 = w[1], w10 = w[2], w11 = w[3];
+            // ex_a = W00 vq + W01 vp ; ey_a = W10 vq + W11 vp
+            float axr = w00.x * qr - w00.y * qi + w01.x * pr - w01.y * pi;
+            float axi = w00.x * qi + w00.y * qr + w01.x * pi + w01.y * pr;
+            float ayr = w10.x * qr - w10.y * qi + w11.x * pr - w11.y * pi;
+            float ayi = w10.x * qi + w10.y * qr + w11.x * pi + w11.y * pr;
+            ex[b][0] += axr; ex[b][1] += axi;
+            ey[b][0] += ayr; ey[b][1] += ayi;
+            nxx[b] += axr * axr + axi * axi;
+            nyy[b] += ayr * ayr + ayi * ayi;
+            // ex * conj(ey)
+            nxy[b][0] += axr * ayr + axi * ayi;
+            nxy[b][1] += axi * ayr - axr * ayi;
+        }
+    }
+    if (!valid) return;
+    const int so = s + soffset;
+#pragma unroll
+    for (int b = 0; b < PB; b++) {
+        int p = p0 + b;
+        if (p >= npointing) break;
+        // src/form_beam.cpp:322-339 (auto-correlations removed; V sign as code)
+        float bnXX = ex[b][0] * ex[b][0] + ex[b][1] * ex[b][1] - nxx[b];
+        float bnYY = ey[b][0] * ey[b][0] + ey[b][1] * ey[b][1] - nyy[b];
+        float xyr = ex[b][0] * ey[b][0] + ex[b][1] * ey[b][1] - nxy[b][0];
+        float xyi = ex[b][1] * ey[b][0] - ex[b][0] * ey[b][1] - nxy[b][1];
+        float *Sp = S + (size_t)p * nstokes * nchan * ns_total;
+        Sp[((size_t)0 * nchan + c) * ns_total + so] = invw * (bnXX + bnYY);
+        if (nstokes == 4) {
+            Sp[((size_t)1 * nchan + c) * ns_total + so] = invw * (bnXX - bnYY);
+            Sp[((size_t)2 * nchan + c) * ns_total + so] = 2.0f * invw * xyr;
+            Sp[((size_t)3 * nchan + c) * ns_total + so] = -2.0f * invw * xyi;
+        }
+        if (e) {
+            cf *ep = e + (((size_t)p * nchan + c) * ns_total + so) * 2;
+            ep[0] = cf{ex[b][0], ex[b][1]};
+            ep[1] = cf{ey[b][0], ey[b][1]};
+        }
+    }
+}
+
+// ----------------------------------------------------- incoherent beam ----
+// One wavefront per (s, c): lane j reads dword j of the 256-input row.
+// I[s][c] = sum_i |v|^2, exact integer arithmetic (src/form_beam.cpp:60-87).
+__global__ __launch_bounds__(256) void k_incoh(const uint8_t *__restrict__ data, int nsample,
+                                               int nchan, int ninput, float *__restrict__ incoh)
+{
+    const int lane = threadIdx.x & 63;
+    const size_t row = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); // = s*nchan + c
+    if (row >= (size_t)nsample * nchan) return;
+    const uint8_t *r = data + row * ninput;
+    int acc = 0;
+    for (int j = lane * 4; j < ninput; j += 256) {
+        uint32_t w = (j + 3 < ninput) ? *(const uint32_t *)(r + j) : 0u;
+        if (j + 3 >= ninput)
+            for (int q = 0; q < 4 && j + q < ninput; q++) w |= (uint32_t)r[j + q] << (8 * q);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int b = (w >> (8 * q)) & 0xff;
+            int re = (b >> 4) & 0xf, im = b & 0xf;
+            re = re >= 8 ? re - 16 : re;
+            im = im >= 8 ? im - 16 : im;
+            acc += re * re + im * im;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) incoh[row] = (float)acc;
+}
+
+// --------------------------------------------------------- renormalise ----
+// Reference layout S[p][s][st][c] (used by cu_form_incoh_beam).
+// block (64 channels, 16 slices); grid (ceil(nchan/64), nstokes, npointing).
+__global__ __launch_bounds__(1024) void k_renorm_ref(const float *__restrict__ S, int nstep,
+                                                     int nstokes, int nchan,
+                                                     float *__restrict__ offsets,
+                                                     float *__restrict__ scales,
+                                                     uint8_t *__restrict__ Ss)
+{
+    __shared__ float smin[16][64], smax[16][64];
+    const int cl = threadIdx.x, sl = threadIdx.y;
+    const int c = blockIdx.x * 64 + cl, st = blockIdx.y, p = blockIdx.z;
+    const bool ok = c < nchan;
+    float mn = 0.f, mx = 0.f;
+    if (ok) {
+        size_t base = ((size_t)p * nstep * nstokes + st) * nchan + c;
+        size_t stride = (size_t)nstokes * nchan;
+        mn = mx = S[base];
+        for (int s = sl; s < nstep; s += 16) {
+            float val = S[base + s * stride];
+            mn = fminf(mn, val);
+            mx = fmaxf(mx, val);
+        }
+    }
+    smin[sl][cl] = mn;
+    smax[sl][cl] = mx;
+    __syncthreads();
+    if (sl == 0 && ok) {
+        for (int q = 1; q < 16; q++) {
+            mn = fminf(mn, smin[q][cl]);
+            mx = fmaxf(mx, smax[q][cl]);
+        }
+        smin[0][cl] = mn;
+        smax[0][cl] = mx;
+    }
+    __syncthreads();
+    if (!ok) return;
+    mn = smin[0][cl];
+    mx = smax[0][cl];
+    // src/form_beam.cpp:436-446
+    float scale = (mx - mn) / 256.0f, offset = mn;
+    size_t base = ((size_t)p * nstep * nstokes + st) * nchan + c;
+    size_t stride = (size_t)nstokes * nchan;
+    for (int s = sl; s < nstep; s += 16) {
+        float val = (S[base + s * stride] - offset) / scale;
+        float r = val + 0.5f;
+        // the reference's cast is undefined at r >= 256 (val == max) and for
+        // scale == 0; saturate instead
+        r = (r != r) ? 0.f : fminf(fmaxf(r, 0.f), 255.f);
+        Ss[base + s * stride] = (uint8_t)r;
+    }
+    if (sl == 0) {
+        scales[((size_t)p * nstokes + st) * nchan + c] = scale;
+        offsets[((size_t)p * nstokes + st) * nchan + c] = offset;
+    }
+}
+
+// Device layout S[p][st][c][s]: min/max per (p, st, c), one workgroup each.
+__global__ __launch_bounds__(256) void k_minmax_dev(const float *__restrict__ S, int nstep,
+                                                    float *__restrict__ offsets,
+                                                    float *__restrict__ scales)
+{
+    __shared__ float smin[256], smax[256];
+    const size_t row = blockIdx.x; // (p*nstokes + st)*nchan + c
+    const float *r = S + row * nstep;
+    float mn = r[0], mx = r[0];
+    for (int s = threadIdx.x; s < nstep; s += 256) {
+        float val = r[s];
+        mn = fminf(mn, val);
+        mx = fmaxf(mx, val);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + w]);
+            smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        scales[row] = (smax[0] - smin[0]) / 256.0f;
+        offsets[row] = smin[0];
+    }
+}
+
+// Quantise + transpose: S[p][st][c][s] -> Ss[p][s][st][c] (uint8).
+// block 256, tile 64 channels x 64 samples; grid (ceil(ns/64), ceil(nc/64), p*nst)
+__global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, int nstep,
+                                                   int nstokes, int nchan,
+                                                   const float *__restrict__ offsets,
+                                                   const float *__restrict__ scales,
+                                                   uint8_t *__restrict__ Ss)
+{
+    __shared__ uint8_t T[64][65];
+    const int pst = blockIdx.z, p = pst / nstokes, st = pst % nstokes;
+    const int c0 = blockIdx.y * 64, s0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        int c = c0 + r, s = s0 + tx;
+        uint8_t q = 0;
+        if (c < nchan && s < nstep) {
+            size_t row = (size_t)pst * nchan + c;
+            float val = (S[row * nstep + s] - offsets[row]) / scales[row];
+            float rr = val + 0.5f;
+            rr = (rr != rr) ? 0.f : fminf(fmaxf(rr, 0.f), 255.f);
+            q = (uint8_t)rr;
+        }
+        T[r][tx] = q;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        int s = s0 + r, c = c0 + tx;
+        if (s < nstep && c < nchan)
+            Ss[(((size_t)p * nstep + s) * nstokes + st) * nchan + c] = T[tx][r];
+    }
+}
+
+// ------------------------------------------------------------ pulls -------
+// S[p][st][c][s] -> [p][s][st][c] float
+__global__ __launch_bounds__(256) void k_S_to_host(const float *__restrict__ S, int nstep,
+                                                   int nstokes, int nchan, float *__restrict__ out)
+{
+    __shared__ float T[64][65];
+    const int pst = blockIdx.z, p = pst / nstokes, st = pst % nstokes;
+    const int c0 = blockIdx.y * 64, s0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        int c = c0 + r, s = s0 + tx;
+        T[r][tx] = (c < nchan && s < nstep) ? S[((size_t)pst * nchan + c) * nstep + s] : 0.f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        int s = s0 + r, c = c0 + tx;
+        if (s < nstep && c < nchan) out[(((size_t)p * nstep + s) * nstokes + st) * nchan + c] = T[tx][r];
+    }
+}
+
+// e[p][c][s][pol] complex float -> [p][s][c][pol] complex double
+__global__ __launch_bounds__(256) void k_e_to_host(const cf *__restrict__ e, int nstep, int nchan,
+                                                   double2 *__restrict__ out)
+{
+    __shared__ cf T[64][2 * 64 + 1];
+    const int p = blockIdx.z;
+    const int c0 = blockIdx.y * 64, s0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        int c = c0 + r, s = s0 + tx;
+        bool ok = c < nchan && s < nstep;
+        size_t b = (((size_t)p * nchan + c) * nstep + s) * 2;
+        T[r][2 * tx] = ok ? e[b] : cf{0.f, 0.f};
+        T[r][2 * tx + 1] = ok ? e[b + 1] : cf{0.f, 0.f};
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        int s = s0 + r, c = c0 + tx;
+        if (s < nstep && c < nchan) {
+            size_t o = (((size_t)p * nstep + s) * nchan + c) * 2;
+            cf a = T[tx][2 * r], b = T[tx][2 * r + 1];
+            out[o] = make_double2(a.x, a.y);
+            out[o + 1] = make_double2(b.x, b.y);
+        }
+    }
+}
+
+// =================================================================== host ==
+extern "C" {
+
+static hipStream_t vm_stream(vcsbeam_context *vm) { return (hipStream_t)vm_dev(vm)->stream; }
+
+static void *stage_buffer(vcsbeam_context *vm, size_t bytes)
+{
+    vm_device_state *d = vm_dev(vm);
+    if (d->stage_bytes < bytes) {
+        if (d->d_stage) HIPCHECK(hipFree(d->d_stage));
+        HIPCHECK(hipMalloc(&d->d_stage, bytes));
+        d->stage_bytes = bytes;
+    }
+    return d->d_stage;
+}
+
+void cu_form_incoh_beam(uint8_t *data, uint8_t *d_data, size_t data_size, float *d_incoh,
+                        unsigned int nsample, int nchan, int ninput, float *offsets,
+                        float *d_offsets, float *scales, float *d_scales, uint8_t *Iscaled,
+                        uint8_t *d_Iscaled, size_t Iscaled_size)
+{
+    // src/form_beam.cpp:463-505 (all copies complete before return)
+    HIPCHECK(hipMemcpy(d_data, data, data_size, hipMemcpyHostToDevice));
+    size_t rows = (size_t)nsample * nchan;
+    hipLaunchKernelGGL(k_incoh, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, 0, d_data,
+                       (int)nsample, nchan, ninput, d_incoh);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_renorm_ref, dim3((nchan + 63) / 64, 1, 1), dim3(64, 16), 0, 0, d_incoh,
+                       (int)nsample, 1, nchan, d_offsets, d_scales, d_Iscaled);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpy(offsets, d_offsets, nchan * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(scales, d_scales, nchan * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(Iscaled, d_Iscaled, Iscaled_size, hipMemcpyDeviceToHost));
+}
+
+// Rebuild W = phi * J^-1 when J or phi changed (once per second).
+void vm_build_weights(vcsbeam_context *vm)
+{
+    vm_device_state *d = vm_dev(vm);
+    if (!d->weights_dirty && d->d_W) return;
+    int nant = (int)vm->obs_metadata->num_ants, nchan = vm->nfine_chan;
+    size_t need = (size_t)vm->npointing * nchan * nant * 4 * sizeof(cf);
+    if (d->W_bytes < need) {
+        if (d->d_W) HIPCHECK(hipFree(d->d_W));
+        HIPCHECK(hipMalloc(&d->d_W, need));
+        d->W_bytes = need;
+    }
+    size_t total = (size_t)vm->npointing * nant * nchan;
+    hipLaunchKernelGGL(k_build_weights, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       vm_stream(vm), (const double2 *)vm->d_J, (const double2 *)vm->gdelays.d_phi,
+                       (int)vm->npointing, nant, nchan, (cf *)d->d_W);
+    HIPCHECK(hipGetLastError());
+    d->weights_dirty = 0;
+}
+
+void vmApplyJChunk(vcsbeam_context *vm)
+{
+    // The J^-1 v product is folded into the beamformer; here the combined
+    // per-second weights are refreshed.
+    vm_build_weights(vm);
+    HIPCHECK(hipStreamSynchronize(vm_stream(vm)));
+}
+
+void vm_beamform_fp32_launch(const void *d_v, vcsbeam_datatype dt, int ns, int nchan, int ni,
+                             const void *d_W, const uint32_t *d_polQ, const uint32_t *d_polP,
+                             int nant, int npointing, float invw, int nstokes, int soffset,
+                             int ns_total, float *d_S, void *d_e, hipStream_t stream)
+{
+    const int PB = 4;
+    dim3 grid((ns + 255) / 256, nchan, (npointing + PB - 1) / PB), block(256);
+    if (dt == VM_INT4) {
+        size_t lds = (size_t)256 * ni;
+        if (lds > 65536) VM_FATAL("too many RF inputs (%d) for the INT4 beamformer", ni);
+        hipLaunchKernelGGL((k_beamform_fp32<PB, true>), grid, block, lds, stream, d_v, ns, nchan,
+                           ni, (const cf *)d_W, d_polQ, d_polP, nant, npointing, invw, nstokes,
+                           soffset, ns_total, d_S, (cf *)d_e);
+    } else if (dt == VM_FLT) {
+        hipLaunchKernelGGL((k_beamform_fp32<PB, false>), grid, block, 0, stream, d_v, ns, nchan,
+                           ni, (const cf *)d_W, d_polQ, d_polP, nant, npointing, invw, nstokes,
+                           soffset, ns_total, d_S, (cf *)d_e);
+    } else {
+        VM_FATAL("beamformer: unsupported voltage datatype %d", (int)dt);
+    }
+    HIPCHECK(hipGetLastError());
+}
+
+void vmBeamformChunk(vcsbeam_context *vm)
+{
+    if (vm->beamform_form != VM_BF_FP32) {
+        vmBeamformChunkMFMA(vm);
+        return;
+    }
+    vm_build_weights(vm);
+    int chunk = vm->chunk_to_load % vm->chunks_per_second;
+    int ns = vm->fine_sample_rate / vm->chunks_per_second;
+    float invw = (float)(1.0 / (double)vm->num_not_flagged);
+    vm_beamform_fp32_launch(vm->d_v, vm->datatype, ns, vm->nfine_chan,
+                            (int)vm->obs_metadata->num_rf_inputs, vm->dev->d_W, vm->d_polQ_idxs,
+                            vm->d_polP_idxs, (int)vm->obs_metadata->num_ants, (int)vm->npointing,
+                            invw, vm->out_nstokes, chunk * ns, vm->fine_sample_rate,
+                            (float *)vm->d_S, vm->d_e, vm_stream(vm));
+    HIPCHECK(hipStreamSynchronize(vm_stream(vm)));
+}
+
+void vmBeamformSecond(vcsbeam_context *vm)
+{
+    // src/form_beam.cpp:587-627
+    for (unsigned int chunk = 0; chunk < vm->chunks_per_second; chunk++) {
+        if (vm->obs_metadata->mwa_version == VCSLegacyRecombined) {
+            vmPushChunk(vm);
+        } else {
+            vmUploadForwardPFBChunk(vm);
+            logger_start_stopwatch(vm->log, "pfb", chunk == 0);
+            vmWOLAChunk(vm);
+            vmFPGARoundingChunk(vm);
+            vmFFTChunk(vm);
+            vmPackChunk(vm);
+            logger_stop_stopwatch(vm->log, "pfb");
+        }
+        logger_start_stopwatch(vm->log, "calc", chunk == 0);
+        vmApplyJChunk(vm);
+        vmBeamformChunk(vm);
+        logger_stop_stopwatch(vm->log, "calc");
+        vm->chunk_to_load++;
+    }
+    HIPCHECK(hipStreamSynchronize(vm_stream(vm)));
+    vm->v->locked = false;
+}
+
+void vmPullE(vcsbeam_context *vm)
+{
+    // Synchronous (the reference's asynchronous copy races with the host
+    // reader, app/make_mwa_tied_array_beam.c:280-283)
+    int ns = vm->fine_sample_rate, nchan = vm->nfine_chan;
+    void *st = stage_buffer(vm, vm->e_size_bytes);
+    dim3 grid((ns + 63) / 64, (nchan + 63) / 64, vm->npointing);
+    hipLaunchKernelGGL(k_e_to_host, grid, dim3(256), 0, vm_stream(vm), (const cf *)vm->d_e, ns,
+                       nchan, (double2 *)st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(vm->e, st, vm->e_size_bytes, hipMemcpyDeviceToHost, vm_stream(vm)));
+    HIPCHECK(hipStreamSynchronize(vm_stream(vm)));
+}
+
+void vmPullS(vcsbeam_context *vm)
+{
+    int ns = vm->fine_sample_rate, nchan = vm->nfine_chan;
+    void *st = stage_buffer(vm, vm->S_size_bytes);
+    dim3 grid((ns + 63) / 64, (nchan + 63) / 64, vm->npointing * vm->out_nstokes);
+    hipLaunchKernelGGL(k_S_to_host, grid, dim3(256), 0, vm_stream(vm), (const float *)vm->d_S, ns,
+                       vm->out_nstokes, nchan, (float *)st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(vm->S, st, vm->S_size_bytes, hipMemcpyDeviceToHost, vm_stream(vm)));
+    HIPCHECK(hipStreamSynchronize(vm_stream(vm)));
+}
+
+void vmSendSToFits(vcsbeam_context *vm, mpi_psrfits *mpfs)
+{
+    // src/form_beam.cpp:653-673: renormalise per (p, st, c) to uint8 and copy
+    // the sub-integration arrays into the PSRFITS structs
+    int ns = vm->fine_sample_rate, nchan = vm->nfine_chan, nst = vm->out_nstokes;
+    hipStream_t s = vm_stream(vm);
+    hipLaunchKernelGGL(k_minmax_dev, dim3(vm->npointing * nst * nchan), dim3(256), 0, s,
+                       (const float *)vm->d_S, ns, vm->d_offsets, vm->d_scales);
+    HIPCHECK(hipGetLastError());
+    dim3 grid((ns + 63) / 64, (nchan + 63) / 64, vm->npointing * nst);
+    hipLaunchKernelGGL(k_quant_dev, grid, dim3(256), 0, s, (const float *)vm->d_S, ns, nst, nchan,
+                       vm->d_offsets, vm->d_scales, vm->d_Cscaled);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(vm->offsets, vm->d_offsets, vm->offsets_size, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(vm->scales, vm->d_scales, vm->scales_size, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(vm->Cscaled, vm->d_Cscaled,
+                            (size_t)vm->npointing * ns * nchan * nst, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (!mpfs) return;
+    for (unsigned int p = 0; p < vm->npointing; p++) {
+        vm_psrfits_sub *sub = &mpfs[p].coarse_chan_pf.sub;
+        if (sub->dat_offsets)
+            memcpy(sub->dat_offsets, vm->offsets + (size_t)p * nchan * nst, nchan * nst * sizeof(float));
+        if (sub->dat_scales)
+            memcpy(sub->dat_scales, vm->scales + (size_t)p * nchan * nst, nchan * nst * sizeof(float));
+        if (sub->data)
+            memcpy(sub->data, vm->Cscaled + (size_t)p * ns * nchan * nst, (size_t)ns * nchan * nst);
+    }
+}
+
+void prepare_data_buffer_fine(gpuDoubleComplex *data_buffer_fine, vcsbeam_context *vm,
+                              uintptr_t timestep_idx)
+{
+    // src/form_beam.cpp:753-780: copy e into the even/odd half of the
+    // 2-second ring; rows [s][c][pol] are contiguous, so copy per pointing
+    size_t nchan = vm->nfine_chan, npol = vm->obs_metadata->num_ant_pols;
+    int file_no = (int)(timestep_idx % 2);
+    size_t offset = (size_t)(file_no % 2) * vm->fine_sample_rate;
+    size_t per_s = nchan * npol;
+    for (unsigned int p = 0; p < vm->npointing; p++) {
+        const gpuDoubleComplex *src = vm->e + (size_t)p * vm->fine_sample_rate * per_s;
+        gpuDoubleComplex *dst = data_buffer_fine + ((size_t)p * vm->fine_sample_rate + offset) * per_s;
+        memcpy(dst, src, (size_t)vm->fine_sample_rate * per_s * sizeof(gpuDoubleComplex));
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,465 @@
+// vm_device.hip -- device-side context management: the vmMalloc*/vmFree*
+// family, host<->device pushes, streams, statistics buffers and context
+// teardown.
+//
+// Reference: MWA-vcsbeam src/metadata.c:448-1076 (allocation, push, streams),
+// src/metadata.c:323-377 (destroy_vcsbeam_context), src/form_beam.cpp:679-804.
+//
+// Every context kernel runs on one HIP stream per context (vm->dev->stream);
+// entry points whose outputs the host reads synchronise before returning.
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "vm_internal.h"
+
+#define HIPCHECK(x)                                                          \
+    do {                                                                     \
+        hipError_t e_ = (x);                                                 \
+        if (e_ != hipSuccess) VM_FATAL("HIP: %s (%s)", hipGetErrorString(e_), #x); \
+    } while (0)
+
+extern "C" {
+
+void *vm_host_alloc_pinned(size_t bytes)
+{
+    void *p = NULL;
+    HIPCHECK(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault));
+    return p;
+}
+
+void vm_host_free_pinned(void *p)
+{
+    if (p) HIPCHECK(hipHostFree(p));
+}
+
+static void *dmalloc(size_t bytes)
+{
+    void *p = NULL;
+    HIPCHECK(hipMalloc(&p, bytes ? bytes : 1));
+    return p;
+}
+
+static void dfree(void *p)
+{
+    if (p) HIPCHECK(hipFree(p));
+}
+
+vm_device_state *vm_dev(vcsbeam_context *vm)
+{
+    if (vm->dev) return vm->dev;
+    vm_device_state *d = (vm_device_state *)calloc(1, sizeof(vm_device_state));
+    if (!d) VM_FATAL("out of host memory");
+    hipStream_t s;
+    HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    d->stream = (void *)s;
+    d->weights_dirty = 1;
+    vm->dev = d;
+    return d;
+}
+
+void vm_dev_free(vcsbeam_context *vm)
+{
+    vm_device_state *d = vm->dev;
+    if (!d) return;
+    HIPCHECK(hipStreamSynchronize((hipStream_t)d->stream));
+    dfree(d->d_W);
+    dfree(d->d_Wm);
+    dfree(d->d_vscale);
+    dfree(d->d_stage);
+    dfree(d->d_pos);
+    dfree(d->d_freqs);
+    dfree(d->d_B);
+    dfree(d->d_unit);
+    dfree(d->d_ipfb_out);
+    dfree(d->d_nterm);
+    HIPCHECK(hipStreamDestroy((hipStream_t)d->stream));
+    free(d);
+    vm->dev = NULL;
+}
+
+gpuStream_t vmGetStream(vcsbeam_context *vm) { return (gpuStream_t)vm_dev(vm)->stream; }
+
+void vmSynchronize(vcsbeam_context *vm)
+{
+    HIPCHECK(hipStreamSynchronize((hipStream_t)vm_dev(vm)->stream));
+}
+
+/* ---------------------------------------------------------- host mallocs */
+void vmMallocJVHost(vcsbeam_context *vm)
+{
+    /* J^-1 v is never materialised (fused into the beamformer); the size is
+     * still recorded as in src/metadata.c:464-473 */
+    vm->Jv_size_bytes = (uintptr_t)vm->npointing * vm->obs_metadata->num_ants *
+                        vm->nfine_chan * vm->fine_sample_rate * sizeof(gpuDoubleComplex);
+    vm->Jv_P = vm->Jv_Q = NULL;
+}
+void vmFreeJVHost(vcsbeam_context *vm) { (void)vm; }
+
+void vmMallocEHost(vcsbeam_context *vm)
+{
+    vm->e_size_bytes = (uintptr_t)vm->npointing * vm->fine_sample_rate * vm->nfine_chan *
+                       vm->obs_metadata->num_ant_pols * sizeof(gpuDoubleComplex);
+    vm->e = (gpuDoubleComplex *)vm_host_alloc_pinned(vm->e_size_bytes);
+}
+void vmFreeEHost(vcsbeam_context *vm) { vm_host_free_pinned(vm->e); vm->e = NULL; }
+
+void vmMallocSHost(vcsbeam_context *vm)
+{
+    vm->S_size_bytes = (uintptr_t)vm->npointing * vm->nfine_chan * vm->out_nstokes *
+                       vm->fine_sample_rate * sizeof(float);
+    vm->S = vm_host_alloc_pinned(vm->S_size_bytes);
+}
+void vmFreeSHost(vcsbeam_context *vm) { vm_host_free_pinned(vm->S); vm->S = NULL; }
+
+void vmMallocJHost(vcsbeam_context *vm)
+{
+    vm->J_size_bytes = (uintptr_t)vm->npointing * vm->obs_metadata->num_ants * vm->nfine_chan *
+                       vm->obs_metadata->num_visibility_pols * sizeof(gpuDoubleComplex);
+    vm->J = (gpuDoubleComplex *)vm_host_alloc_pinned(vm->J_size_bytes);
+    memset(vm->J, 0, vm->J_size_bytes);
+}
+void vmFreeJHost(vcsbeam_context *vm) { vm_host_free_pinned(vm->J); vm->J = NULL; }
+
+void vmMallocDHost(vcsbeam_context *vm)
+{
+    vm->D_size_bytes = (uintptr_t)vm->obs_metadata->num_ants * vm->nfine_chan *
+                       vm->obs_metadata->num_visibility_pols * sizeof(gpuDoubleComplex);
+    vm->D = (gpuDoubleComplex *)vm_host_alloc_pinned(vm->D_size_bytes);
+    memset(vm->D, 0, vm->D_size_bytes);
+}
+void vmFreeDHost(vcsbeam_context *vm) { vm_host_free_pinned(vm->D); vm->D = NULL; }
+
+void vmMallocPQIdxsHost(vcsbeam_context *vm)
+{
+    vm->pol_idxs_size_bytes = vm->obs_metadata->num_ants * sizeof(uint32_t);
+    vm->polP_idxs = (uint32_t *)vm_host_alloc_pinned(vm->pol_idxs_size_bytes);
+    vm->polQ_idxs = (uint32_t *)vm_host_alloc_pinned(vm->pol_idxs_size_bytes);
+}
+void vmFreePQIdxsHost(vcsbeam_context *vm)
+{
+    vm_host_free_pinned(vm->polP_idxs);
+    vm_host_free_pinned(vm->polQ_idxs);
+    vm->polP_idxs = vm->polQ_idxs = NULL;
+}
+
+/* -------------------------------------------------------- device mallocs */
+void vmMallocVDevice(vcsbeam_context *vm)
+{
+    /* src/metadata.c:647-660: legacy input gets its own chunk buffer; MWAX
+     * input aliases the forward PFB's output */
+    if (vm->obs_metadata->mwa_version == VCSLegacyRecombined) {
+        vm->d_v_size_bytes = vm->bytes_per_second / vm->chunks_per_second;
+        vm->d_v = dmalloc(vm->d_v_size_bytes);
+    } else {
+        if (!vm->fpfb) VM_FATAL("vmMallocVDevice: forward PFB not initialised");
+        vm->d_v = vm->fpfb->d_vcs_data;
+        vm->d_v_size_bytes = vm->fpfb->d_vcs_size;
+    }
+}
+void vmFreeVDevice(vcsbeam_context *vm)
+{
+    if (vm->obs_metadata->mwa_version == VCSLegacyRecombined) dfree(vm->d_v);
+    vm->d_v = NULL;
+}
+
+void vmMallocJVDevice(vcsbeam_context *vm)
+{
+    vm->d_Jv_size_bytes = (uintptr_t)vm->npointing * vm->obs_metadata->num_ants *
+                          vm->nfine_chan * vm->fine_sample_rate *
+                          sizeof(gpuDoubleComplex) / vm->chunks_per_second;
+    vm->d_Jv_P = vm->d_Jv_Q = NULL; /* fused: never allocated */
+}
+void vmFreeJVDevice(vcsbeam_context *vm) { (void)vm; }
+
+void vmMallocEDevice(vcsbeam_context *vm)
+{
+    /* device layout [p][c][s][pol] complex float (half the reference's bytes) */
+    vm->d_e_size_bytes = (uintptr_t)vm->npointing * vm->fine_sample_rate * vm->nfine_chan *
+                         vm->obs_metadata->num_ant_pols * sizeof(gpuFloatComplex);
+    vm->d_e = (gpuDoubleComplex *)dmalloc(vm->d_e_size_bytes);
+}
+void vmFreeEDevice(vcsbeam_context *vm) { dfree(vm->d_e); vm->d_e = NULL; }
+
+void vmMallocSDevice(vcsbeam_context *vm)
+{
+    vm->d_S_size_bytes = (uintptr_t)vm->npointing * vm->nfine_chan * vm->out_nstokes *
+                         vm->fine_sample_rate * sizeof(float);
+    vm->d_S = dmalloc(vm->d_S_size_bytes);
+}
+void vmFreeSDevice(vcsbeam_context *vm) { dfree(vm->d_S); vm->d_S = NULL; }
+
+void vmMallocJDevice(vcsbeam_context *vm)
+{
+    vm->d_J_size_bytes = (uintptr_t)vm->npointing * vm->obs_metadata->num_ants * vm->nfine_chan *
+                         vm->obs_metadata->num_visibility_pols * sizeof(gpuDoubleComplex);
+    vm->d_J = (gpuDoubleComplex *)dmalloc(vm->d_J_size_bytes);
+}
+void vmFreeJDevice(vcsbeam_context *vm) { dfree(vm->d_J); vm->d_J = NULL; }
+
+void vmMallocDDevice(vcsbeam_context *vm)
+{
+    vm->d_D_size_bytes = (uintptr_t)vm->obs_metadata->num_ants * vm->nfine_chan *
+                         vm->obs_metadata->num_visibility_pols * sizeof(gpuDoubleComplex);
+    vm->d_D = (gpuDoubleComplex *)dmalloc(vm->d_D_size_bytes);
+}
+void vmFreeDDevice(vcsbeam_context *vm) { dfree(vm->d_D); vm->d_D = NULL; }
+
+void vmMallocPQIdxsDevice(vcsbeam_context *vm)
+{
+    vm->d_pol_idxs_size_bytes = vm->obs_metadata->num_ants * sizeof(uint32_t);
+    vm->d_polP_idxs = (uint32_t *)dmalloc(vm->d_pol_idxs_size_bytes);
+    vm->d_polQ_idxs = (uint32_t *)dmalloc(vm->d_pol_idxs_size_bytes);
+}
+void vmFreePQIdxsDevice(vcsbeam_context *vm)
+{
+    dfree(vm->d_polP_idxs);
+    dfree(vm->d_polQ_idxs);
+    vm->d_polP_idxs = vm->d_polQ_idxs = NULL;
+}
+
+/* ------------------------------------------------------------ set-up ----- */
+void vmSetMaxGPUMem(vcsbeam_context *vm, int nchunks)
+{
+    /* src/metadata.c:892-906 */
+    vm->chunks_per_second = nchunks;
+    if (nchunks <= 0 || vm->sample_rate % vm->chunks_per_second != 0) {
+        logger_timed_message(vm->log, "Bad number of chunks requested: must divide "
+                                      "number of samples per second of data.");
+        VM_FATAL("bad number of chunks (%d)", nchunks);
+    }
+    vm->d_v_size_bytes = vm->bytes_per_second / vm->chunks_per_second;
+    vm->d_Jv_size_bytes = vm->Jv_size_bytes / vm->chunks_per_second;
+}
+
+void vmPushChunk(vcsbeam_context *vm)
+{
+    /* src/metadata.c:913-923 (legacy input) */
+    logger_start_stopwatch(vm->log, "upload", false);
+    int chunk = vm->chunk_to_load % vm->chunks_per_second;
+    const char *src = (const char *)vm->v->buffer + (size_t)chunk * vm->d_v_size_bytes;
+    hipStream_t s = (hipStream_t)vm_dev(vm)->stream;
+    HIPCHECK(hipMemcpyAsync(vm->d_v, src, vm->d_v_size_bytes, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    logger_stop_stopwatch(vm->log, "upload");
+}
+
+void vmPushJ(vcsbeam_context *vm)
+{
+    hipStream_t s = (hipStream_t)vm_dev(vm)->stream;
+    HIPCHECK(hipMemcpyAsync(vm->d_J, vm->J, vm->J_size_bytes, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    vm->dev->weights_dirty = 1;
+}
+
+void vmPushD(vcsbeam_context *vm)
+{
+    hipStream_t s = (hipStream_t)vm_dev(vm)->stream;
+    HIPCHECK(hipMemcpyAsync(vm->d_D, vm->D, vm->D_size_bytes, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+void vmPushPolIdxLists(vcsbeam_context *vm)
+{
+    hipStream_t s = (hipStream_t)vm_dev(vm)->stream;
+    HIPCHECK(hipMemcpyAsync(vm->d_polQ_idxs, vm->polQ_idxs, vm->pol_idxs_size_bytes,
+                            hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(vm->d_polP_idxs, vm->polP_idxs, vm->pol_idxs_size_bytes,
+                            hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    vm->dev->weights_dirty = 1;
+}
+
+void vmCreateGeometricDelays(vcsbeam_context *vm)
+{
+    /* src/geometry.c:142-171 */
+    geometric_delays *g = &vm->gdelays;
+    g->npointings = vm->npointing;
+    g->nant = vm->obs_metadata->num_ants;
+    g->nchan = vm->nfine_chan;
+    g->obs_metadata = vm->obs_metadata;
+    g->chan_freqs_hz = (double *)malloc(sizeof(double) * vm->nfine_chan);
+    int cc = vm->coarse_chan_idxs_to_process[0];
+    CoarseChannel C = vm->obs_metadata->metafits_coarse_chans[cc];
+    uint32_t fine_chan_width = vm->obs_metadata->coarse_chan_width_hz / vm->nfine_chan;
+    for (int c = 0; c < vm->nfine_chan; c++)
+        g->chan_freqs_hz[c] = (double)C.chan_start_hz + (double)fine_chan_width * c;
+    size_t size = g->npointings * g->nant * g->nchan * sizeof(gpuDoubleComplex);
+    g->phi = (gpuDoubleComplex *)vm_host_alloc_pinned(size);
+    memset(g->phi, 0, size);
+    g->d_phi = (gpuDoubleComplex *)dmalloc(size);
+}
+
+void free_geometric_delays(geometric_delays *g)
+{
+    free(g->chan_freqs_hz);
+    vm_host_free_pinned(g->phi);
+    dfree(g->d_phi);
+    g->chan_freqs_hz = NULL;
+    g->phi = g->d_phi = NULL;
+}
+
+void vmPushPhi(vcsbeam_context *vm)
+{
+    geometric_delays *g = &vm->gdelays;
+    size_t size = g->npointings * g->nant * g->nchan * sizeof(gpuDoubleComplex);
+    hipStream_t s = (hipStream_t)vm_dev(vm)->stream;
+    HIPCHECK(hipMemcpyAsync(g->d_phi, g->phi, size, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    vm->dev->weights_dirty = 1;
+}
+
+void vmCreatePrimaryBeam(vcsbeam_context *vm)
+{
+    /* src/primary_beam.c:79-127 allocates B [p][a][npol*npol]; the FEE model
+     * itself (hyperbeam) is outside this library, so B starts as the
+     * identity and the caller may overwrite it */
+    primary_beam *pb = &vm->pb;
+    pb->npointings = vm->npointing;
+    pb->nant = vm->obs_metadata->num_ants;
+    pb->npol = vm->obs_metadata->num_visibility_pols;
+    pb->obs_metadata = vm->obs_metadata;
+    size_t n = pb->npointings * pb->nant * pb->npol;
+    pb->B = (gpuDoubleComplex *)calloc(n, sizeof(gpuDoubleComplex));
+    for (size_t j = 0; j < pb->npointings * pb->nant; j++) {
+        pb->B[4 * j + 0].x = 1.0;
+        pb->B[4 * j + 3].x = 1.0;
+    }
+}
+
+void vmCreateCudaStreams(vcsbeam_context *vm)
+{
+    /* src/metadata.c:1023-1032.  The streams are created for callers that
+     * use them; this library issues all work on vm->dev->stream. */
+    vm->streams = (gpuStream_t *)malloc(sizeof(gpuStream_t) * (vm->npointing ? vm->npointing : 1));
+    for (unsigned int p = 0; p < vm->npointing; p++) {
+        hipStream_t s;
+        HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        vm->streams[p] = (gpuStream_t)s;
+    }
+}
+
+void vmDestroyCudaStreams(vcsbeam_context *vm)
+{
+    if (!vm->streams) return;
+    for (unsigned int p = 0; p < vm->npointing; p++)
+        HIPCHECK(hipStreamDestroy((hipStream_t)vm->streams[p]));
+    free(vm->streams);
+    vm->streams = NULL;
+}
+
+void vmCreateStatistics(vcsbeam_context *vm, mpi_psrfits *mpfs)
+{
+    /* src/metadata.c:1061-1076; the PSRFITS sub-integration size is
+     * nchan * nstokes * nsamples bytes */
+    size_t nchan = vm->nfine_chan;
+    vm->offsets_size = vm->npointing * nchan * vm->out_nstokes * sizeof(float);
+    vm->scales_size = vm->npointing * nchan * vm->out_nstokes * sizeof(float);
+    size_t per = (mpfs && mpfs[0].coarse_chan_pf.sub.bytes_per_subint > 0)
+                     ? (size_t)mpfs[0].coarse_chan_pf.sub.bytes_per_subint
+                     : nchan * vm->out_nstokes * (size_t)vm->fine_sample_rate;
+    vm->Cscaled_size = vm->npointing * per;
+    vm->d_offsets = (float *)dmalloc(vm->offsets_size);
+    vm->d_scales = (float *)dmalloc(vm->scales_size);
+    vm->d_Cscaled = (uint8_t *)dmalloc(vm->Cscaled_size);
+    vm->offsets = (float *)vm_host_alloc_pinned(vm->offsets_size);
+    vm->scales = (float *)vm_host_alloc_pinned(vm->scales_size);
+    vm->Cscaled = (uint8_t *)vm_host_alloc_pinned(vm->Cscaled_size);
+}
+
+void vmDestroyStatistics(vcsbeam_context *vm)
+{
+    vm_host_free_pinned(vm->offsets);
+    vm_host_free_pinned(vm->scales);
+    vm_host_free_pinned(vm->Cscaled);
+    dfree(vm->d_offsets);
+    dfree(vm->d_scales);
+    dfree(vm->d_Cscaled);
+    vm->offsets = vm->scales = vm->d_offsets = vm->d_scales = NULL;
+    vm->Cscaled = vm->d_Cscaled = NULL;
+}
+
+float *create_pinned_data_buffer(size_t size)
+{
+    float *p = (float *)vm_host_alloc_pinned(size);
+    memset(p, 0, size);
+    return p;
+}
+
+gpuDoubleComplex *create_data_buffer_fine(int npointing, int nsamples, int nchan, int npol)
+{
+    size_t n = (size_t)npointing * nsamples * nchan * npol * sizeof(gpuDoubleComplex);
+    gpuDoubleComplex *p = (gpuDoubleComplex *)malloc(n);
+    if (!p) VM_FATAL("out of host memory");
+    memset(p, 0, n);
+    return p;
+}
+
+void allocate_input_output_arrays(void **data, void **d_data, size_t size)
+{
+    *data = vm_host_alloc_pinned(size);
+    *d_data = dmalloc(size);
+}
+
+void free_input_output_arrays(void *data, void *d_data)
+{
+    vm_host_free_pinned(data);
+    dfree(d_data);
+}
+
+/* ---------------------------------------------- device-memory helpers ---- */
+void *vmDeviceMalloc(size_t bytes) { return dmalloc(bytes); }
+void vmDeviceFree(void *p) { dfree(p); }
+void vmMemcpyHtoD(void *dst, const void *src, size_t bytes)
+{
+    HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+void vmMemcpyDtoH(void *dst, const void *src, size_t bytes)
+{
+    HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+void vmMemsetDevice(void *dst, int value, size_t bytes) { HIPCHECK(hipMemset(dst, value, bytes)); }
+void vmDeviceSynchronize(void) { HIPCHECK(hipDeviceSynchronize()); }
+int vmDeviceCount(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+void *vmEventCreate(void)
+{
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return (void *)e;
+}
+void vmEventRecord(void *ev, gpuStream_t stream)
+{
+    HIPCHECK(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+}
+float vmEventElapsedMs(void *start, void *stop)
+{
+    float ms = 0.f;
+    HIPCHECK(hipEventSynchronize((hipEvent_t)stop));
+    HIPCHECK(hipEventElapsedTime(&ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return ms;
+}
+void vmEventDestroy(void *ev) { HIPCHECK(hipEventDestroy((hipEvent_t)ev)); }
+
+void destroy_vcsbeam_context(vcsbeam_context *vm)
+{
+    /* src/metadata.c:323-377: frees what the context itself owns */
+    if (!vm) return;
+    if (vm->fpfb) vmFreeForwardPFB(vm->fpfb);
+    vm->fpfb = NULL;
+    if (vm->v) vmFreeVHost(vm);
+    free_pfb_filter(vm->analysis_filter);
+    free_pfb_filter(vm->synth_filter);
+    free(vm->gps_seconds_to_process);
+    free(vm->coarse_chan_idxs_to_process);
+    free(vm->cal_coarse_chan_idxs_to_process);
+    vm_dev_free(vm);
+    destroy_logger(vm->log);
+    free(vm);
+}
+
+} // extern "C"

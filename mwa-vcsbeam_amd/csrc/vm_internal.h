@@ -1,0 +1,63 @@
+/*
+ * vm_internal.h -- library-private declarations shared by the host C module
+ * and the HIP translation units.  Not part of the C-ABI.
+ */
+#ifndef VM_INTERNAL_H
+#define VM_INTERNAL_H
+
+#include "vcsbeam.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fatal error: print and exit, like the reference's gpuAssert / GPU_CHECK_ERROR
+ * (src/pfb.cpp:19-33, src/gpu_macros.h:27-35). */
+void vm_fatal(const char *file, int line, const char *fmt, ...);
+#define VM_FATAL(...) vm_fatal(__FILE__, __LINE__, __VA_ARGS__)
+
+/* Device-side state hung off vcsbeam_context.dev (created lazily). */
+struct vm_device_state {
+    void *stream;             /* hipStream_t on which all context work runs */
+    /* FP32 beamformer weights W = phi * J^-1, complex float
+       [p][c][a][2][2] */
+    void *d_W;
+    size_t W_bytes;
+    int weights_dirty;
+    /* MFMA weights: [p-block][c][row][k] packed half/bf16 planes */
+    void *d_Wm;
+    size_t Wm_bytes;
+    /* MFMA: per-channel scale applied to voltages before conversion */
+    float *d_vscale;
+    /* staging for host-layout pulls */
+    void *d_stage;
+    size_t stage_bytes;
+    /* tile positions (E, N, H, L) per antenna for device phi */
+    double *d_pos;
+    double *d_freqs;
+    /* primary beam B [p][a][2][2] complex double on device */
+    void *d_B;
+    size_t B_bytes;
+    /* unit vectors per pointing (uE, uN, uH) */
+    double *d_unit;
+    size_t unit_bytes;
+    /* inverse-PFB device buffers for the device-resident path */
+    void *d_ipfb_out;
+    size_t ipfb_out_bytes;
+    /* per-channel N-term scratch for the MFMA form (floats) */
+    void *d_nterm;
+    size_t nterm_bytes;
+};
+
+/* Host helpers implemented in vm_host.c */
+vm_device_state *vm_dev(vcsbeam_context *vm); /* create on first use */
+void vm_dev_free(vcsbeam_context *vm);
+
+/* Pinned host memory (hipHostMalloc), implemented in vm_device.hip */
+void *vm_host_alloc_pinned(size_t bytes);
+void vm_host_free_pinned(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
