@@ -6,7 +6,9 @@ voltages, already channelised by the fused forward PFB and resident in HBM
 (VM_FLT, 128 fine channels x 10,000 samples x 256 inputs), beamformed towards
 512 directions uniform within 15 deg of boresight: per-second FP64 phases and
 Jones inverses on the GPU, weights, the beamformer and full-Stokes detection.
-Metric: beam-samples/s = 512 beams x 128 channels x 10,000 samples / step time.
+Metric: beam-samples/s = 512 beams x 128 channels x 10,000 samples / step time,
+for the precision MFMA form (FP16 voltages, split hi+lo FP16 weights, FP32
+accumulation); the plain FP32 form is timed beside it on the same data.
 
 Also reported: the fused forward-PFB kernel on the same second (HBM roofline),
 the dominant kernel's roofline, and the FP64 CPU oracle timed on a bounded
@@ -82,13 +84,42 @@ def cpu_baseline(sess, obs, az, za, target_s=10.0):
             "cores": int(orc.lib().orc_num_threads())}
 
 
+def kernel_ms(lib, sess, name):
+    import ctypes as C
+    n = C.c_long(0)
+    ms = lib.vmKernelTimeMs(sess.vm, name.encode(), C.byref(n))
+    return ms, n.value
+
+
+def time_form(lib, sess, form, steps, warmup):
+    """Wall-clock ms per step and live per-kernel event timing for one form."""
+    sess.v.beamform_form = form
+    sess.geometry()          # marks the weights dirty for the new form
+    sess.beamform()
+    for _ in range(warmup):
+        sess.geometry()
+        sess.beamform()
+    lib.vmDeviceSynchronize()
+    lib.vmSetKernelTiming(sess.vm, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sess.geometry()
+        sess.beamform()
+    lib.vmDeviceSynchronize()
+    t1 = time.perf_counter()
+    lib.vmSetKernelTiming(sess.vm, 0)
+    return (t1 - t0) * 1e3 / steps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--beams", type=int, default=512)
+    ap.add_argument("--fp32-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the beamforming path shards over coarse channels "
@@ -105,69 +136,74 @@ def main():
     obs = Observation(ntiles=128, blocks_per_second=20)
     az, za = directions_in_cap(args.beams, 15.0)
     sess = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
-                            nstokes=4, form=pkg.VM_BF_FP32)
+                            nstokes=4, form=pkg.VM_BF_MFMA_FP16)
     sess.read(0)
-    stream = sess.stream
     print("[bench] forward PFB", file=sys.stderr, flush=True)
     sess.pfb()
-    pfb_ms = event_time(lib, stream, lambda: lib.vmWOLAChunk(sess.vm), 5)
+    lib.vmSetKernelTiming(sess.vm, 1)
+    for _ in range(5):
+        lib.vmWOLAChunk(sess.vm)
+    pfb_ms, _ = kernel_ms(lib, sess, "pfb")
+    lib.vmSetKernelTiming(sess.vm, 0)
     nsamp_in = obs.ninput * obs.nspectra * 128
     pfb_bytes = nsamp_in * 10  # 2 B in + 8 B out per input complex sample
-    sess.geometry()
-    sess.beamform()
-
-    def step():
-        sess.geometry()
-        sess.beamform()
-
-    print("[bench] warmup", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        step()
-    lib.vmDeviceSynchronize()
-    print("[bench] timed steps", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    lib.vmDeviceSynchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
     beam_samples = args.beams * 128 * obs.nspectra
-    value = beam_samples / ((t1 - t0) / args.steps)
 
-    bf_ms = event_time(lib, stream, lambda: lib.vmBeamformChunk(sess.vm), 3)
+    print("[bench] MFMA FP16 form: warmup + timed steps", file=sys.stderr, flush=True)
+    ms_step = time_form(lib, sess, pkg.VM_BF_MFMA_FP16, args.steps, args.warmup)
+    gemm_ms, gemm_n = kernel_ms(lib, sess, "beamform_mfma")
+    nterm_ms, _ = kernel_ms(lib, sess, "nterms")
+    value = beam_samples / (ms_step * 1e-3)
+    S_mfma = sess.stokes()
     flops = 4096.0 * beam_samples  # complex GEMM: 8 x 2 pols x 256 tile-pols per beam-sample
-    achieved = flops / (bf_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "k_beamform_fp32", "kernel_ms": round(bf_ms, 4)}
-    S_gpu = sess.stokes()
-
+    achieved = flops / (gemm_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP16_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "k_beamform_mfma<fp16>", "kernel_ms": round(gemm_ms, 4),
+                "launches": gemm_n}
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "beam-samples/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16 (split hi+lo f16 weights, f32 accumulate)",
         "data": "synthetic (seeded Gaussian 8+8-bit voltages, sigma 16)",
         "config": {"workload": "128 tiles x 2 pols, 1 coarse channel x 1 s, 512 beams within "
-                               "15 deg of boresight, plain FP32 beamformer, Stokes IQUV",
+                               "15 deg of boresight, MFMA FP16 beamformer, Stokes IQUV",
                    "tiles": 128, "beams": args.beams, "fine_channels": 128,
-                   "samples": obs.nspectra, "form": "fp32"},
+                   "samples": obs.nspectra, "form": "mfma-fp16"},
         "roofline": roofline,
         "stages": {"pfb_fused_ms_per_ch_s": round(pfb_ms, 4),
                    "pfb_GBps": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9, 1),
-                   "pfb_hbm_frac": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                   "pfb_hbm_frac": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "nterms_ms": round(nterm_ms, 4)},
     }
+    S_fp32 = None
+    if not args.no_fp32:
+        print("[bench] plain FP32 form", file=sys.stderr, flush=True)
+        ms32 = time_form(lib, sess, pkg.VM_BF_FP32, args.fp32_steps, 1)
+        k32, _ = kernel_ms(lib, sess, "beamform_fp32")
+        S_fp32 = sess.stokes()
+        result["fp32_form"] = {"value": round(beam_samples / (ms32 * 1e-3), 1),
+                               "ms_per_step": round(ms32, 3), "kernel_ms": round(k32, 3),
+                               "mfma_speedup": round(ms32 / ms_step, 2)}
     if not args.no_cpu:
         print("[bench] CPU baseline (FP64 oracle)", file=sys.stderr, flush=True)
         cb = cpu_baseline(sess, obs, az, za)
         n = cb["nsamp"]
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as orc
-        errs = [orc.rel_err(S_gpu[:, :n, st, :], cb["S"][:, :n, st, :]) for st in range(4)]
         result["cpu_baseline"] = {"value": round(cb["rate"], 1), "unit": "beam-samples/s",
                                   "cores": cb["cores"], "kind": "port",
                                   "sample": f"{args.beams} beams x 128 ch x {n} samples of the same "
                                             f"second, FP64 oracle beamformer ({cb['seconds']:.1f} s)"}
-        result["parity"] = {"stokes_rel_err_vs_fp64": [float(f"{e:.3e}") for e in errs]}
+        par = {}
+        for name, S in (("mfma_fp16", S_mfma), ("fp32", S_fp32)):
+            if S is None:
+                continue
+            e = np.array([[orc.rel_err(S[p, :n, st, :], cb["S"][p, :n, st, :]) for st in range(4)]
+                          for p in range(S.shape[0])])
+            par[name] = {"max_beam_rel_err_IQUV": [float(f"{x:.3e}") for x in e.max(axis=0)]}
+        result["parity_vs_fp64"] = par
     sess.close()
     print(json.dumps(result))
 

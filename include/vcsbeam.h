@@ -703,6 +703,13 @@ void vmEventRecord(void *ev, gpuStream_t stream);
 float vmEventElapsedMs(void *start, void *stop);
 void vmEventDestroy(void *ev);
 
+/* Per-kernel timing with HIP events on the context stream: when on, each
+ * launch of the named kernels ("pfb", "beamform_fp32", "beamform_mfma",
+ * "nterms", "ipfb", "geometry") is bracketed by events; vmKernelTimeMs returns
+ * the average milliseconds per launch since timing was switched on. */
+void vmSetKernelTiming(vcsbeam_context *vm, int on);
+double vmKernelTimeMs(vcsbeam_context *vm, const char *kernel, long *count);
+
 /* Library build/version string. */
 const char *vmLibraryVersion(void);
 

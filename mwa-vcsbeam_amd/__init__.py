@@ -278,6 +278,8 @@ _SIGS = {
     "calc_array_factor": (C.c_double, [C.POINTER(MetafitsMetadata), C.c_uint32,
                                        C.POINTER(BeamGeom), C.POINTER(BeamGeom)]),
     "vmStructLayout": (C.c_int, [C.c_char_p, C.POINTER(C.c_size_t), C.c_int]),
+    "vmSetKernelTiming": (None, [_CTX, C.c_int]),
+    "vmKernelTimeMs": (C.c_double, [_CTX, C.c_char_p, C.POINTER(C.c_long)]),
     "vmDeviceMalloc": (C.c_void_p, [C.c_size_t]),
     "vmDeviceFree": (None, [C.c_void_p]),
     "vmMemcpyHtoD": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -408,7 +408,9 @@ void vm_build_weights(vcsbeam_context *vm)
 void vmApplyJChunk(vcsbeam_context *vm)
 {
     // The J^-1 v product is folded into the beamformer; here the combined
-    // per-second weights are refreshed.
+    // per-second weights are refreshed (the MFMA form builds its own split
+    // weights inside vmBeamformChunkMFMA).
+    if (vm->beamform_form != VM_BF_FP32) return;
     vm_build_weights(vm);
     HIPCHECK(hipStreamSynchronize(vm_stream(vm)));
 }
@@ -446,11 +448,13 @@ void vmBeamformChunk(vcsbeam_context *vm)
     int chunk = vm->chunk_to_load % vm->chunks_per_second;
     int ns = vm->fine_sample_rate / vm->chunks_per_second;
     float invw = (float)(1.0 / (double)vm->num_not_flagged);
+    vm_timer_start(vm->dev, VM_T_BF_FP32);
     vm_beamform_fp32_launch(vm->d_v, vm->datatype, ns, vm->nfine_chan,
                             (int)vm->obs_metadata->num_rf_inputs, vm->dev->d_W, vm->d_polQ_idxs,
                             vm->d_polP_idxs, (int)vm->obs_metadata->num_ants, (int)vm->npointing,
                             invw, vm->out_nstokes, chunk * ns, vm->fine_sample_rate,
                             (float *)vm->d_S, vm->d_e, vm_stream(vm));
+    vm_timer_stop(vm->dev, VM_T_BF_FP32);
     HIPCHECK(hipStreamSynchronize(vm_stream(vm)));
 }
 

@@ -286,7 +286,7 @@ void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_
     if (d->ipfb_out_bytes < need) {
         if (d->d_ipfb_out) HIPCHECK(hipFree(d->d_ipfb_out));
         HIPCHECK(hipMalloc(&d->d_ipfb_out, need));
-        HIPCHECK(hipMemset(d->d_ipfb_out, 0, need));
+        HIPCHECK(hipMemsetAsync(d->d_ipfb_out, 0, need, s)); // same stream as its users
         d->ipfb_out_bytes = need;
     }
     cf *din = (cf *)d->d_ipfb_out;

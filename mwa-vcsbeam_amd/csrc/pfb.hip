@@ -77,17 +77,41 @@ __device__ __forceinline__ size_t mwax_char2(size_t s, int i, int I)
 }
 
 // ------------------------------------------------------------ fused -------
+typedef short vm_short2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int c)
+{
+    // v_dot2_i32_i16: a.lo*b.lo + a.hi*b.hi + c, exact in int32 here
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(vm_short2, a),
+                                  __builtin_bit_cast(vm_short2, b), c, false);
+}
+
+__device__ __forceinline__ uint32_t pack16(int lo, int hi)
+{
+    return (uint32_t)(lo & 0xffff) | ((uint32_t)hi << 16);
+}
+
 // grid (ceil(nspc/TILE_M), I), block 256.  htr points at the chunk's first
 // voltage block (the margin); out is VM_FLT [c][i_vcs][nspc].
-template <bool FPGA, int P>
+//
+// WOLA: b[m][n] = sum_p h[KP - Kp - n - 1] x[(m + 500 - P)K + pK + n]
+// (src/pfb.cpp:149-161).  Lane (col, g) owns samples n = 2col, 2col+1 of
+// spectra 8g..8g+7.  Consecutive-block sample pairs (x[q][n], x[q+1][n]) are
+// assembled as sign-extended int16 pairs with v_perm_b32 and multiplied by
+// tap pairs with v_dot2_i32_i16.  SPLIT: taps wider than int16 are split
+// exactly as h = 2^15 hh + hl (0 <= hl < 2^15) and summed as two products;
+// int32 wrap-around is identical to the reference's int atomics.
+template <bool FPGA, bool SPLIT>
 __global__ __launch_bounds__(256) void k_pfb_fused(const uint32_t *__restrict__ htr,
                                                    const int *__restrict__ h,
                                                    const int *__restrict__ out_idx, int I,
                                                    int nspc, int imag_first,
                                                    cf *__restrict__ out)
 {
-    __shared__ uint32_t win[(TILE_M + P - 1) * 64];
-    __shared__ cf F[TILE_M * ROW];
+    constexpr int P = 12;
+    constexpr int NW = 8 + P - 1;                 // window blocks per lane
+    __shared__ cf F[TILE_M * ROW];                // FFT rows (33 KB)
+    uint32_t *win = (uint32_t *)F;                // input window aliases F (11 KB)
 
     const int t = threadIdx.x;
     const int i = blockIdx.y;
@@ -95,44 +119,82 @@ __global__ __launch_bounds__(256) void k_pfb_fused(const uint32_t *__restrict__ 
     const int nvalid = min(TILE_M, nspc - m0);
     const int nblk = nvalid + P - 1;
 
-    // Phase A: stage the input window (blocks m0+500-P .. m0+nvalid+498)
-    for (int j = t; j < (TILE_M + P - 1) * 64; j += 256) {
-        int ql = j >> 6, col = j & 63;
-        uint32_t val = 0;
+    // Phase A: stage blocks q = m0 + ql + 500 - P, 16 bytes per lane
+    for (int j = t; j < (TILE_M + P - 1) * 16; j += 256) {
+        int ql = j >> 4, c4 = j & 15;
+        uint4 val = make_uint4(0, 0, 0, 0);
         if (ql < nblk) {
-            size_t s = (size_t)(m0 + ql + 500 - P) * KCH + 2 * col;
-            val = htr[mwax_char2(s, i, I) >> 1];
+            int q = m0 + ql + 500 - P;               // 128-sample block index
+            int vb = q / 500;                         // MWAX voltage block
+            int within = (q - vb * 500) * KCH;        // sample offset in the block
+            size_t dw = ((size_t)vb * I + i) * (VSPVB / 2) + (within >> 1);
+            val = ((const uint4 *)(htr + dw))[c4];
         }
-        win[j] = val;
+        ((uint4 *)win)[j] = val;
     }
     const int col = t & 63, g = t >> 6;
-    int h0[P], h1[P];
+    // tap pairs (h_p, h_{p+1}) for p = 0, 2, ..., P-2, for n0 and n1
+    uint32_t Hl0[P / 2], Hl1[P / 2], Hh0[P / 2], Hh1[P / 2];
 #pragma unroll
-    for (int p = 0; p < P; p++) {
-        h0[p] = h[KCH * P - KCH * p - 2 * col - 1];
-        h1[p] = h[KCH * P - KCH * p - 2 * col - 2];
+    for (int k = 0; k < P / 2; k++) {
+        int a0 = h[KCH * P - KCH * (2 * k) - 2 * col - 1];
+        int b0 = h[KCH * P - KCH * (2 * k + 1) - 2 * col - 1];
+        int a1 = h[KCH * P - KCH * (2 * k) - 2 * col - 2];
+        int b1 = h[KCH * P - KCH * (2 * k + 1) - 2 * col - 2];
+        if (SPLIT) {
+            Hl0[k] = pack16(a0 & 0x7fff, b0 & 0x7fff);
+            Hl1[k] = pack16(a1 & 0x7fff, b1 & 0x7fff);
+            Hh0[k] = pack16(a0 >> 15, b0 >> 15);
+            Hh1[k] = pack16(a1 >> 15, b1 >> 15);
+        } else {
+            Hl0[k] = pack16(a0, b0);
+            Hl1[k] = pack16(a1, b1);
+        }
     }
     __syncthreads();
 
-    // Phase B: WOLA for spectra 8g .. 8g+7, samples n = 2col, 2col+1
-    uint32_t w[8 + P - 1];
+    // Phase B: WOLA into registers
+    uint32_t w[NW];
 #pragma unroll
-    for (int q = 0; q < 8 + P - 1; q++) w[q] = win[(8 * g + q) * 64 + col];
+    for (int q = 0; q < NW; q++) w[q] = win[(8 * g + q) * 64 + col];
+    int acc[8][4];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-        int ar0 = 0, ai0 = 0, ar1 = 0, ai1 = 0;
+        int ar0 = 0, ai0 = 0, ar1 = 0, ai1 = 0, br0 = 0, bi0 = 0, br1 = 0, bi1 = 0;
 #pragma unroll
-        for (int p = 0; p < P; p++) {
-            uint32_t x = w[j + p];
-            ar0 += h0[p] * (int)(int8_t)(x & 0xff);
-            ai0 += h0[p] * (int)(int8_t)((x >> 8) & 0xff);
-            ar1 += h1[p] * (int)(int8_t)((x >> 16) & 0xff);
-            ai1 += h1[p] * (int)(int8_t)(x >> 24);
+        for (int k = 0; k < P / 2; k++) {
+            uint32_t lo = w[j + 2 * k], hi = w[j + 2 * k + 1];
+            // bytes of lo: re_n0, im_n0, re_n1, im_n1 (block q); hi: block q+1
+            uint32_t I0 = __builtin_amdgcn_perm(hi, lo, 0x0A050801u);
+            uint32_t I1 = __builtin_amdgcn_perm(hi, lo, 0x0B070903u);
+            uint32_t R0 = __builtin_amdgcn_perm(hi << 8, lo << 8, 0x0A050801u);
+            uint32_t R1 = __builtin_amdgcn_perm(hi << 8, lo << 8, 0x0B070903u);
+            ar0 = dot2(Hl0[k], R0, ar0);
+            ai0 = dot2(Hl0[k], I0, ai0);
+            ar1 = dot2(Hl1[k], R1, ar1);
+            ai1 = dot2(Hl1[k], I1, ai1);
+            if (SPLIT) {
+                br0 = dot2(Hh0[k], R0, br0);
+                bi0 = dot2(Hh0[k], I0, bi0);
+                br1 = dot2(Hh1[k], R1, br1);
+                bi1 = dot2(Hh1[k], I1, bi1);
+            }
         }
-        const float post = 1.0f / KCH; // the /K of the pack step, exact
+        if (SPLIT) {
+            ar0 = (int)((uint32_t)ar0 + ((uint32_t)br0 << 15));
+            ai0 = (int)((uint32_t)ai0 + ((uint32_t)bi0 << 15));
+            ar1 = (int)((uint32_t)ar1 + ((uint32_t)br1 << 15));
+            ai1 = (int)((uint32_t)ai1 + ((uint32_t)bi1 << 15));
+        }
+        acc[j][0] = ar0; acc[j][1] = ai0; acc[j][2] = ar1; acc[j][3] = ai1;
+    }
+    __syncthreads(); // window no longer needed: F may overwrite it
+    const float post = 1.0f / KCH; // the /K of the pack step, exact
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
         cf *dst = &F[(8 * g + j) * ROW + 2 * col];
-        dst[0] = cf{demote<FPGA>(ar0, post), demote<FPGA>(ai0, post)};
-        dst[1] = cf{demote<FPGA>(ar1, post), demote<FPGA>(ai1, post)};
+        dst[0] = cf{demote<FPGA>(acc[j][0], post), demote<FPGA>(acc[j][1], post)};
+        dst[1] = cf{demote<FPGA>(acc[j][2], post), demote<FPGA>(acc[j][3], post)};
     }
     __syncthreads();
 
@@ -408,17 +470,25 @@ static cf *flt_target(vcsbeam_context *vm)
 // Launch of the fused kernel on arbitrary device buffers (used by the chunk
 // API and by the benchmark / batched paths).
 void vm_pfb_fused_launch(const void *d_htr, const int *d_h, const int *d_out_idx, int I,
-                         int nspc, int P, int fpga, int imag_first, void *d_out,
+                         int nspc, int P, int fpga, int imag_first, int split, void *d_out,
                          hipStream_t stream)
 {
     dim3 grid((nspc + TILE_M - 1) / TILE_M, I), block(256);
     if (P != 12) VM_FATAL("fused PFB kernel is built for 12 taps (got %d); use PFB_STAGED", P);
-    if (fpga)
-        hipLaunchKernelGGL((k_pfb_fused<true, 12>), grid, block, 0, stream,
-                           (const uint32_t *)d_htr, d_h, d_out_idx, I, nspc, imag_first, (cf *)d_out);
+    const uint32_t *x = (const uint32_t *)d_htr;
+    cf *o = (cf *)d_out;
+    if (fpga && split)
+        hipLaunchKernelGGL((k_pfb_fused<true, true>), grid, block, 0, stream, x, d_h, d_out_idx, I,
+                           nspc, imag_first, o);
+    else if (fpga)
+        hipLaunchKernelGGL((k_pfb_fused<true, false>), grid, block, 0, stream, x, d_h, d_out_idx,
+                           I, nspc, imag_first, o);
+    else if (split)
+        hipLaunchKernelGGL((k_pfb_fused<false, true>), grid, block, 0, stream, x, d_h, d_out_idx,
+                           I, nspc, imag_first, o);
     else
-        hipLaunchKernelGGL((k_pfb_fused<false, 12>), grid, block, 0, stream,
-                           (const uint32_t *)d_htr, d_h, d_out_idx, I, nspc, imag_first, (cf *)d_out);
+        hipLaunchKernelGGL((k_pfb_fused<false, false>), grid, block, 0, stream, x, d_h, d_out_idx,
+                           I, nspc, imag_first, o);
     HIPCHECK(hipGetLastError());
 }
 
@@ -435,10 +505,13 @@ void vmWOLAChunk(vcsbeam_context *vm)
         HIPCHECK(hipGetLastError());
     } else {
         // fused: WOLA + rounding + FFT + reorder in one pass
+        vm_timer_start(vm_dev(vm), VM_T_PFB);
         vm_pfb_fused_launch(fpfb->d_htr_data, fpfb->d_filter_coeffs, fpfb->d_i_output_idx, fpfb->I,
                             fpfb->nspectra_per_chunk, fpfb->P,
                             (fpfb->flags & PFB_EMULATE_FPGA) != 0,
-                            (fpfb->flags & PFB_IMAG_PART_FIRST) != 0, flt_target(vm), s);
+                            (fpfb->flags & PFB_IMAG_PART_FIRST) != 0, !fpfb->filter_fits_int16,
+                            flt_target(vm), s);
+        vm_timer_stop(vm_dev(vm), VM_T_PFB);
     }
     HIPCHECK(hipStreamSynchronize(s));
     logger_stop_stopwatch(vm->log, "pfb-wola");

@@ -418,7 +418,12 @@ void vmMemcpyDtoH(void *dst, const void *src, size_t bytes)
 {
     HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
 }
-void vmMemsetDevice(void *dst, int value, size_t bytes) { HIPCHECK(hipMemset(dst, value, bytes)); }
+void vmMemsetDevice(void *dst, int value, size_t bytes)
+{
+    /* complete before returning: later work may run on a non-blocking stream */
+    HIPCHECK(hipMemset(dst, value, bytes));
+    HIPCHECK(hipDeviceSynchronize());
+}
 void vmDeviceSynchronize(void) { HIPCHECK(hipDeviceSynchronize()); }
 int vmDeviceCount(void)
 {
@@ -460,6 +465,57 @@ void destroy_vcsbeam_context(vcsbeam_context *vm)
     vm_dev_free(vm);
     destroy_logger(vm->log);
     free(vm);
+}
+
+} // extern "C"
+
+/* ------------------------------------------------------ kernel timing ---- */
+extern "C" {
+
+static const char *vm_timer_names[VM_T_NSLOT] = {"pfb", "beamform_fp32", "beamform_mfma",
+                                                 "nterms", "ipfb", "geometry"};
+
+void vm_timer_start(vm_device_state *d, int slot)
+{
+    if (!d->timing) return;
+    if (!d->ev[slot][0]) {
+        HIPCHECK(hipEventCreate((hipEvent_t *)&d->ev[slot][0]));
+        HIPCHECK(hipEventCreate((hipEvent_t *)&d->ev[slot][1]));
+    }
+    HIPCHECK(hipEventRecord((hipEvent_t)d->ev[slot][0], (hipStream_t)d->stream));
+}
+
+void vm_timer_stop(vm_device_state *d, int slot)
+{
+    if (!d->timing || !d->ev[slot][0]) return;
+    HIPCHECK(hipEventRecord((hipEvent_t)d->ev[slot][1], (hipStream_t)d->stream));
+    HIPCHECK(hipEventSynchronize((hipEvent_t)d->ev[slot][1]));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, (hipEvent_t)d->ev[slot][0], (hipEvent_t)d->ev[slot][1]));
+    d->t_ms[slot] += ms;
+    d->t_n[slot] += 1;
+}
+
+void vmSetKernelTiming(vcsbeam_context *vm, int on)
+{
+    vm_device_state *d = vm_dev(vm);
+    d->timing = on;
+    for (int i = 0; i < VM_T_NSLOT; i++) {
+        d->t_ms[i] = 0.0;
+        d->t_n[i] = 0;
+    }
+}
+
+double vmKernelTimeMs(vcsbeam_context *vm, const char *kernel, long *count)
+{
+    vm_device_state *d = vm_dev(vm);
+    for (int i = 0; i < VM_T_NSLOT; i++)
+        if (strcmp(kernel, vm_timer_names[i]) == 0) {
+            if (count) *count = d->t_n[i];
+            return d->t_n[i] ? d->t_ms[i] / d->t_n[i] : 0.0;
+        }
+    if (count) *count = 0;
+    return -1.0;
 }
 
 } // extern "C"

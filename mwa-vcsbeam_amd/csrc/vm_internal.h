@@ -47,7 +47,18 @@ struct vm_device_state {
     /* per-channel N-term scratch for the MFMA form (floats) */
     void *d_nterm;
     size_t nterm_bytes;
+    /* per-kernel event timing (vmSetKernelTiming) */
+    int timing;
+    void *ev[8][2];
+    double t_ms[8];
+    long t_n[8];
 };
+
+/* kernel timing slots */
+enum { VM_T_PFB, VM_T_BF_FP32, VM_T_BF_MFMA, VM_T_NTERMS, VM_T_IPFB, VM_T_GEOM, VM_T_NSLOT };
+/* bracket a launch on the context stream with events when timing is on */
+void vm_timer_start(vm_device_state *d, int slot);
+void vm_timer_stop(vm_device_state *d, int slot);
 
 /* Host helpers implemented in vm_host.c */
 vm_device_state *vm_dev(vcsbeam_context *vm); /* create on first use */
