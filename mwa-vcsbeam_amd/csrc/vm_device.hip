@@ -500,6 +500,7 @@ void vmSetKernelTiming(vcsbeam_context *vm, int on)
 {
     vm_device_state *d = vm_dev(vm);
     d->timing = on;
+    if (!on) return; /* keep the counts for vmKernelTimeMs */
     for (int i = 0; i < VM_T_NSLOT; i++) {
         d->t_ms[i] = 0.0;
         d->t_n[i] = 0;

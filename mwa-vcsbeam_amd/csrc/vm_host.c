@@ -412,9 +412,12 @@ host_buffer *vmInitReadBuffer(size_t read_size, size_t copy_size)
     rb->buffer_size = read_size + copy_size;
     rb->buffer = vm_host_alloc_pinned(rb->buffer_size);
     /* The reference leaves the margin uninitialised on the first second
-     * (src/buffer.c:63); here it starts as zeros so that results are
-     * defined. */
+     * (src/buffer.c:63).  Here it is defined as zeros: the first
+     * vmReadBufferCopyMargin copies the tail of the read region into the
+     * margin, so that tail is zeroed too (recycled pinned memory would
+     * otherwise leak an earlier buffer's contents into the first spectra). */
     memset(rb->buffer, 0, copy_size);
+    if (copy_size) memset((char *)rb->buffer + read_size, 0, copy_size);
     rb->read_size = read_size;
     rb->copy_size = copy_size;
     rb->read_ptr = (char *)rb->buffer + copy_size;
