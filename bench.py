@@ -149,6 +149,12 @@ def main():
     pfb_bytes = nsamp_in * 10  # 2 B in + 8 B out per input complex sample
     beam_samples = args.beams * 128 * obs.nspectra
 
+    print("[bench] MFMA FP16 form: autotune", file=sys.stderr, flush=True)
+    import ctypes as C
+    sess.v.beamform_form = pkg.VM_BF_MFMA_FP16
+    sess.geometry()
+    best = C.c_float(0)
+    variant = lib.vmTuneBeamformer(sess.vm, 3, C.byref(best))
     print("[bench] MFMA FP16 form: warmup + timed steps", file=sys.stderr, flush=True)
     ms_step = time_form(lib, sess, pkg.VM_BF_MFMA_FP16, args.steps, args.warmup)
     gemm_ms, gemm_n = kernel_ms(lib, sess, "beamform_mfma")
@@ -160,7 +166,8 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP16_PEAK_TFLOPS, 4), "traffic": None,
                 "kernel": "k_beamform_mfma<fp16>", "kernel_ms": round(gemm_ms, 4),
-                "launches": gemm_n}
+                "launches": gemm_n, "autotuned_variant": variant,
+                "autotune_best_ms": round(best.value, 4)}
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "beam-samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),

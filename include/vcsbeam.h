@@ -674,6 +674,13 @@ void vmPushD(vcsbeam_context *vm);
  * BF16); vmBeamformChunk dispatches here when beamform_form != VM_BF_FP32. */
 void vmBeamformChunkMFMA(vcsbeam_context *vm);
 
+/* Autotuner of the MFMA beamformer: times every kernel variant (tile shape,
+ * LDS stage depth, XCD-aware tile order) on the context's current chunk with
+ * HIP events, keeps the fastest for later calls and returns its index (and
+ * its time in *best_ms when not NULL).  vmSetBeamformerVariant fixes one. */
+int vmTuneBeamformer(vcsbeam_context *vm, int reps, float *best_ms);
+void vmSetBeamformerVariant(vcsbeam_context *vm, int variant, int xcd);
+
 /* Device-resident inverse PFB of pointing 0 straight from d_e (no host
  * round trip); output complex float [n][pol] on the device, copied to
  * data_buffer_vdif when it is not NULL. */
@@ -697,6 +704,7 @@ void vmMemcpyHtoD(void *dst, const void *src, size_t bytes);
 void vmMemcpyDtoH(void *dst, const void *src, size_t bytes);
 void vmMemsetDevice(void *dst, int value, size_t bytes);
 void vmDeviceSynchronize(void);
+void vmDeviceCopy(void *dst, const void *src, size_t bytes); /* plain copy kernel */
 int vmDeviceCount(void);
 void *vmEventCreate(void);
 void vmEventRecord(void *ev, gpuStream_t stream);

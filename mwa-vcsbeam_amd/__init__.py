@@ -279,6 +279,8 @@ _SIGS = {
                                        C.POINTER(BeamGeom), C.POINTER(BeamGeom)]),
     "vmStructLayout": (C.c_int, [C.c_char_p, C.POINTER(C.c_size_t), C.c_int]),
     "vmSetKernelTiming": (None, [_CTX, C.c_int]),
+    "vmTuneBeamformer": (C.c_int, [_CTX, C.c_int, C.POINTER(C.c_float)]),
+    "vmSetBeamformerVariant": (None, [_CTX, C.c_int, C.c_int]),
     "vmKernelTimeMs": (C.c_double, [_CTX, C.c_char_p, C.POINTER(C.c_long)]),
     "vmDeviceMalloc": (C.c_void_p, [C.c_size_t]),
     "vmDeviceFree": (None, [C.c_void_p]),
@@ -286,6 +288,7 @@ _SIGS = {
     "vmMemcpyDtoH": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "vmMemsetDevice": (None, [C.c_void_p, C.c_int, C.c_size_t]),
     "vmDeviceSynchronize": (None, []),
+    "vmDeviceCopy": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "vmDeviceCount": (C.c_int, []),
     "vmEventCreate": (C.c_void_p, []),
     "vmEventRecord": (None, [C.c_void_p, C.c_void_p]),

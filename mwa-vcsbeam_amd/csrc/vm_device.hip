@@ -520,3 +520,23 @@ double vmKernelTimeMs(vcsbeam_context *vm, const char *kernel, long *count)
 }
 
 } // extern "C"
+
+/* ------------------------------------------------- plain streaming copy ---- */
+__global__ void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" void vmDeviceCopy(void *dst, const void *src, size_t bytes)
+{
+    /* bytes must be a multiple of 16; a plain HBM-bound copy (used as the
+     * reference kernel when calibrating memory counters) */
+    if (bytes % 16) VM_FATAL("vmDeviceCopy: size %zu is not a multiple of 16", bytes);
+    size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, 0, (const uint4 *)src,
+                       (uint4 *)dst, n);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipDeviceSynchronize());
+}

@@ -47,6 +47,9 @@ struct vm_device_state {
     /* per-channel N-term scratch for the MFMA form (floats) */
     void *d_nterm;
     size_t nterm_bytes;
+    /* MFMA beamformer variant chosen by vmTuneBeamformer (0: not tuned) */
+    int bf_variant;
+    int bf_xcd;
     /* per-kernel event timing (vmSetKernelTiming) */
     int timing;
     void *ev[8][2];
