@@ -158,7 +158,10 @@ def main():
     print("[bench] MFMA FP16 form: warmup + timed steps", file=sys.stderr, flush=True)
     ms_step = time_form(lib, sess, pkg.VM_BF_MFMA_FP16, args.steps, args.warmup)
     gemm_ms, gemm_n = kernel_ms(lib, sess, "beamform_mfma")
-    nterm_ms, _ = kernel_ms(lib, sess, "nterms")
+    prep_ms, _ = kernel_ms(lib, sess, "beamform_prep")
+    weights_ms, _ = kernel_ms(lib, sess, "beamform_weights")
+    vr, vs, vw = C.c_int(0), C.c_int(0), C.c_int(0)
+    lib.vmBeamformerVariantInfo(variant, C.byref(vr), C.byref(vs), C.byref(vw))
     value = beam_samples / (ms_step * 1e-3)
     S_mfma = sess.stokes()
     flops = 4096.0 * beam_samples  # complex GEMM: 8 x 2 pols x 256 tile-pols per beam-sample
@@ -167,6 +170,7 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / FP16_PEAK_TFLOPS, 4), "traffic": None,
                 "kernel": "k_beamform_mfma<fp16>", "kernel_ms": round(gemm_ms, 4),
                 "launches": gemm_n, "autotuned_variant": variant,
+                "workgroup_tile": f"{vr.value} rows x {vs.value} samples, {vw.value} waves",
                 "autotune_best_ms": round(best.value, 4)}
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "beam-samples/s", "n_gpus": 1,
@@ -182,7 +186,8 @@ def main():
         "stages": {"pfb_fused_ms_per_ch_s": round(pfb_ms, 4),
                    "pfb_GBps": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9, 1),
                    "pfb_hbm_frac": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                   "nterms_ms": round(nterm_ms, 4)},
+                   "beamform_prep_ms": round(prep_ms, 4),
+                   "beamform_weights_ms": round(weights_ms, 4)},
     }
     S_fp32 = None
     if not args.no_fp32:

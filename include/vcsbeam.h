@@ -674,12 +674,19 @@ void vmPushD(vcsbeam_context *vm);
  * BF16); vmBeamformChunk dispatches here when beamform_form != VM_BF_FP32. */
 void vmBeamformChunkMFMA(vcsbeam_context *vm);
 
-/* Autotuner of the MFMA beamformer: times every kernel variant (tile shape,
- * LDS stage depth, XCD-aware tile order) on the context's current chunk with
- * HIP events, keeps the fastest for later calls and returns its index (and
- * its time in *best_ms when not NULL).  vmSetBeamformerVariant fixes one. */
+/* Autotuner of the MFMA beamformer: times every kernel variant (register
+ * tile per wavefront, wavefront grid per workgroup) in every tile order on
+ * the context's current chunk with HIP events, keeps the fastest for later
+ * calls and returns its index (and its time in *best_ms when not NULL).
+ * vmSetBeamformerVariant fixes one; order 0 walks channels in turn with row
+ * tiles fastest; orders 1-4 give each XCD its own channels and walk a
+ * channel's tiles in bands of all (1), one (2), two (3) or four (4) row
+ * tiles, column by column. */
 int vmTuneBeamformer(vcsbeam_context *vm, int reps, float *best_ms);
-void vmSetBeamformerVariant(vcsbeam_context *vm, int variant, int xcd);
+void vmSetBeamformerVariant(vcsbeam_context *vm, int variant, int order);
+/* Number of MFMA beamformer variants and the tile shape of one: rows and
+ * samples per workgroup, wavefronts per workgroup. Returns 0 if no such. */
+int vmBeamformerVariantInfo(int variant, int *rows, int *samples, int *waves);
 
 /* Device-resident inverse PFB of pointing 0 straight from d_e (no host
  * round trip); output complex float [n][pol] on the device, copied to

@@ -472,8 +472,9 @@ void destroy_vcsbeam_context(vcsbeam_context *vm)
 /* ------------------------------------------------------ kernel timing ---- */
 extern "C" {
 
-static const char *vm_timer_names[VM_T_NSLOT] = {"pfb", "beamform_fp32", "beamform_mfma",
-                                                 "nterms", "ipfb", "geometry"};
+static const char *vm_timer_names[VM_T_NSLOT] = {"pfb",           "beamform_fp32", "beamform_mfma",
+                                                 "beamform_prep", "ipfb",          "geometry",
+                                                 "beamform_weights"};
 
 void vm_timer_start(vm_device_state *d, int slot)
 {

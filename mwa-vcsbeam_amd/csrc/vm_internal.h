@@ -49,7 +49,7 @@ struct vm_device_state {
     size_t nterm_bytes;
     /* MFMA beamformer variant chosen by vmTuneBeamformer (0: not tuned) */
     int bf_variant;
-    int bf_xcd;
+    int bf_order;
     /* per-kernel event timing (vmSetKernelTiming) */
     int timing;
     void *ev[8][2];
@@ -58,7 +58,16 @@ struct vm_device_state {
 };
 
 /* kernel timing slots */
-enum { VM_T_PFB, VM_T_BF_FP32, VM_T_BF_MFMA, VM_T_NTERMS, VM_T_IPFB, VM_T_GEOM, VM_T_NSLOT };
+enum {
+    VM_T_PFB,
+    VM_T_BF_FP32,
+    VM_T_BF_MFMA,
+    VM_T_BF_PREP,
+    VM_T_IPFB,
+    VM_T_GEOM,
+    VM_T_BF_WEIGHTS,
+    VM_T_NSLOT
+};
 /* bracket a launch on the context stream with events when timing is on */
 void vm_timer_start(vm_device_state *d, int slot);
 void vm_timer_stop(vm_device_state *d, int slot);
