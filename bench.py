@@ -111,6 +111,86 @@ def time_form(lib, sess, form, steps, warmup):
     return (t1 - t0) * 1e3 / steps
 
 
+COPY_BYTES = 1 << 30  # calibration copy for the memory counters
+
+
+def rocprof_summary(dirs, title, command, out_md, out_json):
+    """Summarise rocprofv3 CSV output (a --kernel-trace --stats run and
+    --pmc FETCH_SIZE / WRITE_SIZE runs of the same command) into a short
+    markdown table and a traffic record.  The counters are calibrated with the
+    plain copy kernel that ran in the same command (vmDeviceCopy of
+    COPY_BYTES): factor = COPY_BYTES / (counter KiB * 1024)."""
+    import csv
+    import glob
+    stats, counters = [], {}
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "*_kernel_stats.csv")):
+            stats += list(csv.DictReader(open(f)))
+        for f in glob.glob(os.path.join(d, "*_counter_collection.csv")):
+            for r in csv.DictReader(open(f)):
+                name = r["Kernel_Name"].split("(")[0]
+                counters.setdefault(r["Counter_Name"], {}).setdefault(name, []).append(
+                    float(r["Counter_Value"]))
+    lines = [f"# {title}", "", "Command (from the repository root, on one MI355X):", "",
+             "```", command.strip(), "```", ""]
+    if stats:
+        lines += ["## Kernel trace statistics", "",
+                  "| kernel | calls | avg (us) | total (ms) | % |", "|---|---:|---:|---:|---:|"]
+        for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"])):
+            nm = r["Name"].split("(")[0]
+            lines.append(f"| `{nm}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                         f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.1f} |")
+        lines.append("")
+    traffic = {}
+    fet, wri = counters.get("FETCH_SIZE", {}), counters.get("WRITE_SIZE", {})
+    cp = "k_stream_copy"
+    if fet and wri and cp in fet and cp in wri:
+        ff = COPY_BYTES / (sum(fet[cp]) / len(fet[cp]) * 1024)
+        wf = COPY_BYTES / (sum(wri[cp]) / len(wri[cp]) * 1024)
+        lines += ["## HBM traffic per dispatch", "",
+                  f"Calibration: the 1 GiB copy kernel reads FETCH_SIZE x {ff:.3f} and "
+                  f"WRITE_SIZE x {wf:.3f} of its true bytes.", "",
+                  "| kernel | dispatches | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | read (GB) | write (GB) |",
+                  "|---|---:|---:|---:|---:|---:|"]
+        for nm in sorted(set(fet) & set(wri)):
+            fk = sum(fet[nm]) / len(fet[nm])
+            wk = sum(wri[nm]) / len(wri[nm])
+            rb, wb = fk * 1024 * ff, wk * 1024 * wf
+            traffic[nm] = {"read_bytes": rb, "write_bytes": wb, "dispatches": len(fet[nm])}
+            lines.append(f"| `{nm}` | {len(fet[nm])} | {fk:.4g} | {wk:.4g} | {rb / 1e9:.3f} | "
+                         f"{wb / 1e9:.3f} |")
+        lines.append("")
+    other = {k: v for k, v in counters.items() if k not in ("FETCH_SIZE", "WRITE_SIZE")}
+    if other:
+        lines += ["## Other counters (mean per dispatch)", "", "| counter | kernel | value |",
+                  "|---|---|---:|"]
+        for c, per in sorted(other.items()):
+            for nm, v in sorted(per.items()):
+                lines.append(f"| {c} | `{nm}` | {sum(v) / len(v):.4g} |")
+        lines.append("")
+    open(out_md, "w").write("\n".join(lines))
+    if out_json and traffic:
+        json.dump({"title": title, "command": command.strip(), "kernels": traffic},
+                  open(out_json, "w"), indent=1)
+
+
+def committed_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 summary."""
+    f = os.path.join(ROOT, "profiles", "bench_traffic.json")
+    if not os.path.exists(f):
+        return None
+    try:
+        rec = json.load(open(f))["kernels"]
+    except (ValueError, KeyError):
+        return None
+    for nm, t in rec.items():
+        if nm.startswith(kernel):
+            return {"bytes": round(t["read_bytes"] + t["write_bytes"]),
+                    "read_bytes": round(t["read_bytes"]), "write_bytes": round(t["write_bytes"]),
+                    "source": "profiles/bench_traffic.json"}
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,7 +200,22 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="fix the MFMA beamformer variant instead of autotuning")
+    ap.add_argument("--order", type=int, default=1, help="tile order with --variant")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with one 1 GiB plain copy (memory-counter calibration)")
+    ap.add_argument("--rocprof-summary", nargs="+", metavar="DIR",
+                    help="summarise rocprofv3 CSV output directories (no GPU needed)")
+    ap.add_argument("--title", default="")
+    ap.add_argument("--command", default="")
+    ap.add_argument("--summary-out", default="")
+    ap.add_argument("--traffic-out", default="")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        rocprof_summary(args.rocprof_summary, args.title, args.command, args.summary_out,
+                        args.traffic_out)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the beamforming path shards over coarse channels "
               "(one GPU per channel) and this build runs on a single MI355X; use --gpus 1.",
@@ -154,7 +249,11 @@ def main():
     sess.v.beamform_form = pkg.VM_BF_MFMA_FP16
     sess.geometry()
     best = C.c_float(0)
-    variant = lib.vmTuneBeamformer(sess.vm, 3, C.byref(best))
+    if args.variant >= 0:
+        variant = args.variant
+        lib.vmSetBeamformerVariant(sess.vm, args.variant, args.order)
+    else:
+        variant = lib.vmTuneBeamformer(sess.vm, 3, C.byref(best))
     print("[bench] MFMA FP16 form: warmup + timed steps", file=sys.stderr, flush=True)
     ms_step = time_form(lib, sess, pkg.VM_BF_MFMA_FP16, args.steps, args.warmup)
     gemm_ms, gemm_n = kernel_ms(lib, sess, "beamform_mfma")
@@ -167,7 +266,8 @@ def main():
     flops = 4096.0 * beam_samples  # complex GEMM: 8 x 2 pols x 256 tile-pols per beam-sample
     achieved = flops / (gemm_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP16_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP16_PEAK_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP16_PEAK_TFLOPS, 4),
+                "traffic": committed_traffic("k_beamform_mfma"),
                 "kernel": "k_beamform_mfma<fp16>", "kernel_ms": round(gemm_ms, 4),
                 "launches": gemm_n, "autotuned_variant": variant,
                 "workgroup_tile": f"{vr.value} rows x {vs.value} samples, {vw.value} waves",
@@ -217,6 +317,10 @@ def main():
             par[name] = {"max_beam_rel_err_IQUV": [float(f"{x:.3e}") for x in e.max(axis=0)]}
         result["parity_vs_fp64"] = par
     sess.close()
+    if args.calibrate_copy:
+        a, b = pkg.DeviceArray(COPY_BYTES), pkg.DeviceArray(COPY_BYTES)
+        lib.vmDeviceCopy(b.ptr, a.ptr, COPY_BYTES)
+        del a, b
     print(json.dumps(result))
 
 
