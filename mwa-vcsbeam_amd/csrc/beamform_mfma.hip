@@ -573,25 +573,313 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(
     }
 }
 
+// ------------------------------------------------- persistent GEMM ------
+// k_beamform_mfma_pt: one workgroup per CU walks its share of the tiles
+// (L = blockIdx.x, + gridDim.x, ...), decoded in the same orders as above.
+// Each wavefront keeps two accumulator sets: the k-loop of tile i runs in
+// `acc` while the Stokes epilogue of tile i-1 drains `accp`, one 32 x 32 block
+// per four k-steps, its VALU, LDS and store instructions placed between the
+// MFMAs of that k-step so they issue in the matrix pipe's shadow.  The last
+// k-step of a tile writes its lo products straight into `accp`.  Fragments
+// go through a 4-stage register ring whose last loads of each tile already
+// fetch the next tile's first three k-steps.  Needs nkstep % 4 == 0 and
+// nkstep / 4 >= MB NB (each block's epilogue gets a k-step group of its own).
+__device__ __forceinline__ void bf_decode_tile(int L, int nchan, int ntx, int nty, int order,
+                                               int &c, int &tx, int &ty)
+{
+    const int ntile = ntx * nty;
+    int tile;
+    if (order != 0 && (nchan & 7) == 0) {
+        const int g = L >> 3, x = L & 7;
+        c = (g / ntile) * 8 + x;
+        tile = g % ntile;
+    } else {
+        c = L / ntile;
+        tile = L % ntile;
+    }
+    int gr = order == 2 ? 1 : order == 3 ? 2 : order == 4 ? 4 : nty;
+    if (nty % gr) gr = 1;
+    const int band = tile / (gr * ntx), rem = tile % (gr * ntx);
+    tx = rem / gr;
+    ty = band * gr + rem % gr;
+}
+
+// where a wavefront's operands of one tile are
+template <int NB> struct bf_tile {
+    int c, rb0, sb0, aoff;
+    int boff[NB];
+    rsrc_t rA, rB;
+};
+
+template <int MB, int NB, int WR, int WC>
+__device__ __forceinline__ void bf_tile_at(bf_tile<NB> &T, int L, int w, const uint4 *Af,
+                                           const uint4 *Bf, int nchan, int ntx, int nty,
+                                           int order, int nrb, int nsb, int nkstep)
+{
+    int c, tx, ty;
+    bf_decode_tile(L, nchan, ntx, nty, order, c, tx, ty);
+    const int wr = w / WC, wc = w % WC;
+    const int astr = nkstep * 2048, bstr = nkstep * 1024;
+    T.c = c;
+    T.rb0 = (ty * WR + wr) * MB;
+    T.sb0 = (tx * WC + wc) * NB;
+    T.aoff = T.rb0 * astr;
+#pragma unroll
+    for (int n = 0; n < NB; n++) T.boff[n] = min(T.sb0 + n, nsb - 1) * bstr;
+    T.rA = make_rsrc(Af + (size_t)c * nrb * nkstep * 128, (uint32_t)nrb * astr);
+    T.rB = make_rsrc(Bf + (size_t)c * nsb * nkstep * 64, (uint32_t)nsb * bstr);
+}
+
+// what the epilogue of a finished tile needs besides its accumulators
+template <int NB> struct bf_epi {
+    int c, rb0, sb0;
+    float4 nn[NB]; // auto-correlation terms at this lane's sample of each block
+    float us[NB];  // operand scales removed from |E|^2
+};
+
+template <int NB>
+__device__ __forceinline__ void bf_epi_load(bf_epi<NB> &E, const bf_tile<NB> &T, int lane,
+                                            const float4 *__restrict__ N,
+                                            const int *__restrict__ sexp,
+                                            const uint32_t *__restrict__ wmax, int ns, int nsb)
+{
+    E.c = T.c;
+    E.rb0 = T.rb0;
+    E.sb0 = T.sb0;
+    const int ew = scale_exp(__uint_as_float(wmax[T.c]));
+#pragma unroll
+    for (int n = 0; n < NB; n++) {
+        const int sb = T.sb0 + n, s = SB * sb + (lane & 31);
+        E.nn[n] = s < ns ? N[(size_t)T.c * ns + s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        E.us[n] = ldexpf(1.0f, -2 * (sexp[(size_t)T.c * nsb + min(sb, nsb - 1)] + ew));
+    }
+}
+
+struct bf_out {
+    float *S;
+    size_t pstride, ststride;
+    int ns, ns_total, soffset, npointing, nstokes;
+    float invw;
+    bool vec;
+};
+
+// Part q (0..7) of the epilogue of accumulator block (m, n): parts 0-3 form
+// the Stokes of beam row group jg = q and write them to the wavefront's LDS
+// tile; part 4 reads the 4 x 16 bytes this lane stores; parts 5-7 store them.
+template <int NB>
+__device__ __forceinline__ void bf_epi_part(int q, int m, int n, const f16v &a,
+                                            const bf_epi<NB> &E, float *wl, int lane,
+                                            const bf_out &O, float4 (&v)[4])
+{
+    const int lr = lane & 31, lk = lane >> 5, qq = lane & 7;
+    if (q < 4) {
+        float st4[4];
+        stokes_of(a, q, E.us[n], E.nn[n], O.invw, st4);
+#pragma unroll
+        for (int st = 0; st < 4; st++) wl[((st * 4 + q) * 2 + lk) * 32 + lr] = st4[st];
+    } else if (q == 4) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = *(const float4 *)(wl + (8 * i + (lane >> 3)) * 32 + 4 * qq);
+    } else {
+        const int i0 = q == 5 ? 0 : q == 6 ? 2 : 3, i1 = q == 5 ? 2 : q == 6 ? 3 : 4;
+#pragma unroll
+        for (int i = i0; i < i1; i++) {
+            const int r = 8 * i + (lane >> 3);
+            const int st = r >> 3, bb = 2 * ((r >> 1) & 3) + (r & 1);
+            const int p = 8 * (E.rb0 + m) + bb;
+            const int s4 = SB * (E.sb0 + n) + 4 * qq;
+            if (st < O.nstokes && p < O.npointing && s4 < O.ns) {
+                float *dst = O.S + (size_t)p * O.pstride + (size_t)st * O.ststride +
+                             (size_t)E.c * O.ns_total + O.soffset + s4;
+                if (O.vec && s4 + 3 < O.ns) {
+                    *(float4 *)dst = v[i];
+                } else {
+                    dst[0] = v[i].x;
+                    if (s4 + 1 < O.ns) dst[1] = v[i].y;
+                    if (s4 + 2 < O.ns) dst[2] = v[i].z;
+                    if (s4 + 3 < O.ns) dst[3] = v[i].w;
+                }
+            }
+        }
+    }
+}
+
+// one k-step of MFMAs with the epilogue of block (m0, n0) spread between them
+template <bool BF16, int MB, int NB>
+__device__ __forceinline__ void bf_mma_epi(f16v (&acc)[MB][NB], const stage<MB, NB> &f,
+                                           const f16v &pa, int m0, int n0, const bf_epi<NB> &E,
+                                           float *wl, int lane, const bf_out &O)
+{
+    constexpr int NM = 2 * MB * NB, PER = NM / 8;
+    float4 v[4];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int m = 0; m < MB; m++)
+#pragma unroll
+            for (int n = 0; n < NB; n++) {
+                acc[m][n] = mtype<BF16>::mfma(f.a[m][h], f.b[n], acc[m][n]);
+                const int idx = (h * MB + m) * NB + n;
+                if ((idx + 1) % PER == 0) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    bf_epi_part<NB>((idx + 1) / PER - 1, m0, n0, pa, E, wl, lane, O, v);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+}
+
+// last k-step of a tile: hi products into acc, lo products into accp
+template <bool BF16, int MB, int NB>
+__device__ __forceinline__ void bf_mma_final(f16v (&acc)[MB][NB], f16v (&accp)[MB][NB],
+                                             const stage<MB, NB> &f)
+{
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int n = 0; n < NB; n++) acc[m][n] = mtype<BF16>::mfma(f.a[m][0], f.b[n], acc[m][n]);
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int n = 0; n < NB; n++) accp[m][n] = mtype<BF16>::mfma(f.a[m][1], f.b[n], acc[m][n]);
+}
+
+template <bool BF16, int MB, int NB, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC, 1) void k_beamform_mfma_pt(
+    const uint4 *__restrict__ Bf, const uint4 *__restrict__ Af, const int *__restrict__ sexp,
+    const uint32_t *__restrict__ wmax, const float4 *__restrict__ N, int ns, int nsb, int nkstep,
+    int nrb, int npointing, int nchan, int ntx, int nty, int order, float invw, int nstokes,
+    int soffset, int ns_total, float *__restrict__ S)
+{
+    constexpr int NBLK = MB * NB;
+    __shared__ __attribute__((aligned(16))) float stg[WR * WC][32 * 32];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float *wl = stg[w];
+    const int ntiles = nchan * ntx * nty;
+    const int astr = nkstep * 2048;
+    const int vl = lane * 16;
+    const int ngroup = nkstep >> 2;
+    bf_out O;
+    O.S = S;
+    O.pstride = (size_t)nstokes * nchan * ns_total;
+    O.ststride = (size_t)nchan * ns_total;
+    O.ns = ns;
+    O.ns_total = ns_total;
+    O.soffset = soffset;
+    O.npointing = npointing;
+    O.nstokes = nstokes;
+    O.invw = invw;
+    O.vec = ((ns_total | soffset) & 3) == 0;
+
+    int t = blockIdx.x;
+    if (t >= ntiles) return;
+    bf_tile<NB> cur, nxt;
+    bf_tile_at<MB, NB, WR, WC>(cur, t, w, Af, Bf, nchan, ntx, nty, order, nrb, nsb, nkstep);
+    f16v acc[MB][NB], accp[MB][NB];
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int n = 0; n < NB; n++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) accp[m][n][q] = 0.f;
+    stage<MB, NB> f0, f1, f2, f3;
+    // the first tile has no predecessor: its epilogue steps run with no
+    // pointing to store (npointing 0) instead of taking another code path
+    bf_epi<NB> E;
+    E.c = E.rb0 = E.sb0 = 0;
+#pragma unroll
+    for (int n = 0; n < NB; n++) {
+        E.nn[n] = make_float4(0.f, 0.f, 0.f, 0.f);
+        E.us[n] = 0.f;
+    }
+    O.npointing = 0;
+
+#define VM_SB() __builtin_amdgcn_sched_barrier(0)
+#define VM_LD(f, T, k) bf_load<MB, NB>(f, T.rA, T.rB, vl, T.aoff, astr, T.boff, (k))
+#define VM_MM(f) bf_mma<BF16, MB, NB>(acc, f)
+#define VM_MME(f, B) bf_mma_epi<BF16, MB, NB>(acc, f, accp[(B) / NB][(B) % NB], (B) / NB, (B) % NB, E, wl, lane, O)
+    VM_LD(f0, cur, 0);
+    VM_LD(f1, cur, 1);
+    VM_LD(f2, cur, 2);
+    for (;;) {
+        const int tn = t + (int)gridDim.x;
+        const bool more = tn < ntiles;
+        bf_tile_at<MB, NB, WR, WC>(nxt, more ? tn : t, w, Af, Bf, nchan, ntx, nty, order, nrb,
+                                   nsb, nkstep);
+#pragma unroll
+        for (int m = 0; m < MB; m++)
+#pragma unroll
+            for (int n = 0; n < NB; n++)
+#pragma unroll
+                for (int q = 0; q < 16; q++) acc[m][n][q] = 0.f;
+        // groups 0 .. NBLK-2 carry the epilogue of blocks 0 .. NBLK-2
+#pragma unroll
+        for (int g = 0; g < NBLK - 1; g++) {
+            const int ks = 4 * g;
+            VM_SB(); VM_LD(f3, cur, ks + 3); VM_MME(f0, g); VM_SB();
+            VM_SB(); VM_LD(f0, cur, ks + 4); VM_MM(f1); VM_SB();
+            VM_SB(); VM_LD(f1, cur, ks + 5); VM_MM(f2); VM_SB();
+            VM_SB(); VM_LD(f2, cur, ks + 6); VM_MM(f3); VM_SB();
+        }
+        for (int g = NBLK - 1; g < ngroup - 1; g++) {
+            const int ks = 4 * g;
+            VM_SB(); VM_LD(f3, cur, ks + 3); VM_MM(f0); VM_SB();
+            VM_SB(); VM_LD(f0, cur, ks + 4); VM_MM(f1); VM_SB();
+            VM_SB(); VM_LD(f1, cur, ks + 5); VM_MM(f2); VM_SB();
+            VM_SB(); VM_LD(f2, cur, ks + 6); VM_MM(f3); VM_SB();
+        }
+        // last group: the last block's epilogue, and the ring moves on to the
+        // next tile's first k-steps
+        VM_SB(); VM_LD(f3, cur, nkstep - 1); VM_MME(f0, NBLK - 1); VM_SB();
+        VM_SB(); VM_LD(f0, nxt, 0); VM_MM(f1); VM_SB();
+        VM_SB(); VM_LD(f1, nxt, 1); VM_MM(f2); VM_SB();
+        VM_SB(); VM_LD(f2, nxt, 2); bf_mma_final<BF16, MB, NB>(acc, accp, f3); VM_SB();
+        bf_epi_load<NB>(E, cur, lane, N, sexp, wmax, ns, nsb);
+        O.npointing = npointing;
+        if (!more) break;
+        cur = nxt;
+        t = tn;
+    }
+#undef VM_MME
+#undef VM_MM
+#undef VM_LD
+#undef VM_SB
+    // drain the last tile's epilogue
+    float4 v[4];
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int n = 0; n < NB; n++)
+#pragma unroll
+            for (int q = 0; q < 8; q++) bf_epi_part<NB>(q, m, n, accp[m][n], E, wl, lane, O, v);
+}
+
 // Kernel variants the autotuner chooses from: register tile per wavefront
 // (MB x NB blocks of 32 x 32), wavefront grid per workgroup (WR x WC) and
 // register stages.
-struct bf_variant { int mb, nb, wr, wc, nst, epi, occ; };
+// pers = 1: the persistent kernel with the overlapped epilogue (4 stages).
+struct bf_variant { int mb, nb, wr, wc, nst, epi, occ, pers; };
 static const bf_variant kVariants[] = {
-    {4, 4, 2, 2, 3, 1, 1}, {4, 4, 1, 4, 3, 1, 1}, {4, 4, 2, 2, 4, 1, 1}, {4, 2, 2, 2, 2, 1, 2},
-    {2, 4, 2, 2, 2, 1, 2}, {2, 4, 4, 2, 2, 1, 1}, {4, 4, 2, 2, 3, 0, 1},
+    {4, 4, 2, 2, 3, 1, 1, 0}, {4, 4, 1, 4, 3, 1, 1, 0}, {4, 4, 2, 2, 4, 1, 1, 0},
+    {4, 2, 2, 2, 2, 1, 2, 0}, {2, 4, 2, 2, 2, 1, 2, 0}, {2, 4, 4, 2, 2, 1, 1, 0},
+    {4, 4, 2, 2, 3, 0, 1, 0}, {2, 4, 2, 2, 4, 1, 1, 1}, {2, 4, 4, 1, 4, 1, 1, 1},
+    {2, 4, 1, 4, 4, 1, 1, 1},
 };
 #define N_VARIANTS ((int)(sizeof(kVariants) / sizeof(kVariants[0])))
 #define VM_N_ORDERS 5
 
 template <bool BF16>
-static void launch_variant(int vi, dim3 grid, hipStream_t st, const uint4 *Bf, const uint4 *Af,
+static void launch_variant(int vi, dim3 grid, dim3 pgrid, hipStream_t st, const uint4 *Bf, const uint4 *Af,
                            const int *sexp, const uint32_t *wmax, const float4 *N, int ns, int nsb,
                            int nkstep, int nrb, int np, int nchan, int ntx, int nty, int order,
                            float invw, int nstokes, int soffset, int ns_total, float *S)
 {
 #define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, EPI_, OCC_)                                         \
     hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, EPI_, OCC_>), grid,    \
+                       dim3(64 * WR_ * WC_), 0, st, Bf, Af, sexp, wmax, N, ns, nsb, nkstep, nrb, \
+                       np, nchan, ntx, nty, order, invw, nstokes, soffset, ns_total, S)
+#define VM_LAUNCH_PT(MB_, NB_, WR_, WC_)                                                        \
+    hipLaunchKernelGGL((k_beamform_mfma_pt<BF16, MB_, NB_, WR_, WC_>), pgrid,                  \
                        dim3(64 * WR_ * WC_), 0, st, Bf, Af, sexp, wmax, N, ns, nsb, nkstep, nrb, \
                        np, nchan, ntx, nty, order, invw, nstokes, soffset, ns_total, S)
     switch (vi) {
@@ -601,8 +889,12 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const uint4 *Bf, c
     case 3: VM_LAUNCH(4, 2, 2, 2, 2, 1, 2); break;
     case 4: VM_LAUNCH(2, 4, 2, 2, 2, 1, 2); break;
     case 5: VM_LAUNCH(2, 4, 4, 2, 2, 1, 1); break;
-    default: VM_LAUNCH(4, 4, 2, 2, 3, 0, 1); break;
+    case 6: VM_LAUNCH(4, 4, 2, 2, 3, 0, 1); break;
+    case 7: VM_LAUNCH_PT(2, 4, 2, 2); break;
+    case 8: VM_LAUNCH_PT(2, 4, 4, 1); break;
+    default: VM_LAUNCH_PT(2, 4, 1, 4); break;
     }
+#undef VM_LAUNCH_PT
 #undef VM_LAUNCH
 }
 
@@ -730,7 +1022,11 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
 
 static bool variant_ok(int vi, const mfma_plan *pl)
 {
-    return vi >= 0 && vi < N_VARIANTS && pl->nrb % (kVariants[vi].mb * kVariants[vi].wr) == 0;
+    if (vi < 0 || vi >= N_VARIANTS) return false;
+    const bf_variant &V = kVariants[vi];
+    if (pl->nrb % (V.mb * V.wr) != 0) return false;
+    if (V.pers && (pl->nkstep % 4 != 0 || pl->nkstep / 4 < V.mb * V.nb)) return false;
+    return true;
 }
 
 static void mfma_run(vcsbeam_context *vm, const mfma_plan *pl, int vi, int order)
@@ -740,17 +1036,28 @@ static void mfma_run(vcsbeam_context *vm, const mfma_plan *pl, int vi, int order
     const bf_variant &V = kVariants[vi];
     const int ncol = V.nb * V.wc, nrow = V.mb * V.wr; // fragment blocks per workgroup
     const int ntx = (pl->nsb + ncol - 1) / ncol, nty = pl->nrb / nrow;
-    const dim3 grid((unsigned)((size_t)pl->nchan * ntx * nty));
+    const size_t ntiles = (size_t)pl->nchan * ntx * nty;
+    const dim3 grid((unsigned)ntiles);
+    // persistent kernel: one workgroup per CU, a multiple of 8 (one per XCD)
+    if (!d->ncu) {
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&d->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    size_t g = (size_t)d->ncu;
+    if (g >= 8) g &= ~(size_t)7;
+    if (g > ntiles) g = ntiles;
+    const dim3 pgrid((unsigned)g);
     if (vm->beamform_form == VM_BF_MFMA_BF16)
-        launch_variant<true>(vi, grid, st, pl->Bf, pl->Af, pl->sexp, pl->wmax, pl->N, pl->ns,
-                             pl->nsb, pl->nkstep, pl->nrb, pl->np, pl->nchan, ntx, nty, order,
-                             pl->invw, vm->out_nstokes, pl->chunk * pl->ns, vm->fine_sample_rate,
-                             (float *)vm->d_S);
+        launch_variant<true>(vi, grid, pgrid, st, pl->Bf, pl->Af, pl->sexp, pl->wmax, pl->N,
+                             pl->ns, pl->nsb, pl->nkstep, pl->nrb, pl->np, pl->nchan, ntx, nty,
+                             order, pl->invw, vm->out_nstokes, pl->chunk * pl->ns,
+                             vm->fine_sample_rate, (float *)vm->d_S);
     else
-        launch_variant<false>(vi, grid, st, pl->Bf, pl->Af, pl->sexp, pl->wmax, pl->N, pl->ns,
-                              pl->nsb, pl->nkstep, pl->nrb, pl->np, pl->nchan, ntx, nty, order,
-                              pl->invw, vm->out_nstokes, pl->chunk * pl->ns, vm->fine_sample_rate,
-                              (float *)vm->d_S);
+        launch_variant<false>(vi, grid, pgrid, st, pl->Bf, pl->Af, pl->sexp, pl->wmax, pl->N,
+                              pl->ns, pl->nsb, pl->nkstep, pl->nrb, pl->np, pl->nchan, ntx, nty,
+                              order, pl->invw, vm->out_nstokes, pl->chunk * pl->ns,
+                              vm->fine_sample_rate, (float *)vm->d_S);
     HIPCHECK(hipGetLastError());
 }
 
@@ -799,9 +1106,10 @@ int vmTuneBeamformer(vcsbeam_context *vm, int reps, float *best_ms)
             HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
             ms /= reps;
             if (getenv("VCSBEAM_TUNE_VERBOSE"))
-                fprintf(stderr, "beamformer variant %d (%dx%d per wave, %dx%d waves) order %d: %.3f ms\n",
+                fprintf(stderr,
+                        "beamformer variant %d (%dx%d per wave, %dx%d waves%s) order %d: %.3f ms\n",
                         vi, 32 * kVariants[vi].mb, 32 * kVariants[vi].nb, kVariants[vi].wr,
-                        kVariants[vi].wc, order, ms);
+                        kVariants[vi].wc, kVariants[vi].pers ? ", persistent" : "", order, ms);
             if (ms < best) { best = ms; bv = vi; bo = order; }
         }
     }

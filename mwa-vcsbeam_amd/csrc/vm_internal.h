@@ -50,6 +50,8 @@ struct vm_device_state {
     /* MFMA beamformer variant chosen by vmTuneBeamformer (0: not tuned) */
     int bf_variant;
     int bf_order;
+    /* compute units of the device (persistent launches) */
+    int ncu;
     /* per-kernel event timing (vmSetKernelTiming) */
     int timing;
     void *ev[8][2];

@@ -66,3 +66,41 @@ def test_mfma_chunked_and_fp32_agree(pkg, orc, gpu_lib):
     err = _errors(orc, m["S"][0], f["S"][0].astype(np.float64))
     print("MFMA vs FP32 (2 chunks):", err.max(axis=0))
     assert err[:, 0].max() <= 1e-3
+
+
+@pytest.fixture(scope="module")
+def mfma_case(pkg, orc):
+    obs, az, za = _setup(pkg)
+    return obs, az, za, _oracle(orc, obs, az, za)
+
+
+@pytest.mark.parametrize("variant", range(12))
+def test_every_beamformer_variant_and_tile_order(pkg, orc, gpu_lib, mfma_case, variant):
+    """Every GEMM variant (per-tile and persistent kernels, all tile shapes
+    and epilogues) in every tile order gives the same Stokes as the FP64
+    oracle chain, to the FP16 bar."""
+    import ctypes as C
+    from vcsbeam_amd.apps import TiedArraySession
+    nv = gpu_lib.vmBeamformerVariantInfo(0, None, None, None)
+    if variant >= nv:
+        pytest.skip(f"the library has {nv} beamformer variants")
+    rows, samples, waves = C.c_int(), C.c_int(), C.c_int()
+    gpu_lib.vmBeamformerVariantInfo(variant, C.byref(rows), C.byref(samples), C.byref(waves))
+    obs, az, za, Sref = mfma_case
+    sess = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
+                            nstokes=4, form=pkg.VM_BF_MFMA_FP16)
+    try:
+        sess.read(0)
+        sess.pfb()
+        for order in range(5):
+            print(f"variant {variant} ({rows.value}x{samples.value}, {waves.value} waves) "
+                  f"order {order}", flush=True)
+            gpu_lib.vmSetBeamformerVariant(sess.vm, variant, order)
+            sess.geometry()
+            sess.beamform()
+            err = _errors(orc, sess.stokes(), Sref)
+            print("   max per-beam rel err I,Q,U,V:", err.max(axis=0))
+            assert err[:, 0].max() <= 1e-3
+            assert err[:, 1:].max() <= 3e-3
+    finally:
+        sess.close()
