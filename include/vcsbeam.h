@@ -15,9 +15,10 @@
  *    the fields the path reads (same field names);
  *  - device buffers behind the context (d_v, d_S, d_e, ...) use the layouts
  *    documented at each field; host buffers keep the reference layouts;
- *  - MI355X extensions (precision MFMA beamformer, device FP64 Jones/phase,
- *    device-resident inverse PFB, batched launch, beam surrogate) are
- *    appended at the end of the header and of vcsbeam_context.
+ *  - MI355X extensions (precision MFMA beamformer and its autotuner, device
+ *    FP64 Jones/phase, device-resident inverse PFB and VDIF requantisation,
+ *    device memory/event helpers, kernel timing) are appended at the end of
+ *    the header and of vcsbeam_context.
  */
 #ifndef VCSBEAM_H
 #define VCSBEAM_H
