@@ -184,7 +184,7 @@ def committed_traffic(kernel):
     except (ValueError, KeyError):
         return None
     for nm, t in rec.items():
-        if nm.startswith(kernel):
+        if nm.replace("void ", "").startswith(kernel):
             return {"bytes": round(t["read_bytes"] + t["write_bytes"]),
                     "read_bytes": round(t["read_bytes"]), "write_bytes": round(t["write_bytes"]),
                     "source": "profiles/bench_traffic.json"}

@@ -6,13 +6,14 @@
 // /K, fftshift, (im,re) swap, VCS reorder, and INT4 or FP64 packing.
 //
 // MI355X design:
-//  * Fused path (default): one kernel per chunk does unpack + WOLA + rounding
-//    + FFT + reorder for a tile of 32 consecutive spectra of one RF input.
-//    The 43-block input window is staged once in LDS with coalesced dword
-//    loads; WOLA accumulates exactly in int32 registers (no atomics, no
-//    memset); the FFT runs in registers/LDS (fft128.h); output goes to the
-//    device layout VM_FLT = complex float [c][i_vcs][s] so that each
-//    wavefront stores two 256-byte contiguous runs.
+//  * Fused path (default): one persistent kernel per chunk does unpack +
+//    WOLA + rounding + FFT + reorder, tile by tile (32 consecutive spectra of
+//    one RF input).  The 43-block input window of the next tile is fetched
+//    into registers (16-byte loads) while the current tile computes; WOLA
+//    accumulates exactly in int32 registers (no atomics, no memset); the FFT
+//    runs in registers/LDS (fft128.h); output goes to the device layout
+//    VM_FLT = complex float [c][i_vcs][s] so that each wavefront stores two
+//    256-byte contiguous runs.
 //  * Staged path (PFB_STAGED): the reference's four passes over the
 //    [m][i][n] int2/float2 buffer, kept for callers that inspect the
 //    intermediate buffer; it shares the FFT code and is bit-identical.
@@ -91,8 +92,9 @@ __device__ __forceinline__ uint32_t pack16(int lo, int hi)
     return (uint32_t)(lo & 0xffff) | ((uint32_t)hi << 16);
 }
 
-// grid (ceil(nspc/TILE_M), I), block 256.  htr points at the chunk's first
-// voltage block (the margin); out is VM_FLT [c][i_vcs][nspc].
+// Tile = TILE_M consecutive spectra m0.. of one RF input i; block 256.  htr
+// points at the chunk's first voltage block (the margin); out is VM_FLT
+// [c][i_vcs][nspc].
 //
 // WOLA: b[m][n] = sum_p h[KP - Kp - n - 1] x[(m + 500 - P)K + pK + n]
 // (src/pfb.cpp:149-161).  Lane (col, g) owns samples n = 2col, 2col+1 of
@@ -101,40 +103,23 @@ __device__ __forceinline__ uint32_t pack16(int lo, int hi)
 // tap pairs with v_dot2_i32_i16.  SPLIT: taps wider than int16 are split
 // exactly as h = 2^15 hh + hl (0 <= hl < 2^15) and summed as two products;
 // int32 wrap-around is identical to the reference's int atomics.
-template <bool FPGA, bool SPLIT>
-__global__ __launch_bounds__(256) void k_pfb_fused(const uint32_t *__restrict__ htr,
-                                                   const int *__restrict__ h,
-                                                   const int *__restrict__ out_idx, int I,
-                                                   int nspc, int imag_first,
-                                                   cf *__restrict__ out)
+#define PFB_P 12
+#define PFB_WROWS 48                  // window rows staged (>= TILE_M + PFB_P - 1)
+#define PFB_WIN (PFB_WROWS * 16)      // uint4 of one tile's input window
+#define PFB_WPL (PFB_WIN / 256)       // window uint4 per lane (exact)
+
+typedef __amdgpu_buffer_rsrc_t pfb_rsrc;
+#define PFB_RSRC_DWORD3 0x00020000 /* raw buffer, 32-bit data format (gfx9 family) */
+
+// tap pairs (h_p, h_{p+1}) for p = 0, 2, ..., P-2 at n = 2col and 2col+1
+struct pfb_taps {
+    uint32_t l0[PFB_P / 2], l1[PFB_P / 2], h0[PFB_P / 2], h1[PFB_P / 2];
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ void pfb_load_taps(pfb_taps &T, const int *__restrict__ h, int col)
 {
-    constexpr int P = 12;
-    constexpr int NW = 8 + P - 1;                 // window blocks per lane
-    __shared__ cf F[TILE_M * ROW];                // FFT rows (33 KB)
-    uint32_t *win = (uint32_t *)F;                // input window aliases F (11 KB)
-
-    const int t = threadIdx.x;
-    const int i = blockIdx.y;
-    const int m0 = blockIdx.x * TILE_M;
-    const int nvalid = min(TILE_M, nspc - m0);
-    const int nblk = nvalid + P - 1;
-
-    // Phase A: stage blocks q = m0 + ql + 500 - P, 16 bytes per lane
-    for (int j = t; j < (TILE_M + P - 1) * 16; j += 256) {
-        int ql = j >> 4, c4 = j & 15;
-        uint4 val = make_uint4(0, 0, 0, 0);
-        if (ql < nblk) {
-            int q = m0 + ql + 500 - P;               // 128-sample block index
-            int vb = q / 500;                         // MWAX voltage block
-            int within = (q - vb * 500) * KCH;        // sample offset in the block
-            size_t dw = ((size_t)vb * I + i) * (VSPVB / 2) + (within >> 1);
-            val = ((const uint4 *)(htr + dw))[c4];
-        }
-        ((uint4 *)win)[j] = val;
-    }
-    const int col = t & 63, g = t >> 6;
-    // tap pairs (h_p, h_{p+1}) for p = 0, 2, ..., P-2, for n0 and n1
-    uint32_t Hl0[P / 2], Hl1[P / 2], Hh0[P / 2], Hh1[P / 2];
+    constexpr int P = PFB_P;
 #pragma unroll
     for (int k = 0; k < P / 2; k++) {
         int a0 = h[KCH * P - KCH * (2 * k) - 2 * col - 1];
@@ -142,51 +127,98 @@ __global__ __launch_bounds__(256) void k_pfb_fused(const uint32_t *__restrict__ 
         int a1 = h[KCH * P - KCH * (2 * k) - 2 * col - 2];
         int b1 = h[KCH * P - KCH * (2 * k + 1) - 2 * col - 2];
         if (SPLIT) {
-            Hl0[k] = pack16(a0 & 0x7fff, b0 & 0x7fff);
-            Hl1[k] = pack16(a1 & 0x7fff, b1 & 0x7fff);
-            Hh0[k] = pack16(a0 >> 15, b0 >> 15);
-            Hh1[k] = pack16(a1 >> 15, b1 >> 15);
+            T.l0[k] = pack16(a0 & 0x7fff, b0 & 0x7fff);
+            T.l1[k] = pack16(a1 & 0x7fff, b1 & 0x7fff);
+            T.h0[k] = pack16(a0 >> 15, b0 >> 15);
+            T.h1[k] = pack16(a1 >> 15, b1 >> 15);
         } else {
-            Hl0[k] = pack16(a0, b0);
-            Hl1[k] = pack16(a1, b1);
+            T.l0[k] = pack16(a0, b0);
+            T.l1[k] = pack16(a1, b1);
         }
     }
-    __syncthreads();
+}
 
-    // Phase B: WOLA into registers
+// the tile's input window: 128-sample blocks q = m0 + ql + 500 - P, ql <
+// PFB_WROWS, 16 bytes per slot, by buffer loads that return zeros past the
+// end of the chunk's buffer (rows past the tile's last spectrum + P - 1 only
+// feed spectra that are never stored).  No branches: every lane issues the
+// same PFB_WPL loads for every tile.
+__device__ __forceinline__ void pfb_fetch_window(uint4 (&wr)[PFB_WPL], pfb_rsrc rs, int i, int I,
+                                                 int m0, int t)
+{
+#pragma unroll
+    for (int r = 0; r < PFB_WPL; r++) {
+        const int j = t + 256 * r;
+        const int ql = j >> 4, c4 = j & 15;
+        const int q = m0 + ql + 500 - PFB_P;        // 128-sample block index
+        const int vb = q / 500;                      // MWAX voltage block
+        const int within = (q - vb * 500) * KCH;     // sample offset in the block
+        const uint32_t byte = ((uint32_t)(vb * I + i) * (uint32_t)VSPVB + (uint32_t)within) * 2u +
+                              16u * (uint32_t)c4;
+        wr[r] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)byte, 0, 0));
+    }
+}
+
+__device__ __forceinline__ void pfb_put_window(const uint4 (&wr)[PFB_WPL], uint32_t *win, int t)
+{
+#pragma unroll
+    for (int r = 0; r < PFB_WPL; r++) ((uint4 *)win)[t + 256 * r] = wr[r];
+}
+
+// WOLA + rounding + FFT + reorder of one tile whose window is in LDS (win
+// aliases F); ends with the tile's stores.  Contains workgroup barriers.
+template <bool FPGA, bool SPLIT>
+__device__ __forceinline__ void pfb_tile(cf *F, const uint32_t *win, const cf *tw,
+                                         const pfb_taps &T, int i, int I, int m0, int nspc, int oi,
+                                         int imag_first, cf *__restrict__ out, int t)
+{
+    constexpr int P = PFB_P;
+    constexpr int NW = 8 + P - 1; // window blocks per lane
+    const int nvalid = min(TILE_M, nspc - m0);
+    const int col = t & 63, g = t >> 6;
+
+    // WOLA into registers, walking block pairs (q, q+1): each pair's
+    // sign-extended int16 pairs are built once and feed every spectrum j and
+    // tap pair k with j + 2k = q (per accumulator the k order is ascending,
+    // as in the reference's sum; int32 addition is exact or wraps the same)
     uint32_t w[NW];
 #pragma unroll
     for (int q = 0; q < NW; q++) w[q] = win[(8 * g + q) * 64 + col];
-    int acc[8][4];
+    int acc[8][4], bcc[8][4];
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        int ar0 = 0, ai0 = 0, ar1 = 0, ai1 = 0, br0 = 0, bi0 = 0, br1 = 0, bi1 = 0;
+    for (int j = 0; j < 8; j++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) acc[j][e] = bcc[j][e] = 0;
+#pragma unroll
+    for (int q = 0; q < NW - 1; q++) {
+        const uint32_t lo = w[q], hi = w[q + 1];
+        // bytes of lo: re_n0, im_n0, re_n1, im_n1 (block q); hi: block q+1
+        const uint32_t I0 = __builtin_amdgcn_perm(hi, lo, 0x0A050801u);
+        const uint32_t I1 = __builtin_amdgcn_perm(hi, lo, 0x0B070903u);
+        const uint32_t R0 = __builtin_amdgcn_perm(hi << 8, lo << 8, 0x0A050801u);
+        const uint32_t R1 = __builtin_amdgcn_perm(hi << 8, lo << 8, 0x0B070903u);
 #pragma unroll
         for (int k = 0; k < P / 2; k++) {
-            uint32_t lo = w[j + 2 * k], hi = w[j + 2 * k + 1];
-            // bytes of lo: re_n0, im_n0, re_n1, im_n1 (block q); hi: block q+1
-            uint32_t I0 = __builtin_amdgcn_perm(hi, lo, 0x0A050801u);
-            uint32_t I1 = __builtin_amdgcn_perm(hi, lo, 0x0B070903u);
-            uint32_t R0 = __builtin_amdgcn_perm(hi << 8, lo << 8, 0x0A050801u);
-            uint32_t R1 = __builtin_amdgcn_perm(hi << 8, lo << 8, 0x0B070903u);
-            ar0 = dot2(Hl0[k], R0, ar0);
-            ai0 = dot2(Hl0[k], I0, ai0);
-            ar1 = dot2(Hl1[k], R1, ar1);
-            ai1 = dot2(Hl1[k], I1, ai1);
+            const int j = q - 2 * k;
+            if (j < 0 || j >= 8) continue;
+            acc[j][0] = dot2(T.l0[k], R0, acc[j][0]);
+            acc[j][1] = dot2(T.l0[k], I0, acc[j][1]);
+            acc[j][2] = dot2(T.l1[k], R1, acc[j][2]);
+            acc[j][3] = dot2(T.l1[k], I1, acc[j][3]);
             if (SPLIT) {
-                br0 = dot2(Hh0[k], R0, br0);
-                bi0 = dot2(Hh0[k], I0, bi0);
-                br1 = dot2(Hh1[k], R1, br1);
-                bi1 = dot2(Hh1[k], I1, bi1);
+                bcc[j][0] = dot2(T.h0[k], R0, bcc[j][0]);
+                bcc[j][1] = dot2(T.h0[k], I0, bcc[j][1]);
+                bcc[j][2] = dot2(T.h1[k], R1, bcc[j][2]);
+                bcc[j][3] = dot2(T.h1[k], I1, bcc[j][3]);
             }
         }
-        if (SPLIT) {
-            ar0 = (int)((uint32_t)ar0 + ((uint32_t)br0 << 15));
-            ai0 = (int)((uint32_t)ai0 + ((uint32_t)bi0 << 15));
-            ar1 = (int)((uint32_t)ar1 + ((uint32_t)br1 << 15));
-            ai1 = (int)((uint32_t)ai1 + ((uint32_t)bi1 << 15));
-        }
-        acc[j][0] = ar0; acc[j][1] = ai0; acc[j][2] = ar1; acc[j][3] = ai1;
+    }
+    if (SPLIT) {
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                acc[j][e] = (int)((uint32_t)acc[j][e] + ((uint32_t)bcc[j][e] << 15));
     }
     __syncthreads(); // window no longer needed: F may overwrite it
     const float post = 1.0f / KCH; // the /K of the pack step, exact
@@ -198,24 +230,79 @@ __global__ __launch_bounds__(256) void k_pfb_fused(const uint32_t *__restrict__ 
     }
     __syncthreads();
 
-    // Phase C/D: 32 FFTs, 8 lanes each
+    // 32 FFTs, 8 lanes each
     const int f = t & 31, l = t >> 5;
-    vmfft::fft_stage1(&F[f * ROW], l, g_tw128);
+    vmfft::fft_stage1(&F[f * ROW], l, tw);
     cf y[2][8];
     vmfft::fft_stage2(&F[f * ROW], l, y);
 
-    if (f < nvalid) {
-        const int oi = out_idx[i];
+    // Every lane issues its 16 stores; in a partial tile the lanes past the
+    // last spectrum repeat the store of lane f = nvalid - 1 (same address,
+    // same value), so the instruction stream never branches around stores.
+    const int fs = min(f, nvalid - 1);
+    const int src = (t & 32) + fs; // lane of (fs, l) in this wavefront
 #pragma unroll
-        for (int hh = 0; hh < 2; hh++)
+    for (int hh = 0; hh < 2; hh++)
 #pragma unroll
-            for (int k2 = 0; k2 < 8; k2++) {
-                int k = l + 8 * hh + 16 * k2;
-                int kp = (k + KCH / 2) & (KCH - 1); // fftshift, src/pfb.cpp:315
-                cf v = y[hh][k2];
-                cf o = imag_first ? v : cf{v.y, v.x};
-                out[((size_t)kp * I + oi) * nspc + m0 + f] = o;
-            }
+        for (int k2 = 0; k2 < 8; k2++) {
+            int k = l + 8 * hh + 16 * k2;
+            int kp = (k + KCH / 2) & (KCH - 1); // fftshift, src/pfb.cpp:315
+            cf v = y[hh][k2];
+            v.x = __shfl(v.x, src, 64); // identity when the tile is full
+            v.y = __shfl(v.y, src, 64);
+            cf o = imag_first ? v : cf{v.y, v.x};
+            out[((size_t)kp * I + oi) * nspc + m0 + fs] = o;
+        }
+}
+
+// Fused forward PFB, persistent: workgroup b takes tiles [b per, (b+1) per)
+// of the tile list (input-major, spectra blocks in time order), so
+// consecutive tiles of one workgroup are consecutive spectra of one input.
+// The next tile's window is loaded into registers before this tile's stores
+// are issued, so the loads never queue behind the stores and the stores
+// drain while the next tile computes; with no branches around vector-memory
+// instructions the wait for the window is vmcnt(#stores), not vmcnt(0).
+// Taps and twiddles are loaded once per workgroup.  htr_bytes: size of the
+// chunk's input buffer (loads past it return zeros).
+template <bool FPGA, bool SPLIT>
+__global__ __launch_bounds__(256) void k_pfb_fused(
+    const uint32_t *__restrict__ htr, uint32_t htr_bytes, const int *__restrict__ h,
+    const int *__restrict__ out_idx, int I, int nspc, int imag_first, cf *__restrict__ out,
+    int per)
+{
+    __shared__ cf F[TILE_M * ROW]; // FFT rows (33 KB); the window aliases it
+    __shared__ cf tw[KCH];         // twiddles in LDS: no vector-memory loads in the loop
+    uint32_t *win = (uint32_t *)F;
+    const int t = threadIdx.x;
+    const int nmb = (nspc + TILE_M - 1) / TILE_M;
+    const int ntiles = nmb * I;
+    const int T0 = blockIdx.x * per;
+    const int Tend = min(T0 + per, ntiles);
+    if (T0 >= Tend) return;
+    const pfb_rsrc rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(htr), (short)0,
+                                                          (int)htr_bytes, PFB_RSRC_DWORD3);
+    if (t < KCH) tw[t] = g_tw128[t];
+    pfb_taps T;
+    pfb_load_taps<SPLIT>(T, h, t & 63);
+    uint4 wr[PFB_WPL];
+    pfb_fetch_window(wr, rs, T0 / nmb, I, (T0 % nmb) * TILE_M, t);
+    // first tile peeled, so the loop is entered in the same memory state
+    // (one window in flight behind one tile's stores) on every iteration
+    __syncthreads();
+    pfb_put_window(wr, win, t);
+    const int t1 = min(T0 + 1, Tend - 1);
+    pfb_fetch_window(wr, rs, t1 / nmb, I, (t1 % nmb) * TILE_M, t);
+    __syncthreads();
+    pfb_tile<FPGA, SPLIT>(F, win, tw, T, T0 / nmb, I, (T0 % nmb) * TILE_M, nspc,
+                          out_idx[T0 / nmb], imag_first, out, t);
+    for (int tt = T0 + 1; tt < Tend; tt++) {
+        const int i = tt / nmb, m0 = (tt % nmb) * TILE_M;
+        __syncthreads(); // the previous tile's FFT is done with F
+        pfb_put_window(wr, win, t);
+        const int tn = min(tt + 1, Tend - 1); // the last prefetch repeats this tile's
+        pfb_fetch_window(wr, rs, tn / nmb, I, (tn % nmb) * TILE_M, t);
+        __syncthreads();
+        pfb_tile<FPGA, SPLIT>(F, win, tw, T, i, I, m0, nspc, out_idx[i], imag_first, out, t);
     }
 }
 
@@ -469,26 +556,34 @@ static cf *flt_target(vcsbeam_context *vm)
 
 // Launch of the fused kernel on arbitrary device buffers (used by the chunk
 // API and by the benchmark / batched paths).
-void vm_pfb_fused_launch(const void *d_htr, const int *d_h, const int *d_out_idx, int I,
-                         int nspc, int P, int fpga, int imag_first, int split, void *d_out,
-                         hipStream_t stream)
+void vm_pfb_fused_launch(const void *d_htr, size_t htr_bytes, const int *d_h,
+                         const int *d_out_idx, int I, int nspc, int P, int fpga, int imag_first,
+                         int split, void *d_out, hipStream_t stream)
 {
-    dim3 grid((nspc + TILE_M - 1) / TILE_M, I), block(256);
-    if (P != 12) VM_FATAL("fused PFB kernel is built for 12 taps (got %d); use PFB_STAGED", P);
+    if (P != PFB_P) VM_FATAL("fused PFB kernel is built for 12 taps (got %d); use PFB_STAGED", P);
+    if (htr_bytes >= ((size_t)1 << 31))
+        VM_FATAL("fused PFB: chunk input of %zu bytes exceeds 2 GiB; use more chunks", htr_bytes);
     const uint32_t *x = (const uint32_t *)d_htr;
     cf *o = (cf *)d_out;
-    if (fpga && split)
-        hipLaunchKernelGGL((k_pfb_fused<true, true>), grid, block, 0, stream, x, d_h, d_out_idx, I,
-                           nspc, imag_first, o);
-    else if (fpga)
-        hipLaunchKernelGGL((k_pfb_fused<true, false>), grid, block, 0, stream, x, d_h, d_out_idx,
-                           I, nspc, imag_first, o);
-    else if (split)
-        hipLaunchKernelGGL((k_pfb_fused<false, true>), grid, block, 0, stream, x, d_h, d_out_idx,
-                           I, nspc, imag_first, o);
-    else
-        hipLaunchKernelGGL((k_pfb_fused<false, false>), grid, block, 0, stream, x, d_h, d_out_idx,
-                           I, nspc, imag_first, o);
+    // persistent launch: 4 workgroups per CU (LDS-limited), contiguous runs
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const int ntiles = ((nspc + TILE_M - 1) / TILE_M) * I;
+    const int nwg = min(ntiles, 4 * ncu);
+    const int per = (ntiles + nwg - 1) / nwg;
+    const dim3 grid((ntiles + per - 1) / per), block(256);
+#define VM_PFB(F_, S_)                                                                       \
+    hipLaunchKernelGGL((k_pfb_fused<F_, S_>), grid, block, 0, stream, x, (uint32_t)htr_bytes, \
+                       d_h, d_out_idx, I, nspc, imag_first, o, per)
+    if (fpga && split) VM_PFB(true, true);
+    else if (fpga) VM_PFB(true, false);
+    else if (split) VM_PFB(false, true);
+    else VM_PFB(false, false);
+#undef VM_PFB
     HIPCHECK(hipGetLastError());
 }
 
@@ -506,7 +601,8 @@ void vmWOLAChunk(vcsbeam_context *vm)
     } else {
         // fused: WOLA + rounding + FFT + reorder in one pass
         vm_timer_start(vm_dev(vm), VM_T_PFB);
-        vm_pfb_fused_launch(fpfb->d_htr_data, fpfb->d_filter_coeffs, fpfb->d_i_output_idx, fpfb->I,
+        vm_pfb_fused_launch(fpfb->d_htr_data, fpfb->d_htr_size, fpfb->d_filter_coeffs,
+                            fpfb->d_i_output_idx, fpfb->I,
                             fpfb->nspectra_per_chunk, fpfb->P,
                             (fpfb->flags & PFB_EMULATE_FPGA) != 0,
                             (fpfb->flags & PFB_IMAG_PART_FIRST) != 0, !fpfb->filter_fits_int16,
