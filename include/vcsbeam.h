@@ -16,9 +16,9 @@
  *  - device buffers behind the context (d_v, d_S, d_e, ...) use the layouts
  *    documented at each field; host buffers keep the reference layouts;
  *  - MI355X extensions (precision MFMA beamformer and its autotuner, device
- *    FP64 Jones/phase, device-resident inverse PFB and VDIF requantisation,
- *    device memory/event helpers, kernel timing) are appended at the end of
- *    the header and of vcsbeam_context.
+ *    FP64 Jones/phase, primary-beam surrogate, device-resident inverse PFB
+ *    and VDIF requantisation, device memory/event helpers, kernel timing) are
+ *    appended at the end of the header and of vcsbeam_context.
  */
 #ifndef VCSBEAM_H
 #define VCSBEAM_H
@@ -670,6 +670,45 @@ void vmCalcPhiDevice(vcsbeam_context *vm, beam_geom *beam_geom_vals);
 void vmCalcJDevice(vcsbeam_context *vm);
 /* Upload vm->D to d_D. */
 void vmPushD(vcsbeam_context *vm);
+/* Device FP64 J^-1 = F (D B)^-1 with a per-fine-channel primary beam shared
+ * by all tiles: B is host [npointing][nfine_chan][4] (e.g. from
+ * vmSurrogateJones); result in d_J. */
+void vmCalcJDeviceChanB(vcsbeam_context *vm, const gpuDoubleComplex *B);
+
+/* Primary-beam Jones for many directions (stands in for the FEE beam the
+ * reference evaluates per pointing with hyperbeam, src/primary_beam.c:79-191,
+ * 479-532): a project-defined analytic dipole-array beam (4x4 dual-pol short
+ * dipoles, 1.1 m grid, 0.278 m over ground, seeded 1 % complex gain and
+ * delay jitter per dipole) and a BF16 MLP surrogate (16 -> 256 -> 256 -> 256
+ * -> 8, GELU) trained on the GPU by this library's forward/backward kernels,
+ * with the analytic beam as fallback and judge. */
+typedef struct vm_beam_surrogate vm_beam_surrogate;
+/* Seeded dipole jitter (gain: complex std dev, delay: +-seconds) and an
+ * untrained network (seeded He-uniform weights). */
+vm_beam_surrogate *vmSurrogateCreate(uint64_t seed, double gain_jitter, double delay_jitter_s);
+void vmSurrogateDestroy(vm_beam_surrogate *s);
+/* Adam training on nsamples seeded samples (za <= za_max_deg, 170-230 MHz);
+ * returns the last mean-squared loss; loss_curve (optional) receives the
+ * mean loss of every 100 steps. */
+double vmSurrogateTrain(vm_beam_surrogate *s, long nsamples, int steps, int batch, float lr,
+                        double za_max_deg, uint64_t data_seed, float *loss_curve);
+/* One forward/backward pass on host features X [B][16] and targets T [B][8]:
+ * gradients of the mean-squared loss (packed layer by layer, W [out][in]
+ * then b) and the loss. */
+double vmSurrogateGrad(vm_beam_surrogate *s, const float *X, const float *T, int B, float *dW,
+                       float *db);
+/* Get (set = 0, weights as bf16-rounded fp32) or set (set = 1) the packed
+ * weights and biases. */
+void vmSurrogateParams(vm_beam_surrogate *s, float *W, float *b, int set);
+/* The dipole gains [2][16] (re, im) and delays [2][16] (s): 96 doubles. */
+void vmSurrogateConfig(const vm_beam_surrogate *s, double *cfg96);
+/* Jones B[p][c][4] (host, complex) for directions (az, za)[np] and
+ * frequencies freq[nf] (Hz): mode 0 analytic beam, 1 nominal (jitter-free)
+ * beam, 2 surrogate, 3 surrogate with fallback to the analytic beam where its
+ * norm-wise error exceeds 1e-3.  err (optional, [np][nf]): the surrogate's
+ * error against the analytic beam.  Returns the number of fallbacks. */
+long vmSurrogateJones(vm_beam_surrogate *s, int mode, const double *az, const double *za,
+                      int np, const double *freq, int nf, double *B, float *err);
 
 /* Precision MFMA beamformer for one chunk (vm->beamform_form selects FP16 or
  * BF16); vmBeamformChunk dispatches here when beamform_form != VM_BF_FP32. */

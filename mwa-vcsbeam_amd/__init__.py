@@ -232,6 +232,17 @@ _SIGS = {
     "vmCalcPhi": (None, [_CTX, C.POINTER(BeamGeom)]),
     "vmCalcPhiDevice": (None, [_CTX, C.POINTER(BeamGeom)]),
     "vmCalcJDevice": (None, [_CTX]),
+    "vmCalcJDeviceChanB": (None, [_CTX, C.c_void_p]),
+    "vmSurrogateCreate": (C.c_void_p, [C.c_uint64, C.c_double, C.c_double]),
+    "vmSurrogateDestroy": (None, [C.c_void_p]),
+    "vmSurrogateTrain": (C.c_double, [C.c_void_p, C.c_long, C.c_int, C.c_int, C.c_float,
+                                      C.c_double, C.c_uint64, C.c_void_p]),
+    "vmSurrogateGrad": (C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_void_p]),
+    "vmSurrogateParams": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "vmSurrogateConfig": (None, [C.c_void_p, C.c_void_p]),
+    "vmSurrogateJones": (C.c_long, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "vmPushD": (None, [_CTX]),
     "vmPushJ": (None, [_CTX]),
     "vmPushPhi": (None, [_CTX]),

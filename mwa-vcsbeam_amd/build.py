@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "_lib")
 INC = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["vm_device.hip", "pfb.hip", "beamform.hip", "beamform_mfma.hip",
-               "ipfb.hip", "geometry.hip"]
+               "ipfb.hip", "geometry.hip", "beam_model.hip"]
 C_SOURCES = ["vm_host.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

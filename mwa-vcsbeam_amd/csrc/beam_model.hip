@@ -1,0 +1,711 @@
+// beam_model.hip -- primary-beam Jones for many directions: a project-defined
+// analytic dipole-array beam (FP64) and a BF16 MLP surrogate trained on the
+// GPU with the kernels below.
+//
+// The reference evaluates its primary beam with the FEE model through
+// hyperbeam (src/primary_beam.c:79-191, 479-532), once per pointing at the
+// coarse-channel centre.  That model is not available here; this build
+// defines an analytic beam of the same tile geometry and uses it both to
+// train the surrogate and as the judge of its error:
+//   16 dual-pol short dipoles on a 4x4 grid (1.1 m), 0.278 m over a ground
+//   plane; l = sin za sin az, m = sin za cos az, k = 2 pi f / c
+//   AF_P = sum_d g[P][d] exp(i k (E_d l + N_d m) - i 2 pi f tau[P][d])
+//   GP = 2 sin(k h cos za), N0 = 16 * 2 sin(k h)
+//   J = GP / N0 [[AF_X Xt, AF_X Xp], [AF_Y Yt, AF_Y Yp]] with the short-
+//   dipole projections X = (cos za sin az, cos az), Y = (cos za cos az, -sin az)
+// "True" beam: seeded per-dipole complex gains (1 % jitter) and delays;
+// "nominal" beam: unit gains, zero delays.
+//
+// Surrogate: MLP 16 -> 256 -> 256 -> 256 -> 8, GELU (tanh form), BF16
+// weights and activations on v_mfma_f32_32x32x16_bf16 with FP32
+// accumulation, FP32 master weights and Adam.  It learns the residual
+// S_R (J_true - J_nom) from sin/cos features of (az, za, f); inference gives
+// J_nom + R / S_R and falls back to the analytic beam wherever the error
+// against it exceeds 1e-3 (norm-wise over the 2x2 Jones).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "vm_internal.h"
+
+#define HIPCHECK(x)                                                               \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) VM_FATAL("HIP: %s (%s)", hipGetErrorString(e_), #x); \
+    } while (0)
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+#define SPEED_OF_LIGHT 299792458.0
+#define DIP_SPACING 1.1
+#define DIP_HEIGHT 0.278
+#define NFEAT 16
+#define NOUT 8
+#define HID 256
+#define NLAYER 4
+#define RES_SCALE 32.0 // residual scale of the training targets
+#define FALLBACK_ERR 1e-3
+
+static const int kDims[NLAYER + 1] = {NFEAT, HID, HID, HID, NOUT};
+
+// ------------------------------------------------------------ analytic ----
+// cfg: gains [2][16] complex (re, im) then delays [2][16] seconds (96 doubles)
+__device__ void dipole_beam(double az, double za, double f, const double *cfg, bool nominal,
+                            double2 J[4])
+{
+    double sa, ca, sz, cz;
+    sincos(az, &sa, &ca);
+    sincos(za, &sz, &cz);
+    const double l = sz * sa, m = sz * ca;
+    const double k = 2.0 * M_PI * f / SPEED_OF_LIGHT;
+    double afr[2] = {0.0, 0.0}, afi[2] = {0.0, 0.0};
+#pragma unroll
+    for (int P = 0; P < 2; P++)
+        for (int d = 0; d < 16; d++) {
+            const double E = ((d % 4) - 1.5) * DIP_SPACING, N = ((d / 4) - 1.5) * DIP_SPACING;
+            double ph = k * (E * l + N * m);
+            double gr = 1.0, gi = 0.0;
+            if (!nominal) {
+                ph -= 2.0 * M_PI * f * cfg[64 + P * 16 + d];
+                gr = cfg[2 * (P * 16 + d)];
+                gi = cfg[2 * (P * 16 + d) + 1];
+            }
+            double s, c;
+            sincos(ph, &s, &c);
+            afr[P] += gr * c - gi * s;
+            afi[P] += gr * s + gi * c;
+        }
+    const double sc = 2.0 * sin(k * DIP_HEIGHT * cz) / (32.0 * sin(k * DIP_HEIGHT));
+    const double el[2][2] = {{cz * sa, ca}, {cz * ca, -sa}};
+#pragma unroll
+    for (int P = 0; P < 2; P++)
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            J[2 * P + q] = make_double2(sc * afr[P] * el[P][q], sc * afi[P] * el[P][q]);
+}
+
+// J for the (direction p, frequency c) grid: B[p][c][4]
+__global__ void k_beam_grid(const double *__restrict__ az, const double *__restrict__ za, int np,
+                            const double *__restrict__ freq, int nf,
+                            const double *__restrict__ cfg, int nominal,
+                            double2 *__restrict__ B)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)np * nf) return;
+    const int p = (int)(j / nf), c = (int)(j % nf);
+    double2 J[4];
+    dipole_beam(az[p], za[p], freq[c], cfg, nominal != 0, J);
+#pragma unroll
+    for (int q = 0; q < 4; q++) B[j * 4 + q] = J[q];
+}
+
+// ------------------------------------------------------------ features ----
+__device__ void mlp_features(double az, double za, double f, float x[NFEAT])
+{
+    double sa, ca, sz, cz;
+    sincos(az, &sa, &ca);
+    sincos(za, &sz, &cz);
+    const double l = sz * sa, m = sz * ca;
+    const double k = 2.0 * M_PI * f / SPEED_OF_LIGHT;
+    const double px = k * 0.5 * DIP_SPACING * l, py = k * 0.5 * DIP_SPACING * m;
+    const double gh = k * DIP_HEIGHT * cz;
+    x[0] = (float)l;
+    x[1] = (float)m;
+    x[2] = (float)cz;
+    x[3] = (float)((f - 200e6) / 30e6);
+    x[4] = (float)sa;
+    x[5] = (float)ca;
+    x[6] = (float)sin(px);
+    x[7] = (float)cos(px);
+    x[8] = (float)sin(3.0 * px);
+    x[9] = (float)cos(3.0 * px);
+    x[10] = (float)sin(py);
+    x[11] = (float)cos(py);
+    x[12] = (float)sin(3.0 * py);
+    x[13] = (float)cos(3.0 * py);
+    x[14] = (float)sin(gh);
+    x[15] = (float)cos(gh);
+}
+
+// counter-based uniform in [0, 1) (SplitMix64 finaliser)
+__device__ __forceinline__ double u01(uint64_t seed, uint64_t ctr)
+{
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (ctr + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1p-53;
+}
+
+// training samples: directions uniform on the cap za <= za_max, f uniform in
+// [fmin, fmax]; X bf16 [n][16], T fp32 [n][8] = RES_SCALE (J_true - J_nom)
+__global__ void k_surr_data(uint64_t seed, size_t n, double za_max, double fmin, double fmax,
+                            const double *__restrict__ cfg, __bf16 *__restrict__ X,
+                            float *__restrict__ T)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double cz = 1.0 - u01(seed, 3 * j) * (1.0 - cos(za_max));
+    const double za = acos(cz), az = 2.0 * M_PI * u01(seed, 3 * j + 1);
+    const double f = fmin + (fmax - fmin) * u01(seed, 3 * j + 2);
+    float x[NFEAT];
+    mlp_features(az, za, f, x);
+#pragma unroll
+    for (int q = 0; q < NFEAT; q++) X[j * NFEAT + q] = (__bf16)x[q];
+    double2 Jt[4], Jn[4];
+    dipole_beam(az, za, f, cfg, false, Jt);
+    dipole_beam(az, za, f, cfg, true, Jn);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        T[j * NOUT + 2 * q] = (float)(RES_SCALE * (Jt[q].x - Jn[q].x));
+        T[j * NOUT + 2 * q + 1] = (float)(RES_SCALE * (Jt[q].y - Jn[q].y));
+    }
+}
+
+// features of the (direction, frequency) grid, row p * nf + c
+__global__ void k_grid_features(const double *__restrict__ az, const double *__restrict__ za,
+                                int np, const double *__restrict__ freq, int nf,
+                                __bf16 *__restrict__ X)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)np * nf) return;
+    float x[NFEAT];
+    mlp_features(az[j / nf], za[j / nf], freq[j % nf], x);
+#pragma unroll
+    for (int q = 0; q < NFEAT; q++) X[j * NFEAT + q] = (__bf16)x[q];
+}
+
+// surrogate + fallback: B = J_nom + Y / RES_SCALE where its error against
+// the analytic J_true is <= FALLBACK_ERR, else J_true; err = that error
+__global__ void k_surr_assemble(const double *__restrict__ az, const double *__restrict__ za,
+                                int np, const double *__restrict__ freq, int nf,
+                                const double *__restrict__ cfg, const float *__restrict__ Y,
+                                int use_fallback, double2 *__restrict__ B,
+                                float *__restrict__ err, unsigned int *__restrict__ nfall)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)np * nf) return;
+    const int p = (int)(j / nf), c = (int)(j % nf);
+    double2 Jt[4], Jn[4], Js[4];
+    dipole_beam(az[p], za[p], freq[c], cfg, false, Jt);
+    dipole_beam(az[p], za[p], freq[c], cfg, true, Jn);
+    double num = 0.0, den = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        Js[q] = make_double2(Jn[q].x + (double)Y[j * NOUT + 2 * q] / RES_SCALE,
+                             Jn[q].y + (double)Y[j * NOUT + 2 * q + 1] / RES_SCALE);
+        const double dx = Js[q].x - Jt[q].x, dy = Js[q].y - Jt[q].y;
+        num += dx * dx + dy * dy;
+        den += Jt[q].x * Jt[q].x + Jt[q].y * Jt[q].y;
+    }
+    const double e = den > 0.0 ? sqrt(num / den) : sqrt(num);
+    err[j] = (float)e;
+    const bool fb = use_fallback && e > FALLBACK_ERR;
+    if (fb) atomicAdd(nfall, 1u);
+#pragma unroll
+    for (int q = 0; q < 4; q++) B[j * 4 + q] = fb ? Jt[q] : Js[q];
+}
+
+// ---------------------------------------------------------------- GEMM ----
+// C (M x N) = A (M x K) B (K x N) on v_mfma_f32_32x32x16_bf16; A(m, k) at
+// A[m sam + k sak], B(k, n) at B[k sbk + n sbn] (bf16 elements), any layout.
+// Workgroup 256 lanes = 2 x 2 wavefronts, 64 x 64 tile, K steps of 16 staged
+// in LDS.  Epilogue EPI 0: Cf[m ldc + n] = acc (+ bias[n] if bias);
+// EPI 1: z = acc + bias[n], Cf = z (pre-activation), Hb = bf16(gelu(z)).
+__device__ __forceinline__ float gelu_f(float x)
+{
+    const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
+    return 0.5f * x * (1.0f + tanhf(u));
+}
+
+__device__ __forceinline__ float dgelu_f(float x)
+{
+    const float a = 0.7978845608f, c = 0.044715f;
+    const float t = tanhf(a * (x + c * x * x * x));
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * a * (1.0f + 3.0f * c * x * x);
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const __bf16 *__restrict__ A,
+                                                  long sam, long sak, const __bf16 *__restrict__ Bm,
+                                                  long sbk, long sbn, const float *__restrict__ bias,
+                                                  float *__restrict__ Cf, long ldc,
+                                                  __bf16 *__restrict__ Hb, long ldh)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 As[64][24]; // [m][k], 16 + 8 pad
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[64][24]; // [n][k]
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    f16v acc;
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = 0.f;
+    const int lr = t >> 2, lk = (t & 3) * 4; // loader: row/col lr, k lk..lk+3
+    for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int k = k0 + lk + e;
+            const int m = m0 + lr, n = n0 + lr;
+            As[lr][lk + e] = (m < M && k < K) ? A[(long)m * sam + (long)k * sak] : (__bf16)0.0f;
+            Bs[lr][lk + e] = (n < N && k < K) ? Bm[(long)k * sbk + (long)n * sbn] : (__bf16)0.0f;
+        }
+        __syncthreads();
+        const bf8 a = *(const bf8 *)&As[32 * wr + (lane & 31)][8 * (lane >> 5)];
+        const bf8 b = *(const bf8 *)&Bs[32 * wc + (lane & 31)][8 * (lane >> 5)];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        __syncthreads();
+    }
+    const int n = n0 + 32 * wc + (lane & 31);
+    if (n >= N) return;
+    const float bv = bias ? bias[n] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int m = m0 + 32 * wr + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        if (m >= M) continue;
+        const float z = acc[j] + bv;
+        Cf[(long)m * ldc + n] = z;
+        if (EPI == 1) Hb[(long)m * ldh + n] = (__bf16)gelu_f(z);
+    }
+}
+
+// -------------------------------------------------------- training ops ----
+// loss and dZ of the output layer: dZ = 2 (Y - T) / (B NOUT) as bf16
+__global__ void k_mse(const float *__restrict__ Y, const float *__restrict__ T, size_t n,
+                      float inv, __bf16 *__restrict__ dZ, float *__restrict__ loss)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float d2 = 0.f;
+    if (j < n) {
+        const float d = Y[j] - T[j];
+        dZ[j] = (__bf16)(2.f * d * inv);
+        d2 = d * d;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) d2 += __shfl_xor(d2, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, d2 * inv);
+}
+
+// through the GELU: dZ = bf16(dH * gelu'(Z))
+__global__ void k_gelu_bwd(const float *__restrict__ dH, const float *__restrict__ Z, size_t n,
+                           __bf16 *__restrict__ dZ)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) dZ[j] = (__bf16)(dH[j] * dgelu_f(Z[j]));
+}
+
+// db[o] = sum_b dZ[b][o]; one lane per o, rows strided
+__global__ void k_bias_grad(const __bf16 *__restrict__ dZ, int B, int no, float *__restrict__ db)
+{
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= no) return;
+    float s = 0.f;
+    for (int b = 0; b < B; b++) s += (float)dZ[(size_t)b * no + o];
+    db[o] = s;
+}
+
+// Adam on FP32 masters, refreshing the bf16 copies W (and W^T if given)
+__global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ g, size_t n, float lr, float b1, float b2,
+                       float c1, float c2, float eps, __bf16 *__restrict__ wb, int rows, int cols,
+                       __bf16 *__restrict__ wbt)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float gj = g[j];
+    const float mj = b1 * m[j] + (1.f - b1) * gj;
+    const float vj = b2 * v[j] + (1.f - b2) * gj * gj;
+    m[j] = mj;
+    v[j] = vj;
+    const float wj = w[j] - lr * (mj * c1) / (sqrtf(vj * c2) + eps);
+    w[j] = wj;
+    if (wb) wb[j] = (__bf16)wj;
+    if (wbt) wbt[(j % cols) * rows + j / cols] = (__bf16)wj;
+}
+
+__global__ void k_to_bf16(const float *__restrict__ x, size_t n, __bf16 *__restrict__ y, int rows,
+                          int cols, __bf16 *__restrict__ yt)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    y[j] = (__bf16)x[j];
+    if (yt) yt[(j % cols) * rows + j / cols] = (__bf16)x[j];
+}
+
+__global__ void k_zero_f32(float *p, size_t n)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) p[j] = 0.f;
+}
+
+// =================================================================== host ==
+struct vm_beam_surrogate {
+    double cfg[96];  // dipole gains [2][16] (re, im) and delays [2][16]
+    double *d_cfg;
+    float *d_W[NLAYER], *d_b[NLAYER], *d_mW[NLAYER], *d_vW[NLAYER], *d_mb[NLAYER],
+        *d_vb[NLAYER], *d_gW[NLAYER], *d_gb[NLAYER];
+    __bf16 *d_Wb[NLAYER], *d_Wt[NLAYER]; // bf16 W [out][in] and W^T [in][out]
+    // activations for up to Bmax rows
+    int Bmax;
+    float *d_Z[NLAYER];    // pre-activations (fp32) [B][dims[l+1]]
+    __bf16 *d_H[NLAYER];   // activations (bf16) [B][dims[l+1]] (l < NLAYER-1)
+    float *d_dH;           // gradient w.r.t. an activation (fp32) [B][HID]
+    __bf16 *d_dZ[NLAYER];  // gradient w.r.t. a pre-activation (bf16)
+    float *d_loss;
+    long step;
+    hipStream_t stream;
+};
+
+extern "C" {
+
+static void *dalloc(size_t bytes)
+{
+    void *p = NULL;
+    HIPCHECK(hipMalloc(&p, bytes));
+    return p;
+}
+
+static unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+// host SplitMix64 + Box-Muller for the dipole configuration
+static uint64_t sm64(uint64_t *s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double hu01(uint64_t *s) { return (double)(sm64(s) >> 11) * 0x1p-53; }
+static double hnorm(uint64_t *s)
+{
+    double u1 = hu01(s), u2 = hu01(s);
+    if (u1 < 1e-300) u1 = 1e-300;
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+static void ensure_batch(vm_beam_surrogate *s, int B)
+{
+    if (B <= s->Bmax) return;
+    for (int l = 0; l < NLAYER; l++) {
+        if (s->d_Z[l]) HIPCHECK(hipFree(s->d_Z[l]));
+        if (s->d_H[l]) HIPCHECK(hipFree(s->d_H[l]));
+        if (s->d_dZ[l]) HIPCHECK(hipFree(s->d_dZ[l]));
+        s->d_Z[l] = (float *)dalloc(sizeof(float) * B * kDims[l + 1]);
+        s->d_H[l] = (__bf16 *)dalloc(sizeof(__bf16) * B * kDims[l + 1]);
+        s->d_dZ[l] = (__bf16 *)dalloc(sizeof(__bf16) * B * kDims[l + 1]);
+    }
+    if (s->d_dH) HIPCHECK(hipFree(s->d_dH));
+    s->d_dH = (float *)dalloc(sizeof(float) * B * HID);
+    s->Bmax = B;
+}
+
+vm_beam_surrogate *vmSurrogateCreate(uint64_t seed, double gain_jitter, double delay_jitter_s)
+{
+    vm_beam_surrogate *s = (vm_beam_surrogate *)calloc(1, sizeof(vm_beam_surrogate));
+    if (!s) VM_FATAL("out of host memory");
+    uint64_t st = seed;
+    for (int q = 0; q < 32; q++) {
+        s->cfg[2 * q] = 1.0 + gain_jitter * hnorm(&st) / sqrt(2.0);
+        s->cfg[2 * q + 1] = gain_jitter * hnorm(&st) / sqrt(2.0);
+    }
+    for (int q = 0; q < 32; q++) s->cfg[64 + q] = delay_jitter_s * (2.0 * hu01(&st) - 1.0);
+    HIPCHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->d_cfg = (double *)dalloc(sizeof(s->cfg));
+    HIPCHECK(hipMemcpy(s->d_cfg, s->cfg, sizeof(s->cfg), hipMemcpyHostToDevice));
+    // He-uniform initial weights (seeded), zero biases
+    for (int l = 0; l < NLAYER; l++) {
+        const int ni = kDims[l], no = kDims[l + 1];
+        const size_t nw = (size_t)ni * no;
+        float *hw = (float *)malloc(sizeof(float) * nw);
+        const double lim = sqrt(6.0 / ni) * (l == NLAYER - 1 ? 0.1 : 1.0);
+        for (size_t j = 0; j < nw; j++) hw[j] = (float)(lim * (2.0 * hu01(&st) - 1.0));
+        s->d_W[l] = (float *)dalloc(sizeof(float) * nw);
+        s->d_mW[l] = (float *)dalloc(sizeof(float) * nw);
+        s->d_vW[l] = (float *)dalloc(sizeof(float) * nw);
+        s->d_gW[l] = (float *)dalloc(sizeof(float) * nw);
+        s->d_b[l] = (float *)dalloc(sizeof(float) * no);
+        s->d_mb[l] = (float *)dalloc(sizeof(float) * no);
+        s->d_vb[l] = (float *)dalloc(sizeof(float) * no);
+        s->d_gb[l] = (float *)dalloc(sizeof(float) * no);
+        s->d_Wb[l] = (__bf16 *)dalloc(sizeof(__bf16) * nw);
+        s->d_Wt[l] = (__bf16 *)dalloc(sizeof(__bf16) * nw);
+        HIPCHECK(hipMemcpy(s->d_W[l], hw, sizeof(float) * nw, hipMemcpyHostToDevice));
+        free(hw);
+        hipLaunchKernelGGL(k_zero_f32, dim3(nblk(nw)), dim3(256), 0, s->stream, s->d_mW[l], nw);
+        hipLaunchKernelGGL(k_zero_f32, dim3(nblk(nw)), dim3(256), 0, s->stream, s->d_vW[l], nw);
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s->stream, s->d_b[l], (size_t)no);
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s->stream, s->d_mb[l], (size_t)no);
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s->stream, s->d_vb[l], (size_t)no);
+        hipLaunchKernelGGL(k_to_bf16, dim3(nblk(nw)), dim3(256), 0, s->stream, s->d_W[l], nw,
+                           s->d_Wb[l], no, ni, s->d_Wt[l]);
+    }
+    s->d_loss = (float *)dalloc(sizeof(float));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return s;
+}
+
+void vmSurrogateDestroy(vm_beam_surrogate *s)
+{
+    if (!s) return;
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    for (int l = 0; l < NLAYER; l++) {
+        float *fp[] = {s->d_W[l], s->d_b[l], s->d_mW[l], s->d_vW[l], s->d_mb[l],
+                       s->d_vb[l], s->d_gW[l], s->d_gb[l], s->d_Z[l]};
+        for (float *p : fp)
+            if (p) HIPCHECK(hipFree(p));
+        __bf16 *bp[] = {s->d_Wb[l], s->d_Wt[l], s->d_H[l], s->d_dZ[l]};
+        for (__bf16 *p : bp)
+            if (p) HIPCHECK(hipFree(p));
+    }
+    if (s->d_dH) HIPCHECK(hipFree(s->d_dH));
+    HIPCHECK(hipFree(s->d_loss));
+    HIPCHECK(hipFree(s->d_cfg));
+    HIPCHECK(hipStreamDestroy(s->stream));
+    free(s);
+}
+
+// forward of B rows of bf16 features X; output layer pre-activation in d_Z[NLAYER-1]
+static void mlp_forward(vm_beam_surrogate *s, const __bf16 *X, int B)
+{
+    ensure_batch(s, B);
+    for (int l = 0; l < NLAYER; l++) {
+        const int ni = kDims[l], no = kDims[l + 1];
+        const __bf16 *in = l == 0 ? X : s->d_H[l - 1];
+        const dim3 g((no + 63) / 64, (B + 63) / 64);
+        if (l < NLAYER - 1)
+            hipLaunchKernelGGL(k_mlp_gemm<1>, g, dim3(256), 0, s->stream, B, no, ni, in, (long)ni,
+                               1L, s->d_Wb[l], 1L, (long)ni, s->d_b[l], s->d_Z[l], (long)no,
+                               s->d_H[l], (long)no);
+        else
+            hipLaunchKernelGGL(k_mlp_gemm<0>, g, dim3(256), 0, s->stream, B, no, ni, in, (long)ni,
+                               1L, s->d_Wb[l], 1L, (long)ni, s->d_b[l], s->d_Z[l], (long)no,
+                               (__bf16 *)NULL, 0L);
+    }
+    HIPCHECK(hipGetLastError());
+}
+
+// gradients of the mean-squared loss against fp32 targets T for the batch
+// last run through mlp_forward; loss accumulated into d_loss
+static void mlp_backward(vm_beam_surrogate *s, const __bf16 *X, const float *T, int B)
+{
+    const size_t nout = (size_t)B * NOUT;
+    hipLaunchKernelGGL(k_mse, dim3(nblk(nout)), dim3(256), 0, s->stream, s->d_Z[NLAYER - 1], T,
+                       nout, 1.0f / (float)nout, s->d_dZ[NLAYER - 1], s->d_loss);
+    for (int l = NLAYER - 1; l >= 0; l--) {
+        const int ni = kDims[l], no = kDims[l + 1];
+        const __bf16 *in = l == 0 ? X : s->d_H[l - 1];
+        // dW[o][i] = sum_b dZ[b][o] in[b][i]
+        hipLaunchKernelGGL(k_mlp_gemm<0>, dim3((ni + 63) / 64, (no + 63) / 64), dim3(256), 0,
+                           s->stream, no, ni, B, s->d_dZ[l], 1L, (long)no, in, (long)ni, 1L,
+                           (const float *)NULL, s->d_gW[l], (long)ni, (__bf16 *)NULL, 0L);
+        hipLaunchKernelGGL(k_bias_grad, dim3(nblk(no)), dim3(256), 0, s->stream, s->d_dZ[l], B,
+                           no, s->d_gb[l]);
+        if (l > 0) {
+            // dH[b][i] = sum_o dZ[b][o] W[o][i], then through the GELU
+            hipLaunchKernelGGL(k_mlp_gemm<0>, dim3((ni + 63) / 64, (B + 63) / 64), dim3(256), 0,
+                               s->stream, B, ni, no, s->d_dZ[l], (long)no, 1L, s->d_Wt[l], 1L,
+                               (long)no, (const float *)NULL, s->d_dH, (long)ni, (__bf16 *)NULL,
+                               0L);
+            hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk((size_t)B * ni)), dim3(256), 0, s->stream,
+                               s->d_dH, s->d_Z[l - 1], (size_t)B * ni, s->d_dZ[l - 1]);
+        }
+    }
+    HIPCHECK(hipGetLastError());
+}
+
+static void mlp_adam(vm_beam_surrogate *s, float lr)
+{
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    s->step++;
+    const float c1 = (float)(1.0 / (1.0 - pow(b1, (double)s->step)));
+    const float c2 = (float)(1.0 / (1.0 - pow(b2, (double)s->step)));
+    for (int l = 0; l < NLAYER; l++) {
+        const int ni = kDims[l], no = kDims[l + 1];
+        const size_t nw = (size_t)ni * no;
+        hipLaunchKernelGGL(k_adam, dim3(nblk(nw)), dim3(256), 0, s->stream, s->d_W[l], s->d_mW[l],
+                           s->d_vW[l], s->d_gW[l], nw, lr, b1, b2, c1, c2, eps, s->d_Wb[l], no, ni,
+                           s->d_Wt[l]);
+        hipLaunchKernelGGL(k_adam, dim3(nblk(no)), dim3(256), 0, s->stream, s->d_b[l], s->d_mb[l],
+                           s->d_vb[l], s->d_gb[l], (size_t)no, lr, b1, b2, c1, c2, eps,
+                           (__bf16 *)NULL, 1, no, (__bf16 *)NULL);
+    }
+    HIPCHECK(hipGetLastError());
+}
+
+// Train on nsamples seeded samples (za <= za_max_deg, 170-230 MHz) for
+// `steps` Adam steps of `batch` rows, learning rate lr with cosine decay.
+// loss_curve (optional) receives the mean loss of every 100 steps.
+double vmSurrogateTrain(vm_beam_surrogate *s, long nsamples, int steps, int batch, float lr,
+                        double za_max_deg, uint64_t data_seed, float *loss_curve)
+{
+    if (batch <= 0 || nsamples < batch) VM_FATAL("surrogate: need nsamples >= batch > 0");
+    __bf16 *X = (__bf16 *)dalloc(sizeof(__bf16) * nsamples * NFEAT);
+    float *T = (float *)dalloc(sizeof(float) * nsamples * NOUT);
+    hipLaunchKernelGGL(k_surr_data, dim3(nblk(nsamples)), dim3(256), 0, s->stream, data_seed,
+                       (size_t)nsamples, za_max_deg * M_PI / 180.0, 170e6, 230e6, s->d_cfg, X, T);
+    HIPCHECK(hipGetLastError());
+    const long nb = nsamples / batch;
+    double last = 0.0, acc = 0.0;
+    int nacc = 0;
+    HIPCHECK(hipMemsetAsync(s->d_loss, 0, sizeof(float), s->stream));
+    for (int it = 0; it < steps; it++) {
+        const long bi = it % nb;
+        const __bf16 *xb = X + (size_t)bi * batch * NFEAT;
+        const float *tb = T + (size_t)bi * batch * NOUT;
+        mlp_forward(s, xb, batch);
+        mlp_backward(s, xb, tb, batch);
+        const float lrt = (float)(lr * 0.5 * (1.0 + cos(M_PI * it / (double)steps)));
+        mlp_adam(s, lrt > 0.02f * lr ? lrt : 0.02f * lr);
+        if ((it + 1) % 100 == 0 || it + 1 == steps) {
+            float h = 0.f;
+            HIPCHECK(hipMemcpyAsync(&h, s->d_loss, sizeof(float), hipMemcpyDeviceToHost,
+                                    s->stream));
+            HIPCHECK(hipStreamSynchronize(s->stream));
+            const int n = (it % 100) + 1;
+            last = h / n;
+            if (loss_curve) loss_curve[it / 100] = (float)last;
+            acc += h;
+            nacc += n;
+            HIPCHECK(hipMemsetAsync(s->d_loss, 0, sizeof(float), s->stream));
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    HIPCHECK(hipFree(X));
+    HIPCHECK(hipFree(T));
+    return last;
+}
+
+// One forward + backward pass on host features X [B][16] (rounded to bf16)
+// and targets T [B][8]; copies the fp32 gradients (packed like the weights:
+// layer by layer, W [out][in] then b) and returns the loss.  For checking the
+// training kernels against an FP64 reference.
+double vmSurrogateGrad(vm_beam_surrogate *s, const float *X, const float *T, int B, float *dW,
+                       float *db)
+{
+    float *dxf = (float *)dalloc(sizeof(float) * B * NFEAT);
+    __bf16 *dx = (__bf16 *)dalloc(sizeof(__bf16) * B * NFEAT);
+    float *dt = (float *)dalloc(sizeof(float) * B * NOUT);
+    HIPCHECK(hipMemcpyAsync(dxf, X, sizeof(float) * B * NFEAT, hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(hipMemcpyAsync(dt, T, sizeof(float) * B * NOUT, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_to_bf16, dim3(nblk((size_t)B * NFEAT)), dim3(256), 0, s->stream, dxf,
+                       (size_t)B * NFEAT, dx, 1, 1, (__bf16 *)NULL);
+    HIPCHECK(hipMemsetAsync(s->d_loss, 0, sizeof(float), s->stream));
+    mlp_forward(s, dx, B);
+    mlp_backward(s, dx, dt, B);
+    float h = 0.f;
+    HIPCHECK(hipMemcpyAsync(&h, s->d_loss, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    size_t wo = 0, bo = 0;
+    for (int l = 0; l < NLAYER; l++) {
+        const size_t nw = (size_t)kDims[l] * kDims[l + 1];
+        HIPCHECK(hipMemcpyAsync(dW + wo, s->d_gW[l], sizeof(float) * nw, hipMemcpyDeviceToHost,
+                                s->stream));
+        HIPCHECK(hipMemcpyAsync(db + bo, s->d_gb[l], sizeof(float) * kDims[l + 1],
+                                hipMemcpyDeviceToHost, s->stream));
+        wo += nw;
+        bo += kDims[l + 1];
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    HIPCHECK(hipFree(dxf));
+    HIPCHECK(hipFree(dx));
+    HIPCHECK(hipFree(dt));
+    return h;
+}
+
+// Weights as the forward pass uses them (bf16-rounded, as fp32) and biases,
+// packed layer by layer; set = 1 loads them instead (and resets Adam).
+void vmSurrogateParams(vm_beam_surrogate *s, float *W, float *b, int set)
+{
+    size_t wo = 0, bo = 0;
+    for (int l = 0; l < NLAYER; l++) {
+        const int ni = kDims[l], no = kDims[l + 1];
+        const size_t nw = (size_t)ni * no;
+        if (set) {
+            HIPCHECK(hipMemcpy(s->d_W[l], W + wo, sizeof(float) * nw, hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(s->d_b[l], b + bo, sizeof(float) * no, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_to_bf16, dim3(nblk(nw)), dim3(256), 0, s->stream, s->d_W[l], nw,
+                               s->d_Wb[l], no, ni, s->d_Wt[l]);
+            hipLaunchKernelGGL(k_zero_f32, dim3(nblk(nw)), dim3(256), 0, s->stream, s->d_mW[l], nw);
+            hipLaunchKernelGGL(k_zero_f32, dim3(nblk(nw)), dim3(256), 0, s->stream, s->d_vW[l], nw);
+            HIPCHECK(hipStreamSynchronize(s->stream));
+        } else {
+            // the masters, rounded to bf16 on the host with the kernels' RNE rule
+            HIPCHECK(hipMemcpy(W + wo, s->d_W[l], sizeof(float) * nw, hipMemcpyDeviceToHost));
+            for (size_t j = 0; j < nw; j++) {
+                uint32_t u;
+                memcpy(&u, &W[wo + j], 4);
+                u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+                memcpy(&W[wo + j], &u, 4);
+            }
+            HIPCHECK(hipMemcpy(b + bo, s->d_b[l], sizeof(float) * no, hipMemcpyDeviceToHost));
+        }
+        wo += nw;
+        bo += no;
+    }
+    if (set) s->step = 0;
+}
+
+// The dipole configuration (gains [2][16] (re, im), delays [2][16] s).
+void vmSurrogateConfig(const vm_beam_surrogate *s, double *cfg96) { memcpy(cfg96, s->cfg, sizeof(s->cfg)); }
+
+// Jones of the (direction, frequency) grid, B[p][c][4] complex (host):
+// mode 0 analytic true beam, 1 nominal beam, 2 surrogate without fallback,
+// 3 surrogate with fallback to the analytic beam where its error > 1e-3.
+// err (optional, [np][nf]) receives the surrogate's error against the
+// analytic beam (modes 2, 3); returns the number of fallbacks.
+long vmSurrogateJones(vm_beam_surrogate *s, int mode, const double *az, const double *za, int np,
+                      const double *freq, int nf, double *B, float *err)
+{
+    const size_t n = (size_t)np * nf;
+    double *daz = (double *)dalloc(sizeof(double) * np), *dza = (double *)dalloc(sizeof(double) * np);
+    double *df = (double *)dalloc(sizeof(double) * nf);
+    double2 *dB = (double2 *)dalloc(sizeof(double2) * 4 * n);
+    HIPCHECK(hipMemcpyAsync(daz, az, sizeof(double) * np, hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(hipMemcpyAsync(dza, za, sizeof(double) * np, hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(hipMemcpyAsync(df, freq, sizeof(double) * nf, hipMemcpyHostToDevice, s->stream));
+    long nfall = 0;
+    if (mode <= 1) {
+        hipLaunchKernelGGL(k_beam_grid, dim3(nblk(n)), dim3(256), 0, s->stream, daz, dza, np, df,
+                           nf, s->d_cfg, mode, dB);
+    } else {
+        __bf16 *X = (__bf16 *)dalloc(sizeof(__bf16) * n * NFEAT);
+        float *derr = (float *)dalloc(sizeof(float) * n);
+        unsigned int *dcnt = (unsigned int *)dalloc(sizeof(unsigned int));
+        HIPCHECK(hipMemsetAsync(dcnt, 0, sizeof(unsigned int), s->stream));
+        hipLaunchKernelGGL(k_grid_features, dim3(nblk(n)), dim3(256), 0, s->stream, daz, dza, np,
+                           df, nf, X);
+        // inference in slices of at most 65536 rows
+        const int slice = 65536;
+        float *Y = (float *)dalloc(sizeof(float) * n * NOUT);
+        for (size_t r0 = 0; r0 < n; r0 += slice) {
+            const int rows = (int)(n - r0 < (size_t)slice ? n - r0 : slice);
+            mlp_forward(s, X + r0 * NFEAT, rows);
+            HIPCHECK(hipMemcpyAsync(Y + r0 * NOUT, s->d_Z[NLAYER - 1], sizeof(float) * rows * NOUT,
+                                    hipMemcpyDeviceToDevice, s->stream));
+        }
+        hipLaunchKernelGGL(k_surr_assemble, dim3(nblk(n)), dim3(256), 0, s->stream, daz, dza, np,
+                           df, nf, s->d_cfg, Y, mode == 3 ? 1 : 0, dB, derr, dcnt);
+        unsigned int h = 0;
+        HIPCHECK(hipMemcpyAsync(&h, dcnt, sizeof(unsigned int), hipMemcpyDeviceToHost, s->stream));
+        if (err)
+            HIPCHECK(hipMemcpyAsync(err, derr, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        nfall = h;
+        HIPCHECK(hipFree(X));
+        HIPCHECK(hipFree(derr));
+        HIPCHECK(hipFree(dcnt));
+        HIPCHECK(hipFree(Y));
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(B, dB, sizeof(double2) * 4 * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    HIPCHECK(hipFree(daz));
+    HIPCHECK(hipFree(dza));
+    HIPCHECK(hipFree(df));
+    HIPCHECK(hipFree(dB));
+    return nfall;
+}
+
+} // extern "C"

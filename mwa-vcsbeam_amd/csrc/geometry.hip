@@ -52,9 +52,10 @@ __device__ __forceinline__ double2 cm(double2 a, double2 b)
 }
 __device__ __forceinline__ double2 ca(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 
-// D[a][c][4], B[p][a][4] -> J[p][a][c][4]
+// D[a][c][4], B[p][a][4] -> J[p][a][c][4]; with bchan, B is [p][c][4] (a
+// per-fine-channel beam shared by all tiles)
 __global__ void k_calc_J(const double2 *__restrict__ D, const double2 *__restrict__ B,
-                         int npointing, int nant, int nchan, double2 *__restrict__ J)
+                         int npointing, int nant, int nchan, int bchan, double2 *__restrict__ J)
 {
     size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t total = (size_t)npointing * nant * nchan;
@@ -63,7 +64,7 @@ __global__ void k_calc_J(const double2 *__restrict__ D, const double2 *__restric
     int a = (int)((j / nchan) % nant);
     int p = (int)(j / ((size_t)nchan * nant));
     const double2 *M1 = D + ((size_t)a * nchan + c) * 4;
-    const double2 *M2 = B + ((size_t)p * nant + a) * 4;
+    const double2 *M2 = B + ((size_t)p * (bchan ? nchan : nant) + (bchan ? c : a)) * 4;
     double2 Ji[4];
     Ji[0] = ca(cm(M1[0], M2[0]), cm(M1[1], M2[2]));
     Ji[1] = ca(cm(M1[0], M2[1]), cm(M1[1], M2[3]));
@@ -112,11 +113,11 @@ void vm_calc_phi_launch(const void *d_pos, const double *d_unit, const double *d
 }
 
 void vm_calc_J_launch(const void *d_D, const void *d_B, int npointing, int nant, int nchan,
-                      void *d_J, hipStream_t s)
+                      int bchan, void *d_J, hipStream_t s)
 {
     size_t total = (size_t)npointing * nant * nchan;
     hipLaunchKernelGGL(k_calc_J, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       (const double2 *)d_D, (const double2 *)d_B, npointing, nant, nchan,
+                       (const double2 *)d_D, (const double2 *)d_B, npointing, nant, nchan, bchan,
                        (double2 *)d_J);
     HIPCHECK(hipGetLastError());
 }
@@ -167,7 +168,20 @@ void vmCalcJDevice(vcsbeam_context *vm)
     size_t bbytes = (size_t)np * nant * 4 * sizeof(gpuDoubleComplex);
     ensure(&d->d_B, &d->B_bytes, bbytes);
     HIPCHECK(hipMemcpyAsync(d->d_B, vm->pb.B, bbytes, hipMemcpyHostToDevice, s));
-    vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, vm->d_J, s);
+    vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 0, vm->d_J, s);
+    HIPCHECK(hipStreamSynchronize(s));
+    d->weights_dirty = 1;
+}
+
+void vmCalcJDeviceChanB(vcsbeam_context *vm, const gpuDoubleComplex *B)
+{
+    vm_device_state *d = vm_dev(vm);
+    hipStream_t s = (hipStream_t)d->stream;
+    int nant = (int)vm->obs_metadata->num_ants, nchan = vm->nfine_chan, np = (int)vm->npointing;
+    size_t bbytes = (size_t)np * nchan * 4 * sizeof(gpuDoubleComplex);
+    ensure(&d->d_B, &d->B_bytes, bbytes);
+    HIPCHECK(hipMemcpyAsync(d->d_B, B, bbytes, hipMemcpyHostToDevice, s));
+    vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 1, vm->d_J, s);
     HIPCHECK(hipStreamSynchronize(s));
     d->weights_dirty = 1;
 }

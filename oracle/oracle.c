@@ -467,6 +467,174 @@ void orc_vdif_requant(const float *f, size_t n, uint8_t *out)
     }
 }
 
+/* ------------------------------------------------ primary beam (a19) --- */
+#define ORC_C 299792458.0
+#define ORC_DIP_SPACING 1.1
+#define ORC_DIP_HEIGHT 0.278
+
+void orc_dipole_beam(const double *az, const double *za, const double *freq, size_t n,
+                     const double *g, const double *tau, double *J)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < n; j++) {
+        const double sa = sin(az[j]), ca = cos(az[j]), sz = sin(za[j]), cz = cos(za[j]);
+        const double l = sz * sa, m = sz * ca, f = freq[j];
+        const double k = 2.0 * M_PI * f / ORC_C;
+        double afr[2] = {0, 0}, afi[2] = {0, 0};
+        for (int P = 0; P < 2; P++)
+            for (int d = 0; d < 16; d++) {
+                const double E = ((d % 4) - 1.5) * ORC_DIP_SPACING;
+                const double N = ((d / 4) - 1.5) * ORC_DIP_SPACING;
+                const double ph = k * (E * l + N * m) - 2.0 * M_PI * f * tau[P * 16 + d];
+                const double gr = g[2 * (P * 16 + d)], gi = g[2 * (P * 16 + d) + 1];
+                const double c = cos(ph), s = sin(ph);
+                afr[P] += gr * c - gi * s;
+                afi[P] += gr * s + gi * c;
+            }
+        const double gp = 2.0 * sin(k * ORC_DIP_HEIGHT * cz);
+        const double n0 = 16.0 * 2.0 * sin(k * ORC_DIP_HEIGHT);
+        const double sc = gp / n0;
+        const double el[2][2] = {{cz * sa, ca}, {cz * ca, -sa}};
+        for (int P = 0; P < 2; P++)
+            for (int q = 0; q < 2; q++) {
+                J[8 * j + 4 * P + 2 * q] = sc * afr[P] * el[P][q];
+                J[8 * j + 4 * P + 2 * q + 1] = sc * afi[P] * el[P][q];
+            }
+    }
+}
+
+void orc_mlp_features(const double *az, const double *za, const double *freq, size_t n,
+                      double *X)
+{
+    for (size_t j = 0; j < n; j++) {
+        const double sa = sin(az[j]), ca = cos(az[j]), sz = sin(za[j]), cz = cos(za[j]);
+        const double l = sz * sa, m = sz * ca;
+        const double k = 2.0 * M_PI * freq[j] / ORC_C;
+        const double px = k * 0.5 * ORC_DIP_SPACING * l, py = k * 0.5 * ORC_DIP_SPACING * m;
+        const double gh = k * ORC_DIP_HEIGHT * cz;
+        double *x = X + 16 * j;
+        x[0] = l;
+        x[1] = m;
+        x[2] = cz;
+        x[3] = (freq[j] - 200e6) / 30e6;
+        x[4] = sa;
+        x[5] = ca;
+        x[6] = sin(px);
+        x[7] = cos(px);
+        x[8] = sin(3.0 * px);
+        x[9] = cos(3.0 * px);
+        x[10] = sin(py);
+        x[11] = cos(py);
+        x[12] = sin(3.0 * py);
+        x[13] = cos(3.0 * py);
+        x[14] = sin(gh);
+        x[15] = cos(gh);
+    }
+}
+
+static double orc_gelu(double x, double *dgelu)
+{
+    const double a = 0.7978845608028654, c = 0.044715;
+    const double u = a * (x + c * x * x * x);
+    const double t = tanh(u);
+    if (dgelu) *dgelu = 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * a * (1.0 + 3.0 * c * x * x);
+    return 0.5 * x * (1.0 + t);
+}
+
+/* forward keeping every layer's pre-activation z_l and output a_l */
+static void orc_mlp_run(const double *X, int B, int nl, const int *dims, const double *W,
+                        const double *b, double **z, double **a)
+{
+    size_t wo = 0, bo = 0;
+    for (int L = 0; L < nl; L++) {
+        const int ni = dims[L], no = dims[L + 1];
+        const double *in = L == 0 ? X : a[L - 1];
+        for (int r = 0; r < B; r++)
+            for (int o = 0; o < no; o++) {
+                double acc = b[bo + o];
+                for (int i = 0; i < ni; i++) acc += W[wo + (size_t)o * ni + i] * in[(size_t)r * ni + i];
+                z[L][(size_t)r * no + o] = acc;
+                a[L][(size_t)r * no + o] = L < nl - 1 ? orc_gelu(acc, NULL) : acc;
+            }
+        wo += (size_t)ni * no;
+        bo += no;
+    }
+}
+
+void orc_mlp_forward(const double *X, int B, int nl, const int *dims, const double *W,
+                     const double *b, double *Y)
+{
+    double *z[16], *a[16];
+    for (int L = 0; L < nl; L++) {
+        z[L] = (double *)malloc(sizeof(double) * B * dims[L + 1]);
+        a[L] = (double *)malloc(sizeof(double) * B * dims[L + 1]);
+    }
+    orc_mlp_run(X, B, nl, dims, W, b, z, a);
+    memcpy(Y, a[nl - 1], sizeof(double) * B * dims[nl]);
+    for (int L = 0; L < nl; L++) {
+        free(z[L]);
+        free(a[L]);
+    }
+}
+
+double orc_mlp_grad(const double *X, const double *T, int B, int nl, const int *dims,
+                    const double *W, const double *b, double *dW, double *db)
+{
+    double *z[16], *a[16];
+    size_t woff[17], boff[17];
+    woff[0] = boff[0] = 0;
+    for (int L = 0; L < nl; L++) {
+        z[L] = (double *)malloc(sizeof(double) * B * dims[L + 1]);
+        a[L] = (double *)malloc(sizeof(double) * B * dims[L + 1]);
+        woff[L + 1] = woff[L] + (size_t)dims[L] * dims[L + 1];
+        boff[L + 1] = boff[L] + dims[L + 1];
+    }
+    orc_mlp_run(X, B, nl, dims, W, b, z, a);
+    const int nout = dims[nl];
+    const double inv = 1.0 / ((double)B * nout);
+    double loss = 0.0;
+    double *g = (double *)malloc(sizeof(double) * B * nout); /* dLoss/dz of the current layer */
+    for (size_t j = 0; j < (size_t)B * nout; j++) {
+        const double d = a[nl - 1][j] - T[j];
+        loss += d * d;
+        g[j] = 2.0 * d * inv;
+    }
+    for (int L = nl - 1; L >= 0; L--) {
+        const int ni = dims[L], no = dims[L + 1];
+        const double *in = L == 0 ? X : a[L - 1];
+        for (int o = 0; o < no; o++) {
+            double sb = 0.0;
+            for (int r = 0; r < B; r++) sb += g[(size_t)r * no + o];
+            db[boff[L] + o] = sb;
+            for (int i = 0; i < ni; i++) {
+                double sw = 0.0;
+                for (int r = 0; r < B; r++) sw += g[(size_t)r * no + o] * in[(size_t)r * ni + i];
+                dW[woff[L] + (size_t)o * ni + i] = sw;
+            }
+        }
+        if (L > 0) {
+            double *gp = (double *)malloc(sizeof(double) * B * ni);
+            for (int r = 0; r < B; r++)
+                for (int i = 0; i < ni; i++) {
+                    double acc = 0.0;
+                    for (int o = 0; o < no; o++)
+                        acc += g[(size_t)r * no + o] * W[woff[L] + (size_t)o * ni + i];
+                    double dg;
+                    orc_gelu(z[L - 1][(size_t)r * ni + i], &dg);
+                    gp[(size_t)r * ni + i] = acc * dg;
+                }
+            free(g);
+            g = gp;
+        }
+    }
+    free(g);
+    for (int L = 0; L < nl; L++) {
+        free(z[L]);
+        free(a[L]);
+    }
+    return loss * inv;
+}
+
 double orc_rel_err(const double *x, const double *ref, size_t n)
 {
     long double num = 0.0L, den = 0.0L;

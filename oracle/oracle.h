@@ -133,6 +133,41 @@ void orc_ipfb_gather(const double *ring, int file_no, int nsamples, int K,
  * (src/beam_vdif.c:29-38, 254-260): clamp to [-126,127], rint, ^0x80. */
 void orc_vdif_requant(const float *f, size_t n, uint8_t *out);
 
+/* a19: project-defined analytic dipole-array primary beam (the reference
+ * has no analytic beam; its FEE model, hyperbeam + HDF5, is not available,
+ * src/primary_beam.c:479-532 is the call site it stands in for).  Tile: 16
+ * dual-pol short dipoles on a 4x4 grid (spacing 1.1 m, E = ((d%4)-1.5)*1.1,
+ * N = ((d/4)-1.5)*1.1), 0.278 m above a ground plane.  With l = sin za sin az,
+ * m = sin za cos az, k = 2 pi f / c:
+ *   AF_P = sum_d g[P][d] exp(i k (E_d l + N_d m) - i 2 pi f tau[P][d])
+ *   GP   = 2 sin(k h cos za),   N0 = 16 * 2 sin(k h)
+ *   X (E-W) dipole: (E_theta, E_phi) = (cos za sin az,  cos az)
+ *   Y (N-S) dipole: (E_theta, E_phi) = (cos za cos az, -sin az)
+ *   J = GP / N0 [[AF_X Xt, AF_X Xp], [AF_Y Yt, AF_Y Yp]]
+ * g: complex dipole gains [2][16] as (re, im); tau: dipole delays [2][16]
+ * in seconds.  J: [n][4] complex (J00 J01 J10 J11) as (re, im). */
+void orc_dipole_beam(const double *az, const double *za, const double *freq, size_t n,
+                     const double *g, const double *tau, double *J);
+
+/* The surrogate's 16 input features of (az, za, f): l, m, cos za, f_n,
+ * sin az, cos az, sin/cos(psi_x), sin/cos(3 psi_x), sin/cos(psi_y),
+ * sin/cos(3 psi_y), sin/cos(k h cos za), with f_n = (f - 200 MHz)/30 MHz,
+ * psi_x = k 0.55 l, psi_y = k 0.55 m (half the dipole spacing). X: [n][16]. */
+void orc_mlp_features(const double *az, const double *za, const double *freq, size_t n,
+                      double *X);
+
+/* MLP forward / gradient in FP64 (the gradient check of the GPU training
+ * kernels).  nl linear layers, dims[0..nl]; layer l has W_l [dims[l+1]][dims[l]]
+ * (row-major) and b_l [dims[l+1]], packed one after another in W and b;
+ * GELU (tanh form, 0.5 x (1 + tanh(0.7978845608 (x + 0.044715 x^3)))) after
+ * every layer but the last.  orc_mlp_forward: Y [B][dims[nl]].
+ * orc_mlp_grad: loss = mean over B x dims[nl] of (Y - T)^2; dW, db packed
+ * like W, b.  Returns the loss. */
+void orc_mlp_forward(const double *X, int B, int nl, const int *dims, const double *W,
+                     const double *b, double *Y);
+double orc_mlp_grad(const double *X, const double *T, int B, int nl, const int *dims,
+                    const double *W, const double *b, double *dW, double *db);
+
 /* Number of OpenMP threads the parallel loops use. */
 int orc_num_threads(void);
 

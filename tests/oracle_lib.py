@@ -181,3 +181,83 @@ def rel_err(x, ref):
     r = ref.astype(np.float64).ravel()
     den = np.sqrt(np.sum(r * r))
     return float(np.sqrt(np.sum(d * d)) / den) if den > 0 else float(np.sqrt(np.sum(d * d)))
+
+
+# ------------------------------------------------ primary beam + surrogate --
+MLP_DIMS = (16, 256, 256, 256, 8)
+
+
+def dipole_beam(az, za, freq, cfg):
+    """Analytic dipole-array beam; cfg = 64 gain doubles (re, im) + 32 delays.
+    Returns J [n][2][2] complex."""
+    az = np.ascontiguousarray(az, np.float64)
+    za = np.ascontiguousarray(za, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    cfg = np.ascontiguousarray(cfg, np.float64)
+    n = az.size
+    J = np.empty((n, 8), np.float64)
+    lib().orc_dipole_beam(_p(az), _p(za), _p(freq), C.c_size_t(n), _p(cfg[:64]), _p(cfg[64:]),
+                          _p(J))
+    return (J[:, 0::2] + 1j * J[:, 1::2]).reshape(n, 2, 2)
+
+
+def nominal_cfg():
+    cfg = np.zeros(96)
+    cfg[0:64:2] = 1.0
+    return cfg
+
+
+def mlp_features(az, za, freq):
+    az = np.ascontiguousarray(az, np.float64)
+    za = np.ascontiguousarray(za, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    X = np.empty((az.size, 16), np.float64)
+    lib().orc_mlp_features(_p(az), _p(za), _p(freq), C.c_size_t(az.size), _p(X))
+    return X
+
+
+def _mlp_pack(W, b):
+    return (np.ascontiguousarray(np.concatenate([w.ravel() for w in W]), np.float64),
+            np.ascontiguousarray(np.concatenate([x.ravel() for x in b]), np.float64))
+
+
+def _mlp_unpack(flatW, flatb, dims):
+    W, b, wo, bo = [], [], 0, 0
+    for i in range(len(dims) - 1):
+        n = dims[i] * dims[i + 1]
+        W.append(flatW[wo:wo + n].reshape(dims[i + 1], dims[i]))
+        b.append(flatb[bo:bo + dims[i + 1]])
+        wo += n
+        bo += dims[i + 1]
+    return W, b
+
+
+def mlp_forward(X, W, b):
+    dims = np.array([W[0].shape[1]] + [w.shape[0] for w in W], np.int32)
+    fw, fb = _mlp_pack(W, b)
+    X = np.ascontiguousarray(X, np.float64)
+    Y = np.empty((X.shape[0], dims[-1]), np.float64)
+    lib().orc_mlp_forward(_p(X), X.shape[0], len(W), _p(dims), _p(fw), _p(fb), _p(Y))
+    return Y
+
+
+def mlp_grad(X, T, W, b):
+    """Mean-squared loss and its gradients (lists like W, b), FP64."""
+    lib().orc_mlp_grad.restype = C.c_double
+    dims = np.array([W[0].shape[1]] + [w.shape[0] for w in W], np.int32)
+    fw, fb = _mlp_pack(W, b)
+    X = np.ascontiguousarray(X, np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    dW = np.empty_like(fw)
+    db = np.empty_like(fb)
+    loss = lib().orc_mlp_grad(_p(X), _p(T), X.shape[0], len(W), _p(dims), _p(fw), _p(fb),
+                              _p(dW), _p(db))
+    gW, gb = _mlp_unpack(dW, db, list(dims))
+    return loss, gW, gb
+
+
+def bf16_round(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
