@@ -191,6 +191,51 @@ def committed_traffic(kernel):
     return None
 
 
+def surrogate_stage(lib, nbeams=4096, nchan=128, steps=4000, batch=2048, nsamples=2_000_000):
+    """Train the BF16 MLP primary-beam surrogate on the GPU, then produce Jones
+    for nbeams directions (within 15 deg) x nchan fine channels with fallback
+    to the analytic beam; errors are against the analytic (FP64) beam."""
+    import ctypes as C
+    s = lib.vmSurrogateCreate(7, 0.01, 50e-12)
+    try:
+        curve = np.zeros(steps // 100 + 1, np.float32)
+        t0 = time.perf_counter()
+        loss = lib.vmSurrogateTrain(s, nsamples, steps, batch, 2e-3, 20.0, 11, curve.ctypes.data)
+        t_train = time.perf_counter() - t0
+        rng = np.random.default_rng(5)
+        za = np.arccos(1 - rng.uniform(0, 1, nbeams) * (1 - np.cos(np.radians(15.0))))
+        az = rng.uniform(0, 2 * np.pi, nbeams)
+        freq = 191.36e6 + 10e3 * np.arange(nchan)
+        B = np.empty((nbeams, nchan, 4, 2))
+        err = np.empty((nbeams, nchan), np.float32)
+        args = (az.ctypes.data, za.ctypes.data, nbeams, freq.ctypes.data, nchan, B.ctypes.data)
+        lib.vmSurrogateJones(s, 3, *args, err.ctypes.data)  # warm-up
+        t0 = time.perf_counter()
+        nfb = lib.vmSurrogateJones(s, 3, *args, err.ctypes.data)
+        t_sur = time.perf_counter() - t0
+        Bt = np.empty_like(B)
+        lib.vmSurrogateJones(s, 0, az.ctypes.data, za.ctypes.data, nbeams, freq.ctypes.data,
+                             nchan, Bt.ctypes.data, None)
+        Bn = np.empty_like(B)
+        lib.vmSurrogateJones(s, 1, az.ctypes.data, za.ctypes.data, nbeams, freq.ctypes.data,
+                             nchan, Bn.ctypes.data, None)
+        e_nom = np.sqrt(((Bn - Bt) ** 2).sum((2, 3)) / (Bt ** 2).sum((2, 3)))
+        Bc = np.repeat(Bt[:, nchan // 2:nchan // 2 + 1], nchan, axis=1)
+        e_cen = np.sqrt(((Bc - Bt) ** 2).sum((2, 3)) / (Bt ** 2).sum((2, 3)))
+        return {"train_samples": nsamples, "train_steps": steps, "batch": batch,
+                "train_s": round(t_train, 2), "final_loss": float(f"{loss:.3e}"),
+                "jones_evaluations": nbeams * nchan,
+                "infer_with_fallback_ms": round(t_sur * 1e3, 2),
+                "fallbacks": int(nfb),
+                "err_median": float(f"{np.median(err):.3e}"),
+                "err_p99": float(f"{np.quantile(err, 0.99):.3e}"),
+                "err_max": float(f"{err.max():.3e}"),
+                "nominal_beam_err_median": float(f"{np.median(e_nom):.3e}"),
+                "centre_frequency_beam_err_median": float(f"{np.median(e_cen):.3e}")}
+    finally:
+        lib.vmSurrogateDestroy(s)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +245,8 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the primary-beam surrogate stage (training + 4096-beam inference)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="fix the MFMA beamformer variant instead of autotuning")
     ap.add_argument("--order", type=int, default=1, help="tile order with --variant")
@@ -263,11 +310,24 @@ def main():
     lib.vmBeamformerVariantInfo(variant, C.byref(vr), C.byref(vs), C.byref(vw))
     value = beam_samples / (ms_step * 1e-3)
     S_mfma = sess.stokes()
-    flops = 4096.0 * beam_samples  # complex GEMM: 8 x 2 pols x 256 tile-pols per beam-sample
-    achieved = flops / (gemm_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP16_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP16_PEAK_TFLOPS, 4),
+    # Dominant kernel k_beamform_mfma, per launch (one channel-second, 512 beams):
+    # algorithmic bytes = Stokes IQUV out (4 B x 4 per beam-sample) + J^-1-applied
+    # voltages in (FP16 complex, 4 B per tile-pol-sample) + hi/lo phase weights
+    # (2 x 2 B per beam-row x k') + N terms in (16 B per channel-sample);
+    # algorithmic flops = 8 x 2 pols x tiles per beam-sample (phase GEMM after
+    # J^-1 is applied per antenna), issued = 2 x that (split weights).
+    ntile, nch, nsamp = obs.ntiles, 128, obs.nspectra
+    gemm_bytes = (beam_samples * 4 * 4 + nch * ntile * 2 * nsamp * 4
+                  + nch * (2 * args.beams) * (2 * ntile) * 2 * 2 + nch * nsamp * 16)
+    gemm_flops = 8.0 * 2 * ntile * beam_samples
+    achieved = gemm_bytes / (gemm_ms * 1e-3) / 1e9
+    mfma_tf = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": committed_traffic("k_beamform_mfma"),
+                "algorithmic_bytes": int(gemm_bytes), "algorithmic_flops": gemm_flops,
+                "mfma_tflops": round(mfma_tf, 1),
+                "mfma_frac_issued": round(2 * mfma_tf / FP16_PEAK_TFLOPS, 4),
                 "kernel": "k_beamform_mfma<fp16>", "kernel_ms": round(gemm_ms, 4),
                 "launches": gemm_n, "autotuned_variant": variant,
                 "workgroup_tile": f"{vr.value} rows x {vs.value} samples, {vw.value} waves",
@@ -298,6 +358,10 @@ def main():
         result["fp32_form"] = {"value": round(beam_samples / (ms32 * 1e-3), 1),
                                "ms_per_step": round(ms32, 3), "kernel_ms": round(k32, 3),
                                "mfma_speedup": round(ms32 / ms_step, 2)}
+    if not args.no_surrogate:
+        print("[bench] primary-beam surrogate: train + 4096 beams x 128 channels",
+              file=sys.stderr, flush=True)
+        result["surrogate"] = surrogate_stage(lib)
     if not args.no_cpu:
         print("[bench] CPU baseline (FP64 oracle)", file=sys.stderr, flush=True)
         cb = cpu_baseline(sess, obs, az, za)

@@ -564,7 +564,8 @@ static void orc_mlp_run(const double *X, int B, int nl, const int *dims, const d
 void orc_mlp_forward(const double *X, int B, int nl, const int *dims, const double *W,
                      const double *b, double *Y)
 {
-    double *z[16], *a[16];
+    double *z[16] = {0}, *a[16] = {0};
+    if (nl < 1 || nl > 16) return;
     for (int L = 0; L < nl; L++) {
         z[L] = (double *)malloc(sizeof(double) * B * dims[L + 1]);
         a[L] = (double *)malloc(sizeof(double) * B * dims[L + 1]);
