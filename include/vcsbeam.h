@@ -728,6 +728,44 @@ void vmSetBeamformerVariant(vcsbeam_context *vm, int variant, int order);
  * samples per workgroup, wavefronts per workgroup. Returns 0 if no such. */
 int vmBeamformerVariantInfo(int variant, int *rows, int *samples, int *waves);
 
+/* Batched beamforming of several coarse channels (no counterpart in the
+ * reference, which runs one process per coarse channel, app/
+ * make_mwa_tied_array_beam.c:250-311 once per rank).  Each context holds one
+ * coarse channel-second: forward PFB initialised (VM_FLT, not staged, one
+ * chunk per second), MFMA form (all the same), S, J^-1 (pointing 0 used),
+ * phases and pol lists on the device, and the second uploaded
+ * (vmUploadForwardPFBChunk).  Contexts whose forward PFB was initialised
+ * without PFB_MALLOC_DEVICE_OUTPUT share one PFB output buffer owned by the
+ * batch.  Channels may differ in beam count and flagged tiles: tiles whose
+ * J^-1 is zero in every fine channel (D = 0) are left out of that channel's
+ * GEMM.  vmBatchBeamformSecond runs, per channel, the fused PFB and the prep
+ * pass, then ONE persistent MFMA launch over all channels' work items; with
+ * use_graph the sequence is captured in a hipGraph on first use and replayed
+ * (re-captured when beams, flags or the schedule change).  Results are in
+ * each context's d_S on return.  Destroy the batch before the contexts. */
+typedef struct vm_batch vm_batch;
+vm_batch *vmBatchCreate(vcsbeam_context **vms, int n);
+void vmBatchBeamformSecond(vm_batch *b, int use_graph);
+/* Autotuner: times every usable GEMM variant with bands of 1, 2, 4, 8 or all
+ * row tiles per work item on the current second; keeps the fastest and
+ * returns its time (ms).  vmBatchSetSchedule fixes one (band 0: all). */
+float vmBatchTune(vm_batch *b, int reps);
+void vmBatchSetSchedule(vm_batch *b, int variant, int band);
+/* Event-timed eager run: summed PFB, prep and GEMM times (ms). */
+void vmBatchStageTimes(vm_batch *b, float *pfb_ms, float *prep_ms, float *gemm_ms);
+/* Number of channels; the schedule in use and its work-item count. */
+int vmBatchInfo(vm_batch *b, int *variant, int *band, int *nitems);
+/* Tiles of channel g that enter its GEMM (non-zero J^-1). */
+int vmBatchActiveTiles(vm_batch *b, int g);
+void vmBatchDestroy(vm_batch *b);
+
+/* Seeded MWAX 8+8-bit voltages generated in place on the device: the same
+ * counter-based generator (SplitMix64, Box-Muller, sigma, round, clip to
+ * +-127) and layout [nblocks][ninput][spb] (re, im) as the host generator of
+ * the synthetic workloads, for inputs too large to build on the host. */
+void vmSynthMWAXDevice(uint64_t seed, uint64_t first_block, int nblocks, int ninput, int spb,
+                       double sigma, int zero_blocks, void *d_out);
+
 /* Device-resident inverse PFB of pointing 0 straight from d_e (no host
  * round trip); output complex float [n][pol] on the device, copied to
  * data_buffer_vdif when it is not NULL. */

@@ -302,6 +302,18 @@ _SIGS = {
     "vmMemsetDevice": (None, [C.c_void_p, C.c_int, C.c_size_t]),
     "vmDeviceSynchronize": (None, []),
     "vmDeviceCopy": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "vmSynthMWAXDevice": (None, [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double,
+                                 C.c_int, C.c_void_p]),
+    "vmBatchCreate": (C.c_void_p, [C.POINTER(_CTX), C.c_int]),
+    "vmBatchBeamformSecond": (None, [C.c_void_p, C.c_int]),
+    "vmBatchTune": (C.c_float, [C.c_void_p, C.c_int]),
+    "vmBatchSetSchedule": (None, [C.c_void_p, C.c_int, C.c_int]),
+    "vmBatchStageTimes": (None, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float)]),
+    "vmBatchInfo": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                              C.POINTER(C.c_int)]),
+    "vmBatchActiveTiles": (C.c_int, [C.c_void_p, C.c_int]),
+    "vmBatchDestroy": (None, [C.c_void_p]),
     "vmDeviceCount": (C.c_int, []),
     "vmEventCreate": (C.c_void_p, []),
     "vmEventRecord": (None, [C.c_void_p, C.c_void_p]),

@@ -207,7 +207,10 @@ class TiedArraySession:
     """
 
     def __init__(self, obs, az, za, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, form=VM_BF_FP32,
-                 want_e=False):
+                 want_e=False, host_buffers=True):
+        """host_buffers=False leaves out the host S and J buffers and frees the
+        pinned read buffer (for seconds generated on the device and outputs
+        read back in slices)."""
         lib = self.lib = load()
         self.obs = obs
         az = np.atleast_1d(np.asarray(az, dtype=np.float64))
@@ -226,9 +229,12 @@ class TiedArraySession:
         lib.vmSetNumPointings(vm, self.npointing)
         self.bg = _beam_geoms(az, za)
         lib.vmSetMaxGPUMem(vm, 1)
-        names = ["vmMallocVDevice", "vmMallocSHost", "vmMallocSDevice", "vmMallocJHost",
-                 "vmMallocJDevice", "vmMallocDHost", "vmMallocDDevice", "vmMallocPQIdxsHost",
-                 "vmMallocPQIdxsDevice"]
+        names = ["vmMallocVDevice", "vmMallocSDevice", "vmMallocJDevice", "vmMallocDHost",
+                 "vmMallocDDevice", "vmMallocPQIdxsHost", "vmMallocPQIdxsDevice"]
+        if host_buffers:
+            names += ["vmMallocSHost", "vmMallocJHost"]
+        else:
+            lib.vmFreeVHost(vm)
         if want_e:
             names += ["vmMallocEHost", "vmMallocEDevice"]
         for f in names:
@@ -278,13 +284,100 @@ class TiedArraySession:
         self.lib.vmMemcpyDtoH(out.ctypes.data, v.d_v, out.nbytes)
         return out.reshape(v.nfine_chan, self.obs.ninput, v.fine_sample_rate)
 
+    def stokes_slice(self, p0, p1, s0, s1):
+        """Device S [p][st][c][s] for pointings p0:p1 and samples s0:s1, read
+        without a full host copy; returned as [p][s][st][c] like stokes()."""
+        v = self.v
+        nst, nch, ns = v.out_nstokes, v.nfine_chan, v.fine_sample_rate
+        out = np.empty((p1 - p0, nst, nch, s1 - s0), np.float32)
+        base = C.cast(v.d_S, C.c_void_p).value
+        for p in range(p0, p1):
+            for st in range(nst):
+                for c in range(nch):
+                    off = ((p * nst + st) * nch + c) * ns + s0
+                    self.lib.vmMemcpyDtoH(out[p - p0, st, c].ctypes.data, base + 4 * off,
+                                          4 * (s1 - s0))
+        return np.ascontiguousarray(out.transpose(0, 3, 1, 2))
+
     def close(self):
         lib, vm = self.lib, self.vm
-        for f in ("vmFreeVDevice", "vmFreeSHost", "vmFreeSDevice", "vmFreeJDevice", "vmFreeJHost",
+        for f in ("vmFreeVDevice", "vmFreeSDevice", "vmFreeJDevice",
                   "vmFreeDHost", "vmFreeDDevice", "vmFreePQIdxsDevice", "vmFreePQIdxsHost"):
             getattr(lib, f)(vm)
+        if self.v.S:
+            lib.vmFreeSHost(vm)
+        if self.v.J:
+            lib.vmFreeJHost(vm)
         if self.v.e:
             lib.vmFreeEHost(vm)
             lib.vmFreeEDevice(vm)
         lib.destroy_vcsbeam_context(vm)
         self.vm = None
+
+
+class BatchSession:
+    """Several coarse channels, each its own context, beamformed together by
+    the batched persistent MFMA launch (vmBatch*).
+
+    obs_list[g] and dirs[g] = (az, za) describe channel g; channels may
+    differ in beam count and flags.  With share_pfb_output the contexts'
+    forward PFBs are initialised without PFB_MALLOC_DEVICE_OUTPUT and share
+    one output buffer owned by the batch.  With device_data each second is
+    generated in place on the device (vmSynthMWAXDevice, the same generator
+    as Observation.second) instead of read from the host.
+    """
+
+    def __init__(self, obs_list, dirs, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, form=1,
+                 share_pfb_output=True, device_data=False):
+        self.lib = lib = load()
+        flags = pfb_flags & ~0x08 if share_pfb_output else pfb_flags
+        self.device_data = device_data
+        self.sessions = [TiedArraySession(o, az, za, pfb_flags=flags, nstokes=nstokes, form=form,
+                                          host_buffers=not device_data)
+                         for o, (az, za) in zip(obs_list, dirs)]
+        for ses in self.sessions:
+            ses.geometry()
+        arr = (type(self.sessions[0].vm) * len(self.sessions))(*[x.vm for x in self.sessions])
+        self.b = lib.vmBatchCreate(arr, len(self.sessions))
+
+    def read(self, t):
+        from .workloads import SEED_DATA, SPB
+        for ses in self.sessions:
+            if not self.device_data:
+                ses.read(t)
+                continue
+            o, fp = ses.obs, ses.v.fpfb.contents
+            base = C.cast(fp.d_htr_data, C.c_void_p).value
+            nb = o.blocks_per_second
+            block = o.ninput * SPB * 2
+            # margin block = last block of the previous second (zeros at t = 0)
+            self.lib.vmSynthMWAXDevice(SEED_DATA + o.seed_offset, max(t * nb - 1, 0), 1, o.ninput,
+                                       SPB, 16.0, 1 if t == 0 else 0, base)
+            self.lib.vmSynthMWAXDevice(SEED_DATA + o.seed_offset, t * nb, nb, o.ninput, SPB,
+                                       16.0, 0, base + block)
+
+    def run(self, use_graph=True):
+        self.lib.vmBatchBeamformSecond(self.b, 1 if use_graph else 0)
+
+    def tune(self, reps=3):
+        return self.lib.vmBatchTune(self.b, reps)
+
+    def info(self):
+        v, band, n = C.c_int(), C.c_int(), C.c_int()
+        self.lib.vmBatchInfo(self.b, C.byref(v), C.byref(band), C.byref(n))
+        return {"variant": v.value, "band": band.value, "items": n.value}
+
+    def stage_times(self):
+        t = [C.c_float() for _ in range(3)]
+        self.lib.vmBatchStageTimes(self.b, *[C.byref(x) for x in t])
+        return {"pfb_ms": t[0].value, "prep_ms": t[1].value, "gemm_ms": t[2].value}
+
+    def active_tiles(self, g):
+        return self.lib.vmBatchActiveTiles(self.b, g)
+
+    def close(self):
+        if self.b:
+            self.lib.vmBatchDestroy(self.b)
+            self.b = None
+        for ses in self.sessions:
+            ses.close()

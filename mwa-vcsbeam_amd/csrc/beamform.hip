@@ -389,9 +389,9 @@ void cu_form_incoh_beam(uint8_t *data, uint8_t *d_data, size_t data_size, float 
 void vm_build_weights(vcsbeam_context *vm)
 {
     vm_device_state *d = vm_dev(vm);
-    if (!d->weights_dirty && d->d_W) return;
     int nant = (int)vm->obs_metadata->num_ants, nchan = vm->nfine_chan;
     size_t need = (size_t)vm->npointing * nchan * nant * 4 * sizeof(cf);
+    if (d->d_W && d->W_gen == d->weights_gen && d->W_bytes >= need) return;
     if (d->W_bytes < need) {
         if (d->d_W) HIPCHECK(hipFree(d->d_W));
         HIPCHECK(hipMalloc(&d->d_W, need));
@@ -402,7 +402,7 @@ void vm_build_weights(vcsbeam_context *vm)
                        vm_stream(vm), (const double2 *)vm->d_J, (const double2 *)vm->gdelays.d_phi,
                        (int)vm->npointing, nant, nchan, (cf *)d->d_W);
     HIPCHECK(hipGetLastError());
-    d->weights_dirty = 0;
+    d->W_gen = d->weights_gen;
 }
 
 void vmApplyJChunk(vcsbeam_context *vm)

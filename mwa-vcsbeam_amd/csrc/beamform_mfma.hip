@@ -110,11 +110,14 @@ __device__ __forceinline__ int scale_exp(float mx)
 // 0..7, i.e. tiles a = 8 ks + 4 (l/32) + j/2 with j even the coefficient of
 // u.re and j odd of u.im.  Rows: 2 beam + {re, im}: re rows (Re phi, -Im phi),
 // im rows (Im phi, Re phi); rows of beams >= npointing are zero.  Weights are
-// scaled by 2^PHASE_EXP.  phi: [p][a][c] complex double.
+// scaled by 2^PHASE_EXP.  phi: [p][a][c] complex double over nant_src
+// antennas; GEMM tile a is antenna amap[a] (amap NULL: a itself), nant tiles.
 template <bool BF16>
 __global__ __launch_bounds__(256) void k_phase_weights(const double2 *__restrict__ phi,
                                                        int npointing, int nant, int nchan,
-                                                       int nrb, int nkstep, uint4 *__restrict__ Af)
+                                                       int nrb, int nkstep,
+                                                       const int *__restrict__ amap, int nant_src,
+                                                       uint4 *__restrict__ Af)
 {
     typedef mtype<BF16> M;
     const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,7 +137,8 @@ __global__ __launch_bounds__(256) void k_phase_weights(const double2 *__restrict
         uint16_t hb[2] = {0, 0}, lb[2] = {0, 0};
         const int a = 8 * ks + 4 * (lane >> 5) + q;
         if (p < npointing && a < nant) {
-            const double2 f = phi[((size_t)p * nant + a) * nchan + c];
+            const int as = amap ? amap[a] : a;
+            const double2 f = phi[((size_t)p * nant_src + as) * nchan + c];
             const double vv[2] = {(im ? f.y : f.x) * ws, (im ? f.x : -f.y) * ws};
 #pragma unroll
             for (int e2 = 0; e2 < 2; e2++) {
@@ -156,15 +160,18 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v)
     if (j < n) p[j] = v;
 }
 
-// J^-1 of pointing 0 as float [c][a][4]
-__global__ void k_jf(const double2 *__restrict__ J, int nant, int nchan, cf *__restrict__ Jf)
+// J^-1 of pointing 0 as float [c][a][4], a = 0..nant-1 standing for
+// antenna amap[a] (amap NULL: a itself)
+__global__ void k_jf(const double2 *__restrict__ J, int nant, int nchan,
+                     const int *__restrict__ amap, cf *__restrict__ Jf)
 {
     size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= (size_t)nant * nchan) return;
     int c = (int)(j % nchan), a = (int)(j / nchan);
+    const int as = amap ? amap[a] : a;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        double2 m = J[((size_t)a * nchan + c) * 4 + q];
+        double2 m = J[((size_t)as * nchan + c) * 4 + q];
         Jf[((size_t)c * nant + a) * 4 + q] = cf{(float)m.x, (float)m.y};
     }
 }
@@ -362,44 +369,38 @@ __device__ __forceinline__ void stokes4(float xr, float xi, float yr, float yi, 
     st[3] = -2.0f * invw * bxyi;
 }
 
-// 1-D grid over (channel, tile); a workgroup of WR x WC wavefronts covers
-// (WR MB 32) rows (16 WR MB beams) x (WC NB/2 32) samples of one channel;
-// a wavefront's NB column blocks are NB/2 sample blocks x 2 polarisations
-// (block n = sample block n/2, polarisation n%2).  NST register stages keep
-// NST-1 k-steps of fragments in flight while one is multiplied.
-// Tile order: order 0 walks the channels in turn, row tiles fastest.  Orders
-// 1-4 are XCD-aware: the 8 workgroups dispatched together (one per XCD) take
-// the same tile of 8 different channels, so each XCD walks its own channels;
-// within a channel tiles go in bands of gr row tiles, column by column (gr =
-// all row tiles for order 1, 1 for order 2, 2 for order 3, 4 for order 4).
+// Operands of one GEMM problem: a whole chunk in the single-channel call, or
+// one coarse channel of a batch.  Af, Bf, sexp and N are indexed by the fine
+// channel c as laid out by k_phase_weights / k_bf_prep; S is [p][st][c][s_total]
+// (device layout) written from sample soffset on.
+struct bf_geom {
+    const uint4 *Af;
+    const uint4 *Bf;
+    const int *sexp;
+    const float4 *N;
+    float *S;
+    int nkstep, nrb, npointing, nstokes;
+    int soffset, ns_total;
+    float invw;
+    int pad;
+};
+
+// One workgroup tile (tx, ty) of fine channel c: WR x WC wavefronts cover
+// (WR MB 32) rows (16 WR MB beams) x (WC NB/2 32) samples; a wavefront's NB
+// column blocks are NB/2 sample blocks x 2 polarisations (block n = sample
+// block n/2, polarisation n%2).  NST register stages keep NST-1 k-steps of
+// fragments in flight while one is multiplied.
 // Epilogue: each (32-row block, sample block) pair is passed through a 4 KB
-// per-wavefront LDS tile, 8 beams at a time, so that every lane stores four
-// consecutive samples of one (beam, Stokes) row with one 16-byte store.
-// S: [p][st][c][s_total] (device layout), sample offset soffset.
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC>
-__global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(
-    const uint4 *__restrict__ Bf, const uint4 *__restrict__ Af, const int *__restrict__ sexp,
-    const float4 *__restrict__ N, int ns, int nsb, int nkstep, int nrb, int npointing, int nchan,
-    int ntx, int nty, int order, float invw, int nstokes, int soffset, int ns_total,
-    float *__restrict__ S)
+// per-wavefront LDS tile (stg + 1024 w), 8 beams at a time, so that every
+// lane stores four consecutive samples of one (beam, Stokes) row with one
+// 16-byte store.
+template <bool BF16, int MB, int NB, int WR, int WC, int NST>
+__device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int nchan, int c,
+                                        int tx, int ty, float *stg)
 {
     static_assert(NB % 2 == 0, "column blocks come in polarisation pairs");
     constexpr int NSB = NB / 2; // sample blocks per wavefront
-    const int ntile = ntx * nty;
-    const int L = blockIdx.x;
-    int c, tile;
-    if (order != 0 && (nchan & 7) == 0) {
-        const int g = L >> 3, x = L & 7;
-        c = (g / ntile) * 8 + x;
-        tile = g % ntile;
-    } else {
-        c = L / ntile;
-        tile = L % ntile;
-    }
-    int gr = order == 2 ? 1 : order == 3 ? 2 : order == 4 ? 4 : nty;
-    if (nty % gr) gr = 1;
-    const int band = tile / (gr * ntx), rem = tile % (gr * ntx);
-    const int tx = rem / gr, ty = band * gr + rem % gr;
+    const int nkstep = G.nkstep, nrb = G.nrb;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wr = w / WC, wc = w % WC;
@@ -408,8 +409,8 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(
 
     const int astr = nkstep * 2048;              // bytes per 32-row block of A
     const int bstr = nkstep * 2048;              // bytes per sample block of B (2 pols)
-    const rsrc_t rA = make_rsrc(Af + (size_t)c * nrb * nkstep * 128, (uint32_t)nrb * astr);
-    const rsrc_t rB = make_rsrc(Bf + (size_t)c * nsb * nkstep * 128, (uint32_t)nsb * bstr);
+    const rsrc_t rA = make_rsrc(G.Af + (size_t)c * nrb * nkstep * 128, (uint32_t)nrb * astr);
+    const rsrc_t rB = make_rsrc(G.Bf + (size_t)c * nsb * nkstep * 128, (uint32_t)nsb * bstr);
     const int vl = lane * 16;
     const int aoff = rb0 * astr;
     int boff[NB];
@@ -485,22 +486,23 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(
     // (jg = 2 h + jj) is 8 beams; LDS rows r = st * 8 + (4 jj + 2 b + lk) for
     // beam 4 jg + 2 lk + b: the two lane halves write rows 32 floats apart and
     // each 16-lane read group covers two whole rows.
-    __shared__ __attribute__((aligned(16))) float stg[WR * WC][32 * 32];
-    float *wl = stg[w];
+    float *wl = stg + 1024 * w;
     const int lr = lane & 31, lk = lane >> 5, qq = lane & 7;
-    const size_t pstride = (size_t)nstokes * nchan * ns_total;
-    const size_t ststride = (size_t)nchan * ns_total;
-    const bool vec = ((ns_total | soffset) & 3) == 0;
+    const int nstokes = G.nstokes, npointing = G.npointing;
+    const float invw = G.invw;
+    const size_t pstride = (size_t)nstokes * nchan * G.ns_total;
+    const size_t ststride = (size_t)nchan * G.ns_total;
+    const bool vec = ((G.ns_total | G.soffset) & 3) == 0;
 #pragma unroll
     for (int j = 0; j < NSB; j++) {
         const int sb = sb0 + j;
         const int s = SB * sb + lr;
         float4 nn = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (s < ns) nn = N[(size_t)c * ns + s];
+        if (s < ns) nn = G.N[(size_t)c * ns + s];
         const float unscale2 =
-            ldexpf(1.0f, -2 * (sexp[(size_t)c * nsb + min(sb, nsb - 1)] + PHASE_EXP));
+            ldexpf(1.0f, -2 * (G.sexp[(size_t)c * nsb + min(sb, nsb - 1)] + PHASE_EXP));
         const int s4 = SB * sb + 4 * qq;
-        float *Sc = S + (size_t)c * ns_total + soffset + s4;
+        float *Sc = G.S + (size_t)c * G.ns_total + G.soffset + s4;
 #pragma unroll
         for (int m = 0; m < MB; m++)
 #pragma unroll
@@ -542,6 +544,70 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(
     }
 }
 
+// 1-D grid over (channel, tile) of one GEMM problem.
+// Tile order: order 0 walks the channels in turn, row tiles fastest.  Orders
+// 1-4 are XCD-aware: the 8 workgroups dispatched together (one per XCD) take
+// the same tile of 8 different channels, so each XCD walks its own channels;
+// within a channel tiles go in bands of gr row tiles, column by column (gr =
+// all row tiles for order 1, 1 for order 2, 2 for order 3, 4 for order 4).
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC>
+__global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, int ns, int nsb,
+                                                                     int nchan, int ntx, int nty,
+                                                                     int order)
+{
+    __shared__ __attribute__((aligned(16))) float stg[WR * WC * 1024];
+    const int ntile = ntx * nty;
+    const int L = blockIdx.x;
+    int c, tile;
+    if (order != 0 && (nchan & 7) == 0) {
+        const int g = L >> 3, x = L & 7;
+        c = (g / ntile) * 8 + x;
+        tile = g % ntile;
+    } else {
+        c = L / ntile;
+        tile = L % ntile;
+    }
+    int gr = order == 2 ? 1 : order == 3 ? 2 : order == 4 ? 4 : nty;
+    if (nty % gr) gr = 1;
+    const int band = tile / (gr * ntx), rem = tile % (gr * ntx);
+    const int tx = rem / gr, ty = band * gr + rem % gr;
+    bf_tile<BF16, MB, NB, WR, WC, NST>(G, ns, nsb, nchan, c, tx, ty, stg);
+}
+
+// Batched form: one persistent launch over the GEMM problems of several
+// coarse channels (geoms[g]), all with ns samples in nsb blocks and nchan
+// fine channels.  Work items are (g, c, tx, ty0, cnt): cnt row tiles from
+// ty0 of column tile tx of fine channel c, encoded as x = g << 16 | c,
+// y = tx << 16 | ty0 << 8 | cnt and listed in the order they should start
+// (most expensive first).  Workgroups take items from a queue counter that
+// must be zero at launch; every workgroup leaves the loop when the counter
+// passes nitems, so the grid always drains.
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC>
+__global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
+    const bf_geom *__restrict__ geoms, const uint2 *__restrict__ items, int nitems,
+    uint32_t *__restrict__ queue, int ns, int nsb, int nchan)
+{
+    __shared__ __attribute__((aligned(16))) float stg[WR * WC * 1024];
+    __shared__ int slot[2];
+    for (int it = 0;; it++) {
+        // two slots: a slot is rewritten only after the barrier of the next
+        // item, which every wavefront passes after reading it
+        if (threadIdx.x == 0) slot[it & 1] = (int)atomicAdd(queue, 1u);
+        __syncthreads();
+        const int idx = __builtin_amdgcn_readfirstlane(slot[it & 1]);
+        if (idx >= nitems) break;
+        const uint2 e = items[idx];
+        const int g = __builtin_amdgcn_readfirstlane((int)(e.x >> 16));
+        const int c = __builtin_amdgcn_readfirstlane((int)(e.x & 0xffffu));
+        const int tx = __builtin_amdgcn_readfirstlane((int)(e.y >> 16));
+        const int ty0 = __builtin_amdgcn_readfirstlane((int)((e.y >> 8) & 0xffu));
+        const int cnt = __builtin_amdgcn_readfirstlane((int)(e.y & 0xffu));
+        const bf_geom G = geoms[g];
+        for (int ty = ty0; ty < ty0 + cnt; ty++)
+            bf_tile<BF16, MB, NB, WR, WC, NST>(G, ns, nsb, nchan, c, tx, ty, stg);
+    }
+}
+
 // Kernel variants the autotuner chooses from: register tile per wavefront
 // (MB row blocks of 32 x NB column blocks of 32, NB = 2 x sample blocks),
 // wavefront grid per workgroup (WR x WC), register stages and the minimum
@@ -555,15 +621,12 @@ static const bf_variant kVariants[] = {
 #define VM_N_ORDERS 5
 
 template <bool BF16>
-static void launch_variant(int vi, dim3 grid, hipStream_t st, const uint4 *Bf, const uint4 *Af,
-                           const int *sexp, const float4 *N, int ns, int nsb, int nkstep, int nrb,
-                           int np, int nchan, int ntx, int nty, int order, float invw, int nstokes,
-                           int soffset, int ns_total, float *S)
+static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, int ns, int nsb,
+                           int nchan, int ntx, int nty, int order)
 {
 #define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, OCC_)                                             \
     hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, OCC_>), grid,        \
-                       dim3(64 * WR_ * WC_), 0, st, Bf, Af, sexp, N, ns, nsb, nkstep, nrb, np, \
-                       nchan, ntx, nty, order, invw, nstokes, soffset, ns_total, S)
+                       dim3(64 * WR_ * WC_), 0, st, G, ns, nsb, nchan, ntx, nty, order)
     switch (vi) {
     case 0: VM_LAUNCH(4, 4, 2, 2, 3, 1); break;
     case 1: VM_LAUNCH(4, 4, 1, 4, 3, 1); break;
@@ -574,6 +637,42 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const uint4 *Bf, c
     default: VM_LAUNCH(4, 4, 4, 1, 3, 1); break;
     }
 #undef VM_LAUNCH
+}
+
+template <bool BF16>
+static void launch_batch_variant(int vi, unsigned nwg, hipStream_t st, const bf_geom *geoms,
+                                 const uint2 *items, int nitems, uint32_t *queue, int ns, int nsb,
+                                 int nchan)
+{
+#define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, OCC_)                                             \
+    hipLaunchKernelGGL((k_beamform_batch<BF16, MB_, NB_, WR_, WC_, NST_, OCC_>), dim3(nwg),  \
+                       dim3(64 * WR_ * WC_), 0, st, geoms, items, nitems, queue, ns, nsb, nchan)
+    switch (vi) {
+    case 0: VM_LAUNCH(4, 4, 2, 2, 3, 1); break;
+    case 1: VM_LAUNCH(4, 4, 1, 4, 3, 1); break;
+    case 2: VM_LAUNCH(4, 4, 2, 2, 4, 1); break;
+    case 3: VM_LAUNCH(2, 4, 2, 2, 3, 2); break;
+    case 4: VM_LAUNCH(4, 2, 2, 2, 3, 2); break;
+    case 5: VM_LAUNCH(2, 4, 4, 1, 3, 2); break;
+    default: VM_LAUNCH(4, 4, 4, 1, 3, 1); break;
+    }
+#undef VM_LAUNCH
+}
+
+// live[a] = 1 when J^-1 of antenna a (pointing 0, [a][c][4]) is non-zero in
+// some channel: the antennas whose terms can be non-zero.  One wavefront per
+// antenna.
+__global__ void k_ant_live(const double2 *__restrict__ J, int nant, int nchan,
+                           uint32_t *__restrict__ live)
+{
+    const int a = blockIdx.x, t = threadIdx.x;
+    bool nz = false;
+    for (int k = t; k < nchan * 4; k += 64) {
+        const double2 m = J[(size_t)a * nchan * 4 + k];
+        nz |= (m.x != 0.0) || (m.y != 0.0);
+    }
+    const bool any = __any(nz);
+    if (t == 0) live[a] = any ? 1u : 0u;
 }
 
 // =================================================================== host ==
@@ -588,6 +687,28 @@ static void ensure_buf(void **p, size_t *cap, size_t need)
 }
 
 #define ROW_PAD 256 /* GEMM rows are padded to a multiple of the usual workgroup row tile */
+
+// k_bf_prep over (nsb sample blocks) x (nchan channels) for nant <= 256
+// antennas (the template's antennas-per-thread bound is chosen from nant)
+static void launch_prep(bool bf16, hipStream_t st, const cf *v, int ns, int ni, int nsb,
+                        int nkstep, int nchan, const cf *Jf, const uint32_t *polQ,
+                        const uint32_t *polP, int nant, uint4 *Bf, int *sexp, float4 *N)
+{
+    if (nant > 256) VM_FATAL("MFMA beamformer: at most 256 antennas (%d)", nant);
+    const dim3 pg((unsigned)nsb, (unsigned)nchan);
+#define VM_PREP(B_, A_)                                                                       \
+    hipLaunchKernelGGL((k_bf_prep<B_, A_>), pg, dim3(256), 0, st, v, ns, ni, nsb, nkstep, Jf,  \
+                       polQ, polP, nant, Bf, sexp, N)
+    if (nant <= 64) {
+        if (bf16) VM_PREP(true, 8); else VM_PREP(false, 8);
+    } else if (nant <= 128) {
+        if (bf16) VM_PREP(true, 16); else VM_PREP(false, 16);
+    } else {
+        if (bf16) VM_PREP(true, 32); else VM_PREP(false, 32);
+    }
+#undef VM_PREP
+    HIPCHECK(hipGetLastError());
+}
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -624,9 +745,10 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     const size_t se_bytes = align256((size_t)nchan * nsb * sizeof(int));
     const size_t jf_bytes = align256((size_t)nchan * nant * 4 * sizeof(cf));
     const size_t need = af_bytes + bf_bytes + n_bytes + se_bytes + jf_bytes + 256;
+    bool rebuild = d->Wm_gen != d->weights_gen || d->Wm_form != vm->beamform_form;
     if (d->Wm_bytes < need) {
         ensure_buf(&d->d_Wm, &d->Wm_bytes, need);
-        d->weights_dirty = 1;
+        rebuild = true;
     }
     uint8_t *q = (uint8_t *)d->d_Wm;
     uint4 *Af = (uint4 *)q;                 q += af_bytes;
@@ -636,18 +758,20 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     cf *Jf = (cf *)q;                       q += jf_bytes;
     uint32_t *jflag = (uint32_t *)q;
 
-    if (d->weights_dirty) {
+    if (rebuild) {
         vm_timer_start(d, VM_T_BF_WEIGHTS);
         hipLaunchKernelGGL(k_jf, dim3((unsigned)((nant * nchan + 255) / 256)), dim3(256), 0, st,
-                           (const double2 *)vm->d_J, nant, nchan, Jf);
+                           (const double2 *)vm->d_J, nant, nchan, (const int *)NULL, Jf);
         const size_t total = (size_t)nchan * nrb * nkstep * 64;
         const dim3 g((unsigned)((total + 255) / 256));
         if (bf16)
             hipLaunchKernelGGL(k_phase_weights<true>, g, dim3(256), 0, st,
-                               (const double2 *)vm->gdelays.d_phi, np, nant, nchan, nrb, nkstep, Af);
+                               (const double2 *)vm->gdelays.d_phi, np, nant, nchan, nrb, nkstep,
+                               (const int *)NULL, nant, Af);
         else
             hipLaunchKernelGGL(k_phase_weights<false>, g, dim3(256), 0, st,
-                               (const double2 *)vm->gdelays.d_phi, np, nant, nchan, nrb, nkstep, Af);
+                               (const double2 *)vm->gdelays.d_phi, np, nant, nchan, nrb, nkstep,
+                               (const int *)NULL, nant, Af);
         if (np > 1) {
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, jflag, (size_t)1, 0u);
             const size_t per = (size_t)nant * nchan * 4, tot = per * np;
@@ -656,7 +780,8 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
         }
         HIPCHECK(hipGetLastError());
         vm_timer_stop(d, VM_T_BF_WEIGHTS);
-        d->weights_dirty = 0;
+        d->Wm_gen = d->weights_gen;
+        d->Wm_form = vm->beamform_form;
         if (np > 1) {
             uint32_t hflag = 0;
             HIPCHECK(hipMemcpyAsync(&hflag, jflag, 4, hipMemcpyDeviceToHost, st));
@@ -668,20 +793,8 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     }
     // u = J^-1 v, auto-correlation terms, block scales and the B operand
     vm_timer_start(d, VM_T_BF_PREP);
-    const dim3 pg((unsigned)nsb, (unsigned)nchan);
-#define VM_PREP(B_, A_)                                                                       \
-    hipLaunchKernelGGL((k_bf_prep<B_, A_>), pg, dim3(256), 0, st, (const cf *)vm->d_v, ns, ni, \
-                       nsb, nkstep, (const cf *)Jf, vm->d_polQ_idxs, vm->d_polP_idxs, nant, Bf,  \
-                       sexp, N)
-    if (nant <= 64) {
-        if (bf16) VM_PREP(true, 8); else VM_PREP(false, 8);
-    } else if (nant <= 128) {
-        if (bf16) VM_PREP(true, 16); else VM_PREP(false, 16);
-    } else {
-        if (bf16) VM_PREP(true, 32); else VM_PREP(false, 32);
-    }
-#undef VM_PREP
-    HIPCHECK(hipGetLastError());
+    launch_prep(bf16, st, (const cf *)vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, vm->d_polQ_idxs,
+                vm->d_polP_idxs, nant, Bf, sexp, N);
     vm_timer_stop(d, VM_T_BF_PREP);
     pl->nant = nant; pl->ni = ni; pl->nchan = nchan; pl->np = np; pl->nkstep = nkstep;
     pl->nrb = nrb; pl->ns = ns; pl->nsb = nsb;
@@ -703,16 +816,15 @@ static void mfma_run(vcsbeam_context *vm, const mfma_plan *pl, int vi, int order
     const int ncol = (V.nb / 2) * V.wc, nrow = V.mb * V.wr; // sample / row blocks per workgroup
     const int ntx = (pl->nsb + ncol - 1) / ncol, nty = pl->nrb / nrow;
     const dim3 grid((unsigned)((size_t)pl->nchan * ntx * nty));
+    bf_geom G;
+    G.Af = pl->Af; G.Bf = pl->Bf; G.sexp = pl->sexp; G.N = pl->N; G.S = (float *)vm->d_S;
+    G.nkstep = pl->nkstep; G.nrb = pl->nrb; G.npointing = pl->np; G.nstokes = vm->out_nstokes;
+    G.soffset = pl->chunk * pl->ns; G.ns_total = vm->fine_sample_rate; G.invw = pl->invw;
+    G.pad = 0;
     if (vm->beamform_form == VM_BF_MFMA_BF16)
-        launch_variant<true>(vi, grid, st, pl->Bf, pl->Af, pl->sexp, pl->N, pl->ns, pl->nsb,
-                             pl->nkstep, pl->nrb, pl->np, pl->nchan, ntx, nty, order, pl->invw,
-                             vm->out_nstokes, pl->chunk * pl->ns, vm->fine_sample_rate,
-                             (float *)vm->d_S);
+        launch_variant<true>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
     else
-        launch_variant<false>(vi, grid, st, pl->Bf, pl->Af, pl->sexp, pl->N, pl->ns, pl->nsb,
-                              pl->nkstep, pl->nrb, pl->np, pl->nchan, ntx, nty, order, pl->invw,
-                              vm->out_nstokes, pl->chunk * pl->ns, vm->fine_sample_rate,
-                              (float *)vm->d_S);
+        launch_variant<false>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
     HIPCHECK(hipGetLastError());
 }
 
@@ -792,6 +904,476 @@ void vmBeamformChunkMFMA(vcsbeam_context *vm)
     mfma_run(vm, &pl, vi, order);
     vm_timer_stop(d, VM_T_BF_MFMA);
     HIPCHECK(hipStreamSynchronize((hipStream_t)d->stream));
+}
+
+// ------------------------------------------------------------- batches ---
+// Several coarse channels (one context each) beamformed together: per
+// channel the fused forward PFB and the prep pass, then ONE persistent GEMM
+// launch over every channel's tiles, the whole sequence captured once in a
+// hipGraph and replayed each second.  Channels may differ in beam count and
+// in flagged tiles: antennas whose J^-1 is zero in every fine channel (flagged
+// tiles, D = 0) are dropped from the channel's GEMM (its K), and each channel
+// has its own row count, so the work items differ in cost; they are listed
+// most expensive first and handed out from a queue counter.
+
+void vm_pfb_fused_launch(const void *d_htr, size_t htr_bytes, const int *d_h,
+                         const int *d_out_idx, int I, int nspc, int P, int fpga, int imag_first,
+                         int split, void *d_out, hipStream_t stream);
+
+struct vm_batch_chan {
+    vcsbeam_context *vm;
+    int valid;
+    unsigned gen;                 // context weights generation the weights were built from
+    int nant_src, nact, nkstep, nrb, np;
+    float invw;
+    void *mem;
+    size_t mem_bytes;
+    int *d_amap;
+    uint32_t *d_polQ, *d_polP;
+    cf *Jf;
+    uint4 *Af, *Bf;
+    float4 *N;
+    int *sexp;
+};
+
+struct vm_batch {
+    int n, nchan, ns, nsb, bf16, ncu;
+    vm_batch_chan *ch;
+    void *d_v;                    // forward-PFB output shared by contexts without their own
+    bf_geom *d_geoms;
+    uint2 *d_items;
+    size_t items_cap;
+    int nitems, variant, band;
+    uint32_t *d_small;            // [0] queue counter, [1] J check flag, [64..319] live antennas
+    hipStream_t st;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    int layout_changed;
+};
+
+static bool batch_variant_ok(int vi)
+{
+    return vi >= 0 && vi < N_VARIANTS && ROW_PAD % (RBK * kVariants[vi].mb * kVariants[vi].wr) == 0;
+}
+
+static void batch_drop_graph(vm_batch *b)
+{
+    if (b->exec) HIPCHECK(hipGraphExecDestroy(b->exec));
+    if (b->graph) HIPCHECK(hipGraphDestroy(b->graph));
+    b->exec = NULL;
+    b->graph = NULL;
+}
+
+// Work items for variant vi, band row tiles per item (0: a whole column).
+static void batch_items(vm_batch *b, int vi, int band)
+{
+    const bf_variant &V = kVariants[vi];
+    const int TR = RBK * V.mb * V.wr, cols = (V.nb / 2) * V.wc;
+    const int ntx = (b->nsb + cols - 1) / cols;
+    size_t total = 0;
+    for (int g = 0; g < b->n; g++) {
+        const vm_batch_chan &C = b->ch[g];
+        const int nty = (2 * C.np + TR - 1) / TR;
+        const int R = band > 0 ? band : nty;
+        total += (size_t)b->nchan * ntx * ((nty + R - 1) / R);
+    }
+    if (total >= ((size_t)1 << 31)) VM_FATAL("batch: %zu work items", total);
+    uint2 *h = (uint2 *)malloc(total * sizeof(uint2));
+    uint32_t *cost = (uint32_t *)malloc(total * sizeof(uint32_t));
+    if (!h || !cost) VM_FATAL("out of host memory");
+    size_t k = 0;
+    uint32_t maxcost = 0;
+    for (int g = 0; g < b->n; g++) {
+        const vm_batch_chan &C = b->ch[g];
+        const int nty = (2 * C.np + TR - 1) / TR;
+        const int R = band > 0 ? band : nty;
+        if (nty * TR > C.nrb * RBK) VM_FATAL("batch: row tiles exceed the padded rows");
+        if (nty > 255 || R > 255)
+            VM_FATAL("batch: %d beams in channel %d need %d row tiles (at most 255)", C.np, g, nty);
+        if (b->nchan > 65535 || ntx > 65535) VM_FATAL("batch: too many channels / sample tiles");
+        for (int c = 0; c < b->nchan; c++)
+            for (int tx = 0; tx < ntx; tx++)
+                for (int ty0 = 0; ty0 < nty; ty0 += R) {
+                    const int cnt = min(R, nty - ty0);
+                    h[k] = make_uint2(((uint32_t)g << 16) | (uint32_t)c,
+                                      ((uint32_t)tx << 16) | ((uint32_t)ty0 << 8) | (uint32_t)cnt);
+                    cost[k] = (uint32_t)(C.nkstep * cnt);
+                    if (cost[k] > maxcost) maxcost = cost[k];
+                    k++;
+                }
+    }
+    // most expensive first (a stable counting sort on cost keeps each
+    // column's items together)
+    uint2 *sorted = (uint2 *)malloc(total * sizeof(uint2));
+    size_t *start = (size_t *)calloc((size_t)maxcost + 2, sizeof(size_t));
+    if (!sorted || !start) VM_FATAL("out of host memory");
+    for (size_t i = 0; i < total; i++) start[maxcost - cost[i] + 1]++;
+    for (uint32_t c = 1; c <= maxcost + 1; c++) start[c] += start[c - 1];
+    for (size_t i = 0; i < total; i++) sorted[start[maxcost - cost[i]]++] = h[i];
+    if (b->items_cap < total) {
+        if (b->d_items) HIPCHECK(hipFree(b->d_items));
+        HIPCHECK(hipMalloc((void **)&b->d_items, total * sizeof(uint2)));
+        b->items_cap = total;
+    }
+    HIPCHECK(hipMemcpy(b->d_items, sorted, total * sizeof(uint2), hipMemcpyHostToDevice));
+    free(h);
+    free(cost);
+    free(sorted);
+    free(start);
+    b->nitems = (int)total;
+    b->variant = vi;
+    b->band = band;
+    batch_drop_graph(b);
+}
+
+// Bring each channel's weights, antenna list and operand buffers up to date
+// with its context (J^-1, phases, pointings); outside any graph.
+static void batch_refresh(vm_batch *b)
+{
+    hipStream_t st = b->st;
+    uint32_t *queue_flag = b->d_small + 1, *d_live = b->d_small + 64;
+    for (int g = 0; g < b->n; g++) {
+        vm_batch_chan &C = b->ch[g];
+        vcsbeam_context *vm = C.vm;
+        vm_device_state *d = vm_dev(vm);
+        HIPCHECK(hipStreamSynchronize((hipStream_t)d->stream));
+        const int np = (int)vm->npointing;
+        if (C.valid && C.gen == d->weights_gen && C.np == np) continue;
+        const int nant = (int)vm->obs_metadata->num_ants, nchan = b->nchan, nsb = b->nsb;
+        hipLaunchKernelGGL(k_ant_live, dim3((unsigned)nant), dim3(64), 0, st,
+                           (const double2 *)vm->d_J, nant, nchan, d_live);
+        HIPCHECK(hipGetLastError());
+        uint32_t live[256];
+        HIPCHECK(hipMemcpyAsync(live, d_live, nant * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        int amap[256];
+        uint32_t pq[256], pp[256];
+        int nact = 0;
+        for (int a = 0; a < nant; a++)
+            if (live[a]) {
+                pq[nact] = vm->polQ_idxs[a];
+                pp[nact] = vm->polP_idxs[a];
+                amap[nact++] = a;
+            }
+        const int nkstep = max(1, (nact + 7) / 8);
+        const int nrb = ((2 * np + ROW_PAD - 1) / ROW_PAD) * ROW_PAD / RBK;
+        const size_t af = align256((size_t)nchan * nrb * nkstep * 128 * sizeof(uint4));
+        const size_t bfb = align256((size_t)nchan * nsb * nkstep * 128 * sizeof(uint4));
+        const size_t nb = align256((size_t)nchan * b->ns * sizeof(float4));
+        const size_t se = align256((size_t)nchan * nsb * sizeof(int));
+        const size_t jf = align256((size_t)nchan * max(nact, 1) * 4 * sizeof(cf));
+        const size_t need = af + bfb + nb + se + jf + 3 * 1024;
+        if (C.mem_bytes < need) {
+            if (C.mem) HIPCHECK(hipFree(C.mem));
+            HIPCHECK(hipMalloc(&C.mem, need));
+            C.mem_bytes = need;
+            b->layout_changed = 1;
+        }
+        uint8_t *q = (uint8_t *)C.mem;
+        C.Af = (uint4 *)q;     q += af;
+        C.Bf = (uint4 *)q;     q += bfb;
+        C.N = (float4 *)q;     q += nb;
+        C.sexp = (int *)q;     q += se;
+        C.Jf = (cf *)q;        q += jf;
+        C.d_amap = (int *)q;   q += 1024;
+        C.d_polQ = (uint32_t *)q; q += 1024;
+        C.d_polP = (uint32_t *)q;
+        if (!C.valid || nact != C.nact || nkstep != C.nkstep || nrb != C.nrb || np != C.np)
+            b->layout_changed = 1;
+        if (nact > 0) {
+            HIPCHECK(hipMemcpyAsync(C.d_amap, amap, nact * sizeof(int), hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(C.d_polQ, pq, nact * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+            HIPCHECK(hipMemcpyAsync(C.d_polP, pp, nact * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_jf, dim3((unsigned)((nact * nchan + 255) / 256)), dim3(256), 0, st,
+                               (const double2 *)vm->d_J, nact, nchan, (const int *)C.d_amap, C.Jf);
+        }
+        const size_t total = (size_t)nchan * nrb * nkstep * 64;
+        const dim3 gw((unsigned)((total + 255) / 256));
+        if (b->bf16)
+            hipLaunchKernelGGL(k_phase_weights<true>, gw, dim3(256), 0, st,
+                               (const double2 *)vm->gdelays.d_phi, np, nact, nchan, nrb, nkstep,
+                               (const int *)C.d_amap, nant, C.Af);
+        else
+            hipLaunchKernelGGL(k_phase_weights<false>, gw, dim3(256), 0, st,
+                               (const double2 *)vm->gdelays.d_phi, np, nact, nchan, nrb, nkstep,
+                               (const int *)C.d_amap, nant, C.Af);
+        uint32_t hflag = 0;
+        if (np > 1) {
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, queue_flag, (size_t)1, 0u);
+            const size_t per = (size_t)nant * nchan * 4, tot = per * np;
+            hipLaunchKernelGGL(k_jcheck, dim3((unsigned)((tot - per + 255) / 256)), dim3(256), 0,
+                               st, (const double2 *)vm->d_J, per, tot, queue_flag);
+            HIPCHECK(hipMemcpyAsync(&hflag, queue_flag, 4, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(st));
+        if (hflag)
+            VM_FATAL("batch channel %d: J^-1 differs between pointings; the MFMA form needs a "
+                     "beam-independent J^-1", g);
+        C.nant_src = nant;
+        C.nact = nact;
+        C.nkstep = nkstep;
+        C.nrb = nrb;
+        C.np = np;
+        C.invw = (float)(1.0 / (double)vm->num_not_flagged);
+        C.gen = d->weights_gen;
+        C.valid = 1;
+    }
+    if (b->layout_changed) {
+        bf_geom *G = (bf_geom *)calloc(b->n, sizeof(bf_geom));
+        if (!G) VM_FATAL("out of host memory");
+        for (int g = 0; g < b->n; g++) {
+            const vm_batch_chan &C = b->ch[g];
+            G[g].Af = C.Af; G[g].Bf = C.Bf; G[g].sexp = C.sexp; G[g].N = C.N;
+            G[g].S = (float *)C.vm->d_S;
+            G[g].nkstep = C.nkstep; G[g].nrb = C.nrb; G[g].npointing = C.np;
+            G[g].nstokes = C.vm->out_nstokes; G[g].soffset = 0;
+            G[g].ns_total = C.vm->fine_sample_rate; G[g].invw = C.invw;
+        }
+        HIPCHECK(hipMemcpy(b->d_geoms, G, b->n * sizeof(bf_geom), hipMemcpyHostToDevice));
+        free(G);
+        batch_items(b, b->variant, b->band);
+        b->layout_changed = 0;
+    }
+}
+
+static void batch_pfb(vm_batch *b, int g, hipStream_t st)
+{
+    forward_pfb *f = b->ch[g].vm->fpfb;
+    vm_pfb_fused_launch(f->d_htr_data, f->d_htr_size, f->d_filter_coeffs, f->d_i_output_idx, f->I,
+                        f->nspectra_per_chunk, f->P, (f->flags & PFB_EMULATE_FPGA) != 0,
+                        (f->flags & PFB_IMAG_PART_FIRST) != 0, !f->filter_fits_int16,
+                        f->d_vcs_data, st);
+}
+
+static void batch_prep(vm_batch *b, int g, hipStream_t st)
+{
+    const vm_batch_chan &C = b->ch[g];
+    launch_prep(b->bf16, st, (const cf *)C.vm->fpfb->d_vcs_data, b->ns,
+                (int)C.vm->obs_metadata->num_rf_inputs, b->nsb, C.nkstep, b->nchan, C.Jf, C.d_polQ,
+                C.d_polP, C.nact, C.Bf, C.sexp, C.N);
+}
+
+static void batch_gemm(vm_batch *b, hipStream_t st)
+{
+    const bf_variant &V = kVariants[b->variant];
+    // queue counter cleared by a kernel (no memset node in the graph)
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, b->d_small, (size_t)1, 0u);
+    const int per_cu = max(1, 4 * V.occ / (V.wr * V.wc));
+    const unsigned nwg = (unsigned)min(b->nitems, b->ncu * per_cu);
+    if (b->bf16)
+        launch_batch_variant<true>(b->variant, nwg, st, b->d_geoms, b->d_items, b->nitems,
+                                   b->d_small, b->ns, b->nsb, b->nchan);
+    else
+        launch_batch_variant<false>(b->variant, nwg, st, b->d_geoms, b->d_items, b->nitems,
+                                    b->d_small, b->ns, b->nsb, b->nchan);
+    HIPCHECK(hipGetLastError());
+}
+
+static void batch_sequence(vm_batch *b, hipStream_t st)
+{
+    for (int g = 0; g < b->n; g++) {
+        batch_pfb(b, g, st);
+        batch_prep(b, g, st);
+    }
+    batch_gemm(b, st);
+}
+
+vm_batch *vmBatchCreate(vcsbeam_context **vms, int n)
+{
+    if (!vms || n < 1 || n > 65535) VM_FATAL("vmBatchCreate: %d contexts (1..65535)", n);
+    vm_batch *b = (vm_batch *)calloc(1, sizeof(vm_batch));
+    if (!b) VM_FATAL("out of host memory");
+    b->ch = (vm_batch_chan *)calloc(n, sizeof(vm_batch_chan));
+    if (!b->ch) VM_FATAL("out of host memory");
+    b->n = n;
+    const vcsbeam_context *v0 = vms[0];
+    b->nchan = v0->nfine_chan;
+    b->ns = (int)v0->fine_sample_rate;
+    b->nsb = (b->ns + SB - 1) / SB;
+    b->bf16 = v0->beamform_form == VM_BF_MFMA_BF16;
+    size_t vneed = 0;
+    for (int g = 0; g < n; g++) {
+        vcsbeam_context *vm = vms[g];
+        forward_pfb *f = vm ? vm->fpfb : NULL;
+        if (!f) VM_FATAL("vmBatchCreate: context %d has no forward PFB", g);
+        if (vm->datatype != VM_FLT || (f->flags & PFB_STAGED))
+            VM_FATAL("vmBatchCreate: context %d: the batch runs the fused PFB to VM_FLT", g);
+        if (vm->chunks_per_second != 1)
+            VM_FATAL("vmBatchCreate: context %d: one chunk per second is required", g);
+        if (vm->nfine_chan != b->nchan || (int)vm->fine_sample_rate != b->ns)
+            VM_FATAL("vmBatchCreate: context %d differs in channels or samples", g);
+        if (vm->beamform_form != v0->beamform_form ||
+            (vm->beamform_form != VM_BF_MFMA_FP16 && vm->beamform_form != VM_BF_MFMA_BF16))
+            VM_FATAL("vmBatchCreate: context %d: all contexts need the same MFMA form", g);
+        if (!vm->d_S || !vm->d_J || !vm->gdelays.d_phi || !vm->polQ_idxs || !f->d_htr_data)
+            VM_FATAL("vmBatchCreate: context %d lacks S, J, phi, pol lists or PFB input", g);
+        if (vm->obs_metadata->num_rf_inputs != 2 * vm->obs_metadata->num_ants ||
+            vm->obs_metadata->num_ants > 256)
+            VM_FATAL("vmBatchCreate: context %d: %u antennas / %u inputs", g,
+                     vm->obs_metadata->num_ants, vm->obs_metadata->num_rf_inputs);
+        if (!f->d_vcs_data && f->d_vcs_size > vneed) vneed = f->d_vcs_size;
+        b->ch[g].vm = vm;
+    }
+    if (vneed) {
+        HIPCHECK(hipMalloc(&b->d_v, vneed));
+        for (int g = 0; g < n; g++) {
+            vcsbeam_context *vm = vms[g];
+            if (vm->fpfb->d_vcs_data) continue;
+            vm->fpfb->d_vcs_data = b->d_v;
+            vm->d_v = b->d_v;
+            vm->d_v_size_bytes = vm->fpfb->d_vcs_size;
+        }
+    }
+    HIPCHECK(hipMalloc((void **)&b->d_geoms, n * sizeof(bf_geom)));
+    HIPCHECK(hipMalloc((void **)&b->d_small, 1024 * sizeof(uint32_t)));
+    HIPCHECK(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    b->variant = 1;
+    b->band = 4;
+    b->layout_changed = 1;
+    return b;
+}
+
+void vmBatchBeamformSecond(vm_batch *b, int use_graph)
+{
+    batch_refresh(b);
+    if (!use_graph) {
+        batch_sequence(b, b->st);
+    } else {
+        if (!b->exec) {
+            HIPCHECK(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
+            batch_sequence(b, b->st);
+            HIPCHECK(hipStreamEndCapture(b->st, &b->graph));
+            HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0));
+        }
+        HIPCHECK(hipGraphLaunch(b->exec, b->st));
+    }
+    HIPCHECK(hipStreamSynchronize(b->st));
+}
+
+void vmBatchStageTimes(vm_batch *b, float *pfb_ms, float *prep_ms, float *gemm_ms)
+{
+    batch_refresh(b);
+    const int ne = 2 * b->n + 3;
+    hipEvent_t *ev = (hipEvent_t *)calloc(ne, sizeof(hipEvent_t));
+    if (!ev) VM_FATAL("out of host memory");
+    for (int i = 0; i < ne; i++) HIPCHECK(hipEventCreate(&ev[i]));
+    HIPCHECK(hipEventRecord(ev[0], b->st));
+    for (int g = 0; g < b->n; g++) {
+        batch_pfb(b, g, b->st);
+        HIPCHECK(hipEventRecord(ev[2 * g + 1], b->st));
+        batch_prep(b, g, b->st);
+        HIPCHECK(hipEventRecord(ev[2 * g + 2], b->st));
+    }
+    batch_gemm(b, b->st);
+    HIPCHECK(hipEventRecord(ev[2 * b->n + 1], b->st));
+    HIPCHECK(hipEventSynchronize(ev[2 * b->n + 1]));
+    float tp = 0.f, tq = 0.f, t;
+    for (int g = 0; g < b->n; g++) {
+        HIPCHECK(hipEventElapsedTime(&t, ev[2 * g], ev[2 * g + 1]));
+        tp += t;
+        HIPCHECK(hipEventElapsedTime(&t, ev[2 * g + 1], ev[2 * g + 2]));
+        tq += t;
+    }
+    HIPCHECK(hipEventElapsedTime(&t, ev[2 * b->n], ev[2 * b->n + 1]));
+    if (pfb_ms) *pfb_ms = tp;
+    if (prep_ms) *prep_ms = tq;
+    if (gemm_ms) *gemm_ms = t;
+    for (int i = 0; i < ne; i++) HIPCHECK(hipEventDestroy(ev[i]));
+    free(ev);
+}
+
+void vmBatchSetSchedule(vm_batch *b, int variant, int band)
+{
+    if (!batch_variant_ok(variant)) VM_FATAL("batch: no usable variant %d", variant);
+    if (band < 0 || band > 255) VM_FATAL("batch: band %d (0..255)", band);
+    b->variant = variant;
+    b->band = band;
+    if (!b->layout_changed && b->nitems) batch_items(b, variant, band);
+}
+
+float vmBatchTune(vm_batch *b, int reps)
+{
+    // times every usable (variant, band) on the current second and keeps
+    // the fastest; S holds the chosen schedule's output afterwards
+    batch_refresh(b);
+    for (int g = 0; g < b->n; g++) {
+        batch_pfb(b, g, b->st);
+        batch_prep(b, g, b->st);
+    }
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    static const int bands[] = {1, 2, 4, 8, 0};
+    float best = 1e30f;
+    int bv = b->variant, bb = b->band;
+    for (int vi = 0; vi < N_VARIANTS; vi++) {
+        if (!batch_variant_ok(vi)) continue;
+        for (int bi = 0; bi < (int)(sizeof(bands) / sizeof(bands[0])); bi++) {
+            batch_items(b, vi, bands[bi]);
+            batch_gemm(b, b->st); // warm-up
+            HIPCHECK(hipEventRecord(e0, b->st));
+            for (int r = 0; r < reps; r++) batch_gemm(b, b->st);
+            HIPCHECK(hipEventRecord(e1, b->st));
+            HIPCHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            if (getenv("VCSBEAM_TUNE_VERBOSE"))
+                fprintf(stderr, "batch variant %d band %d: %d items, %.3f ms\n", vi, bands[bi],
+                        b->nitems, ms);
+            if (ms < best) { best = ms; bv = vi; bb = bands[bi]; }
+        }
+    }
+    HIPCHECK(hipEventDestroy(e0));
+    HIPCHECK(hipEventDestroy(e1));
+    batch_items(b, bv, bb);
+    batch_gemm(b, b->st);
+    HIPCHECK(hipStreamSynchronize(b->st));
+    return best;
+}
+
+int vmBatchInfo(vm_batch *b, int *variant, int *band, int *nitems)
+{
+    if (variant) *variant = b->variant;
+    if (band) *band = b->band;
+    if (nitems) *nitems = b->nitems;
+    return b->n;
+}
+
+int vmBatchActiveTiles(vm_batch *b, int g)
+{
+    if (g < 0 || g >= b->n) return -1;
+    batch_refresh(b);
+    return b->ch[g].nact;
+}
+
+void vmBatchDestroy(vm_batch *b)
+{
+    if (!b) return;
+    HIPCHECK(hipStreamSynchronize(b->st));
+    batch_drop_graph(b);
+    for (int g = 0; g < b->n; g++) {
+        vm_batch_chan &C = b->ch[g];
+        if (C.mem) HIPCHECK(hipFree(C.mem));
+        if (b->d_v && C.vm->fpfb && C.vm->fpfb->d_vcs_data == b->d_v) {
+            C.vm->fpfb->d_vcs_data = NULL;
+            if (C.vm->d_v == b->d_v) {
+                C.vm->d_v = NULL;
+                C.vm->d_v_size_bytes = 0;
+            }
+        }
+    }
+    if (b->d_v) HIPCHECK(hipFree(b->d_v));
+    if (b->d_items) HIPCHECK(hipFree(b->d_items));
+    HIPCHECK(hipFree(b->d_geoms));
+    HIPCHECK(hipFree(b->d_small));
+    HIPCHECK(hipStreamDestroy(b->st));
+    free(b->ch);
+    free(b);
 }
 
 } // extern "C"

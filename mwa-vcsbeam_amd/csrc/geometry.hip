@@ -157,7 +157,7 @@ void vmCalcPhiDevice(vcsbeam_context *vm, beam_geom *bg)
     vm_calc_phi_launch(d->d_pos, d->d_unit, d->d_freqs, np, nant, nchan, g->d_phi, s);
     HIPCHECK(hipStreamSynchronize(s));
     free(unit);
-    d->weights_dirty = 1;
+    d->weights_gen++;
 }
 
 void vmCalcJDevice(vcsbeam_context *vm)
@@ -170,7 +170,7 @@ void vmCalcJDevice(vcsbeam_context *vm)
     HIPCHECK(hipMemcpyAsync(d->d_B, vm->pb.B, bbytes, hipMemcpyHostToDevice, s));
     vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 0, vm->d_J, s);
     HIPCHECK(hipStreamSynchronize(s));
-    d->weights_dirty = 1;
+    d->weights_gen++;
 }
 
 void vmCalcJDeviceChanB(vcsbeam_context *vm, const gpuDoubleComplex *B)
@@ -183,7 +183,7 @@ void vmCalcJDeviceChanB(vcsbeam_context *vm, const gpuDoubleComplex *B)
     HIPCHECK(hipMemcpyAsync(d->d_B, B, bbytes, hipMemcpyHostToDevice, s));
     vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 1, vm->d_J, s);
     HIPCHECK(hipStreamSynchronize(s));
-    d->weights_dirty = 1;
+    d->weights_gen++;
 }
 
 } // extern "C"

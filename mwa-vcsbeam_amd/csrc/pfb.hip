@@ -499,9 +499,14 @@ void vmInitForwardPFB(vcsbeam_context *vm, int M, pfb_flags flags)
     HIPCHECK(hipMemcpy(fpfb->d_filter_coeffs, fpfb->filter_coeffs, filter_size,
                        hipMemcpyHostToDevice));
 
+    // src/pfb.cpp:464-474: the device input / output buffers are allocated
+    // only with their PFB_MALLOC_DEVICE_* flag; otherwise the caller sets the
+    // pointer (a buffer shared by several contexts, for instance) and keeps
+    // ownership of it
     if (flags & PFB_MALLOC_HOST_OUTPUT) fpfb->vcs_data = vm_host_alloc_pinned(fpfb->vcs_size);
-    HIPCHECK(hipMalloc((void **)&fpfb->d_htr_data, fpfb->d_htr_size));
-    HIPCHECK(hipMalloc(&fpfb->d_vcs_data, fpfb->d_vcs_size));
+    if (flags & PFB_MALLOC_DEVICE_INPUT)
+        HIPCHECK(hipMalloc((void **)&fpfb->d_htr_data, fpfb->d_htr_size));
+    if (flags & PFB_MALLOC_DEVICE_OUTPUT) HIPCHECK(hipMalloc(&fpfb->d_vcs_data, fpfb->d_vcs_size));
     if (vm->datatype == VM_INT4)
         HIPCHECK(hipMalloc(&fpfb->d_flt_out, nsamp_chunk * sizeof(gpuFloatComplex)));
     if (flags & PFB_STAGED) {
@@ -524,8 +529,10 @@ void vmFreeForwardPFB(forward_pfb *fpfb)
     if (fpfb->vcs_data) HIPCHECK(hipHostFree(fpfb->vcs_data));
     if (fpfb->i_output_idx) HIPCHECK(hipHostFree(fpfb->i_output_idx));
     if (fpfb->d_filter_coeffs) HIPCHECK(hipFree(fpfb->d_filter_coeffs));
-    if (fpfb->d_htr_data) HIPCHECK(hipFree(fpfb->d_htr_data));
-    if (fpfb->d_vcs_data) HIPCHECK(hipFree(fpfb->d_vcs_data));
+    if (fpfb->d_htr_data && (fpfb->flags & PFB_MALLOC_DEVICE_INPUT))
+        HIPCHECK(hipFree(fpfb->d_htr_data));
+    if (fpfb->d_vcs_data && (fpfb->flags & PFB_MALLOC_DEVICE_OUTPUT))
+        HIPCHECK(hipFree(fpfb->d_vcs_data));
     if (fpfb->d_i_output_idx) HIPCHECK(hipFree(fpfb->d_i_output_idx));
     if (fpfb->d_weighted_overlap_add) HIPCHECK(hipFree(fpfb->d_weighted_overlap_add));
     if (fpfb->d_flt_out) HIPCHECK(hipFree(fpfb->d_flt_out));
@@ -539,6 +546,9 @@ void vmUploadForwardPFBChunk(vcsbeam_context *vm)
     logger_start_stopwatch(vm->log, "upload", false);
     int chunk = vm->chunk_to_load % vm->chunks_per_second;
     hipStream_t s = vm_stream(vm);
+    if (!vm->fpfb->d_htr_data)
+        VM_FATAL("vmUploadForwardPFBChunk: no device input buffer (PFB_MALLOC_DEVICE_INPUT "
+                 "not set and d_htr_data not provided)");
     HIPCHECK(hipMemcpyAsync(vm->fpfb->d_htr_data,
                             (char *)vm->v->buffer + (size_t)chunk * vm->fpfb->htr_stride,
                             vm->fpfb->d_htr_size, hipMemcpyHostToDevice, s));
@@ -591,6 +601,10 @@ void vmWOLAChunk(vcsbeam_context *vm)
 {
     forward_pfb *fpfb = vm->fpfb;
     hipStream_t s = vm_stream(vm);
+    if (!fpfb->d_htr_data || !fpfb->d_vcs_data)
+        VM_FATAL("vmWOLAChunk: no device %s buffer (PFB_MALLOC_DEVICE_%s not set and none "
+                 "provided)", fpfb->d_htr_data ? "output" : "input",
+                 fpfb->d_htr_data ? "OUTPUT" : "INPUT");
     logger_start_stopwatch(vm->log, "pfb-wola", false);
     if (fpfb->flags & PFB_STAGED) {
         dim3 grid(fpfb->nspectra_per_chunk, fpfb->I), block(KCH);

@@ -53,7 +53,7 @@ vm_device_state *vm_dev(vcsbeam_context *vm)
     hipStream_t s;
     HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     d->stream = (void *)s;
-    d->weights_dirty = 1;
+    d->weights_gen++;
     vm->dev = d;
     return d;
 }
@@ -249,7 +249,7 @@ void vmPushJ(vcsbeam_context *vm)
     hipStream_t s = (hipStream_t)vm_dev(vm)->stream;
     HIPCHECK(hipMemcpyAsync(vm->d_J, vm->J, vm->J_size_bytes, hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
-    vm->dev->weights_dirty = 1;
+    vm->dev->weights_gen++;
 }
 
 void vmPushD(vcsbeam_context *vm)
@@ -267,7 +267,7 @@ void vmPushPolIdxLists(vcsbeam_context *vm)
     HIPCHECK(hipMemcpyAsync(vm->d_polP_idxs, vm->polP_idxs, vm->pol_idxs_size_bytes,
                             hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
-    vm->dev->weights_dirty = 1;
+    vm->dev->weights_gen++;
 }
 
 void vmCreateGeometricDelays(vcsbeam_context *vm)
@@ -306,7 +306,7 @@ void vmPushPhi(vcsbeam_context *vm)
     hipStream_t s = (hipStream_t)vm_dev(vm)->stream;
     HIPCHECK(hipMemcpyAsync(g->d_phi, g->phi, size, hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
-    vm->dev->weights_dirty = 1;
+    vm->dev->weights_gen++;
 }
 
 void vmCreatePrimaryBeam(vcsbeam_context *vm)
@@ -538,6 +538,63 @@ extern "C" void vmDeviceCopy(void *dst, const void *src, size_t bytes)
     size_t n = bytes / 16;
     hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, 0, (const uint4 *)src,
                        (uint4 *)dst, n);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipDeviceSynchronize());
+}
+
+/* ---------------------------------------------- seeded synthetic voltages ---- */
+/* Counter-based generator: element idx of stream `seed` is the SplitMix64
+ * finaliser of seed * 0xD1B54A32D192ED03 + (idx + 1) * 0x9E3779B97F4A7C15;
+ * uniforms take its top 53 bits; a Gaussian pair is Box-Muller on uniforms
+ * 2 pair and 2 pair + 1 (r = sqrt(-2 ln(1 - u1)), angle 2 pi u2). */
+__device__ __forceinline__ uint64_t synth_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double synth_unif(uint64_t seed, uint64_t idx)
+{
+    const uint64_t z = synth_mix(seed * 0xD1B54A32D192ED03ULL + (idx + 1) * 0x9E3779B97F4A7C15ULL);
+    return (double)(z >> 11) * 0x1.0p-53;
+}
+
+__device__ __forceinline__ int8_t synth_q8(double x)
+{
+    double r = round(x);
+    r = r > 127.0 ? 127.0 : (r < -127.0 ? -127.0 : r);
+    return (int8_t)r;
+}
+
+// out: [nblocks][ninput][spb] char2; block b draws pair ((first + b) I + i) spb + t
+__global__ void k_synth_mwax(uint64_t seed, uint64_t first_block, int nblocks, int ninput, int spb,
+                             double sigma, int zero_blocks, uint16_t *__restrict__ out)
+{
+    const size_t per_block = (size_t)ninput * spb, n = per_block * nblocks;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const size_t b = j / per_block;
+        uint16_t v = 0;
+        if ((long)b >= zero_blocks) {
+            const uint64_t pair = (first_block + b) * per_block + (j - b * per_block);
+            const double u1 = synth_unif(seed, 2 * pair), u2 = synth_unif(seed, 2 * pair + 1);
+            const double r = sqrt(-2.0 * log(1.0 - u1)), t = 2.0 * 3.141592653589793 * u2;
+            const int8_t re = synth_q8(sigma * (r * cos(t))), im = synth_q8(sigma * (r * sin(t)));
+            v = (uint16_t)(uint8_t)re | ((uint16_t)(uint8_t)im << 8);
+        }
+        out[j] = v;
+    }
+}
+
+extern "C" void vmSynthMWAXDevice(uint64_t seed, uint64_t first_block, int nblocks, int ninput,
+                                  int spb, double sigma, int zero_blocks, void *d_out)
+{
+    if (nblocks < 0 || ninput < 0 || spb < 0)
+        VM_FATAL("vmSynthMWAXDevice: negative size (%d blocks, %d inputs, %d samples)", nblocks,
+                 ninput, spb);
+    hipLaunchKernelGGL(k_synth_mwax, dim3(8192), dim3(256), 0, 0, seed, first_block, nblocks,
+                       ninput, spb, sigma, zero_blocks, (uint16_t *)d_out);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipDeviceSynchronize());
 }

@@ -23,7 +23,12 @@ struct vm_device_state {
        [p][c][a][2][2] */
     void *d_W;
     size_t W_bytes;
-    int weights_dirty;
+    /* bumped whenever J, D, phi or the pol index lists change; each set of
+       derived weights remembers the generation it was built from */
+    unsigned weights_gen;
+    unsigned W_gen;  /* FP32 weights d_W */
+    unsigned Wm_gen; /* MFMA weights in d_Wm */
+    int Wm_form;     /* beamform_form (FP16 / BF16) the MFMA weights are in */
     /* MFMA weights: [p-block][c][row][k] packed half/bf16 planes */
     void *d_Wm;
     size_t Wm_bytes;

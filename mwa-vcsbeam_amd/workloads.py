@@ -89,7 +89,7 @@ class Observation:
 
     def __init__(self, ntiles=128, blocks_per_second=20, nseconds=1, nflagged=4,
                  rec_chan=133, legacy=False, filter_coeffs=None, obs_id=1300000000,
-                 seed_offset=0):
+                 seed_offset=0, geometry_seed_offset=None):
         self.ntiles = ntiles
         self.ninput = 2 * ntiles
         self.blocks_per_second = blocks_per_second
@@ -99,7 +99,10 @@ class Observation:
         I = self.ninput
         # geometry: uniform in a disc of radius 1.5 km, small height scatter,
         # electrical lengths 0..300 m
-        g = uniform(SEED_GEOM + seed_offset, 4 * ntiles)
+        # geometry_seed_offset: the array layout (positions, cables, VCS order)
+        # shared by coarse channels that differ in data, flags and Jones
+        gso = seed_offset if geometry_seed_offset is None else geometry_seed_offset
+        g = uniform(SEED_GEOM + gso, 4 * ntiles)
         r = 1500.0 * np.sqrt(g[0::4])
         th = 2 * np.pi * g[1::4]
         self.east = r * np.cos(th)
@@ -107,7 +110,7 @@ class Observation:
         self.height = MWA_ALTITUDE_METRES + 3.0 * (g[2::4] - 0.5)
         self.cable = 300.0 * g[3::4]
         # VCS order: seeded permutation of the RF inputs
-        perm_keys = uniform(SEED_GEOM + seed_offset, I, offset=10 ** 6)
+        perm_keys = uniform(SEED_GEOM + gso, I, offset=10 ** 6)
         self.vcs_order = np.argsort(perm_keys, kind="stable").astype(np.uint32)
         # flags: nflagged tiles (both pols)
         fk = uniform(SEED_FLAGS + seed_offset, ntiles)
