@@ -236,6 +236,97 @@ def surrogate_stage(lib, nbeams=4096, nchan=128, steps=4000, batch=2048, nsample
         lib.vmSurrogateDestroy(s)
 
 
+def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_check=4):
+    """Batched uneven workload: nchan coarse channels x 1 s of ntiles tiles,
+    per channel a seeded flagged-tile count ~ U{0..40} and beam count ~
+    floor(exp U[0, ln 2049)) (log-uniform 1..2048) within 15 deg; FP32
+    forward PFB + FP16 MFMA beamformer, Stokes IQUV, every channel in one
+    persistent launch inside one replayed hipGraph; seconds generated on the
+    device.  Per channel, beams_check beams x nsamp_check samples x 128
+    channels are checked against the FP64 oracle beamformer fed with the
+    GPU's own PFB output of that channel."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc  # the checker (per-channel error only)
+    from vcsbeam_amd.apps import BatchSession
+    from vcsbeam_amd.workloads import (SEED_BEAMS, SEED_FLAGS, Observation, directions_in_cap,
+                                       uniform, unit_vectors)
+    u = uniform(SEED_FLAGS, 2 * nchan, offset=10 ** 8)
+    nflag = np.minimum((u[0::2] * 41).astype(int), 40)
+    nbeam = np.floor(np.exp(u[1::2] * np.log(2049.0))).astype(int)
+    obs = [Observation(ntiles=ntiles, blocks_per_second=20, nflagged=int(nflag[g]),
+                       rec_chan=109 + g, seed_offset=1000 * (g + 1), geometry_seed_offset=0)
+           for g in range(nchan)]
+    dirs = [directions_in_cap(int(nbeam[g]), 15.0, seed=SEED_BEAMS + 100 * (g + 1))
+            for g in range(nchan)]
+    t0 = time.perf_counter()
+    bs = BatchSession(obs, dirs, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
+                      nstokes=4, form=pkg.VM_BF_MFMA_FP16, share_pfb_output=True,
+                      device_data=True)
+    try:
+        bs.read(0)
+        t_setup = time.perf_counter() - t0
+        print(f"[bench] batch: {nchan} channels, {int(nbeam.sum())} beams, set up in "
+              f"{t_setup:.1f} s; autotune", file=sys.stderr, flush=True)
+        tune_ms = bs.tune(reps=2)
+        bs.run(use_graph=True)  # capture + first replay
+        lib.vmDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bs.run(use_graph=True)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        st = bs.stage_times()
+        info = bs.info()
+        nact = [bs.active_tiles(g) for g in range(nchan)]
+        nspc = obs[0].nspectra
+        bsamp = int(nbeam.sum()) * 128 * nspc
+        flops = sum(16.0 * nact[g] * nbeam[g] * 128 * nspc for g in range(nchan))
+        gemm_bytes = sum(nbeam[g] * 128 * nspc * 16 + 128 * nact[g] * 2 * nspc * 4
+                         + 128 * (2 * nbeam[g]) * (2 * nact[g]) * 4 + 128 * nspc * 16
+                         for g in range(nchan))
+        print("[bench] batch: per-channel check against the FP64 oracle", file=sys.stderr,
+              flush=True)
+        per = []
+        for g, ses in enumerate(bs.sessions):
+            o, (az, za) = obs[g], dirs[g]
+            ses.pfb()
+            v = np.empty((128, o.ninput, nsamp_check), np.complex64)
+            lib.vmMemcpy2DDtoH(v.ctypes.data, nsamp_check * 8, ses.v.d_v, nspc * 8,
+                               nsamp_check * 8, 128 * o.ninput)
+            pk = min(beams_check, len(az))
+            B = np.zeros((pk, o.ntiles, 2, 2), complex)
+            B[..., 0, 0] = B[..., 1, 1] = 1
+            E, N, H, L = o.positions()
+            phi = orc.calc_phi(E, N, H, L, unit_vectors(az[:pk], za[:pk]), o.freqs)
+            polQ, polP = o.pol_idx()
+            Sref, _ = orc.beamform(np.ascontiguousarray(v.transpose(2, 0, 1)).astype(np.complex128),
+                                   orc.calc_J(o.calibration(), B), phi, polQ, polP,
+                                   1.0 / (2 * int((~o.flag_mask).sum())), nstokes=4)
+            got = ses.stokes_slice(0, pk, 0, nsamp_check)
+            err = np.array([[orc.rel_err(got[p, :, k, :], Sref[p, :, k, :]) for k in range(4)]
+                            for p in range(pk)]).max(axis=0)
+            per.append({"beams": int(nbeam[g]), "flagged_tiles": int(nflag[g]),
+                        "gemm_tiles": nact[g],
+                        "max_beam_rel_err_IQUV": [float(f"{x:.3e}") for x in err]})
+        return {"workload": f"{nchan} coarse channels x 1 s, {ntiles} tiles x 2 pols, flagged "
+                            f"tiles U{{0..40}} and beams log-uniform 1..2048 per channel, FP32 PFB "
+                            f"+ FP16 MFMA beamformer, Stokes IQUV, one persistent launch in one "
+                            f"hipGraph",
+                "value": round(bsamp / (ms * 1e-3), 1), "unit": "beam-samples/s",
+                "beams_total": int(nbeam.sum()), "ms_per_second_of_data": round(ms, 2),
+                "stages_eager_ms": {k: round(x, 2) for k, x in st.items()},
+                "gemm_tflops_algorithmic": round(flops / (st["gemm_ms"] * 1e-3) / 1e12, 1),
+                "gemm_GBps_algorithmic": round(gemm_bytes / (st["gemm_ms"] * 1e-3) / 1e9, 1),
+                "pfb_hbm_frac": round(nchan * 2 * ntiles * nspc * 128 * 10
+                                      / (st["pfb_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "schedule": {**info, "tuned_gemm_ms": round(tune_ms, 2)},
+                "setup_s": round(t_setup, 1),
+                "max_channel_rel_err_I": max(p["max_beam_rel_err_IQUV"][0] for p in per),
+                "per_channel": per}
+    finally:
+        bs.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +336,10 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched 24-channel uneven workload")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched 24-channel workload")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the primary-beam surrogate stage (training + 4096-beam inference)")
     ap.add_argument("--variant", type=int, default=-1,
@@ -271,6 +366,9 @@ def main():
 
     pkg = load_pkg()
     lib = pkg.load()
+    if args.batch_only:
+        print(json.dumps({"batch": batch_stage(pkg, lib)}))
+        return
     from vcsbeam_amd.apps import TiedArraySession
     from vcsbeam_amd.workloads import Observation, directions_in_cap
 
@@ -381,6 +479,10 @@ def main():
             par[name] = {"max_beam_rel_err_IQUV": [float(f"{x:.3e}") for x in e.max(axis=0)]}
         result["parity_vs_fp64"] = par
     sess.close()
+    if not args.no_batch:
+        print("[bench] batched uneven workload (24 channels x 256 tiles)", file=sys.stderr,
+              flush=True)
+        result["batch"] = batch_stage(pkg, lib)
     if args.calibrate_copy:
         a, b = pkg.DeviceArray(COPY_BYTES), pkg.DeviceArray(COPY_BYTES)
         lib.vmDeviceCopy(b.ptr, a.ptr, COPY_BYTES)

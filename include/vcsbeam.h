@@ -787,6 +787,9 @@ void *vmDeviceMalloc(size_t bytes);
 void vmDeviceFree(void *p);
 void vmMemcpyHtoD(void *dst, const void *src, size_t bytes);
 void vmMemcpyDtoH(void *dst, const void *src, size_t bytes);
+/* height rows of width bytes, rows spitch apart on the device, dpitch on the host */
+void vmMemcpy2DDtoH(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                    size_t height);
 void vmMemsetDevice(void *dst, int value, size_t bytes);
 void vmDeviceSynchronize(void);
 void vmDeviceCopy(void *dst, const void *src, size_t bytes); /* plain copy kernel */

@@ -418,6 +418,11 @@ void vmMemcpyDtoH(void *dst, const void *src, size_t bytes)
 {
     HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
 }
+void vmMemcpy2DDtoH(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                    size_t height)
+{
+    HIPCHECK(hipMemcpy2D(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost));
+}
 void vmMemsetDevice(void *dst, int value, size_t bytes)
 {
     /* complete before returning: later work may run on a non-blocking stream */
