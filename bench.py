@@ -327,6 +327,69 @@ def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_c
         bs.close()
 
 
+def pipeline_stage(pkg, lib, nchan=24, steps=2):
+    """Full plain pipeline: nchan coarse channels x 1 s of 128 tiles (4
+    flagged), fused FP32 forward PFB, FP64 device geometry, the plain FP32
+    tied-array beam towards one pointing (za 20 deg), Stokes I (the PSRFITS
+    product, 100 us samples), inverse PFB back to 1.28 MHz complex float32 and
+    8-bit VDIF requantisation, all device-resident; the channels run in turn
+    and share one forward-PFB output buffer owned by the caller."""
+    from vcsbeam_amd.apps import TiedArraySession, device_second
+    from vcsbeam_amd.workloads import Observation
+    obs = [Observation(ntiles=128, blocks_per_second=20, nflagged=4, rec_chan=109 + g,
+                       seed_offset=1000 * (g + 1), geometry_seed_offset=0) for g in range(nchan)]
+    nspc = obs[0].nspectra
+    az, za = np.array([0.7]), np.radians([20.0])
+    dv = pkg.DeviceArray(128 * obs[0].ninput * nspc * 8)
+    flags = (pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA) & ~pkg.PFB_MALLOC_DEVICE_OUTPUT
+    sess = []
+    try:
+        for o in obs:
+            sess.append(TiedArraySession(o, az, za, pfb_flags=flags, nstokes=1,
+                                         form=pkg.VM_BF_FP32, host_buffers=False, ipfb=True,
+                                         d_v=dv.ptr))
+        for x in sess:
+            device_second(x, 0)
+
+        def second(t):
+            for x in sess:
+                x.pfb()
+                x.geometry()
+                x.beamform()
+                x.invert(t)
+
+        second(0)
+        for x in sess:
+            lib.vmSetKernelTiming(x.vm, 1)
+        lib.vmDeviceSynchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            second(1 + k)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        stages = {}
+        for name in ("pfb", "geometry", "beamform_fp32", "ipfb"):
+            tot = 0.0
+            for x in sess:
+                avg, n = kernel_ms(lib, x, name)
+                tot += avg * n / steps
+            stages[name + "_ms"] = round(tot, 3)
+        ipfb_bytes = nchan * nspc * 128 * 2 * 16  # 8 B fine in + 8 B coarse out per sample-pol
+        return {"workload": f"{nchan} coarse channels x 1 s, 128 tiles x 2 pols, one beam: FP32 "
+                            f"PFB, FP32 beamformer (Stokes I), inverse PFB, 8-bit VDIF, "
+                            f"device-resident",
+                "value": round(nchan * 128 * nspc / (ms * 1e-3), 1), "unit": "beam-samples/s",
+                "ms_per_second_of_data": round(ms, 2),
+                "kernel_ms_per_second_of_data": stages,
+                "pfb_hbm_frac": round(nchan * 2 * 128 * nspc * 128 * 10 /
+                                      (stages["pfb_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "ipfb_hbm_frac": round(ipfb_bytes / (stages["ipfb_ms"] * 1e-3) / 1e9
+                                       / HBM_PEAK_GBS, 4)}
+    finally:
+        for x in sess:
+            x.close()
+        del dv
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -340,6 +403,10 @@ def main():
                     help="skip the batched 24-channel uneven workload")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched 24-channel workload")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="skip the 24-channel plain pipeline with inverse PFB")
+    ap.add_argument("--pipeline-only", action="store_true",
+                    help="run only the 24-channel plain pipeline with inverse PFB")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the primary-beam surrogate stage (training + 4096-beam inference)")
     ap.add_argument("--variant", type=int, default=-1,
@@ -368,6 +435,9 @@ def main():
     lib = pkg.load()
     if args.batch_only:
         print(json.dumps({"batch": batch_stage(pkg, lib)}))
+        return
+    if args.pipeline_only:
+        print(json.dumps({"pipeline": pipeline_stage(pkg, lib)}))
         return
     from vcsbeam_amd.apps import TiedArraySession
     from vcsbeam_amd.workloads import Observation, directions_in_cap
@@ -483,6 +553,10 @@ def main():
         print("[bench] batched uneven workload (24 channels x 256 tiles)", file=sys.stderr,
               flush=True)
         result["batch"] = batch_stage(pkg, lib)
+    if not args.no_pipeline:
+        print("[bench] plain pipeline with inverse PFB (24 channels x 128 tiles)",
+              file=sys.stderr, flush=True)
+        result["pipeline"] = pipeline_stage(pkg, lib)
     if args.calibrate_copy:
         a, b = pkg.DeviceArray(COPY_BYTES), pkg.DeviceArray(COPY_BYTES)
         lib.vmDeviceCopy(b.ptr, a.ptr, COPY_BYTES)

@@ -207,10 +207,14 @@ class TiedArraySession:
     """
 
     def __init__(self, obs, az, za, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, form=VM_BF_FP32,
-                 want_e=False, host_buffers=True):
+                 want_e=False, host_buffers=True, ipfb=False, d_v=None):
         """host_buffers=False leaves out the host S and J buffers and frees the
         pinned read buffer (for seconds generated on the device and outputs
-        read back in slices)."""
+        read back in slices).  ipfb=True adds the inverse PFB of pointing 0
+        (mirror synthesis filter x 15/7.2, app/make_mwa_tied_array_beam.c:
+        155-165) and device VDIF requantisation (:meth:`invert`).  d_v: a
+        caller-owned device buffer for the forward PFB output when pfb_flags
+        lacks PFB_MALLOC_DEVICE_OUTPUT."""
         lib = self.lib = load()
         self.obs = obs
         az = np.atleast_1d(np.asarray(az, dtype=np.float64))
@@ -226,8 +230,16 @@ class TiedArraySession:
         lib.vmSetFilter(vm, self.coeffs.ctypes.data_as(C.POINTER(C.c_double)), len(self.coeffs),
                         ANALYSIS_FILTER, 128)
         lib.vmInitForwardPFB(vm, 128, pfb_flags)
+        if d_v is not None:
+            v.fpfb.contents.d_vcs_data = d_v
         lib.vmSetNumPointings(vm, self.npointing)
         self.bg = _beam_geoms(az, za)
+        self.ipfb = ipfb
+        if ipfb:
+            sf = np.ascontiguousarray(self.coeffs / self.coeffs.max())
+            lib.vmSetFilter(vm, sf.ctypes.data_as(C.POINTER(C.c_double)), len(sf),
+                            SYNTHESIS_FILTER, 128)
+            lib.vmScaleFilterCoeffs(vm, SYNTHESIS_FILTER, 15.0 / 7.2)
         lib.vmSetMaxGPUMem(vm, 1)
         names = ["vmMallocVDevice", "vmMallocSDevice", "vmMallocJDevice", "vmMallocDHost",
                  "vmMallocDDevice", "vmMallocPQIdxsHost", "vmMallocPQIdxsDevice"]
@@ -237,8 +249,16 @@ class TiedArraySession:
             lib.vmFreeVHost(vm)
         if want_e:
             names += ["vmMallocEHost", "vmMallocEDevice"]
+        elif ipfb:
+            names += ["vmMallocEDevice"]
         for f in names:
             getattr(lib, f)(vm)
+        if ipfb:
+            self.gi = GpuIpfbArrays()
+            lib.malloc_ipfb(C.byref(self.gi), v.synth_filter, v.fine_sample_rate, 2, 1)
+            lib.cu_load_ipfb_filter(v.synth_filter, C.byref(self.gi))
+            self.nvdif = v.fine_sample_rate * v.nfine_chan * 2 * 2
+            self.d_vdif = lib.vmDeviceMalloc(self.nvdif)
         lib.vmSetPolIdxLists(vm)
         lib.vmPushPolIdxLists(vm)
         D = np.ascontiguousarray(obs.calibration(v.nfine_chan))
@@ -259,6 +279,13 @@ class TiedArraySession:
     def pfb(self):
         for f in ("vmWOLAChunk", "vmFPGARoundingChunk", "vmFFTChunk", "vmPackChunk"):
             getattr(self.lib, f)(self.vm)
+
+    def invert(self, t):
+        """Inverse PFB of pointing 0 from the device e (second t) and its 8-bit
+        VDIF requantisation, both on the device."""
+        self.lib.vmInvertPFBDevice(self.vm, C.byref(self.gi), t, None)
+        self.lib.vmRequantVDIFDevice(self.gi.d_out, self.d_vdif, self.nvdif, self.stream)
+        self.lib.vmSynchronize(self.vm)
 
     def geometry(self):
         self.lib.vmCalcPhiDevice(self.vm, self.bg)
@@ -301,6 +328,11 @@ class TiedArraySession:
 
     def close(self):
         lib, vm = self.lib, self.vm
+        if self.ipfb:
+            lib.free_ipfb(C.byref(self.gi))
+            lib.vmDeviceFree(self.d_vdif)
+            if not self.v.e and self.v.d_e:
+                lib.vmFreeEDevice(vm)
         for f in ("vmFreeVDevice", "vmFreeSDevice", "vmFreeJDevice",
                   "vmFreeDHost", "vmFreeDDevice", "vmFreePQIdxsDevice", "vmFreePQIdxsHost"):
             getattr(lib, f)(vm)
@@ -313,6 +345,22 @@ class TiedArraySession:
             lib.vmFreeEDevice(vm)
         lib.destroy_vcsbeam_context(vm)
         self.vm = None
+
+
+def device_second(ses, t):
+    """Generate second t of ses.obs in place in the session's forward-PFB
+    input buffer (margin block + the second's blocks) with the device copy of
+    the workload generator, as vmUploadForwardPFBChunk would have left it."""
+    from .workloads import SEED_DATA, SPB
+    o, fp = ses.obs, ses.v.fpfb.contents
+    base = C.cast(fp.d_htr_data, C.c_void_p).value
+    nb = o.blocks_per_second
+    block = o.ninput * SPB * 2
+    # margin block = last block of the previous second (zeros at t = 0)
+    ses.lib.vmSynthMWAXDevice(SEED_DATA + o.seed_offset, max(t * nb - 1, 0), 1, o.ninput, SPB,
+                              16.0, 1 if t == 0 else 0, base)
+    ses.lib.vmSynthMWAXDevice(SEED_DATA + o.seed_offset, t * nb, nb, o.ninput, SPB, 16.0, 0,
+                              base + block)
 
 
 class BatchSession:
@@ -341,20 +389,11 @@ class BatchSession:
         self.b = lib.vmBatchCreate(arr, len(self.sessions))
 
     def read(self, t):
-        from .workloads import SEED_DATA, SPB
         for ses in self.sessions:
-            if not self.device_data:
+            if self.device_data:
+                device_second(ses, t)
+            else:
                 ses.read(t)
-                continue
-            o, fp = ses.obs, ses.v.fpfb.contents
-            base = C.cast(fp.d_htr_data, C.c_void_p).value
-            nb = o.blocks_per_second
-            block = o.ninput * SPB * 2
-            # margin block = last block of the previous second (zeros at t = 0)
-            self.lib.vmSynthMWAXDevice(SEED_DATA + o.seed_offset, max(t * nb - 1, 0), 1, o.ninput,
-                                       SPB, 16.0, 1 if t == 0 else 0, base)
-            self.lib.vmSynthMWAXDevice(SEED_DATA + o.seed_offset, t * nb, nb, o.ninput, SPB,
-                                       16.0, 0, base + block)
 
     def run(self, use_graph=True):
         self.lib.vmBatchBeamformSecond(self.b, 1 if use_graph else 0)

@@ -154,7 +154,9 @@ void vmCalcPhiDevice(vcsbeam_context *vm, beam_geom *bg)
     }
     ensure((void **)&d->d_unit, &d->unit_bytes, (size_t)np * 3 * sizeof(double));
     HIPCHECK(hipMemcpyAsync(d->d_unit, unit, (size_t)np * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    vm_timer_start(d, VM_T_GEOM);
     vm_calc_phi_launch(d->d_pos, d->d_unit, d->d_freqs, np, nant, nchan, g->d_phi, s);
+    vm_timer_stop(d, VM_T_GEOM);
     HIPCHECK(hipStreamSynchronize(s));
     free(unit);
     d->weights_gen++;
@@ -168,7 +170,9 @@ void vmCalcJDevice(vcsbeam_context *vm)
     size_t bbytes = (size_t)np * nant * 4 * sizeof(gpuDoubleComplex);
     ensure(&d->d_B, &d->B_bytes, bbytes);
     HIPCHECK(hipMemcpyAsync(d->d_B, vm->pb.B, bbytes, hipMemcpyHostToDevice, s));
+    vm_timer_start(d, VM_T_GEOM);
     vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 0, vm->d_J, s);
+    vm_timer_stop(d, VM_T_GEOM);
     HIPCHECK(hipStreamSynchronize(s));
     d->weights_gen++;
 }

@@ -291,6 +291,7 @@ void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_
     }
     cf *din = (cf *)d->d_ipfb_out;
     cf *hist = din + in_elems;
+    vm_timer_start(d, VM_T_IPFB);
     hipLaunchKernelGGL(k_ipfb_gather, dim3((unsigned)((in_elems + 255) / 256)), dim3(256), 0, s,
                        (const cf *)vm->d_e, hist, ns, P, npol, file_no, din);
     HIPCHECK(hipGetLastError());
@@ -298,6 +299,7 @@ void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_
     hipLaunchKernelGGL(k_ipfb_save_hist, dim3((unsigned)((hist_elems + 255) / 256)), dim3(256), 0,
                        s, (const cf *)vm->d_e, ns, P, npol, hist);
     HIPCHECK(hipGetLastError());
+    vm_timer_stop(d, VM_T_IPFB);
     if (data_buffer_vdif)
         HIPCHECK(hipMemcpyAsync(data_buffer_vdif, g->d_out, g->out_size, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
