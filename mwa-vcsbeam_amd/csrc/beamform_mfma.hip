@@ -190,19 +190,19 @@ __global__ void k_jcheck(const double2 *__restrict__ J, size_t per_pointing, siz
 }
 
 // ------------------------------------------------------- voltage prep -----
-// One workgroup per (32-sample block sb, channel c), one pass over the block:
-// lane (sl, g) loads sample 32 sb + sl of both inputs of tiles g, g+8, ...
-// (APT tiles at most), forms u = J^-1 v with Jf = J^-1 (float, [c][a][4]),
-// the auto-correlation sums N = sum_a (|u_x|^2, |u_y|^2, Re u_x u_y^*,
-// Im u_x u_y^*) and the block's largest |component| of u.  The 8 partial
-// sums per sample are reduced in LDS and written as N[c][s]; u, scaled by
+// One workgroup of 32 NG lanes per (32-sample block sb, channel c), one pass
+// over the block: lane (sl, g) loads sample 32 sb + sl of both inputs of
+// tiles g, g+NG, ... (APT tiles at most), forms u = J^-1 v with Jf = J^-1
+// (float, [c][a][4]), the auto-correlation sums N = sum_a (|u_x|^2, |u_y|^2,
+// Re u_x u_y^*, Im u_x u_y^*) and the block's largest |component| of u.  The
+// NG partial sums per sample are reduced in LDS and written as N[c][s]; u, scaled by
 // 2^es, is placed in LDS in fragment order and written out as B fragments
 // Bf[c][sb][ks][pol][lane] (uint4): lane l holds sample 32 sb + l%32 and
 // tiles 8 ks + 4 (l/32) + {0..3} as (re, im) pairs of polarisation pol.
 // Samples >= ns are zero; tiles >= nant are zero.  sexp[c][sb] = es.
-// v: VM_FLT [c][i][s].  Needs nant <= 8 APT.
-template <bool BF16, int APT>
-__global__ __launch_bounds__(256) void k_bf_prep(const cf *__restrict__ v, int ns, int ni, int nsb,
+// v: VM_FLT [c][i][s].  Needs nant <= NG APT.
+template <bool BF16, int APT, int NG>
+__global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, int ns, int ni, int nsb,
                                                  int nkstep, const cf *__restrict__ Jf,
                                                  const uint32_t *__restrict__ polQ,
                                                  const uint32_t *__restrict__ polP, int nant,
@@ -210,9 +210,10 @@ __global__ __launch_bounds__(256) void k_bf_prep(const cf *__restrict__ v, int n
                                                  float4 *__restrict__ N)
 {
     typedef mtype<BF16> M;
-    __shared__ float4 red[8][SB];
-    __shared__ float wmx[4];
-    __shared__ uint4 frag[APT * 2 * 64]; // nkstep (<= APT) x 2 pols x 64 lanes
+    constexpr int NT = 32 * NG; // lanes
+    __shared__ float4 red[NG][SB];
+    __shared__ float wmx[NG / 2];
+    __shared__ uint4 frag[(NG * APT / 8) * 2 * 64]; // nkstep x 2 pols x 64 lanes
     const int sb = blockIdx.x, c = blockIdx.y;
     const int t = threadIdx.x, sl = t & 31, g = t >> 5;
     const int s = SB * sb + sl;
@@ -220,12 +221,12 @@ __global__ __launch_bounds__(256) void k_bf_prep(const cf *__restrict__ v, int n
     const cf *vc = v + (size_t)c * ni * ns;
     const int nfrag = nkstep * 2 * 64;
 
-    for (int k = t; k < nfrag; k += 256) frag[k] = make_uint4(0u, 0u, 0u, 0u);
+    for (int k = t; k < nfrag; k += NT) frag[k] = make_uint4(0u, 0u, 0u, 0u);
 
     cf ux[APT], uy[APT];
 #pragma unroll
     for (int j = 0; j < APT; j++) {
-        const int a = g + 8 * j;
+        const int a = g + NG * j;
         cf q = {0.f, 0.f}, p = {0.f, 0.f};
         if (ok && a < nant) {
             q = vc[(size_t)polQ[a] * ns + s];
@@ -259,13 +260,16 @@ __global__ __launch_bounds__(256) void k_bf_prep(const cf *__restrict__ v, int n
     if (g == 0 && ok) {
         float4 acc = red[0][sl];
 #pragma unroll
-        for (int k = 1; k < 8; k++) {
+        for (int k = 1; k < NG; k++) {
             const float4 r = red[k][sl];
             acc.x += r.x; acc.y += r.y; acc.z += r.z; acc.w += r.w;
         }
         N[(size_t)c * ns + s] = acc;
     }
-    const int es = scale_exp(fmaxf(fmaxf(wmx[0], wmx[1]), fmaxf(wmx[2], wmx[3])));
+    float bm = wmx[0];
+#pragma unroll
+    for (int k = 1; k < NG / 2; k++) bm = fmaxf(bm, wmx[k]);
+    const int es = scale_exp(bm);
     if (t == 0) sexp[(size_t)c * nsb + sb] = es;
     const float scale = ldexpf(1.0f, es);
 
@@ -274,7 +278,7 @@ __global__ __launch_bounds__(256) void k_bf_prep(const cf *__restrict__ v, int n
     uint32_t *fd = (uint32_t *)frag;
 #pragma unroll
     for (int j = 0; j < APT; j++) {
-        const int a = g + 8 * j;
+        const int a = g + NG * j;
         if (a >= nant) break;
         const int base = ((a >> 3) * 2) * 256 + (sl + 32 * ((a >> 2) & 1)) * 4 + (a & 3);
         fd[base] = (uint32_t)M::bits(ux[j].x * scale) | ((uint32_t)M::bits(ux[j].y * scale) << 16);
@@ -283,7 +287,7 @@ __global__ __launch_bounds__(256) void k_bf_prep(const cf *__restrict__ v, int n
     }
     __syncthreads();
     uint4 *dst = Bf + ((size_t)c * nsb + sb) * nfrag;
-    for (int k = t; k < nfrag; k += 256) dst[k] = frag[k];
+    for (int k = t; k < nfrag; k += NT) dst[k] = frag[k];
 }
 
 // ------------------------------------------------------------- GEMM -------
@@ -696,15 +700,15 @@ static void launch_prep(bool bf16, hipStream_t st, const cf *v, int ns, int ni, 
 {
     if (nant > 256) VM_FATAL("MFMA beamformer: at most 256 antennas (%d)", nant);
     const dim3 pg((unsigned)nsb, (unsigned)nchan);
-#define VM_PREP(B_, A_)                                                                       \
-    hipLaunchKernelGGL((k_bf_prep<B_, A_>), pg, dim3(256), 0, st, v, ns, ni, nsb, nkstep, Jf,  \
-                       polQ, polP, nant, Bf, sexp, N)
+#define VM_PREP(B_, A_, G_)                                                                   \
+    hipLaunchKernelGGL((k_bf_prep<B_, A_, G_>), pg, dim3(32 * G_), 0, st, v, ns, ni, nsb,      \
+                       nkstep, Jf, polQ, polP, nant, Bf, sexp, N)
     if (nant <= 64) {
-        if (bf16) VM_PREP(true, 8); else VM_PREP(false, 8);
+        if (bf16) VM_PREP(true, 8, 8); else VM_PREP(false, 8, 8);
     } else if (nant <= 128) {
-        if (bf16) VM_PREP(true, 16); else VM_PREP(false, 16);
+        if (bf16) VM_PREP(true, 8, 16); else VM_PREP(false, 8, 16);
     } else {
-        if (bf16) VM_PREP(true, 32); else VM_PREP(false, 32);
+        if (bf16) VM_PREP(true, 16, 16); else VM_PREP(false, 16, 16);
     }
 #undef VM_PREP
     HIPCHECK(hipGetLastError());
