@@ -58,6 +58,12 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 
 struct cf { float x, y; };
 
+// global-memory qualifier for pointers the compiler cannot otherwise place
+// (loaded from memory rather than passed as kernel arguments): global_*
+// rather than flat_* instructions, so stores do not hold LDS waits
+#define gmem __attribute__((address_space(1)))
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 template <bool BF16> struct mtype;
 template <> struct mtype<false> {
     typedef half8 v8;
@@ -308,8 +314,10 @@ __device__ __forceinline__ uint4 bload(rsrc_t r, int voff, int soff)
     return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
-template <int MB, int NB> struct stage {
-    uint4 a[MB][2];
+template <int MB> struct afrag {
+    uint4 a[MB][2]; // hi, lo
+};
+template <int NB> struct bfrag {
     uint4 b[NB];
 };
 
@@ -317,15 +325,22 @@ template <int MB, int NB> struct stage {
 // k-step ks at + 2048 ks (hi) and + 1024 more (lo); B block n at boff[n],
 // k-step ks at + 2048 ks (the two polarisation fragments of a k-step are
 // adjacent).
-template <int MB, int NB>
-__device__ __forceinline__ void bf_load(stage<MB, NB> &f, rsrc_t rA, rsrc_t rB, int vl, int aoff,
-                                        int astr, const int (&boff)[NB], int ks)
+// (valid false: an offset past any weights buffer, so the loads return zeros)
+template <int MB>
+__device__ __forceinline__ void load_a(afrag<MB> &f, rsrc_t rA, int vl, int aoff, int astr, int ks,
+                                       bool valid)
 {
+    const int base = valid ? aoff + 2048 * ks : 0x7fff0000;
 #pragma unroll
     for (int m = 0; m < MB; m++) {
-        f.a[m][0] = bload(rA, vl, aoff + m * astr + 2048 * ks);
-        f.a[m][1] = bload(rA, vl, aoff + m * astr + 2048 * ks + 1024);
+        f.a[m][0] = bload(rA, vl, base + m * astr);
+        f.a[m][1] = bload(rA, vl, base + m * astr + 1024);
     }
+}
+
+template <int NB>
+__device__ __forceinline__ void load_b(bfrag<NB> &f, rsrc_t rB, int vl, const int (&boff)[NB], int ks)
+{
 #pragma unroll
     for (int n = 0; n < NB; n++) f.b[n] = bload(rB, vl, boff[n] + 2048 * ks);
 }
@@ -344,17 +359,17 @@ __device__ __forceinline__ void interleave_loads()
 }
 
 template <bool BF16, int MB, int NB>
-__device__ __forceinline__ void bf_mma(f16v (&acc)[MB][NB], const stage<MB, NB> &f)
+__device__ __forceinline__ void mma_ab(f16v (&acc)[MB][NB], const afrag<MB> &A, const bfrag<NB> &B)
 {
     // independent hi products first, then the lo products
 #pragma unroll
     for (int m = 0; m < MB; m++)
 #pragma unroll
-        for (int n = 0; n < NB; n++) acc[m][n] = mtype<BF16>::mfma(f.a[m][0], f.b[n], acc[m][n]);
+        for (int n = 0; n < NB; n++) acc[m][n] = mtype<BF16>::mfma(A.a[m][0], B.b[n], acc[m][n]);
 #pragma unroll
     for (int m = 0; m < MB; m++)
 #pragma unroll
-        for (int n = 0; n < NB; n++) acc[m][n] = mtype<BF16>::mfma(f.a[m][1], f.b[n], acc[m][n]);
+        for (int n = 0; n < NB; n++) acc[m][n] = mtype<BF16>::mfma(A.a[m][1], B.b[n], acc[m][n]);
 }
 
 // Stokes (I, Q, U, V) from a beam's (Ex, Ey) = (xr + i xi, yr + i yi): |E|^2
@@ -392,13 +407,16 @@ struct bf_geom {
 // One workgroup tile (tx, ty) of fine channel c: WR x WC wavefronts cover
 // (WR MB 32) rows (16 WR MB beams) x (WC NB/2 32) samples; a wavefront's NB
 // column blocks are NB/2 sample blocks x 2 polarisations (block n = sample
-// block n/2, polarisation n%2).  NST register stages keep NST-1 k-steps of
-// fragments in flight while one is multiplied.
+// block n/2, polarisation n%2).  The B ring keeps NST-1 k-steps of voltage
+// fragments in flight, the A ring one k-step of weights.
 // Epilogue: each (32-row block, sample block) pair is passed through a 4 KB
 // per-wavefront LDS tile (stg + 1024 w), 8 beams at a time, so that every
 // lane stores four consecutive samples of one (beam, Stokes) row with one
 // 16-byte store.
-template <bool BF16, int MB, int NB, int WR, int WC, int NST>
+// VEC: ns, ns_total and soffset are multiples of 4, so every in-range
+// 4-sample group is stored whole with one 16-byte store (the other form
+// stores sample by sample).
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool VEC>
 __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int nchan, int c,
                                         int tx, int ty, float *stg)
 {
@@ -433,55 +451,33 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
 #pragma unroll
             for (int q = 0; q < 16; q++) acc[m][n][q] = 0.f;
 
-    // The scheduling barriers pin each stage's loads ahead of the previous
-    // stage's MFMAs, so the loads stay NST-1 k-steps ahead of their use.
-    // Steady state has no branches; loads past the last k-step re-read it.
+    // Two register rings: A (weights, small and L2-resident) two k-steps
+    // deep, B (voltage fragments, streamed from HBM) NST deep, so each B
+    // fragment is requested NST-1 k-steps before its MFMAs.  The scheduling
+    // barriers pin each step's loads ahead of its MFMAs; a super-step of U
+    // k-steps returns both rings to slot 0, so ring slots are compile-time.
+    // The loop runs whole super-steps with no branches: past the last k-step
+    // the B loads re-read it and the A loads fall outside the weights buffer,
+    // which returns zeros, so those MFMAs add nothing.
 #define VM_SB() __builtin_amdgcn_sched_barrier(0)
-#define VM_LD(f, k) bf_load<MB, NB>(f, rA, rB, vl, aoff, astr, boff, min((k), kl))
-#define VM_MM(f) bf_mma<BF16, MB, NB>(acc, f)
-#define VM_STEP(fl, k, fm)                                                                    \
-    VM_SB();                                                                                  \
-    VM_LD(fl, k);                                                                             \
-    VM_MM(fm);                                                                                \
-    interleave_loads<2 * MB + NB, 2 * MB * NB>();                                            \
-    VM_SB()
-    stage<MB, NB> f0, f1, f2, f3;
+    constexpr int U = (NST % 2 == 0) ? NST : 2 * NST;
+    afrag<MB> fa[2];
+    bfrag<NB> fb[NST];
     const int kl = nkstep - 1;
-    int ks = 0;
-    if constexpr (NST == 4) {
-        VM_LD(f0, 0); VM_LD(f1, 1); VM_LD(f2, 2);
-        for (; ks + 4 <= nkstep; ks += 4) {
-            VM_STEP(f3, ks + 3, f0);
-            VM_STEP(f0, ks + 4, f1);
-            VM_STEP(f1, ks + 5, f2);
-            VM_STEP(f2, ks + 6, f3);
+    load_a<MB>(fa[0], rA, vl, aoff, astr, 0, true);
+#pragma unroll
+    for (int r = 0; r < NST - 1; r++) load_b<NB>(fb[r], rB, vl, boff, min(r, kl));
+    for (int ks = 0; ks < nkstep; ks += U) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            VM_SB();
+            load_a<MB>(fa[(u + 1) & 1], rA, vl, aoff, astr, ks + u + 1, ks + u + 1 <= kl);
+            load_b<NB>(fb[(u + NST - 1) % NST], rB, vl, boff, min(ks + u + NST - 1, kl));
+            mma_ab<BF16, MB, NB>(acc, fa[u & 1], fb[u % NST]);
+            interleave_loads<2 * MB + NB, 2 * MB * NB>();
+            VM_SB();
         }
-        VM_SB();
-        if (ks < nkstep) VM_MM(f0);
-        if (ks + 1 < nkstep) VM_MM(f1);
-        if (ks + 2 < nkstep) VM_MM(f2);
-    } else if constexpr (NST == 3) {
-        VM_LD(f0, 0); VM_LD(f1, 1);
-        for (; ks + 3 <= nkstep; ks += 3) {
-            VM_STEP(f2, ks + 2, f0);
-            VM_STEP(f0, ks + 3, f1);
-            VM_STEP(f1, ks + 4, f2);
-        }
-        VM_SB();
-        if (ks < nkstep) VM_MM(f0);
-        if (ks + 1 < nkstep) VM_MM(f1);
-    } else {
-        VM_LD(f0, 0);
-        for (; ks + 2 <= nkstep; ks += 2) {
-            VM_STEP(f1, ks + 1, f0);
-            VM_STEP(f0, ks + 2, f1);
-        }
-        VM_SB();
-        if (ks < nkstep) VM_MM(f0);
     }
-#undef VM_STEP
-#undef VM_MM
-#undef VM_LD
 #undef VM_SB
 
     // epilogue: in a 32-row block lane (lr, lk) holds rows 8 jg + 4 lk + {0..3}
@@ -496,17 +492,20 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
     const float invw = G.invw;
     const size_t pstride = (size_t)nstokes * nchan * G.ns_total;
     const size_t ststride = (size_t)nchan * G.ns_total;
-    const bool vec = ((G.ns_total | G.soffset) & 3) == 0;
 #pragma unroll
     for (int j = 0; j < NSB; j++) {
         const int sb = sb0 + j;
         const int s = SB * sb + lr;
         float4 nn = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (s < ns) nn = G.N[(size_t)c * ns + s];
+        if (s < ns) {
+            const f4v n4 = ((const gmem f4v *)G.N)[(size_t)c * ns + s];
+            nn = make_float4(n4.x, n4.y, n4.z, n4.w);
+        }
         const float unscale2 =
-            ldexpf(1.0f, -2 * (G.sexp[(size_t)c * nsb + min(sb, nsb - 1)] + PHASE_EXP));
+            ldexpf(1.0f, -2 * (((const gmem int *)G.sexp)[(size_t)c * nsb + min(sb, nsb - 1)] +
+                               PHASE_EXP));
         const int s4 = SB * sb + 4 * qq;
-        float *Sc = G.S + (size_t)c * G.ns_total + G.soffset + s4;
+        gmem float *Sc = (gmem float *)G.S + (size_t)c * G.ns_total + G.soffset + s4;
 #pragma unroll
         for (int m = 0; m < MB; m++)
 #pragma unroll
@@ -534,9 +533,9 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                     const int p = 16 * (rb0 + m) + 8 * h + bb;
                     const float4 val = *(const float4 *)(wl + r * 32 + 4 * qq);
                     if (st >= nstokes || p >= npointing || s4 >= ns) continue;
-                    float *dst = Sc + (size_t)p * pstride + (size_t)st * ststride;
-                    if (vec && s4 + 3 < ns) {
-                        *(float4 *)dst = val;
+                    gmem float *dst = Sc + (size_t)p * pstride + (size_t)st * ststride;
+                    if constexpr (VEC) {
+                        *(gmem f4v *)dst = f4v{val.x, val.y, val.z, val.w};
                     } else {
                         dst[0] = val.x;
                         if (s4 + 1 < ns) dst[1] = val.y;
@@ -554,7 +553,7 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
 // the same tile of 8 different channels, so each XCD walks its own channels;
 // within a channel tiles go in bands of gr row tiles, column by column (gr =
 // all row tiles for order 1, 1 for order 2, 2 for order 3, 4 for order 4).
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC>
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC, bool VEC>
 __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, int ns, int nsb,
                                                                      int nchan, int ntx, int nty,
                                                                      int order)
@@ -575,7 +574,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, 
     if (nty % gr) gr = 1;
     const int band = tile / (gr * ntx), rem = tile % (gr * ntx);
     const int tx = rem / gr, ty = band * gr + rem % gr;
-    bf_tile<BF16, MB, NB, WR, WC, NST>(G, ns, nsb, nchan, c, tx, ty, stg);
+    bf_tile<BF16, MB, NB, WR, WC, NST, VEC>(G, ns, nsb, nchan, c, tx, ty, stg);
 }
 
 // Batched form: one persistent launch over the GEMM problems of several
@@ -585,7 +584,8 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, 
 // y = tx << 16 | ty0 << 8 | cnt and listed in the order they should start
 // (most expensive first).  Workgroups take items from a queue counter that
 // must be zero at launch; every workgroup leaves the loop when the counter
-// passes nitems, so the grid always drains.
+// passes nitems, so the grid always drains.  Every channel's ns, ns_total
+// and sample offset are multiples of 4 (16-byte Stokes stores).
 template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC>
 __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
     const bf_geom *__restrict__ geoms, const uint2 *__restrict__ items, int nitems,
@@ -608,18 +608,19 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
         const int cnt = __builtin_amdgcn_readfirstlane((int)(e.y & 0xffu));
         const bf_geom G = geoms[g];
         for (int ty = ty0; ty < ty0 + cnt; ty++)
-            bf_tile<BF16, MB, NB, WR, WC, NST>(G, ns, nsb, nchan, c, tx, ty, stg);
+            bf_tile<BF16, MB, NB, WR, WC, NST, true>(G, ns, nsb, nchan, c, tx, ty, stg);
     }
 }
 
 // Kernel variants the autotuner chooses from: register tile per wavefront
 // (MB row blocks of 32 x NB column blocks of 32, NB = 2 x sample blocks),
-// wavefront grid per workgroup (WR x WC), register stages and the minimum
-// wavefronts per SIMD the registers must allow.
+// wavefront grid per workgroup (WR x WC), depth of the B register ring (the
+// A ring is two deep) and the minimum wavefronts per SIMD the registers must
+// allow.
 struct bf_variant { int mb, nb, wr, wc, nst, occ; };
 static const bf_variant kVariants[] = {
-    {4, 4, 2, 2, 3, 1}, {4, 4, 1, 4, 3, 1}, {4, 4, 2, 2, 4, 1}, {2, 4, 2, 2, 3, 2},
-    {4, 2, 2, 2, 3, 2}, {2, 4, 4, 1, 3, 2}, {4, 4, 4, 1, 3, 1},
+    {4, 4, 2, 2, 4, 1}, {4, 4, 1, 4, 4, 1}, {4, 4, 1, 4, 6, 1}, {2, 4, 2, 2, 4, 2},
+    {4, 2, 2, 2, 4, 2}, {2, 4, 4, 1, 4, 2}, {4, 4, 4, 1, 4, 1}, {4, 4, 2, 2, 6, 1},
 };
 #define N_VARIANTS ((int)(sizeof(kVariants) / sizeof(kVariants[0])))
 #define VM_N_ORDERS 5
@@ -628,17 +629,27 @@ template <bool BF16>
 static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, int ns, int nsb,
                            int nchan, int ntx, int nty, int order)
 {
+    if ((ns | G.ns_total | G.soffset) & 3) {
+        // sample counts not a multiple of 4: the element-wise store form,
+        // built for the first variant's shape only (its grid is the caller's
+        // when vi = 0)
+        if (vi != 0) VM_FATAL("MFMA beamformer: variant %d needs ns %% 4 == 0", vi);
+        hipLaunchKernelGGL((k_beamform_mfma<BF16, 4, 4, 2, 2, 4, 1, false>), grid, dim3(256), 0,
+                           st, G, ns, nsb, nchan, ntx, nty, order);
+        return;
+    }
 #define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, OCC_)                                             \
-    hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, OCC_>), grid,        \
+    hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, OCC_, true>), grid,  \
                        dim3(64 * WR_ * WC_), 0, st, G, ns, nsb, nchan, ntx, nty, order)
     switch (vi) {
-    case 0: VM_LAUNCH(4, 4, 2, 2, 3, 1); break;
-    case 1: VM_LAUNCH(4, 4, 1, 4, 3, 1); break;
-    case 2: VM_LAUNCH(4, 4, 2, 2, 4, 1); break;
-    case 3: VM_LAUNCH(2, 4, 2, 2, 3, 2); break;
-    case 4: VM_LAUNCH(4, 2, 2, 2, 3, 2); break;
-    case 5: VM_LAUNCH(2, 4, 4, 1, 3, 2); break;
-    default: VM_LAUNCH(4, 4, 4, 1, 3, 1); break;
+    case 0: VM_LAUNCH(4, 4, 2, 2, 4, 1); break;
+    case 1: VM_LAUNCH(4, 4, 1, 4, 4, 1); break;
+    case 2: VM_LAUNCH(4, 4, 1, 4, 6, 1); break;
+    case 3: VM_LAUNCH(2, 4, 2, 2, 4, 2); break;
+    case 4: VM_LAUNCH(4, 2, 2, 2, 4, 2); break;
+    case 5: VM_LAUNCH(2, 4, 4, 1, 4, 2); break;
+    case 6: VM_LAUNCH(4, 4, 4, 1, 4, 1); break;
+    default: VM_LAUNCH(4, 4, 2, 2, 6, 1); break;
     }
 #undef VM_LAUNCH
 }
@@ -652,13 +663,14 @@ static void launch_batch_variant(int vi, unsigned nwg, hipStream_t st, const bf_
     hipLaunchKernelGGL((k_beamform_batch<BF16, MB_, NB_, WR_, WC_, NST_, OCC_>), dim3(nwg),  \
                        dim3(64 * WR_ * WC_), 0, st, geoms, items, nitems, queue, ns, nsb, nchan)
     switch (vi) {
-    case 0: VM_LAUNCH(4, 4, 2, 2, 3, 1); break;
-    case 1: VM_LAUNCH(4, 4, 1, 4, 3, 1); break;
-    case 2: VM_LAUNCH(4, 4, 2, 2, 4, 1); break;
-    case 3: VM_LAUNCH(2, 4, 2, 2, 3, 2); break;
-    case 4: VM_LAUNCH(4, 2, 2, 2, 3, 2); break;
-    case 5: VM_LAUNCH(2, 4, 4, 1, 3, 2); break;
-    default: VM_LAUNCH(4, 4, 4, 1, 3, 1); break;
+    case 0: VM_LAUNCH(4, 4, 2, 2, 4, 1); break;
+    case 1: VM_LAUNCH(4, 4, 1, 4, 4, 1); break;
+    case 2: VM_LAUNCH(4, 4, 1, 4, 6, 1); break;
+    case 3: VM_LAUNCH(2, 4, 2, 2, 4, 2); break;
+    case 4: VM_LAUNCH(4, 2, 2, 2, 4, 2); break;
+    case 5: VM_LAUNCH(2, 4, 4, 1, 4, 2); break;
+    case 6: VM_LAUNCH(4, 4, 4, 1, 4, 1); break;
+    default: VM_LAUNCH(4, 4, 2, 2, 6, 1); break;
     }
 #undef VM_LAUNCH
 }
@@ -809,6 +821,8 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
 
 static bool variant_ok(int vi, const mfma_plan *pl)
 {
+    // sample counts that are not multiples of 4 have only variant 0
+    if (pl->ns & 3 && vi != 0) return false;
     return vi >= 0 && vi < N_VARIANTS && pl->nrb % (kVariants[vi].mb * kVariants[vi].wr) == 0;
 }
 
@@ -1194,6 +1208,7 @@ vm_batch *vmBatchCreate(vcsbeam_context **vms, int n)
     const vcsbeam_context *v0 = vms[0];
     b->nchan = v0->nfine_chan;
     b->ns = (int)v0->fine_sample_rate;
+    if (b->ns & 3) VM_FATAL("vmBatchCreate: %d samples per second (a multiple of 4 is needed)", b->ns);
     b->nsb = (b->ns + SB - 1) / SB;
     b->bf16 = v0->beamform_form == VM_BF_MFMA_BF16;
     size_t vneed = 0;
