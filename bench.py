@@ -361,6 +361,7 @@ def pipeline_stage(pkg, lib, nchan=24, steps=2):
         second(0)
         for x in sess:
             lib.vmSetKernelTiming(x.vm, 1)
+        lib.vmSetKernelTiming(None, 1)
         lib.vmDeviceSynchronize()
         t0 = time.perf_counter()
         for k in range(steps):
@@ -373,7 +374,11 @@ def pipeline_stage(pkg, lib, nchan=24, steps=2):
                 avg, n = kernel_ms(lib, x, name)
                 tot += avg * n / steps
             stages[name + "_ms"] = round(tot, 3)
+        vd_ms, vd_n = kernel_ms_global(lib, "vdif")
+        lib.vmSetKernelTiming(None, 0)
+        stages["vdif_ms"] = round(vd_ms * vd_n / steps, 3)
         ipfb_bytes = nchan * nspc * 128 * 2 * 16  # 8 B fine in + 8 B coarse out per sample-pol
+        vdif_bytes = nchan * nspc * 128 * 2 * 10  # 8 B in, 2 B out per sample-pol
         return {"workload": f"{nchan} coarse channels x 1 s, 128 tiles x 2 pols, one beam: FP32 "
                             f"PFB, FP32 beamformer (Stokes I), inverse PFB, 8-bit VDIF, "
                             f"device-resident",
@@ -383,11 +388,77 @@ def pipeline_stage(pkg, lib, nchan=24, steps=2):
                 "pfb_hbm_frac": round(nchan * 2 * 128 * nspc * 128 * 10 /
                                       (stages["pfb_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "ipfb_hbm_frac": round(ipfb_bytes / (stages["ipfb_ms"] * 1e-3) / 1e9
+                                       / HBM_PEAK_GBS, 4),
+                "vdif_hbm_frac": round(vdif_bytes / (stages["vdif_ms"] * 1e-3) / 1e9
                                        / HBM_PEAK_GBS, 4)}
     finally:
         for x in sess:
             x.close()
         del dv
+
+
+def aux_stage(pkg, lib, reps=5):
+    """Row kernels outside the beamformer at one coarse channel-second of 128
+    tiles: the incoherent beam (cu_form_incoh_beam on 4+4-bit legacy input
+    [s][c][i]: the 4+4 unpack and the power sum) and the 8-bit renormalise of
+    one beam's Stokes IQUV (vmSendSToFits), each with its HBM roofline."""
+    import ctypes as C
+    from vcsbeam_amd.apps import TiedArraySession, device_second
+    from vcsbeam_amd.workloads import Observation, legacy_int4
+    ns, nch, ni = 10000, 128, 256
+    data = legacy_int4(ns * nch * ni)
+    h, d = C.c_void_p(), C.c_void_p()
+    lib.allocate_input_output_arrays(C.byref(h), C.byref(d), data.nbytes)
+    C.memmove(h, data.ctypes.data, data.nbytes)
+    d_incoh, d_off, d_scl, d_Is = (pkg.DeviceArray(n) for n in (ns * nch * 4, nch * 4, nch * 4,
+                                                                ns * nch))
+    off, scl, Is = np.zeros(nch, np.float32), np.zeros(nch, np.float32), np.zeros(ns * nch, np.uint8)
+    call = lambda: lib.cu_form_incoh_beam(h, d, data.nbytes, d_incoh.ptr, ns, nch, ni,
+                                          off.ctypes.data, d_off.ptr, scl.ctypes.data, d_scl.ptr,
+                                          Is.ctypes.data, d_Is.ptr, Is.nbytes)
+    call()
+    lib.vmSetKernelTiming(None, 1)
+    for _ in range(reps):
+        call()
+    incoh_ms, _ = kernel_ms_global(lib, "incoh")
+    lib.vmSetKernelTiming(None, 0)
+    lib.free_input_output_arrays(h, d)
+    incoh_bytes = data.nbytes + ns * nch * 4
+    obs = Observation(ntiles=128, blocks_per_second=20)
+    ses = TiedArraySession(obs, np.array([0.7]), np.radians([20.0]),
+                           pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA, nstokes=4,
+                           form=pkg.VM_BF_FP32, host_buffers=False)
+    try:
+        device_second(ses, 0)
+        ses.pfb()
+        ses.geometry()
+        ses.beamform()
+        lib.vmCreateStatistics(ses.vm, None)
+        lib.vmSendSToFits(ses.vm, None)
+        lib.vmSetKernelTiming(ses.vm, 1)
+        for _ in range(reps):
+            lib.vmSendSToFits(ses.vm, None)
+        renorm_ms, _ = kernel_ms(lib, ses, "renormalise")
+        lib.vmSetKernelTiming(ses.vm, 0)
+        lib.vmDestroyStatistics(ses.vm)
+    finally:
+        ses.close()
+    renorm_bytes = 2 * 4 * nch * ns * 4 + 4 * nch * ns  # S read twice, uint8 out
+    return {"workload": "one coarse channel-second, 128 tiles x 2 pols",
+            "incoh_ms": round(incoh_ms, 4),
+            "incoh_hbm_frac": round(incoh_bytes / (incoh_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "incoh_algorithmic_bytes": incoh_bytes,
+            "renormalise_ms": round(renorm_ms, 4),
+            "renormalise_hbm_frac": round(renorm_bytes / (renorm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                          4),
+            "renormalise_algorithmic_bytes": renorm_bytes}
+
+
+def kernel_ms_global(lib, name):
+    import ctypes as C
+    n = C.c_long(0)
+    ms = lib.vmKernelTimeMs(None, name.encode(), C.byref(n))
+    return ms, n.value
 
 
 def main():
@@ -403,6 +474,8 @@ def main():
                     help="skip the batched 24-channel uneven workload")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched 24-channel workload")
+    ap.add_argument("--no-aux", action="store_true",
+                    help="skip the incoherent-beam and renormalise timings")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="skip the 24-channel plain pipeline with inverse PFB")
     ap.add_argument("--pipeline-only", action="store_true",
@@ -437,7 +510,7 @@ def main():
         print(json.dumps({"batch": batch_stage(pkg, lib)}))
         return
     if args.pipeline_only:
-        print(json.dumps({"pipeline": pipeline_stage(pkg, lib)}))
+        print(json.dumps({"pipeline": pipeline_stage(pkg, lib), "aux": aux_stage(pkg, lib)}))
         return
     from vcsbeam_amd.apps import TiedArraySession
     from vcsbeam_amd.workloads import Observation, directions_in_cap
@@ -557,6 +630,9 @@ def main():
         print("[bench] plain pipeline with inverse PFB (24 channels x 128 tiles)",
               file=sys.stderr, flush=True)
         result["pipeline"] = pipeline_stage(pkg, lib)
+    if not args.no_aux:
+        print("[bench] incoherent beam and renormalise", file=sys.stderr, flush=True)
+        result["aux"] = aux_stage(pkg, lib)
     if args.calibrate_copy:
         a, b = pkg.DeviceArray(COPY_BYTES), pkg.DeviceArray(COPY_BYTES)
         lib.vmDeviceCopy(b.ptr, a.ptr, COPY_BYTES)

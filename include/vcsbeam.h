@@ -801,8 +801,11 @@ void vmEventDestroy(void *ev);
 
 /* Per-kernel timing with HIP events on the context stream: when on, each
  * launch of the named kernels ("pfb", "beamform_fp32", "beamform_mfma",
- * "nterms", "ipfb", "geometry") is bracketed by events; vmKernelTimeMs returns
- * the average milliseconds per launch since timing was switched on. */
+ * "beamform_prep", "beamform_weights", "ipfb", "geometry", "renormalise") is
+ * bracketed by events; vmKernelTimeMs returns the average milliseconds per
+ * launch since timing was switched on.  With vm NULL they apply to the calls
+ * that take no context: "incoh" (cu_form_incoh_beam's sum) and "vdif"
+ * (vmRequantVDIFDevice). */
 void vmSetKernelTiming(vcsbeam_context *vm, int on);
 double vmKernelTimeMs(vcsbeam_context *vm, const char *kernel, long *count);
 

@@ -156,32 +156,41 @@ __global__ __launch_bounds__(256) void k_beamform_fp32(
 }
 
 // ----------------------------------------------------- incoherent beam ----
-// One wavefront per (s, c): lane j reads dword j of the 256-input row.
-// I[s][c] = sum_i |v|^2, exact integer arithmetic (src/form_beam.cpp:60-87).
+// I[s][c] = sum_i |v|^2 over all inputs, exact integer arithmetic
+// (src/form_beam.cpp:60-87).  16 lanes per (s, c) row, 16 bytes per lane
+// per load, so one wavefront load covers 4 rows = 1 KB.  With the other
+// nibble masked off, a byte is (16 re) or (16 im) as a signed int8, so
+// v_dot4_i32_i8 of the masked word with itself gives 256 sum(re^2) (resp.
+// im^2) over its 4 bytes, exactly; the row sum is shifted down at the end.
+__device__ __forceinline__ int incoh_sq4(uint32_t w)
+{
+    const uint32_t hi = w & 0xF0F0F0F0u, lo = (w << 4) & 0xF0F0F0F0u;
+    return __builtin_amdgcn_sdot4((int)hi, (int)hi, __builtin_amdgcn_sdot4((int)lo, (int)lo, 0, false),
+                                  false);
+}
+
 __global__ __launch_bounds__(256) void k_incoh(const uint8_t *__restrict__ data, int nsample,
                                                int nchan, int ninput, float *__restrict__ incoh)
 {
-    const int lane = threadIdx.x & 63;
-    const size_t row = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); // = s*nchan + c
-    if (row >= (size_t)nsample * nchan) return;
-    const uint8_t *r = data + row * ninput;
-    int acc = 0;
-    for (int j = lane * 4; j < ninput; j += 256) {
-        uint32_t w = (j + 3 < ninput) ? *(const uint32_t *)(r + j) : 0u;
-        if (j + 3 >= ninput)
-            for (int q = 0; q < 4 && j + q < ninput; q++) w |= (uint32_t)r[j + q] << (8 * q);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            int b = (w >> (8 * q)) & 0xff;
-            int re = (b >> 4) & 0xf, im = b & 0xf;
-            re = re >= 8 ? re - 16 : re;
-            im = im >= 8 ? im - 16 : im;
-            acc += re * re + im * im;
+    const int g = threadIdx.x & 15;
+    const size_t nrows = (size_t)nsample * nchan;
+    const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 4;
+    for (size_t row = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; row < nrows;
+         row += stride) {
+        const uint8_t *r = data + row * ninput;
+        int acc = 0; // 256 x the row's sum
+        if ((ninput & 15) == 0) {
+            for (int j = 16 * g; j < ninput; j += 256) {
+                const uint4 w = *(const uint4 *)(r + j);
+                acc += incoh_sq4(w.x) + incoh_sq4(w.y) + incoh_sq4(w.z) + incoh_sq4(w.w);
+            }
+        } else {
+            for (int j = g; j < ninput; j += 16) acc += incoh_sq4((uint32_t)r[j]);
         }
-    }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) incoh[row] = (float)acc;
+        for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+        if (g == 0) incoh[row] = (float)(acc >> 8);
+    }
 }
 
 // --------------------------------------------------------- renormalise ----
@@ -374,9 +383,14 @@ void cu_form_incoh_beam(uint8_t *data, uint8_t *d_data, size_t data_size, float 
     // src/form_beam.cpp:463-505 (all copies complete before return)
     HIPCHECK(hipMemcpy(d_data, data, data_size, hipMemcpyHostToDevice));
     size_t rows = (size_t)nsample * nchan;
-    hipLaunchKernelGGL(k_incoh, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, 0, d_data,
-                       (int)nsample, nchan, ninput, d_incoh);
+    vm_timer_start_on(vm_dev_global(), VM_T_INCOH, NULL);
+    // 16 lanes per row; about 16 rows per lane group
+    const size_t want = (rows + 255) / 256;
+    const unsigned nblk = (unsigned)(want < 65536 ? want : 65536);
+    hipLaunchKernelGGL(k_incoh, dim3(nblk), dim3(256), 0, 0, d_data, (int)nsample, nchan, ninput,
+                       d_incoh);
     HIPCHECK(hipGetLastError());
+    vm_timer_stop_on(vm_dev_global(), VM_T_INCOH, NULL);
     hipLaunchKernelGGL(k_renorm_ref, dim3((nchan + 63) / 64, 1, 1), dim3(64, 16), 0, 0, d_incoh,
                        (int)nsample, 1, nchan, d_offsets, d_scales, d_Iscaled);
     HIPCHECK(hipGetLastError());
@@ -515,6 +529,7 @@ void vmSendSToFits(vcsbeam_context *vm, mpi_psrfits *mpfs)
     // the sub-integration arrays into the PSRFITS structs
     int ns = vm->fine_sample_rate, nchan = vm->nfine_chan, nst = vm->out_nstokes;
     hipStream_t s = vm_stream(vm);
+    vm_timer_start(vm_dev(vm), VM_T_RENORM);
     hipLaunchKernelGGL(k_minmax_dev, dim3(vm->npointing * nst * nchan), dim3(256), 0, s,
                        (const float *)vm->d_S, ns, vm->d_offsets, vm->d_scales);
     HIPCHECK(hipGetLastError());
@@ -522,6 +537,7 @@ void vmSendSToFits(vcsbeam_context *vm, mpi_psrfits *mpfs)
     hipLaunchKernelGGL(k_quant_dev, grid, dim3(256), 0, s, (const float *)vm->d_S, ns, nst, nchan,
                        vm->d_offsets, vm->d_scales, vm->d_Cscaled);
     HIPCHECK(hipGetLastError());
+    vm_timer_stop(vm_dev(vm), VM_T_RENORM);
     HIPCHECK(hipMemcpyAsync(vm->offsets, vm->d_offsets, vm->offsets_size, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(vm->scales, vm->d_scales, vm->scales_size, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(vm->Cscaled, vm->d_Cscaled,

@@ -308,9 +308,11 @@ void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_
 void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n, gpuStream_t stream)
 {
     size_t lanes = (n + 3) / 4;
+    vm_timer_start_on(vm_dev_global(), VM_T_VDIF, stream);
     hipLaunchKernelGGL(k_vdif_requant, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, d_in, d_out, n);
     HIPCHECK(hipGetLastError());
+    vm_timer_stop_on(vm_dev_global(), VM_T_VDIF, stream);
 }
 
 } // extern "C"

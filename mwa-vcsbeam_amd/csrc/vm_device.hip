@@ -477,24 +477,32 @@ void destroy_vcsbeam_context(vcsbeam_context *vm)
 /* ------------------------------------------------------ kernel timing ---- */
 extern "C" {
 
-static const char *vm_timer_names[VM_T_NSLOT] = {"pfb",           "beamform_fp32", "beamform_mfma",
-                                                 "beamform_prep", "ipfb",          "geometry",
-                                                 "beamform_weights"};
+static const char *vm_timer_names[VM_T_NSLOT] = {
+    "pfb",      "beamform_fp32",    "beamform_mfma", "beamform_prep", "ipfb",
+    "geometry", "beamform_weights", "incoh",         "renormalise",   "vdif"};
 
-void vm_timer_start(vm_device_state *d, int slot)
+/* timing state of the calls that take no context (their own streams) */
+static vm_device_state g_dev_global;
+vm_device_state *vm_dev_global(void) { return &g_dev_global; }
+
+void vm_timer_start_on(vm_device_state *d, int slot, void *stream)
 {
     if (!d->timing) return;
     if (!d->ev[slot][0]) {
         HIPCHECK(hipEventCreate((hipEvent_t *)&d->ev[slot][0]));
         HIPCHECK(hipEventCreate((hipEvent_t *)&d->ev[slot][1]));
     }
-    HIPCHECK(hipEventRecord((hipEvent_t)d->ev[slot][0], (hipStream_t)d->stream));
+    HIPCHECK(hipEventRecord((hipEvent_t)d->ev[slot][0], (hipStream_t)stream));
 }
 
-void vm_timer_stop(vm_device_state *d, int slot)
+void vm_timer_start(vm_device_state *d, int slot) { vm_timer_start_on(d, slot, d->stream); }
+
+void vm_timer_stop(vm_device_state *d, int slot) { vm_timer_stop_on(d, slot, d->stream); }
+
+void vm_timer_stop_on(vm_device_state *d, int slot, void *stream)
 {
     if (!d->timing || !d->ev[slot][0]) return;
-    HIPCHECK(hipEventRecord((hipEvent_t)d->ev[slot][1], (hipStream_t)d->stream));
+    HIPCHECK(hipEventRecord((hipEvent_t)d->ev[slot][1], (hipStream_t)stream));
     HIPCHECK(hipEventSynchronize((hipEvent_t)d->ev[slot][1]));
     float ms = 0.f;
     HIPCHECK(hipEventElapsedTime(&ms, (hipEvent_t)d->ev[slot][0], (hipEvent_t)d->ev[slot][1]));
@@ -504,7 +512,7 @@ void vm_timer_stop(vm_device_state *d, int slot)
 
 void vmSetKernelTiming(vcsbeam_context *vm, int on)
 {
-    vm_device_state *d = vm_dev(vm);
+    vm_device_state *d = vm ? vm_dev(vm) : vm_dev_global();
     d->timing = on;
     if (!on) return; /* keep the counts for vmKernelTimeMs */
     for (int i = 0; i < VM_T_NSLOT; i++) {
@@ -515,7 +523,7 @@ void vmSetKernelTiming(vcsbeam_context *vm, int on)
 
 double vmKernelTimeMs(vcsbeam_context *vm, const char *kernel, long *count)
 {
-    vm_device_state *d = vm_dev(vm);
+    vm_device_state *d = vm ? vm_dev(vm) : vm_dev_global();
     for (int i = 0; i < VM_T_NSLOT; i++)
         if (strcmp(kernel, vm_timer_names[i]) == 0) {
             if (count) *count = d->t_n[i];

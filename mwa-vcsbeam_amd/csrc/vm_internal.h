@@ -59,9 +59,9 @@ struct vm_device_state {
     int ncu;
     /* per-kernel event timing (vmSetKernelTiming) */
     int timing;
-    void *ev[8][2];
-    double t_ms[8];
-    long t_n[8];
+    void *ev[16][2];
+    double t_ms[16];
+    long t_n[16];
 };
 
 /* kernel timing slots */
@@ -73,11 +73,18 @@ enum {
     VM_T_IPFB,
     VM_T_GEOM,
     VM_T_BF_WEIGHTS,
+    VM_T_INCOH,
+    VM_T_RENORM,
+    VM_T_VDIF,
     VM_T_NSLOT
 };
 /* bracket a launch on the context stream with events when timing is on */
 void vm_timer_start(vm_device_state *d, int slot);
 void vm_timer_stop(vm_device_state *d, int slot);
+/* the same on another stream (calls without a context use vm_dev_global()) */
+void vm_timer_start_on(vm_device_state *d, int slot, void *stream);
+void vm_timer_stop_on(vm_device_state *d, int slot, void *stream);
+vm_device_state *vm_dev_global(void);
 
 /* Host helpers implemented in vm_host.c */
 vm_device_state *vm_dev(vcsbeam_context *vm); /* create on first use */
