@@ -109,6 +109,117 @@ __device__ __forceinline__ int scale_exp(float mx)
 // scale of the phase weights: |Re phi|, |Im phi| <= 1
 #define PHASE_EXP 13
 
+// ------------------------------------------- per-pointing primary beam ----
+// With a primary beam B[p][c] that is the same for every tile but differs
+// between pointings (vmCalcJDeviceChanB), J^-1_pac = F_pac B_pc^-1 D_ac^-1
+// with F_pac = ||D_ac B_pc||_F (src/jones.c:97-157), so for every pointing
+//   E_p = B_p^-1 sum_a (phi_pa F_pa) u_a,   u_a = D_a^-1 v_a,
+// i.e. the same GEMM with weights phi F and a 2x2 B^-1 in the epilogue, and
+//   N_p = B_p^-1 (sum_a F_pa^2 u_a u_a^H) B_p^-H.
+// F_pa^2 = tr(P_a Q_p), P_a = D_a^H D_a, Q_p = B_p B_p^H, which for Hermitian
+// 2x2 matrices is sum_k g_a[k] h_p[k] with g = (P00, P11, Re P01, Im P01)
+// and h = (Q00, Q11, 2 Re Q01, 2 Im Q01); hence
+//   sum_a F_pa^2 u_a u_a^H = sum_k h_p[k] T_k,  T_k = sum_a g_a[k] u_a u_a^H,
+// and the four T_k are beam-independent sums formed in the prep pass.  The
+// epilogue forms X = B^-1 (|E'|^2 - sum_k h_k T_k) B^-H per beam and sample.
+// The weights of pointing p are scaled by 2^esp[c][p], chosen so that
+// max_a F_pa 2^esp lies in [2^12, 2^13).
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// ||D B||_F^2 of row-major 2x2 complex matrices
+__device__ __forceinline__ double frob2_DB(const double2 *D, const double2 *B)
+{
+    double f = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const double2 u = zmul(D[2 * i], B[k]), w = zmul(D[2 * i + 1], B[2 + k]);
+            const double re = u.x + w.x, im = u.y + w.y;
+            f += re * re + im * im;
+        }
+    return f;
+}
+
+// Per (c, a): D^-1 as float [c][a][4] (zero for a singular D, whose J^-1 is
+// zero) and g of P = D^H D as float4 [c][a].  D: [a][c][4] complex double.
+__global__ void k_pb_dinv(const double2 *__restrict__ D, int nant, int nchan, cf *__restrict__ Dinv,
+                          float4 *__restrict__ g)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)nant * nchan) return;
+    const int c = (int)(j % nchan), a = (int)(j / nchan);
+    const double2 *m = D + ((size_t)a * nchan + c) * 4;
+    const double2 d00 = m[0], d01 = m[1], d10 = m[2], d11 = m[3];
+    const double2 det = make_double2(d00.x * d11.x - d00.y * d11.y - (d01.x * d10.x - d01.y * d10.y),
+                                     d00.x * d11.y + d00.y * d11.x - (d01.x * d10.y + d01.y * d10.x));
+    const double dd = det.x * det.x + det.y * det.y;
+    cf *o = Dinv + ((size_t)c * nant + a) * 4;
+    if (dd == 0.0) {
+        o[0] = o[1] = o[2] = o[3] = cf{0.f, 0.f};
+    } else {
+        const double2 r = make_double2(det.x / dd, -det.y / dd); // 1 / det
+        const double2 e00 = zmul(d11, r), e11 = zmul(d00, r);
+        const double2 e01 = zmul(make_double2(-d01.x, -d01.y), r);
+        const double2 e10 = zmul(make_double2(-d10.x, -d10.y), r);
+        o[0] = cf{(float)e00.x, (float)e00.y};
+        o[1] = cf{(float)e01.x, (float)e01.y};
+        o[2] = cf{(float)e10.x, (float)e10.y};
+        o[3] = cf{(float)e11.x, (float)e11.y};
+    }
+    // P = D^H D: P00 = |d00|^2 + |d10|^2, P11 = |d01|^2 + |d11|^2,
+    // P01 = conj(d00) d01 + conj(d10) d11
+    const double p00 = d00.x * d00.x + d00.y * d00.y + d10.x * d10.x + d10.y * d10.y;
+    const double p11 = d01.x * d01.x + d01.y * d01.y + d11.x * d11.x + d11.y * d11.y;
+    const double p01r = d00.x * d01.x + d00.y * d01.y + d10.x * d11.x + d10.y * d11.y;
+    const double p01i = d00.x * d01.y - d00.y * d01.x + d10.x * d11.y - d10.y * d11.x;
+    g[(size_t)c * nant + a] = make_float4((float)p00, (float)p11, (float)p01r, (float)p01i);
+}
+
+// Per (c, p): B^-1 (4 complex) and h of Q = B B^H as bx[c][p][12] floats,
+// and the weight exponent esp[c][p].  B: [p][c][4], D: [a][c][4].
+__global__ void k_pb_beam(const double2 *__restrict__ D, const double2 *__restrict__ B, int np,
+                          int nant, int nchan, float *__restrict__ bx, int *__restrict__ esp)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)np * nchan) return;
+    const int c = (int)(j % nchan), p = (int)(j / nchan);
+    double2 b[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) b[q] = B[((size_t)p * nchan + c) * 4 + q];
+    const double2 det = make_double2(b[0].x * b[3].x - b[0].y * b[3].y - (b[1].x * b[2].x - b[1].y * b[2].y),
+                                     b[0].x * b[3].y + b[0].y * b[3].x - (b[1].x * b[2].y + b[1].y * b[2].x));
+    const double dd = det.x * det.x + det.y * det.y;
+    const double2 r = dd > 0.0 ? make_double2(det.x / dd, -det.y / dd) : make_double2(0.0, 0.0);
+    const double2 inv[4] = {zmul(b[3], r), zmul(make_double2(-b[1].x, -b[1].y), r),
+                            zmul(make_double2(-b[2].x, -b[2].y), r), zmul(b[0], r)};
+    float *o = bx + ((size_t)c * np + p) * 12;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        o[2 * q] = (float)inv[q].x;
+        o[2 * q + 1] = (float)inv[q].y;
+    }
+    // Q = B B^H: Q00 = |b00|^2 + |b01|^2, Q11 = |b10|^2 + |b11|^2,
+    // Q01 = b00 conj(b10) + b01 conj(b11)
+    const double q00 = b[0].x * b[0].x + b[0].y * b[0].y + b[1].x * b[1].x + b[1].y * b[1].y;
+    const double q11 = b[2].x * b[2].x + b[2].y * b[2].y + b[3].x * b[3].x + b[3].y * b[3].y;
+    const double q01r = b[0].x * b[2].x + b[0].y * b[2].y + b[1].x * b[3].x + b[1].y * b[3].y;
+    const double q01i = b[0].y * b[2].x - b[0].x * b[2].y + b[1].y * b[3].x - b[1].x * b[3].y;
+    o[8] = (float)q00;
+    o[9] = (float)q11;
+    o[10] = (float)(2.0 * q01r);
+    o[11] = (float)(2.0 * q01i);
+    double f2 = 0.0;
+    for (int a = 0; a < nant; a++) f2 = fmax(f2, frob2_DB(D + ((size_t)a * nchan + c) * 4, b));
+    int ex = 0;
+    if (f2 > 0.0) frexp(sqrt(f2), &ex);
+    esp[(size_t)c * np + p] = 13 - ex;
+}
+
 // ------------------------------------------------------------ weights -----
 // Operand A in fragment order: Af[c][rb][ks][h][lane] (uint4), rb a block of
 // 32 GEMM rows (16 beams), ks a step of 16 k' (8 tiles), h = 0 hi part, 1 lo
@@ -118,11 +229,16 @@ __device__ __forceinline__ int scale_exp(float mx)
 // im rows (Im phi, Re phi); rows of beams >= npointing are zero.  Weights are
 // scaled by 2^PHASE_EXP.  phi: [p][a][c] complex double over nant_src
 // antennas; GEMM tile a is antenna amap[a] (amap NULL: a itself), nant tiles.
-template <bool BF16>
+// PB (per-pointing primary beam): the weight is phi F_pa 2^esp[c][p] with
+// F_pa = ||D_a B_p||_F (D [a][c][4], B [p][c][4]).
+template <bool BF16, bool PB>
 __global__ __launch_bounds__(256) void k_phase_weights(const double2 *__restrict__ phi,
                                                        int npointing, int nant, int nchan,
                                                        int nrb, int nkstep,
                                                        const int *__restrict__ amap, int nant_src,
+                                                       const double2 *__restrict__ D,
+                                                       const double2 *__restrict__ B,
+                                                       const int *__restrict__ esp,
                                                        uint4 *__restrict__ Af)
 {
     typedef mtype<BF16> M;
@@ -136,7 +252,8 @@ __global__ __launch_bounds__(256) void k_phase_weights(const double2 *__restrict
     const int c = (int)(frag / ((size_t)nkstep * nrb));
     const int row = RBK * rb + (lane & 31);
     const int p = row >> 1, im = row & 1;
-    const double ws = ldexp(1.0, PHASE_EXP);
+    double ws = ldexp(1.0, PHASE_EXP);
+    if (PB && p < npointing) ws = ldexp(1.0, esp[(size_t)c * npointing + p]);
     uint32_t hi[4], lo[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -145,7 +262,11 @@ __global__ __launch_bounds__(256) void k_phase_weights(const double2 *__restrict
         if (p < npointing && a < nant) {
             const int as = amap ? amap[a] : a;
             const double2 f = phi[((size_t)p * nant_src + as) * nchan + c];
-            const double vv[2] = {(im ? f.y : f.x) * ws, (im ? f.x : -f.y) * ws};
+            double wf = ws;
+            if (PB)
+                wf *= sqrt(frob2_DB(D + ((size_t)as * nchan + c) * 4,
+                                    B + ((size_t)p * nchan + c) * 4));
+            const double vv[2] = {(im ? f.y : f.x) * wf, (im ? f.x : -f.y) * wf};
 #pragma unroll
             for (int e2 = 0; e2 < 2; e2++) {
                 hb[e2] = M::bits((float)vv[e2]);
@@ -207,9 +328,13 @@ __global__ void k_jcheck(const double2 *__restrict__ J, size_t per_pointing, siz
 // tiles 8 ks + 4 (l/32) + {0..3} as (re, im) pairs of polarisation pol.
 // Samples >= ns are zero; tiles >= nant are zero.  sexp[c][sb] = es.
 // v: VM_FLT [c][i][s].  Needs nant <= NG APT.
-template <bool BF16, int APT, int NG>
+// PB (per-pointing primary beam): Jf holds D^-1 and, instead of the four N
+// terms, N receives T_k = sum_a gw[c][a].k u_a u_a^H (k = 0..3, each as
+// (XX, YY, Re XY, Im XY)) as float4 [c][s][4].
+template <bool BF16, int APT, int NG, bool PB>
 __global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, int ns, int ni, int nsb,
                                                  int nkstep, const cf *__restrict__ Jf,
+                                                 const float4 *__restrict__ gw,
                                                  const uint32_t *__restrict__ polQ,
                                                  const uint32_t *__restrict__ polP, int nant,
                                                  uint4 *__restrict__ Bf, int *__restrict__ sexp,
@@ -249,28 +374,66 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, i
         }
     }
     float nxx = 0.f, nyy = 0.f, xyr = 0.f, xyi = 0.f, mx = 0.f;
+    float4 tk[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) tk[k] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int j = 0; j < APT; j++) {
         const cf x = ux[j], y = uy[j];
         mx = fmaxf(mx, fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(y.x), fabsf(y.y))));
-        nxx += x.x * x.x + x.y * x.y;
-        nyy += y.x * y.x + y.y * y.y;
-        xyr += x.x * y.x + x.y * y.y;
-        xyi += x.y * y.x - x.x * y.y;
+        const float rxx = x.x * x.x + x.y * x.y, ryy = y.x * y.x + y.y * y.y;
+        const float rxr = x.x * y.x + x.y * y.y, rxi = x.y * y.x - x.x * y.y;
+        if constexpr (PB) {
+            const int a = g + NG * j;
+            if (a < nant) {
+                const float4 ga = gw[(size_t)c * nant + a];
+                const float gk[4] = {ga.x, ga.y, ga.z, ga.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    tk[k].x += gk[k] * rxx;
+                    tk[k].y += gk[k] * ryy;
+                    tk[k].z += gk[k] * rxr;
+                    tk[k].w += gk[k] * rxi;
+                }
+            }
+        } else {
+            nxx += rxx;
+            nyy += ryy;
+            xyr += rxr;
+            xyi += rxi;
+        }
     }
-    red[g][sl] = make_float4(nxx, nyy, xyr, xyi);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
     if ((t & 63) == 0) wmx[t >> 6] = mx;
-    __syncthreads();
-    if (g == 0 && ok) {
-        float4 acc = red[0][sl];
+    if constexpr (PB) {
 #pragma unroll
-        for (int k = 1; k < NG; k++) {
-            const float4 r = red[k][sl];
-            acc.x += r.x; acc.y += r.y; acc.z += r.z; acc.w += r.w;
+        for (int k = 0; k < 4; k++) {
+            red[g][sl] = tk[k];
+            __syncthreads();
+            if (g == 0 && ok) {
+                float4 acc = red[0][sl];
+#pragma unroll
+                for (int q = 1; q < NG; q++) {
+                    const float4 r = red[q][sl];
+                    acc.x += r.x; acc.y += r.y; acc.z += r.z; acc.w += r.w;
+                }
+                N[((size_t)c * ns + s) * 4 + k] = acc;
+            }
+            __syncthreads();
         }
-        N[(size_t)c * ns + s] = acc;
+    } else {
+        red[g][sl] = make_float4(nxx, nyy, xyr, xyi);
+        __syncthreads();
+        if (g == 0 && ok) {
+            float4 acc = red[0][sl];
+#pragma unroll
+            for (int k = 1; k < NG; k++) {
+                const float4 r = red[k][sl];
+                acc.x += r.x; acc.y += r.y; acc.z += r.z; acc.w += r.w;
+            }
+            N[(size_t)c * ns + s] = acc;
+        }
     }
     float bm = wmx[0];
 #pragma unroll
@@ -392,17 +555,56 @@ __device__ __forceinline__ void stokes4(float xr, float xi, float yr, float yi, 
 // one coarse channel of a batch.  Af, Bf, sexp and N are indexed by the fine
 // channel c as laid out by k_phase_weights / k_bf_prep; S is [p][st][c][s_total]
 // (device layout) written from sample soffset on.
+// Per-pointing primary-beam form: N holds T_k as float4 [c][s][4], bx the
+// pointing's B^-1 and h as float [c][p][12] and esp its weight exponents.
 struct bf_geom {
     const uint4 *Af;
     const uint4 *Bf;
     const int *sexp;
     const float4 *N;
     float *S;
+    const float *bx;
+    const int *esp;
     int nkstep, nrb, npointing, nstokes;
     int soffset, ns_total;
     float invw;
     int pad;
 };
+
+struct c2f { float x, y; };
+__device__ __forceinline__ c2f cmulf(c2f a, c2f b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ c2f cconjf(c2f a) { return {a.x, -a.y}; }
+
+// Stokes of one pointing with its own primary beam from the GEMM output E'
+// (x, y; scaled, u2 removes the scales squared): X' = E' E'^H u2 -
+// sum_k h_k T_k, then X = B^-1 X' B^-H, whose (00, 11, 01) are (bnXX, bnYY,
+// bnXY) of src/form_beam.cpp:339-355.  bx: B^-1 row-major (4 complex), h.
+__device__ __forceinline__ void stokes_pb(float xr, float xi, float yr, float yi, float u2,
+                                          const float4 (&T)[4], const float4 (&bx)[3], float invw,
+                                          float (&st)[4])
+{
+    const float h0 = bx[2].x, h1 = bx[2].y, h2 = bx[2].z, h3 = bx[2].w;
+    const float X00 = (xr * xr + xi * xi) * u2 - (h0 * T[0].x + h1 * T[1].x + h2 * T[2].x + h3 * T[3].x);
+    const float X11 = (yr * yr + yi * yi) * u2 - (h0 * T[0].y + h1 * T[1].y + h2 * T[2].y + h3 * T[3].y);
+    const c2f X01 = {(xr * yr + xi * yi) * u2 - (h0 * T[0].z + h1 * T[1].z + h2 * T[2].z + h3 * T[3].z),
+                     (xi * yr - xr * yi) * u2 - (h0 * T[0].w + h1 * T[1].w + h2 * T[2].w + h3 * T[3].w)};
+    const c2f b00 = {bx[0].x, bx[0].y}, b01 = {bx[0].z, bx[0].w};
+    const c2f b10 = {bx[1].x, bx[1].y}, b11 = {bx[1].z, bx[1].w};
+    const c2f p0 = cmulf(b00, cconjf(b01)), p1 = cmulf(b10, cconjf(b11));
+    const float Y00 = (b00.x * b00.x + b00.y * b00.y) * X00 + (b01.x * b01.x + b01.y * b01.y) * X11 +
+                      2.0f * (p0.x * X01.x - p0.y * X01.y);
+    const float Y11 = (b10.x * b10.x + b10.y * b10.y) * X00 + (b11.x * b11.x + b11.y * b11.y) * X11 +
+                      2.0f * (p1.x * X01.x - p1.y * X01.y);
+    const c2f q0 = cmulf(b00, cconjf(b10)), q1 = cmulf(b01, cconjf(b11));
+    const c2f r0 = cmulf(cmulf(b00, X01), cconjf(b11));
+    const c2f r1 = cmulf(cmulf(b01, cconjf(X01)), cconjf(b10));
+    const float Y01r = q0.x * X00 + q1.x * X11 + r0.x + r1.x;
+    const float Y01i = q0.y * X00 + q1.y * X11 + r0.y + r1.y;
+    st[0] = invw * (Y00 + Y11);
+    st[1] = invw * (Y00 - Y11);
+    st[2] = 2.0f * invw * Y01r;
+    st[3] = -2.0f * invw * Y01i;
+}
 
 // One workgroup tile (tx, ty) of fine channel c: WR x WC wavefronts cover
 // (WR MB 32) rows (16 WR MB beams) x (WC NB/2 32) samples; a wavefront's NB
@@ -416,7 +618,8 @@ struct bf_geom {
 // VEC: ns, ns_total and soffset are multiples of 4, so every in-range
 // 4-sample group is stored whole with one 16-byte store (the other form
 // stores sample by sample).
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool VEC>
+// PB: the per-pointing primary-beam epilogue (stokes_pb).
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool VEC, bool PB>
 __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int nchan, int c,
                                         int tx, int ty, float *stg)
 {
@@ -497,13 +700,22 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
         const int sb = sb0 + j;
         const int s = SB * sb + lr;
         float4 nn = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (s < ns) {
+        float4 T4[4];
+        if constexpr (PB) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                T4[k] = nn;
+                if (s < ns) {
+                    const f4v t4 = ((const gmem f4v *)G.N)[((size_t)c * ns + s) * 4 + k];
+                    T4[k] = make_float4(t4.x, t4.y, t4.z, t4.w);
+                }
+            }
+        } else if (s < ns) {
             const f4v n4 = ((const gmem f4v *)G.N)[(size_t)c * ns + s];
             nn = make_float4(n4.x, n4.y, n4.z, n4.w);
         }
-        const float unscale2 =
-            ldexpf(1.0f, -2 * (((const gmem int *)G.sexp)[(size_t)c * nsb + min(sb, nsb - 1)] +
-                               PHASE_EXP));
+        const int esb = ((const gmem int *)G.sexp)[(size_t)c * nsb + min(sb, nsb - 1)];
+        const float unscale2 = ldexpf(1.0f, -2 * (esb + PHASE_EXP));
         const int s4 = SB * sb + 4 * qq;
         gmem float *Sc = (gmem float *)G.S + (size_t)c * G.ns_total + G.soffset + s4;
 #pragma unroll
@@ -517,8 +729,28 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                     for (int b = 0; b < 2; b++) {
                         const f16v &ax = acc[m][2 * j], &ay = acc[m][2 * j + 1];
                         float st4[4];
-                        stokes4(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1], ay[4 * jg + 2 * b],
-                                ay[4 * jg + 2 * b + 1], unscale2, nn, invw, st4);
+                        if constexpr (PB) {
+                            const int pb = 16 * (rb0 + m) + 8 * h + 4 * jj + 2 * lk + b;
+                            st4[0] = st4[1] = st4[2] = st4[3] = 0.f;
+                            if (pb < npointing) {
+                                const size_t cp = (size_t)c * npointing + pb;
+                                const gmem f4v *bp = (const gmem f4v *)(G.bx + cp * 12);
+                                float4 bx[3];
+#pragma unroll
+                                for (int q = 0; q < 3; q++) {
+                                    const f4v t = bp[q];
+                                    bx[q] = make_float4(t.x, t.y, t.z, t.w);
+                                }
+                                const float u2 =
+                                    ldexpf(1.0f, -2 * (esb + ((const gmem int *)G.esp)[cp]));
+                                stokes_pb(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1],
+                                          ay[4 * jg + 2 * b], ay[4 * jg + 2 * b + 1], u2, T4, bx,
+                                          invw, st4);
+                            }
+                        } else {
+                            stokes4(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1], ay[4 * jg + 2 * b],
+                                    ay[4 * jg + 2 * b + 1], unscale2, nn, invw, st4);
+                        }
 #pragma unroll
                         for (int st = 0; st < 4; st++)
                             wl[(st * 8 + 4 * jj + 2 * b + lk) * 32 + lr] = st4[st];
@@ -553,7 +785,7 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
 // the same tile of 8 different channels, so each XCD walks its own channels;
 // within a channel tiles go in bands of gr row tiles, column by column (gr =
 // all row tiles for order 1, 1 for order 2, 2 for order 3, 4 for order 4).
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC, bool VEC>
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC, bool VEC, bool PB>
 __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, int ns, int nsb,
                                                                      int nchan, int ntx, int nty,
                                                                      int order)
@@ -574,7 +806,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, 
     if (nty % gr) gr = 1;
     const int band = tile / (gr * ntx), rem = tile % (gr * ntx);
     const int tx = rem / gr, ty = band * gr + rem % gr;
-    bf_tile<BF16, MB, NB, WR, WC, NST, VEC>(G, ns, nsb, nchan, c, tx, ty, stg);
+    bf_tile<BF16, MB, NB, WR, WC, NST, VEC, PB>(G, ns, nsb, nchan, c, tx, ty, stg);
 }
 
 // Batched form: one persistent launch over the GEMM problems of several
@@ -608,7 +840,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
         const int cnt = __builtin_amdgcn_readfirstlane((int)(e.y & 0xffu));
         const bf_geom G = geoms[g];
         for (int ty = ty0; ty < ty0 + cnt; ty++)
-            bf_tile<BF16, MB, NB, WR, WC, NST, true>(G, ns, nsb, nchan, c, tx, ty, stg);
+            bf_tile<BF16, MB, NB, WR, WC, NST, true, false>(G, ns, nsb, nchan, c, tx, ty, stg);
     }
 }
 
@@ -625,7 +857,7 @@ static const bf_variant kVariants[] = {
 #define N_VARIANTS ((int)(sizeof(kVariants) / sizeof(kVariants[0])))
 #define VM_N_ORDERS 5
 
-template <bool BF16>
+template <bool BF16, bool PB>
 static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, int ns, int nsb,
                            int nchan, int ntx, int nty, int order)
 {
@@ -634,12 +866,12 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
         // built for the first variant's shape only (its grid is the caller's
         // when vi = 0)
         if (vi != 0) VM_FATAL("MFMA beamformer: variant %d needs ns %% 4 == 0", vi);
-        hipLaunchKernelGGL((k_beamform_mfma<BF16, 4, 4, 2, 2, 4, 1, false>), grid, dim3(256), 0,
-                           st, G, ns, nsb, nchan, ntx, nty, order);
+        hipLaunchKernelGGL((k_beamform_mfma<BF16, 4, 4, 2, 2, 4, 1, false, PB>), grid, dim3(256),
+                           0, st, G, ns, nsb, nchan, ntx, nty, order);
         return;
     }
 #define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, OCC_)                                             \
-    hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, OCC_, true>), grid,  \
+    hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, OCC_, true, PB>), grid, \
                        dim3(64 * WR_ * WC_), 0, st, G, ns, nsb, nchan, ntx, nty, order)
     switch (vi) {
     case 0: VM_LAUNCH(4, 4, 2, 2, 4, 1); break;
@@ -706,22 +938,28 @@ static void ensure_buf(void **p, size_t *cap, size_t need)
 
 // k_bf_prep over (nsb sample blocks) x (nchan channels) for nant <= 256
 // antennas (the template's antennas-per-thread bound is chosen from nant)
+// gw non-NULL: the per-pointing primary-beam form (Jf = D^-1, N = T_k)
 static void launch_prep(bool bf16, hipStream_t st, const cf *v, int ns, int ni, int nsb,
-                        int nkstep, int nchan, const cf *Jf, const uint32_t *polQ,
-                        const uint32_t *polP, int nant, uint4 *Bf, int *sexp, float4 *N)
+                        int nkstep, int nchan, const cf *Jf, const float4 *gw,
+                        const uint32_t *polQ, const uint32_t *polP, int nant, uint4 *Bf, int *sexp,
+                        float4 *N)
 {
     if (nant > 256) VM_FATAL("MFMA beamformer: at most 256 antennas (%d)", nant);
     const dim3 pg((unsigned)nsb, (unsigned)nchan);
-#define VM_PREP(B_, A_, G_)                                                                   \
-    hipLaunchKernelGGL((k_bf_prep<B_, A_, G_>), pg, dim3(32 * G_), 0, st, v, ns, ni, nsb,      \
-                       nkstep, Jf, polQ, polP, nant, Bf, sexp, N)
+#define VM_PREP(B_, A_, G_, P_)                                                               \
+    hipLaunchKernelGGL((k_bf_prep<B_, A_, G_, P_>), pg, dim3(32 * G_), 0, st, v, ns, ni, nsb,  \
+                       nkstep, Jf, gw, polQ, polP, nant, Bf, sexp, N)
+#define VM_PREP2(A_, G_)                                                                      \
+    if (gw) { if (bf16) VM_PREP(true, A_, G_, true); else VM_PREP(false, A_, G_, true); }     \
+    else { if (bf16) VM_PREP(true, A_, G_, false); else VM_PREP(false, A_, G_, false); }
     if (nant <= 64) {
-        if (bf16) VM_PREP(true, 8, 8); else VM_PREP(false, 8, 8);
+        VM_PREP2(8, 8);
     } else if (nant <= 128) {
-        if (bf16) VM_PREP(true, 8, 16); else VM_PREP(false, 8, 16);
+        VM_PREP2(8, 16);
     } else {
-        if (bf16) VM_PREP(true, 16, 16); else VM_PREP(false, 16, 16);
+        VM_PREP2(16, 16);
     }
+#undef VM_PREP2
 #undef VM_PREP
     HIPCHECK(hipGetLastError());
 }
@@ -735,6 +973,9 @@ struct mfma_plan {
     float4 *N;
     int *sexp;
     float invw;
+    int pb;      // per-pointing primary-beam form
+    float *bx;   // pb: B^-1 and h per (c, p)
+    int *esp;    // pb: weight exponents per (c, p)
 };
 
 static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
@@ -754,14 +995,45 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     const int ns = vm->fine_sample_rate / vm->chunks_per_second;
     const int nsb = (ns + SB - 1) / SB;
 
-    // scratch: Af | Bf | N | sexp | Jf | jflag
+    bool rebuild = d->Wm_gen != d->weights_gen || d->Wm_form != vm->beamform_form;
+    if (rebuild) {
+        // the form: J^-1 shared by every pointing (u = J^-1 v), or a
+        // per-pointing primary beam B[p][c] shared by every tile
+        int pb = 0;
+        if (np > 1) {
+            ensure_buf(&d->d_flag, &d->flag_bytes, 256);
+            uint32_t *jflag = (uint32_t *)d->d_flag;
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, jflag, (size_t)1, 0u);
+            const size_t per = (size_t)nant * nchan * 4, tot = per * np;
+            hipLaunchKernelGGL(k_jcheck, dim3((unsigned)((tot - per + 255) / 256)), dim3(256), 0,
+                               st, (const double2 *)vm->d_J, per, tot, jflag);
+            HIPCHECK(hipGetLastError());
+            uint32_t hflag = 0;
+            HIPCHECK(hipMemcpyAsync(&hflag, jflag, 4, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            if (hflag) {
+                if (!d->B_chan || !d->d_B)
+                    VM_FATAL("MFMA beamformer: J^-1 differs between pointings with a primary beam "
+                             "per tile (vmCalcJDevice); this form takes a per-pointing beam shared "
+                             "by all tiles (vmCalcJDeviceChanB), or use the FP32 form");
+                pb = 1;
+            }
+        }
+        d->Wm_pb = pb;
+    }
+    const int pb = d->Wm_pb;
+
+    // scratch: Af | Bf | N (T_k for pb) | sexp | Jf (D^-1 for pb) | g | bx | esp
     const size_t af_bytes = align256((size_t)nchan * nrb * nkstep * 128 * sizeof(uint4));
     const size_t bf_bytes = align256((size_t)nchan * nsb * nkstep * 128 * sizeof(uint4));
-    const size_t n_bytes = align256((size_t)nchan * ns * sizeof(float4));
+    const size_t n_bytes = align256((size_t)nchan * ns * sizeof(float4) * (pb ? 4 : 1));
     const size_t se_bytes = align256((size_t)nchan * nsb * sizeof(int));
     const size_t jf_bytes = align256((size_t)nchan * nant * 4 * sizeof(cf));
-    const size_t need = af_bytes + bf_bytes + n_bytes + se_bytes + jf_bytes + 256;
-    bool rebuild = d->Wm_gen != d->weights_gen || d->Wm_form != vm->beamform_form;
+    const size_t g_bytes = pb ? align256((size_t)nchan * nant * sizeof(float4)) : 0;
+    const size_t bx_bytes = pb ? align256((size_t)nchan * np * 12 * sizeof(float)) : 0;
+    const size_t esp_bytes = pb ? align256((size_t)nchan * np * sizeof(int)) : 0;
+    const size_t need = af_bytes + bf_bytes + n_bytes + se_bytes + jf_bytes + g_bytes + bx_bytes +
+                        esp_bytes;
     if (d->Wm_bytes < need) {
         ensure_buf(&d->d_Wm, &d->Wm_bytes, need);
         rebuild = true;
@@ -772,50 +1044,58 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     float4 *N = (float4 *)q;                q += n_bytes;
     int *sexp = (int *)q;                   q += se_bytes;
     cf *Jf = (cf *)q;                       q += jf_bytes;
-    uint32_t *jflag = (uint32_t *)q;
+    float4 *gw = pb ? (float4 *)q : NULL;   q += g_bytes;
+    float *bx = pb ? (float *)q : NULL;     q += bx_bytes;
+    int *esp = pb ? (int *)q : NULL;
 
     if (rebuild) {
         vm_timer_start(d, VM_T_BF_WEIGHTS);
-        hipLaunchKernelGGL(k_jf, dim3((unsigned)((nant * nchan + 255) / 256)), dim3(256), 0, st,
-                           (const double2 *)vm->d_J, nant, nchan, (const int *)NULL, Jf);
         const size_t total = (size_t)nchan * nrb * nkstep * 64;
         const dim3 g((unsigned)((total + 255) / 256));
-        if (bf16)
-            hipLaunchKernelGGL(k_phase_weights<true>, g, dim3(256), 0, st,
-                               (const double2 *)vm->gdelays.d_phi, np, nant, nchan, nrb, nkstep,
-                               (const int *)NULL, nant, Af);
-        else
-            hipLaunchKernelGGL(k_phase_weights<false>, g, dim3(256), 0, st,
-                               (const double2 *)vm->gdelays.d_phi, np, nant, nchan, nrb, nkstep,
-                               (const int *)NULL, nant, Af);
-        if (np > 1) {
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, jflag, (size_t)1, 0u);
-            const size_t per = (size_t)nant * nchan * 4, tot = per * np;
-            hipLaunchKernelGGL(k_jcheck, dim3((unsigned)((tot - per + 255) / 256)), dim3(256), 0,
-                               st, (const double2 *)vm->d_J, per, tot, jflag);
+        const double2 *dD = (const double2 *)vm->d_D, *dB = (const double2 *)d->d_B;
+        const double2 *phi = (const double2 *)vm->gdelays.d_phi;
+        if (pb) {
+            hipLaunchKernelGGL(k_pb_dinv, dim3((unsigned)((nant * nchan + 255) / 256)), dim3(256),
+                               0, st, dD, nant, nchan, Jf, gw);
+            hipLaunchKernelGGL(k_pb_beam, dim3((unsigned)((np * nchan + 255) / 256)), dim3(256), 0,
+                               st, dD, dB, np, nant, nchan, bx, esp);
+            if (bf16)
+                hipLaunchKernelGGL((k_phase_weights<true, true>), g, dim3(256), 0, st, phi, np,
+                                   nant, nchan, nrb, nkstep, (const int *)NULL, nant, dD, dB,
+                                   (const int *)esp, Af);
+            else
+                hipLaunchKernelGGL((k_phase_weights<false, true>), g, dim3(256), 0, st, phi, np,
+                                   nant, nchan, nrb, nkstep, (const int *)NULL, nant, dD, dB,
+                                   (const int *)esp, Af);
+        } else {
+            hipLaunchKernelGGL(k_jf, dim3((unsigned)((nant * nchan + 255) / 256)), dim3(256), 0,
+                               st, (const double2 *)vm->d_J, nant, nchan, (const int *)NULL, Jf);
+            if (bf16)
+                hipLaunchKernelGGL((k_phase_weights<true, false>), g, dim3(256), 0, st, phi, np,
+                                   nant, nchan, nrb, nkstep, (const int *)NULL, nant,
+                                   (const double2 *)NULL, (const double2 *)NULL,
+                                   (const int *)NULL, Af);
+            else
+                hipLaunchKernelGGL((k_phase_weights<false, false>), g, dim3(256), 0, st, phi, np,
+                                   nant, nchan, nrb, nkstep, (const int *)NULL, nant,
+                                   (const double2 *)NULL, (const double2 *)NULL,
+                                   (const int *)NULL, Af);
         }
         HIPCHECK(hipGetLastError());
         vm_timer_stop(d, VM_T_BF_WEIGHTS);
         d->Wm_gen = d->weights_gen;
         d->Wm_form = vm->beamform_form;
-        if (np > 1) {
-            uint32_t hflag = 0;
-            HIPCHECK(hipMemcpyAsync(&hflag, jflag, 4, hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipStreamSynchronize(st));
-            if (hflag)
-                VM_FATAL("MFMA beamformer: J^-1 differs between pointings (per-beam primary "
-                         "beam); this form needs a beam-independent J^-1 (use the FP32 form)");
-        }
     }
-    // u = J^-1 v, auto-correlation terms, block scales and the B operand
+    // u = J^-1 v (D^-1 v), auto-correlation terms, block scales and the B operand
     vm_timer_start(d, VM_T_BF_PREP);
-    launch_prep(bf16, st, (const cf *)vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, vm->d_polQ_idxs,
-                vm->d_polP_idxs, nant, Bf, sexp, N);
+    launch_prep(bf16, st, (const cf *)vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw,
+                vm->d_polQ_idxs, vm->d_polP_idxs, nant, Bf, sexp, N);
     vm_timer_stop(d, VM_T_BF_PREP);
     pl->nant = nant; pl->ni = ni; pl->nchan = nchan; pl->np = np; pl->nkstep = nkstep;
     pl->nrb = nrb; pl->ns = ns; pl->nsb = nsb;
     pl->chunk = vm->chunk_to_load % vm->chunks_per_second;
     pl->Af = Af; pl->Bf = Bf; pl->N = N; pl->sexp = sexp;
+    pl->pb = pb; pl->bx = bx; pl->esp = esp;
     pl->invw = (float)(1.0 / (double)vm->num_not_flagged);
 }
 
@@ -838,11 +1118,16 @@ static void mfma_run(vcsbeam_context *vm, const mfma_plan *pl, int vi, int order
     G.Af = pl->Af; G.Bf = pl->Bf; G.sexp = pl->sexp; G.N = pl->N; G.S = (float *)vm->d_S;
     G.nkstep = pl->nkstep; G.nrb = pl->nrb; G.npointing = pl->np; G.nstokes = vm->out_nstokes;
     G.soffset = pl->chunk * pl->ns; G.ns_total = vm->fine_sample_rate; G.invw = pl->invw;
-    G.pad = 0;
-    if (vm->beamform_form == VM_BF_MFMA_BF16)
-        launch_variant<true>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
+    G.bx = pl->bx; G.esp = pl->esp; G.pad = 0;
+    const bool bf16 = vm->beamform_form == VM_BF_MFMA_BF16;
+    if (bf16 && pl->pb)
+        launch_variant<true, true>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
+    else if (bf16)
+        launch_variant<true, false>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
+    else if (pl->pb)
+        launch_variant<false, true>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
     else
-        launch_variant<false>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
+        launch_variant<false, false>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
     HIPCHECK(hipGetLastError());
 }
 
@@ -1108,13 +1393,15 @@ static void batch_refresh(vm_batch *b)
         const size_t total = (size_t)nchan * nrb * nkstep * 64;
         const dim3 gw((unsigned)((total + 255) / 256));
         if (b->bf16)
-            hipLaunchKernelGGL(k_phase_weights<true>, gw, dim3(256), 0, st,
+            hipLaunchKernelGGL((k_phase_weights<true, false>), gw, dim3(256), 0, st,
                                (const double2 *)vm->gdelays.d_phi, np, nact, nchan, nrb, nkstep,
-                               (const int *)C.d_amap, nant, C.Af);
+                               (const int *)C.d_amap, nant, (const double2 *)NULL,
+                               (const double2 *)NULL, (const int *)NULL, C.Af);
         else
-            hipLaunchKernelGGL(k_phase_weights<false>, gw, dim3(256), 0, st,
+            hipLaunchKernelGGL((k_phase_weights<false, false>), gw, dim3(256), 0, st,
                                (const double2 *)vm->gdelays.d_phi, np, nact, nchan, nrb, nkstep,
-                               (const int *)C.d_amap, nant, C.Af);
+                               (const int *)C.d_amap, nant, (const double2 *)NULL,
+                               (const double2 *)NULL, (const int *)NULL, C.Af);
         uint32_t hflag = 0;
         if (np > 1) {
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, queue_flag, (size_t)1, 0u);
@@ -1168,8 +1455,8 @@ static void batch_prep(vm_batch *b, int g, hipStream_t st)
 {
     const vm_batch_chan &C = b->ch[g];
     launch_prep(b->bf16, st, (const cf *)C.vm->fpfb->d_vcs_data, b->ns,
-                (int)C.vm->obs_metadata->num_rf_inputs, b->nsb, C.nkstep, b->nchan, C.Jf, C.d_polQ,
-                C.d_polP, C.nact, C.Bf, C.sexp, C.N);
+                (int)C.vm->obs_metadata->num_rf_inputs, b->nsb, C.nkstep, b->nchan, C.Jf,
+                (const float4 *)NULL, C.d_polQ, C.d_polP, C.nact, C.Bf, C.sexp, C.N);
 }
 
 static void batch_gemm(vm_batch *b, hipStream_t st)

@@ -173,6 +173,7 @@ void vmCalcJDevice(vcsbeam_context *vm)
     vm_timer_start(d, VM_T_GEOM);
     vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 0, vm->d_J, s);
     vm_timer_stop(d, VM_T_GEOM);
+    d->B_chan = 0;
     HIPCHECK(hipStreamSynchronize(s));
     d->weights_gen++;
 }
@@ -186,6 +187,7 @@ void vmCalcJDeviceChanB(vcsbeam_context *vm, const gpuDoubleComplex *B)
     ensure(&d->d_B, &d->B_bytes, bbytes);
     HIPCHECK(hipMemcpyAsync(d->d_B, B, bbytes, hipMemcpyHostToDevice, s));
     vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 1, vm->d_J, s);
+    d->B_chan = 1;
     HIPCHECK(hipStreamSynchronize(s));
     d->weights_gen++;
 }

@@ -29,6 +29,9 @@ struct vm_device_state {
     unsigned W_gen;  /* FP32 weights d_W */
     unsigned Wm_gen; /* MFMA weights in d_Wm */
     int Wm_form;     /* beamform_form (FP16 / BF16) the MFMA weights are in */
+    int Wm_pb;       /* MFMA weights in the per-pointing primary-beam form */
+    void *d_flag;    /* small device flag buffer */
+    size_t flag_bytes;
     /* MFMA weights: [p-block][c][row][k] packed half/bf16 planes */
     void *d_Wm;
     size_t Wm_bytes;
@@ -40,9 +43,12 @@ struct vm_device_state {
     /* tile positions (E, N, H, L) per antenna for device phi */
     double *d_pos;
     double *d_freqs;
-    /* primary beam B [p][a][2][2] complex double on device */
+    /* primary beam on device, complex double: [p][a][2][2] after
+       vmCalcJDevice (B_chan 0), [p][c][2][2] after vmCalcJDeviceChanB
+       (B_chan 1: the same beam for every tile, per fine channel) */
     void *d_B;
     size_t B_bytes;
+    int B_chan;
     /* unit vectors per pointing (uE, uN, uH) */
     double *d_unit;
     size_t unit_bytes;

@@ -104,3 +104,52 @@ def test_every_beamformer_variant_and_tile_order(pkg, orc, gpu_lib, mfma_case, v
             assert err[:, 1:].max() <= 3e-3
     finally:
         sess.close()
+
+
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 2e-2, 5e-2)])
+def test_mfma_per_pointing_primary_beam(pkg, orc, gpu_lib, form, tol_i, tol_quv):
+    """Every pointing with its own primary beam B[p][c] (the analytic
+    dipole-array beam, the same for all tiles), so J^-1 = F B^-1 D^-1
+    differs between pointings: the MFMA form weights phi by F_pa, applies
+    B_p^-1 and the beam's auto-correlation terms in its epilogue.  Same bars
+    as the beam-independent form, against the FP64 oracle chain with
+    J^-1 from D and B per fine channel.  Workload: 128 tiles, 0.1 s, 40
+    directions within 15 deg."""
+    from test_gpu_beam_surrogate import _jones
+    from vcsbeam_amd.apps import TiedArraySession
+    from vcsbeam_amd.workloads import unit_vectors
+    lib = gpu_lib
+    obs, az, za = _setup(pkg)
+    sgt = lib.vmSurrogateCreate(7, 0.01, 50e-12)
+    try:
+        B = np.ascontiguousarray(_jones(lib, sgt, 0, az, za, obs.freqs)[0])  # [p][c][2][2]
+    finally:
+        lib.vmSurrogateDestroy(sgt)
+    D = obs.calibration()
+    J = np.empty((len(az), obs.ntiles, 128, 2, 2), complex)
+    for c in range(128):
+        J[:, :, c] = orc.calc_J(D[:, c:c + 1], np.repeat(B[:, c][:, None], obs.ntiles, axis=1))[:, :, 0]
+    cur = obs.second(0)
+    htr = np.ascontiguousarray(np.concatenate([np.zeros_like(cur[:1]), cur], axis=0))
+    v = orc.pfb_forward(htr, obs.ninput, obs.nspectra, obs.filter.astype(np.int32), obs.out_idx(),
+                        fpga=True)
+    E, N, H, L = obs.positions()
+    phi = orc.calc_phi(E, N, H, L, unit_vectors(az, za), obs.freqs)
+    polQ, polP = obs.pol_idx()
+    Sref, _ = orc.beamform(v, J, phi, polQ, polP, 1.0 / (2 * int((~obs.flag_mask).sum())), 4)
+    print(f"MFMA form {form} with per-pointing beams, {len(az)} beams", flush=True)
+    ses = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
+                           form=form)
+    try:
+        ses.read(0)
+        ses.pfb()
+        lib.vmCalcPhiDevice(ses.vm, ses.bg)
+        lib.vmCalcJDeviceChanB(ses.vm, B.ctypes.data)
+        ses.beamform()
+        S = ses.stokes()
+    finally:
+        ses.close()
+    err = _errors(orc, S, Sref)
+    print("max per-beam rel err I,Q,U,V:", err.max(axis=0))
+    assert err[:, 0].max() <= tol_i
+    assert err[:, 1:].max() <= tol_quv
