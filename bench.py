@@ -191,10 +191,13 @@ def committed_traffic(kernel):
     return None
 
 
-def surrogate_stage(lib, nbeams=4096, nchan=128, steps=4000, batch=2048, nsamples=2_000_000):
+def surrogate_stage(lib, nbeams=4096, nchan=128, steps=4000, batch=2048, nsamples=2_000_000,
+                    feed=None):
     """Train the BF16 MLP primary-beam surrogate on the GPU, then produce Jones
     for nbeams directions (within 15 deg) x nchan fine channels with fallback
-    to the analytic beam; errors are against the analytic (FP64) beam."""
+    to the analytic beam; errors are against the analytic (FP64) beam.  With
+    feed = (pkg, session, obs, az, za) the surrogate's Jones for the session's
+    pointings drive its MFMA beamformer (surrogate_feed)."""
     import ctypes as C
     s = lib.vmSurrogateCreate(7, 0.01, 50e-12)
     try:
@@ -222,7 +225,9 @@ def surrogate_stage(lib, nbeams=4096, nchan=128, steps=4000, batch=2048, nsample
         e_nom = np.sqrt(((Bn - Bt) ** 2).sum((2, 3)) / (Bt ** 2).sum((2, 3)))
         Bc = np.repeat(Bt[:, nchan // 2:nchan // 2 + 1], nchan, axis=1)
         e_cen = np.sqrt(((Bc - Bt) ** 2).sum((2, 3)) / (Bt ** 2).sum((2, 3)))
-        return {"train_samples": nsamples, "train_steps": steps, "batch": batch,
+        fed = surrogate_feed(lib, s, *feed) if feed else None
+        return {"feeding_mfma_beamformer": fed,
+                "train_samples": nsamples, "train_steps": steps, "batch": batch,
                 "train_s": round(t_train, 2), "final_loss": float(f"{loss:.3e}"),
                 "jones_evaluations": nbeams * nchan,
                 "infer_with_fallback_ms": round(t_sur * 1e3, 2),
@@ -461,6 +466,57 @@ def kernel_ms_global(lib, name):
     return ms, n.value
 
 
+def surrogate_feed(lib, s, pkg, sess, obs, az, za, steps=5, nbeam_check=16, nsamp_check=64):
+    """The session's pointings with per-pointing, per-channel primary beams
+    from the surrogate (with fallback) on its MFMA FP16 beamformer: a step is
+    FP64 phases, J^-1 from D and those beams (vmCalcJDeviceChanB), weights,
+    prep and GEMM; error against the FP64 oracle chain with the analytic beam
+    on nbeam_check beams x nsamp_check samples x 128 channels."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc  # the checker (error only)
+    from vcsbeam_amd.workloads import unit_vectors
+    npnt, nch = len(az), 128
+    B = np.empty((npnt, nch, 4, 2))
+    err = np.empty((npnt, nch), np.float32)
+    nfb = lib.vmSurrogateJones(s, 3, az.ctypes.data, za.ctypes.data, npnt, obs.freqs.ctypes.data,
+                               nch, B.ctypes.data, err.ctypes.data)
+    sess.v.beamform_form = pkg.VM_BF_MFMA_FP16
+
+    def step():
+        lib.vmCalcPhiDevice(sess.vm, sess.bg)
+        lib.vmCalcJDeviceChanB(sess.vm, B.ctypes.data)
+        sess.beamform()
+
+    step()
+    lib.vmDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    lib.vmDeviceSynchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    S = sess.stokes()
+    pk, n = min(nbeam_check, npnt), nsamp_check
+    Bt = np.empty((pk, nch, 4, 2))
+    lib.vmSurrogateJones(s, 0, az.ctypes.data, za.ctypes.data, pk, obs.freqs.ctypes.data, nch,
+                         Bt.ctypes.data, None)
+    Bt = (Bt[..., 0] + 1j * Bt[..., 1]).reshape(pk, nch, 2, 2)
+    D = obs.calibration()
+    J = np.empty((pk, obs.ntiles, nch, 2, 2), complex)
+    for c in range(nch):
+        J[:, :, c] = orc.calc_J(D[:, c:c + 1], np.repeat(Bt[:, c][:, None], obs.ntiles, axis=1))[:, :, 0]
+    v = np.ascontiguousarray(sess.voltages()[:, :, :n].transpose(2, 0, 1)).astype(np.complex128)
+    E, N, H, L = obs.positions()
+    phi = orc.calc_phi(E, N, H, L, unit_vectors(az[:pk], za[:pk]), obs.freqs)
+    polQ, polP = obs.pol_idx()
+    Sref, _ = orc.beamform(v, J, phi, polQ, polP, 1.0 / (2 * int((~obs.flag_mask).sum())), 4)
+    e = np.array([[orc.rel_err(S[p, :n, k, :], Sref[p, :, k, :]) for k in range(4)]
+                  for p in range(pk)]).max(axis=0)
+    return {"beams": npnt, "value": round(npnt * nch * obs.nspectra / (ms * 1e-3), 1),
+            "unit": "beam-samples/s", "ms_per_step": round(ms, 3), "fallbacks": int(nfb),
+            "max_beam_rel_err_IQUV_vs_analytic_fp64": [float(f"{x:.3e}") for x in e],
+            "check": f"{pk} beams x {n} samples x 128 ch"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -522,12 +578,30 @@ def main():
                             nstokes=4, form=pkg.VM_BF_MFMA_FP16)
     sess.read(0)
     print("[bench] forward PFB", file=sys.stderr, flush=True)
-    sess.pfb()
+    for _ in range(10):  # warm-up: clocks settle under load
+        sess.pfb()
     lib.vmSetKernelTiming(sess.vm, 1)
-    for _ in range(5):
+    for _ in range(10):
         lib.vmWOLAChunk(sess.vm)
     pfb_ms, _ = kernel_ms(lib, sess, "pfb")
     lib.vmSetKernelTiming(sess.vm, 0)
+    # measured HBM ceilings on this device: a 2.62 GB write (hipMemset) and
+    # a 1 GiB copy kernel (read + write)
+    nsamp_in_bytes = obs.ninput * obs.nspectra * 128 * 8
+    ceil_buf = pkg.DeviceArray(nsamp_in_bytes)
+    lib.vmMemsetDevice(ceil_buf.ptr, 0, nsamp_in_bytes)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        lib.vmMemsetDevice(ceil_buf.ptr, 0, nsamp_in_bytes)
+    write_GBps = 5 * nsamp_in_bytes / (time.perf_counter() - t0) / 1e9
+    del ceil_buf
+    ca, cb = pkg.DeviceArray(COPY_BYTES), pkg.DeviceArray(COPY_BYTES)
+    lib.vmDeviceCopy(cb.ptr, ca.ptr, COPY_BYTES)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        lib.vmDeviceCopy(cb.ptr, ca.ptr, COPY_BYTES)
+    copy_GBps = 5 * 2 * COPY_BYTES / (time.perf_counter() - t0) / 1e9
+    del ca, cb
     nsamp_in = obs.ninput * obs.nspectra * 128
     pfb_bytes = nsamp_in * 10  # 2 B in + 8 B out per input complex sample
     beam_samples = args.beams * 128 * obs.nspectra
@@ -587,6 +661,10 @@ def main():
         "stages": {"pfb_fused_ms_per_ch_s": round(pfb_ms, 4),
                    "pfb_GBps": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9, 1),
                    "pfb_hbm_frac": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "measured_write_GBps": round(write_GBps, 1),
+                   "measured_copy_GBps": round(copy_GBps, 1),
+                   "pfb_frac_of_measured_write": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9
+                                                       / write_GBps, 4),
                    "beamform_prep_ms": round(prep_ms, 4),
                    "beamform_weights_ms": round(weights_ms, 4)},
     }
@@ -602,7 +680,7 @@ def main():
     if not args.no_surrogate:
         print("[bench] primary-beam surrogate: train + 4096 beams x 128 channels",
               file=sys.stderr, flush=True)
-        result["surrogate"] = surrogate_stage(lib)
+        result["surrogate"] = surrogate_stage(lib, feed=(pkg, sess, obs, az, za))
     if not args.no_cpu:
         print("[bench] CPU baseline (FP64 oracle)", file=sys.stderr, flush=True)
         cb = cpu_baseline(sess, obs, az, za)

@@ -550,6 +550,8 @@ void vmSetMaxGPUMem(vcsbeam_context *vm, int nchunks);
 void vmPushChunk(vcsbeam_context *vm);
 void vmPushJ(vcsbeam_context *vm);
 void vmCreatePrimaryBeam(vcsbeam_context *vm);
+/* include/vcsbeam.h.in:1213; called by app/make_mwa_tied_array_beam.c:375 */
+void free_primary_beam(primary_beam *pb);
 void vmCreateGeometricDelays(vcsbeam_context *vm);
 void vmCreateCudaStreams(vcsbeam_context *vm);
 void vmDestroyCudaStreams(vcsbeam_context *vm);

@@ -248,6 +248,7 @@ _SIGS = {
     "vmPushPhi": (None, [_CTX]),
     "vmCreateGeometricDelays": (None, [_CTX]),
     "vmCreatePrimaryBeam": (None, [_CTX]),
+    "free_primary_beam": (None, [C.c_void_p]),
     "vmCreateCudaStreams": (None, [_CTX]),
     "vmDestroyCudaStreams": (None, [_CTX]),
     "vmCreateStatistics": (None, [_CTX, C.c_void_p]),

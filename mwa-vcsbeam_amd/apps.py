@@ -131,6 +131,7 @@ def tied_array_beam(obs, az, za, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, nch
               "vmFreeSDevice", "vmFreeJDevice", "vmFreeJHost", "vmFreeDHost", "vmFreeDDevice",
               "vmFreePQIdxsDevice", "vmFreePQIdxsHost"):
         getattr(lib, f)(vm)
+    lib.free_primary_beam(C.addressof(v.pb))
     lib.destroy_vcsbeam_context(vm)
     return out
 
@@ -343,6 +344,7 @@ class TiedArraySession:
         if self.v.e:
             lib.vmFreeEHost(vm)
             lib.vmFreeEDevice(vm)
+        lib.free_primary_beam(C.addressof(self.v.pb))
         lib.destroy_vcsbeam_context(vm)
         self.vm = None
 

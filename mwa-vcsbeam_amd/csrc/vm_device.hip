@@ -321,11 +321,21 @@ void vmCreatePrimaryBeam(vcsbeam_context *vm)
     pb->npol = vm->obs_metadata->num_visibility_pols;
     pb->obs_metadata = vm->obs_metadata;
     size_t n = pb->npointings * pb->nant * pb->npol;
-    pb->B = (gpuDoubleComplex *)calloc(n, sizeof(gpuDoubleComplex));
+    /* pinned, as it is copied to the device every second (vmCalcJDevice) */
+    pb->B = (gpuDoubleComplex *)vm_host_alloc_pinned(n * sizeof(gpuDoubleComplex));
+    memset(pb->B, 0, n * sizeof(gpuDoubleComplex));
     for (size_t j = 0; j < pb->npointings * pb->nant; j++) {
         pb->B[4 * j + 0].x = 1.0;
         pb->B[4 * j + 3].x = 1.0;
     }
+}
+
+void free_primary_beam(primary_beam *pb)
+{
+    /* src/primary_beam.c (free_primary_beam): releases B */
+    if (!pb) return;
+    vm_host_free_pinned(pb->B);
+    pb->B = NULL;
 }
 
 void vmCreateCudaStreams(vcsbeam_context *vm)
