@@ -482,18 +482,31 @@ def surrogate_feed(lib, s, pkg, sess, obs, az, za, steps=5, nbeam_check=16, nsam
                                nch, B.ctypes.data, err.ctypes.data)
     sess.v.beamform_form = pkg.VM_BF_MFMA_FP16
 
+    host = {"phi": 0.0, "jones": 0.0, "beamform": 0.0}
+
     def step():
+        t = time.perf_counter()
         lib.vmCalcPhiDevice(sess.vm, sess.bg)
+        t1 = time.perf_counter()
         lib.vmCalcJDeviceChanB(sess.vm, B.ctypes.data)
+        t2 = time.perf_counter()
         sess.beamform()
+        t3 = time.perf_counter()
+        host["phi"] += t1 - t
+        host["jones"] += t2 - t1
+        host["beamform"] += t3 - t2
 
     step()
     lib.vmDeviceSynchronize()
+    lib.vmSetKernelTiming(sess.vm, 1)
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     lib.vmDeviceSynchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    kern = {k: round(kernel_ms(lib, sess, k)[0], 3)
+            for k in ("geometry", "beamform_weights", "beamform_prep", "beamform_mfma")}
+    lib.vmSetKernelTiming(sess.vm, 0)
     S = sess.stokes()
     pk, n = min(nbeam_check, npnt), nsamp_check
     Bt = np.empty((pk, nch, 4, 2))
@@ -512,7 +525,9 @@ def surrogate_feed(lib, s, pkg, sess, obs, az, za, steps=5, nbeam_check=16, nsam
     e = np.array([[orc.rel_err(S[p, :n, k, :], Sref[p, :, k, :]) for k in range(4)]
                   for p in range(pk)]).max(axis=0)
     return {"beams": npnt, "value": round(npnt * nch * obs.nspectra / (ms * 1e-3), 1),
-            "unit": "beam-samples/s", "ms_per_step": round(ms, 3), "fallbacks": int(nfb),
+            "unit": "beam-samples/s", "ms_per_step": round(ms, 3), "kernel_ms": kern,
+            "call_ms": {k: round(v * 1e3 / (steps + 1), 3) for k, v in host.items()},
+            "fallbacks": int(nfb),
             "max_beam_rel_err_IQUV_vs_analytic_fp64": [float(f"{x:.3e}") for x in e],
             "check": f"{pk} beams x {n} samples x 128 ch"}
 

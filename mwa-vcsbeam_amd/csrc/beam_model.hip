@@ -46,6 +46,7 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 #define NOUT 8
 #define HID 256
 #define NLAYER 4
+#define KSPLIT 16 // batch ranges of the weight-gradient GEMMs
 #define RES_SCALE 32.0 // residual scale of the training targets
 #define FALLBACK_ERR 1e-3
 
@@ -215,6 +216,8 @@ __global__ void k_surr_assemble(const double *__restrict__ az, const double *__r
 // Workgroup 256 lanes = 2 x 2 wavefronts, 64 x 64 tile, K steps of 16 staged
 // in LDS.  Epilogue EPI 0: Cf[m ldc + n] = acc (+ bias[n] if bias);
 // EPI 1: z = acc + bias[n], Cf = z (pre-activation), Hb = bf16(gelu(z)).
+// Split K: workgroup z sums k in [z kc, (z+1) kc) into the partial C at
+// Cf + z M ldc (kc a multiple of 16; one split: kc >= K).
 __device__ __forceinline__ float gelu_f(float x)
 {
     const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
@@ -233,7 +236,7 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const __b
                                                   long sam, long sak, const __bf16 *__restrict__ Bm,
                                                   long sbk, long sbn, const float *__restrict__ bias,
                                                   float *__restrict__ Cf, long ldc,
-                                                  __bf16 *__restrict__ Hb, long ldh)
+                                                  __bf16 *__restrict__ Hb, long ldh, int kc)
 {
     __shared__ __attribute__((aligned(16))) __bf16 As[64][24]; // [m][k], 16 + 8 pad
     __shared__ __attribute__((aligned(16))) __bf16 Bs[64][24]; // [n][k]
@@ -244,13 +247,15 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const __b
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = 0.f;
     const int lr = t >> 2, lk = (t & 3) * 4; // loader: row/col lr, k lk..lk+3
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    const int kbeg = blockIdx.z * kc, kend = min(K, kbeg + kc);
+    Cf += (long)blockIdx.z * M * ldc;
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int k = k0 + lk + e;
             const int m = m0 + lr, n = n0 + lr;
-            As[lr][lk + e] = (m < M && k < K) ? A[(long)m * sam + (long)k * sak] : (__bf16)0.0f;
-            Bs[lr][lk + e] = (n < N && k < K) ? Bm[(long)k * sbk + (long)n * sbn] : (__bf16)0.0f;
+            As[lr][lk + e] = (m < M && k < kend) ? A[(long)m * sam + (long)k * sak] : (__bf16)0.0f;
+            Bs[lr][lk + e] = (n < N && k < kend) ? Bm[(long)k * sbk + (long)n * sbn] : (__bf16)0.0f;
         }
         __syncthreads();
         const bf8 a = *(const bf8 *)&As[32 * wr + (lane & 31)][8 * (lane >> 5)];
@@ -296,14 +301,31 @@ __global__ void k_gelu_bwd(const float *__restrict__ dH, const float *__restrict
     if (j < n) dZ[j] = (__bf16)(dH[j] * dgelu_f(Z[j]));
 }
 
-// db[o] = sum_b dZ[b][o]; one lane per o, rows strided
-__global__ void k_bias_grad(const __bf16 *__restrict__ dZ, int B, int no, float *__restrict__ db)
+// db[o] = sum_b dZ[b][o]; one workgroup per o, lanes over rows, then a
+// fixed-order reduction (the same sum on every run)
+__global__ __launch_bounds__(256) void k_bias_grad(const __bf16 *__restrict__ dZ, int B, int no,
+                                                   float *__restrict__ db)
 {
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= no) return;
+    __shared__ float part[4];
+    const int o = blockIdx.x, t = threadIdx.x;
     float s = 0.f;
-    for (int b = 0; b < B; b++) s += (float)dZ[(size_t)b * no + o];
-    db[o] = s;
+    for (int b = t; b < B; b += 256) s += (float)dZ[(size_t)b * no + o];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((t & 63) == 0) part[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) db[o] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+// out[j] = sum_z part[z][j] in order z = 0, 1, ...
+__global__ void k_sum_parts(const float *__restrict__ part, int nz, size_t n,
+                            float *__restrict__ out)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float s = part[j];
+    for (int z = 1; z < nz; z++) s += part[(size_t)z * n + j];
+    out[j] = s;
 }
 
 // Adam on FP32 masters, refreshing the bf16 copies W (and W^T if given)
@@ -353,6 +375,7 @@ struct vm_beam_surrogate {
     __bf16 *d_H[NLAYER];   // activations (bf16) [B][dims[l+1]] (l < NLAYER-1)
     float *d_dH;           // gradient w.r.t. an activation (fp32) [B][HID]
     __bf16 *d_dZ[NLAYER];  // gradient w.r.t. a pre-activation (bf16)
+    float *d_part;         // split-K partial weight gradients [KSPLIT][HID][HID]
     float *d_loss;
     long step;
     hipStream_t stream;
@@ -398,6 +421,7 @@ static void ensure_batch(vm_beam_surrogate *s, int B)
     }
     if (s->d_dH) HIPCHECK(hipFree(s->d_dH));
     s->d_dH = (float *)dalloc(sizeof(float) * B * HID);
+    if (!s->d_part) s->d_part = (float *)dalloc(sizeof(float) * KSPLIT * HID * HID);
     s->Bmax = B;
 }
 
@@ -461,6 +485,7 @@ void vmSurrogateDestroy(vm_beam_surrogate *s)
             if (p) HIPCHECK(hipFree(p));
     }
     if (s->d_dH) HIPCHECK(hipFree(s->d_dH));
+    if (s->d_part) HIPCHECK(hipFree(s->d_part));
     HIPCHECK(hipFree(s->d_loss));
     HIPCHECK(hipFree(s->d_cfg));
     HIPCHECK(hipStreamDestroy(s->stream));
@@ -478,11 +503,11 @@ static void mlp_forward(vm_beam_surrogate *s, const __bf16 *X, int B)
         if (l < NLAYER - 1)
             hipLaunchKernelGGL(k_mlp_gemm<1>, g, dim3(256), 0, s->stream, B, no, ni, in, (long)ni,
                                1L, s->d_Wb[l], 1L, (long)ni, s->d_b[l], s->d_Z[l], (long)no,
-                               s->d_H[l], (long)no);
+                               s->d_H[l], (long)no, ni);
         else
             hipLaunchKernelGGL(k_mlp_gemm<0>, g, dim3(256), 0, s->stream, B, no, ni, in, (long)ni,
                                1L, s->d_Wb[l], 1L, (long)ni, s->d_b[l], s->d_Z[l], (long)no,
-                               (__bf16 *)NULL, 0L);
+                               (__bf16 *)NULL, 0L, ni);
     }
     HIPCHECK(hipGetLastError());
 }
@@ -497,18 +522,23 @@ static void mlp_backward(vm_beam_surrogate *s, const __bf16 *X, const float *T, 
     for (int l = NLAYER - 1; l >= 0; l--) {
         const int ni = kDims[l], no = kDims[l + 1];
         const __bf16 *in = l == 0 ? X : s->d_H[l - 1];
-        // dW[o][i] = sum_b dZ[b][o] in[b][i]
-        hipLaunchKernelGGL(k_mlp_gemm<0>, dim3((ni + 63) / 64, (no + 63) / 64), dim3(256), 0,
+        // dW[o][i] = sum_b dZ[b][o] in[b][i]: the batch split into KSPLIT
+        // ranges (workgroups in parallel), partials summed in order
+        const int kc = ((B + KSPLIT - 1) / KSPLIT + 15) / 16 * 16;
+        const int nz = (B + kc - 1) / kc;
+        hipLaunchKernelGGL(k_mlp_gemm<0>, dim3((ni + 63) / 64, (no + 63) / 64, nz), dim3(256), 0,
                            s->stream, no, ni, B, s->d_dZ[l], 1L, (long)no, in, (long)ni, 1L,
-                           (const float *)NULL, s->d_gW[l], (long)ni, (__bf16 *)NULL, 0L);
-        hipLaunchKernelGGL(k_bias_grad, dim3(nblk(no)), dim3(256), 0, s->stream, s->d_dZ[l], B,
-                           no, s->d_gb[l]);
+                           (const float *)NULL, s->d_part, (long)ni, (__bf16 *)NULL, 0L, kc);
+        hipLaunchKernelGGL(k_sum_parts, dim3(nblk((size_t)no * ni)), dim3(256), 0, s->stream,
+                           s->d_part, nz, (size_t)no * ni, s->d_gW[l]);
+        hipLaunchKernelGGL(k_bias_grad, dim3(no), dim3(256), 0, s->stream, s->d_dZ[l], B, no,
+                           s->d_gb[l]);
         if (l > 0) {
             // dH[b][i] = sum_o dZ[b][o] W[o][i], then through the GELU
             hipLaunchKernelGGL(k_mlp_gemm<0>, dim3((ni + 63) / 64, (B + 63) / 64), dim3(256), 0,
                                s->stream, B, ni, no, s->d_dZ[l], (long)no, 1L, s->d_Wt[l], 1L,
                                (long)no, (const float *)NULL, s->d_dH, (long)ni, (__bf16 *)NULL,
-                               0L);
+                               0L, no);
             hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk((size_t)B * ni)), dim3(256), 0, s->stream,
                                s->d_dH, s->d_Z[l - 1], (size_t)B * ni, s->d_dZ[l - 1]);
         }

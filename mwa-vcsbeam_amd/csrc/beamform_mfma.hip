@@ -303,17 +303,23 @@ __global__ void k_jf(const double2 *__restrict__ J, int nant, int nchan,
     }
 }
 
-// flag when J^-1 of any pointing p >= 1 differs from pointing 0's
-__global__ void k_jcheck(const double2 *__restrict__ J, size_t per_pointing, size_t total,
+// flag when J^-1 of any pointing p >= 1 differs from pointing 0's; one
+// workgroup per (pointing p >= 1, block of 4096 entries), 32-bit indexing,
+// and a wavefront sets the flag only while it is still clear
+__global__ void k_jcheck(const double2 *__restrict__ J, uint32_t per_pointing,
                          uint32_t *__restrict__ flag)
 {
-    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x + per_pointing;
+    const uint32_t p = blockIdx.y + 1;
+    const double2 *Jp = J + (size_t)p * per_pointing;
     bool d = false;
-    if (j < total) {
-        const double2 m = J[j], m0 = J[j % per_pointing];
-        d = (m.x != m0.x) || (m.y != m0.y);
+    for (uint32_t j = blockIdx.x * 4096u + threadIdx.x; j < min(per_pointing, blockIdx.x * 4096u + 4096u);
+         j += 256u) {
+        const double2 m = Jp[j], m0 = J[j];
+        d |= (m.x != m0.x) || (m.y != m0.y);
     }
-    if (__any(d) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+    if (__any(d) && (threadIdx.x & 63) == 0 && __hip_atomic_load(flag, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT) == 0u)
+        atomicOr(flag, 1u);
 }
 
 // ------------------------------------------------------- voltage prep -----
@@ -1004,9 +1010,9 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
             ensure_buf(&d->d_flag, &d->flag_bytes, 256);
             uint32_t *jflag = (uint32_t *)d->d_flag;
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, jflag, (size_t)1, 0u);
-            const size_t per = (size_t)nant * nchan * 4, tot = per * np;
-            hipLaunchKernelGGL(k_jcheck, dim3((unsigned)((tot - per + 255) / 256)), dim3(256), 0,
-                               st, (const double2 *)vm->d_J, per, tot, jflag);
+            const uint32_t per = (uint32_t)nant * nchan * 4;
+            hipLaunchKernelGGL(k_jcheck, dim3((per + 4095) / 4096, (unsigned)(np - 1)), dim3(256),
+                               0, st, (const double2 *)vm->d_J, per, jflag);
             HIPCHECK(hipGetLastError());
             uint32_t hflag = 0;
             HIPCHECK(hipMemcpyAsync(&hflag, jflag, 4, hipMemcpyDeviceToHost, st));
@@ -1405,9 +1411,9 @@ static void batch_refresh(vm_batch *b)
         uint32_t hflag = 0;
         if (np > 1) {
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, queue_flag, (size_t)1, 0u);
-            const size_t per = (size_t)nant * nchan * 4, tot = per * np;
-            hipLaunchKernelGGL(k_jcheck, dim3((unsigned)((tot - per + 255) / 256)), dim3(256), 0,
-                               st, (const double2 *)vm->d_J, per, tot, queue_flag);
+            const uint32_t per = (uint32_t)nant * nchan * 4;
+            hipLaunchKernelGGL(k_jcheck, dim3((per + 4095) / 4096, (unsigned)(np - 1)), dim3(256),
+                               0, st, (const double2 *)vm->d_J, per, queue_flag);
             HIPCHECK(hipMemcpyAsync(&hflag, queue_flag, 4, hipMemcpyDeviceToHost, st));
         }
         HIPCHECK(hipGetLastError());
