@@ -575,15 +575,31 @@ void vm_pfb_fused_launch(const void *d_htr, size_t htr_bytes, const int *d_h,
         VM_FATAL("fused PFB: chunk input of %zu bytes exceeds 2 GiB; use more chunks", htr_bytes);
     const uint32_t *x = (const uint32_t *)d_htr;
     cf *o = (cf *)d_out;
-    // persistent launch: 4 workgroups per CU (LDS-limited), contiguous runs
-    static int ncu = 0;
+    // persistent launch, contiguous runs: exactly as many workgroups as are
+    // resident at once (registers and LDS decide how many per CU), so no
+    // workgroup starts after the first ones finish
+    static int ncu = 0, bpc[4] = {0, 0, 0, 0};
     if (!ncu) {
         int dev = 0;
         HIPCHECK(hipGetDevice(&dev));
         HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     }
+    const int v = (fpga ? 2 : 0) + (split ? 1 : 0);
+    if (!bpc[v]) {
+        hipError_t e;
+        if (fpga && split)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc[v], k_pfb_fused<true, true>, 256, 0);
+        else if (fpga)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc[v], k_pfb_fused<true, false>, 256, 0);
+        else if (split)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc[v], k_pfb_fused<false, true>, 256, 0);
+        else
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc[v], k_pfb_fused<false, false>, 256, 0);
+        HIPCHECK(e);
+        if (bpc[v] < 1) bpc[v] = 1;
+    }
     const int ntiles = ((nspc + TILE_M - 1) / TILE_M) * I;
-    const int nwg = min(ntiles, 4 * ncu);
+    const int nwg = min(ntiles, bpc[v] * ncu);
     const int per = (ntiles + nwg - 1) / nwg;
     const dim3 grid((ntiles + per - 1) / per), block(256);
 #define VM_PFB(F_, S_)                                                                       \
