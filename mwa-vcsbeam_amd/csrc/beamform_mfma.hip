@@ -624,7 +624,35 @@ __device__ __forceinline__ void stokes_pb(float xr, float xi, float yr, float yi
 // VEC: ns, ns_total and soffset are multiples of 4, so every in-range
 // 4-sample group is stored whole with one 16-byte store (the other form
 // stores sample by sample).
-// PB: the per-pointing primary-beam epilogue (stokes_pb).
+// PB: the per-pointing primary-beam epilogue (stokes_pb); stg then also
+// holds a 16-float row per pointing of each wavefront after the tiles.
+// Each wavefront's staging tile holds 32 rows of 64 samples (68-float rows,
+// padded against bank conflicts).
+#define BF_WAVE_LDS (32 * 68)
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool PB>
+constexpr int bf_lds_floats() { return WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 16 : 0); }
+
+// Stokes of one beam-sample from the GEMM outputs (x, y), in either form;
+// the per-pointing form reads the pointing's B^-1, h and weight exponent from
+// its LDS row pl of wpb and gives zeros for pointings past the last (valid
+// false).
+template <bool PB>
+__device__ __forceinline__ void bf_stokes(float xr, float xi, float yr, float yi, float unscale2,
+                                          int esb, const float4 &nn, const float4 (&T4)[4],
+                                          const float *wpb, int pl, bool valid, float invw,
+                                          float (&st4)[4])
+{
+    if constexpr (PB) {
+        const float4 *bl = (const float4 *)(wpb + 16 * pl);
+        float4 bx[3] = {bl[0], bl[1], bl[2]};
+        const float u2 = ldexpf(1.0f, -2 * (esb + __float_as_int(wpb[16 * pl + 12])));
+        stokes_pb(xr, xi, yr, yi, u2, T4, bx, invw, st4);
+        if (!valid) st4[0] = st4[1] = st4[2] = st4[3] = 0.f;
+    } else {
+        stokes4(xr, xi, yr, yi, unscale2, nn, invw, st4);
+    }
+}
+
 template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool VEC, bool PB>
 __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int nchan, int c,
                                         int tx, int ty, float *stg)
@@ -659,6 +687,28 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
         for (int n = 0; n < NB; n++)
 #pragma unroll
             for (int q = 0; q < 16; q++) acc[m][n][q] = 0.f;
+
+    // The epilogue's per-sample inputs (auto-correlation terms N, block
+    // exponents) are requested before the k-loop: the vector memory counter
+    // retires loads and stores in issue order, so a load issued after the
+    // first Stokes stores would wait for all of them to complete.
+    const int lr = lane & 31;
+    float4 nnv[NSB];
+    int esbv[NSB];
+#pragma unroll
+    for (int j = 0; j < NSB; j++) {
+        const int sb = sb0 + j;
+        const int s = SB * sb + lr;
+        nnv[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!PB && s < ns) {
+            const f4v n4 = ((const gmem f4v *)G.N)[(size_t)c * ns + s];
+            nnv[j] = make_float4(n4.x, n4.y, n4.z, n4.w);
+        }
+        // one exponent per sample block: a wave-uniform address, read by a
+        // scalar load (its own counter) through the constant address space
+        esbv[j] = ((const __attribute__((address_space(4))) int *)G.sexp)[(size_t)c * nsb +
+                                                                          min(sb, nsb - 1)];
+    }
 
     // Two register rings: A (weights, small and L2-resident) two k-steps
     // deep, B (voltage fragments, streamed from HBM) NST deep, so each B
@@ -695,33 +745,95 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
     // (jg = 2 h + jj) is 8 beams; LDS rows r = st * 8 + (4 jj + 2 b + lk) for
     // beam 4 jg + 2 lk + b: the two lane halves write rows 32 floats apart and
     // each 16-lane read group covers two whole rows.
-    float *wl = stg + 1024 * w;
-    const int lr = lane & 31, lk = lane >> 5, qq = lane & 7;
+    // per-pointing-beam form: its T_k terms (16 floats per sample) would
+    // cost registers through the k-loop, so they are requested here, after
+    // it and before the first store
+    // Its pointings' B^-1 and h (12 floats) and weight exponents go to a
+    // 16-float LDS row per pointing of the wavefront (16 MB pointings).
+    float4 T4v[PB ? NSB : 1][4];
+    float *wpb = stg + WR * WC * BF_WAVE_LDS + (PB ? w * MB * 16 * 16 : 0);
+    if constexpr (PB) {
+        for (int pl = lane; pl < MB * 16; pl += 64) {
+            const int cp = c * G.npointing + min(16 * rb0 + pl, G.npointing - 1);
+            const gmem f4v *bp = (const gmem f4v *)(G.bx + (size_t)cp * 12);
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const f4v t = bp[q];
+                *(float4 *)(wpb + 16 * pl + 4 * q) = make_float4(t.x, t.y, t.z, t.w);
+            }
+            wpb[16 * pl + 12] = __int_as_float(((const gmem int *)G.esp)[cp]);
+        }
+#pragma unroll
+        for (int j = 0; j < NSB; j++) {
+            const int s = SB * (sb0 + j) + lr;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                T4v[j][k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (s < ns) {
+                    const f4v t4 = ((const gmem f4v *)G.N)[((size_t)c * ns + s) * 4 + k];
+                    T4v[j][k] = make_float4(t4.x, t4.y, t4.z, t4.w);
+                }
+            }
+        }
+    }
+    float *wl = stg + BF_WAVE_LDS * w;
+    const int lk = lane >> 5;
     const int nstokes = G.nstokes, npointing = G.npointing;
     const float invw = G.invw;
     const size_t pstride = (size_t)nstokes * nchan * G.ns_total;
     const size_t ststride = (size_t)nchan * G.ns_total;
+    float us2[NSB];
+#pragma unroll
+    for (int j = 0; j < NSB; j++) us2[j] = ldexpf(1.0f, -2 * (esbv[j] + PHASE_EXP));
+    if constexpr (VEC && NSB == 2) {
+        // Both sample blocks of a half block go through one 32-row x 64-sample
+        // LDS tile, and 16 lanes store one row: every store instruction writes
+        // 4 rows x 256 contiguous bytes (HBM takes such runs markedly faster
+        // than 128-byte ones).
+        const int q16 = lane & 15;
+        const int s4 = SB * sb0 + 4 * q16;
+        gmem float *Sc = (gmem float *)G.S + (size_t)c * G.ns_total + G.soffset + s4;
+#pragma unroll
+        for (int m = 0; m < MB; m++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+                        for (int b = 0; b < 2; b++) {
+                            const int jg = 2 * h + jj;
+                            const f16v &ax = acc[m][2 * j], &ay = acc[m][2 * j + 1];
+                            const int pl = 16 * m + 8 * h + 4 * jj + 2 * lk + b;
+                            float st4[4];
+                            bf_stokes<PB>(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1],
+                                          ay[4 * jg + 2 * b], ay[4 * jg + 2 * b + 1], us2[j],
+                                          esbv[j], nnv[j], T4v[PB ? j : 0], wpb, pl,
+                                          16 * rb0 + pl < npointing, invw, st4);
+#pragma unroll
+                            for (int st = 0; st < 4; st++)
+                                wl[(st * 8 + 4 * jj + 2 * b + lk) * 68 + 32 * j + lr] = st4[st];
+                        }
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int r = 4 * i + (lane >> 4);
+                    const int st = r >> 3, bbp = r & 7;
+                    // bbp = 4 jj + 2 b + lk  ->  beam in the half = 4 jj + 2 lk + b
+                    const int bb = 4 * (bbp >> 2) + 2 * (bbp & 1) + ((bbp >> 1) & 1);
+                    const int p = 16 * (rb0 + m) + 8 * h + bb;
+                    const float4 val = *(const float4 *)(wl + r * 68 + 4 * q16);
+                    if (st >= nstokes || p >= npointing || s4 >= ns) continue;
+                    *(gmem f4v *)(Sc + (size_t)p * pstride + (size_t)st * ststride) =
+                        f4v{val.x, val.y, val.z, val.w};
+                }
+            }
+        return;
+    }
+    const int qq = lane & 7;
 #pragma unroll
     for (int j = 0; j < NSB; j++) {
         const int sb = sb0 + j;
-        const int s = SB * sb + lr;
-        float4 nn = make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 T4[4];
-        if constexpr (PB) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                T4[k] = nn;
-                if (s < ns) {
-                    const f4v t4 = ((const gmem f4v *)G.N)[((size_t)c * ns + s) * 4 + k];
-                    T4[k] = make_float4(t4.x, t4.y, t4.z, t4.w);
-                }
-            }
-        } else if (s < ns) {
-            const f4v n4 = ((const gmem f4v *)G.N)[(size_t)c * ns + s];
-            nn = make_float4(n4.x, n4.y, n4.z, n4.w);
-        }
-        const int esb = ((const gmem int *)G.sexp)[(size_t)c * nsb + min(sb, nsb - 1)];
-        const float unscale2 = ldexpf(1.0f, -2 * (esb + PHASE_EXP));
         const int s4 = SB * sb + 4 * qq;
         gmem float *Sc = (gmem float *)G.S + (size_t)c * G.ns_total + G.soffset + s4;
 #pragma unroll
@@ -734,29 +846,12 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
 #pragma unroll
                     for (int b = 0; b < 2; b++) {
                         const f16v &ax = acc[m][2 * j], &ay = acc[m][2 * j + 1];
+                        const int pl = 16 * m + 8 * h + 4 * jj + 2 * lk + b;
                         float st4[4];
-                        if constexpr (PB) {
-                            const int pb = 16 * (rb0 + m) + 8 * h + 4 * jj + 2 * lk + b;
-                            st4[0] = st4[1] = st4[2] = st4[3] = 0.f;
-                            if (pb < npointing) {
-                                const size_t cp = (size_t)c * npointing + pb;
-                                const gmem f4v *bp = (const gmem f4v *)(G.bx + cp * 12);
-                                float4 bx[3];
-#pragma unroll
-                                for (int q = 0; q < 3; q++) {
-                                    const f4v t = bp[q];
-                                    bx[q] = make_float4(t.x, t.y, t.z, t.w);
-                                }
-                                const float u2 =
-                                    ldexpf(1.0f, -2 * (esb + ((const gmem int *)G.esp)[cp]));
-                                stokes_pb(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1],
-                                          ay[4 * jg + 2 * b], ay[4 * jg + 2 * b + 1], u2, T4, bx,
-                                          invw, st4);
-                            }
-                        } else {
-                            stokes4(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1], ay[4 * jg + 2 * b],
-                                    ay[4 * jg + 2 * b + 1], unscale2, nn, invw, st4);
-                        }
+                        bf_stokes<PB>(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1],
+                                      ay[4 * jg + 2 * b], ay[4 * jg + 2 * b + 1], us2[j], esbv[j],
+                                      nnv[j], T4v[PB ? j : 0], wpb, pl,
+                                      16 * rb0 + pl < npointing, invw, st4);
 #pragma unroll
                         for (int st = 0; st < 4; st++)
                             wl[(st * 8 + 4 * jj + 2 * b + lk) * 32 + lr] = st4[st];
@@ -766,7 +861,6 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                 for (int i = 0; i < 4; i++) {
                     const int r = 8 * i + (lane >> 3);
                     const int st = r >> 3, bbp = r & 7;
-                    // bbp = 4 jj + 2 b + lk  ->  beam in the half = 4 jj + 2 lk + b
                     const int bb = 4 * (bbp >> 2) + 2 * (bbp & 1) + ((bbp >> 1) & 1);
                     const int p = 16 * (rb0 + m) + 8 * h + bb;
                     const float4 val = *(const float4 *)(wl + r * 32 + 4 * qq);
@@ -796,7 +890,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, 
                                                                      int nchan, int ntx, int nty,
                                                                      int order)
 {
-    __shared__ __attribute__((aligned(16))) float stg[WR * WC * 1024];
+    __shared__ __attribute__((aligned(16))) float stg[bf_lds_floats<BF16, MB, NB, WR, WC, NST, PB>()];
     const int ntile = ntx * nty;
     const int L = blockIdx.x;
     int c, tile;
@@ -829,7 +923,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
     const bf_geom *__restrict__ geoms, const uint2 *__restrict__ items, int nitems,
     uint32_t *__restrict__ queue, int ns, int nsb, int nchan)
 {
-    __shared__ __attribute__((aligned(16))) float stg[WR * WC * 1024];
+    __shared__ __attribute__((aligned(16))) float stg[bf_lds_floats<BF16, MB, NB, WR, WC, NST, false>()];
     __shared__ int slot[2];
     for (int it = 0;; it++) {
         // two slots: a slot is rewritten only after the barrier of the next
