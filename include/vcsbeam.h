@@ -420,7 +420,8 @@ typedef struct vcsbeam_context_t {
     uintptr_t d_v_size_bytes;
 
     void *S, *d_S;             /* host S: [p][s][st][c] float (C_IDX);
-                                  device d_S: [p][st][c][s] float */
+                                  device d_S: [c][s/64][p][st][s%64] float,
+                                  the last 64-sample block padded */
     uintptr_t S_size_bytes;
     uintptr_t d_S_size_bytes;
 

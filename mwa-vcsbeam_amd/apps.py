@@ -18,6 +18,9 @@ from . import (ANALYSIS_FILTER, SYNTHESIS_FILTER, BeamGeom, GpuIpfbArrays, MpiPs
                PFB_FULL_PRECISION, PFB_MALLOC_HOST_OUTPUT, PFB_SMART, VM_BF_FP32, VM_SUCCESS, load)
 from .workloads import unit_vectors
 
+# samples per block of the device Stokes layout d_S [c][s / 64][p][st][s % 64]
+S_BLOCK = 64
+
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
@@ -313,19 +316,23 @@ class TiedArraySession:
         return out.reshape(v.nfine_chan, self.obs.ninput, v.fine_sample_rate)
 
     def stokes_slice(self, p0, p1, s0, s1):
-        """Device S [p][st][c][s] for pointings p0:p1 and samples s0:s1, read
-        without a full host copy; returned as [p][s][st][c] like stokes()."""
+        """Device S for pointings p0:p1 and samples s0:s1, read without a full
+        host copy; returned as [p][s][st][c] like stokes().  The device layout
+        is [c][s / 64][p][st][s % 64]: per channel, the sample blocks covering
+        s0:s1 are rows of a strided copy."""
         v = self.v
-        nst, nch, ns = v.out_nstokes, v.nfine_chan, v.fine_sample_rate
-        out = np.empty((p1 - p0, nst, nch, s1 - s0), np.float32)
+        nst, nch, ns, npt = v.out_nstokes, v.nfine_chan, v.fine_sample_rate, self.npointing
+        nblk = (ns + S_BLOCK - 1) // S_BLOCK
+        b0, b1 = s0 // S_BLOCK, (s1 + S_BLOCK - 1) // S_BLOCK
+        width = (p1 - p0) * nst * S_BLOCK * 4
+        pitch = npt * nst * S_BLOCK * 4
+        buf = np.empty((nch, b1 - b0, p1 - p0, nst, S_BLOCK), np.float32)
         base = C.cast(v.d_S, C.c_void_p).value
-        for p in range(p0, p1):
-            for st in range(nst):
-                for c in range(nch):
-                    off = ((p * nst + st) * nch + c) * ns + s0
-                    self.lib.vmMemcpyDtoH(out[p - p0, st, c].ctypes.data, base + 4 * off,
-                                          4 * (s1 - s0))
-        return np.ascontiguousarray(out.transpose(0, 3, 1, 2))
+        for c in range(nch):
+            src = base + 4 * ((c * nblk + b0) * npt + p0) * nst * S_BLOCK
+            self.lib.vmMemcpy2DDtoH(buf[c].ctypes.data, width, src, pitch, width, b1 - b0)
+        out = buf.transpose(2, 1, 4, 3, 0).reshape(p1 - p0, (b1 - b0) * S_BLOCK, nst, nch)
+        return np.ascontiguousarray(out[:, s0 - b0 * S_BLOCK:s1 - b0 * S_BLOCK])
 
     def close(self):
         lib, vm = self.lib, self.vm

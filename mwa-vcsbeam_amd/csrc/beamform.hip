@@ -14,9 +14,9 @@
 //  * One lane per (channel, sample), PB beams per lane; the antenna loop reads
 //    weights with wave-uniform (scalar) loads and voltages with coalesced loads
 //    along the sample axis of the VM_FLT layout.
-//  * Device outputs use sample-innermost layouts (d_S [p][st][c][s], d_e
-//    [p][c][s][pol]) so stores are coalesced; vmPullS / vmPullE transpose to the
-//    reference host layouts.
+//  * Device outputs use sample-innermost layouts (d_S in 64-sample blocks
+//    [c][s/64][p][st][s%64], d_e [p][c][s][pol]) so stores are coalesced;
+//    vmPullS / vmPullE transpose to the reference host layouts.
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
@@ -140,12 +140,12 @@ __global__ __launch_bounds__(256) void k_beamform_fp32(
         float bnYY = ey[b][0] * ey[b][0] + ey[b][1] * ey[b][1] - nyy[b];
         float xyr = ex[b][0] * ey[b][0] + ex[b][1] * ey[b][1] - nxy[b][0];
         float xyi = ex[b][1] * ey[b][0] - ex[b][0] * ey[b][1] - nxy[b][1];
-        float *Sp = S + (size_t)p * nstokes * nchan * ns_total;
-        Sp[((size_t)0 * nchan + c) * ns_total + so] = invw * (bnXX + bnYY);
+        float *Sp = S + VM_S_IDX(p, 0, c, so, npointing, nstokes, VM_S_NBLK(ns_total));
+        Sp[0] = invw * (bnXX + bnYY);
         if (nstokes == 4) {
-            Sp[((size_t)1 * nchan + c) * ns_total + so] = invw * (bnXX - bnYY);
-            Sp[((size_t)2 * nchan + c) * ns_total + so] = 2.0f * invw * xyr;
-            Sp[((size_t)3 * nchan + c) * ns_total + so] = -2.0f * invw * xyi;
+            Sp[VM_SBLK] = invw * (bnXX - bnYY);
+            Sp[2 * VM_SBLK] = 2.0f * invw * xyr;
+            Sp[3 * VM_SBLK] = -2.0f * invw * xyi;
         }
         if (e) {
             cf *ep = e + (((size_t)p * nchan + c) * ns_total + so) * 2;
@@ -250,17 +250,22 @@ __global__ __launch_bounds__(1024) void k_renorm_ref(const float *__restrict__ S
     }
 }
 
-// Device layout S[p][st][c][s]: min/max per (p, st, c), one workgroup each.
+// Device layout d_S (VM_S_IDX): min/max per (p, st, c), one workgroup each.
 __global__ __launch_bounds__(256) void k_minmax_dev(const float *__restrict__ S, int nstep,
+                                                    int npointing, int nstokes, int nchan,
                                                     float *__restrict__ offsets,
                                                     float *__restrict__ scales)
 {
     __shared__ float smin[256], smax[256];
     const size_t row = blockIdx.x; // (p*nstokes + st)*nchan + c
-    const float *r = S + row * nstep;
-    float mn = r[0], mx = r[0];
+    const int c = (int)(row % nchan), st = (int)(row / nchan % nstokes);
+    const int p = (int)(row / nchan / nstokes);
+    const int nblk = VM_S_NBLK(nstep);
+    const float *r0 = S + VM_S_IDX(p, st, c, 0, npointing, nstokes, nblk);
+    const size_t bstride = (size_t)npointing * nstokes * VM_SBLK; // next sample block
+    float mn = r0[0], mx = r0[0];
     for (int s = threadIdx.x; s < nstep; s += 256) {
-        float val = r[s];
+        float val = r0[(size_t)(s / VM_SBLK) * bstride + s % VM_SBLK];
         mn = fminf(mn, val);
         mx = fmaxf(mx, val);
     }
@@ -280,10 +285,10 @@ __global__ __launch_bounds__(256) void k_minmax_dev(const float *__restrict__ S,
     }
 }
 
-// Quantise + transpose: S[p][st][c][s] -> Ss[p][s][st][c] (uint8).
+// Quantise + transpose: d_S (VM_S_IDX) -> Ss[p][s][st][c] (uint8).
 // block 256, tile 64 channels x 64 samples; grid (ceil(ns/64), ceil(nc/64), p*nst)
 __global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, int nstep,
-                                                   int nstokes, int nchan,
+                                                   int npointing, int nstokes, int nchan,
                                                    const float *__restrict__ offsets,
                                                    const float *__restrict__ scales,
                                                    uint8_t *__restrict__ Ss)
@@ -297,7 +302,9 @@ __global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, 
         uint8_t q = 0;
         if (c < nchan && s < nstep) {
             size_t row = (size_t)pst * nchan + c;
-            float val = (S[row * nstep + s] - offsets[row]) / scales[row];
+            float val = (S[VM_S_IDX(p, st, c, s, npointing, nstokes, VM_S_NBLK(nstep))] -
+                         offsets[row]) /
+                        scales[row];
             float rr = val + 0.5f;
             rr = (rr != rr) ? 0.f : fminf(fmaxf(rr, 0.f), 255.f);
             q = (uint8_t)rr;
@@ -313,9 +320,10 @@ __global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, 
 }
 
 // ------------------------------------------------------------ pulls -------
-// S[p][st][c][s] -> [p][s][st][c] float
+// d_S (VM_S_IDX) -> [p][s][st][c] float
 __global__ __launch_bounds__(256) void k_S_to_host(const float *__restrict__ S, int nstep,
-                                                   int nstokes, int nchan, float *__restrict__ out)
+                                                   int npointing, int nstokes, int nchan,
+                                                   float *__restrict__ out)
 {
     __shared__ float T[64][65];
     const int pst = blockIdx.z, p = pst / nstokes, st = pst % nstokes;
@@ -323,7 +331,9 @@ __global__ __launch_bounds__(256) void k_S_to_host(const float *__restrict__ S, 
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     for (int r = ty; r < 64; r += 4) {
         int c = c0 + r, s = s0 + tx;
-        T[r][tx] = (c < nchan && s < nstep) ? S[((size_t)pst * nchan + c) * nstep + s] : 0.f;
+        T[r][tx] = (c < nchan && s < nstep)
+                       ? S[VM_S_IDX(p, st, c, s, npointing, nstokes, VM_S_NBLK(nstep))]
+                       : 0.f;
     }
     __syncthreads();
     for (int r = ty; r < 64; r += 4) {
@@ -517,7 +527,7 @@ void vmPullS(vcsbeam_context *vm)
     void *st = stage_buffer(vm, vm->S_size_bytes);
     dim3 grid((ns + 63) / 64, (nchan + 63) / 64, vm->npointing * vm->out_nstokes);
     hipLaunchKernelGGL(k_S_to_host, grid, dim3(256), 0, vm_stream(vm), (const float *)vm->d_S, ns,
-                       vm->out_nstokes, nchan, (float *)st);
+                       (int)vm->npointing, vm->out_nstokes, nchan, (float *)st);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(vm->S, st, vm->S_size_bytes, hipMemcpyDeviceToHost, vm_stream(vm)));
     HIPCHECK(hipStreamSynchronize(vm_stream(vm)));
@@ -531,10 +541,12 @@ void vmSendSToFits(vcsbeam_context *vm, mpi_psrfits *mpfs)
     hipStream_t s = vm_stream(vm);
     vm_timer_start(vm_dev(vm), VM_T_RENORM);
     hipLaunchKernelGGL(k_minmax_dev, dim3(vm->npointing * nst * nchan), dim3(256), 0, s,
-                       (const float *)vm->d_S, ns, vm->d_offsets, vm->d_scales);
+                       (const float *)vm->d_S, ns, (int)vm->npointing, nst, nchan, vm->d_offsets,
+                       vm->d_scales);
     HIPCHECK(hipGetLastError());
     dim3 grid((ns + 63) / 64, (nchan + 63) / 64, vm->npointing * nst);
-    hipLaunchKernelGGL(k_quant_dev, grid, dim3(256), 0, s, (const float *)vm->d_S, ns, nst, nchan,
+    hipLaunchKernelGGL(k_quant_dev, grid, dim3(256), 0, s, (const float *)vm->d_S, ns,
+                       (int)vm->npointing, nst, nchan,
                        vm->d_offsets, vm->d_scales, vm->d_Cscaled);
     HIPCHECK(hipGetLastError());
     vm_timer_stop(vm_dev(vm), VM_T_RENORM);

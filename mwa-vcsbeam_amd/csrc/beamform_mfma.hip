@@ -559,8 +559,9 @@ __device__ __forceinline__ void stokes4(float xr, float xi, float yr, float yi, 
 
 // Operands of one GEMM problem: a whole chunk in the single-channel call, or
 // one coarse channel of a batch.  Af, Bf, sexp and N are indexed by the fine
-// channel c as laid out by k_phase_weights / k_bf_prep; S is [p][st][c][s_total]
-// (device layout) written from sample soffset on.
+// channel c as laid out by k_phase_weights / k_bf_prep; S is the blocked
+// device layout (VM_S_IDX) of a second of ns_total samples, written from
+// sample soffset on.
 // Per-pointing primary-beam form: N holds T_k as float4 [c][s][4], bx the
 // pointing's B^-1 and h as float [c][p][12] and esp its weight exponents.
 struct bf_geom {
@@ -780,8 +781,7 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
     const int lk = lane >> 5;
     const int nstokes = G.nstokes, npointing = G.npointing;
     const float invw = G.invw;
-    const size_t pstride = (size_t)nstokes * nchan * G.ns_total;
-    const size_t ststride = (size_t)nchan * G.ns_total;
+    const int nblk = VM_S_NBLK(G.ns_total);
     float us2[NSB];
 #pragma unroll
     for (int j = 0; j < NSB; j++) us2[j] = ldexpf(1.0f, -2 * (esbv[j] + PHASE_EXP));
@@ -790,9 +790,13 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
         // LDS tile, and 16 lanes store one row: every store instruction writes
         // 4 rows x 256 contiguous bytes (HBM takes such runs markedly faster
         // than 128-byte ones).
+        // With the blocked layout of S a wavefront's 64 pointings x nstokes x
+        // 64 samples are one contiguous run when the chunk offset is a
+        // multiple of 64 (a lane's 4 samples never straddle a block).
         const int q16 = lane & 15;
         const int s4 = SB * sb0 + 4 * q16;
-        gmem float *Sc = (gmem float *)G.S + (size_t)c * G.ns_total + G.soffset + s4;
+        gmem float *Sc =
+            (gmem float *)G.S + VM_S_IDX(0, 0, c, G.soffset + s4, npointing, nstokes, nblk);
 #pragma unroll
         for (int m = 0; m < MB; m++)
 #pragma unroll
@@ -824,7 +828,7 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                     const int p = 16 * (rb0 + m) + 8 * h + bb;
                     const float4 val = *(const float4 *)(wl + r * 68 + 4 * q16);
                     if (st >= nstokes || p >= npointing || s4 >= ns) continue;
-                    *(gmem f4v *)(Sc + (size_t)p * pstride + (size_t)st * ststride) =
+                    *(gmem f4v *)(Sc + ((size_t)p * nstokes + st) * VM_SBLK) =
                         f4v{val.x, val.y, val.z, val.w};
                 }
             }
@@ -835,7 +839,6 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
     for (int j = 0; j < NSB; j++) {
         const int sb = sb0 + j;
         const int s4 = SB * sb + 4 * qq;
-        gmem float *Sc = (gmem float *)G.S + (size_t)c * G.ns_total + G.soffset + s4;
 #pragma unroll
         for (int m = 0; m < MB; m++)
 #pragma unroll
@@ -865,14 +868,19 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                     const int p = 16 * (rb0 + m) + 8 * h + bb;
                     const float4 val = *(const float4 *)(wl + r * 32 + 4 * qq);
                     if (st >= nstokes || p >= npointing || s4 >= ns) continue;
-                    gmem float *dst = Sc + (size_t)p * pstride + (size_t)st * ststride;
+                    gmem float *S = (gmem float *)G.S;
+                    const int so = G.soffset + s4;
                     if constexpr (VEC) {
-                        *(gmem f4v *)dst = f4v{val.x, val.y, val.z, val.w};
+                        // so is a multiple of 4: the 4 samples share a block
+                        *(gmem f4v *)(S + VM_S_IDX(p, st, c, so, npointing, nstokes, nblk)) =
+                            f4v{val.x, val.y, val.z, val.w};
                     } else {
-                        dst[0] = val.x;
-                        if (s4 + 1 < ns) dst[1] = val.y;
-                        if (s4 + 2 < ns) dst[2] = val.z;
-                        if (s4 + 3 < ns) dst[3] = val.w;
+                        // any offset: each sample finds its own block
+                        const float v4[4] = {val.x, val.y, val.z, val.w};
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            if (s4 + k < ns)
+                                S[VM_S_IDX(p, st, c, so + k, npointing, nstokes, nblk)] = v4[k];
                     }
                 }
             }

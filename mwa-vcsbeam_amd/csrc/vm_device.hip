@@ -184,8 +184,9 @@ void vmFreeEDevice(vcsbeam_context *vm) { dfree(vm->d_e); vm->d_e = NULL; }
 
 void vmMallocSDevice(vcsbeam_context *vm)
 {
+    /* blocked layout, the last sample block padded to VM_SBLK */
     vm->d_S_size_bytes = (uintptr_t)vm->npointing * vm->nfine_chan * vm->out_nstokes *
-                         vm->fine_sample_rate * sizeof(float);
+                         VM_S_NBLK(vm->fine_sample_rate) * VM_SBLK * sizeof(float);
     vm->d_S = dmalloc(vm->d_S_size_bytes);
 }
 void vmFreeSDevice(vcsbeam_context *vm) { dfree(vm->d_S); vm->d_S = NULL; }

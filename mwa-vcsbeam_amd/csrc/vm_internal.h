@@ -16,6 +16,21 @@ extern "C" {
 void vm_fatal(const char *file, int line, const char *fmt, ...);
 #define VM_FATAL(...) vm_fatal(__FILE__, __LINE__, __VA_ARGS__)
 
+/* Device Stokes layout d_S: [c][s / VM_SBLK][p][st][s % VM_SBLK] float, the
+   second's samples cut into blocks of VM_SBLK, the last block padded.  A
+   tile of pointings x VM_SBLK samples is then one contiguous run of memory
+   (the beamformers write whole runs; HBM takes them at near its sequential
+   rate).  VM_S_IDX gives the element of (p, st, c, s) for np pointings, nst
+   Stokes and nblk = VM_S_NBLK(samples per second) blocks. */
+#define VM_SBLK 64
+#define VM_S_NBLK(ns) (((ns) + VM_SBLK - 1) / VM_SBLK)
+#define VM_S_IDX(p, st, c, s, np, nst, nblk)                                              \
+    (((((size_t)(c) * (size_t)(nblk) + (size_t)((s) / VM_SBLK)) * (size_t)(np) + (size_t)(p)) * \
+          (size_t)(nst) +                                                                  \
+      (size_t)(st)) *                                                                      \
+         VM_SBLK +                                                                         \
+     (size_t)((s) % VM_SBLK))
+
 /* Device-side state hung off vcsbeam_context.dev (created lazily). */
 struct vm_device_state {
     void *stream;             /* hipStream_t on which all context work runs */

@@ -89,8 +89,7 @@ def test_batch_graph_replays_match_eager(pkg, orc, gpu_lib, batch_case):
     for r in range(3):
         for ses in bs.sessions:
             v = ses.v
-            lib.vmMemsetDevice(v.d_S, 0, len(ses.bg) * v.out_nstokes * v.nfine_chan *
-                               v.fine_sample_rate * 4)
+            lib.vmMemsetDevice(v.d_S, 0, v.d_S_size_bytes)
         print(f"graph replay {r}", flush=True)
         bs.run(use_graph=True)
         for g, ses in enumerate(bs.sessions):
