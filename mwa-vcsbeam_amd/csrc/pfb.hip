@@ -63,8 +63,10 @@ template <bool FPGA>
 __device__ __forceinline__ float demote(int X, float post)
 {
     if (FPGA) {
-        // src/pfb.cpp:225-240: (X > 0 ? X + 0x2000 : X) >> 14
-        int y = (X > 0 ? X + 0x2000 : X) >> 14;
+        // src/pfb.cpp:225-240: (X > 0 ? X + 0x2000 : X) >> 14; the sum
+        // wraps like the reference's int
+        const uint32_t bias = X > 0 ? 0x2000u : 0u;
+        const int y = (int)((uint32_t)X + bias) >> 14;
         return (float)y * post;
     } else {
         // src/pfb.cpp:266: (float)X * 2^-14
@@ -167,10 +169,11 @@ __device__ __forceinline__ void pfb_put_window(const uint4 (&wr)[PFB_WPL], uint3
 
 // WOLA + rounding + FFT + reorder of one tile whose window is in LDS (win
 // aliases F); ends with the tile's stores.  Contains workgroup barriers.
-template <bool FPGA, bool SPLIT>
+// IMF: output as (im, re) instead of (re, im) (src/pfb.cpp:322-331).
+template <bool FPGA, bool SPLIT, bool IMF>
 __device__ __forceinline__ void pfb_tile(cf *F, const uint32_t *win, const cf *tw,
                                          const pfb_taps &T, int i, int I, int m0, int nspc, int oi,
-                                         int imag_first, cf *__restrict__ out, int t)
+                                         cf *__restrict__ out, int t)
 {
     constexpr int P = PFB_P;
     constexpr int NW = 8 + P - 1; // window blocks per lane
@@ -238,20 +241,32 @@ __device__ __forceinline__ void pfb_tile(cf *F, const uint32_t *win, const cf *t
 
     // Every lane issues its 16 stores; in a partial tile the lanes past the
     // last spectrum repeat the store of lane f = nvalid - 1 (same address,
-    // same value), so the instruction stream never branches around stores.
+    // same value, taken by a lane shuffle), so the instruction stream never
+    // branches around stores.
+    // Output channel kp = (k + 64) & 127 (fftshift, src/pfb.cpp:315) of
+    // k = l + 8 hh + 16 k2: with l + 8 hh < 16 it is l + 64 + 8 hh + 16 k2,
+    // less 128 from k2 = 4 on, so each store is a compile-time multiple of
+    // the channel stride I nspc from one per-tile base.
     const int fs = min(f, nvalid - 1);
     const int src = (t & 32) + fs; // lane of (fs, l) in this wavefront
+    const size_t cstride = (size_t)I * nspc;
+    cf *ob = out + ((size_t)(l + KCH / 2) * I + oi) * nspc + m0 + fs;
+    if (__builtin_expect(nvalid < TILE_M, 0)) { // the last tile of an input only
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+            for (int k2 = 0; k2 < 8; k2++) {
+                y[hh][k2].x = __shfl(y[hh][k2].x, src, 64);
+                y[hh][k2].y = __shfl(y[hh][k2].y, src, 64);
+            }
+    }
 #pragma unroll
     for (int hh = 0; hh < 2; hh++)
 #pragma unroll
         for (int k2 = 0; k2 < 8; k2++) {
-            int k = l + 8 * hh + 16 * k2;
-            int kp = (k + KCH / 2) & (KCH - 1); // fftshift, src/pfb.cpp:315
-            cf v = y[hh][k2];
-            v.x = __shfl(v.x, src, 64); // identity when the tile is full
-            v.y = __shfl(v.y, src, 64);
-            cf o = imag_first ? v : cf{v.y, v.x};
-            out[((size_t)kp * I + oi) * nspc + m0 + fs] = o;
+            const long coef = 8 * hh + 16 * k2 - (k2 >= 4 ? KCH : 0);
+            const cf v = y[hh][k2];
+            ob[coef * (long)cstride] = IMF ? v : cf{v.y, v.x};
         }
 }
 
@@ -264,11 +279,10 @@ __device__ __forceinline__ void pfb_tile(cf *F, const uint32_t *win, const cf *t
 // instructions the wait for the window is vmcnt(#stores), not vmcnt(0).
 // Taps and twiddles are loaded once per workgroup.  htr_bytes: size of the
 // chunk's input buffer (loads past it return zeros).
-template <bool FPGA, bool SPLIT>
+template <bool FPGA, bool SPLIT, bool IMF>
 __global__ __launch_bounds__(256) void k_pfb_fused(
     const uint32_t *__restrict__ htr, uint32_t htr_bytes, const int *__restrict__ h,
-    const int *__restrict__ out_idx, int I, int nspc, int imag_first, cf *__restrict__ out,
-    int per)
+    const int *__restrict__ out_idx, int I, int nspc, cf *__restrict__ out, int per)
 {
     __shared__ cf F[TILE_M * ROW]; // FFT rows (33 KB); the window aliases it
     __shared__ cf tw[KCH];         // twiddles in LDS: no vector-memory loads in the loop
@@ -293,8 +307,8 @@ __global__ __launch_bounds__(256) void k_pfb_fused(
     const int t1 = min(T0 + 1, Tend - 1);
     pfb_fetch_window(wr, rs, t1 / nmb, I, (t1 % nmb) * TILE_M, t);
     __syncthreads();
-    pfb_tile<FPGA, SPLIT>(F, win, tw, T, T0 / nmb, I, (T0 % nmb) * TILE_M, nspc,
-                          out_idx[T0 / nmb], imag_first, out, t);
+    pfb_tile<FPGA, SPLIT, IMF>(F, win, tw, T, T0 / nmb, I, (T0 % nmb) * TILE_M, nspc,
+                               out_idx[T0 / nmb], out, t);
     for (int tt = T0 + 1; tt < Tend; tt++) {
         const int i = tt / nmb, m0 = (tt % nmb) * TILE_M;
         __syncthreads(); // the previous tile's FFT is done with F
@@ -302,7 +316,7 @@ __global__ __launch_bounds__(256) void k_pfb_fused(
         const int tn = min(tt + 1, Tend - 1); // the last prefetch repeats this tile's
         pfb_fetch_window(wr, rs, tn / nmb, I, (tn % nmb) * TILE_M, t);
         __syncthreads();
-        pfb_tile<FPGA, SPLIT>(F, win, tw, T, i, I, m0, nspc, out_idx[i], imag_first, out, t);
+        pfb_tile<FPGA, SPLIT, IMF>(F, win, tw, T, i, I, m0, nspc, out_idx[i], out, t);
     }
 }
 
@@ -409,6 +423,30 @@ __global__ __launch_bounds__(256) void k_flt_to_legacy(const cf *__restrict__ in
         else
             ((double2 *)out)[o] = make_double2((double)v.x, (double)v.y);
     }
+}
+
+// Persistent launch, contiguous runs: exactly as many workgroups as are
+// resident at once (registers and LDS decide how many per CU, queried once
+// per kernel instantiation), so no workgroup starts after the first ones
+// finish.
+template <bool FPGA, bool SPLIT, bool IMF>
+static void pfb_fused_run(const uint32_t *x, size_t htr_bytes, const int *d_h, const int *d_out_idx,
+                          int I, int nspc, cf *o, hipStream_t stream)
+{
+    static int ncu = 0, bpc = 0;
+    if (!ncu) {
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_pfb_fused<FPGA, SPLIT, IMF>,
+                                                              256, 0));
+        if (bpc < 1) bpc = 1;
+    }
+    const int ntiles = ((nspc + TILE_M - 1) / TILE_M) * I;
+    const int nwg = min(ntiles, bpc * ncu);
+    const int per = (ntiles + nwg - 1) / nwg;
+    hipLaunchKernelGGL((k_pfb_fused<FPGA, SPLIT, IMF>), dim3((ntiles + per - 1) / per), dim3(256),
+                       0, stream, x, (uint32_t)htr_bytes, d_h, d_out_idx, I, nspc, o, per);
 }
 
 // =================================================================== host ==
@@ -575,41 +613,17 @@ void vm_pfb_fused_launch(const void *d_htr, size_t htr_bytes, const int *d_h,
         VM_FATAL("fused PFB: chunk input of %zu bytes exceeds 2 GiB; use more chunks", htr_bytes);
     const uint32_t *x = (const uint32_t *)d_htr;
     cf *o = (cf *)d_out;
-    // persistent launch, contiguous runs: exactly as many workgroups as are
-    // resident at once (registers and LDS decide how many per CU), so no
-    // workgroup starts after the first ones finish
-    static int ncu = 0, bpc[4] = {0, 0, 0, 0};
-    if (!ncu) {
-        int dev = 0;
-        HIPCHECK(hipGetDevice(&dev));
-        HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int sel = (fpga ? 4 : 0) + (split ? 2 : 0) + (imag_first ? 1 : 0);
+    switch (sel) {
+    case 0: pfb_fused_run<false, false, false>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
+    case 1: pfb_fused_run<false, false, true>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
+    case 2: pfb_fused_run<false, true, false>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
+    case 3: pfb_fused_run<false, true, true>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
+    case 4: pfb_fused_run<true, false, false>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
+    case 5: pfb_fused_run<true, false, true>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
+    case 6: pfb_fused_run<true, true, false>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
+    default: pfb_fused_run<true, true, true>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
     }
-    const int v = (fpga ? 2 : 0) + (split ? 1 : 0);
-    if (!bpc[v]) {
-        hipError_t e;
-        if (fpga && split)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc[v], k_pfb_fused<true, true>, 256, 0);
-        else if (fpga)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc[v], k_pfb_fused<true, false>, 256, 0);
-        else if (split)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc[v], k_pfb_fused<false, true>, 256, 0);
-        else
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc[v], k_pfb_fused<false, false>, 256, 0);
-        HIPCHECK(e);
-        if (bpc[v] < 1) bpc[v] = 1;
-    }
-    const int ntiles = ((nspc + TILE_M - 1) / TILE_M) * I;
-    const int nwg = min(ntiles, bpc[v] * ncu);
-    const int per = (ntiles + nwg - 1) / nwg;
-    const dim3 grid((ntiles + per - 1) / per), block(256);
-#define VM_PFB(F_, S_)                                                                       \
-    hipLaunchKernelGGL((k_pfb_fused<F_, S_>), grid, block, 0, stream, x, (uint32_t)htr_bytes, \
-                       d_h, d_out_idx, I, nspc, imag_first, o, per)
-    if (fpga && split) VM_PFB(true, true);
-    else if (fpga) VM_PFB(true, false);
-    else if (split) VM_PFB(false, true);
-    else VM_PFB(false, false);
-#undef VM_PFB
     HIPCHECK(hipGetLastError());
 }
 
