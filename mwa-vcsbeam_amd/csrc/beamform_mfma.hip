@@ -828,8 +828,9 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                     const int p = 16 * (rb0 + m) + 8 * h + bb;
                     const float4 val = *(const float4 *)(wl + r * 68 + 4 * q16);
                     if (st >= nstokes || p >= npointing || s4 >= ns) continue;
-                    *(gmem f4v *)(Sc + ((size_t)p * nstokes + st) * VM_SBLK) =
-                        f4v{val.x, val.y, val.z, val.w};
+                    // streamed output, never read back by this kernel
+                    __builtin_nontemporal_store(f4v{val.x, val.y, val.z, val.w},
+                                                (gmem f4v *)(Sc + ((size_t)p * nstokes + st) * VM_SBLK));
                 }
             }
         return;
@@ -872,8 +873,9 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                     const int so = G.soffset + s4;
                     if constexpr (VEC) {
                         // so is a multiple of 4: the 4 samples share a block
-                        *(gmem f4v *)(S + VM_S_IDX(p, st, c, so, npointing, nstokes, nblk)) =
-                            f4v{val.x, val.y, val.z, val.w};
+                        __builtin_nontemporal_store(
+                            f4v{val.x, val.y, val.z, val.w},
+                            (gmem f4v *)(S + VM_S_IDX(p, st, c, so, npointing, nstokes, nblk)));
                     } else {
                         // any offset: each sample finds its own block
                         const float v4[4] = {val.x, val.y, val.z, val.w};
@@ -961,6 +963,7 @@ struct bf_variant { int mb, nb, wr, wc, nst, occ; };
 static const bf_variant kVariants[] = {
     {4, 4, 2, 2, 4, 1}, {4, 4, 1, 4, 4, 1}, {4, 4, 1, 4, 6, 1}, {2, 4, 2, 2, 4, 2},
     {4, 2, 2, 2, 4, 2}, {2, 4, 4, 1, 4, 2}, {4, 4, 4, 1, 4, 1}, {4, 4, 2, 2, 6, 1},
+    {2, 4, 1, 4, 4, 2}, {2, 4, 4, 2, 4, 2},
 };
 #define N_VARIANTS ((int)(sizeof(kVariants) / sizeof(kVariants[0])))
 #define VM_N_ORDERS 5
@@ -989,7 +992,9 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
     case 4: VM_LAUNCH(4, 2, 2, 2, 4, 2); break;
     case 5: VM_LAUNCH(2, 4, 4, 1, 4, 2); break;
     case 6: VM_LAUNCH(4, 4, 4, 1, 4, 1); break;
-    default: VM_LAUNCH(4, 4, 2, 2, 6, 1); break;
+    case 7: VM_LAUNCH(4, 4, 2, 2, 6, 1); break;
+    case 8: VM_LAUNCH(2, 4, 1, 4, 4, 2); break;
+    default: VM_LAUNCH(2, 4, 4, 2, 4, 2); break;
     }
 #undef VM_LAUNCH
 }
@@ -1010,7 +1015,9 @@ static void launch_batch_variant(int vi, unsigned nwg, hipStream_t st, const bf_
     case 4: VM_LAUNCH(4, 2, 2, 2, 4, 2); break;
     case 5: VM_LAUNCH(2, 4, 4, 1, 4, 2); break;
     case 6: VM_LAUNCH(4, 4, 4, 1, 4, 1); break;
-    default: VM_LAUNCH(4, 4, 2, 2, 6, 1); break;
+    case 7: VM_LAUNCH(4, 4, 2, 2, 6, 1); break;
+    case 8: VM_LAUNCH(2, 4, 1, 4, 4, 2); break;
+    default: VM_LAUNCH(2, 4, 4, 2, 4, 2); break;
     }
 #undef VM_LAUNCH
 }
