@@ -630,8 +630,16 @@ __device__ __forceinline__ void stokes_pb(float xr, float xi, float yr, float yi
 // Each wavefront's staging tile holds 32 rows of 64 samples (68-float rows,
 // padded against bank conflicts).
 #define BF_WAVE_LDS (32 * 68)
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool PB>
-constexpr int bf_lds_floats() { return WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 16 : 0); }
+// Fragments (1 KB each) of one k-step of a workgroup in the shared-operand
+// form: A (hi, lo) of its WR MB row blocks and B of its WC NB column blocks.
+template <int MB, int NB, int WR, int WC>
+constexpr int bf_wg_frags() { return 2 * WR * MB + WC * NB; }
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool PB, bool LDSOP = false>
+constexpr int bf_lds_floats()
+{
+    return WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 16 : 0) +
+           (LDSOP ? 2 * bf_wg_frags<MB, NB, WR, WC>() * 256 : 0);
+}
 
 // Stokes of one beam-sample from the GEMM outputs (x, y), in either form;
 // the per-pointing form reads the pointing's B^-1, h and weight exponent from
@@ -654,7 +662,121 @@ __device__ __forceinline__ void bf_stokes(float xr, float xi, float yr, float yi
     }
 }
 
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool VEC, bool PB>
+// Workgroup barrier that orders LDS only: the fences wait for this
+// wavefront's LDS operations (lgkmcnt), not for its global loads, so
+// prefetches stay in flight across it.
+__device__ __forceinline__ void lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// k-loop with workgroup-shared operands (the LDSOP form of bf_tile).  The
+// fragments of one k-step of the whole workgroup (bf_wg_frags: A hi/lo of
+// its WR MB row blocks, B of its WC NB column blocks) are fetched once per
+// workgroup -- wavefront w takes fragments w, w + NW, ... -- two k-steps
+// ahead into registers (R0, R1), written to one of two LDS stages, and read
+// from there by each wavefront one k-step ahead of its MFMAs (fa0/fb0,
+// fa1/fb1).  Each operand byte then crosses a CU's vector-memory path once
+// per workgroup instead of once per wavefront.  One barrier per k-step;
+// stage s is rewritten only after every wavefront's reads of it completed
+// before the previous barrier.  Prefetches past the last k-step re-read it.
+template <int MB, int NB, int WR, int WC, int PER>
+__device__ __forceinline__ void wg_gload(uint4 (&R)[PER], rsrc_t rA, rsrc_t rB, int vl,
+                                         const int (&goff)[PER], int k)
+{
+    constexpr int NW = WR * WC, FA = 2 * WR * MB;
+#pragma unroll
+    for (int i = 0; i < PER; i++) R[i] = bload(NW * i < FA ? rA : rB, vl, goff[i] + 2048 * k);
+}
+
+template <int PER, int F>
+__device__ __forceinline__ void wg_lwrite(uint4 *ops, const uint4 (&R)[PER], int stage, int w,
+                                          int lane)
+{
+    constexpr int NW = F / PER;
+#pragma unroll
+    for (int i = 0; i < PER; i++) ops[(stage * F + w + NW * i) * 64 + lane] = R[i];
+}
+
+template <int MB, int NB, int F, int FA>
+__device__ __forceinline__ void wg_lread(afrag<MB> &fa, bfrag<NB> &fb, const uint4 *ops, int stage,
+                                         int wr, int wc, int lane)
+{
+    const uint4 *o = ops + stage * F * 64 + lane;
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+        fa.a[m][0] = o[(2 * (wr * MB + m)) * 64];
+        fa.a[m][1] = o[(2 * (wr * MB + m) + 1) * 64];
+    }
+#pragma unroll
+    for (int n = 0; n < NB; n++) fb.b[n] = o[(FA + wc * NB + n) * 64];
+}
+
+template <bool BF16, int MB, int NB, int WR, int WC, bool PB>
+__device__ __forceinline__ void bf_kloop_lds(f16v (&acc)[MB][NB], rsrc_t rA, rsrc_t rB, int vl,
+                                             int astr, int bstr, int nkstep, int nsb, int ty,
+                                             int tx, int w, int lane, float *stg)
+{
+    constexpr int NW = WR * WC, FA = 2 * WR * MB, F = bf_wg_frags<MB, NB, WR, WC>();
+    static_assert(F % NW == 0 && FA % NW == 0, "fragments divide among the wavefronts");
+    // two fragment sets and two prefetch sets beside the accumulators: small
+    // wavefront tiles only (a 128 x 128 tile's 256 accumulators leave no room)
+    static_assert(MB * NB <= 8, "shared-operand k-loop is for 64 x 64 wavefront tiles");
+    constexpr int PER = F / NW, NSB = NB / 2;
+    uint4 *ops = (uint4 *)(stg + WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 16 : 0));
+    const int wr = w / WC, wc = w % WC;
+    const int rbw0 = ty * WR * MB, sbw0 = tx * WC * NSB;
+    int goff[PER]; // this wavefront's loader fragments: byte offsets at k-step 0
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int f = w + NW * i;
+        if (NW * i < FA) {
+            goff[i] = (rbw0 + (f >> 1)) * astr + 1024 * (f & 1);
+        } else {
+            const int g = f - FA;
+            // blocks past the last sample block re-read it (in bounds); their
+            // columns are never stored
+            goff[i] = min(sbw0 + (g >> 1), nsb - 1) * bstr + 1024 * (g & 1);
+        }
+    }
+    const int kl = nkstep - 1;
+    uint4 R0[PER], R1[PER];
+    afrag<MB> fa0, fa1;
+    bfrag<NB> fb0, fb1;
+    wg_gload<MB, NB, WR, WC, PER>(R0, rA, rB, vl, goff, 0);
+    wg_gload<MB, NB, WR, WC, PER>(R1, rA, rB, vl, goff, min(1, kl));
+    wg_lwrite<PER, F>(ops, R0, 0, w, lane);
+    wg_gload<MB, NB, WR, WC, PER>(R0, rA, rB, vl, goff, min(2, kl));
+    wg_lwrite<PER, F>(ops, R1, 1, w, lane);
+    wg_gload<MB, NB, WR, WC, PER>(R1, rA, rB, vl, goff, min(3, kl));
+    lds_sync();
+    wg_lread<MB, NB, F, FA>(fa0, fb0, ops, 0, wr, wc, lane);
+    lds_sync(); // every wavefront holds k-step 0 before stage 0 is rewritten
+    for (int ks = 0; ks < nkstep; ks += 2) {
+        // fa0/fb0: k-step ks; stage 1: ks + 1; R0: ks + 2; R1: ks + 3
+        wg_lread<MB, NB, F, FA>(fa1, fb1, ops, 1, wr, wc, lane);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_ab<BF16, MB, NB>(acc, fa0, fb0);
+        __builtin_amdgcn_sched_barrier(0);
+        wg_lwrite<PER, F>(ops, R0, 0, w, lane);
+        wg_gload<MB, NB, WR, WC, PER>(R0, rA, rB, vl, goff, min(ks + 4, kl));
+        lds_sync();
+        if (ks + 1 < nkstep) {
+            // fa1/fb1: k-step ks + 1; stage 0: ks + 2; R1: ks + 3; R0: ks + 4
+            wg_lread<MB, NB, F, FA>(fa0, fb0, ops, 0, wr, wc, lane);
+            __builtin_amdgcn_sched_barrier(0);
+            mma_ab<BF16, MB, NB>(acc, fa1, fb1);
+            __builtin_amdgcn_sched_barrier(0);
+            wg_lwrite<PER, F>(ops, R1, 1, w, lane);
+            wg_gload<MB, NB, WR, WC, PER>(R1, rA, rB, vl, goff, min(ks + 5, kl));
+            lds_sync();
+        }
+    }
+}
+
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool VEC, bool PB, bool LDSOP>
 __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int nchan, int c,
                                         int tx, int ty, float *stg)
 {
@@ -720,6 +842,10 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
     // the B loads re-read it and the A loads fall outside the weights buffer,
     // which returns zeros, so those MFMAs add nothing.
 #define VM_SB() __builtin_amdgcn_sched_barrier(0)
+    if constexpr (LDSOP) {
+        bf_kloop_lds<BF16, MB, NB, WR, WC, PB>(acc, rA, rB, vl, astr, bstr, nkstep, nsb, ty, tx, w,
+                                               lane, stg);
+    } else {
     constexpr int U = (NST % 2 == 0) ? NST : 2 * NST;
     afrag<MB> fa[2];
     bfrag<NB> fb[NST];
@@ -737,6 +863,7 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
             interleave_loads<2 * MB + NB, 2 * MB * NB>();
             VM_SB();
         }
+    }
     }
 #undef VM_SB
 
@@ -895,12 +1022,14 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
 // the same tile of 8 different channels, so each XCD walks its own channels;
 // within a channel tiles go in bands of gr row tiles, column by column (gr =
 // all row tiles for order 1, 1 for order 2, 2 for order 3, 4 for order 4).
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC, bool VEC, bool PB>
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC, bool VEC, bool PB,
+          bool LDSOP>
 __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, int ns, int nsb,
                                                                      int nchan, int ntx, int nty,
                                                                      int order)
 {
-    __shared__ __attribute__((aligned(16))) float stg[bf_lds_floats<BF16, MB, NB, WR, WC, NST, PB>()];
+    __shared__ __attribute__((aligned(16))) float
+        stg[bf_lds_floats<BF16, MB, NB, WR, WC, NST, PB, LDSOP>()];
     const int ntile = ntx * nty;
     const int L = blockIdx.x;
     int c, tile;
@@ -916,7 +1045,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, 
     if (nty % gr) gr = 1;
     const int band = tile / (gr * ntx), rem = tile % (gr * ntx);
     const int tx = rem / gr, ty = band * gr + rem % gr;
-    bf_tile<BF16, MB, NB, WR, WC, NST, VEC, PB>(G, ns, nsb, nchan, c, tx, ty, stg);
+    bf_tile<BF16, MB, NB, WR, WC, NST, VEC, PB, LDSOP>(G, ns, nsb, nchan, c, tx, ty, stg);
 }
 
 // Batched form: one persistent launch over the GEMM problems of several
@@ -950,7 +1079,8 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
         const int cnt = __builtin_amdgcn_readfirstlane((int)(e.y & 0xffu));
         const bf_geom G = geoms[g];
         for (int ty = ty0; ty < ty0 + cnt; ty++)
-            bf_tile<BF16, MB, NB, WR, WC, NST, true, false>(G, ns, nsb, nchan, c, tx, ty, stg);
+            bf_tile<BF16, MB, NB, WR, WC, NST, true, false, false>(G, ns, nsb, nchan, c, tx, ty,
+                                                                   stg);
     }
 }
 
@@ -959,11 +1089,12 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
 // wavefront grid per workgroup (WR x WC), depth of the B register ring (the
 // A ring is two deep) and the minimum wavefronts per SIMD the registers must
 // allow.
-struct bf_variant { int mb, nb, wr, wc, nst, occ; };
+// lds: the k-loop with workgroup-shared operands (bf_kloop_lds)
+struct bf_variant { int mb, nb, wr, wc, nst, occ, lds; };
 static const bf_variant kVariants[] = {
     {4, 4, 2, 2, 4, 1}, {4, 4, 1, 4, 4, 1}, {4, 4, 1, 4, 6, 1}, {2, 4, 2, 2, 4, 2},
     {4, 2, 2, 2, 4, 2}, {2, 4, 4, 1, 4, 2}, {4, 4, 4, 1, 4, 1}, {4, 4, 2, 2, 6, 1},
-    {2, 4, 1, 4, 4, 2}, {2, 4, 4, 2, 4, 2},
+    {2, 4, 1, 4, 4, 2}, {2, 4, 4, 2, 4, 2}, {2, 4, 2, 2, 4, 2, 1},
 };
 #define N_VARIANTS ((int)(sizeof(kVariants) / sizeof(kVariants[0])))
 #define VM_N_ORDERS 5
@@ -977,24 +1108,26 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
         // built for the first variant's shape only (its grid is the caller's
         // when vi = 0)
         if (vi != 0) VM_FATAL("MFMA beamformer: variant %d needs ns %% 4 == 0", vi);
-        hipLaunchKernelGGL((k_beamform_mfma<BF16, 4, 4, 2, 2, 4, 1, false, PB>), grid, dim3(256),
+        hipLaunchKernelGGL((k_beamform_mfma<BF16, 4, 4, 2, 2, 4, 1, false, PB, false>), grid,
+                           dim3(256),
                            0, st, G, ns, nsb, nchan, ntx, nty, order);
         return;
     }
-#define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, OCC_)                                             \
-    hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, OCC_, true, PB>), grid, \
-                       dim3(64 * WR_ * WC_), 0, st, G, ns, nsb, nchan, ntx, nty, order)
+#define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, OCC_, LDS_)                                       \
+    hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, OCC_, true, PB, LDS_>),  \
+                       grid, dim3(64 * WR_ * WC_), 0, st, G, ns, nsb, nchan, ntx, nty, order)
     switch (vi) {
-    case 0: VM_LAUNCH(4, 4, 2, 2, 4, 1); break;
-    case 1: VM_LAUNCH(4, 4, 1, 4, 4, 1); break;
-    case 2: VM_LAUNCH(4, 4, 1, 4, 6, 1); break;
-    case 3: VM_LAUNCH(2, 4, 2, 2, 4, 2); break;
-    case 4: VM_LAUNCH(4, 2, 2, 2, 4, 2); break;
-    case 5: VM_LAUNCH(2, 4, 4, 1, 4, 2); break;
-    case 6: VM_LAUNCH(4, 4, 4, 1, 4, 1); break;
-    case 7: VM_LAUNCH(4, 4, 2, 2, 6, 1); break;
-    case 8: VM_LAUNCH(2, 4, 1, 4, 4, 2); break;
-    default: VM_LAUNCH(2, 4, 4, 2, 4, 2); break;
+    case 0: VM_LAUNCH(4, 4, 2, 2, 4, 1, false); break;
+    case 1: VM_LAUNCH(4, 4, 1, 4, 4, 1, false); break;
+    case 2: VM_LAUNCH(4, 4, 1, 4, 6, 1, false); break;
+    case 3: VM_LAUNCH(2, 4, 2, 2, 4, 2, false); break;
+    case 4: VM_LAUNCH(4, 2, 2, 2, 4, 2, false); break;
+    case 5: VM_LAUNCH(2, 4, 4, 1, 4, 2, false); break;
+    case 6: VM_LAUNCH(4, 4, 4, 1, 4, 1, false); break;
+    case 7: VM_LAUNCH(4, 4, 2, 2, 6, 1, false); break;
+    case 8: VM_LAUNCH(2, 4, 1, 4, 4, 2, false); break;
+    case 9: VM_LAUNCH(2, 4, 4, 2, 4, 2, false); break;
+    default: VM_LAUNCH(2, 4, 2, 2, 4, 2, true); break;
     }
 #undef VM_LAUNCH
 }
@@ -1093,6 +1226,47 @@ struct mfma_plan {
     int *esp;    // pb: weight exponents per (c, p)
 };
 
+// Whether J^-1 (vm->d_J) differs between pointings, decided on the device
+// (flag: a device word, cleared here) and read back (synchronises st).
+static int j_differs(vcsbeam_context *vm, hipStream_t st, uint32_t *flag)
+{
+    const int np = (int)vm->npointing;
+    const uint32_t per = (uint32_t)vm->obs_metadata->num_ants * vm->nfine_chan * 4;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, flag, (size_t)1, 0u);
+    hipLaunchKernelGGL(k_jcheck, dim3((per + 4095) / 4096, (unsigned)(np - 1)), dim3(256), 0, st,
+                       (const double2 *)vm->d_J, per, flag);
+    HIPCHECK(hipGetLastError());
+    uint32_t h = 0;
+    HIPCHECK(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return h != 0;
+}
+
+// Whether J^-1 may differ between pointings.  When it was computed on the
+// device from D and the primary beam d_B (vmCalcJDevice*), J^-1[p] is a
+// fixed function of D and B[p], so it is the same for every pointing
+// wherever B is: the check reads B (at most a few MB) instead of J^-1
+// (hundreds of MB).  A beam that differs only on flagged tiles (D = 0) is
+// reported as differing, and a per-tile beam that differs is then checked
+// on J^-1 itself.
+static int pointings_differ(vcsbeam_context *vm, hipStream_t st, uint32_t *flag)
+{
+    vm_device_state *d = vm_dev(vm);
+    const int np = (int)vm->npointing;
+    if (!d->J_fromB || !d->d_B) return j_differs(vm, st, flag);
+    const uint32_t per =
+        (uint32_t)(d->B_chan ? vm->nfine_chan : (int)vm->obs_metadata->num_ants) * 4;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, flag, (size_t)1, 0u);
+    hipLaunchKernelGGL(k_jcheck, dim3((per + 4095) / 4096, (unsigned)(np - 1)), dim3(256), 0, st,
+                       (const double2 *)d->d_B, per, flag);
+    HIPCHECK(hipGetLastError());
+    uint32_t h = 0;
+    HIPCHECK(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (h && !d->B_chan) return j_differs(vm, st, flag);
+    return h != 0;
+}
+
 static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
 {
     vm_device_state *d = vm_dev(vm);
@@ -1117,15 +1291,7 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
         int pb = 0;
         if (np > 1) {
             ensure_buf(&d->d_flag, &d->flag_bytes, 256);
-            uint32_t *jflag = (uint32_t *)d->d_flag;
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, jflag, (size_t)1, 0u);
-            const uint32_t per = (uint32_t)nant * nchan * 4;
-            hipLaunchKernelGGL(k_jcheck, dim3((per + 4095) / 4096, (unsigned)(np - 1)), dim3(256),
-                               0, st, (const double2 *)vm->d_J, per, jflag);
-            HIPCHECK(hipGetLastError());
-            uint32_t hflag = 0;
-            HIPCHECK(hipMemcpyAsync(&hflag, jflag, 4, hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipStreamSynchronize(st));
+            const int hflag = pointings_differ(vm, st, (uint32_t *)d->d_flag);
             if (hflag) {
                 if (!d->B_chan || !d->d_B)
                     VM_FATAL("MFMA beamformer: J^-1 differs between pointings with a primary beam "
@@ -1371,7 +1537,9 @@ struct vm_batch {
 
 static bool batch_variant_ok(int vi)
 {
-    return vi >= 0 && vi < N_VARIANTS && ROW_PAD % (RBK * kVariants[vi].mb * kVariants[vi].wr) == 0;
+    // the persistent batch kernel has the register-ring k-loop only
+    return vi >= 0 && vi < N_VARIANTS && !kVariants[vi].lds &&
+           ROW_PAD % (RBK * kVariants[vi].mb * kVariants[vi].wr) == 0;
 }
 
 static void batch_drop_graph(vm_batch *b)
@@ -1517,16 +1685,12 @@ static void batch_refresh(vm_batch *b)
                                (const double2 *)vm->gdelays.d_phi, np, nact, nchan, nrb, nkstep,
                                (const int *)C.d_amap, nant, (const double2 *)NULL,
                                (const double2 *)NULL, (const int *)NULL, C.Af);
-        uint32_t hflag = 0;
-        if (np > 1) {
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, queue_flag, (size_t)1, 0u);
-            const uint32_t per = (uint32_t)nant * nchan * 4;
-            hipLaunchKernelGGL(k_jcheck, dim3((per + 4095) / 4096, (unsigned)(np - 1)), dim3(256),
-                               0, st, (const double2 *)vm->d_J, per, queue_flag);
-            HIPCHECK(hipMemcpyAsync(&hflag, queue_flag, 4, hipMemcpyDeviceToHost, st));
-        }
+        int hflag = 0;
+        if (np > 1) hflag = pointings_differ(vm, st, queue_flag);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(st));
+        if (hflag && vm->dev->B_chan)
+            hflag = j_differs(vm, st, queue_flag); // a per-pointing beam, but is J^-1 uniform?
         if (hflag)
             VM_FATAL("batch channel %d: J^-1 differs between pointings; the MFMA form needs a "
                      "beam-independent J^-1", g);

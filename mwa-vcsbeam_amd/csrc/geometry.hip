@@ -52,14 +52,11 @@ __device__ __forceinline__ double2 cm(double2 a, double2 b)
 }
 __device__ __forceinline__ double2 ca(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 
-// D[a][c][4], B[p][a][4] -> J[p][a][c][4]; with bchan, B is [p][c][4] (a
-// per-fine-channel beam shared by all tiles)
-__global__ void k_calc_J(const double2 *__restrict__ D, const double2 *__restrict__ B,
-                         int npointing, int nant, int nchan, int bchan, double2 *__restrict__ J)
+// J^-1 of element j = (p, a, c) into out[0..3]
+__device__ __forceinline__ void calc_J_one(const double2 *__restrict__ D,
+                                           const double2 *__restrict__ B, int nant, int nchan,
+                                           int bchan, size_t j, double2 *out)
 {
-    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t total = (size_t)npointing * nant * nchan;
-    if (j >= total) return;
     int c = (int)(j % nchan);
     int a = (int)((j / nchan) % nant);
     int p = (int)(j / ((size_t)nchan * nant));
@@ -74,7 +71,6 @@ __global__ void k_calc_J(const double2 *__restrict__ D, const double2 *__restric
 #pragma unroll
     for (int q = 0; q < 4; q++) F += Ji[q].x * Ji[q].x + Ji[q].y * Ji[q].y;
     F = sqrt(F);
-    double2 *out = J + j * 4;
     if (F == 0.0) {
 #pragma unroll
         for (int q = 0; q < 4; q++) out[q] = make_double2(0.0, 0.0);
@@ -90,6 +86,30 @@ __global__ void k_calc_J(const double2 *__restrict__ D, const double2 *__restric
     out[1] = cm(ninv, Ji[1]);
     out[2] = cm(ninv, Ji[2]);
     out[3] = cm(inv, Ji[0]);
+}
+
+// D[a][c][4], B[p][a][4] -> J[p][a][c][4]; with bchan, B is [p][c][4] (a
+// per-fine-channel beam shared by all tiles).
+// block 256.  Each lane's 2x2 result (64 bytes) goes through LDS so that
+// the workgroup's 16 KB of output (consecutive j) is stored with 16-byte
+// lanes side by side.
+__global__ __launch_bounds__(256) void k_calc_J(const double2 *__restrict__ D,
+                                                const double2 *__restrict__ B, int npointing,
+                                                int nant, int nchan, int bchan,
+                                                double2 *__restrict__ J)
+{
+    __shared__ double2 sh[256 * 4];
+    const size_t j0 = (size_t)blockIdx.x * 256;
+    const size_t j = j0 + threadIdx.x;
+    const size_t total = (size_t)npointing * nant * nchan;
+    calc_J_one(D, B, nant, nchan, bchan, j < total ? j : total - 1, &sh[threadIdx.x * 4]);
+    __syncthreads();
+    const size_t nout = ((total < j0 + 256 ? total : j0 + 256) - j0) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const size_t e = threadIdx.x + 256 * q;
+        if (e < nout) J[j0 * 4 + e] = sh[e];
+    }
 }
 
 extern "C" {
@@ -174,6 +194,7 @@ void vmCalcJDevice(vcsbeam_context *vm)
     vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 0, vm->d_J, s);
     vm_timer_stop(d, VM_T_GEOM);
     d->B_chan = 0;
+    d->J_fromB = 1;
     HIPCHECK(hipStreamSynchronize(s));
     d->weights_gen++;
 }
@@ -188,6 +209,7 @@ void vmCalcJDeviceChanB(vcsbeam_context *vm, const gpuDoubleComplex *B)
     HIPCHECK(hipMemcpyAsync(d->d_B, B, bbytes, hipMemcpyHostToDevice, s));
     vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 1, vm->d_J, s);
     d->B_chan = 1;
+    d->J_fromB = 1;
     HIPCHECK(hipStreamSynchronize(s));
     d->weights_gen++;
 }

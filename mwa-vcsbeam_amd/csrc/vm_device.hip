@@ -252,6 +252,7 @@ void vmPushJ(vcsbeam_context *vm)
     HIPCHECK(hipMemcpyAsync(vm->d_J, vm->J, vm->J_size_bytes, hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
     vm->dev->weights_gen++;
+    vm->dev->J_fromB = 0;
 }
 
 void vmPushD(vcsbeam_context *vm)

@@ -64,6 +64,9 @@ struct vm_device_state {
     void *d_B;
     size_t B_bytes;
     int B_chan;
+    /* d_J was last computed on the device from D and d_B (vmCalcJDevice*),
+       so it is the same for every pointing wherever d_B is; 0 after vmPushJ */
+    int J_fromB;
     /* unit vectors per pointing (uE, uN, uH) */
     double *d_unit;
     size_t unit_bytes;
