@@ -174,9 +174,14 @@ def rocprof_summary(dirs, title, command, out_md, out_json):
                   open(out_json, "w"), indent=1)
 
 
-def committed_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 summary."""
+def committed_traffic(kernel, variant=None, order=None):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    summaries.  The beamformer GEMM's traffic depends on its tile shape and
+    order (how often voltage fragments are re-read), so its records are kept
+    per (variant, order): profiles/bench_traffic_v<variant>_o<order>.json."""
     f = os.path.join(ROOT, "profiles", "bench_traffic.json")
+    if variant is not None:
+        f = os.path.join(ROOT, "profiles", f"bench_traffic_v{variant}_o{order}.json")
     if not os.path.exists(f):
         return None
     try:
@@ -187,7 +192,7 @@ def committed_traffic(kernel):
         if nm.replace("void ", "").startswith(kernel):
             return {"bytes": round(t["read_bytes"] + t["write_bytes"]),
                     "read_bytes": round(t["read_bytes"]), "write_bytes": round(t["write_bytes"]),
-                    "source": "profiles/bench_traffic.json"}
+                    "source": os.path.relpath(f, ROOT)}
     return None
 
 
@@ -634,6 +639,8 @@ def main():
     print("[bench] MFMA FP16 form: warmup + timed steps", file=sys.stderr, flush=True)
     ms_step = time_form(lib, sess, pkg.VM_BF_MFMA_FP16, args.steps, args.warmup)
     gemm_ms, gemm_n = kernel_ms(lib, sess, "beamform_mfma")
+    sel_v, sel_o = C.c_int(0), C.c_int(0)
+    lib.vmBeamformerSelection(sess.vm, C.byref(sel_v), C.byref(sel_o))
     prep_ms, _ = kernel_ms(lib, sess, "beamform_prep")
     weights_ms, _ = kernel_ms(lib, sess, "beamform_weights")
     vr, vs, vw = C.c_int(0), C.c_int(0), C.c_int(0)
@@ -654,12 +661,12 @@ def main():
     mfma_tf = gemm_flops / (gemm_ms * 1e-3) / 1e12
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": committed_traffic("k_beamform_mfma"),
+                "traffic": committed_traffic("k_beamform_mfma", sel_v.value, sel_o.value),
                 "algorithmic_bytes": int(gemm_bytes), "algorithmic_flops": gemm_flops,
                 "mfma_tflops": round(mfma_tf, 1),
                 "mfma_frac_issued": round(2 * mfma_tf / FP16_PEAK_TFLOPS, 4),
                 "kernel": "k_beamform_mfma<fp16>", "kernel_ms": round(gemm_ms, 4),
-                "launches": gemm_n, "autotuned_variant": variant,
+                "launches": gemm_n, "autotuned_variant": variant, "tile_order": sel_o.value,
                 "workgroup_tile": f"{vr.value} rows x {vs.value} samples, {vw.value} waves",
                 "autotune_best_ms": round(best.value, 4)}
     result = {

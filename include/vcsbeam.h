@@ -727,6 +727,10 @@ void vmBeamformChunkMFMA(vcsbeam_context *vm);
  * tiles, column by column. */
 int vmTuneBeamformer(vcsbeam_context *vm, int reps, float *best_ms);
 void vmSetBeamformerVariant(vcsbeam_context *vm, int variant, int order);
+/* The variant and tile order the MFMA beamformer runs with; returns 1 when
+ * they were set by vmTuneBeamformer / vmSetBeamformerVariant, 0 for the
+ * defaults. */
+int vmBeamformerSelection(vcsbeam_context *vm, int *variant, int *order);
 /* Number of MFMA beamformer variants and the tile shape of one: rows and
  * samples per workgroup, wavefronts per workgroup. Returns 0 if no such. */
 int vmBeamformerVariantInfo(int variant, int *rows, int *samples, int *waves);

@@ -1422,6 +1422,15 @@ int vmBeamformerVariantInfo(int variant, int *rows, int *samples, int *waves)
     return N_VARIANTS;
 }
 
+int vmBeamformerSelection(vcsbeam_context *vm, int *variant, int *order)
+{
+    vm_device_state *d = vm_dev(vm);
+    const int tuned = d->bf_variant > 0;
+    if (variant) *variant = tuned ? d->bf_variant - 1 : 0;
+    if (order) *order = tuned ? d->bf_order : 1;
+    return tuned;
+}
+
 void vmSetBeamformerVariant(vcsbeam_context *vm, int variant, int order)
 {
     vm_device_state *d = vm_dev(vm);
