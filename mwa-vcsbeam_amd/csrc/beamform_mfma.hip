@@ -1094,7 +1094,7 @@ struct bf_variant { int mb, nb, wr, wc, nst, occ, lds; };
 static const bf_variant kVariants[] = {
     {4, 4, 2, 2, 4, 1}, {4, 4, 1, 4, 4, 1}, {4, 4, 1, 4, 6, 1}, {2, 4, 2, 2, 4, 2},
     {4, 2, 2, 2, 4, 2}, {2, 4, 4, 1, 4, 2}, {4, 4, 4, 1, 4, 1}, {4, 4, 2, 2, 6, 1},
-    {2, 4, 1, 4, 4, 2}, {2, 4, 4, 2, 4, 2}, {2, 4, 2, 2, 4, 2, 1},
+    {2, 4, 1, 4, 4, 2}, {2, 4, 4, 2, 4, 2}, {2, 4, 2, 2, 4, 2, 1}, {2, 4, 8, 1, 4, 2},
 };
 #define N_VARIANTS ((int)(sizeof(kVariants) / sizeof(kVariants[0])))
 #define VM_N_ORDERS 5
@@ -1127,7 +1127,8 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
     case 7: VM_LAUNCH(4, 4, 2, 2, 6, 1, false); break;
     case 8: VM_LAUNCH(2, 4, 1, 4, 4, 2, false); break;
     case 9: VM_LAUNCH(2, 4, 4, 2, 4, 2, false); break;
-    default: VM_LAUNCH(2, 4, 2, 2, 4, 2, true); break;
+    case 10: VM_LAUNCH(2, 4, 2, 2, 4, 2, true); break;
+    default: VM_LAUNCH(2, 4, 8, 1, 4, 2, false); break;
     }
 #undef VM_LAUNCH
 }
@@ -1150,7 +1151,8 @@ static void launch_batch_variant(int vi, unsigned nwg, hipStream_t st, const bf_
     case 6: VM_LAUNCH(4, 4, 4, 1, 4, 1); break;
     case 7: VM_LAUNCH(4, 4, 2, 2, 6, 1); break;
     case 8: VM_LAUNCH(2, 4, 1, 4, 4, 2); break;
-    default: VM_LAUNCH(2, 4, 4, 2, 4, 2); break;
+    case 9: VM_LAUNCH(2, 4, 4, 2, 4, 2); break;
+    default: VM_FATAL("batched beamformer: no persistent kernel for variant %d", vi);
     }
 #undef VM_LAUNCH
 }
