@@ -108,6 +108,8 @@ __device__ __forceinline__ int scale_exp(float mx)
 
 // scale of the phase weights: |Re phi|, |Im phi| <= 1
 #define PHASE_EXP 13
+// floats per (channel, pointing) of the per-pointing-beam constants (k_pb_beam)
+#define PB_ROW 24
 
 // ------------------------------------------- per-pointing primary beam ----
 // With a primary beam B[p][c] that is the same for every tile but differs
@@ -180,8 +182,9 @@ __global__ void k_pb_dinv(const double2 *__restrict__ D, int nant, int nchan, cf
     g[(size_t)c * nant + a] = make_float4((float)p00, (float)p11, (float)p01r, (float)p01i);
 }
 
-// Per (c, p): B^-1 (4 complex) and h of Q = B B^H as bx[c][p][12] floats,
-// and the weight exponent esp[c][p].  B: [p][c][4], D: [a][c][4].
+// Per (c, p): h of Q = B B^H and the products of the entries of B^-1 that
+// stokes_pb uses, as bx[c][p][PB_ROW] floats, and the weight exponent
+// esp[c][p].  B: [p][c][4], D: [a][c][4].
 __global__ void k_pb_beam(const double2 *__restrict__ D, const double2 *__restrict__ B, int np,
                           int nant, int nchan, float *__restrict__ bx, int *__restrict__ esp)
 {
@@ -197,22 +200,31 @@ __global__ void k_pb_beam(const double2 *__restrict__ D, const double2 *__restri
     const double2 r = dd > 0.0 ? make_double2(det.x / dd, -det.y / dd) : make_double2(0.0, 0.0);
     const double2 inv[4] = {zmul(b[3], r), zmul(make_double2(-b[1].x, -b[1].y), r),
                             zmul(make_double2(-b[2].x, -b[2].y), r), zmul(b[0], r)};
-    float *o = bx + ((size_t)c * np + p) * 12;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        o[2 * q] = (float)inv[q].x;
-        o[2 * q + 1] = (float)inv[q].y;
-    }
+    float *o = bx + ((size_t)c * np + p) * PB_ROW;
     // Q = B B^H: Q00 = |b00|^2 + |b01|^2, Q11 = |b10|^2 + |b11|^2,
     // Q01 = b00 conj(b10) + b01 conj(b11)
     const double q00 = b[0].x * b[0].x + b[0].y * b[0].y + b[1].x * b[1].x + b[1].y * b[1].y;
     const double q11 = b[2].x * b[2].x + b[2].y * b[2].y + b[3].x * b[3].x + b[3].y * b[3].y;
     const double q01r = b[0].x * b[2].x + b[0].y * b[2].y + b[1].x * b[3].x + b[1].y * b[3].y;
     const double q01i = b[0].y * b[2].x - b[0].x * b[2].y + b[1].y * b[3].x - b[1].x * b[3].y;
-    o[8] = (float)q00;
-    o[9] = (float)q11;
-    o[10] = (float)(2.0 * q01r);
-    o[11] = (float)(2.0 * q01i);
+    o[0] = (float)q00;
+    o[1] = (float)q11;
+    o[2] = (float)(2.0 * q01r);
+    o[3] = (float)(2.0 * q01i);
+    // products of the entries of M = B^-1 that the epilogue's M X M^H needs
+    // (per pointing, so formed here once in FP64): |m_ij|^2, then m00 m01*,
+    // m10 m11*, m00 m10*, m01 m11*, m00 m11*, m01 m10*
+#pragma unroll
+    for (int q = 0; q < 4; q++) o[4 + q] = (float)(inv[q].x * inv[q].x + inv[q].y * inv[q].y);
+    const int pr[6][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {0, 3}, {1, 2}};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const double2 u = inv[pr[k][0]], w = inv[pr[k][1]];
+        o[8 + 2 * k] = (float)(u.x * w.x + u.y * w.y);
+        o[9 + 2 * k] = (float)(u.y * w.x - u.x * w.y);
+    }
+#pragma unroll
+    for (int k = 20; k < PB_ROW; k++) o[k] = 0.f;
     double f2 = 0.0;
     for (int a = 0; a < nant; a++) f2 = fmax(f2, frob2_DB(D + ((size_t)a * nchan + c) * 4, b));
     int ex = 0;
@@ -563,7 +575,8 @@ __device__ __forceinline__ void stokes4(float xr, float xi, float yr, float yi, 
 // device layout (VM_S_IDX) of a second of ns_total samples, written from
 // sample soffset on.
 // Per-pointing primary-beam form: N holds T_k as float4 [c][s][4], bx the
-// pointing's B^-1 and h as float [c][p][12] and esp its weight exponents.
+// pointing's constants as float [c][p][PB_ROW] (k_pb_beam) and esp its
+// weight exponents.
 struct bf_geom {
     const uint4 *Af;
     const uint4 *Bf;
@@ -584,27 +597,29 @@ __device__ __forceinline__ c2f cconjf(c2f a) { return {a.x, -a.y}; }
 
 // Stokes of one pointing with its own primary beam from the GEMM output E'
 // (x, y; scaled, u2 removes the scales squared): X' = E' E'^H u2 -
-// sum_k h_k T_k, then X = B^-1 X' B^-H, whose (00, 11, 01) are (bnXX, bnYY,
-// bnXY) of src/form_beam.cpp:339-355.  bx: B^-1 row-major (4 complex), h.
+// sum_k h_k T_k, then X = M X' M^H with M = B^-1, whose (00, 11, 01) are
+// (bnXX, bnYY, bnXY) of src/form_beam.cpp:339-355.  R: the pointing's
+// constants from k_pb_beam: h (R[0]), |m00|^2 |m01|^2 |m10|^2 |m11|^2 (R[1]),
+// m00 m01* and m10 m11* (R[2]), m00 m10* and m01 m11* (R[3]), m00 m11* and
+// m01 m10* (R[4]).
 __device__ __forceinline__ void stokes_pb(float xr, float xi, float yr, float yi, float u2,
-                                          const float4 (&T)[4], const float4 (&bx)[3], float invw,
+                                          const float4 (&T)[4], const float4 (&R)[5], float invw,
                                           float (&st)[4])
 {
-    const float h0 = bx[2].x, h1 = bx[2].y, h2 = bx[2].z, h3 = bx[2].w;
+    const float h0 = R[0].x, h1 = R[0].y, h2 = R[0].z, h3 = R[0].w;
     const float X00 = (xr * xr + xi * xi) * u2 - (h0 * T[0].x + h1 * T[1].x + h2 * T[2].x + h3 * T[3].x);
     const float X11 = (yr * yr + yi * yi) * u2 - (h0 * T[0].y + h1 * T[1].y + h2 * T[2].y + h3 * T[3].y);
     const c2f X01 = {(xr * yr + xi * yi) * u2 - (h0 * T[0].z + h1 * T[1].z + h2 * T[2].z + h3 * T[3].z),
                      (xi * yr - xr * yi) * u2 - (h0 * T[0].w + h1 * T[1].w + h2 * T[2].w + h3 * T[3].w)};
-    const c2f b00 = {bx[0].x, bx[0].y}, b01 = {bx[0].z, bx[0].w};
-    const c2f b10 = {bx[1].x, bx[1].y}, b11 = {bx[1].z, bx[1].w};
-    const c2f p0 = cmulf(b00, cconjf(b01)), p1 = cmulf(b10, cconjf(b11));
-    const float Y00 = (b00.x * b00.x + b00.y * b00.y) * X00 + (b01.x * b01.x + b01.y * b01.y) * X11 +
-                      2.0f * (p0.x * X01.x - p0.y * X01.y);
-    const float Y11 = (b10.x * b10.x + b10.y * b10.y) * X00 + (b11.x * b11.x + b11.y * b11.y) * X11 +
-                      2.0f * (p1.x * X01.x - p1.y * X01.y);
-    const c2f q0 = cmulf(b00, cconjf(b10)), q1 = cmulf(b01, cconjf(b11));
-    const c2f r0 = cmulf(cmulf(b00, X01), cconjf(b11));
-    const c2f r1 = cmulf(cmulf(b01, cconjf(X01)), cconjf(b10));
+    const c2f p0 = {R[2].x, R[2].y}, p1 = {R[2].z, R[2].w};
+    const c2f q0 = {R[3].x, R[3].y}, q1 = {R[3].z, R[3].w};
+    const c2f c0 = {R[4].x, R[4].y}, c1 = {R[4].z, R[4].w};
+    // (M X M^H)_00 = |m00|^2 X00 + |m01|^2 X11 + 2 Re(m00 m01* X01*), with X01
+    // = X'_01 and X10 = X01*; likewise row 1; (M X M^H)_01 = m00 m10* X00 +
+    // m01 m11* X11 + m00 m11* X01 + m01 m10* X01*
+    const float Y00 = R[1].x * X00 + R[1].y * X11 + 2.0f * (p0.x * X01.x - p0.y * X01.y);
+    const float Y11 = R[1].z * X00 + R[1].w * X11 + 2.0f * (p1.x * X01.x - p1.y * X01.y);
+    const c2f r0 = cmulf(X01, c0), r1 = cmulf(cconjf(X01), c1);
     const float Y01r = q0.x * X00 + q1.x * X11 + r0.x + r1.x;
     const float Y01i = q0.y * X00 + q1.y * X11 + r0.y + r1.y;
     st[0] = invw * (Y00 + Y11);
@@ -613,20 +628,6 @@ __device__ __forceinline__ void stokes_pb(float xr, float xi, float yr, float yi
     st[3] = -2.0f * invw * Y01i;
 }
 
-// One workgroup tile (tx, ty) of fine channel c: WR x WC wavefronts cover
-// (WR MB 32) rows (16 WR MB beams) x (WC NB/2 32) samples; a wavefront's NB
-// column blocks are NB/2 sample blocks x 2 polarisations (block n = sample
-// block n/2, polarisation n%2).  The B ring keeps NST-1 k-steps of voltage
-// fragments in flight, the A ring one k-step of weights.
-// Epilogue: each (32-row block, sample block) pair is passed through a 4 KB
-// per-wavefront LDS tile (stg + 1024 w), 8 beams at a time, so that every
-// lane stores four consecutive samples of one (beam, Stokes) row with one
-// 16-byte store.
-// VEC: ns, ns_total and soffset are multiples of 4, so every in-range
-// 4-sample group is stored whole with one 16-byte store (the other form
-// stores sample by sample).
-// PB: the per-pointing primary-beam epilogue (stokes_pb); stg then also
-// holds a 16-float row per pointing of each wavefront after the tiles.
 // Each wavefront's staging tile holds 32 rows of 64 samples (68-float rows,
 // padded against bank conflicts).
 #define BF_WAVE_LDS (32 * 68)
@@ -637,7 +638,7 @@ constexpr int bf_wg_frags() { return 2 * WR * MB + WC * NB; }
 template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool PB, bool LDSOP = false>
 constexpr int bf_lds_floats()
 {
-    return WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 16 : 0) +
+    return WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 32 : 0) +
            (LDSOP ? 2 * bf_wg_frags<MB, NB, WR, WC>() * 256 : 0);
 }
 
@@ -652,10 +653,10 @@ __device__ __forceinline__ void bf_stokes(float xr, float xi, float yr, float yi
                                           float (&st4)[4])
 {
     if constexpr (PB) {
-        const float4 *bl = (const float4 *)(wpb + 16 * pl);
-        float4 bx[3] = {bl[0], bl[1], bl[2]};
-        const float u2 = ldexpf(1.0f, -2 * (esb + __float_as_int(wpb[16 * pl + 12])));
-        stokes_pb(xr, xi, yr, yi, u2, T4, bx, invw, st4);
+        const float4 *bl = (const float4 *)(wpb + 32 * pl);
+        const float4 R[5] = {bl[0], bl[1], bl[2], bl[3], bl[4]};
+        const float u2 = ldexpf(1.0f, -2 * (esb + __float_as_int(wpb[32 * pl + PB_ROW])));
+        stokes_pb(xr, xi, yr, yi, u2, T4, R, invw, st4);
         if (!valid) st4[0] = st4[1] = st4[2] = st4[3] = 0.f;
     } else {
         stokes4(xr, xi, yr, yi, unscale2, nn, invw, st4);
@@ -725,7 +726,7 @@ __device__ __forceinline__ void bf_kloop_lds(f16v (&acc)[MB][NB], rsrc_t rA, rsr
     // wavefront tiles only (a 128 x 128 tile's 256 accumulators leave no room)
     static_assert(MB * NB <= 8, "shared-operand k-loop is for 64 x 64 wavefront tiles");
     constexpr int PER = F / NW, NSB = NB / 2;
-    uint4 *ops = (uint4 *)(stg + WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 16 : 0));
+    uint4 *ops = (uint4 *)(stg + WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 32 : 0));
     const int wr = w / WC, wc = w % WC;
     const int rbw0 = ty * WR * MB, sbw0 = tx * WC * NSB;
     int goff[PER]; // this wavefront's loader fragments: byte offsets at k-step 0
@@ -876,20 +877,21 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
     // per-pointing-beam form: its T_k terms (16 floats per sample) would
     // cost registers through the k-loop, so they are requested here, after
     // it and before the first store
-    // Its pointings' B^-1 and h (12 floats) and weight exponents go to a
-    // 16-float LDS row per pointing of the wavefront (16 MB pointings).
+    // Its pointings' constants (PB_ROW floats from k_pb_beam) and weight
+    // exponents go to a 32-float LDS row per pointing of the wavefront (16 MB
+    // pointings).
     float4 T4v[PB ? NSB : 1][4];
-    float *wpb = stg + WR * WC * BF_WAVE_LDS + (PB ? w * MB * 16 * 16 : 0);
+    float *wpb = stg + WR * WC * BF_WAVE_LDS + (PB ? w * MB * 16 * 32 : 0);
     if constexpr (PB) {
         for (int pl = lane; pl < MB * 16; pl += 64) {
             const int cp = c * G.npointing + min(16 * rb0 + pl, G.npointing - 1);
-            const gmem f4v *bp = (const gmem f4v *)(G.bx + (size_t)cp * 12);
+            const gmem f4v *bp = (const gmem f4v *)(G.bx + (size_t)cp * PB_ROW);
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
+            for (int q = 0; q < PB_ROW / 4; q++) {
                 const f4v t = bp[q];
-                *(float4 *)(wpb + 16 * pl + 4 * q) = make_float4(t.x, t.y, t.z, t.w);
+                *(float4 *)(wpb + 32 * pl + 4 * q) = make_float4(t.x, t.y, t.z, t.w);
             }
-            wpb[16 * pl + 12] = __int_as_float(((const gmem int *)G.esp)[cp]);
+            wpb[32 * pl + PB_ROW] = __int_as_float(((const gmem int *)G.esp)[cp]);
         }
 #pragma unroll
         for (int j = 0; j < NSB; j++) {
@@ -1313,7 +1315,7 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     const size_t se_bytes = align256((size_t)nchan * nsb * sizeof(int));
     const size_t jf_bytes = align256((size_t)nchan * nant * 4 * sizeof(cf));
     const size_t g_bytes = pb ? align256((size_t)nchan * nant * sizeof(float4)) : 0;
-    const size_t bx_bytes = pb ? align256((size_t)nchan * np * 12 * sizeof(float)) : 0;
+    const size_t bx_bytes = pb ? align256((size_t)nchan * np * PB_ROW * sizeof(float)) : 0;
     const size_t esp_bytes = pb ? align256((size_t)nchan * np * sizeof(int)) : 0;
     const size_t need = af_bytes + bf_bytes + n_bytes + se_bytes + jf_bytes + g_bytes + bx_bytes +
                         esp_bytes;
