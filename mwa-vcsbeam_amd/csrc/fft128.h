@@ -102,12 +102,17 @@ __device__ __forceinline__ void dft8(cf x[8])
 // at n = 8*n1 + l, 16-point DFT over n1, twiddles by W128^{l k1} (table tw),
 // and writes back at index k1*8 + l of the same row.  Contains two
 // workgroup barriers, so every lane of the workgroup must call it.
+// SCALE: the caller's table carries a factor 1/KCH (exact: a power of two),
+// and the k1 = 0 term, which has no twiddle, is scaled here, so the whole
+// transform comes out divided by KCH with bit-identical values.
+template <bool SCALE = false>
 __device__ __forceinline__ void fft_stage1(cf *row, int l, const cf *tw)
 {
     cf x[16];
 #pragma unroll
     for (int n1 = 0; n1 < 16; n1++) x[n1] = row[8 * n1 + l];
     dft16(x);
+    if (SCALE) x[0] = cf{x[0].x * (1.0f / KCH), x[0].y * (1.0f / KCH)};
 #pragma unroll
     for (int k1 = 1; k1 < 16; k1++) x[k1] = mul(x[k1], tw[l * k1]);
     __syncthreads();

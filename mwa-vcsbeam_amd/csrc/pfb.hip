@@ -63,9 +63,12 @@ template <bool FPGA>
 __device__ __forceinline__ float demote(int X, float post)
 {
     if (FPGA) {
-        // src/pfb.cpp:225-240: (X > 0 ? X + 0x2000 : X) >> 14; the sum
-        // wraps like the reference's int
-        const uint32_t bias = X > 0 ? 0x2000u : 0u;
+        // src/pfb.cpp:225-240: (X > 0 ? X + 0x2000 : X) >> 14; the bias is
+        // (median of X, 0, 1) << 13, one v_med3_i32 (the compiler would
+        // otherwise compare and select); the sum wraps like the reference's int
+        int m;
+        asm("v_med3_i32 %0, %1, 0, 1" : "=v"(m) : "v"(X));
+        const uint32_t bias = (uint32_t)m << 13;
         const int y = (int)((uint32_t)X + bias) >> 14;
         return (float)y * post;
     } else {
@@ -224,18 +227,19 @@ __device__ __forceinline__ void pfb_tile(cf *F, const uint32_t *win, const cf *t
                 acc[j][e] = (int)((uint32_t)acc[j][e] + ((uint32_t)bcc[j][e] << 15));
     }
     __syncthreads(); // window no longer needed: F may overwrite it
-    const float post = 1.0f / KCH; // the /K of the pack step, exact
+    // the /K of the pack step is applied by the FFT (scaled twiddles): a
+    // power of two, so the values are those of scaling here
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         cf *dst = &F[(8 * g + j) * ROW + 2 * col];
-        dst[0] = cf{demote<FPGA>(acc[j][0], post), demote<FPGA>(acc[j][1], post)};
-        dst[1] = cf{demote<FPGA>(acc[j][2], post), demote<FPGA>(acc[j][3], post)};
+        dst[0] = cf{demote<FPGA>(acc[j][0], 1.0f), demote<FPGA>(acc[j][1], 1.0f)};
+        dst[1] = cf{demote<FPGA>(acc[j][2], 1.0f), demote<FPGA>(acc[j][3], 1.0f)};
     }
     __syncthreads();
 
     // 32 FFTs, 8 lanes each
     const int f = t & 31, l = t >> 5;
-    vmfft::fft_stage1(&F[f * ROW], l, tw);
+    vmfft::fft_stage1<true>(&F[f * ROW], l, tw);
     cf y[2][8];
     vmfft::fft_stage2(&F[f * ROW], l, y);
 
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(256) void k_pfb_fused(
     if (T0 >= Tend) return;
     const pfb_rsrc rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(htr), (short)0,
                                                           (int)htr_bytes, PFB_RSRC_DWORD3);
-    if (t < KCH) tw[t] = g_tw128[t];
+    if (t < KCH) tw[t] = cf{g_tw128[t].x * (1.0f / KCH), g_tw128[t].y * (1.0f / KCH)};
     pfb_taps T;
     pfb_load_taps<SPLIT>(T, h, t & 63);
     uint4 wr[PFB_WPL];
