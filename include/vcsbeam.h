@@ -669,7 +669,9 @@ void vmSetFilter(vcsbeam_context *vm, const double *coeffs, int ncoeffs,
 /* Device FP64: phi from tile positions and pointing unit vectors, with
  * argument reduction in cycles; result in gdelays.d_phi (and not on host). */
 void vmCalcPhiDevice(vcsbeam_context *vm, beam_geom *beam_geom_vals);
-/* Device FP64: J^-1 = F (D B)^-1 from d_D and pb.B; result in d_J. */
+/* Device FP64: J^-1 = F (D B)^-1 from d_D and pb.B; result in d_J.
+ * d_J is written by this call, vmCalcJDeviceChanB or vmPushJ only: the MFMA
+ * form learns from them whether J^-1 can differ between pointings. */
 void vmCalcJDevice(vcsbeam_context *vm);
 /* Upload vm->D to d_D. */
 void vmPushD(vcsbeam_context *vm);

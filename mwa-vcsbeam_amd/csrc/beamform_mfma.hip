@@ -1257,7 +1257,9 @@ static int pointings_differ(vcsbeam_context *vm, hipStream_t st, uint32_t *flag)
 {
     vm_device_state *d = vm_dev(vm);
     const int np = (int)vm->npointing;
-    if (!d->J_fromB || !d->d_B) return j_differs(vm, st, flag);
+    // d_B must hold the current pointings (a changed pointing count without
+    // a new J^-1 falls back to comparing J^-1 itself)
+    if (!d->J_fromB || !d->d_B || d->B_np != np) return j_differs(vm, st, flag);
     const uint32_t per =
         (uint32_t)(d->B_chan ? vm->nfine_chan : (int)vm->obs_metadata->num_ants) * 4;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, flag, (size_t)1, 0u);

@@ -195,6 +195,7 @@ void vmCalcJDevice(vcsbeam_context *vm)
     vm_timer_stop(d, VM_T_GEOM);
     d->B_chan = 0;
     d->J_fromB = 1;
+    d->B_np = np;
     HIPCHECK(hipStreamSynchronize(s));
     d->weights_gen++;
 }
@@ -210,6 +211,7 @@ void vmCalcJDeviceChanB(vcsbeam_context *vm, const gpuDoubleComplex *B)
     vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 1, vm->d_J, s);
     d->B_chan = 1;
     d->J_fromB = 1;
+    d->B_np = np;
     HIPCHECK(hipStreamSynchronize(s));
     d->weights_gen++;
 }

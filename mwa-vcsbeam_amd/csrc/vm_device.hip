@@ -196,6 +196,7 @@ void vmMallocJDevice(vcsbeam_context *vm)
     vm->d_J_size_bytes = (uintptr_t)vm->npointing * vm->obs_metadata->num_ants * vm->nfine_chan *
                          vm->obs_metadata->num_visibility_pols * sizeof(gpuDoubleComplex);
     vm->d_J = (gpuDoubleComplex *)dmalloc(vm->d_J_size_bytes);
+    vm_dev(vm)->J_fromB = 0; /* not computed from d_B yet */
 }
 void vmFreeJDevice(vcsbeam_context *vm) { dfree(vm->d_J); vm->d_J = NULL; }
 

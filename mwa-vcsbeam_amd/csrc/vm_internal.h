@@ -65,8 +65,11 @@ struct vm_device_state {
     size_t B_bytes;
     int B_chan;
     /* d_J was last computed on the device from D and d_B (vmCalcJDevice*),
-       so it is the same for every pointing wherever d_B is; 0 after vmPushJ */
+       so it is the same for every pointing wherever d_B is; 0 after vmPushJ
+       or a new d_J.  B_np: the pointings d_B holds (B_chan 0: [B_np][a][4],
+       1: [B_np][c][4]). */
     int J_fromB;
+    int B_np;
     /* unit vectors per pointing (uE, uN, uH) */
     double *d_unit;
     size_t unit_bytes;
