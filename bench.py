@@ -174,14 +174,12 @@ def rocprof_summary(dirs, title, command, out_md, out_json):
                   open(out_json, "w"), indent=1)
 
 
-def committed_traffic(kernel, variant=None, order=None):
+def committed_traffic(kernel, variant, order):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3
     summaries.  The beamformer GEMM's traffic depends on its tile shape and
     order (how often voltage fragments are re-read), so its records are kept
     per (variant, order): profiles/bench_traffic_v<variant>_o<order>.json."""
-    f = os.path.join(ROOT, "profiles", "bench_traffic.json")
-    if variant is not None:
-        f = os.path.join(ROOT, "profiles", f"bench_traffic_v{variant}_o{order}.json")
+    f = os.path.join(ROOT, "profiles", f"bench_traffic_v{variant}_o{order}.json")
     if not os.path.exists(f):
         return None
     try:
