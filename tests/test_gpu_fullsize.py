@@ -1,0 +1,91 @@
+"""GPU parity at the workloads' full sizes (128 tiles x 2 pols, one whole
+second), through checks that do not need the oracle to run at that size.
+
+* Forward PFB over a full second (20 MWAX blocks, 10 000 spectra): the fused
+  kernel and the staged four-pass path (WOLA, rounding, FFT, pack) give
+  identical bits.  The staged path is pinned to the oracle bit for bit at
+  0.1 s (test_gpu_pfb.py); its per-element arithmetic does not depend on the
+  size.
+* Precision form at 512 beams (the benchmark workload): every Stokes value
+  of the second is finite, whole-second rows of 8 beams (the first and last
+  four, all 10 000 samples x 128 channels) agree with the FP32 form to
+  <= 1e-3 on Stokes I, and 4 beams x 64 samples agree with the FP64 oracle to
+  <= 1e-3 on Stokes I.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def full_obs(pkg):
+    from vcsbeam_amd.workloads import Observation
+    return Observation(ntiles=128, blocks_per_second=20)
+
+
+def test_full_second_pfb_fused_equals_staged(pkg, gpu_lib, full_obs):
+    from vcsbeam_amd.apps import fine_pfb_offline
+    flags = pkg.PFB_COMPLEX_FLOAT32 | pkg.PFB_EMULATE_FPGA | pkg.PFB_MALLOC_HOST_OUTPUT | 0x0C
+    print("full-second PFB, 128 tiles: fused", flush=True)
+    fused = fine_pfb_offline(full_obs, pfb_flags=flags)[0]
+    print("full-second PFB, 128 tiles: staged", flush=True)
+    staged = fine_pfb_offline(full_obs, pfb_flags=flags | pkg.PFB_STAGED)[0]
+    assert fused.shape == staged.shape and fused.size == 128 * 256 * 10000
+    np.testing.assert_array_equal(fused.view(np.uint32), staged.view(np.uint32))
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
+
+
+def test_full_second_512_beams_mfma(pkg, orc, gpu_lib, full_obs):
+    from vcsbeam_amd.apps import TiedArraySession
+    from vcsbeam_amd.workloads import directions_in_cap, unit_vectors
+    obs = full_obs
+    az, za = directions_in_cap(512, 15.0)
+    sess = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
+                            nstokes=4, form=pkg.VM_BF_MFMA_FP16)
+    try:
+        sess.read(0)
+        sess.pfb()
+        print("512 beams x 1 s, MFMA FP16", flush=True)
+        sess.geometry()
+        sess.beamform()
+        beams = [(0, 4), (508, 512)]
+        m = np.concatenate([sess.stokes_slice(p0, p1, 0, 10000) for p0, p1 in beams])
+        # every value of the second is finite: a device reduction would need
+        # a kernel of its own, so the full S comes to the host once
+        S = sess.stokes()
+        assert np.isfinite(S).all()
+        del S
+        print("512 beams x 1 s, FP32", flush=True)
+        sess.v.beamform_form = pkg.VM_BF_FP32
+        sess.geometry()
+        sess.beamform()
+        f = np.concatenate([sess.stokes_slice(p0, p1, 0, 10000) for p0, p1 in beams])
+        for b in range(m.shape[0]):
+            e = _rel(m[b, :, 0, :].astype(np.float64), f[b, :, 0, :].astype(np.float64))
+            print(f"beam {b}: MFMA vs FP32 Stokes I {e:.2e}", flush=True)
+            assert e <= 1e-3
+        # FP64 oracle on 4 beams x 64 samples of the same device voltages
+        vfl = sess.voltages()
+        n = 64
+        v = np.ascontiguousarray(vfl[:, :, :n].transpose(2, 0, 1)).astype(np.complex128)
+        del vfl
+        D = obs.calibration()
+        B = np.zeros((4, obs.ntiles, 2, 2), complex)
+        B[..., 0, 0] = B[..., 1, 1] = 1
+        E, N, H, L = obs.positions()
+        phi = orc.calc_phi(E, N, H, L, unit_vectors(az[:4], za[:4]), obs.freqs)
+        polQ, polP = obs.pol_idx()
+        ref, _ = orc.beamform(v, orc.calc_J(D, B), phi, polQ, polP,
+                              1.0 / (2 * int((~obs.flag_mask).sum())), nstokes=4)
+        for b in range(4):
+            e = _rel(m[b, :n, 0, :].astype(np.float64), ref[b, :, 0, :])
+            print(f"beam {b}: MFMA vs FP64 Stokes I {e:.2e}", flush=True)
+            assert e <= 1e-3
+    finally:
+        sess.close()
