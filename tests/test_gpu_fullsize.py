@@ -11,9 +11,12 @@ second), through checks that do not need the oracle to run at that size.
   four, all 10 000 samples x 128 channels) agree with the FP32 form to
   <= 1e-3 on Stokes I, and 4 beams x 64 samples agree with the FP64 oracle to
   <= 1e-3 on Stokes I.
+* Batched workload (24 coarse channels x 1 s of 256 tiles, flagged tiles
+  U{0..40} and beams log-uniform 1..2048 per channel, seconds generated on
+  the device, one replayed hipGraph): per channel, its first two and last two
+  beams x 32 samples x 128 channels agree with the FP64 oracle fed that
+  channel's own PFB output (1e-3 on Stokes I, 3e-3 on Q, U, V).
 """
-import ctypes as C
-
 import numpy as np
 import pytest
 
@@ -89,3 +92,52 @@ def test_full_second_512_beams_mfma(pkg, orc, gpu_lib, full_obs):
             assert e <= 1e-3
     finally:
         sess.close()
+
+
+def test_full_batched_workload(pkg, orc, gpu_lib):
+    from vcsbeam_amd.apps import BatchSession
+    from vcsbeam_amd.workloads import (SEED_BEAMS, SEED_FLAGS, Observation, directions_in_cap,
+                                       uniform, unit_vectors)
+    lib = gpu_lib
+    nchan, ntiles, ns = 24, 256, 32
+    u = uniform(SEED_FLAGS, 2 * nchan, offset=10 ** 8)
+    nflag = np.minimum((u[0::2] * 41).astype(int), 40)
+    nbeam = np.floor(np.exp(u[1::2] * np.log(2049.0))).astype(int)
+    obs = [Observation(ntiles=ntiles, blocks_per_second=20, nflagged=int(nflag[g]),
+                       rec_chan=109 + g, seed_offset=1000 * (g + 1), geometry_seed_offset=0)
+           for g in range(nchan)]
+    dirs = [directions_in_cap(int(nbeam[g]), 15.0, seed=SEED_BEAMS + 100 * (g + 1))
+            for g in range(nchan)]
+    print(f"batch: {nchan} channels x 1 s, {ntiles} tiles, {int(nbeam.sum())} beams", flush=True)
+    bs = BatchSession(obs, dirs, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
+                      nstokes=4, form=pkg.VM_BF_MFMA_FP16, share_pfb_output=True,
+                      device_data=True)
+    try:
+        bs.read(0)
+        bs.run(use_graph=True)
+        lib.vmDeviceSynchronize()
+        nspc = obs[0].nspectra
+        for g, ses in enumerate(bs.sessions):
+            o, (az, za) = obs[g], dirs[g]
+            ses.pfb()  # the shared PFB output buffer holds the last channel's
+            v = np.empty((128, o.ninput, ns), np.complex64)
+            lib.vmMemcpy2DDtoH(v.ctypes.data, ns * 8, ses.v.d_v, nspc * 8, ns * 8,
+                               128 * o.ninput)
+            npt = len(az)
+            sel = sorted(set([0, 1, npt - 2, npt - 1]) & set(range(npt)))
+            B = np.zeros((len(sel), o.ntiles, 2, 2), complex)
+            B[..., 0, 0] = B[..., 1, 1] = 1
+            E, N, H, L = o.positions()
+            phi = orc.calc_phi(E, N, H, L, unit_vectors(az[sel], za[sel]), o.freqs)
+            polQ, polP = o.pol_idx()
+            ref, _ = orc.beamform(np.ascontiguousarray(v.transpose(2, 0, 1)).astype(np.complex128),
+                                  orc.calc_J(o.calibration(), B), phi, polQ, polP,
+                                  1.0 / (2 * int((~o.flag_mask).sum())), nstokes=4)
+            for i, p in enumerate(sel):
+                got = ses.stokes_slice(p, p + 1, 0, ns)[0].astype(np.float64)
+                err = [orc.rel_err(got[:, k, :], ref[i, :, k, :]) for k in range(4)]
+                print(f"channel {g} ({npt} beams, {int(nflag[g])} flagged) beam {p}: "
+                      f"IQUV {np.round(err, 6)}", flush=True)
+                assert err[0] <= 1e-3 and max(err[1:]) <= 3e-3
+    finally:
+        bs.close()
