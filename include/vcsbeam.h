@@ -261,6 +261,51 @@ typedef struct primary_beam_t {
     MetafitsMetadata *obs_metadata;
 } primary_beam;
 
+/* VDIF frame header: 8 little-endian 32-bit words, the 32-byte header of
+ * the VDIF 1.0 specification (vdifio's vdif_header, used at
+ * include/vcsbeam.h.in:637 and src/beam_vdif.c). */
+#define VDIF_HEADER_BYTES 32
+typedef struct vdif_header_t {
+    uint32_t w[8];
+} vdif_header;
+
+/* VDIF output description of one pointing (include/vcsbeam.h.in:1107-1146),
+ * filled by vmPopulateVDIFHeader. */
+struct vdifinfo {
+    int frame_length;         /* bytes per frame, header included (544) */
+    int frame_rate;           /* frames per second */
+    size_t samples_per_frame; /* time samples per frame (128) */
+    int dataarraylength;      /* payload bytes per frame (512) */
+    int bits;                 /* bits per sample (8) */
+    int nchan;                /* channels per frame: one per polarisation (2) */
+    int chan_width;           /* channel width, Hz */
+    int sample_rate;          /* sample rate, Hz */
+    int npol;
+    int iscomplex;
+    int threadid;
+    char stationid[3];
+    char exp_name[17];
+    char scan_name[17];
+    size_t block_size;        /* bytes of one second of frames, headers included */
+    size_t sizeof_buffer;     /* payload values of one second */
+    size_t sizeof_beam;       /* payload values of one frame */
+    float *b_scales;
+    float *b_offsets;
+    char telescope[24];
+    char source[24];
+    char obs_mode[8];
+    double fctr;              /* centre frequency, MHz */
+    char ra_str[16];
+    char dec_str[16];
+    double BW;                /* bandwidth, MHz */
+    double MJD_start;
+    double sec_offset;
+    double MJD_epoch;
+    char date_obs[24];
+    char basefilename[1024];
+    int got_scales;
+};
+
 typedef struct beam_geom_t {
     double mean_ra;
     double mean_dec;
@@ -480,7 +525,7 @@ typedef struct vcsbeam_context_t {
 
     int out_nstokes;
 
-    uint8_t vhdr[32];          /* vdif_header (32-byte VDIF frame header) */
+    vdif_header vhdr;          /* VDIF frame header of the next frame (32 bytes) */
     struct vdifinfo *vf;
 
     float *d_offsets, *offsets;
@@ -784,6 +829,38 @@ void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g,
 /* Device VDIF requantisation of n floats (float2int8_trunc + offset binary). */
 void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n,
                          gpuStream_t stream);
+
+/* VDIF output (src/beam_vdif.c).
+ * vmPopulateVDIFHeader fills vm->vf (one struct vdifinfo per pointing: 8-bit
+ * complex samples, 2 channels = 2 polarisations, 128 samples per frame,
+ * fine_sample_rate frames per second, station "mw", file base name
+ * <project>_<obs id>_<RA>_<Dec>_ch<rec chan>) and vm->vhdr (epoch from
+ * beam_geom_vals->intmjd, seconds from intmjd and fracmjd, frame 0)
+ * (src/beam_vdif.c:150-246).
+ * vdif_write_second frames one second of data_buffer_vdif (sizeof_buffer
+ * floats [sample][pol][re, im]) on the GPU: each frame is the current
+ * header and sizeof_beam values clamped to [-126, 127], rounded half to even
+ * and stored as offset binary; the header advances by one frame per frame,
+ * as vdifio's nextVDIFHeader does.  It appends the frames to
+ * <basefilename>.vdif and writes the DSPSR ASCII header <basefilename>.hdr
+ * (vdif_write_data) (src/beam_vdif.c:52-143).  data_buffer_vdif is not
+ * modified (the reference clamps it in place).
+ * float2int8_trunc and to_offset_binary are the reference's host helpers
+ * (src/beam_vdif.c:29-38, 254-260). */
+void vmPopulateVDIFHeader(vcsbeam_context *vm, beam_geom *beam_geom_vals, double mjd_start,
+                          double sec_offset);
+void vdif_write_second(struct vdifinfo *vf, vdif_header *vhdr, float *data_buffer_vdif);
+void vdif_write_data(struct vdifinfo *vf, int8_t *output);
+void float2int8_trunc(float *f, int n, float min, float max, int8_t *i);
+void to_offset_binary(int8_t *i, int n);
+/* Device VDIF frames (no counterpart in the reference, which frames on the
+ * host): nframes frames of payload_bytes requantised values each, from the
+ * device floats d_in, headers from *hdr onwards (advanced by nframes on
+ * return) at frame_rate frames per second, into d_out (nframes x (32 +
+ * payload_bytes) bytes).  payload_bytes is a multiple of 4; asynchronous on
+ * stream. */
+void vmVDIFFramesDevice(const float *d_in, size_t nframes, int payload_bytes, vdif_header *hdr,
+                        int frame_rate, uint8_t *d_out, gpuStream_t stream);
 
 /* Synchronise the context's device work. */
 void vmSynchronize(vcsbeam_context *vm);

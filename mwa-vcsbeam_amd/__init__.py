@@ -104,6 +104,24 @@ class BeamGeom(C.Structure):
                                           "intmjd", "unit_N", "unit_E", "unit_H")]
 
 
+class Vdifinfo(C.Structure):
+    """struct vdifinfo (one pointing's VDIF output description)."""
+    _fields_ = [("frame_length", C.c_int), ("frame_rate", C.c_int),
+                ("samples_per_frame", C.c_size_t), ("dataarraylength", C.c_int),
+                ("bits", C.c_int), ("nchan", C.c_int), ("chan_width", C.c_int),
+                ("sample_rate", C.c_int), ("npol", C.c_int), ("iscomplex", C.c_int),
+                ("threadid", C.c_int), ("stationid", C.c_char * 3), ("exp_name", C.c_char * 17),
+                ("scan_name", C.c_char * 17), ("block_size", C.c_size_t),
+                ("sizeof_buffer", C.c_size_t), ("sizeof_beam", C.c_size_t),
+                ("b_scales", C.c_void_p), ("b_offsets", C.c_void_p),
+                ("telescope", C.c_char * 24), ("source", C.c_char * 24),
+                ("obs_mode", C.c_char * 8), ("fctr", C.c_double), ("ra_str", C.c_char * 16),
+                ("dec_str", C.c_char * 16), ("BW", C.c_double), ("MJD_start", C.c_double),
+                ("sec_offset", C.c_double), ("MJD_epoch", C.c_double),
+                ("date_obs", C.c_char * 24), ("basefilename", C.c_char * 1024),
+                ("got_scales", C.c_int)]
+
+
 class HostBuffer(C.Structure):
     _fields_ = [("buffer", C.c_void_p), ("buffer_size", C.c_size_t), ("read_ptr", C.c_void_p),
                 ("read_size", C.c_size_t), ("copy_from_ptr", C.c_void_p),
@@ -180,7 +198,7 @@ class VcsbeamContext(C.Structure):
         ("sample_rate", C.c_int), ("fine_sample_rate", C.c_int), ("bytes_per_second", C.c_int),
         ("num_not_flagged", C.c_int), ("nchan", C.c_int), ("nfine_chan", C.c_int),
         ("out_nstokes", C.c_int),
-        ("vhdr", C.c_uint8 * 32), ("vf", _P),
+        ("vhdr", C.c_uint32 * 8), ("vf", _P),
         ("d_offsets", _P), ("offsets", _P), ("d_scales", _P), ("scales", _P),
         ("d_Cscaled", _P), ("Cscaled", _P),
         ("offsets_size", _SZ), ("scales_size", _SZ), ("Cscaled_size", _SZ),
@@ -207,6 +225,8 @@ LAYOUT_FIELDS = {
                                             "project_id"]),
     "Rfinput": (Rfinput, ["pol", "electrical_length_m", "vcs_order", "flagged"]),
     "gpu_ipfb_arrays": (GpuIpfbArrays, ["in_real", "d_out"]),
+    "vdifinfo": (Vdifinfo, ["samples_per_frame", "stationid", "block_size", "b_scales", "fctr",
+                            "ra_str", "BW", "date_obs", "basefilename", "got_scales"]),
 }
 
 _CTX = C.POINTER(VcsbeamContext)
@@ -285,6 +305,11 @@ _SIGS = {
     "vmRequantVDIFDevice": (None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "create_data_buffer_fine": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "prepare_data_buffer_fine": (None, [C.c_void_p, _CTX, C.c_size_t]),
+    "vmVDIFFramesDevice": (None, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int,
+                                  C.c_void_p, C.c_void_p]),
+    "vmPopulateVDIFHeader": (None, [_CTX, C.POINTER(BeamGeom), C.c_double, C.c_double]),
+    "vdif_write_second": (None, [C.POINTER(Vdifinfo), C.c_void_p, C.c_void_p]),
+    "vdif_write_data": (None, [C.POINTER(Vdifinfo), C.c_void_p]),
     "float2int8_trunc": (None, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_void_p]),
     "to_offset_binary": (None, [C.c_void_p, C.c_int]),
     "calc_array_factor": (C.c_double, [C.POINTER(MetafitsMetadata), C.c_uint32,

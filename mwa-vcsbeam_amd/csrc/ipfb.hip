@@ -22,6 +22,8 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+#include <stdio.h>
 
 #include "fft128.h"
 #include "vm_internal.h"
@@ -164,28 +166,139 @@ __global__ void k_ipfb_save_hist(const cf *__restrict__ e, int ns, int P, int np
 
 // VDIF requantisation: clamp to [-126, 127], round half to even, int8,
 // offset binary (XOR 0x80).  Four values per lane.
+__device__ __forceinline__ uint32_t vdif_q4(float4 v)
+{
+    const float a[4] = {v.x, v.y, v.z, v.w};
+    uint32_t w = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float x = fminf(a[q], 127.0f);
+        x = fmaxf(x, -126.0f);
+        int iq = (int)rintf(x);
+        w |= (uint32_t)((uint8_t)((int8_t)iq ^ 0x80)) << (8 * q);
+    }
+    return w;
+}
+
 __global__ void k_vdif_requant(const float *__restrict__ in, uint8_t *__restrict__ out, size_t n)
 {
     size_t j = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (j >= n) return;
     if (j + 4 <= n) {
-        float4 v = *(const float4 *)(in + j);
-        float a[4] = {v.x, v.y, v.z, v.w};
-        uint32_t w = 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            float x = fminf(a[q], 127.0f);
-            x = fmaxf(x, -126.0f);
-            int iq = (int)rintf(x);
-            w |= (uint32_t)((uint8_t)((int8_t)iq ^ 0x80)) << (8 * q);
-        }
-        *(uint32_t *)(out + j) = w;
+        *(uint32_t *)(out + j) = vdif_q4(*(const float4 *)(in + j));
     } else {
         for (size_t q = j; q < n; q++) {
             float x = fmaxf(fminf(in[q], 127.0f), -126.0f);
             out[q] = (uint8_t)((int8_t)(int)rintf(x) ^ 0x80);
         }
     }
+}
+
+// VDIF frames: nframes x (32-byte header + pw payload words), one lane per
+// output 32-bit word.  Frame f's header is the template t with the seconds
+// (word 0, bits 0-29) and frame number (word 1, bits 0-23) of frame fr0 + f
+// of a second of frame_rate frames counted from sec0 (the VDIF 1.0 header;
+// nextVDIFHeader's increment, src/beam_vdif.c:79); its payload word q is
+// samples 4 q .. 4 q + 3 of the frame requantised like k_vdif_requant.
+__global__ void k_vdif_frames(const float *__restrict__ in, size_t nframes, int pw, uint4 tlo,
+                              uint4 thi, uint32_t sec0, uint32_t fr0, uint32_t frame_rate,
+                              uint32_t *__restrict__ out)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t fw = 8 + (size_t)pw;
+    if (j >= nframes * fw) return;
+    const size_t f = j / fw;
+    const int q = (int)(j - f * fw);
+    uint32_t w;
+    if (q >= 8) {
+        w = vdif_q4(*(const float4 *)(in + (f * pw + (q - 8)) * 4));
+    } else {
+        const uint64_t g = (uint64_t)fr0 + f;
+        const uint32_t sec = sec0 + (uint32_t)(g / frame_rate), fn = (uint32_t)(g % frame_rate);
+        const uint32_t t[8] = {tlo.x, tlo.y, tlo.z, tlo.w, thi.x, thi.y, thi.z, thi.w};
+        w = t[q];
+        if (q == 0) w = (w & 0xC0000000u) | (sec & 0x3FFFFFFFu);
+        if (q == 1) w = (w & 0xFF000000u) | (fn & 0x00FFFFFFu);
+    }
+    out[j] = w;
+}
+
+// ------------------------------------------------ VDIF headers (host) ----
+// The VDIF 1.0 frame header as vdifio builds it for src/beam_vdif.c:216-225
+// (createVDIFHeader, setVDIFEpochMJD, setVDIFFrameMJDSec, setVDIFFrameNumber):
+//   word 0: invalid (31), legacy (30), seconds from the reference epoch (29-0)
+//   word 1: reference epoch in half-years from 2000 (29-24), frame number (23-0)
+//   word 2: version (31-29), log2 channels (28-24), frame bytes / 8 (23-0)
+//   word 3: complex (31), bits per sample - 1 (30-26), thread (25-16),
+//           station: two ASCII characters, first in bits 15-8 (15-0)
+//   words 4-7: extended user data (version 0: zero)
+// vdifio itself is not available here; these follow the specification.
+
+// MJD of a Gregorian calendar date (integer day number arithmetic)
+static int64_t vm_mjd_of_date(int64_t y, int64_t m, int64_t d)
+{
+    const int64_t a = (14 - m) / 12, yy = y + 4800 - a, mm = m + 12 * a - 3;
+    const int64_t jdn = d + (153 * mm + 2) / 5 + 365 * yy + yy / 4 - yy / 100 + yy / 400 - 32045;
+    return jdn - 2400001;
+}
+
+// MJD at which VDIF reference epoch e (half-years since 2000-01-01) starts
+static int64_t vm_vdif_epoch_mjd(int e)
+{
+    return vm_mjd_of_date(2000 + e / 2, (e % 2) ? 7 : 1, 1);
+}
+
+static void vm_vdif_create(uint32_t *w, int dataarraylength, int threadid, int bits, int nchan,
+                           int iscomplex, const char *stationid)
+{
+    int lg = 0;
+    while ((1 << lg) < nchan) lg++;
+    memset(w, 0, 32);
+    w[2] = ((uint32_t)(lg & 0x1F) << 24) | (uint32_t)(((dataarraylength + 32) / 8) & 0xFFFFFF);
+    const uint32_t st = ((uint32_t)(uint8_t)stationid[0] << 8) |
+                        (stationid[0] ? (uint32_t)(uint8_t)stationid[1] : 0u);
+    w[3] = ((uint32_t)(iscomplex ? 1 : 0) << 31) | ((uint32_t)((bits - 1) & 0x1F) << 26) |
+           ((uint32_t)(threadid & 0x3FF) << 16) | (st & 0xFFFF);
+}
+
+static void vm_vdif_set_epoch_mjd(uint32_t *w, int64_t mjd)
+{
+    int e = 0;
+    while (e < 63 && vm_vdif_epoch_mjd(e + 1) <= mjd) e++;
+    w[1] = (w[1] & 0x00FFFFFFu) | ((uint32_t)e << 24);
+}
+
+static void vm_vdif_set_mjdsec(uint32_t *w, int64_t mjdsec)
+{
+    const int e = (int)((w[1] >> 24) & 0x3F);
+    const int64_t sec = mjdsec - vm_vdif_epoch_mjd(e) * 86400;
+    w[0] = (w[0] & 0xC0000000u) | (uint32_t)(sec & 0x3FFFFFFF);
+}
+
+// the header after n more frames of a second of frame_rate frames
+static void vm_vdif_advance(uint32_t *w, uint64_t n, uint32_t frame_rate)
+{
+    const uint64_t g = (uint64_t)(w[1] & 0x00FFFFFFu) + n;
+    const uint32_t sec = (w[0] & 0x3FFFFFFFu) + (uint32_t)(g / frame_rate);
+    w[0] = (w[0] & 0xC0000000u) | (sec & 0x3FFFFFFFu);
+    w[1] = (w[1] & 0xFF000000u) | (uint32_t)(g % frame_rate);
+}
+
+static void vdif_frames_launch(const float *d_in, size_t nframes, int payload_bytes,
+                               const uint32_t *hdr, uint32_t frame_rate, uint8_t *d_out,
+                               hipStream_t stream)
+{
+    if (payload_bytes <= 0 || payload_bytes % 4 || frame_rate == 0)
+        VM_FATAL("VDIF frames: payload of %d bytes (a positive multiple of 4) at %u frames/s",
+                 payload_bytes, frame_rate);
+    const size_t words = nframes * (8 + (size_t)payload_bytes / 4);
+    if (!words) return;
+    const uint4 tlo = make_uint4(hdr[0], hdr[1], hdr[2], hdr[3]);
+    const uint4 thi = make_uint4(hdr[4], hdr[5], hdr[6], hdr[7]);
+    hipLaunchKernelGGL(k_vdif_frames, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream,
+                       d_in, nframes, payload_bytes / 4, tlo, thi, hdr[0] & 0x3FFFFFFFu,
+                       hdr[1] & 0x00FFFFFFu, frame_rate, (uint32_t *)d_out);
+    HIPCHECK(hipGetLastError());
 }
 
 // =================================================================== host ==
@@ -313,6 +426,183 @@ void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n, gpuStream_
                        (hipStream_t)stream, d_in, d_out, n);
     HIPCHECK(hipGetLastError());
     vm_timer_stop_on(vm_dev_global(), VM_T_VDIF, stream);
+}
+
+
+// ---------------------------------------------------------- VDIF output --
+
+void vmVDIFFramesDevice(const float *d_in, size_t nframes, int payload_bytes, vdif_header *hdr,
+                        int frame_rate, uint8_t *d_out, gpuStream_t stream)
+{
+    if (!hdr || frame_rate <= 0) VM_FATAL("vmVDIFFramesDevice: no header or frame rate %d", frame_rate);
+    vm_timer_start_on(vm_dev_global(), VM_T_VDIF, stream);
+    vdif_frames_launch(d_in, nframes, payload_bytes, hdr->w, (uint32_t)frame_rate, d_out,
+                       (hipStream_t)stream);
+    vm_timer_stop_on(vm_dev_global(), VM_T_VDIF, stream);
+    vm_vdif_advance(hdr->w, nframes, (uint32_t)frame_rate);
+}
+
+// sexagesimal string of `in` (hours or degrees): [sign]HH:MM:SS.SS, with a
+// '+' for non-negative values when sflag (src/geometry.c:301-342)
+static void vm_sexagesimal(char *out, double in, int sflag)
+{
+    const int neg = in < 0.0;
+    if (neg) in = -in;
+    int h = (int)in;
+    in = (in - h) * 60.0;
+    int m = (int)in;
+    double sec = (in - m) * 60.0;
+    if (m >= 60) { h += 1; m -= 60; }
+    if (sec >= 59.995) { m += 1; sec = 0.0; }
+    sprintf(out, "%s%2.2d:%2.2d:%05.2f", neg ? "-" : (sflag ? "+" : ""), h, m, sec);
+}
+
+void vmPopulateVDIFHeader(vcsbeam_context *vm, beam_geom *beam_geom_vals, double mjd_start,
+                          double sec_offset)
+{
+    // src/beam_vdif.c:150-246
+    if (!vm->obs_metadata || !vm->coarse_chan_idxs_to_process)
+        VM_FATAL("vmPopulateVDIFHeader: no observation bound to the context");
+    const CoarseChannel *cc = &vm->obs_metadata->metafits_coarse_chans[vm->coarse_chan_idxs_to_process[0]];
+    if (vm->vf) {
+        for (unsigned p = 0; p < vm->npointing; p++) {
+            free(vm->vf[p].b_scales);
+            free(vm->vf[p].b_offsets);
+        }
+        free(vm->vf);
+    }
+    vm->vf = (struct vdifinfo *)calloc(vm->npointing, sizeof(struct vdifinfo));
+    if (!vm->vf) VM_FATAL("out of host memory");
+    // UTC of the scheduled start (MJD 40587 = 1970-01-01)
+    char time_utc[24];
+    {
+        const double days = vm->obs_metadata->sched_start_mjd - 40587.0;
+        const time_t t = (time_t)llround(days * 86400.0);
+        struct tm ts;
+        gmtime_r(&t, &ts);
+        strftime(time_utc, sizeof(time_utc), "%Y-%m-%dT%H:%M:%S", &ts);
+    }
+    for (unsigned p = 0; p < vm->npointing; p++) {
+        struct vdifinfo *v = &vm->vf[p];
+        v->bits = 8;
+        v->iscomplex = 1;
+        v->nchan = 2; // one channel per polarisation
+        v->samples_per_frame = 128;
+        v->sample_rate = vm->fine_sample_rate * vm->nfine_chan;
+        v->BW = vm->obs_metadata->coarse_chan_width_hz / 1e6;
+        v->dataarraylength = (int)(v->nchan * (v->iscomplex + 1) * v->samples_per_frame);
+        v->frame_length = v->dataarraylength + VDIF_HEADER_BYTES;
+        v->threadid = 0;
+        strcpy(v->stationid, "mw");
+        v->frame_rate = vm->fine_sample_rate;
+        v->block_size = (size_t)v->frame_length * v->frame_rate;
+        v->sizeof_beam = v->samples_per_frame * v->nchan * (v->iscomplex + 1);
+        v->sizeof_buffer = (size_t)v->frame_rate * v->sizeof_beam;
+
+        vm_vdif_create(vm->vhdr.w, v->dataarraylength, v->threadid, v->bits, v->nchan,
+                       v->iscomplex, v->stationid);
+        const uint64_t start_day = (uint64_t)beam_geom_vals->intmjd;
+        const uint64_t start_sec = (uint64_t)roundf((float)(beam_geom_vals->fracmjd * 86400.0));
+        vm_vdif_set_epoch_mjd(vm->vhdr.w, (int64_t)start_day);
+        vm_vdif_set_mjdsec(vm->vhdr.w, (int64_t)(start_day * 86400 + start_sec));
+        vm->vhdr.w[1] &= 0xFF000000u; // frame 0
+
+        snprintf(v->exp_name, sizeof(v->exp_name), "%s",
+                 vm->obs_metadata->project_id ? vm->obs_metadata->project_id : "");
+        snprintf(v->scan_name, 17, "%d", (int)vm->obs_metadata->obs_id);
+        v->b_scales = (float *)calloc(v->nchan, sizeof(float));
+        v->b_offsets = (float *)calloc(v->nchan, sizeof(float));
+        v->got_scales = 1;
+        strncpy(v->telescope, "MWA", 24);
+        strncpy(v->obs_mode, "PSR", 8);
+        vm_sexagesimal(v->ra_str, beam_geom_vals[p].mean_ra * R2D / 15.0, 0);
+        vm_sexagesimal(v->dec_str, beam_geom_vals[p].mean_dec * R2D, 1);
+        strncpy(v->date_obs, time_utc, sizeof(v->date_obs));
+        v->MJD_start = mjd_start;
+        v->sec_offset = sec_offset;
+        v->MJD_epoch = beam_geom_vals->intmjd + beam_geom_vals->fracmjd;
+        v->fctr = cc->chan_centre_hz / 1e6;
+        v->npol = 2;
+        strncpy(v->source, "unset", 24);
+        snprintf(v->basefilename, sizeof(v->basefilename), "%s_%s_%s_%s_ch%03ld", v->exp_name,
+                 v->scan_name, v->ra_str, v->dec_str, (long)cc->rec_chan_number);
+    }
+}
+
+void vdif_write_data(struct vdifinfo *vf, int8_t *output)
+{
+    // src/beam_vdif.c:98-143: append the frames; write the DSPSR ASCII
+    // header (4096 bytes of "KEY value" lines, NUL-padded)
+    char fname[1100];
+    snprintf(fname, sizeof(fname), "%s.vdif", vf->basefilename);
+    FILE *fs = fopen(fname, "a");
+    if (!fs) VM_FATAL("vdif_write_data: cannot open %s", fname);
+    if (fwrite(output, vf->block_size, 1, fs) != 1) VM_FATAL("vdif_write_data: short write to %s", fname);
+    fclose(fs);
+    char hdr[4096];
+    memset(hdr, 0, sizeof(hdr));
+    int n = 0;
+#define VM_KV(k, fmt, ...)                                                                   \
+    n += snprintf(hdr + n, sizeof(hdr) - (size_t)n, "%-12s " fmt "\n", k, __VA_ARGS__)
+    VM_KV("HDR_VERSION", "%s", "0.2");
+    VM_KV("MWA_CAPTURE_VERSION", "%s", "0.1");
+    VM_KV("MWA_SAMPLE_VERSION", "%s", "0.1");
+    VM_KV("VCSBEAM_VERSION", "%s", vmLibraryVersion());
+    VM_KV("TELESCOPE", "%s", vf->telescope);
+    VM_KV("MODE", "%s", vf->obs_mode);
+    VM_KV("INSTRUMENT", "%s", "VDIF");
+    VM_KV("DATAFILE", "%s", fname);
+    VM_KV("MJD_START", "%f", vf->MJD_start);
+    VM_KV("MJD_EPOCH", "%f", vf->MJD_epoch);
+    VM_KV("SEC_OFFSET", "%f", vf->sec_offset);
+    VM_KV("SOURCE", "%s", vf->source);
+    VM_KV("RA", "%s", vf->ra_str);
+    VM_KV("DEC", "%s", vf->dec_str);
+    VM_KV("FREQ", "%f", vf->fctr);
+    VM_KV("BW", "%f", vf->BW);
+    VM_KV("TSAMP", "%f", 1e6 / (float)vf->sample_rate);
+    VM_KV("NBIT", "%d", vf->bits);
+    VM_KV("NDIM", "%d", vf->iscomplex + 1);
+    VM_KV("NPOL", "%d", vf->npol);
+#undef VM_KV
+    if (n >= (int)sizeof(hdr)) VM_FATAL("vdif_write_data: header exceeds 4096 bytes");
+    snprintf(fname, sizeof(fname), "%s.hdr", vf->basefilename);
+    fs = fopen(fname, "w");
+    if (!fs) VM_FATAL("vdif_write_data: cannot open %s", fname);
+    if (fwrite(hdr, sizeof(hdr), 1, fs) != 1) VM_FATAL("vdif_write_data: short write to %s", fname);
+    fclose(fs);
+}
+
+void vdif_write_second(struct vdifinfo *vf, vdif_header *vhdr, float *data_buffer_vdif)
+{
+    // src/beam_vdif.c:52-88, the framing on the GPU.  The reference clamps
+    // data_buffer_vdif in place as a side effect; here it is left unchanged.
+    if (vf->frame_length != vf->dataarraylength + VDIF_HEADER_BYTES ||
+        (size_t)vf->frame_length * vf->frame_rate != vf->block_size ||
+        (size_t)vf->dataarraylength != vf->sizeof_beam)
+        VM_FATAL("vdif_write_second: inconsistent frame description");
+    static float *d_in = NULL;
+    static uint8_t *d_out = NULL;
+    static size_t in_cap = 0, out_cap = 0;
+    const size_t in_bytes = vf->sizeof_buffer * sizeof(float), out_bytes = vf->block_size;
+    if (in_cap < in_bytes) {
+        if (d_in) HIPCHECK(hipFree(d_in));
+        HIPCHECK(hipMalloc((void **)&d_in, in_bytes));
+        in_cap = in_bytes;
+    }
+    if (out_cap < out_bytes) {
+        if (d_out) HIPCHECK(hipFree(d_out));
+        HIPCHECK(hipMalloc((void **)&d_out, out_bytes));
+        out_cap = out_bytes;
+    }
+    int8_t *frames = (int8_t *)malloc(out_bytes);
+    if (!frames) VM_FATAL("out of host memory");
+    HIPCHECK(hipMemcpy(d_in, data_buffer_vdif, in_bytes, hipMemcpyHostToDevice));
+    vmVDIFFramesDevice(d_in, (size_t)vf->frame_rate, vf->dataarraylength, vhdr, vf->frame_rate,
+                       d_out, NULL);
+    HIPCHECK(hipMemcpy(frames, d_out, out_bytes, hipMemcpyDeviceToHost));
+    vdif_write_data(vf, frames);
+    free(frames);
 }
 
 } // extern "C"

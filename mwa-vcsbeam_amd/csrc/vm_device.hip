@@ -481,6 +481,13 @@ void destroy_vcsbeam_context(vcsbeam_context *vm)
     free(vm->gps_seconds_to_process);
     free(vm->coarse_chan_idxs_to_process);
     free(vm->cal_coarse_chan_idxs_to_process);
+    if (vm->vf) {
+        for (unsigned p = 0; p < vm->npointing; p++) {
+            free(vm->vf[p].b_scales);
+            free(vm->vf[p].b_offsets);
+        }
+        free(vm->vf);
+    }
     vm_dev_free(vm);
     destroy_logger(vm->log);
     free(vm);

@@ -746,6 +746,18 @@ int vmStructLayout(const char *name, size_t *out, int n)
         PUT(sizeof(struct gpu_ipfb_arrays));
         PUT(OFF(struct gpu_ipfb_arrays, in_real));
         PUT(OFF(struct gpu_ipfb_arrays, d_out));
+    } else if (strcmp(name, "vdifinfo") == 0) {
+        PUT(sizeof(struct vdifinfo));
+        PUT(OFF(struct vdifinfo, samples_per_frame));
+        PUT(OFF(struct vdifinfo, stationid));
+        PUT(OFF(struct vdifinfo, block_size));
+        PUT(OFF(struct vdifinfo, b_scales));
+        PUT(OFF(struct vdifinfo, fctr));
+        PUT(OFF(struct vdifinfo, ra_str));
+        PUT(OFF(struct vdifinfo, BW));
+        PUT(OFF(struct vdifinfo, date_obs));
+        PUT(OFF(struct vdifinfo, basefilename));
+        PUT(OFF(struct vdifinfo, got_scales));
     } else {
         return -1;
     }

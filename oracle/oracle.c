@@ -467,6 +467,68 @@ void orc_vdif_requant(const float *f, size_t n, uint8_t *out)
     }
 }
 
+/* VDIF 1.0 frame headers and frames, as src/beam_vdif.c:52-88 and 196-205
+ * produce them through vdifio (createVDIFHeader, setVDIFEpochMJD,
+ * setVDIFFrameMJDSec, nextVDIFHeader).  vdifio is not part of the reference
+ * tree; the header follows the VDIF 1.0 specification (vlbi.org, 2009):
+ *   word 0: invalid (bit 31), legacy (30), seconds since the epoch (29-0)
+ *   word 1: epoch as half-years since 2000-01-01 (29-24), frame (23-0)
+ *   word 2: version (31-29), log2 channels (28-24), frame bytes / 8 (23-0)
+ *   word 3: complex (31), bits - 1 (30-26), thread (25-16), station (15-0,
+ *           two ASCII characters, the first in the high byte)
+ * Dates: days since 1970-01-01 by the era decomposition of the proleptic
+ * Gregorian calendar (H. Hinnant, "chrono-compatible low-level date
+ * algorithms"), MJD = days + 40587. */
+static long orc_days_from_civil(long y, long m, long d)
+{
+    y -= m <= 2;
+    const long era = (y >= 0 ? y : y - 399) / 400;
+    const long yoe = y - era * 400;
+    const long doy = (153 * (m + (m > 2 ? -3 : 9)) + 2) / 5 + d - 1;
+    const long doe = yoe * 365 + yoe / 4 - yoe / 100 + doy;
+    return era * 146097 + doe - 719468;
+}
+
+long orc_vdif_epoch_mjd(int e)
+{
+    return orc_days_from_civil(2000 + e / 2, (e & 1) ? 7 : 1, 1) + 40587;
+}
+
+void orc_vdif_header(int payload_bytes, int threadid, int bits, int nchan, int iscomplex,
+                     const char *station, long mjd, long sec_of_day, uint32_t *w)
+{
+    int e = 0;
+    while (e < 63 && orc_vdif_epoch_mjd(e + 1) <= mjd) e++;
+    const long secs = (mjd - orc_vdif_epoch_mjd(e)) * 86400 + sec_of_day;
+    int lg = 0;
+    while ((1 << lg) < nchan) lg++;
+    memset(w, 0, 8 * sizeof(uint32_t));
+    w[0] = (uint32_t)secs & 0x3FFFFFFFu;
+    w[1] = (uint32_t)e << 24;
+    w[2] = ((uint32_t)lg << 24) | (uint32_t)((payload_bytes + 32) / 8);
+    w[3] = ((uint32_t)(iscomplex != 0) << 31) | ((uint32_t)(bits - 1) << 26) |
+           ((uint32_t)threadid << 16) | ((uint32_t)(uint8_t)station[0] << 8) |
+           (uint32_t)(uint8_t)station[1];
+}
+
+void orc_vdif_frames(const float *in, size_t nframes, int payload_bytes, uint32_t *hdr,
+                     int frame_rate, uint8_t *out)
+{
+    /* one frame at a time: header copy, requantised payload, next header */
+    for (size_t f = 0; f < nframes; f++) {
+        uint8_t *o = out + f * (size_t)(32 + payload_bytes);
+        memcpy(o, hdr, 32);
+        orc_vdif_requant(in + f * (size_t)payload_bytes, (size_t)payload_bytes, o + 32);
+        uint32_t frame = (hdr[1] & 0x00FFFFFFu) + 1, sec = hdr[0] & 0x3FFFFFFFu;
+        if ((int)frame >= frame_rate) {
+            frame = 0;
+            sec++;
+        }
+        hdr[0] = (hdr[0] & 0xC0000000u) | (sec & 0x3FFFFFFFu);
+        hdr[1] = (hdr[1] & 0xFF000000u) | frame;
+    }
+}
+
 /* ------------------------------------------------ primary beam (a19) --- */
 #define ORC_C 299792458.0
 #define ORC_DIP_SPACING 1.1

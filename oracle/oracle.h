@@ -133,6 +133,16 @@ void orc_ipfb_gather(const double *ring, int file_no, int nsamples, int K,
  * (src/beam_vdif.c:29-38, 254-260): clamp to [-126,127], rint, ^0x80. */
 void orc_vdif_requant(const float *f, size_t n, uint8_t *out);
 
+/* a20: VDIF 1.0 frame header of the VDIF writer (src/beam_vdif.c:196-205
+ * through vdifio, restated from the VDIF 1.0 specification) and whole frames
+ * (src/beam_vdif.c:52-88: header, requantised payload, next header; hdr is
+ * advanced in place, the frame counter wrapping at frame_rate). */
+long orc_vdif_epoch_mjd(int e);
+void orc_vdif_header(int payload_bytes, int threadid, int bits, int nchan, int iscomplex,
+                     const char *station, long mjd, long sec_of_day, uint32_t *w);
+void orc_vdif_frames(const float *in, size_t nframes, int payload_bytes, uint32_t *hdr,
+                     int frame_rate, uint8_t *out);
+
 /* a19: project-defined analytic dipole-array primary beam (the reference
  * has no analytic beam; its FEE model, hyperbeam + HDF5, is not available,
  * src/primary_beam.c:479-532 is the call site it stands in for).  Tile: 16

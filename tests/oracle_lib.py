@@ -23,6 +23,7 @@ def lib():
         _lib.orc_array_factor.restype = C.c_double
         _lib.orc_rel_err.restype = C.c_double
         _lib.orc_demote.restype = C.c_uint8
+        _lib.orc_vdif_epoch_mjd.restype = C.c_long
     return _lib
 
 
@@ -169,6 +170,29 @@ def vdif_requant(f):
     out = np.empty(f.shape, np.uint8)
     lib().orc_vdif_requant(_p(f), C.c_size_t(f.size), _p(out))
     return out
+
+
+def vdif_epoch_mjd(e):
+    return int(lib().orc_vdif_epoch_mjd(C.c_int(e)))
+
+
+def vdif_header(payload_bytes=512, threadid=0, bits=8, nchan=2, iscomplex=1, station=b"mw",
+                mjd=58849, sec_of_day=0):
+    """The eight 32-bit words of a VDIF 1.0 header at frame 0 of that second."""
+    w = np.zeros(8, np.uint32)
+    lib().orc_vdif_header(payload_bytes, threadid, bits, nchan, iscomplex, C.c_char_p(station),
+                          C.c_long(mjd), C.c_long(sec_of_day), _p(w))
+    return w
+
+
+def vdif_frames(f, nframes, payload_bytes, hdr, frame_rate):
+    """Frames (uint8, nframes x (32 + payload_bytes)) and the header after them."""
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    assert f.size >= nframes * payload_bytes
+    h = np.array(hdr, np.uint32)
+    out = np.empty(nframes * (32 + payload_bytes), np.uint8)
+    lib().orc_vdif_frames(_p(f), C.c_size_t(nframes), payload_bytes, _p(h), frame_rate, _p(out))
+    return out, h
 
 
 def rel_err(x, ref):

@@ -7,7 +7,13 @@ cu_invert_pfb) and through the device-resident path.  The oracle inverts the
 same beamformed voltages in FP64 with the reference's direct form; outputs the
 reference leaves undefined (n < P*K with n % K != 0) are masked.  Bars:
 <= 1e-5 norm-wise relative error; VDIF bytes within +-1 LSB of the oracle's.
+
+VDIF framing (src/beam_vdif.c:52-88): the device writes whole frames, header
+words and requantised payload, bit for bit equal to the oracle's frame loop
+across second boundaries, and vdif_write_second's file holds those frames.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -56,3 +62,60 @@ def test_vdif_requant_device(pkg, orc, gpu_lib):
     y = x + rng.uniform(-1e-4, 1e-4, x.size).astype(np.float32)
     d = np.abs(got.astype(int) - orc.vdif_requant(y).astype(int))
     assert d.max() <= 1
+
+
+def _hdr(orc, frame):
+    h = orc.vdif_header(payload_bytes=512, mjd=59000, sec_of_day=3723)
+    h[1] |= frame
+    return h
+
+
+def test_vdif_frames_device(pkg, orc, gpu_lib):
+    """10 005 frames of 512 bytes at 10 000 frames/s from frame 9997: the
+    frame counter wraps twice; a second call continues from the advanced
+    header."""
+    rng = np.random.default_rng(5)
+    rate, pb, n1, n2 = 10000, 512, 10005, 3
+    x = (rng.standard_normal((n1 + n2) * pb) * 50).astype(np.float32)
+    x[:64] = np.arange(-140, 140, 280 / 64, dtype=np.float32)
+    hdr = _hdr(orc, rate - 3)
+    want1, h1 = orc.vdif_frames(x[:n1 * pb], n1, pb, hdr, rate)
+    want2, h2 = orc.vdif_frames(x[n1 * pb:], n2, pb, h1, rate)
+    d_in = pkg.DeviceArray.from_numpy(x)
+    d_out = pkg.DeviceArray((n1 + n2) * (32 + pb))
+    vh = (C.c_uint32 * 8)(*[int(w) for w in hdr])
+    print(f"VDIF frames: {n1} + {n2} frames from frame {rate - 3}", flush=True)
+    gpu_lib.vmVDIFFramesDevice(d_in.ptr, n1, pb, C.byref(vh), rate, d_out.ptr, None)
+    np.testing.assert_array_equal(np.array(vh[:], np.uint32), h1)
+    gpu_lib.vmVDIFFramesDevice(d_in.ptr + n1 * pb * 4, n2, pb, C.byref(vh), rate,
+                               d_out.ptr + n1 * (32 + pb), None)
+    np.testing.assert_array_equal(np.array(vh[:], np.uint32), h2)
+    gpu_lib.vmDeviceSynchronize()
+    got = d_out.to_numpy(np.uint8, (n1 + n2) * (32 + pb))
+    np.testing.assert_array_equal(got[:n1 * (32 + pb)], want1)
+    np.testing.assert_array_equal(got[n1 * (32 + pb):], want2)
+
+
+def test_vdif_write_second_file(pkg, orc, gpu_lib, tmp_path):
+    """Two seconds through vdif_write_second: the .vdif file holds the
+    oracle's frames, the header advances by one second per call."""
+    rng = np.random.default_rng(6)
+    rate, pb = 10000, 512
+    f = pkg.Vdifinfo()
+    f.frame_length, f.dataarraylength, f.frame_rate = 32 + pb, pb, rate
+    f.block_size, f.sizeof_beam, f.sizeof_buffer = (32 + pb) * rate, pb, pb * rate
+    f.bits, f.iscomplex, f.npol, f.sample_rate = 8, 1, 2, 1280000
+    f.basefilename = str(tmp_path / "beam").encode()
+    hdr = _hdr(orc, 0)
+    vh = (C.c_uint32 * 8)(*[int(w) for w in hdr])
+    want = []
+    for t in range(2):
+        x = (rng.standard_normal(pb * rate) * 40).astype(np.float32)
+        w, hdr = orc.vdif_frames(x, rate, pb, hdr, rate)
+        want.append(w)
+        print(f"vdif_write_second: second {t}", flush=True)
+        gpu_lib.vdif_write_second(C.byref(f), C.byref(vh), x.ctypes.data)
+        np.testing.assert_array_equal(np.array(vh[:], np.uint32), hdr)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "beam.vdif", np.uint8),
+                                  np.concatenate(want))
+    assert (tmp_path / "beam.hdr").stat().st_size == 4096

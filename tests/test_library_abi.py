@@ -36,7 +36,7 @@ def test_header_symbols_exported(pkg):
 
 
 @pytest.mark.parametrize("name", ["vcsbeam_context", "forward_pfb", "MetafitsMetadata",
-                                  "Rfinput", "gpu_ipfb_arrays"])
+                                  "Rfinput", "gpu_ipfb_arrays", "vdifinfo"])
 def test_struct_layout_matches(pkg, name):
     c_side, py_side = pkg.struct_layout(name)
     assert c_side == py_side
@@ -97,3 +97,85 @@ def test_read_buffer_margin_and_lock(pkg):
     hb.locked = False
     assert lib.vmReadNextSecondFrom(vm, s1.ctypes.data, s1.nbytes) == pkg.VM_END_OF_DATA
     lib.destroy_vcsbeam_context(vm)
+
+
+def _vdif_context(pkg, lib, obs, bg):
+    """A context bound by hand to obs's metadata (pure host state): the
+    fields vmPopulateVDIFHeader reads."""
+    vm = lib.vmInit(False)
+    v = vm.contents
+    v.obs_metadata = C.pointer(obs.md)
+    idx = (C.c_int * 1)(0)
+    v.coarse_chan_idxs_to_process = C.cast(idx, C.c_void_p)
+    v.npointing = len(bg)
+    v.fine_sample_rate = 10000
+    v.nfine_chan = 128
+    return vm, idx
+
+
+def test_populate_vdif_header(pkg, orc):
+    """vmPopulateVDIFHeader (src/beam_vdif.c:150-246): the per-pointing
+    frame description, file name and the context's frame-0 header."""
+    from vcsbeam_amd.workloads import Observation
+    lib = pkg.load()
+    obs = Observation(ntiles=4, blocks_per_second=1, rec_chan=109)
+    bg = (pkg.BeamGeom * 2)()
+    # RA 05:34:31.94, Dec +22:00:52.2 and RA 0:00:00.00, Dec -0:30:00.00
+    bg[0].mean_ra = np.deg2rad(15 * (5 + 34 / 60 + 31.94 / 3600))
+    bg[0].mean_dec = np.deg2rad(22 + 0 / 60 + 52.2 / 3600)
+    bg[1].mean_ra = 0.0
+    bg[1].mean_dec = np.deg2rad(-0.5)
+    bg[0].intmjd, bg[0].fracmjd = 59000.0, 3723.2 / 86400
+    vm, idx = _vdif_context(pkg, lib, obs, bg)
+    try:
+        lib.vmPopulateVDIFHeader(vm, bg, 59000.25, 1.0)
+        v = vm.contents
+        vf = C.cast(v.vf, C.POINTER(pkg.Vdifinfo))
+        f = vf[0]
+        assert (f.bits, f.iscomplex, f.nchan, f.samples_per_frame) == (8, 1, 2, 128)
+        assert (f.dataarraylength, f.frame_length, f.frame_rate) == (512, 544, 10000)
+        assert (f.block_size, f.sizeof_beam, f.sizeof_buffer) == (5440000, 512, 5120000)
+        assert f.sample_rate == 1280000 and f.BW == 1.28 and f.npol == 2
+        assert f.fctr == 109 * 1.28 and f.MJD_start == 59000.25 and f.sec_offset == 1.0
+        assert abs(f.MJD_epoch - (59000 + 3723.2 / 86400)) < 1e-12
+        assert f.stationid == b"mw" and f.telescope == b"MWA" and f.obs_mode == b"PSR"
+        assert f.source == b"unset" and f.scan_name == str(obs.md.obs_id).encode()
+        assert f.ra_str == b"05:34:31.94" and f.dec_str == b"+22:00:52.20"
+        assert vf[1].ra_str == b"00:00:00.00" and vf[1].dec_str == b"-00:30:00.00"
+        assert f.date_obs == b"2020-05-31T12:00:00"  # MJD 59000.5
+        proj = C.cast(obs.md.project_id, C.c_char_p).value
+        assert f.basefilename == proj + b"_" + f.scan_name + b"_05:34:31.94_+22:00:52.20_ch109"
+        want = orc.vdif_header(mjd=59000, sec_of_day=3723)
+        np.testing.assert_array_equal(np.array(v.vhdr[:], np.uint32), want)
+    finally:
+        vm.contents.coarse_chan_idxs_to_process = None
+        vm.contents.obs_metadata = None
+        lib.destroy_vcsbeam_context(vm)
+
+
+def test_vdif_write_data_files(pkg, tmp_path):
+    """vdif_write_data appends one second of frames to <base>.vdif and
+    rewrites <base>.hdr, a 4096-byte "KEY value" header."""
+    lib = pkg.load()
+    f = pkg.Vdifinfo()
+    f.frame_length, f.frame_rate, f.block_size = 40, 3, 120
+    f.bits, f.iscomplex, f.npol, f.sample_rate = 8, 1, 2, 1280000
+    f.telescope, f.obs_mode, f.source = b"MWA", b"PSR", b"unset"
+    f.ra_str, f.dec_str = b"05:34:31.94", b"+22:00:52.20"
+    f.MJD_start, f.MJD_epoch, f.sec_offset, f.fctr, f.BW = 59000.25, 59000.5, 2.0, 139.52, 1.28
+    f.basefilename = str(tmp_path / "beam").encode()
+    a = np.arange(120, dtype=np.uint8)
+    lib.vdif_write_data(C.byref(f), a.ctypes.data)
+    lib.vdif_write_data(C.byref(f), (a[::-1].copy()).ctypes.data)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "beam.vdif", np.uint8),
+                                  np.concatenate([a, a[::-1]]))
+    raw = (tmp_path / "beam.hdr").read_bytes()
+    assert len(raw) == 4096
+    kv = dict(line.split(None, 1) for line in raw.rstrip(b"\0").decode().splitlines() if line)
+    assert kv["INSTRUMENT"] == "VDIF" and kv["DATAFILE"] == str(tmp_path / "beam.vdif")
+    assert kv["TELESCOPE"] == "MWA" and kv["MODE"] == "PSR" and kv["SOURCE"] == "unset"
+    assert kv["RA"] == "05:34:31.94" and kv["DEC"] == "+22:00:52.20"
+    assert float(kv["MJD_START"]) == 59000.25 and float(kv["MJD_EPOCH"]) == 59000.5
+    assert float(kv["SEC_OFFSET"]) == 2.0 and float(kv["FREQ"]) == 139.52
+    assert float(kv["BW"]) == 1.28 and abs(float(kv["TSAMP"]) - 1 / 1.28) < 1e-6
+    assert (kv["NBIT"], kv["NDIM"], kv["NPOL"]) == ("8", "2", "2")

@@ -336,3 +336,60 @@ def test_vdif_requant_grid(orc):
     got = orc.vdif_requant(x)
     want = (np.rint(np.clip(x, -126, 127)).astype(np.int8).view(np.uint8) ^ 0x80)
     np.testing.assert_array_equal(got, want)
+
+
+def test_vdif_epoch_mjd_against_calendar(orc):
+    """VDIF reference epochs start on 1 January and 1 July of 2000 + e/2."""
+    import datetime
+    mjd0 = datetime.date(1858, 11, 17)
+    for e in range(64):
+        d = datetime.date(2000 + e // 2, 7 if e % 2 else 1, 1)
+        assert orc.vdif_epoch_mjd(e) == (d - mjd0).days
+
+
+def _vdif_fields(w):
+    """Decode a VDIF 1.0 header from its 32 bytes as a reader does."""
+    b = np.asarray(w, np.uint32).astype("<u4").tobytes()
+    import struct
+    w0, w1, w2, w3 = struct.unpack("<4I", b[:16])
+    return dict(invalid=w0 >> 31, legacy=(w0 >> 30) & 1, seconds=w0 & (2 ** 30 - 1),
+                epoch=(w1 >> 24) & 63, frame=w1 & (2 ** 24 - 1), version=w2 >> 29,
+                log2nchan=(w2 >> 24) & 31, length8=w2 & (2 ** 24 - 1), complex=w3 >> 31,
+                bitsm1=(w3 >> 26) & 31, thread=(w3 >> 16) & 1023,
+                station=bytes([b[13], b[12]]), edv=b[16:32])
+
+
+def test_vdif_header_known_answers(orc):
+    """The MWA beam's header (8-bit complex, 2 channels, 512-byte payload,
+    station "mw") at MJD 59000 (2020-05-31) + 3723 s: epoch 40 starts on MJD
+    58849 (2020-01-01), 151 days earlier."""
+    h = _vdif_fields(orc.vdif_header(mjd=59000, sec_of_day=3723))
+    assert h == dict(invalid=0, legacy=0, seconds=151 * 86400 + 3723, epoch=40, frame=0,
+                     version=0, log2nchan=1, length8=544 // 8, complex=1, bitsm1=7, thread=0,
+                     station=b"mw", edv=bytes(16))
+    # the first day of an epoch and the day before it
+    assert _vdif_fields(orc.vdif_header(mjd=58849))["epoch"] == 40
+    h = _vdif_fields(orc.vdif_header(mjd=58848, sec_of_day=86399))
+    assert h["epoch"] == 39 and h["seconds"] == (58848 - 58665) * 86400 + 86399
+    h = _vdif_fields(orc.vdif_header(payload_bytes=8000, threadid=5, bits=2, nchan=16,
+                                     iscomplex=0, station=b"Ab"))
+    assert (h["length8"], h["thread"], h["bitsm1"], h["log2nchan"], h["complex"],
+            h["station"]) == (1004, 5, 1, 4, 0, b"Ab")
+
+
+def test_vdif_frames_counter_wraps_and_payload(orc):
+    rng = np.random.default_rng(21)
+    pb, rate, n = 64, 7, 5
+    x = (rng.standard_normal(n * pb) * 60).astype(np.float32)
+    hdr = orc.vdif_header(payload_bytes=pb, mjd=59000, sec_of_day=10)
+    hdr[1] |= rate - 2
+    out, after = orc.vdif_frames(x, n, pb, hdr, rate)
+    fr = out.reshape(n, 32 + pb)
+    got = [_vdif_fields(fr[f, :32].view(np.uint32)) for f in range(n)]
+    s0 = 151 * 86400 + 10
+    assert [g["frame"] for g in got] == [5, 6, 0, 1, 2]
+    assert [g["seconds"] for g in got] == [s0, s0, s0 + 1, s0 + 1, s0 + 1]
+    for f in range(n):
+        np.testing.assert_array_equal(fr[f, 8:32], hdr.view(np.uint8)[8:32])
+        np.testing.assert_array_equal(fr[f, 32:], orc.vdif_requant(x[f * pb:(f + 1) * pb]))
+    assert _vdif_fields(after)["frame"] == 3 and _vdif_fields(after)["seconds"] == s0 + 1
