@@ -107,6 +107,8 @@ typedef struct MetafitsMetadata {
     double ra_tile_pointing_deg;
     double dec_tile_pointing_deg;
     char *project_id;
+    size_t num_corr_fine_chans_per_coarse; /* calibration solutions per coarse channel */
+    uint32_t corr_fine_chan_width_hz;
 } MetafitsMetadata;
 
 typedef struct VoltageMetadata {
@@ -154,8 +156,16 @@ void logger_stopwatch_report_stats(logger *log, const char *stopwatch_name);
 void logger_report_all_stats(logger *log);
 
 /* ------------------------------------------------------------ calibration */
-/* include/vcsbeam.h.in:185-202 (loaders are outside the path; D is supplied
- * directly in vm->D) */
+/* include/vcsbeam.h.in:169-202.  D reaches vm->D either from the loaders
+ * below (Offringa binary, RTS text) or directly from the caller. */
+#define CAL_NONE 0
+#define CAL_RTS 1
+#define CAL_OFFRINGA 2
+#define OFFRINGA_HEADER_SIZE_BYTES 48
+#define JONES_SIZE_BYTES 64 /* 4 complex doubles */
+#define NDBL_PER_JONES 8
+#define BANDPASS_ROWS_PER_ANT 8
+#define CAL_BUFSIZE 4096
 typedef struct calibration_t {
     char *metafits;
     char *caldir;
@@ -170,6 +180,19 @@ typedef struct calibration_t {
     int nflags;
     char **flagged_tilenames;
 } calibration;
+
+/* include/vcsbeam.h.in:205-217 */
+void read_dijones_file(gpuDoubleComplex **Dd, gpuDoubleComplex *A, double *amp, uintptr_t nant,
+                       char *fname);
+void read_bandpass_file(gpuDoubleComplex ***Jm, gpuDoubleComplex ***Jf,
+                        MetafitsMetadata *cal_metadata, char *filename);
+void remove_reference_phase(gpuDoubleComplex *J, gpuDoubleComplex *Jref);
+void zero_PQ_and_QP(gpuDoubleComplex *J);
+/* reads <VCSBEAM_RUNTIME_DIR>/pq_phase_correction.txt */
+void parse_calibration_correction_file(uint32_t gpstime, calibration *cal);
+bool tilename_is_flagged(char *tilename, calibration *cal);
+void init_calibration(calibration *cal);
+void free_calibration(calibration *cal);
 
 /* ------------------------------------------------------------------ jones */
 #define NSTOKES 4
@@ -607,6 +630,15 @@ void vmSetPolIdxLists(vcsbeam_context *vm);
 void vmCalcJ(vcsbeam_context *vm);
 void vmSetNumPointings(vcsbeam_context *vm, unsigned int npointings);
 void vmSetNumNotFlaggedRFInputs(vcsbeam_context *vm);
+/* include/vcsbeam.h.in:794-809: calibration into vm->D (host, before vmPushD /
+ * vmCalcJ).  The Offringa loader reads vm->cal.caldir as a file, the RTS loader
+ * as the directory of DI_JonesMatrices_node%03d.dat and
+ * BandpassCalibration_node%03d.dat; both use vm->cal_metadata. */
+void vmParseFlaggedTilenamesFile(char *filename, calibration *cal);
+void vmSetCustomTileFlags(vcsbeam_context *vm);
+void vmLoadRTSSolution(vcsbeam_context *vm);
+void vmLoadOffringaSolution(vcsbeam_context *vm);
+void vmApplyCalibrationCorrections(vcsbeam_context *vm);
 
 void vmLoadFilter(vcsbeam_context *vm, char *filtername, filter_type type, int nchans);
 void free_pfb_filter(pfb_filter *filter);

@@ -65,7 +65,8 @@ class MetafitsMetadata(C.Structure):
                 ("metafits_coarse_chans", C.POINTER(CoarseChannel)),
                 ("coarse_chan_width_hz", C.c_uint32), ("num_volt_fine_chans_per_coarse", C.c_size_t),
                 ("ra_tile_pointing_deg", C.c_double), ("dec_tile_pointing_deg", C.c_double),
-                ("project_id", C.c_char_p)]
+                ("project_id", C.c_char_p), ("num_corr_fine_chans_per_coarse", C.c_size_t),
+                ("corr_fine_chan_width_hz", C.c_uint32)]
 
 
 class VoltageMetadata(C.Structure):
@@ -222,7 +223,8 @@ LAYOUT_FIELDS = {
                                  "nspectra", "P", "flags", "d_flt_out", "filter_fits_int16"]),
     "MetafitsMetadata": (MetafitsMetadata, ["sched_start_mjd", "rf_inputs",
                                             "metafits_coarse_chans", "coarse_chan_width_hz",
-                                            "project_id"]),
+                                            "project_id", "num_corr_fine_chans_per_coarse",
+                                            "corr_fine_chan_width_hz"]),
     "Rfinput": (Rfinput, ["pol", "electrical_length_m", "vcs_order", "flagged"]),
     "gpu_ipfb_arrays": (GpuIpfbArrays, ["in_real", "d_out"]),
     "vdifinfo": (Vdifinfo, ["samples_per_frame", "stationid", "block_size", "b_scales", "fctr",
@@ -305,6 +307,17 @@ _SIGS = {
     "vmRequantVDIFDevice": (None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "create_data_buffer_fine": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "prepare_data_buffer_fine": (None, [C.c_void_p, _CTX, C.c_size_t]),
+    "vmLoadOffringaSolution": (None, [_CTX]),
+    "vmLoadRTSSolution": (None, [_CTX]),
+    "vmApplyCalibrationCorrections": (None, [_CTX]),
+    "vmSetCustomTileFlags": (None, [_CTX]),
+    "parse_calibration_correction_file": (None, [C.c_uint32, C.POINTER(Calibration)]),
+    "tilename_is_flagged": (C.c_bool, [C.c_char_p, C.POINTER(Calibration)]),
+    "vmParseFlaggedTilenamesFile": (None, [C.c_char_p, C.POINTER(Calibration)]),
+    "init_calibration": (None, [C.POINTER(Calibration)]),
+    "free_calibration": (None, [C.POINTER(Calibration)]),
+    "remove_reference_phase": (None, [C.c_void_p, C.c_void_p]),
+    "zero_PQ_and_QP": (None, [C.c_void_p]),
     "vmVDIFFramesDevice": (None, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_int,
                                   C.c_void_p, C.c_void_p]),
     "vmPopulateVDIFHeader": (None, [_CTX, C.POINTER(BeamGeom), C.c_double, C.c_double]),

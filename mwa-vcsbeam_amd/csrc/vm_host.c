@@ -736,6 +736,8 @@ int vmStructLayout(const char *name, size_t *out, int n)
         PUT(OFF(MetafitsMetadata, metafits_coarse_chans));
         PUT(OFF(MetafitsMetadata, coarse_chan_width_hz));
         PUT(OFF(MetafitsMetadata, project_id));
+        PUT(OFF(MetafitsMetadata, num_corr_fine_chans_per_coarse));
+        PUT(OFF(MetafitsMetadata, corr_fine_chan_width_hz));
     } else if (strcmp(name, "Rfinput") == 0) {
         PUT(sizeof(Rfinput));
         PUT(OFF(Rfinput, pol));
