@@ -342,6 +342,24 @@ typedef struct beam_geom_t {
     double unit_H;
 } beam_geom;
 
+/* include/vcsbeam.h.in:909-920: pointing geometry (J2000 RA in hours, Dec in
+ * degrees, UTC MJD) and sexagesimal strings */
+void calc_beam_geom(double ras_hours, double decs_degs, double mjd, beam_geom *bg);
+void dec2hms(char *out, double in, int sflag);
+void mjd2lst(double mjd, double *lst);
+double parse_dec(char *dec_ddmmss);
+double parse_ra(char *ra_hhmmss);
+/* No counterpart in the reference (PAL provides these there): the models
+ * calc_beam_geom uses.  Greenwich mean sidereal time (rad) of a UT MJD;
+ * IAU 1980 nutation in longitude and obliquity and the mean obliquity (rad);
+ * J2000 mean place to apparent place (stages: 1 precession, 2 nutation,
+ * 4 aberration, 7 all); hour angle and declination to azimuth (from North
+ * through East) and elevation at latitude phi. */
+double vmGMST(double mjd);
+void vmNutation(double mjd, double *dpsi, double *deps, double *eps0);
+void vmMeanToApparent(double ra, double dec, double mjd, int stages, double *ra_ap, double *dec_ap);
+void vmHaDecToAzEl(double ha, double dec, double phi, double *az, double *el);
+
 /* Minimal stand-in for the PSRFITS splice struct: only the per-subint arrays
  * that vmSendSToFits fills (src/form_beam.cpp:666-671). */
 typedef struct vm_psrfits_sub_t {

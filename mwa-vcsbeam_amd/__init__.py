@@ -307,6 +307,18 @@ _SIGS = {
     "vmRequantVDIFDevice": (None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "create_data_buffer_fine": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "prepare_data_buffer_fine": (None, [C.c_void_p, _CTX, C.c_size_t]),
+    "calc_beam_geom": (None, [C.c_double, C.c_double, C.c_double, C.POINTER(BeamGeom)]),
+    "dec2hms": (None, [C.c_char_p, C.c_double, C.c_int]),
+    "mjd2lst": (None, [C.c_double, C.POINTER(C.c_double)]),
+    "parse_ra": (C.c_double, [C.c_char_p]),
+    "parse_dec": (C.c_double, [C.c_char_p]),
+    "vmGMST": (C.c_double, [C.c_double]),
+    "vmNutation": (None, [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                          C.POINTER(C.c_double)]),
+    "vmMeanToApparent": (None, [C.c_double, C.c_double, C.c_double, C.c_int,
+                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "vmHaDecToAzEl": (None, [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
+                             C.POINTER(C.c_double)]),
     "vmLoadOffringaSolution": (None, [_CTX]),
     "vmLoadRTSSolution": (None, [_CTX]),
     "vmApplyCalibrationCorrections": (None, [_CTX]),

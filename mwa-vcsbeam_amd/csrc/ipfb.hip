@@ -442,21 +442,6 @@ void vmVDIFFramesDevice(const float *d_in, size_t nframes, int payload_bytes, vd
     vm_vdif_advance(hdr->w, nframes, (uint32_t)frame_rate);
 }
 
-// sexagesimal string of `in` (hours or degrees): [sign]HH:MM:SS.SS, with a
-// '+' for non-negative values when sflag (src/geometry.c:301-342)
-static void vm_sexagesimal(char *out, double in, int sflag)
-{
-    const int neg = in < 0.0;
-    if (neg) in = -in;
-    int h = (int)in;
-    in = (in - h) * 60.0;
-    int m = (int)in;
-    double sec = (in - m) * 60.0;
-    if (m >= 60) { h += 1; m -= 60; }
-    if (sec >= 59.995) { m += 1; sec = 0.0; }
-    sprintf(out, "%s%2.2d:%2.2d:%05.2f", neg ? "-" : (sflag ? "+" : ""), h, m, sec);
-}
-
 void vmPopulateVDIFHeader(vcsbeam_context *vm, beam_geom *beam_geom_vals, double mjd_start,
                           double sec_offset)
 {
@@ -515,8 +500,8 @@ void vmPopulateVDIFHeader(vcsbeam_context *vm, beam_geom *beam_geom_vals, double
         v->got_scales = 1;
         strncpy(v->telescope, "MWA", 24);
         strncpy(v->obs_mode, "PSR", 8);
-        vm_sexagesimal(v->ra_str, beam_geom_vals[p].mean_ra * R2D / 15.0, 0);
-        vm_sexagesimal(v->dec_str, beam_geom_vals[p].mean_dec * R2D, 1);
+        dec2hms(v->ra_str, beam_geom_vals[p].mean_ra * R2D / 15.0, 0);
+        dec2hms(v->dec_str, beam_geom_vals[p].mean_dec * R2D, 1);
         strncpy(v->date_obs, time_utc, sizeof(v->date_obs));
         v->MJD_start = mjd_start;
         v->sec_offset = sec_offset;
