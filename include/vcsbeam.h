@@ -452,6 +452,25 @@ typedef struct forward_pfb_t {
     int filter_fits_int16;       /* 1 -> packed dot2 WOLA path */
 } forward_pfb;
 
+/* Metafits subset and voltage files, replacing the mwalib contexts
+ * (src/metadata.c:1297-1424, 930-993).  vmLoadMetafits fills md (and vmd when
+ * not NULL) from a metafits file: primary keys GPSTIME, MJD, PROJECT, MODE,
+ * CHANNELS, BANDWDTH, NCHANS, FINECHAN, RA, DEC, NINPUTS and the TILEDATA
+ * table; rf_inputs come sorted by subfile order (antenna, then X before Y).
+ * Returns 0; malformed files end the process.  vmFreeMetafits releases what
+ * it allocated.  vmReadSecondFromFiles is a vm_read_second_fn over MWAX
+ * subfiles (<obsid>_<subobs>_<rec>.sub) or legacy recombined files
+ * (<obsid>_<gps>_ch<rec>.dat) in dir; user points to a vm_voltage_files. */
+typedef struct vm_voltage_files_t {
+    const char *dir;
+    const MetafitsMetadata *md;
+    const VoltageMetadata *vmd;
+} vm_voltage_files;
+int vmLoadMetafits(const char *path, MetafitsMetadata *md, VoltageMetadata *vmd);
+void vmFreeMetafits(MetafitsMetadata *md);
+int vmReadSecondFromFiles(void *user, uint64_t gps_second, int coarse_chan_idx, void *dst,
+                          size_t nbytes);
+
 /* ---------------------------------------------------------------- context */
 typedef struct vm_device_state vm_device_state; /* library-private */
 

@@ -123,6 +123,12 @@ class Vdifinfo(C.Structure):
                 ("got_scales", C.c_int)]
 
 
+class VoltageFiles(C.Structure):
+    """vm_voltage_files: the reader state of vmReadSecondFromFiles."""
+    _fields_ = [("dir", C.c_char_p), ("md", C.POINTER(MetafitsMetadata)),
+                ("vmd", C.POINTER(VoltageMetadata))]
+
+
 class HostBuffer(C.Structure):
     _fields_ = [("buffer", C.c_void_p), ("buffer_size", C.c_size_t), ("read_ptr", C.c_void_p),
                 ("read_size", C.c_size_t), ("copy_from_ptr", C.c_void_p),
@@ -307,6 +313,10 @@ _SIGS = {
     "vmRequantVDIFDevice": (None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "create_data_buffer_fine": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "prepare_data_buffer_fine": (None, [C.c_void_p, _CTX, C.c_size_t]),
+    "vmLoadMetafits": (C.c_int, [C.c_char_p, C.POINTER(MetafitsMetadata),
+                                 C.POINTER(VoltageMetadata)]),
+    "vmFreeMetafits": (None, [C.POINTER(MetafitsMetadata)]),
+    "vmReadSecondFromFiles": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_size_t]),
     "calc_beam_geom": (None, [C.c_double, C.c_double, C.c_double, C.POINTER(BeamGeom)]),
     "dec2hms": (None, [C.c_char_p, C.c_double, C.c_int]),
     "mjd2lst": (None, [C.c_double, C.POINTER(C.c_double)]),

@@ -139,14 +139,23 @@ def tied_array_beam(obs, az, za, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, nch
     return out
 
 
-def fine_pfb_offline(obs, *, nseconds=1, nchunks=1, pfb_flags=PFB_SMART | PFB_MALLOC_HOST_OUTPUT):
+def fine_pfb_offline(obs, *, nseconds=1, nchunks=1, pfb_flags=PFB_SMART | PFB_MALLOC_HOST_OUTPUT,
+                     voltage_dir=None, first_gps=None):
     """Forward PFB of `nseconds` seconds to the legacy recombined format.
     Returns a list of per-second host outputs (uint8 [s][c][i] for SMART,
-    complex128 [s][c][i] for FULL_PRECISION + host output)."""
+    complex128 [s][c][i] for FULL_PRECISION + host output).  With
+    `voltage_dir`, the seconds from `first_gps` on are read from MWAX
+    subfiles there (vmReadSecondFromFiles) instead of generated."""
+    from . import VoltageFiles
     lib = load()
     vm = lib.vmInit(False)
     v = vm.contents
-    lib.vmBindObsMetadata(vm, C.byref(obs.md), C.byref(obs.vmd), 0, obs.md.obs_id, nseconds)
+    gps0 = obs.md.obs_id if first_gps is None else first_gps
+    lib.vmBindObsMetadata(vm, C.byref(obs.md), C.byref(obs.vmd), 0, gps0, nseconds)
+    if voltage_dir is not None:
+        files = VoltageFiles(str(voltage_dir).encode(), C.pointer(obs.md), C.pointer(obs.vmd))
+        v.read_second = C.cast(lib.vmReadSecondFromFiles, C.c_void_p).value
+        v.read_second_user = C.addressof(files)
     coeffs = np.ascontiguousarray(obs.filter, dtype=np.float64)
     lib.vmSetFilter(vm, coeffs.ctypes.data_as(C.POINTER(C.c_double)), len(coeffs),
                     ANALYSIS_FILTER, 128)
@@ -155,8 +164,12 @@ def fine_pfb_offline(obs, *, nseconds=1, nchunks=1, pfb_flags=PFB_SMART | PFB_MA
     fp = v.fpfb.contents
     outs = []
     for t in range(nseconds):
-        sec = obs.second(t)
-        if lib.vmReadNextSecondFrom(vm, _ptr(sec), sec.nbytes) != VM_SUCCESS:
+        if voltage_dir is not None:
+            err = lib.vmReadNextSecond(vm)
+        else:
+            sec = obs.second(t)
+            err = lib.vmReadNextSecondFrom(vm, _ptr(sec), sec.nbytes)
+        if err != VM_SUCCESS:
             raise RuntimeError("read failed")
         lib.vmExecuteForwardPFB(vm)
         v.v.contents.locked = False
