@@ -360,18 +360,47 @@ void vmNutation(double mjd, double *dpsi, double *deps, double *eps0);
 void vmMeanToApparent(double ra, double dec, double mjd, int stages, double *ra_ap, double *dec_ap);
 void vmHaDecToAzEl(double ha, double dec, double phi, double *az, double *el);
 
-/* Minimal stand-in for the PSRFITS splice struct: only the per-subint arrays
- * that vmSendSToFits fills (src/form_beam.cpp:666-671). */
+/* PSRFITS search-mode output: the header and sub-integration fields that
+ * populate_psrfits_header / populate_spliced_psrfits_header set
+ * (src/beam_psrfits.c:39-466; psrfits_utils' struct psrfits, which is not
+ * part of the reference tree), and this library's own writer state.  The
+ * first four sub fields are the arrays vmSendSToFits fills
+ * (src/form_beam.cpp:666-671). */
 typedef struct vm_psrfits_sub_t {
-    float *dat_offsets;
-    float *dat_scales;
-    uint8_t *data;
+    float *dat_offsets;       /* [npol][nchan] */
+    float *dat_scales;        /* [npol][nchan] */
+    uint8_t *data;            /* [nsblk][npol][nchan] */
     int bytes_per_subint;
+    float *dat_freqs;         /* [nchan], MHz */
+    float *dat_weights;       /* [nchan] */
+    double tsubint, offs, lst, ra, dec, glon, glat; /* s, s, s, deg x 4 */
+    float feed_ang, pos_ang, par_ang, tel_az, tel_zen; /* deg */
 } vm_psrfits_sub;
-typedef struct vm_psrfits_t { vm_psrfits_sub sub; } vm_psrfits;
+typedef struct vm_psrfits_hdr_t {
+    char project_id[24], obs_mode[8], observer[24], telescope[24], frontend[24], backend[24];
+    char source[24], date_obs[64], poln_type[8], track_mode[16], cal_mode[8], feed_mode[8];
+    char ra_str[16], dec_str[16];
+    double scanlen, dt, fctr, BW, df, ra2000, dec2000, azimuth, zenith_ang, beam_FWHM;
+    double start_lst, MJD_epoch, fd_sang, fd_xyph, chan_dm;
+    int npol, nchan, nbits, nsblk, summed_polns, rcvr_polns, fd_hand, be_phase;
+    int start_day, start_sec, scan_number;
+} vm_psrfits_hdr;
+typedef struct vm_psrfits_t {
+    vm_psrfits_sub sub;
+    vm_psrfits_hdr hdr;
+    char basefilename[1024];
+    int rows_per_file; /* sub-integrations (seconds) per file */
+    int filenum;       /* files opened so far; files are <basefilename>_%04d.fits */
+    int tot_rows;      /* sub-integrations written */
+    int rows_in_file;
+    void *fptr;        /* the open file (FILE *) */
+    long naxis2_pos;   /* byte offset of the SUBINT NAXIS2 card in it */
+    long data_start;   /* byte offset of the SUBINT rows in it */
+} vm_psrfits;
 typedef struct mpi_psrfits_t {
     vm_psrfits coarse_chan_pf;
     int writer_id;
+    vm_psrfits spliced_pf;
 } mpi_psrfits;
 
 typedef struct host_buffer_t {
@@ -751,6 +780,33 @@ void vmPullS(vcsbeam_context *vm);
 void prepare_data_buffer_fine(gpuDoubleComplex *data_buffer_fine, vcsbeam_context *vm,
                               uintptr_t timestep_idx);
 void vmSendSToFits(vcsbeam_context *vm, mpi_psrfits *mpfs);
+/* The one-process form of the coarse-channel splice (vmSendSToFits on each
+ * rank + gather_splice_psrfits, src/form_beam.cpp:653-673 and
+ * src/beam_psrfits.c:608-642): vms[g] holds coarse channel g, in ascending
+ * frequency, all with the same pointings, Stokes, fine channels and samples,
+ * and each with vmCreateStatistics done.  Each context's Stokes are
+ * renormalised on the device straight into one spliced buffer, downloaded
+ * once into pf[p].sub (data [s][st][g*nfine + c], offsets and scales
+ * [st][g*nfine + c]) for every pointing p. */
+void vmSpliceSToFits(vcsbeam_context **vms, int ncoarse, vm_psrfits *pf);
+/* src/beam_psrfits.c:39-258 (spliced: vm is the lowest channel's context and
+ * vm->ncoarse_chans the number of channels spliced), 281-466 (one channel),
+ * 475-488.  The arrays are allocated here; free_psrfits releases them. */
+void populate_spliced_psrfits_header(vcsbeam_context *vm, vm_psrfits *pf, int max_sec_per_file,
+                                     int outpol, beam_geom *beam_geom_vals, char *basename,
+                                     bool is_coherent);
+void populate_psrfits_header(vcsbeam_context *vm, vm_psrfits *pf, int max_sec_per_file,
+                             int outpol, beam_geom *beam_geom_vals, char *incoh_basename,
+                             bool is_coherent);
+void free_psrfits(vm_psrfits *pf);
+/* No counterpart in the reference (psrfits_utils writes there): appends
+ * pf->sub as the next SUBINT row of <basefilename>_%04d.fits, starting a new
+ * file every rows_per_file rows, then advances offs; vmClosePsrfits finishes
+ * the open file.  Returns 0 on success. */
+int vmWritePsrfitsSubint(vm_psrfits *pf);
+void vmClosePsrfits(vm_psrfits *pf);
+/* J2000 RA/Dec (rad) to galactic longitude and latitude (rad) */
+void vmEqToGal(double ra, double dec, double *glon, double *glat);
 float *create_pinned_data_buffer(size_t size);
 void vmPushPolIdxLists(vcsbeam_context *vm);
 gpuDoubleComplex *create_data_buffer_fine(int npointing, int nsamples, int nchan, int npol);

@@ -152,12 +152,35 @@ class ForwardPfb(C.Structure):
 
 class PsrfitsSub(C.Structure):
     _fields_ = [("dat_offsets", C.c_void_p), ("dat_scales", C.c_void_p), ("data", C.c_void_p),
-                ("bytes_per_subint", C.c_int)]
+                ("bytes_per_subint", C.c_int), ("dat_freqs", C.c_void_p),
+                ("dat_weights", C.c_void_p)] + \
+        [(n, C.c_double) for n in ("tsubint", "offs", "lst", "ra", "dec", "glon", "glat")] + \
+        [(n, C.c_float) for n in ("feed_ang", "pos_ang", "par_ang", "tel_az", "tel_zen")]
+
+
+class PsrfitsHdr(C.Structure):
+    _fields_ = [(n, C.c_char * k) for n, k in (
+        ("project_id", 24), ("obs_mode", 8), ("observer", 24), ("telescope", 24),
+        ("frontend", 24), ("backend", 24), ("source", 24), ("date_obs", 64), ("poln_type", 8),
+        ("track_mode", 16), ("cal_mode", 8), ("feed_mode", 8), ("ra_str", 16), ("dec_str", 16))] + \
+        [(n, C.c_double) for n in ("scanlen", "dt", "fctr", "BW", "df", "ra2000", "dec2000",
+                                   "azimuth", "zenith_ang", "beam_FWHM", "start_lst", "MJD_epoch",
+                                   "fd_sang", "fd_xyph", "chan_dm")] + \
+        [(n, C.c_int) for n in ("npol", "nchan", "nbits", "nsblk", "summed_polns", "rcvr_polns",
+                                "fd_hand", "be_phase", "start_day", "start_sec", "scan_number")]
+
+
+class Psrfits(C.Structure):
+    """vm_psrfits: PSRFITS header, one sub-integration and the writer state."""
+    _fields_ = [("sub", PsrfitsSub), ("hdr", PsrfitsHdr), ("basefilename", C.c_char * 1024),
+                ("rows_per_file", C.c_int), ("filenum", C.c_int), ("tot_rows", C.c_int),
+                ("rows_in_file", C.c_int), ("fptr", C.c_void_p), ("naxis2_pos", C.c_long),
+                ("data_start", C.c_long)]
 
 
 class MpiPsrfits(C.Structure):
-    # mpi_psrfits { vm_psrfits coarse_chan_pf { vm_psrfits_sub sub }; int writer_id }
-    _fields_ = [("sub", PsrfitsSub), ("writer_id", C.c_int)]
+    # mpi_psrfits { vm_psrfits coarse_chan_pf; int writer_id; vm_psrfits spliced_pf }
+    _fields_ = [("coarse_chan_pf", Psrfits), ("writer_id", C.c_int), ("spliced_pf", Psrfits)]
 
 
 class GpuIpfbArrays(C.Structure):
@@ -233,6 +256,7 @@ LAYOUT_FIELDS = {
                                             "corr_fine_chan_width_hz"]),
     "Rfinput": (Rfinput, ["pol", "electrical_length_m", "vcs_order", "flagged"]),
     "gpu_ipfb_arrays": (GpuIpfbArrays, ["in_real", "d_out"]),
+    "vm_psrfits": (Psrfits, ["hdr", "basefilename", "fptr", "data_start"]),
     "vdifinfo": (Vdifinfo, ["samples_per_frame", "stationid", "block_size", "b_scales", "fctr",
                             "ra_str", "BW", "date_obs", "basefilename", "got_scales"]),
 }
@@ -313,6 +337,15 @@ _SIGS = {
     "vmRequantVDIFDevice": (None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "create_data_buffer_fine": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "prepare_data_buffer_fine": (None, [C.c_void_p, _CTX, C.c_size_t]),
+    "vmSpliceSToFits": (None, [C.c_void_p, C.c_int, C.POINTER(Psrfits)]),
+    "populate_spliced_psrfits_header": (None, [_CTX, C.POINTER(Psrfits), C.c_int, C.c_int,
+                                               C.POINTER(BeamGeom), C.c_char_p, C.c_bool]),
+    "populate_psrfits_header": (None, [_CTX, C.POINTER(Psrfits), C.c_int, C.c_int,
+                                       C.POINTER(BeamGeom), C.c_char_p, C.c_bool]),
+    "free_psrfits": (None, [C.POINTER(Psrfits)]),
+    "vmWritePsrfitsSubint": (C.c_int, [C.POINTER(Psrfits)]),
+    "vmClosePsrfits": (None, [C.POINTER(Psrfits)]),
+    "vmEqToGal": (None, [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "vmLoadMetafits": (C.c_int, [C.c_char_p, C.POINTER(MetafitsMetadata),
                                  C.POINTER(VoltageMetadata)]),
     "vmFreeMetafits": (None, [C.POINTER(MetafitsMetadata)]),

@@ -20,7 +20,7 @@ INC = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["vm_device.hip", "pfb.hip", "beamform.hip", "beamform_mfma.hip",
                "ipfb.hip", "geometry.hip", "beam_model.hip"]
-C_SOURCES = ["vm_host.c", "calibration.c", "astrometry.c", "obsfiles.c"]
+C_SOURCES = ["vm_host.c", "calibration.c", "astrometry.c", "obsfiles.c", "psrfits.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code (they have given

@@ -251,8 +251,12 @@ __global__ __launch_bounds__(1024) void k_renorm_ref(const float *__restrict__ S
 }
 
 // Device layout d_S (VM_S_IDX): min/max per (p, st, c), one workgroup each.
+// Results go to row (p*nstokes + st)*ctot + coff + c of offsets/scales: ctot =
+// nchan, coff = 0 for one coarse channel; a spliced band of ctot channels
+// takes this channel's nchan at coff.
 __global__ __launch_bounds__(256) void k_minmax_dev(const float *__restrict__ S, int nstep,
                                                     int npointing, int nstokes, int nchan,
+                                                    int ctot, int coff,
                                                     float *__restrict__ offsets,
                                                     float *__restrict__ scales)
 {
@@ -280,15 +284,18 @@ __global__ __launch_bounds__(256) void k_minmax_dev(const float *__restrict__ S,
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        scales[row] = (smax[0] - smin[0]) / 256.0f;
-        offsets[row] = smin[0];
+        const size_t orow = (size_t)(p * nstokes + st) * ctot + coff + c;
+        scales[orow] = (smax[0] - smin[0]) / 256.0f;
+        offsets[orow] = smin[0];
     }
 }
 
-// Quantise + transpose: d_S (VM_S_IDX) -> Ss[p][s][st][c] (uint8).
+// Quantise + transpose: d_S (VM_S_IDX) -> Ss[p][s][st][ctot] (uint8), this
+// channel's nchan at coff (offsets/scales rows as in k_minmax_dev).
 // block 256, tile 64 channels x 64 samples; grid (ceil(ns/64), ceil(nc/64), p*nst)
 __global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, int nstep,
                                                    int npointing, int nstokes, int nchan,
+                                                   int ctot, int coff,
                                                    const float *__restrict__ offsets,
                                                    const float *__restrict__ scales,
                                                    uint8_t *__restrict__ Ss)
@@ -301,7 +308,7 @@ __global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, 
         int c = c0 + r, s = s0 + tx;
         uint8_t q = 0;
         if (c < nchan && s < nstep) {
-            size_t row = (size_t)pst * nchan + c;
+            size_t row = (size_t)pst * ctot + coff + c;
             float val = (S[VM_S_IDX(p, st, c, s, npointing, nstokes, VM_S_NBLK(nstep))] -
                          offsets[row]) /
                         scales[row];
@@ -315,7 +322,7 @@ __global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, 
     for (int r = ty; r < 64; r += 4) {
         int s = s0 + r, c = c0 + tx;
         if (s < nstep && c < nchan)
-            Ss[(((size_t)p * nstep + s) * nstokes + st) * nchan + c] = T[tx][r];
+            Ss[(((size_t)p * nstep + s) * nstokes + st) * ctot + coff + c] = T[tx][r];
     }
 }
 
@@ -541,12 +548,12 @@ void vmSendSToFits(vcsbeam_context *vm, mpi_psrfits *mpfs)
     hipStream_t s = vm_stream(vm);
     vm_timer_start(vm_dev(vm), VM_T_RENORM);
     hipLaunchKernelGGL(k_minmax_dev, dim3(vm->npointing * nst * nchan), dim3(256), 0, s,
-                       (const float *)vm->d_S, ns, (int)vm->npointing, nst, nchan, vm->d_offsets,
-                       vm->d_scales);
+                       (const float *)vm->d_S, ns, (int)vm->npointing, nst, nchan, nchan, 0,
+                       vm->d_offsets, vm->d_scales);
     HIPCHECK(hipGetLastError());
     dim3 grid((ns + 63) / 64, (nchan + 63) / 64, vm->npointing * nst);
     hipLaunchKernelGGL(k_quant_dev, grid, dim3(256), 0, s, (const float *)vm->d_S, ns,
-                       (int)vm->npointing, nst, nchan,
+                       (int)vm->npointing, nst, nchan, nchan, 0,
                        vm->d_offsets, vm->d_scales, vm->d_Cscaled);
     HIPCHECK(hipGetLastError());
     vm_timer_stop(vm_dev(vm), VM_T_RENORM);
@@ -564,6 +571,67 @@ void vmSendSToFits(vcsbeam_context *vm, mpi_psrfits *mpfs)
             memcpy(sub->dat_scales, vm->scales + (size_t)p * nchan * nst, nchan * nst * sizeof(float));
         if (sub->data)
             memcpy(sub->data, vm->Cscaled + (size_t)p * ns * nchan * nst, (size_t)ns * nchan * nst);
+    }
+}
+
+void vmSpliceSToFits(vcsbeam_context **vms, int ncoarse, vm_psrfits *pf)
+{
+    // One process holds every coarse channel: each context's Stokes are
+    // renormalised into its slot of one spliced device buffer (the layout
+    // the reference's MPI_Igather builds on the writer rank), then one
+    // download per pointing.
+    if (ncoarse <= 0 || !vms || !pf) VM_FATAL("vmSpliceSToFits: no channels");
+    vcsbeam_context *v0 = vms[0];
+    const int ns = v0->fine_sample_rate, nchan = v0->nfine_chan, nst = v0->out_nstokes;
+    const unsigned np = v0->npointing;
+    for (int g = 0; g < ncoarse; g++) {
+        const vcsbeam_context *v = vms[g];
+        if (v->fine_sample_rate != ns || v->nfine_chan != nchan || v->out_nstokes != nst ||
+            v->npointing != np || !v->d_S)
+            VM_FATAL("vmSpliceSToFits: channel %d differs in shape or has no Stokes", g);
+    }
+    const int ctot = nchan * ncoarse;
+    for (unsigned p = 0; p < np; p++)
+        if (!pf[p].sub.data || !pf[p].sub.dat_offsets || !pf[p].sub.dat_scales ||
+            pf[p].sub.bytes_per_subint != ns * nst * ctot)
+            VM_FATAL("vmSpliceSToFits: pointing %u's buffers do not hold %d x %d x %d bytes", p, ns,
+                     nst, ctot);
+    static uint8_t *d_data = NULL;
+    static float *d_stat = NULL;
+    static size_t data_cap = 0, stat_cap = 0;
+    const size_t data_bytes = (size_t)np * ns * nst * ctot, stat_n = (size_t)np * nst * ctot;
+    if (data_cap < data_bytes) {
+        if (d_data) HIPCHECK(hipFree(d_data));
+        HIPCHECK(hipMalloc((void **)&d_data, data_bytes));
+        data_cap = data_bytes;
+    }
+    if (stat_cap < 2 * stat_n * sizeof(float)) {
+        if (d_stat) HIPCHECK(hipFree(d_stat));
+        HIPCHECK(hipMalloc((void **)&d_stat, 2 * stat_n * sizeof(float)));
+        stat_cap = 2 * stat_n * sizeof(float);
+    }
+    float *d_offs = d_stat, *d_scl = d_stat + stat_n;
+    vm_timer_start(vm_dev(v0), VM_T_RENORM);
+    for (int g = 0; g < ncoarse; g++) {
+        hipStream_t s = vm_stream(vms[g]);
+        hipLaunchKernelGGL(k_minmax_dev, dim3(np * nst * nchan), dim3(256), 0, s,
+                           (const float *)vms[g]->d_S, ns, (int)np, nst, nchan, ctot, g * nchan,
+                           d_offs, d_scl);
+        HIPCHECK(hipGetLastError());
+        dim3 grid((ns + 63) / 64, (nchan + 63) / 64, np * nst);
+        hipLaunchKernelGGL(k_quant_dev, grid, dim3(256), 0, s, (const float *)vms[g]->d_S, ns,
+                           (int)np, nst, nchan, ctot, g * nchan, d_offs, d_scl, d_data);
+        HIPCHECK(hipGetLastError());
+    }
+    for (int g = 0; g < ncoarse; g++) HIPCHECK(hipStreamSynchronize(vm_stream(vms[g])));
+    vm_timer_stop(vm_dev(v0), VM_T_RENORM);
+    for (unsigned p = 0; p < np; p++) {
+        HIPCHECK(hipMemcpy(pf[p].sub.data, d_data + (size_t)p * ns * nst * ctot,
+                           (size_t)ns * nst * ctot, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(pf[p].sub.dat_offsets, d_offs + (size_t)p * nst * ctot,
+                           (size_t)nst * ctot * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(pf[p].sub.dat_scales, d_scl + (size_t)p * nst * ctot,
+                           (size_t)nst * ctot * sizeof(float), hipMemcpyDeviceToHost));
     }
 }
 

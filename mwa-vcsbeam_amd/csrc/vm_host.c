@@ -748,6 +748,12 @@ int vmStructLayout(const char *name, size_t *out, int n)
         PUT(sizeof(struct gpu_ipfb_arrays));
         PUT(OFF(struct gpu_ipfb_arrays, in_real));
         PUT(OFF(struct gpu_ipfb_arrays, d_out));
+    } else if (strcmp(name, "vm_psrfits") == 0) {
+        PUT(sizeof(vm_psrfits));
+        PUT(OFF(vm_psrfits, hdr));
+        PUT(OFF(vm_psrfits, basefilename));
+        PUT(OFF(vm_psrfits, fptr));
+        PUT(OFF(vm_psrfits, data_start));
     } else if (strcmp(name, "vdifinfo") == 0) {
         PUT(sizeof(struct vdifinfo));
         PUT(OFF(struct vdifinfo, samples_per_frame));

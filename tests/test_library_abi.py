@@ -36,7 +36,7 @@ def test_header_symbols_exported(pkg):
 
 
 @pytest.mark.parametrize("name", ["vcsbeam_context", "forward_pfb", "MetafitsMetadata",
-                                  "Rfinput", "gpu_ipfb_arrays", "vdifinfo"])
+                                  "Rfinput", "gpu_ipfb_arrays", "vdifinfo", "vm_psrfits"])
 def test_struct_layout_matches(pkg, name):
     c_side, py_side = pkg.struct_layout(name)
     assert c_side == py_side
