@@ -58,7 +58,7 @@ def event_time(lib, stream, fn, reps):
 def cpu_baseline(sess, obs, az, za, target_s=10.0):
     """FP64 oracle beamformer on the first n samples of the same second."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as orc  # the checker, used here only as the CPU baseline
+    import oracle_lib as orc  # the oracle, used here only as the CPU baseline
     from vcsbeam_amd.workloads import unit_vectors
     vfl = sess.voltages()  # [c][i][s]
     D = obs.calibration()
@@ -574,7 +574,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the beamforming path shards over coarse channels "
-              "(one GPU per channel) and this build runs on a single MI355X; use --gpus 1.",
+              "(one GPU per channel) and this benchmark runs on a single MI355X; use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

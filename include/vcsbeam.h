@@ -856,7 +856,7 @@ void vmCalcJDeviceChanB(vcsbeam_context *vm, const gpuDoubleComplex *B);
  * dipoles, 1.1 m grid, 0.278 m over ground, seeded 1 % complex gain and
  * delay jitter per dipole) and a BF16 MLP surrogate (16 -> 256 -> 256 -> 256
  * -> 8, GELU) trained on the GPU by this library's forward/backward kernels,
- * with the analytic beam as fallback and judge. */
+ * with the analytic beam as fallback and as the measure of its error. */
 typedef struct vm_beam_surrogate vm_beam_surrogate;
 /* Seeded dipole jitter (gain: complex std dev, delay: +-seconds) and an
  * untrained network (seeded He-uniform weights). */

@@ -4,9 +4,9 @@
 //
 // The reference evaluates its primary beam with the FEE model through
 // hyperbeam (src/primary_beam.c:79-191, 479-532), once per pointing at the
-// coarse-channel centre.  That model is not available here; this build
+// coarse-channel centre.  That model is not available here; this library
 // defines an analytic beam of the same tile geometry and uses it both to
-// train the surrogate and as the judge of its error:
+// train the surrogate and as the measure of its error:
 //   16 dual-pol short dipoles on a 4x4 grid (1.1 m), 0.278 m over a ground
 //   plane; l = sin za sin az, m = sin za cos az, k = 2 pi f / c
 //   AF_P = sum_d g[P][d] exp(i k (E_d l + N_d m) - i 2 pi f tau[P][d])

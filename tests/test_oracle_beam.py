@@ -1,6 +1,6 @@
 """CPU pins of the oracle's primary-beam model and MLP reference.
 
-The analytic dipole-array beam is this build's own definition (the reference
+The analytic dipole-array beam is this library's own definition (the reference
 has no analytic beam), so it is pinned by what its definition fixes: the
 short-dipole projections at zenith, the Dirichlet-kernel array factor of the
 nominal 4x4 grid, a single dipole's phase (position indexing and delay sign)
