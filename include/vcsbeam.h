@@ -787,7 +787,9 @@ void vmSendSToFits(vcsbeam_context *vm, mpi_psrfits *mpfs);
  * and each with vmCreateStatistics done.  Each context's Stokes are
  * renormalised on the device straight into one spliced buffer, downloaded
  * once into pf[p].sub (data [s][st][g*nfine + c], offsets and scales
- * [st][g*nfine + c]) for every pointing p. */
+ * [st][g*nfine + c]) for every pointing p.  The spliced device buffer is
+ * kept by the library between calls (grown as needed, on the current
+ * device); concurrent calls from several host threads race on it. */
 void vmSpliceSToFits(vcsbeam_context **vms, int ncoarse, vm_psrfits *pf);
 /* src/beam_psrfits.c:39-258 (spliced: vm is the lowest channel's context and
  * vm->ncoarse_chans the number of channels spliced), 281-466 (one channel),
@@ -974,6 +976,9 @@ void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n,
  * (src/beam_vdif.c:29-38, 254-260). */
 void vmPopulateVDIFHeader(vcsbeam_context *vm, beam_geom *beam_geom_vals, double mjd_start,
                           double sec_offset);
+/* vdif_write_second frames on the GPU through device buffers the library
+ * keeps between calls (grown as needed, on the current device); concurrent
+ * calls from several host threads race on them. */
 void vdif_write_second(struct vdifinfo *vf, vdif_header *vhdr, float *data_buffer_vdif);
 void vdif_write_data(struct vdifinfo *vf, int8_t *output);
 void float2int8_trunc(float *f, int n, float min, float max, int8_t *i);

@@ -121,6 +121,7 @@ static char *fits_string(const fits_hdu *h, const char *key)
                 if (p + 1 < end && p[1] == '\'') {
                     p += 2;
                     if (len + 1 >= cap) out = (char *)realloc(out, cap *= 2);
+                    if (!out) VM_FATAL("out of host memory");
                     out[len++] = '\'';
                     continue;
                 }
