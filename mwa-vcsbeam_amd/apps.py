@@ -15,7 +15,8 @@ import ctypes as C
 import numpy as np
 
 from . import (ANALYSIS_FILTER, SYNTHESIS_FILTER, BeamGeom, GpuIpfbArrays, MpiPsrfits,
-               PFB_FULL_PRECISION, PFB_MALLOC_HOST_OUTPUT, PFB_SMART, VM_BF_FP32, VM_SUCCESS, load)
+               PFB_EMULATE_FPGA, PFB_FULL_PRECISION, PFB_MALLOC_HOST_OUTPUT, PFB_SMART, VCSMWAXv2, VM_BF_FP32,
+               VM_SUCCESS, VcsbeamContext, Vdifinfo, load)
 from .workloads import unit_vectors
 
 # samples per block of the device Stokes layout d_S [c][s / 64][p][st][s % 64]
@@ -187,6 +188,161 @@ def fine_pfb_offline(obs, *, nseconds=1, nchunks=1, pfb_flags=PFB_SMART | PFB_MA
             outs.append(raw.view(np.complex128).reshape(fp.nspectra, fp.K, fp.I))
     lib.destroy_vcsbeam_context(vm)
     return outs
+
+
+def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, *,
+                               coarse_chan_idx=0, first_gps=None, nseconds=1, cal=None,
+                               ref_ant=None, keep_cross_terms=False, flags_file=None,
+                               psrfits=True, vdif=True, max_sec_per_file=200,
+                               pfb_flags=PFB_FULL_PRECISION | PFB_EMULATE_FPGA):
+    """app/make_mwa_tied_array_beam.c's sequence over files, with this
+    library's readers and writers in place of mwalib, PAL, psrfits_utils and
+    vdifio: metafits + MWAX subfiles in, calibration from cal = ("offringa",
+    file) or ("rts", directory) (None: identity D for unflagged tiles),
+    pointings as J2000 RA (hours) / Dec (degrees), beam geometry at the
+    first second (lines 139-147), FINEPFB analysis and MIRROR synthesis
+    filters from the runtime directory, PSRFITS (IQUV, 8-bit, one file per
+    pointing and max_sec_per_file seconds) and VDIF (pointing 0, as the
+    reference's inverse PFB) written under outdir.  Returns the output base
+    names, the per-second Stokes [p][s][st][c] and VDIF floats."""
+    import os
+    from . import Calibration, MetafitsMetadata, MpiPsrfits, VoltageFiles, VoltageMetadata
+    lib = load()
+    ras = np.atleast_1d(np.asarray(ras_hours, np.float64))
+    decs = np.atleast_1d(np.asarray(decs_degs, np.float64))
+    npointing = len(ras)
+    md, vmd = MetafitsMetadata(), VoltageMetadata()
+    lib.vmLoadMetafits(str(metafits).encode(), C.byref(md), C.byref(vmd))
+    if md.mwa_version != VCSMWAXv2:
+        raise ValueError("make_tied_array_beam_files reads MWAX subfiles")
+    vm = lib.vmInit(False)
+    v = vm.contents
+    gps0 = md.obs_id if first_gps is None else first_gps
+    lib.vmBindObsMetadata(vm, C.byref(md), C.byref(vmd), coarse_chan_idx, gps0, nseconds)
+    files = VoltageFiles(str(datadir).encode(), C.pointer(md), C.pointer(vmd))
+    v.read_second = C.cast(lib.vmReadSecondFromFiles, C.c_void_p).value
+    v.read_second_user = C.addressof(files)
+    v.out_nstokes = 4
+    v.chunks_per_second = 1
+    lib.vmLoadFilter(vm, b"FINEPFB", ANALYSIS_FILTER, 128)
+    lib.vmInitForwardPFB(vm, 128, pfb_flags)
+    lib.vmSetNumPointings(vm, npointing)
+    nchans, npols, nsamples = v.nfine_chan, 2, v.fine_sample_rate
+    if vdif:
+        v.do_inverse_pfb = True
+        lib.vmLoadFilter(vm, b"MIRROR", SYNTHESIS_FILTER, nchans)
+        lib.vmScaleFilterCoeffs(vm, SYNTHESIS_FILTER, 15.0 / 7.2)
+    lib.vmSetMaxGPUMem(vm, 1)
+    allocs = ("vmMallocVDevice", "vmMallocJVDevice", "vmMallocEHost", "vmMallocEDevice",
+              "vmMallocSHost", "vmMallocSDevice", "vmMallocJHost", "vmMallocJDevice",
+              "vmMallocDHost", "vmMallocDDevice", "vmMallocPQIdxsHost", "vmMallocPQIdxsDevice")
+    for f in allocs:
+        getattr(lib, f)(vm)
+    lib.vmSetPolIdxLists(vm)
+    lib.vmPushPolIdxLists(vm)
+    # calibration into D
+    keep = []
+    v.cal_metadata = C.cast(C.pointer(md), C.c_void_p)
+    D = np.ctypeslib.as_array(C.cast(v.D, C.POINTER(C.c_double)),
+                              shape=(md.num_ants, nchans, 2, 2, 2))
+    if cal is None:
+        D[:] = 0
+        D[..., 0, 0, 0] = D[..., 1, 1, 0] = 1.0
+    else:
+        kind, path = cal
+        buf = C.create_string_buffer(str(path).encode())
+        keep.append(buf)
+        v.cal.caldir = C.cast(buf, C.c_void_p)
+        if kind == "offringa":
+            lib.vmLoadOffringaSolution(vm)
+        elif kind == "rts":
+            lib.vmLoadRTSSolution(vm)
+        else:
+            raise ValueError(f"calibration kind {kind!r}")
+    if ref_ant is not None:
+        rb = C.create_string_buffer(ref_ant.encode())
+        keep.append(rb)
+        v.cal.ref_ant = C.cast(rb, C.c_void_p)
+    v.cal.keep_cross_terms = keep_cross_terms
+    lib.parse_calibration_correction_file(gps0, C.byref(v.cal))
+    lib.vmApplyCalibrationCorrections(vm)
+    if flags_file is not None:
+        fb = C.create_string_buffer(str(flags_file).encode())
+        keep.append(fb)
+        v.cal.flags_file = C.cast(fb, C.c_void_p)
+    lib.vmSetCustomTileFlags(vm)
+    # zero D also for tiles the metadata flags
+    for i in range(md.num_rf_inputs):
+        if md.rf_inputs[i].flagged:
+            D[md.rf_inputs[i].ant] = 0
+    lib.vmCreatePrimaryBeam(vm)
+    lib.vmCreateGeometricDelays(vm)
+    lib.vmCreateCudaStreams(vm)
+    # pointing geometry at the first second
+    sec_offset = gps0 - md.sched_start_gps_time_ms / 1000.0
+    mjd = md.sched_start_mjd + sec_offset / 86400.0
+    bg = (BeamGeom * npointing)()
+    for p in range(npointing):
+        lib.calc_beam_geom(float(ras[p]), float(decs[p]), mjd, C.byref(bg[p]))
+    os.makedirs(outdir, exist_ok=True)
+    mpfs = (MpiPsrfits * npointing)()
+    if psrfits:
+        for p in range(npointing):
+            lib.populate_psrfits_header(vm, C.byref(mpfs[p].coarse_chan_pf), max_sec_per_file, 4,
+                                        C.byref(bg[p]), None, True)
+            base = os.path.join(str(outdir), mpfs[p].coarse_chan_pf.basefilename.decode())
+            mpfs[p].coarse_chan_pf.basefilename = base.encode()
+        lib.vmCreateStatistics(vm, mpfs)
+    gi = GpuIpfbArrays()
+    vdif_buf = None
+    if vdif:
+        lib.malloc_ipfb(C.byref(gi), v.synth_filter, nsamples, npols, 1)
+        lib.cu_load_ipfb_filter(v.synth_filter, C.byref(gi))
+        lib.vmPopulateVDIFHeader(vm, bg, md.sched_start_mjd, sec_offset)
+        vf = C.cast(v.vf, C.POINTER(Vdifinfo))
+        base = os.path.join(str(outdir), vf[0].basefilename.decode())
+        vf[0].basefilename = base.encode()
+        vdif_buf = np.zeros(nsamples * nchans * npols * 2, np.float32)
+    out = {"S": [], "vdif": [], "psrfits": [], "vdif_base": None}
+    for t in range(nseconds):
+        if lib.vmReadNextSecond(vm) != VM_SUCCESS:
+            raise RuntimeError("vmReadNextSecond failed")
+        lib.vmCalcPhi(vm, bg)
+        lib.vmCalcJ(vm)
+        lib.vmPushPhi(vm)
+        lib.vmPushJ(vm)
+        lib.vmBeamformSecond(vm)
+        v.v.contents.locked = False
+        lib.vmPullS(vm)
+        out["S"].append(np.ctypeslib.as_array(C.cast(v.S, C.POINTER(C.c_float)),
+                                              shape=(npointing, nsamples, 4, nchans)).copy())
+        if psrfits:
+            lib.vmSendSToFits(vm, mpfs)
+            for p in range(npointing):
+                if lib.vmWritePsrfitsSubint(C.byref(mpfs[p].coarse_chan_pf)) != 0:
+                    raise RuntimeError("PSRFITS write failed")
+        if vdif:
+            lib.vmInvertPFBDevice(vm, C.byref(gi), t, _ptr(vdif_buf))
+            out["vdif"].append(vdif_buf.copy())
+            lib.vdif_write_second(vf, C.byref(v, VcsbeamContext.vhdr.offset), _ptr(vdif_buf))
+    if psrfits:
+        for p in range(npointing):
+            out["psrfits"].append(mpfs[p].coarse_chan_pf.basefilename.decode())
+            lib.free_psrfits(C.byref(mpfs[p].coarse_chan_pf))
+        lib.vmDestroyStatistics(vm)
+    if vdif:
+        out["vdif_base"] = vf[0].basefilename.decode()
+        lib.free_ipfb(C.byref(gi))
+    lib.vmDestroyCudaStreams(vm)
+    for f in allocs:
+        getattr(lib, f.replace("Malloc", "Free"))(vm)
+    lib.free_primary_beam(C.addressof(v.pb))
+    v.cal = Calibration()
+    v.cal_metadata = None
+    lib.destroy_vcsbeam_context(vm)
+    lib.vmFreeMetafits(C.byref(md))
+    out["bg"] = bg
+    return out
 
 
 def incoherent_beam(data, nsample, nchan, ninput):

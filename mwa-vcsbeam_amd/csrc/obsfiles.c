@@ -379,9 +379,15 @@ int vmLoadMetafits(const char *path, MetafitsMetadata *md, VoltageMetadata *vmd)
         if (strncmp(lp, "EL_", 3) == 0) lp += 3;
         rf->electrical_length_m = strtod(lp, NULL);
         /* MWAX subfiles hold inputs by antenna then polarisation; legacy
-         * recombined data by the PFB's interleave of each board's 64 inputs */
+         * recombined data by the PFB's interleave of each board's 64 inputs
+         * (whole boards only: with a partial board the order is the input) */
+        if (rf->input >= (uint32_t)nrows || rf->ant >= (uint32_t)(nrows / 2))
+            VM_FATAL("vmLoadMetafits: TILEDATA row %ld has input %u, antenna %u for %ld inputs",
+                     r, rf->input, rf->ant, nrows);
         rf->subfile_order = rf->ant * 2 + (rf->pol[0] == 'Y');
-        rf->vcs_order = (rf->input & ~63u) | ((rf->input & 15u) << 2) | ((rf->input >> 4) & 3u);
+        rf->vcs_order = (nrows % 64 == 0)
+                            ? (rf->input & ~63u) | ((rf->input & 15u) << 2) | ((rf->input >> 4) & 3u)
+                            : rf->input;
     }
     free(tab);
     qsort(md->rf_inputs, (size_t)nrows, sizeof(Rfinput), cmp_subfile_order);

@@ -485,7 +485,8 @@ void vmInitForwardPFB(vcsbeam_context *vm, int M, pfb_flags flags)
         uint32_t ant = vm->obs_metadata->rf_inputs[i].ant;
         char pol = *(vm->obs_metadata->rf_inputs[i].pol);
         int mwax_idx = 2 * ant + (pol - 'X');
-        if (mwax_idx < 0 || mwax_idx >= fpfb->I)
+        if (mwax_idx < 0 || mwax_idx >= fpfb->I || vm->obs_metadata->rf_inputs[i].vcs_order >=
+                                                       (uint32_t)fpfb->I)
             VM_FATAL("vmInitForwardPFB: rf input %d maps outside the input range", i);
         fpfb->i_output_idx[mwax_idx] = vm->obs_metadata->rf_inputs[i].vcs_order;
     }

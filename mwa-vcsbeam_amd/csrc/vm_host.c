@@ -312,9 +312,12 @@ void vmCalcJ(vcsbeam_context *vm)
             }
 }
 
+static void check_rf_inputs(const MetafitsMetadata *md);
+
 /* vmSetPolIdxLists, src/jones.c:40-64 */
 void vmSetPolIdxLists(vcsbeam_context *vm)
 {
+    check_rf_inputs(vm->obs_metadata);
     for (size_t i = 0; i < vm->obs_metadata->num_rf_inputs; i++) {
         Rfinput *rf = &vm->obs_metadata->rf_inputs[i];
         if (*(rf->pol) == 'X') vm->polQ_idxs[rf->ant] = rf->vcs_order;
@@ -546,12 +549,40 @@ void vmSetNumPointings(vcsbeam_context *vm, unsigned int npointings)
     vm->d_e_size_bytes = vm->e_size_bytes;
 }
 
+/* Every index the kernels take from the RF inputs must stay inside the
+ * buffers: antennas < num_ants, pols X or Y with each (antenna, pol) once,
+ * and vcs_order a permutation of 0 .. num_rf_inputs - 1. */
+static void check_rf_inputs(const MetafitsMetadata *md)
+{
+    const size_t n = md->num_rf_inputs;
+    if (!md->rf_inputs || n == 0 || n != 2 * md->num_ants)
+        VM_FATAL("metadata: %zu RF inputs for %zu antennas", n, md->num_ants);
+    uint8_t *seen_order = (uint8_t *)calloc(n, 1), *seen_input = (uint8_t *)calloc(n, 1);
+    if (!seen_order || !seen_input) VM_FATAL("out of host memory");
+    for (size_t i = 0; i < n; i++) {
+        const Rfinput *rf = &md->rf_inputs[i];
+        if (!rf->pol || (rf->pol[0] != 'X' && rf->pol[0] != 'Y'))
+            VM_FATAL("metadata: RF input %zu has no X/Y polarisation", i);
+        if (rf->ant >= md->num_ants)
+            VM_FATAL("metadata: RF input %zu has antenna %u of %zu", i, rf->ant, md->num_ants);
+        if (rf->vcs_order >= n || seen_order[rf->vcs_order]++)
+            VM_FATAL("metadata: vcs_order %u of RF input %zu is outside 0..%zu or repeated",
+                     rf->vcs_order, i, n - 1);
+        const size_t slot = 2 * (size_t)rf->ant + (rf->pol[0] == 'Y');
+        if (seen_input[slot]++)
+            VM_FATAL("metadata: antenna %u polarisation %c appears twice", rf->ant, rf->pol[0]);
+    }
+    free(seen_order);
+    free(seen_input);
+}
+
 void vmBindObsMetadata(vcsbeam_context *vm, MetafitsMetadata *obs_metadata,
                        VoltageMetadata *vcs_metadata, int coarse_chan_idx,
                        uint32_t first_gps_second, int num_gps_seconds)
 {
     /* vmBindObsData, src/metadata.c:166-233, with the mwalib look-ups replaced
      * by caller-supplied metadata */
+    check_rf_inputs(obs_metadata);
     vm->obs_metadata = obs_metadata;
     vm->vcs_metadata = vcs_metadata;
     vm->num_gps_seconds_to_process = num_gps_seconds;

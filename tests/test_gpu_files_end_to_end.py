@@ -1,0 +1,128 @@
+"""End to end from files to files (app/make_mwa_tied_array_beam.c's
+sequence): a metafits (8 tiles, one flagged), one second of MWAX voltages in
+a subfile, an Offringa calibration file and a J2000 pointing go through
+make_tied_array_beam_files, which writes PSRFITS (IQUV) and VDIF.  The same
+second, metadata, filters and D are also run through the in-memory path
+(tied_array_beam), which the other GPU tests hold to the FP64 oracle:
+- Stokes agree to 1e-6 (the only difference is the unit vectors, from
+  calc_beam_geom's az/el here and from az/za there);
+- the PSRFITS rows dequantise to those Stokes within one quantisation step;
+- the VDIF payload is the oracle's requantisation of the inverse-PFB floats,
+  its headers the oracle's for that second.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from test_obsfiles import _bintable, _card, _long_string, _pad
+from test_psrfits import _col, _columns, read_fits
+
+pytestmark = pytest.mark.gpu
+NT = 8
+GPS = 1300000000
+
+
+def _metafits(path, east, north, height, cable, flagged):
+    n = 2 * NT
+    ant = np.repeat(np.arange(NT), 2)
+    pol = ["X", "Y"] * NT
+    inp = np.arange(n)
+    i2 = lambda a: np.asarray(a, ">i2")
+    f4 = lambda a: np.asarray(a, ">f4")
+    prim = [_card("SIMPLE", True), _card("BITPIX", 8), _card("NAXIS", 0), _card("EXTEND", True),
+            _card("GPSTIME", GPS), _card("MJD", 59000.5), _card("PROJECT", "G0001"),
+            _card("MODE", "MWAX_VCS"), _card("NCHANS", 24), _card("BANDWDTH", 30.72),
+            _card("FINECHAN", 10.0), _card("NINPUTS", n), _card("RA", 80.0), _card("DEC", -26.7),
+            _long_string("CHANNELS", ",".join(str(c) for c in range(109, 133))), _card("END")]
+    cols = [("Input", "I", i2(inp)), ("Antenna", "I", i2(ant)), ("Tile", "I", i2(1000 + ant)),
+            ("TileName", "8A", [f"Tile{a:03d}".encode().ljust(8) for a in ant]),
+            ("Pol", "1A", [p.encode() for p in pol]),
+            ("Flag", "I", i2([1 if a == flagged else 0 for a in ant])),
+            ("Length", "14A", [f"EL_{float(cable[a]):.3f}".encode().ljust(14) for a in ant]),
+            ("North", "E", f4(north[ant])), ("East", "E", f4(east[ant])),
+            ("Height", "E", f4(height[ant]))]
+    path.write_bytes(_pad(b"".join(prim)) + _bintable("TILEDATA", cols, n))
+
+
+def _offringa(path, sol):
+    import struct
+    hdr = b"MWAOCAL\0" + struct.pack("<IIIIII", 0, 0, 1, sol.shape[0], sol.shape[1], 4)
+    hdr += struct.pack("<dd", 0.0, 1.0)
+    path.write_bytes(hdr + np.ascontiguousarray(sol).view(np.float64).tobytes())
+
+
+class _Obs:
+    """The in-memory path's view of the same observation."""
+
+    def __init__(self, md, vmd, data, filt, D):
+        self.md, self.vmd, self.data, self.filter, self.D = md, vmd, data, filt, D
+        self.legacy = False
+
+    def second(self, t):
+        return self.data
+
+    def calibration(self, nchan=128):
+        return self.D
+
+
+def test_make_tied_array_beam_files(pkg, orc, gpu_lib, tmp_path):
+    from vcsbeam_amd.apps import make_tied_array_beam_files, tied_array_beam
+    from vcsbeam_amd.workloads import mwax_voltages
+    lib = gpu_lib
+    rng = np.random.default_rng(5)
+    east, north = rng.uniform(-700, 700, NT), rng.uniform(-700, 700, NT)
+    height = 377.0 + rng.uniform(-2, 2, NT)
+    cable = np.round(rng.uniform(50, 300, NT), 3)
+    flagged = 5
+    _metafits(tmp_path / "obs.metafits", east, north, height, cable, flagged)
+    data = mwax_voltages(2 * NT, 20, first_block=0, seed=31)
+    hdr = f"HDR_SIZE 4096\nNINPUTS {2 * NT}\n".encode().ljust(4096, b"\0")
+    (tmp_path / f"{GPS}_{GPS}_109.sub").write_bytes(hdr + bytes(data[0].nbytes) + data.tobytes())
+    sol = rng.standard_normal((NT, 128, 2, 2)) + 1j * rng.standard_normal((NT, 128, 2, 2))
+    _offringa(tmp_path / "cal.bin", sol)
+    print("files -> PSRFITS + VDIF", flush=True)
+    out = make_tied_array_beam_files(tmp_path / "obs.metafits", tmp_path, tmp_path / "out",
+                                     [5.575], [-26.7], cal=("offringa", tmp_path / "cal.bin"))
+    bg = out["bg"][0]
+    assert 0 < bg.el < math.pi / 2
+    # the in-memory path with the same inputs
+    md, vmd = pkg.MetafitsMetadata(), pkg.VoltageMetadata()
+    lib.vmLoadMetafits(str(tmp_path / "obs.metafits").encode(), C.byref(md), C.byref(vmd))
+    D = sol.copy()
+    D[..., 0, 1] = D[..., 1, 0] = 0           # cross terms zeroed (keep_cross_terms off)
+    D[flagged] = 0
+    runtime = pkg.RUNTIME_DIR
+    fine = np.loadtxt(os.path.join(runtime, "FINEPFB.dat"))
+    mirror = np.loadtxt(os.path.join(runtime, "MIRROR.dat"))
+    print("same second in memory", flush=True)
+    ref = tied_array_beam(_Obs(md, vmd, data, fine, D), np.array([bg.az]),
+                          np.array([math.pi / 2 - bg.el]), pfb_flags=557, ipfb=True,
+                          synth_filter=mirror, ipfb_device=True, want_e=False)
+    lib.vmFreeMetafits(C.byref(md))
+    S, Sref = out["S"][0][0], ref["S"][0][0]            # [s][st][c]
+    assert orc.rel_err(S[:, 0, :], Sref[:, 0, :]) < 1e-6
+    # PSRFITS: one row, dequantised within one step of the Stokes
+    (prim, _), (sub, tab) = read_fits(pathlib_of(out["psrfits"][0] + "_0001.fits"))
+    assert sub["NAXIS2"] == 1 and prim["OBSNCHAN"] == 128 and prim["STT_IMJD"] == 59000
+    cols = _columns(sub)
+    row = tab[:sub["NAXIS1"]]
+    q = _col(row, cols["DATA"]).reshape(10000, 4, 128).astype(np.float64)
+    off = _col(row, cols["DAT_OFFS"]).reshape(4, 128).astype(np.float64)
+    scl = _col(row, cols["DAT_SCL"]).reshape(4, 128).astype(np.float64)
+    deq = q * scl[None] + off[None]
+    assert np.all(np.abs(deq - S) <= scl[None] * 1.0001 + 1e-6 * np.abs(S))
+    # VDIF: one second of pointing 0, frames as the oracle builds them
+    vd = np.fromfile(out["vdif_base"] + ".vdif", np.uint8)
+    assert vd.size == 10000 * 544
+    h0 = orc.vdif_header(mjd=59000, sec_of_day=round(bg.fracmjd * 86400))
+    want, _ = orc.vdif_frames(out["vdif"][0], 10000, 512, h0, 10000)
+    np.testing.assert_array_equal(vd, want)
+    assert os.path.getsize(out["vdif_base"] + ".hdr") == 4096
+
+
+def pathlib_of(p):
+    import pathlib
+    return pathlib.Path(p)

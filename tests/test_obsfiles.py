@@ -77,7 +77,7 @@ def write_metafits(path, ntiles=4, mode="MWAX_VCS", gps=1300000000, mjd=True):
     ant = np.array([i // 2 for i in range(n)])[order]
     pol = [("X", "Y")[i % 2] for i in range(n)]
     pol = [pol[i] for i in order]
-    inp = np.array([(i // 2) * 2 + (i % 2) + 64 * ((i // 2) % 2) for i in range(n)])[order]
+    inp = rng.permutation(n)  # metafits input numbers: a permutation of 0 .. n-1
     north = rng.uniform(-1500, 1500, ntiles)
     east = rng.uniform(-1500, 1500, ntiles)
     height = 377.0 + rng.uniform(-3, 3, ntiles)
@@ -135,10 +135,27 @@ def test_load_metafits(pkg, tmp_path):
             assert rf.flagged == (a == 2)
             assert abs(rf.north_m - t["north"][a]) < 1e-3 and abs(rf.height_m - t["height"][a]) < 1e-4
             assert abs(rf.electrical_length_m - round(t["cable"][a], 3)) < 1e-9
-            inp = int(t["inp"][k])
-            assert rf.vcs_order == (inp & ~63) | ((inp & 15) << 2) | ((inp >> 4) & 3)
+            assert rf.vcs_order == t["inp"][k]  # a partial 64-input board: the input
         assert vmd.num_samples_per_voltage_block == 64000 and vmd.num_voltage_blocks_per_second == 20
         assert vmd.voltage_block_size_bytes == 64000 * 2 * 8
+    finally:
+        lib.vmFreeMetafits(C.byref(md))
+
+
+def test_vcs_order_interleaves_whole_boards(pkg, tmp_path):
+    """With whole 64-input boards the legacy order interleaves each board's
+    inputs (parity unpinned against mwalib); it is a permutation."""
+    lib = pkg.load()
+    f = tmp_path / "big.metafits"
+    t = write_metafits(f, ntiles=64)
+    md = pkg.MetafitsMetadata()
+    lib.vmLoadMetafits(str(f).encode(), C.byref(md), None)
+    try:
+        order = [md.rf_inputs[i].vcs_order for i in range(128)]
+        for i in range(128):
+            inp = md.rf_inputs[i].input
+            assert order[i] == (inp & ~63) | ((inp & 15) << 2) | ((inp >> 4) & 3)
+        assert sorted(order) == list(range(128))
     finally:
         lib.vmFreeMetafits(C.byref(md))
 
@@ -238,3 +255,27 @@ def test_read_legacy_recombined_file(pkg, tmp_path):
         np.testing.assert_array_equal(buf, sec)
     finally:
         lib.vmFreeMetafits(C.byref(md))
+
+
+@pytest.mark.parametrize("field, value, message", [
+    ("vcs_order", 999, "vcs_order 999"),
+    ("ant", 77, "antenna 77"),
+])
+def test_bind_rejects_indices_outside_the_buffers(field, value, message):
+    """Metadata whose RF-input indices would send the kernels outside their
+    buffers ends the process with a message at vmBindObsMetadata, before any
+    device work (run in a child process)."""
+    import os
+    import subprocess
+    import sys
+    code = ("import sys, ctypes as C; sys.path.insert(0, 'tests');"
+            "from conftest import load_package; pkg = load_package();"
+            "from vcsbeam_amd.workloads import Observation;"
+            "o = Observation(ntiles=4, blocks_per_second=1);"
+            f"o.rf[3].{field} = {value};"
+            "lib = pkg.load(); vm = lib.vmInit(False);"
+            "lib.vmBindObsMetadata(vm, C.byref(o.md), C.byref(o.vmd), 0, o.md.obs_id, 1)")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and message in r.stderr, r.stderr[-500:]
