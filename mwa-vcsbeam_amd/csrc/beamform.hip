@@ -398,6 +398,9 @@ void cu_form_incoh_beam(uint8_t *data, uint8_t *d_data, size_t data_size, float 
                         uint8_t *d_Iscaled, size_t Iscaled_size)
 {
     // src/form_beam.cpp:463-505 (all copies complete before return)
+    if (data_size < (size_t)nsample * nchan * ninput || Iscaled_size > (size_t)nsample * nchan)
+        VM_FATAL("cu_form_incoh_beam: %zu data bytes for %u x %d x %d samples, %zu scaled bytes",
+                 data_size, nsample, nchan, ninput, Iscaled_size);
     HIPCHECK(hipMemcpy(d_data, data, data_size, hipMemcpyHostToDevice));
     size_t rows = (size_t)nsample * nchan;
     vm_timer_start_on(vm_dev_global(), VM_T_INCOH, NULL);

@@ -357,6 +357,10 @@ void cu_invert_pfb(gpuDoubleComplex *data_buffer_fine, int file_no, int npointin
     if (nchan != KCH) VM_FATAL("cu_invert_pfb: only %d channels are supported", KCH);
     int P = g->ntaps;
     size_t per_s = (size_t)nchan * npol;
+    if (nsamples <= P || (size_t)(nsamples + P) * per_s * sizeof(double2) > (size_t)g->in_size ||
+        (size_t)nsamples * per_s * 2 * sizeof(float) > (size_t)g->out_size)
+        VM_FATAL("cu_invert_pfb: %d samples x %d pols exceed what malloc_ipfb allocated", nsamples,
+                 npol);
     int start_s = (file_no % 2 == 0 ? 2 * nsamples - P : nsamples - P);
     double2 *din = (double2 *)g->d_in_real;
     // contiguous pieces of the ring: s_in in [0, nsamples + P)
@@ -393,6 +397,10 @@ void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_
     vm_device_state *d = vm_dev(vm);
     hipStream_t s = (hipStream_t)d->stream;
     int ns = vm->fine_sample_rate, P = g->ntaps, npol = (int)vm->obs_metadata->num_ant_pols;
+    if (vm->nfine_chan != KCH || !vm->d_e || ns <= P ||
+        (size_t)ns * KCH * npol * 2 * sizeof(float) > (size_t)g->out_size)
+        VM_FATAL("vmInvertPFBDevice: %d samples x %d pols of %d channels do not fit the arrays "
+                 "malloc_ipfb allocated", ns, npol, vm->nfine_chan);
     size_t in_elems = (size_t)(ns + P) * KCH * npol;
     size_t hist_elems = (size_t)P * KCH * npol;
     size_t need = (in_elems + hist_elems) * sizeof(cf);
