@@ -345,6 +345,65 @@ def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, 
     return out
 
 
+def make_incoh_beam_files(metafits, datadir, outdir, *, coarse_chan_idx=0, first_gps=None,
+                          nseconds=1, max_sec_per_file=200):
+    """app/make_mwa_incoh_beam.c's sequence over files: legacy recombined
+    seconds (metafits + <obsid>_<gps>_ch<rec>.dat) through
+    cu_form_incoh_beam into an incoherent-beam PSRFITS (Stokes I, the tile
+    pointing, geometry at the scheduled start).  Returns the PSRFITS base
+    name and, per second, (offsets, scales, Iscaled [s][c])."""
+    import os
+    from . import DeviceArray, MetafitsMetadata, Psrfits, VoltageFiles, VoltageMetadata
+    lib = load()
+    md, vmd = MetafitsMetadata(), VoltageMetadata()
+    lib.vmLoadMetafits(str(metafits).encode(), C.byref(md), C.byref(vmd))
+    if md.mwa_version == VCSMWAXv2:
+        raise ValueError("make_incoh_beam_files reads legacy recombined files")
+    vm = lib.vmInit(False)
+    v = vm.contents
+    gps0 = md.obs_id if first_gps is None else first_gps
+    lib.vmBindObsMetadata(vm, C.byref(md), C.byref(vmd), coarse_chan_idx, gps0, nseconds)
+    files = VoltageFiles(str(datadir).encode(), C.pointer(md), C.pointer(vmd))
+    nchans, ninputs, nsamples = md.num_volt_fine_chans_per_coarse, md.num_rf_inputs, v.sample_rate
+    data_size = v.bytes_per_second
+    h, d = C.c_void_p(), C.c_void_p()
+    lib.allocate_input_output_arrays(C.byref(h), C.byref(d), data_size)
+    d_incoh = DeviceArray(nchans * nsamples * 4)
+    d_off, d_scl = DeviceArray(nchans * 4), DeviceArray(nchans * 4)
+    d_Is = DeviceArray(nchans * nsamples)
+    bg = BeamGeom()
+    lib.calc_beam_geom(md.ra_tile_pointing_deg / 15.0, md.dec_tile_pointing_deg,
+                       md.sched_start_mjd, C.byref(bg))
+    pf = Psrfits()
+    lib.populate_psrfits_header(vm, C.byref(pf), max_sec_per_file, 1, C.byref(bg), None, False)
+    os.makedirs(outdir, exist_ok=True)
+    pf.basefilename = os.path.join(str(outdir), pf.basefilename.decode()).encode()
+    out = {"psrfits": pf.basefilename.decode(), "seconds": []}
+    try:
+        for t in range(nseconds):
+            if lib.vmReadSecondFromFiles(C.byref(files), gps0 + t, coarse_chan_idx, h,
+                                         data_size) != 0:
+                raise RuntimeError(f"no data for GPS second {gps0 + t}")
+            lib.cu_form_incoh_beam(h, d, data_size, d_incoh.ptr, nsamples, nchans, ninputs,
+                                   pf.sub.dat_offsets, d_off.ptr, pf.sub.dat_scales, d_scl.ptr,
+                                   pf.sub.data, d_Is.ptr, nchans * nsamples)
+            off = np.ctypeslib.as_array(C.cast(pf.sub.dat_offsets, C.POINTER(C.c_float)),
+                                        (nchans,)).copy()
+            scl = np.ctypeslib.as_array(C.cast(pf.sub.dat_scales, C.POINTER(C.c_float)),
+                                        (nchans,)).copy()
+            Is = np.ctypeslib.as_array(C.cast(pf.sub.data, C.POINTER(C.c_uint8)),
+                                       (nsamples, nchans)).copy()
+            out["seconds"].append((off, scl, Is))
+            if lib.vmWritePsrfitsSubint(C.byref(pf)) != 0:
+                raise RuntimeError("PSRFITS write failed")
+    finally:
+        lib.free_psrfits(C.byref(pf))
+        lib.free_input_output_arrays(h, d)
+        lib.destroy_vcsbeam_context(vm)
+        lib.vmFreeMetafits(C.byref(md))
+    return out
+
+
 def incoherent_beam(data, nsample, nchan, ninput):
     """cu_form_incoh_beam on one second of legacy 4+4-bit data [s][c][i].
     Returns (incoh float [s][c] before scaling, offsets, scales, Iscaled)."""

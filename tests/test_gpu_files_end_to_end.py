@@ -25,7 +25,7 @@ NT = 8
 GPS = 1300000000
 
 
-def _metafits(path, east, north, height, cable, flagged):
+def _metafits(path, east, north, height, cable, flagged, mode="MWAX_VCS"):
     n = 2 * NT
     ant = np.repeat(np.arange(NT), 2)
     pol = ["X", "Y"] * NT
@@ -34,7 +34,7 @@ def _metafits(path, east, north, height, cable, flagged):
     f4 = lambda a: np.asarray(a, ">f4")
     prim = [_card("SIMPLE", True), _card("BITPIX", 8), _card("NAXIS", 0), _card("EXTEND", True),
             _card("GPSTIME", GPS), _card("MJD", 59000.5), _card("PROJECT", "G0001"),
-            _card("MODE", "MWAX_VCS"), _card("NCHANS", 24), _card("BANDWDTH", 30.72),
+            _card("MODE", mode), _card("NCHANS", 24), _card("BANDWDTH", 30.72),
             _card("FINECHAN", 10.0), _card("NINPUTS", n), _card("RA", 80.0), _card("DEC", -26.7),
             _long_string("CHANNELS", ",".join(str(c) for c in range(109, 133))), _card("END")]
     cols = [("Input", "I", i2(inp)), ("Antenna", "I", i2(ant)), ("Tile", "I", i2(1000 + ant)),
@@ -126,3 +126,32 @@ def test_make_tied_array_beam_files(pkg, orc, gpu_lib, tmp_path):
 def pathlib_of(p):
     import pathlib
     return pathlib.Path(p)
+
+
+def test_make_incoh_beam_files(pkg, gpu_lib, tmp_path):
+    """app/make_mwa_incoh_beam.c's sequence: a legacy recombined second in a
+    .dat file through make_incoh_beam_files; the PSRFITS row holds exactly
+    what cu_form_incoh_beam gives for the same bytes from memory."""
+    from vcsbeam_amd.apps import incoherent_beam, make_incoh_beam_files
+    from vcsbeam_amd.workloads import legacy_int4
+    rng = np.random.default_rng(8)
+    _metafits(tmp_path / "legacy.metafits", rng.uniform(-700, 700, NT), rng.uniform(-700, 700, NT),
+              np.full(NT, 377.0), np.full(NT, 100.0), flagged=2, mode="VOLTAGE_START")
+    data = legacy_int4(10000 * 128 * 2 * NT)
+    data.tofile(tmp_path / f"{GPS}_{GPS}_ch109.dat")
+    print("legacy file -> incoherent PSRFITS", flush=True)
+    out = make_incoh_beam_files(tmp_path / "legacy.metafits", tmp_path, tmp_path / "out")
+    off, scl, Is = out["seconds"][0]
+    print("same second from memory", flush=True)
+    _, off_m, scl_m, Is_m = incoherent_beam(data, 10000, 128, 2 * NT)
+    np.testing.assert_array_equal(Is, Is_m)
+    np.testing.assert_array_equal(off, off_m)
+    np.testing.assert_array_equal(scl, scl_m)
+    assert os.path.basename(out["psrfits"]) == f"G0001_{GPS}_incoh_ch109"
+    (prim, _), (sub, tab) = read_fits(pathlib_of(out["psrfits"] + "_0001.fits"))
+    assert prim["RA"] == "05:20:00.00" and prim["DEC"] == "-26:42:00.00"
+    assert sub["NPOL"] == 1 and sub["POL_TYPE"] == "AA+BB" and sub["NAXIS2"] == 1
+    cols = _columns(sub)
+    row = tab[:sub["NAXIS1"]]
+    np.testing.assert_array_equal(_col(row, cols["DATA"]), Is.ravel())
+    np.testing.assert_array_equal(_col(row, cols["DAT_SCL"]), scl)
