@@ -313,6 +313,7 @@ _SIGS = {
     "vmPullS": (None, [_CTX]),
     "vmSendSToFits": (None, [_CTX, C.c_void_p]),
     "vmExecuteForwardPFB": (None, [_CTX]),
+    "vmWritePFBOutputToFile": (None, [_CTX]),
     "vmUploadForwardPFBChunk": (None, [_CTX]),
     "vmWOLAChunk": (None, [_CTX]),
     "vmFPGARoundingChunk": (None, [_CTX]),

@@ -345,6 +345,48 @@ def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, 
     return out
 
 
+def fine_pfb_offline_files(metafits, datadir, outdir, *, coarse_chan_idx=0, first_gps=None,
+                           nseconds=1, nchunks=1, filter_name="FINEPFB"):
+    """app/fine_pfb_offline.c's sequence over files: MWAX subfiles (metafits +
+    <obsid>_<subobs>_<rec>.sub) through the forward PFB (PFB_SMART, host
+    output) into legacy recombined files <obsid>_<gps>_ch<rec>.dat in
+    outdir.  Returns the file paths."""
+    import os
+    from . import MetafitsMetadata, VoltageFiles, VoltageMetadata
+    lib = load()
+    md, vmd = MetafitsMetadata(), VoltageMetadata()
+    lib.vmLoadMetafits(str(metafits).encode(), C.byref(md), C.byref(vmd))
+    if md.mwa_version != VCSMWAXv2:
+        raise ValueError(f"observation {md.obs_id} is not coarse-channelised MWAX data")
+    vm = lib.vmInit(False)
+    v = vm.contents
+    gps0 = md.obs_id if first_gps is None else first_gps
+    lib.vmBindObsMetadata(vm, C.byref(md), C.byref(vmd), coarse_chan_idx, gps0, nseconds)
+    files = VoltageFiles(str(datadir).encode(), C.pointer(md), C.pointer(vmd))
+    v.read_second = C.cast(lib.vmReadSecondFromFiles, C.c_void_p).value
+    v.read_second_user = C.addressof(files)
+    os.makedirs(outdir, exist_ok=True)
+    outbuf = C.create_string_buffer(str(outdir).encode())
+    v.datadir = C.cast(outbuf, C.c_void_p)
+    lib.vmLoadFilter(vm, filter_name.encode(), ANALYSIS_FILTER, 128)
+    v.chunks_per_second = nchunks
+    lib.vmInitForwardPFB(vm, 128, PFB_SMART | PFB_MALLOC_HOST_OUTPUT)
+    rec = md.metafits_coarse_chans[coarse_chan_idx].rec_chan_number
+    paths = []
+    try:
+        while lib.vmReadNextSecond(vm) == VM_SUCCESS:
+            lib.vmExecuteForwardPFB(vm)
+            v.v.contents.locked = False
+            lib.vmWritePFBOutputToFile(vm)
+            gps = gps0 + len(paths)
+            paths.append(os.path.join(str(outdir), f"{md.obs_id}_{gps}_ch{rec:03d}.dat"))
+    finally:
+        v.datadir = None
+        lib.destroy_vcsbeam_context(vm)
+        lib.vmFreeMetafits(C.byref(md))
+    return paths
+
+
 def make_incoh_beam_files(metafits, datadir, outdir, *, coarse_chan_idx=0, first_gps=None,
                           nseconds=1, max_sec_per_file=200):
     """app/make_mwa_incoh_beam.c's sequence over files: legacy recombined

@@ -768,13 +768,20 @@ void vmExecuteForwardPFB(vcsbeam_context *vm)
 
 void vmWritePFBOutputToFile(vcsbeam_context *vm)
 {
-    // src/pfb.cpp:781-796 (legacy file name generation needs mwalib; the
-    // file is named from the GPS second and coarse channel index instead)
+    // src/pfb.cpp:781-796: the legacy recombined file of this second and
+    // coarse channel, <obsid>_<gps>_ch<receiver channel>.dat (the name
+    // mwalib gives it, src/metadata.c:1245-1262), in vm->datadir (default
+    // the working directory)
     logger_start_stopwatch(vm->log, "write", true);
-    char filename[256];
+    char filename[1200];
     uint32_t gps = vm->gps_seconds_to_process[vm->current_gps_idx ? vm->current_gps_idx - 1 : 0];
-    snprintf(filename, sizeof(filename), "%s/%u_%u_ch%03d.dat", vm->datadir ? vm->datadir : ".",
-             vm->obs_metadata->obs_id, gps, vm->coarse_chan_idxs_to_process[0]);
+    const int cidx = vm->coarse_chan_idxs_to_process[0];
+    if (cidx < 0 || (size_t)cidx >= vm->obs_metadata->num_metafits_coarse_chans)
+        VM_FATAL("vmWritePFBOutputToFile: coarse channel index %d of %zu", cidx,
+                 vm->obs_metadata->num_metafits_coarse_chans);
+    snprintf(filename, sizeof(filename), "%s/%u_%u_ch%03u.dat",
+             vm->datadir ? (const char *)vm->datadir : ".", vm->obs_metadata->obs_id, gps,
+             vm->obs_metadata->metafits_coarse_chans[cidx].rec_chan_number);
     FILE *f = fopen(filename, "w");
     if (!f) VM_FATAL("vmWritePFBOutputToFile: cannot open %s", filename);
     fwrite(vm->fpfb->vcs_data, vm->fpfb->vcs_size, sizeof(uint8_t), f);

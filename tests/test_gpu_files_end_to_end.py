@@ -155,3 +155,40 @@ def test_make_incoh_beam_files(pkg, gpu_lib, tmp_path):
     row = tab[:sub["NAXIS1"]]
     np.testing.assert_array_equal(_col(row, cols["DATA"]), Is.ravel())
     np.testing.assert_array_equal(_col(row, cols["DAT_SCL"]), scl)
+
+
+def test_offline_pfb_files_feed_the_incoherent_beam(pkg, gpu_lib, tmp_path):
+    """app/fine_pfb_offline.c then app/make_mwa_incoh_beam.c on its output: an
+    MWAX subfile through fine_pfb_offline_files gives the legacy recombined
+    file <obsid>_<gps>_ch109.dat, byte-equal to the same second's PFB from
+    memory; that file, read back as legacy data, gives the incoherent beam of
+    those bytes."""
+    from vcsbeam_amd.apps import (fine_pfb_offline, fine_pfb_offline_files, incoherent_beam,
+                                  make_incoh_beam_files)
+    from vcsbeam_amd.workloads import mwax_voltages
+    lib = gpu_lib
+    rng = np.random.default_rng(9)
+    geo = (rng.uniform(-700, 700, NT), rng.uniform(-700, 700, NT), np.full(NT, 377.0),
+           np.full(NT, 100.0))
+    _metafits(tmp_path / "mwax.metafits", *geo, flagged=-1)
+    _metafits(tmp_path / "legacy.metafits", *geo, flagged=-1, mode="VOLTAGE_START")
+    data = mwax_voltages(2 * NT, 20, first_block=0, seed=41)
+    hdr = f"HDR_SIZE 4096\nNINPUTS {2 * NT}\n".encode().ljust(4096, b"\0")
+    (tmp_path / f"{GPS}_{GPS}_109.sub").write_bytes(hdr + bytes(data[0].nbytes) + data.tobytes())
+    print("subfile -> offline PFB -> legacy file", flush=True)
+    paths = fine_pfb_offline_files(tmp_path / "mwax.metafits", tmp_path, tmp_path / "pfb")
+    assert [os.path.basename(p) for p in paths] == [f"{GPS}_{GPS}_ch109.dat"]
+    md, vmd = pkg.MetafitsMetadata(), pkg.VoltageMetadata()
+    lib.vmLoadMetafits(str(tmp_path / "mwax.metafits").encode(), C.byref(md), C.byref(vmd))
+    fine = np.loadtxt(os.path.join(pkg.RUNTIME_DIR, "FINEPFB.dat"))
+    print("same second's PFB from memory", flush=True)
+    mem = fine_pfb_offline(_Obs(md, vmd, data, fine, None))[0]
+    lib.vmFreeMetafits(C.byref(md))
+    got = np.fromfile(paths[0], np.uint8)
+    np.testing.assert_array_equal(got, mem.ravel())
+    print("legacy file -> incoherent beam", flush=True)
+    out = make_incoh_beam_files(tmp_path / "legacy.metafits", tmp_path / "pfb", tmp_path / "incoh")
+    _, off_m, scl_m, Is_m = incoherent_beam(got, 10000, 128, 2 * NT)
+    off, scl, Is = out["seconds"][0]
+    np.testing.assert_array_equal(Is, Is_m)
+    np.testing.assert_array_equal(scl, scl_m)
