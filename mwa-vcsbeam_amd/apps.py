@@ -206,7 +206,7 @@ def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, 
     reference's inverse PFB) written under outdir.  Returns the output base
     names, the per-second Stokes [p][s][st][c] and VDIF floats."""
     import os
-    from . import Calibration, MetafitsMetadata, MpiPsrfits, VoltageFiles, VoltageMetadata
+    from . import MetafitsMetadata, MpiPsrfits, VoltageFiles, VoltageMetadata
     lib = load()
     ras = np.atleast_1d(np.asarray(ras_hours, np.float64))
     decs = np.atleast_1d(np.asarray(decs_degs, np.float64))
@@ -337,7 +337,13 @@ def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, 
     for f in allocs:
         getattr(lib, f.replace("Malloc", "Free"))(vm)
     lib.free_primary_beam(C.addressof(v.pb))
-    v.cal = Calibration()
+    # detach the buffers Python owns; free_calibration then releases what the
+    # library allocated (the PQ table's reference tile, the flagged-tile list)
+    v.cal.caldir = None
+    v.cal.flags_file = None
+    if ref_ant is not None:
+        v.cal.ref_ant = None
+    lib.free_calibration(C.byref(v.cal))
     v.cal_metadata = None
     lib.destroy_vcsbeam_context(vm)
     lib.vmFreeMetafits(C.byref(md))
