@@ -696,6 +696,9 @@ void vmSetPolIdxLists(vcsbeam_context *vm);
 void vmCalcJ(vcsbeam_context *vm);
 void vmSetNumPointings(vcsbeam_context *vm, unsigned int npointings);
 void vmSetNumNotFlaggedRFInputs(vcsbeam_context *vm);
+/* include/vcsbeam.h.in (src/metadata.c:1656-1705): "RA Dec" string pairs
+ * from a file into vm->ras_hours / vm->decs_degs, setting the pointing count */
+void vmParsePointingFile(vcsbeam_context *vm, const char *filename);
 /* include/vcsbeam.h.in:794-809: calibration into vm->D (host, before vmPushD /
  * vmCalcJ).  The Offringa loader reads vm->cal.caldir as a file, the RTS loader
  * as the directory of DI_JonesMatrices_node%03d.dat and

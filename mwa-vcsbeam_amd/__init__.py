@@ -351,6 +351,7 @@ _SIGS = {
                                  C.POINTER(VoltageMetadata)]),
     "vmFreeMetafits": (None, [C.POINTER(MetafitsMetadata)]),
     "vmReadSecondFromFiles": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_size_t]),
+    "vmParsePointingFile": (None, [_CTX, C.c_char_p]),
     "calc_beam_geom": (None, [C.c_double, C.c_double, C.c_double, C.POINTER(BeamGeom)]),
     "dec2hms": (None, [C.c_char_p, C.c_double, C.c_int]),
     "mjd2lst": (None, [C.c_double, C.POINTER(C.c_double)]),

@@ -199,7 +199,8 @@ def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, 
     library's readers and writers in place of mwalib, PAL, psrfits_utils and
     vdifio: metafits + MWAX subfiles in, calibration from cal = ("offringa",
     file) or ("rts", directory) (None: identity D for unflagged tiles),
-    pointings as J2000 RA (hours) / Dec (degrees), beam geometry at the
+    pointings as J2000 RA (hours) / Dec (degrees) (or ras_hours = a pointings
+    file of "RA Dec" strings, decs_degs ignored), beam geometry at the
     first second (lines 139-147), FINEPFB analysis and MIRROR synthesis
     filters from the runtime directory, PSRFITS (IQUV, 8-bit, one file per
     pointing and max_sec_per_file seconds) and VDIF (pointing 0, as the
@@ -208,11 +209,23 @@ def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, 
     import os
     from . import MetafitsMetadata, MpiPsrfits, VoltageFiles, VoltageMetadata
     lib = load()
+    md, vmd = MetafitsMetadata(), VoltageMetadata()
+    lib.vmLoadMetafits(str(metafits).encode(), C.byref(md), C.byref(vmd))
+    if isinstance(ras_hours, (str, os.PathLike)):
+        # a pointings file of "RA Dec" string pairs (vmParsePointingFile)
+        tmp = lib.vmInit(False)
+        tmp.contents.obs_metadata = C.pointer(md)
+        lib.vmParsePointingFile(tmp, str(ras_hours).encode())
+        n = tmp.contents.npointing
+        ras_hours = np.ctypeslib.as_array(C.cast(tmp.contents.ras_hours, C.POINTER(C.c_double)),
+                                          (n,)).copy()
+        decs_degs = np.ctypeslib.as_array(C.cast(tmp.contents.decs_degs, C.POINTER(C.c_double)),
+                                          (n,)).copy()
+        tmp.contents.obs_metadata = None
+        lib.destroy_vcsbeam_context(tmp)
     ras = np.atleast_1d(np.asarray(ras_hours, np.float64))
     decs = np.atleast_1d(np.asarray(decs_degs, np.float64))
     npointing = len(ras)
-    md, vmd = MetafitsMetadata(), VoltageMetadata()
-    lib.vmLoadMetafits(str(metafits).encode(), C.byref(md), C.byref(vmd))
     if md.mwa_version != VCSMWAXv2:
         raise ValueError("make_tied_array_beam_files reads MWAX subfiles")
     vm = lib.vmInit(False)

@@ -274,6 +274,32 @@ void dec2hms(char *out, double in, int sflag)
     sprintf(out, "%s%2.2d:%2.2d:%05.2f", neg ? "-" : (sflag ? "+" : ""), h, m, s);
 }
 
+void vmParsePointingFile(vcsbeam_context *vm, const char *filename)
+{
+    /* src/metadata.c:1656-1705: whitespace-separated "RA Dec" pairs
+     * (HH:MM:SS.S, DD:MM:SS.S) into vm->ras_hours / vm->decs_degs; the
+     * pointing count is set from the file */
+    FILE *f = fopen(filename, "r");
+    if (!f) VM_FATAL("cannot open pointings file %s", filename);
+    char word[64], ra[64], dec[64];
+    int nwords = 0;
+    while (fscanf(f, "%63s", word) == 1) nwords++;
+    if (nwords % 2 != 0 || nwords == 0) VM_FATAL("cannot parse pointings file %s", filename);
+    vmSetNumPointings(vm, (unsigned)(nwords / 2));
+    free(vm->ras_hours);
+    free(vm->decs_degs);
+    vm->ras_hours = (double *)malloc(sizeof(double) * vm->npointing);
+    vm->decs_degs = (double *)malloc(sizeof(double) * vm->npointing);
+    if (!vm->ras_hours || !vm->decs_degs) VM_FATAL("out of host memory");
+    rewind(f);
+    for (unsigned p = 0; p < vm->npointing; p++) {
+        if (fscanf(f, "%63s %63s", ra, dec) != 2) VM_FATAL("cannot parse pointings file %s", filename);
+        vm->ras_hours[p] = parse_ra(ra);
+        vm->decs_degs[p] = parse_dec(dec);
+    }
+    fclose(f);
+}
+
 double parse_ra(char *ra_hhmmss)
 {
     /* "HH:MM:SS.S" -> decimal hours; fields out of range end the process */

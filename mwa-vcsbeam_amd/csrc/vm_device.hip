@@ -481,6 +481,8 @@ void destroy_vcsbeam_context(vcsbeam_context *vm)
     free(vm->gps_seconds_to_process);
     free(vm->coarse_chan_idxs_to_process);
     free(vm->cal_coarse_chan_idxs_to_process);
+    free(vm->ras_hours);
+    free(vm->decs_degs);
     if (vm->vf) {
         for (unsigned p = 0; p < vm->npointing; p++) {
             free(vm->vf[p].b_scales);

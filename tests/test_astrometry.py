@@ -163,3 +163,26 @@ def test_parse_dec_and_round_trip(pkg, text, degs):
     lib.dec2hms(out, v, 1)
     want = text if text[0] in "+-" else "+" + text
     assert out.value.decode() == (want if "." in want else want + ".00")
+
+
+def test_parse_pointing_file(pkg, tmp_path):
+    """vmParsePointingFile (src/metadata.c:1656-1705): RA/Dec pairs set the
+    pointing count and the decimal RAs (hours) and Decs (degrees)."""
+    from vcsbeam_amd.workloads import Observation
+    lib = pkg.load()
+    f = tmp_path / "pointings.txt"
+    f.write_text("05:34:31.94 +22:00:52.20\n  00:00:00 -00:30:00\n12:00:00.0  -45:00:00.0\n")
+    obs = Observation(ntiles=4, blocks_per_second=1)
+    vm = lib.vmInit(False)
+    v = vm.contents
+    v.obs_metadata = C.pointer(obs.md)
+    try:
+        lib.vmParsePointingFile(vm, str(f).encode())
+        assert v.npointing == 3
+        ras = np.ctypeslib.as_array(C.cast(v.ras_hours, C.POINTER(C.c_double)), (3,))
+        decs = np.ctypeslib.as_array(C.cast(v.decs_degs, C.POINTER(C.c_double)), (3,))
+        np.testing.assert_allclose(ras, [5 + 34 / 60 + 31.94 / 3600, 0.0, 12.0], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(decs, [22 + 52.2 / 3600, -0.5, -45.0], rtol=0, atol=1e-12)
+    finally:
+        v.obs_metadata = None
+        lib.destroy_vcsbeam_context(vm)
