@@ -84,8 +84,11 @@ def test_make_tied_array_beam_files(pkg, orc, gpu_lib, tmp_path):
     sol = rng.standard_normal((NT, 128, 2, 2)) + 1j * rng.standard_normal((NT, 128, 2, 2))
     _offringa(tmp_path / "cal.bin", sol)
     print("files -> PSRFITS + VDIF", flush=True)
+    (tmp_path / "pointings.txt").write_text("05:34:30.00 -26:42:00.00\n")  # 5.575 h, -26.7 deg
     out = make_tied_array_beam_files(tmp_path / "obs.metafits", tmp_path, tmp_path / "out",
-                                     [5.575], [-26.7], cal=("offringa", tmp_path / "cal.bin"))
+                                     str(tmp_path / "pointings.txt"), None,
+                                     cal=("offringa", tmp_path / "cal.bin"))
+    assert abs(out["bg"][0].mean_ra - 5.575 * math.pi / 12) < 1e-12
     bg = out["bg"][0]
     assert 0 < bg.el < math.pi / 2
     # the in-memory path with the same inputs
