@@ -109,6 +109,8 @@ typedef struct MetafitsMetadata {
     char *project_id;
     size_t num_corr_fine_chans_per_coarse; /* calibration solutions per coarse channel */
     uint32_t corr_fine_chan_width_hz;
+    uint64_t good_time_gps_ms;      /* first good data (metafits GOODTIME) */
+    size_t num_metafits_timesteps;  /* 1 s timesteps from the scheduled start */
 } MetafitsMetadata;
 
 typedef struct VoltageMetadata {
@@ -699,6 +701,32 @@ void vmSetNumNotFlaggedRFInputs(vcsbeam_context *vm);
 /* include/vcsbeam.h.in (src/metadata.c:1656-1705): "RA Dec" string pairs
  * from a file into vm->ras_hours / vm->decs_degs, setting the pointing count */
 void vmParsePointingFile(vcsbeam_context *vm, const char *filename);
+/* include/vcsbeam.h.in:689-703, 801-819: the observation and calibration by
+ * file name, with this library's metafits reader and voltage-file reader in
+ * place of mwalib.  vmLoadObsMetafits / vmLoadCalMetafits load into
+ * vm->obs_metadata / vm->cal_metadata (the context owns them);
+ * vmBindObsData then binds channels ("+n" index, "-n" from the end, or a
+ * receiver number) and seconds ("+n" from the first good second, "-n" from
+ * the end, or a GPS second) and registers vmReadSecondFromFiles over datadir
+ * (vmGetVoltageMetadata).  destroy_vcsbeam_context releases all of it. */
+void vmLoadObsMetafits(vcsbeam_context *vm, char *filename);
+void vmLoadCalMetafits(vcsbeam_context *vm, char *filename);
+void vmGetVoltageMetadata(vcsbeam_context *vm);
+void vmBindObsData(vcsbeam_context *vm, char *first_coarse_chan_str, int num_coarse_chans_to_process,
+                   int coarse_chan_idx_offset, char *first_gps_second_str,
+                   int num_gps_seconds_to_process, int gps_second_offset, char *datadir);
+void vmBindCalibrationData(vcsbeam_context *vm, char *caldir, int cal_type, bool use_bandpass,
+                           char *flags_file);
+void vmReadCalibration(vcsbeam_context *vm);
+void vmPrintTitle(vcsbeam_context *vm, const char *title);
+void vmGetVoltFilename(vcsbeam_context *vm, unsigned int coarse_chan_idx, uint64_t gps_second,
+                       char *filename);
+void vmGetLegacyVoltFilename(vcsbeam_context *vm, unsigned int coarse_chan_idx, uint64_t gps_second,
+                             char *filename);
+long unsigned int get_relative_gps(MetafitsMetadata *obs_metadata, long int relative_begin);
+long unsigned int parse_begin_string(MetafitsMetadata *obs_metadata, char *begin_str);
+uintptr_t parse_coarse_chan_string(MetafitsMetadata *obs_metadata, char *begin_coarse_chan_str);
+Rfinput *find_matching_rf_input(MetafitsMetadata *metadata, Rfinput *rfinput);
 /* include/vcsbeam.h.in:794-809: calibration into vm->D (host, before vmPushD /
  * vmCalcJ).  The Offringa loader reads vm->cal.caldir as a file, the RTS loader
  * as the directory of DI_JonesMatrices_node%03d.dat and

@@ -66,7 +66,8 @@ class MetafitsMetadata(C.Structure):
                 ("coarse_chan_width_hz", C.c_uint32), ("num_volt_fine_chans_per_coarse", C.c_size_t),
                 ("ra_tile_pointing_deg", C.c_double), ("dec_tile_pointing_deg", C.c_double),
                 ("project_id", C.c_char_p), ("num_corr_fine_chans_per_coarse", C.c_size_t),
-                ("corr_fine_chan_width_hz", C.c_uint32)]
+                ("corr_fine_chan_width_hz", C.c_uint32), ("good_time_gps_ms", C.c_uint64),
+                ("num_metafits_timesteps", C.c_size_t)]
 
 
 class VoltageMetadata(C.Structure):
@@ -253,7 +254,8 @@ LAYOUT_FIELDS = {
     "MetafitsMetadata": (MetafitsMetadata, ["sched_start_mjd", "rf_inputs",
                                             "metafits_coarse_chans", "coarse_chan_width_hz",
                                             "project_id", "num_corr_fine_chans_per_coarse",
-                                            "corr_fine_chan_width_hz"]),
+                                            "corr_fine_chan_width_hz", "good_time_gps_ms",
+                                            "num_metafits_timesteps"]),
     "Rfinput": (Rfinput, ["pol", "electrical_length_m", "vcs_order", "flagged"]),
     "gpu_ipfb_arrays": (GpuIpfbArrays, ["in_real", "d_out"]),
     "vm_psrfits": (Psrfits, ["hdr", "basefilename", "fptr", "data_start"]),
@@ -352,6 +354,19 @@ _SIGS = {
     "vmFreeMetafits": (None, [C.POINTER(MetafitsMetadata)]),
     "vmReadSecondFromFiles": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_size_t]),
     "vmParsePointingFile": (None, [_CTX, C.c_char_p]),
+    "vmLoadObsMetafits": (None, [_CTX, C.c_char_p]),
+    "vmLoadCalMetafits": (None, [_CTX, C.c_char_p]),
+    "vmGetVoltageMetadata": (None, [_CTX]),
+    "vmBindObsData": (None, [_CTX, C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.c_int, C.c_int,
+                             C.c_char_p]),
+    "vmBindCalibrationData": (None, [_CTX, C.c_char_p, C.c_int, C.c_bool, C.c_char_p]),
+    "vmReadCalibration": (None, [_CTX]),
+    "vmPrintTitle": (None, [_CTX, C.c_char_p]),
+    "vmGetVoltFilename": (None, [_CTX, C.c_uint, C.c_uint64, C.c_char_p]),
+    "vmGetLegacyVoltFilename": (None, [_CTX, C.c_uint, C.c_uint64, C.c_char_p]),
+    "get_relative_gps": (C.c_ulong, [C.POINTER(MetafitsMetadata), C.c_long]),
+    "parse_begin_string": (C.c_ulong, [C.POINTER(MetafitsMetadata), C.c_char_p]),
+    "parse_coarse_chan_string": (C.c_size_t, [C.POINTER(MetafitsMetadata), C.c_char_p]),
     "calc_beam_geom": (None, [C.c_double, C.c_double, C.c_double, C.POINTER(BeamGeom)]),
     "dec2hms": (None, [C.c_char_p, C.c_double, C.c_int]),
     "mjd2lst": (None, [C.c_double, C.POINTER(C.c_double)]),

@@ -41,8 +41,9 @@ static gpuDoubleComplex cmul(gpuDoubleComplex a, gpuDoubleComplex b)
     return r;
 }
 
-/* the RF input of md with rf's tile id and polarisation */
-static Rfinput *matching_rf_input(MetafitsMetadata *md, const Rfinput *rf)
+/* the RF input of md with rf's tile id and polarisation, or NULL
+ * (src/metadata.c:1575-1594) */
+Rfinput *find_matching_rf_input(MetafitsMetadata *md, Rfinput *rf)
 {
     for (size_t i = 0; i < md->num_rf_inputs; i++)
         if (md->rf_inputs[i].tile_id == rf->tile_id && md->rf_inputs[i].pol[0] == rf->pol[0])
@@ -245,7 +246,7 @@ void vmLoadRTSSolution(vcsbeam_context *vm)
     for (size_t i = 0; i < om->num_rf_inputs; i++) {
         Rfinput *orf = &om->rf_inputs[i];
         if (orf->pol[0] == 'Y') continue;
-        Rfinput *crf = matching_rf_input(cm, orf);
+        Rfinput *crf = find_matching_rf_input(cm, orf);
         if (!crf) VM_FATAL("vmLoadRTSSolution: tile %u has no input in the calibration metadata",
                            orf->tile_id);
         const size_t di = crf->input / 2;

@@ -283,14 +283,16 @@ int vmLoadMetafits(const char *path, MetafitsMetadata *md, VoltageMetadata *vmd)
     if (fits_read_header(f, &ph) != 0 || strncmp(ph.cards, "SIMPLE  =", 9) != 0)
         VM_FATAL("vmLoadMetafits: %s is not a FITS file", path);
     fits_data_size(&ph);
-    long gps = 0, nchans = 24, ninputs = -1;
-    double mjd = NAN, bw = 30.72, finechan = 10.0;
+    long gps = 0, nchans = 24, ninputs = -1, exposure = 0;
+    double mjd = NAN, bw = 30.72, finechan = 10.0, goodtime = NAN;
     if (!fits_long(&ph, "GPSTIME", &gps)) VM_FATAL("vmLoadMetafits: %s has no GPSTIME", path);
     fits_double(&ph, "MJD", &mjd);
     fits_long(&ph, "NCHANS", &nchans);
     fits_double(&ph, "BANDWDTH", &bw);
     fits_double(&ph, "FINECHAN", &finechan);
     fits_long(&ph, "NINPUTS", &ninputs);
+    fits_long(&ph, "EXPOSURE", &exposure);
+    fits_double(&ph, "GOODTIME", &goodtime);
     fits_double(&ph, "RA", &md->ra_tile_pointing_deg);
     fits_double(&ph, "DEC", &md->dec_tile_pointing_deg);
     char *mode = fits_string(&ph, "MODE");
@@ -309,6 +311,10 @@ int vmLoadMetafits(const char *path, MetafitsMetadata *md, VoltageMetadata *vmd)
     /* without an MJD card: GPS epoch MJD 44244, less the 18 leap seconds
      * GPS has gained on UTC since 2017 */
     md->sched_start_mjd = isnan(mjd) ? 44244.0 + ((double)gps - 18.0) / 86400.0 : mjd;
+    /* GOODTIME is a UNIX time: GPS = UNIX - 315964800 + 18 leap seconds */
+    md->good_time_gps_ms = isnan(goodtime) ? (uint64_t)gps * 1000
+                                           : (uint64_t)llround((goodtime - 315964800.0 + 18.0) * 1e3);
+    md->num_metafits_timesteps = exposure > 0 ? (size_t)exposure : 0;
     md->coarse_chan_width_hz = (uint32_t)llround(bw * 1e6 / (double)(nchans > 0 ? nchans : 24));
     md->corr_fine_chan_width_hz = (uint32_t)llround(finechan * 1e3);
     md->num_corr_fine_chans_per_coarse =
@@ -489,4 +495,177 @@ int vmReadSecondFromFiles(void *user, uint64_t gps_second, int coarse_chan_idx, 
     fclose(f);
     if (!ok) fprintf(stderr, "vmReadSecondFromFiles: %s: short read at byte %ld\n", path, off);
     return ok ? 0 : -1;
+}
+
+/* ------------------------------------------ context-level observation -- */
+/* vm->obs_context / vm->cal_context (mwalib's contexts in the reference)
+ * hold the metadata loaded here; vm->vcs_context holds the file reader's
+ * state.  destroy_vcsbeam_context releases all three. */
+typedef struct {
+    MetafitsMetadata md;
+    VoltageMetadata vmd;
+} vm_metafits_ctx;
+
+static MetafitsMetadata *load_into(void **ctx_slot, const char *filename)
+{
+    if (*ctx_slot) {
+        vmFreeMetafits(&((vm_metafits_ctx *)*ctx_slot)->md);
+        free(*ctx_slot);
+    }
+    vm_metafits_ctx *c = (vm_metafits_ctx *)calloc(1, sizeof(vm_metafits_ctx));
+    if (!c) VM_FATAL("out of host memory");
+    vmLoadMetafits(filename, &c->md, &c->vmd);
+    *ctx_slot = c;
+    return &c->md;
+}
+
+void vmLoadObsMetafits(vcsbeam_context *vm, char *filename)
+{
+    /* src/metadata.c:1297-1330 */
+    vm->obs_metadata = load_into((void **)&vm->obs_context, filename);
+}
+
+void vmLoadCalMetafits(vcsbeam_context *vm, char *filename)
+{
+    /* src/metadata.c:1339-1363 */
+    vm->cal_metadata = load_into((void **)&vm->cal_context, filename);
+}
+
+void vmGetVoltageMetadata(vcsbeam_context *vm)
+{
+    /* src/metadata.c:1373-1424: the voltage description of the loaded
+     * observation and the reader over the files in vm->datadir */
+    vm_metafits_ctx *c = (vm_metafits_ctx *)vm->obs_context;
+    if (!c) VM_FATAL("vmGetVoltageMetadata: no metafits loaded (vmLoadObsMetafits)");
+    vm->vcs_metadata = &c->vmd;
+    vm_voltage_files *f = (vm_voltage_files *)vm->vcs_context;
+    if (!f) {
+        f = (vm_voltage_files *)calloc(1, sizeof(vm_voltage_files));
+        if (!f) VM_FATAL("out of host memory");
+        vm->vcs_context = (VoltageContext *)f;
+    }
+    f->dir = vm->datadir ? vm->datadir : ".";
+    f->md = &c->md;
+    f->vmd = &c->vmd;
+    vm->read_second = vmReadSecondFromFiles;
+    vm->read_second_user = f;
+}
+
+void vm_free_obs_contexts(vcsbeam_context *vm)
+{
+    void **slots[2] = {(void **)&vm->obs_context, (void **)&vm->cal_context};
+    for (int k = 0; k < 2; k++)
+        if (*slots[k]) {
+            vm_metafits_ctx *c = (vm_metafits_ctx *)*slots[k];
+            if (vm->obs_metadata == &c->md) vm->obs_metadata = NULL;
+            if (vm->cal_metadata == &c->md) vm->cal_metadata = NULL;
+            if (vm->vcs_metadata == &c->vmd) vm->vcs_metadata = NULL;
+            vmFreeMetafits(&c->md);
+            free(c);
+            *slots[k] = NULL;
+        }
+    if (vm->vcs_context) {
+        if (vm->read_second_user == (void *)vm->vcs_context) {
+            vm->read_second = NULL;
+            vm->read_second_user = NULL;
+        }
+        free(vm->vcs_context);
+        vm->vcs_context = NULL;
+    }
+}
+
+long unsigned int get_relative_gps(MetafitsMetadata *obs_metadata, long int relative_begin)
+{
+    /* src/metadata.c:1448-1455: from the first good second, or (negative)
+     * counting back from the end of the 1 s timesteps */
+    if (relative_begin >= 0) return obs_metadata->good_time_gps_ms / 1000 + (unsigned long)relative_begin;
+    const long idx = (long)obs_metadata->num_metafits_timesteps + relative_begin;
+    if (idx < 0) VM_FATAL("get_relative_gps: %ld is before the first timestep", relative_begin);
+    return obs_metadata->sched_start_gps_time_ms / 1000 + (unsigned long)idx;
+}
+
+long unsigned int parse_begin_string(MetafitsMetadata *obs_metadata, char *begin_str)
+{
+    /* src/metadata.c:1473-1489 */
+    if (!begin_str) VM_FATAL("parse_begin_string: no string");
+    if (begin_str[0] == '+' || begin_str[0] == '-') return get_relative_gps(obs_metadata, atol(begin_str));
+    if (!(begin_str[0] >= '0' && begin_str[0] <= '9'))
+        VM_FATAL("parse_begin_string: cannot parse '%s' as a valid gps time", begin_str);
+    return (unsigned long)atol(begin_str);
+}
+
+uintptr_t parse_coarse_chan_string(MetafitsMetadata *obs_metadata, char *begin_coarse_chan_str)
+{
+    /* src/metadata.c:1509-1545: "+n" an index, "-n" from the end, else a
+     * receiver channel number */
+    if (!begin_coarse_chan_str) VM_FATAL("parse_coarse_chan_string: begin_coarse_chan_str cannot be NULL");
+    if (begin_coarse_chan_str[0] == '+') return (uintptr_t)atoi(begin_coarse_chan_str);
+    if (begin_coarse_chan_str[0] == '-')
+        return (uintptr_t)((long)obs_metadata->num_metafits_coarse_chans + atoi(begin_coarse_chan_str));
+    if (!(begin_coarse_chan_str[0] >= '0' && begin_coarse_chan_str[0] <= '9'))
+        VM_FATAL("parse_coarse_chan_string: cannot parse '%s' as a valid coarse channel",
+                 begin_coarse_chan_str);
+    const uint32_t rec = (uint32_t)atoi(begin_coarse_chan_str);
+    for (size_t i = 0; i < obs_metadata->num_metafits_coarse_chans; i++)
+        if (obs_metadata->metafits_coarse_chans[i].rec_chan_number == rec) return i;
+    VM_FATAL("parse_coarse_chan_string: no coarse channel with receiver number %u", rec);
+    return 0;
+}
+
+void vmGetVoltFilename(vcsbeam_context *vm, unsigned int coarse_chan_idx, uint64_t gps_second,
+                       char *filename)
+{
+    /* the file holding gps_second of the coarse channel: the MWAX subfile
+     * <obsid>_<subobs>_<rec>.sub or the legacy <obsid>_<gps>_ch<rec>.dat */
+    const MetafitsMetadata *md = vm->obs_metadata;
+    if (coarse_chan_idx >= md->num_metafits_coarse_chans)
+        VM_FATAL("vmGetVoltFilename: coarse channel index %u", coarse_chan_idx);
+    const unsigned rec = md->metafits_coarse_chans[coarse_chan_idx].rec_chan_number;
+    if (md->mwa_version == VCSMWAXv2) {
+        const uint64_t sub = md->obs_id + (gps_second - md->obs_id) / 8 * 8;
+        sprintf(filename, "%u_%lu_%03u.sub", md->obs_id, (unsigned long)sub, rec);
+    } else {
+        vmGetLegacyVoltFilename(vm, coarse_chan_idx, gps_second, filename);
+    }
+}
+
+void vmGetLegacyVoltFilename(vcsbeam_context *vm, unsigned int coarse_chan_idx, uint64_t gps_second,
+                             char *filename)
+{
+    /* src/metadata.c:1245-1262 */
+    const MetafitsMetadata *md = vm->obs_metadata;
+    if (coarse_chan_idx >= md->num_metafits_coarse_chans)
+        VM_FATAL("vmGetLegacyVoltFilename: coarse channel index %u", coarse_chan_idx);
+    sprintf(filename, "%u_%lu_ch%03u.dat", md->obs_id, (unsigned long)gps_second,
+            md->metafits_coarse_chans[coarse_chan_idx].rec_chan_number);
+}
+
+void vmBindCalibrationData(vcsbeam_context *vm, char *caldir, int cal_type, bool use_bandpass,
+                           char *flags_file)
+{
+    /* src/metadata.c:247-264 */
+    vm->cal.cal_type = cal_type;
+    vm->cal.use_bandpass = use_bandpass;
+    free(vm->cal.caldir);
+    vm->cal.caldir = caldir ? strdup(caldir) : NULL;
+    free(vm->cal.flags_file);
+    vm->cal.flags_file = flags_file ? strdup(flags_file) : NULL;
+}
+
+void vmReadCalibration(vcsbeam_context *vm)
+{
+    /* src/metadata.c:275-286 */
+    if (vm->cal.cal_type == CAL_RTS)
+        vmLoadRTSSolution(vm);
+    else if (vm->cal.cal_type == CAL_OFFRINGA)
+        vmLoadOffringaSolution(vm);
+    vmSetCustomTileFlags(vm);
+}
+
+void vmPrintTitle(vcsbeam_context *vm, const char *title)
+{
+    /* src/metadata.c:1733-1741 */
+    snprintf(vm->log_message, sizeof(vm->log_message), "------- VCSBeam (%s): %s -------",
+             vmLibraryVersion(), title);
+    if (vm->log) logger_message(vm->log, vm->log_message);
 }

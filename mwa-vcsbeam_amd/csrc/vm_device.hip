@@ -483,6 +483,9 @@ void destroy_vcsbeam_context(vcsbeam_context *vm)
     free(vm->cal_coarse_chan_idxs_to_process);
     free(vm->ras_hours);
     free(vm->decs_degs);
+    vm_free_obs_contexts(vm);
+    free(vm->datadir);
+    free_calibration(&vm->cal);
     if (vm->vf) {
         for (unsigned p = 0; p < vm->npointing; p++) {
             free(vm->vf[p].b_scales);

@@ -576,32 +576,39 @@ static void check_rf_inputs(const MetafitsMetadata *md)
     free(seen_input);
 }
 
-void vmBindObsMetadata(vcsbeam_context *vm, MetafitsMetadata *obs_metadata,
-                       VoltageMetadata *vcs_metadata, int coarse_chan_idx,
-                       uint32_t first_gps_second, int num_gps_seconds)
+/* the channel, second and size set-up shared by vmBindObsMetadata and
+ * vmBindObsData (src/metadata.c:177-232) */
+static void bind_common(vcsbeam_context *vm, int first_coarse_chan_idx, int num_coarse_chans,
+                        uint32_t first_gps_second, int num_gps_seconds)
 {
-    /* vmBindObsData, src/metadata.c:166-233, with the mwalib look-ups replaced
-     * by caller-supplied metadata */
-    check_rf_inputs(obs_metadata);
-    vm->obs_metadata = obs_metadata;
-    vm->vcs_metadata = vcs_metadata;
+    MetafitsMetadata *md = vm->obs_metadata;
+    VoltageMetadata *vcs_metadata = vm->vcs_metadata;
+    if (num_coarse_chans < 1 || first_coarse_chan_idx < 0 ||
+        (size_t)(first_coarse_chan_idx + num_coarse_chans) > md->num_metafits_coarse_chans)
+        VM_FATAL("bind: coarse channels %d..%d of %zu", first_coarse_chan_idx,
+                 first_coarse_chan_idx + num_coarse_chans - 1, md->num_metafits_coarse_chans);
     vm->num_gps_seconds_to_process = num_gps_seconds;
-    vm->num_coarse_chans_to_process = 1;
+    vm->num_coarse_chans_to_process = num_coarse_chans;
+    free(vm->gps_seconds_to_process);
+    free(vm->coarse_chan_idxs_to_process);
+    free(vm->cal_coarse_chan_idxs_to_process);
     vm->gps_seconds_to_process = (uint32_t *)xmalloc(sizeof(uint32_t) * num_gps_seconds);
     for (int g = 0; g < num_gps_seconds; g++) vm->gps_seconds_to_process[g] = first_gps_second + g;
-    vm->coarse_chan_idxs_to_process = (int *)xmalloc(sizeof(int));
-    vm->cal_coarse_chan_idxs_to_process = (int *)xmalloc(sizeof(int));
-    vm->coarse_chan_idxs_to_process[0] = coarse_chan_idx;
-    vm->cal_coarse_chan_idxs_to_process[0] = 0;
-    vm->coarse_chan_idx = coarse_chan_idx;
-    vmSetOutputChannelisation(vm, obs_metadata->mwa_version == VCSLegacyRecombined,
-                              obs_metadata->mwa_version == VCSMWAXv2);
+    vm->coarse_chan_idxs_to_process = (int *)xmalloc(sizeof(int) * num_coarse_chans);
+    vm->cal_coarse_chan_idxs_to_process = (int *)xmalloc(sizeof(int) * num_coarse_chans);
+    for (int c = 0; c < num_coarse_chans; c++) {
+        vm->coarse_chan_idxs_to_process[c] = first_coarse_chan_idx + c;
+        vm->cal_coarse_chan_idxs_to_process[c] = c;
+    }
+    vm->coarse_chan_idx = first_coarse_chan_idx;
+    vmSetOutputChannelisation(vm, md->mwa_version == VCSLegacyRecombined,
+                              md->mwa_version == VCSMWAXv2);
     vm->sample_rate = (int)(vcs_metadata->num_samples_per_voltage_block *
                             vcs_metadata->num_voltage_blocks_per_second);
     vm->bytes_per_second = (int)(vcs_metadata->num_voltage_blocks_per_second *
                                  vcs_metadata->voltage_block_size_bytes);
-    vm->nchan = (int)obs_metadata->num_volt_fine_chans_per_coarse;
-    if (obs_metadata->mwa_version == VCSLegacyRecombined) {
+    vm->nchan = (int)md->num_volt_fine_chans_per_coarse;
+    if (md->mwa_version == VCSLegacyRecombined) {
         vm->fine_sample_rate = vm->sample_rate;
         vm->nfine_chan = vm->nchan;
         vm->datatype = VM_INT4;
@@ -611,6 +618,37 @@ void vmBindObsMetadata(vcsbeam_context *vm, MetafitsMetadata *obs_metadata,
     }
     vmSetNumNotFlaggedRFInputs(vm);
     vmMallocVHost(vm);
+}
+
+void vmBindObsMetadata(vcsbeam_context *vm, MetafitsMetadata *obs_metadata,
+                       VoltageMetadata *vcs_metadata, int coarse_chan_idx,
+                       uint32_t first_gps_second, int num_gps_seconds)
+{
+    /* vmBindObsData, src/metadata.c:166-233, with the mwalib look-ups replaced
+     * by caller-supplied metadata */
+    check_rf_inputs(obs_metadata);
+    vm->obs_metadata = obs_metadata;
+    vm->vcs_metadata = vcs_metadata;
+    bind_common(vm, coarse_chan_idx, 1, first_gps_second, num_gps_seconds);
+}
+
+void vmBindObsData(vcsbeam_context *vm, char *first_coarse_chan_str, int num_coarse_chans_to_process,
+                   int coarse_chan_idx_offset, char *first_gps_second_str,
+                   int num_gps_seconds_to_process, int gps_second_offset, char *datadir)
+{
+    /* src/metadata.c:166-233: the observation from vmLoadObsMetafits, the
+     * voltages from the MWAX subfiles or legacy files in datadir */
+    if (!vm->obs_metadata) VM_FATAL("vmBindObsData: no metafits loaded (vmLoadObsMetafits)");
+    check_rf_inputs(vm->obs_metadata);
+    const uint32_t first_gps =
+        (uint32_t)(parse_begin_string(vm->obs_metadata, first_gps_second_str) + gps_second_offset);
+    const int first_cc =
+        (int)parse_coarse_chan_string(vm->obs_metadata, first_coarse_chan_str) + coarse_chan_idx_offset;
+    free(vm->datadir);
+    vm->datadir = strdup(datadir ? datadir : ".");
+    if (!vm->datadir) VM_FATAL("out of host memory");
+    vmGetVoltageMetadata(vm);
+    bind_common(vm, first_cc, num_coarse_chans_to_process, first_gps, num_gps_seconds_to_process);
 }
 
 vm_error vmReadNextSecond(vcsbeam_context *vm)
@@ -769,6 +807,8 @@ int vmStructLayout(const char *name, size_t *out, int n)
         PUT(OFF(MetafitsMetadata, project_id));
         PUT(OFF(MetafitsMetadata, num_corr_fine_chans_per_coarse));
         PUT(OFF(MetafitsMetadata, corr_fine_chan_width_hz));
+        PUT(OFF(MetafitsMetadata, good_time_gps_ms));
+        PUT(OFF(MetafitsMetadata, num_metafits_timesteps));
     } else if (strcmp(name, "Rfinput") == 0) {
         PUT(sizeof(Rfinput));
         PUT(OFF(Rfinput, pol));

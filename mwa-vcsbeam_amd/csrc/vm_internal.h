@@ -16,6 +16,10 @@ extern "C" {
 void vm_fatal(const char *file, int line, const char *fmt, ...);
 #define VM_FATAL(...) vm_fatal(__FILE__, __LINE__, __VA_ARGS__)
 
+/* releases the metadata vmLoadObsMetafits / vmLoadCalMetafits loaded and the
+ * file reader vmGetVoltageMetadata set up (obsfiles.c) */
+void vm_free_obs_contexts(vcsbeam_context *vm);
+
 /* Device Stokes layout d_S: [c][s / VM_SBLK][p][st][s % VM_SBLK] float, the
    second's samples cut into blocks of VM_SBLK, the last block padded.  A
    tile of pointings x VM_SBLK samples is then one contiguous run of memory

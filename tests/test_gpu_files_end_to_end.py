@@ -195,3 +195,44 @@ def test_offline_pfb_files_feed_the_incoherent_beam(pkg, gpu_lib, tmp_path):
     off, scl, Is = out["seconds"][0]
     np.testing.assert_array_equal(Is, Is_m)
     np.testing.assert_array_equal(scl, scl_m)
+
+
+def test_bind_obs_data_and_read_calibration(pkg, gpu_lib, tmp_path):
+    """The reference's own set-up calls by file name: vmLoadObsMetafits,
+    vmLoadCalMetafits, vmBindObsData ("109", "+0", datadir) and
+    vmReadNextSecond read the subfile's second into the read buffer;
+    vmBindCalibrationData + vmReadCalibration put the Offringa solution
+    into D."""
+    from vcsbeam_amd.workloads import mwax_voltages
+    lib = gpu_lib
+    rng = np.random.default_rng(12)
+    _metafits(tmp_path / "obs.metafits", rng.uniform(-700, 700, NT), rng.uniform(-700, 700, NT),
+              np.full(NT, 377.0), np.full(NT, 100.0), flagged=-1)
+    data = mwax_voltages(2 * NT, 20, first_block=0, seed=43)
+    hdr = f"HDR_SIZE 4096\nNINPUTS {2 * NT}\n".encode().ljust(4096, b"\0")
+    (tmp_path / f"{GPS}_{GPS}_109.sub").write_bytes(hdr + bytes(data[0].nbytes) + data.tobytes())
+    sol = rng.standard_normal((NT, 128, 2, 2)) + 1j * rng.standard_normal((NT, 128, 2, 2))
+    _offringa(tmp_path / "cal.bin", sol)
+    mf = str(tmp_path / "obs.metafits").encode()
+    vm = lib.vmInit(False)
+    v = vm.contents
+    try:
+        lib.vmLoadObsMetafits(vm, mf)
+        lib.vmLoadCalMetafits(vm, mf)
+        print("vmBindObsData + vmReadNextSecond", flush=True)
+        lib.vmBindObsData(vm, b"109", 1, 0, b"+0", 1, 0, str(tmp_path).encode())
+        assert C.cast(v.gps_seconds_to_process, C.POINTER(C.c_uint32))[0] == GPS
+        assert lib.vmReadNextSecond(vm) == pkg.VM_SUCCESS
+        got = np.ctypeslib.as_array(C.cast(v.v.contents.read_ptr, C.POINTER(C.c_int8)),
+                                    (data.size,))
+        np.testing.assert_array_equal(got, data.ravel())
+        lib.vmLoadFilter(vm, b"FINEPFB", pkg.ANALYSIS_FILTER, 128)
+        lib.vmInitForwardPFB(vm, 128, pkg.PFB_FULL_PRECISION)
+        lib.vmMallocDHost(vm)
+        lib.vmBindCalibrationData(vm, str(tmp_path / "cal.bin").encode(), 2, False, None)
+        lib.vmReadCalibration(vm)
+        D = np.ctypeslib.as_array(C.cast(v.D, C.POINTER(C.c_double)), (NT, 128, 2, 2, 2))
+        np.testing.assert_array_equal(D[..., 0] + 1j * D[..., 1], sol)
+        lib.vmFreeDHost(vm)
+    finally:
+        lib.destroy_vcsbeam_context(vm)

@@ -86,6 +86,7 @@ def write_metafits(path, ntiles=4, mode="MWAX_VCS", gps=1300000000, mjd=True):
     flag[[k for k in range(n) if ant[k] == 2]] = 1
     prim = [_card("SIMPLE", True), _card("BITPIX", 8), _card("NAXIS", 0), _card("EXTEND", True),
             _card("GPSTIME", gps, "obs id"), _card("EXPOSURE", 600),
+            _card("GOODTIME", float(gps + 8 + 315964800 - 18), "UNIX time of good data"),
             _card("PROJECT", "G0057"), _card("MODE", mode), _card("NCHANS", 24),
             _card("BANDWDTH", 30.72), _card("FINECHAN", 10.0), _card("NINPUTS", n),
             _card("RA", 83.6331), _card("DEC", 22.0145)]
@@ -279,3 +280,52 @@ def test_bind_rejects_indices_outside_the_buffers(field, value, message):
     r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
                        timeout=120)
     assert r.returncode != 0 and message in r.stderr, r.stderr[-500:]
+
+
+def test_begin_and_channel_strings(pkg, tmp_path):
+    """parse_begin_string / parse_coarse_chan_string (src/metadata.c:1448-1545):
+    "+n" from the first good second (GOODTIME, a UNIX time, 8 s after the
+    start here) or channel index n, "-n" from the end, else absolute."""
+    lib = pkg.load()
+    f = tmp_path / "obs.metafits"
+    write_metafits(f)
+    md = pkg.MetafitsMetadata()
+    lib.vmLoadMetafits(str(f).encode(), C.byref(md), None)
+    try:
+        assert md.good_time_gps_ms == 1300000008000 and md.num_metafits_timesteps == 600
+        assert lib.parse_begin_string(C.byref(md), b"+0") == 1300000008
+        assert lib.parse_begin_string(C.byref(md), b"+3") == 1300000011
+        assert lib.parse_begin_string(C.byref(md), b"-1") == 1300000599
+        assert lib.parse_begin_string(C.byref(md), b"1300000123") == 1300000123
+        assert lib.parse_coarse_chan_string(C.byref(md), b"+2") == 2
+        assert lib.parse_coarse_chan_string(C.byref(md), b"-1") == 23
+        assert lib.parse_coarse_chan_string(C.byref(md), b"110") == 1
+        assert lib.parse_coarse_chan_string(C.byref(md), b"133") == 23
+    finally:
+        lib.vmFreeMetafits(C.byref(md))
+
+
+def test_context_owned_metafits_and_file_names(pkg, tmp_path):
+    """vmLoadObsMetafits / vmLoadCalMetafits load into the context, which
+    releases them; the MWAX and legacy file names of a second."""
+    lib = pkg.load()
+    f = tmp_path / "obs.metafits"
+    write_metafits(f)
+    vm = lib.vmInit(False)
+    try:
+        lib.vmLoadObsMetafits(vm, str(f).encode())
+        lib.vmLoadCalMetafits(vm, str(f).encode())
+        v = vm.contents
+        assert v.obs_metadata.contents.obs_id == 1300000000
+        cal = C.cast(v.cal_metadata, C.POINTER(pkg.MetafitsMetadata)).contents
+        assert cal.num_rf_inputs == 8
+        name = C.create_string_buffer(256)
+        lib.vmGetVoltFilename(vm, 0, 1300000013, name)
+        assert name.value == b"1300000000_1300000008_109.sub"
+        lib.vmGetLegacyVoltFilename(vm, 23, 1300000013, name)
+        assert name.value == b"1300000000_1300000013_ch133.dat"
+        lib.vmPrintTitle(vm, b"test")
+        lib.vmBindCalibrationData(vm, b"/cal/dir", 1, True, None)
+        assert C.cast(v.cal.caldir, C.c_char_p).value == b"/cal/dir" and v.cal.cal_type == 1
+    finally:
+        lib.destroy_vcsbeam_context(vm)
