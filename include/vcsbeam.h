@@ -361,6 +361,9 @@ double vmGMST(double mjd);
 void vmNutation(double mjd, double *dpsi, double *deps, double *eps0);
 void vmMeanToApparent(double ra, double dec, double mjd, int stages, double *ra_ap, double *dec_ap);
 void vmHaDecToAzEl(double ha, double dec, double phi, double *az, double *el);
+/* include/vcsbeam.h.in:1201-1205: P = [-cos chi, -sin chi; sin chi, -cos chi]
+ * for the parallactic angle chi of (az, za) at latitude lat (radians) */
+void parallactic_angle_correction(double *P, double lat, double az, double za);
 
 /* PSRFITS search-mode output: the header and sub-integration fields that
  * populate_psrfits_header / populate_spliced_psrfits_header set

@@ -354,6 +354,7 @@ _SIGS = {
     "vmFreeMetafits": (None, [C.POINTER(MetafitsMetadata)]),
     "vmReadSecondFromFiles": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_size_t]),
     "vmParsePointingFile": (None, [_CTX, C.c_char_p]),
+    "parallactic_angle_correction": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "vmLoadObsMetafits": (None, [_CTX, C.c_char_p]),
     "vmLoadCalMetafits": (None, [_CTX, C.c_char_p]),
     "vmGetVoltageMetadata": (None, [_CTX]),

@@ -231,6 +231,24 @@ void vmHaDecToAzEl(double ha, double dec, double phi, double *az, double *el)
     *el = atan2(z, r);
 }
 
+void parallactic_angle_correction(double *P, double lat, double az, double za)
+{
+    /* src/primary_beam.c:436-456: the rotation by the parallactic angle chi
+     * of the direction (az from North through East, za) at latitude lat:
+     * az/el back to hour angle and declination, then
+     * chi = atan2(cos lat sin H, sin lat cos dec - cos lat sin dec cos H) */
+    const double el = M_PI / 2 - za;
+    const double sd = sin(el) * sin(lat) + cos(el) * cos(lat) * cos(az);
+    const double dec = asin(fmax(-1.0, fmin(1.0, sd)));
+    const double ha = atan2(-sin(az) * cos(el), sin(el) * cos(lat) - cos(el) * sin(lat) * cos(az));
+    const double chi = atan2(cos(lat) * sin(ha), sin(lat) * cos(dec) - cos(lat) * sin(dec) * cos(ha));
+    const double s = sin(chi), c = cos(chi);
+    P[0] = -c;
+    P[1] = -s;
+    P[2] = s;
+    P[3] = -c;
+}
+
 void calc_beam_geom(double ras_hours, double decs_degs, double mjd, beam_geom *bg)
 {
     /* src/geometry.c:226-286 */

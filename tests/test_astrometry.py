@@ -186,3 +186,33 @@ def test_parse_pointing_file(pkg, tmp_path):
     finally:
         v.obs_metadata = None
         lib.destroy_vcsbeam_context(vm)
+
+
+@pytest.mark.parametrize("ha, dec", [(0.7, -0.9), (-1.2, 0.3), (2.5, -0.2), (0.0, -1.2), (0.0, 0.2)])
+def test_parallactic_angle(pkg, ha, dec):
+    """parallactic_angle_correction (src/primary_beam.c:436-456) from the
+    az/za of a known hour angle and declination: the angle between the
+    directions to the celestial pole and to the zenith at the source, measured
+    here from the spherical triangle's vectors (not the closed form), and
+    positive west of the meridian."""
+    lib = pkg.load()
+    lat = -0.4660608448386394
+    az, el = C.c_double(), C.c_double()
+    lib.vmHaDecToAzEl(ha, dec, lat, C.byref(az), C.byref(el))
+    P = np.zeros(4)
+    lib.parallactic_angle_correction(P.ctypes.data, lat, az.value, math.pi / 2 - el.value)
+    chi = math.atan2(-P[1], -P[0])
+    assert abs(P[2] - math.sin(chi)) < 1e-15 and abs(P[3] - P[0]) < 1e-15
+    # source s, pole n, zenith z as unit vectors in (HA=0 meridian) equatorial axes
+    s = np.array([math.cos(dec) * math.cos(-ha), math.cos(dec) * math.sin(-ha), math.sin(dec)])
+    n = np.array([0.0, 0.0, 1.0])
+    z = np.array([math.cos(lat), 0.0, math.sin(lat)])
+    tn = n - (n @ s) * s      # towards the pole, in the sky plane at s
+    tz = z - (z @ s) * s      # towards the zenith
+    # the angle from tn to tz, counted positive north-through-east on the
+    # observer's sky (clockwise seen from outside the sphere), so positive
+    # for a source west of the meridian
+    ang = math.atan2(np.cross(tz, tn) @ s, tn @ tz)
+    assert abs((chi - ang + math.pi) % (2 * math.pi) - math.pi) < 1e-9
+    if abs(ha) > 1e-9:
+        assert (chi > 0) == (ha > 0)
