@@ -4,7 +4,8 @@ Produces, under ``mwa-vcsbeam_amd/_lib/``:
 
 * ``libvcsbeam_amd.so`` -- the C-ABI library (HIP kernels + host C module)
   declared in ``include/vcsbeam.h``;
-* ``libvmsynth.so`` -- the seeded synthetic-input generator.
+* ``libvmsynth.so`` -- the seeded synthetic-input generator;
+* ``tied_array_beam`` -- a C caller of the library (examples/tied_array_beam.c).
 
 Run as ``python mwa-vcsbeam_amd/build.py`` or call :func:`build`.
 """
@@ -70,6 +71,13 @@ def build(verbose=False):
     lib = os.path.join(LIB, "libvcsbeam_amd.so")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", lib + ".tmp"] + objs)
     os.replace(lib + ".tmp", lib)
+    # a C caller of the library (examples/tied_array_beam.c), linked against
+    # the installed header and the .so next to it
+    example = os.path.join(LIB, "tied_array_beam")
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I", INC,
+          os.path.join(HERE, "examples", "tied_array_beam.c"), "-L", LIB, "-lvcsbeam_amd",
+          "-Wl,-rpath,$ORIGIN", "-lm", "-o", example + ".tmp"])
+    os.replace(example + ".tmp", example)
     synth = os.path.join(LIB, "libvmsynth.so")
     _run(["gcc", "-O3", "-fPIC", "-shared", "-fopenmp", "-o", synth + ".tmp",
           os.path.join(CSRC, "vmsynth.c"), "-lm"])

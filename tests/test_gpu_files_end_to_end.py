@@ -236,3 +236,41 @@ def test_bind_obs_data_and_read_calibration(pkg, gpu_lib, tmp_path):
         lib.vmFreeDHost(vm)
     finally:
         lib.destroy_vcsbeam_context(vm)
+
+
+def test_c_caller_matches_python_driver(pkg, gpu_lib, tmp_path):
+    """The C caller examples/tied_array_beam.c (built with the library,
+    include/vcsbeam.h only, the reference's by-file set-up calls) writes the
+    same PSRFITS rows as the Python file driver for the same files."""
+    import subprocess
+    from vcsbeam_amd.apps import make_tied_array_beam_files
+    from vcsbeam_amd.workloads import mwax_voltages
+    rng = np.random.default_rng(14)
+    _metafits(tmp_path / "obs.metafits", rng.uniform(-700, 700, NT), rng.uniform(-700, 700, NT),
+              np.full(NT, 377.0), np.round(rng.uniform(50, 300, NT), 3), flagged=3)
+    data = mwax_voltages(2 * NT, 20, first_block=0, seed=47)
+    hdr = f"HDR_SIZE 4096\nNINPUTS {2 * NT}\n".encode().ljust(4096, b"\0")
+    (tmp_path / f"{GPS}_{GPS}_109.sub").write_bytes(hdr + bytes(data[0].nbytes) + data.tobytes())
+    sol = rng.standard_normal((NT, 128, 2, 2)) + 1j * rng.standard_normal((NT, 128, 2, 2))
+    _offringa(tmp_path / "cal.bin", sol)
+    (tmp_path / "pointings.txt").write_text("05:34:30.00 -26:42:00.00\n06:00:00.00 -20:00:00.00\n")
+    print("Python file driver", flush=True)
+    py = make_tied_array_beam_files(tmp_path / "obs.metafits", tmp_path, tmp_path / "py",
+                                    str(tmp_path / "pointings.txt"), None,
+                                    cal=("offringa", tmp_path / "cal.bin"), vdif=False)
+    exe = os.path.join(os.path.dirname(pkg.LIBPATH), "tied_array_beam")
+    (tmp_path / "c").mkdir()
+    print("C caller", flush=True)
+    r = subprocess.run([exe, str(tmp_path / "obs.metafits"), str(tmp_path),
+                        str(tmp_path / "pointings.txt"), str(tmp_path / "cal.bin"),
+                        str(tmp_path / "c"), "109", "+0", "1"],
+                       capture_output=True, text=True, timeout=120, env=dict(os.environ))
+    assert r.returncode == 0, r.stderr[-2000:]
+    cfiles = r.stdout.split()
+    assert len(cfiles) == 2
+    for p in range(2):
+        (_, _), (sub_py, tab_py) = read_fits(pathlib_of(py["psrfits"][p] + "_0001.fits"))
+        (_, _), (sub_c, tab_c) = read_fits(pathlib_of(cfiles[p] + "_0001.fits"))
+        assert os.path.basename(cfiles[p]) == os.path.basename(py["psrfits"][p])
+        assert sub_py["NAXIS2"] == sub_c["NAXIS2"] == 1
+        assert tab_py == tab_c
