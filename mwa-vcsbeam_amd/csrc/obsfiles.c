@@ -150,7 +150,7 @@ static void fits_data_size(fits_hdu *h)
     fits_long(h, "PCOUNT", &pcount);
     fits_long(h, "GCOUNT", &gcount);
     for (long a = 1; a <= naxis; a++) {
-        char k[16];
+        char k[32];
         long v = 0;
         snprintf(k, sizeof(k), "NAXIS%ld", a);
         fits_long(h, k, &v);
@@ -186,7 +186,7 @@ static int fits_column(const fits_hdu *h, const char *name, fits_col *col)
     fits_long(h, "TFIELDS", &nf);
     int offset = 0;
     for (long i = 1; i <= nf; i++) {
-        char k[16];
+        char k[32];
         snprintf(k, sizeof(k), "TFORM%ld", i);
         char *form = fits_string(h, k);
         if (!form) VM_FATAL("metafits: TILEDATA has no %s", k);

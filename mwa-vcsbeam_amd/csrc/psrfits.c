@@ -462,10 +462,10 @@ void vmClosePsrfits(vm_psrfits *pf)
     static const char zeros[FITS_BLOCK] = {0};
     fseek(f, end, SEEK_SET);
     fwrite(zeros, 1, (size_t)((FITS_BLOCK - end % FITS_BLOCK) % FITS_BLOCK), f);
-    char c[96];
+    char c[40];
     snprintf(c, sizeof(c), "%-8.8s= %20d", "NAXIS2", pf->rows_in_file);
     char card[81];
-    snprintf(card, sizeof(card), "%-80s", c);
+    snprintf(card, sizeof(card), "%-80.80s", c);
     fseek(f, pf->naxis2_pos, SEEK_SET);
     fwrite(card, 1, 80, f);
     fclose(f);
