@@ -78,6 +78,9 @@ typedef struct Rfinput {
     uint32_t vcs_order;
     uint32_t subfile_order;
     bool flagged;
+    /* mwalib's per-input analog beamformer delays: num_delays (16) steps of
+     * 435 ps, 32 marking a dead dipole; NULL when the metadata has none */
+    uint32_t *dipole_delays;
 } Rfinput;
 
 typedef struct CoarseChannel {
@@ -111,6 +114,7 @@ typedef struct MetafitsMetadata {
     uint32_t corr_fine_chan_width_hz;
     uint64_t good_time_gps_ms;      /* first good data (metafits GOODTIME) */
     size_t num_metafits_timesteps;  /* 1 s timesteps from the scheduled start */
+    size_t num_delays;              /* dipoles per tile (16), the length of dipole_delays */
 } MetafitsMetadata;
 
 typedef struct VoltageMetadata {
@@ -364,6 +368,36 @@ void vmHaDecToAzEl(double ha, double dec, double phi, double *az, double *el);
 /* include/vcsbeam.h.in:1201-1205: P = [-cos chi, -sin chi; sin chi, -cos chi]
  * for the parallactic angle chi of (az, za) at latitude lat (radians) */
 void parallactic_angle_correction(double *P, double lat, double az, double za);
+
+/* ---------------------------------------------------------- primary beam */
+/* include/vcsbeam.h.in:1181-1183 */
+#define NCONFIGS           138
+#define DEAD_CONFIG        (NCONFIGS - 1)
+#define MANY_DEAD_DIPOLES  -1
+/* Seconds per analog beamformer delay step (the unit of dipole_delays) */
+#define MWA_DELAY_STEP_S   435e-12
+#define MWA_DEAD_DIPOLE_DELAY 32
+
+/* include/vcsbeam.h.in:1195-1199 (src/primary_beam.c:250-297): per RF input
+ * the dipole delays (pointers into the metadata) and amplitudes (1, or 0
+ * for a dipole whose delay is 32); free_delays_amps releases them. */
+void create_delays_amps_from_metafits(MetafitsMetadata *metafits_metadata, uint32_t ***delays,
+                                      double ***amps);
+void free_delays_amps(MetafitsMetadata *metafits_metadata, uint32_t **delays, double **amps);
+/* include/vcsbeam.h.in:1207 (src/primary_beam.c:355-414): index of a
+ * live/dead dipole configuration of 16 amplitudes (0 = dead): 0 for none
+ * dead, 1 + d for one, 16 d1 - (d1 + 2)(d1 + 1)/2 + d2 + 17 for two
+ * (d1 < d2), 137 for all sixteen, MANY_DEAD_DIPOLES otherwise. */
+int hash_dipole_config(double *amps);
+/* include/vcsbeam.h.in:1215 (src/primary_beam.c:479-532): the beam Jones
+ * J of (az, za, freq_hz) for the given dipole delays and amplitudes (the
+ * analytic model; `beam` is unused and may be NULL), optionally rotated
+ * J <- P J, and the response IQUV[4] to pure Stokes I, Q, U and V skies:
+ * I = Re(C00 + C11)/2, Q = Re(C00 - C11)/2, U = Re C01, V = -Im C01 for the
+ * coherency C = J S J^H of each sky S. */
+void calc_normalised_beam_response(void *beam, double az, double za, double freq_hz,
+                                   uint32_t *delays, double *amps, double *IQUV,
+                                   gpuDoubleComplex *J, bool apply_pa_correction);
 
 /* PSRFITS search-mode output: the header and sub-integration fields that
  * populate_psrfits_header / populate_spliced_psrfits_header set
@@ -698,7 +732,29 @@ void vmDestroyCudaStreams(vcsbeam_context *vm);
 void vmCreateStatistics(vcsbeam_context *vm, mpi_psrfits *mpfs);
 void vmDestroyStatistics(vcsbeam_context *vm);
 void vmSetPolIdxLists(vcsbeam_context *vm);
+/* include/vcsbeam.h.in:1209-1211 (src/primary_beam.c:79-191): B[p][a] for
+ * every pointing and tile, one beam evaluation per dipole configuration
+ * (hash_dipole_config) and pointing, times the parallactic-angle rotation
+ * on the right (B = J P).  The beam is this library's analytic 4 x 4
+ * dipole-array model at the coarse-channel centre (the reference's FEE
+ * model, hyperbeam, is not part of this library): dipole amplitudes from
+ * create_delays_amps_from_metafits, analog delays dipole_delays x 435 ps,
+ * normalised to the all-live array at zenith.  Each configuration keeps
+ * its own beam (the reference's cache aliases one buffer, so a tile whose
+ * configuration was seen before receives the last beam evaluated); tiles
+ * with more than two dead dipoles are evaluated with their own
+ * amplitudes and also rotated; a tile whose sixteen dipoles are dead gets
+ * B = 0 (the reference leaves it unset).  Without dipole delays in the
+ * metadata B stays the identity. */
+void vmCalcB(vcsbeam_context *vm, beam_geom *beam_geom_vals);
 void vmCalcJ(vcsbeam_context *vm);
+/* include/vcsbeam.h.in:789 (src/jones.c:170-188): the beam geometry of
+ * every pointing at the middle of the second about to be beamformed
+ * (MJD = sched_start_mjd + (t + first GPS second - obs_id + 0.5) / 86400,
+ * t = chunk_to_load / chunks_per_second), then vmCalcPhi, vmCalcB and
+ * vmCalcJ.  Call it after vmReadNextSecond and before vmBeamformSecond. */
+void vmCalcJonesAndDelays(vcsbeam_context *vm, double *ras_hours, double *decs_degs,
+                          beam_geom *beam_geom_vals);
 void vmSetNumPointings(vcsbeam_context *vm, unsigned int npointings);
 void vmSetNumNotFlaggedRFInputs(vcsbeam_context *vm);
 /* include/vcsbeam.h.in (src/metadata.c:1656-1705): "RA Dec" string pairs

@@ -47,7 +47,8 @@ class Rfinput(C.Structure):
                 ("tile_name", C.c_char_p), ("pol", C.c_char_p),
                 ("electrical_length_m", C.c_double), ("north_m", C.c_double),
                 ("east_m", C.c_double), ("height_m", C.c_double),
-                ("vcs_order", C.c_uint32), ("subfile_order", C.c_uint32), ("flagged", C.c_bool)]
+                ("vcs_order", C.c_uint32), ("subfile_order", C.c_uint32), ("flagged", C.c_bool),
+                ("dipole_delays", C.POINTER(C.c_uint32))]
 
 
 class CoarseChannel(C.Structure):
@@ -67,7 +68,7 @@ class MetafitsMetadata(C.Structure):
                 ("ra_tile_pointing_deg", C.c_double), ("dec_tile_pointing_deg", C.c_double),
                 ("project_id", C.c_char_p), ("num_corr_fine_chans_per_coarse", C.c_size_t),
                 ("corr_fine_chan_width_hz", C.c_uint32), ("good_time_gps_ms", C.c_uint64),
-                ("num_metafits_timesteps", C.c_size_t)]
+                ("num_metafits_timesteps", C.c_size_t), ("num_delays", C.c_size_t)]
 
 
 class VoltageMetadata(C.Structure):
@@ -255,8 +256,8 @@ LAYOUT_FIELDS = {
                                             "metafits_coarse_chans", "coarse_chan_width_hz",
                                             "project_id", "num_corr_fine_chans_per_coarse",
                                             "corr_fine_chan_width_hz", "good_time_gps_ms",
-                                            "num_metafits_timesteps"]),
-    "Rfinput": (Rfinput, ["pol", "electrical_length_m", "vcs_order", "flagged"]),
+                                            "num_metafits_timesteps", "num_delays"]),
+    "Rfinput": (Rfinput, ["pol", "electrical_length_m", "vcs_order", "flagged", "dipole_delays"]),
     "gpu_ipfb_arrays": (GpuIpfbArrays, ["in_real", "d_out"]),
     "vm_psrfits": (Psrfits, ["hdr", "basefilename", "fptr", "data_start"]),
     "vdifinfo": (Vdifinfo, ["samples_per_frame", "stationid", "block_size", "b_scales", "fctr",
@@ -283,6 +284,15 @@ _SIGS = {
     "vmSetPolIdxLists": (None, [_CTX]),
     "vmPushPolIdxLists": (None, [_CTX]),
     "vmCalcJ": (None, [_CTX]),
+    "vmCalcB": (None, [_CTX, C.POINTER(BeamGeom)]),
+    "vmCalcJonesAndDelays": (None, [_CTX, C.c_void_p, C.c_void_p, C.POINTER(BeamGeom)]),
+    "hash_dipole_config": (C.c_int, [C.c_void_p]),
+    "calc_normalised_beam_response": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_bool]),
+    "create_delays_amps_from_metafits": (None, [C.POINTER(MetafitsMetadata), C.c_void_p,
+                                                C.c_void_p]),
+    "free_delays_amps": (None, [C.POINTER(MetafitsMetadata), C.c_void_p, C.c_void_p]),
     "vmCalcPhi": (None, [_CTX, C.POINTER(BeamGeom)]),
     "vmCalcPhiDevice": (None, [_CTX, C.POINTER(BeamGeom)]),
     "vmCalcJDevice": (None, [_CTX]),

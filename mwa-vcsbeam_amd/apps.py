@@ -200,8 +200,10 @@ def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, 
     vdifio: metafits + MWAX subfiles in, calibration from cal = ("offringa",
     file) or ("rts", directory) (None: identity D for unflagged tiles),
     pointings as J2000 RA (hours) / Dec (degrees) (or ras_hours = a pointings
-    file of "RA Dec" strings, decs_degs ignored), beam geometry at the
-    first second (lines 139-147), FINEPFB analysis and MIRROR synthesis
+    file of "RA Dec" strings, decs_degs ignored), header geometry at the
+    first second (lines 139-147) and the beamformer's geometry, primary
+    beam and Jones inverses at the middle of every second
+    (vmCalcJonesAndDelays, line 257), FINEPFB analysis and MIRROR synthesis
     filters from the runtime directory, PSRFITS (IQUV, 8-bit, one file per
     pointing and max_sec_per_file seconds) and VDIF (pointing 0, as the
     reference's inverse PFB) written under outdir.  Returns the output base
@@ -283,15 +285,15 @@ def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, 
         fb = C.create_string_buffer(str(flags_file).encode())
         keep.append(fb)
         v.cal.flags_file = C.cast(fb, C.c_void_p)
+    # custom-flagged tiles get D = 0; tiles the metadata flags keep their
+    # solutions and only change w (src/calibration.c:952-1013,
+    # src/metadata.c:1554-1561)
     lib.vmSetCustomTileFlags(vm)
-    # zero D also for tiles the metadata flags
-    for i in range(md.num_rf_inputs):
-        if md.rf_inputs[i].flagged:
-            D[md.rf_inputs[i].ant] = 0
     lib.vmCreatePrimaryBeam(vm)
     lib.vmCreateGeometricDelays(vm)
     lib.vmCreateCudaStreams(vm)
-    # pointing geometry at the first second
+    # pointing geometry at the first second, for the output headers (lines
+    # 139-147); the beamformer's geometry is recomputed every second
     sec_offset = gps0 - md.sched_start_gps_time_ms / 1000.0
     mjd = md.sched_start_mjd + sec_offset / 86400.0
     bg = (BeamGeom * npointing)()
@@ -316,12 +318,17 @@ def make_tied_array_beam_files(metafits, datadir, outdir, ras_hours, decs_degs, 
         base = os.path.join(str(outdir), vf[0].basefilename.decode())
         vf[0].basefilename = base.encode()
         vdif_buf = np.zeros(nsamples * nchans * npols * 2, np.float32)
-    out = {"S": [], "vdif": [], "psrfits": [], "vdif_base": None}
+    out = {"S": [], "vdif": [], "psrfits": [], "vdif_base": None, "bg_per_second": []}
+    ras_c = np.ascontiguousarray(ras, np.float64)
+    decs_c = np.ascontiguousarray(decs, np.float64)
+    bg_t = (BeamGeom * npointing)()
     for t in range(nseconds):
         if lib.vmReadNextSecond(vm) != VM_SUCCESS:
             raise RuntimeError("vmReadNextSecond failed")
-        lib.vmCalcPhi(vm, bg)
-        lib.vmCalcJ(vm)
+        # geometry, primary beam and Jones inverses at the middle of this
+        # second (app/make_mwa_tied_array_beam.c:257)
+        lib.vmCalcJonesAndDelays(vm, _ptr(ras_c), _ptr(decs_c), bg_t)
+        out["bg_per_second"].append([(bg_t[p].az, bg_t[p].el) for p in range(npointing)])
         lib.vmPushPhi(vm)
         lib.vmPushJ(vm)
         lib.vmBeamformSecond(vm)

@@ -54,12 +54,12 @@ static const int kDims[NLAYER + 1] = {NFEAT, HID, HID, HID, NOUT};
 
 // ------------------------------------------------------------ analytic ----
 // cfg: gains [2][16] complex (re, im) then delays [2][16] seconds (96 doubles)
-__device__ void dipole_beam(double az, double za, double f, const double *cfg, bool nominal,
-                            double2 J[4])
+// (host and device: vmCalcB evaluates it on the host once per configuration)
+__host__ __device__ void dipole_beam(double az, double za, double f, const double *cfg,
+                                     bool nominal, double2 J[4])
 {
-    double sa, ca, sz, cz;
-    sincos(az, &sa, &ca);
-    sincos(za, &sz, &cz);
+    const double sa = sin(az), ca = cos(az);
+    const double sz = sin(za), cz = cos(za);
     const double l = sz * sa, m = sz * ca;
     const double k = 2.0 * M_PI * f / SPEED_OF_LIGHT;
     double afr[2] = {0.0, 0.0}, afi[2] = {0.0, 0.0};
@@ -74,8 +74,7 @@ __device__ void dipole_beam(double az, double za, double f, const double *cfg, b
                 gr = cfg[2 * (P * 16 + d)];
                 gi = cfg[2 * (P * 16 + d) + 1];
             }
-            double s, c;
-            sincos(ph, &s, &c);
+            const double s = sin(ph), c = cos(ph);
             afr[P] += gr * c - gi * s;
             afi[P] += gr * s + gi * c;
         }
@@ -736,6 +735,102 @@ long vmSurrogateJones(vm_beam_surrogate *s, int mode, const double *az, const do
     HIPCHECK(hipFree(df));
     HIPCHECK(hipFree(dB));
     return nfall;
+}
+
+// ------------------------------------------------------ primary beam B ----
+// The analytic beam of one tile: dipole amplitudes (1 live, 0 dead; NULL all
+// live) on both polarisations and its analog delays in 435 ps steps (NULL:
+// zero), evaluated in FP64 on the host.
+static void tile_beam(double az, double za, double freq_hz, const uint32_t *delays,
+                      const double *amps, gpuDoubleComplex J[4])
+{
+    double cfg[96];
+    for (int P = 0; P < 2; P++)
+        for (int d = 0; d < 16; d++) {
+            const double a = amps ? amps[d] : 1.0;
+            cfg[2 * (P * 16 + d)] = a;
+            cfg[2 * (P * 16 + d) + 1] = 0.0;
+            cfg[64 + P * 16 + d] = (delays && a != 0.0) ? delays[d] * MWA_DELAY_STEP_S : 0.0;
+        }
+    double2 j[4];
+    dipole_beam(az, za, freq_hz, cfg, false, j);
+    for (int q = 0; q < 4; q++) J[q] = gpuDoubleComplex{j[q].x, j[q].y};
+}
+
+void vmCalcB(vcsbeam_context *vm, beam_geom *beam_geom_vals)
+{
+    // src/primary_beam.c:79-191: per pointing, one beam per dipole
+    // configuration (right-multiplied by the parallactic rotation), copied to
+    // every tile of that configuration
+    primary_beam *pb = &vm->pb;
+    if (!pb->B) VM_FATAL("vmCalcB: no primary beam allocated (vmCreatePrimaryBeam)");
+    if (!pb->delays || !pb->amps) return; // no dipole data: B as created (identity)
+    MetafitsMetadata *md = pb->obs_metadata;
+    if (md->num_delays != 16)
+        VM_FATAL("vmCalcB: %zu dipole delays per tile (16 expected)", md->num_delays);
+    const size_t nant = pb->nant, npol = pb->npol;
+    gpuDoubleComplex cache[NCONFIGS][4];
+    bool have[NCONFIGS];
+    for (uintptr_t p = 0; p < pb->npointings; p++) {
+        memset(have, 0, sizeof(have));
+        const double az = beam_geom_vals[p].az, za = M_PI / 2 - beam_geom_vals[p].el;
+        double P[4];
+        parallactic_angle_correction(P, MWA_LATITUDE_RADIANS, az, za);
+        for (size_t rf = 0; rf < md->num_rf_inputs; rf++) {
+            if (*(md->rf_inputs[rf].pol) == 'Y') continue; // Y shares X's dipoles
+            const uint32_t ant = md->rf_inputs[rf].ant;
+            if (ant >= nant) VM_FATAL("vmCalcB: antenna %u of %zu", ant, nant);
+            if (!pb->delays[rf]) VM_FATAL("vmCalcB: RF input %zu has no dipole delays", rf);
+            gpuDoubleComplex *out = &pb->B[PB_IDX(p, ant, 0, nant, npol)];
+            const int k = hash_dipole_config(pb->amps[rf]);
+            if (k == NCONFIGS - 1) { // every dipole dead: no response
+                for (int q = 0; q < 4; q++) out[q] = gpuDoubleComplex{0.0, 0.0};
+                continue;
+            }
+            if (k >= 0 && have[k]) {
+                cp2x2(cache[k], out);
+                continue;
+            }
+            gpuDoubleComplex Jb[4];
+            tile_beam(az, za, pb->freq_hz, pb->delays[rf], pb->amps[rf], Jb);
+            mult2x2d_CxR(Jb, P, out);
+            if (k >= 0) {
+                cp2x2(out, cache[k]);
+                have[k] = true;
+            }
+        }
+    }
+}
+
+void calc_normalised_beam_response(void *beam, double az, double za, double freq_hz,
+                                   uint32_t *delays, double *amps, double *IQUV,
+                                   gpuDoubleComplex *J, bool apply_pa_correction)
+{
+    // src/primary_beam.c:479-532 with the analytic beam in place of the FEE
+    // model: J (optionally P J), then the responses to the four pure-Stokes
+    // skies S: C = J S J^H
+    (void)beam;
+    tile_beam(az, za, freq_hz, delays, amps, J);
+    if (apply_pa_correction) {
+        double P[4];
+        parallactic_angle_correction(P, MWA_LATITUDE_RADIANS, az, za);
+        mult2x2d_RxC(P, J, J);
+    }
+    // sky coherencies: I = diag(1/2, 1/2), Q = diag(1/2, -1/2),
+    // U = [[0, 1/2], [1/2, 0]], V = [[0, -i/2], [i/2, 0]]
+    const gpuDoubleComplex z = {0.0, 0.0}, h = {0.5, 0.0}, mh = {-0.5, 0.0};
+    const gpuDoubleComplex sky[4][4] = {{h, z, z, h}, {h, z, z, mh}, {z, h, h, z},
+                                        {z, {0.0, -0.5}, {0.0, 0.5}, z}};
+    gpuDoubleComplex JH[4], SJH[4], Cm[4];
+    calc_hermitian(J, JH);
+    for (int st = 0; st < 4; st++) {
+        mult2x2d((gpuDoubleComplex *)sky[st], JH, SJH);
+        mult2x2d(J, SJH, Cm);
+        if (st == 0) IQUV[0] = 0.5 * (Cm[0].x + Cm[3].x);
+        else if (st == 1) IQUV[1] = 0.5 * (Cm[0].x - Cm[3].x);
+        else if (st == 2) IQUV[2] = Cm[1].x;
+        else IQUV[3] = -Cm[1].y;
+    }
 }
 
 } // extern "C"

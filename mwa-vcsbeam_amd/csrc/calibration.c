@@ -82,7 +82,7 @@ void vmLoadOffringaSolution(vcsbeam_context *vm)
         VM_FATAL("vmLoadOffringaSolution: calibration metadata, file or channel not set");
     const int cidx = vm->cal_coarse_chan_idxs_to_process[0];
     FILE *fp = fopen(vm->cal.caldir, "rb");
-    if (!fp) VM_FATAL("Failed to open %s", vm->cal.caldir);
+    if (!fp) VM_FATAL("vmLoadOffringaSolution: cannot open %s", vm->cal.caldir);
     uint32_t counts[4];
     if (fseek(fp, 16, SEEK_SET) != 0 || fread(counts, sizeof(uint32_t), 4, fp) != 4)
         VM_FATAL("vmLoadOffringaSolution: %s: short header", vm->cal.caldir);
@@ -91,20 +91,20 @@ void vmLoadOffringaSolution(vcsbeam_context *vm)
     size_t nchan = cm->num_corr_fine_chans_per_coarse;
     size_t nChan = nchan * (size_t)(vm->mpi_size > 0 ? vm->mpi_size : 1);
     if (nintervals > 1)
-        fprintf(stdout, "Warning: Only the first interval in the calibration solution (%s) "
-                        "will be used\n", vm->cal.caldir);
+        fprintf(stdout, "Warning: %s holds %u time intervals; using interval 0 only\n",
+                vm->cal.caldir, nintervals);
     if (file_nant != nant)
-        VM_FATAL("Calibration solution (%s) contains a different number of antennas (%u) than "
-                 "specified (%zu)", vm->cal.caldir, file_nant, nant);
+        VM_FATAL("vmLoadOffringaSolution: %s has %u antennas, the calibration metadata %zu",
+                 vm->cal.caldir, file_nant, nant);
     if (file_nchan != nChan) {
-        fprintf(stdout, "Warning: Calibration solution (%s) contains a different number (%u) "
-                        "than the requested (%zu) channels.\n", vm->cal.caldir, file_nchan, nChan);
+        fprintf(stdout, "Warning: %s has %u channels where %zu were expected; using the "
+                        "file's count\n", vm->cal.caldir, file_nchan, nChan);
         nChan = file_nchan;
         nchan = nChan / (size_t)(vm->mpi_size > 0 ? vm->mpi_size : 1);
     }
     if (cidx >= (int)cm->num_metafits_coarse_chans)
-        VM_FATAL("Requested coarse channel number (%d) is more than the number of channels "
-                 "available in the calibration solution (%s)", cidx, vm->cal.caldir);
+        VM_FATAL("vmLoadOffringaSolution: coarse channel index %d is past the %zu channels "
+                 "of %s", cidx, cm->num_metafits_coarse_chans, vm->cal.caldir);
     if (nchan == 0 || vcs_nchan % nchan)
         VM_FATAL("vmLoadOffringaSolution: %zu solution channels per coarse channel do not divide "
                  "%zu fine channels", nchan, vcs_nchan);
@@ -136,7 +136,7 @@ void read_dijones_file(gpuDoubleComplex **Dd, gpuDoubleComplex *A, double *amp, 
      * matrix per antenna (8 doubles each), separated by whitespace or
      * commas. */
     FILE *fp = fopen(fname, "r");
-    if (!fp) VM_FATAL("cannot open gain Jones matrix file: %s", fname);
+    if (!fp) VM_FATAL("read_dijones_file: cannot open %s", fname);
     double val = 0.0, *a = (double *)A, J[8];
     if (fscanf(fp, "%lf", &val) != 1) VM_FATAL("read_dijones_file: %s: no amplitude", fname);
     if (amp) *amp = val;
@@ -164,15 +164,16 @@ void read_bandpass_file(gpuDoubleComplex ***Jm, gpuDoubleComplex ***Jf,
     const int nchan = (int)cal_metadata->num_corr_fine_chans_per_coarse;
     const int nant = (int)cal_metadata->num_ants;
     FILE *f = fopen(filename, "r");
-    if (!f) VM_FATAL("cannot open Bandpass file: %s", filename);
+    if (!f) VM_FATAL("read_bandpass_file: cannot open %s", filename);
     char line[4096];
-    if (!fgets(line, sizeof(line), f)) VM_FATAL("could not read first line of %s", filename);
+    if (!fgets(line, sizeof(line), f)) VM_FATAL("read_bandpass_file: %s is empty", filename);
     int *chan_idxs = (int *)malloc(sizeof(int) * (size_t)(nchan > 0 ? nchan : 1));
     if (!chan_idxs) VM_FATAL("out of host memory");
     int count = 0, pos = 0;
     double mhz;
     for (char *p = line; sscanf(p, "%lf,%n", &mhz, &pos) == 1; p += pos) {
-        if (++count > nchan) VM_FATAL("More than nchan = %d columns in Bandpass file %s", nchan, filename);
+        if (++count > nchan)
+            VM_FATAL("read_bandpass_file: %s lists more than %d channels", filename, nchan);
         const int idx = (int)roundf((float)(mhz * 1e6 / (double)chan_width));
         if (idx < 0 || idx >= nchan) VM_FATAL("Bandpass file %s: channel at %f MHz out of range", filename, mhz);
         chan_idxs[count - 1] = idx;
@@ -186,8 +187,8 @@ void read_bandpass_file(gpuDoubleComplex ***Jm, gpuDoubleComplex ***Jf,
         for (int row = 0; row < 8; row++) {
             if (row > 0) {
                 if (fscanf(f, "%d,", &cur) != 1 || cur - 1 != ant)
-                    VM_FATAL("read_bandpass_file: fewer than eight rows detected for antenna %d "
-                             "(labelled '%d') in %s", ant, ant + 1, filename);
+                    VM_FATAL("read_bandpass_file: %s: antenna %d has %d of its 8 rows",
+                             filename, ant + 1, row);
             }
             gpuDoubleComplex ***J = (row % 2 == 0) ? Jm : Jf;
             if (!J) {
@@ -303,7 +304,7 @@ void parse_calibration_correction_file(uint32_t gpstime, calibration *cal)
     char path[4200], line[4096], tile[4096];
     snprintf(path, sizeof(path), "%s/pq_phase_correction.txt", vm_runtime_dir());
     FILE *f = fopen(path, "r");
-    if (!f) VM_FATAL("Cannot find file '%s'", path);
+    if (!f) VM_FATAL("parse_calibration_correction_file: cannot open %s", path);
     unsigned from, to;
     double slope = 0.0, offset = 0.0, s, o;
     int found = 0;
@@ -344,8 +345,8 @@ void vmApplyCalibrationCorrections(vcsbeam_context *vm)
     } else {
         aref = ant_by_tile_name(om, vm->cal.ref_ant);
         if (aref < 0)
-            VM_FATAL("apply_calibration_corrections: tile %s does not exist in this observation",
-                     vm->cal.ref_ant);
+            VM_FATAL("vmApplyCalibrationCorrections: reference tile %s is not in the "
+                     "observation", vm->cal.ref_ant);
     }
     if (!apply_ref && !apply_zero && !apply_slope) return;
     const size_t nant = om->num_ants, nchan = (size_t)vm->nfine_chan;
@@ -378,7 +379,7 @@ void vmParseFlaggedTilenamesFile(char *filename, calibration *cal)
 {
     /* whitespace-separated tile names */
     FILE *f = fopen(filename, "r");
-    if (!f) VM_FATAL("vmParseFlaggedTilenamesFile: could not open file '%s' for reading", filename);
+    if (!f) VM_FATAL("vmParseFlaggedTilenamesFile: cannot open %s", filename);
     char name[MAX_COMMAND_LENGTH];
     int n = 0;
     while (fscanf(f, "%1023s", name) == 1) n++;
@@ -418,8 +419,8 @@ void vmSetCustomTileFlags(vcsbeam_context *vm)
         const long a = ant_by_tile_name(om, vm->cal.flagged_tilenames[i]);
         if (a < 0) {
             snprintf(vm->log_message, sizeof(vm->log_message),
-                     "Warning: Tile '%s' (listed in '%s') not found\n", vm->cal.flagged_tilenames[i],
-                     vm->cal.flags_file);
+                     "Warning: %s names tile %s, which the observation does not have\n",
+                     vm->cal.flags_file, vm->cal.flagged_tilenames[i]);
             if (vm->log) logger_message(vm->log, vm->log_message);
             continue;
         }

@@ -15,7 +15,7 @@
  *   - the metafits primary keys GPSTIME, MJD, PROJECT, MODE, CHANNELS,
  *     BANDWDTH, NCHANS, FINECHAN, RA, DEC, NINPUTS and the TILEDATA table
  *     columns Input, Antenna, Tile, TileName, Pol, Flag, Length ("EL_<m>"),
- *     North, East, Height;
+ *     North, East, Height, and Delays (16 dipole delay steps) when present;
  *   - MWAX VCS subfiles <obsid>_<subobs>_<rec>.sub: a 4096-byte ASCII header
  *     ("KEY value" lines; HDR_SIZE and NINPUTS are checked), one delay block
  *     the size of a data block, then 160 data blocks of 50 ms (8 s),
@@ -355,7 +355,14 @@ int vmLoadMetafits(const char *path, MetafitsMetadata *md, VoltageMetadata *vmd)
     fits_col *cols[] = {&cin, &cant, &ctile, &cname, &cpol, &cflag, &clen, &cn, &ce, &ch};
     for (int k = 0; k < 10; k++)
         if (!fits_column(&th, need[k], cols[k])) VM_FATAL("vmLoadMetafits: TILEDATA has no column %s", need[k]);
+    /* optional: the analog beamformer's dipole delays (16 integers per
+     * input, 32 = dead dipole) */
+    fits_col cdel;
+    const int have_delays = fits_column(&th, "Delays", &cdel);
+    if (have_delays && (cdel.count < 1 || tform_size(cdel.type) < 1 || cdel.type == 'A'))
+        VM_FATAL("vmLoadMetafits: TILEDATA column Delays has type %d%c", cdel.count, cdel.type);
     free(th.cards);
+    md->num_delays = have_delays ? (size_t)cdel.count : 0;
     md->num_rf_inputs = (size_t)nrows;
     md->num_ants = (size_t)nrows / 2;
     md->rf_inputs = (Rfinput *)calloc((size_t)nrows, sizeof(Rfinput));
@@ -384,6 +391,13 @@ int vmLoadMetafits(const char *path, MetafitsMetadata *md, VoltageMetadata *vmd)
         while (*lp == ' ') lp++;
         if (strncmp(lp, "EL_", 3) == 0) lp += 3;
         rf->electrical_length_m = strtod(lp, NULL);
+        if (have_delays) {
+            rf->dipole_delays = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cdel.count);
+            if (!rf->dipole_delays) VM_FATAL("out of host memory");
+            const int sz = tform_size(cdel.type);
+            for (int d = 0; d < cdel.count; d++)
+                rf->dipole_delays[d] = (uint32_t)be_num(row + cdel.offset + d * sz, cdel.type);
+        }
         /* MWAX subfiles hold inputs by antenna then polarisation; legacy
          * recombined data by the PFB's interleave of each board's 64 inputs
          * (whole boards only: with a partial board the order is the input) */
@@ -418,6 +432,7 @@ void vmFreeMetafits(MetafitsMetadata *md)
     for (size_t i = 0; md->rf_inputs && i < md->num_rf_inputs; i++) {
         free(md->rf_inputs[i].tile_name);
         free(md->rf_inputs[i].pol);
+        free(md->rf_inputs[i].dipole_delays);
     }
     free(md->rf_inputs);
     free(md->metafits_coarse_chans);

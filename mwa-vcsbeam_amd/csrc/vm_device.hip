@@ -315,14 +315,26 @@ void vmPushPhi(vcsbeam_context *vm)
 
 void vmCreatePrimaryBeam(vcsbeam_context *vm)
 {
-    /* src/primary_beam.c:79-127 allocates B [p][a][npol*npol]; the FEE model
-     * itself (hyperbeam) is outside this library, so B starts as the
-     * identity and the caller may overwrite it */
+    /* src/primary_beam.c:200-229 allocates B [p][a][npol*npol] and takes
+     * the dipole delays and amplitudes from the metadata, at the
+     * coarse-channel centre frequency.  B starts as the identity: vmCalcB
+     * replaces it from the dipole configurations (when the metadata has
+     * dipole delays), or the caller may overwrite it */
     primary_beam *pb = &vm->pb;
     pb->npointings = vm->npointing;
     pb->nant = vm->obs_metadata->num_ants;
     pb->npol = vm->obs_metadata->num_visibility_pols;
     pb->obs_metadata = vm->obs_metadata;
+    pb->beam = NULL;
+    pb->delays = NULL;
+    pb->amps = NULL;
+    if (vm->obs_metadata->num_delays > 0 && vm->obs_metadata->num_rf_inputs > 0 &&
+        vm->obs_metadata->rf_inputs[0].dipole_delays)
+        create_delays_amps_from_metafits(vm->obs_metadata, &pb->delays, &pb->amps);
+    pb->freq_hz = (vm->obs_metadata->metafits_coarse_chans && vm->coarse_chan_idx >= 0 &&
+                   (size_t)vm->coarse_chan_idx < vm->obs_metadata->num_metafits_coarse_chans)
+                      ? vm->obs_metadata->metafits_coarse_chans[vm->coarse_chan_idx].chan_centre_hz
+                      : 0;
     size_t n = pb->npointings * pb->nant * pb->npol;
     /* pinned, as it is copied to the device every second (vmCalcJDevice) */
     pb->B = (gpuDoubleComplex *)vm_host_alloc_pinned(n * sizeof(gpuDoubleComplex));
@@ -335,10 +347,14 @@ void vmCreatePrimaryBeam(vcsbeam_context *vm)
 
 void free_primary_beam(primary_beam *pb)
 {
-    /* src/primary_beam.c (free_primary_beam): releases B */
+    /* src/primary_beam.c (free_primary_beam): releases B and the dipole
+     * delays and amplitudes */
     if (!pb) return;
     vm_host_free_pinned(pb->B);
     pb->B = NULL;
+    if (pb->amps) free_delays_amps(pb->obs_metadata, pb->delays, pb->amps);
+    pb->delays = NULL;
+    pb->amps = NULL;
 }
 
 void vmCreateCudaStreams(vcsbeam_context *vm)

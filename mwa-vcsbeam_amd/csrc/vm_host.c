@@ -312,6 +312,70 @@ void vmCalcJ(vcsbeam_context *vm)
             }
 }
 
+/* src/jones.c:170-188: geometry at the middle of the current second, then
+ * phases, primary beam and Jones inverses */
+void vmCalcJonesAndDelays(vcsbeam_context *vm, double *ras_hours, double *decs_degs,
+                          beam_geom *beam_geom_vals)
+{
+    logger_start_stopwatch(vm->log, "delay", true);
+    const uintptr_t t = (uintptr_t)(vm->chunk_to_load / vm->chunks_per_second);
+    const double sec_offset =
+        (double)t + (double)vm->gps_seconds_to_process[0] - (double)vm->obs_metadata->obs_id;
+    const double mjd = vm->obs_metadata->sched_start_mjd + (sec_offset + 0.5) / 86400.0;
+    for (unsigned int p = 0; p < vm->npointing; p++)
+        calc_beam_geom(ras_hours[p], decs_degs[p], mjd, &beam_geom_vals[p]);
+    vmCalcPhi(vm, beam_geom_vals);
+    vmCalcB(vm, beam_geom_vals);
+    vmCalcJ(vm);
+    logger_stop_stopwatch(vm->log, "delay");
+}
+
+/* ====================================================== primary beam ==== */
+/* src/primary_beam.c:250-297: delays point into the metadata; amplitudes
+ * are 0 for a dead dipole (delay 32) and 1 otherwise */
+void create_delays_amps_from_metafits(MetafitsMetadata *md, uint32_t ***delays, double ***amps)
+{
+    const size_t ni = md->num_rf_inputs, nd = md->num_delays;
+    *delays = (uint32_t **)xmalloc(sizeof(uint32_t *) * (ni ? ni : 1));
+    *amps = (double **)xmalloc(sizeof(double *) * (ni ? ni : 1));
+    for (size_t i = 0; i < ni; i++) {
+        (*delays)[i] = md->rf_inputs[i].dipole_delays;
+        (*amps)[i] = (double *)xmalloc(sizeof(double) * (nd ? nd : 1));
+        for (size_t d = 0; d < nd; d++)
+            (*amps)[i][d] = md->rf_inputs[i].dipole_delays &&
+                                    md->rf_inputs[i].dipole_delays[d] == MWA_DEAD_DIPOLE_DELAY
+                                ? 0.0
+                                : 1.0;
+    }
+}
+
+void free_delays_amps(MetafitsMetadata *md, uint32_t **delays, double **amps)
+{
+    if (amps)
+        for (size_t i = 0; i < md->num_rf_inputs; i++) free(amps[i]);
+    free(amps);
+    free(delays);
+}
+
+/* src/primary_beam.c:355-414.  Configurations are numbered by how many of
+ * the 16 dipoles are dead: none (0), one at d (1 + d), two at d1 < d2
+ * (17 onwards, pairs in lexicographic order: the pairs before d1 number
+ * 15 + 14 + ... = 16 d1 - d1 (d1 + 1) / 2, so the index is
+ * 17 + 16 d1 - d1 (d1 + 1) / 2 + (d2 - d1 - 1)), all sixteen (137). */
+int hash_dipole_config(double *amps)
+{
+    int dead[16], ndead = 0;
+    for (int i = 0; i < 16; i++)
+        if (amps[i] == 0.0) dead[ndead++] = i;
+    switch (ndead) {
+    case 0: return 0;
+    case 1: return 1 + dead[0];
+    case 2: return 17 + 16 * dead[0] - dead[0] * (dead[0] + 1) / 2 + (dead[1] - dead[0] - 1);
+    case 16: return NCONFIGS - 1;
+    default: return MANY_DEAD_DIPOLES;
+    }
+}
+
 static void check_rf_inputs(const MetafitsMetadata *md);
 
 /* vmSetPolIdxLists, src/jones.c:40-64 */
@@ -809,12 +873,14 @@ int vmStructLayout(const char *name, size_t *out, int n)
         PUT(OFF(MetafitsMetadata, corr_fine_chan_width_hz));
         PUT(OFF(MetafitsMetadata, good_time_gps_ms));
         PUT(OFF(MetafitsMetadata, num_metafits_timesteps));
+        PUT(OFF(MetafitsMetadata, num_delays));
     } else if (strcmp(name, "Rfinput") == 0) {
         PUT(sizeof(Rfinput));
         PUT(OFF(Rfinput, pol));
         PUT(OFF(Rfinput, electrical_length_m));
         PUT(OFF(Rfinput, vcs_order));
         PUT(OFF(Rfinput, flagged));
+        PUT(OFF(Rfinput, dipole_delays));
     } else if (strcmp(name, "gpu_ipfb_arrays") == 0) {
         PUT(sizeof(struct gpu_ipfb_arrays));
         PUT(OFF(struct gpu_ipfb_arrays, in_real));

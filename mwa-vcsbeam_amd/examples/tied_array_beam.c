@@ -50,16 +50,11 @@ int main(int argc, char **argv)
     vmReadCalibration(vm);
     parse_calibration_correction_file(vm->gps_seconds_to_process[0], &vm->cal);
     vmApplyCalibrationCorrections(vm);
-    /* tiles the metadata flags carry no weight */
-    for (size_t i = 0; i < vm->obs_metadata->num_rf_inputs; i++)
-        if (vm->obs_metadata->rf_inputs[i].flagged)
-            memset(&vm->D[D_IDX(vm->obs_metadata->rf_inputs[i].ant, 0, 0, 0, vm->nfine_chan, 2)], 0,
-                   sizeof(gpuDoubleComplex) * 4 * vm->nfine_chan);
     vmCreatePrimaryBeam(vm);
     vmCreateGeometricDelays(vm);
     vmCreateCudaStreams(vm);
 
-    /* geometry at the first second (app/make_mwa_tied_array_beam.c:139-147) */
+    /* header geometry at the first second (app/make_mwa_tied_array_beam.c:139-147) */
     const double sec_offset =
         vm->gps_seconds_to_process[0] - vm->obs_metadata->sched_start_gps_time_ms / 1000.0;
     const double mjd = vm->obs_metadata->sched_start_mjd + sec_offset / 86400.0;
@@ -80,8 +75,9 @@ int main(int argc, char **argv)
             fprintf(stderr, "no data for second %d\n", s);
             return 1;
         }
-        vmCalcPhi(vm, bg);
-        vmCalcJ(vm);
+        /* geometry, beam and Jones inverses at the middle of this second
+         * (app/make_mwa_tied_array_beam.c:257) */
+        vmCalcJonesAndDelays(vm, vm->ras_hours, vm->decs_degs, bg);
         vmPushPhi(vm);
         vmPushJ(vm);
         vmBeamformSecond(vm);

@@ -89,7 +89,12 @@ class Observation:
 
     def __init__(self, ntiles=128, blocks_per_second=20, nseconds=1, nflagged=4,
                  rec_chan=133, legacy=False, filter_coeffs=None, obs_id=1300000000,
-                 seed_offset=0, geometry_seed_offset=None):
+                 seed_offset=0, geometry_seed_offset=None, dipole_delays=None,
+                 dead_dipoles=None):
+        """dipole_delays: 16 analog beamformer delay steps shared by every
+        tile (None: the metadata carries no dipole delays, so the primary
+        beam stays the identity); dead_dipoles: {tile: [dipole, ...]}, each
+        listed dipole's delay set to 32 (dead)."""
         self.ntiles = ntiles
         self.ninput = 2 * ntiles
         self.blocks_per_second = blocks_per_second
@@ -135,6 +140,15 @@ class Observation:
             rf[i].vcs_order = int(self.vcs_order[i])
             rf[i].subfile_order = i
             rf[i].flagged = bool(self.flag_mask[a])
+        self.dipole_delays = None
+        if dipole_delays is not None:
+            dd = np.tile(np.asarray(dipole_delays, np.uint32).reshape(1, 16), (ntiles, 1))
+            for a, dead in (dead_dipoles or {}).items():
+                dd[a, list(dead)] = 32
+            self.dipole_delays = np.ascontiguousarray(dd)
+            for i in range(I):
+                rf[i].dipole_delays = self.dipole_delays[i // 2].ctypes.data_as(
+                    C.POINTER(C.c_uint32))
         self.rf = rf
         cc = (CoarseChannel * 1)()
         cc[0].corr_chan_number = 0
@@ -163,6 +177,7 @@ class Observation:
         md.ra_tile_pointing_deg = 0.0
         md.dec_tile_pointing_deg = -26.7
         md.project_id = self._proj
+        md.num_delays = 16 if dipole_delays is not None else 0
         self.md = md
         vmd = VoltageMetadata()
         vmd.mwa_version = md.mwa_version

@@ -222,7 +222,7 @@ def test_unknown_reference_tile_exits(pkg):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
                        timeout=120)
-    assert r.returncode != 0 and "Tile999 does not exist" in r.stderr
+    assert r.returncode != 0 and "reference tile Tile999 is not in the observation" in r.stderr
 
 
 @pytest.mark.parametrize("gps, ref, slope, offset", [

@@ -5,7 +5,8 @@ make_tied_array_beam_files, which writes PSRFITS (IQUV) and VDIF.  The same
 second, metadata, filters and D are also run through the in-memory path
 (tied_array_beam), which the other GPU tests hold to the FP64 oracle:
 - Stokes agree to 1e-6 (the only difference is the unit vectors, from
-  calc_beam_geom's az/el here and from az/za there);
+  calc_beam_geom's az/el at the middle of the second here and from az/za
+  there);
 - the PSRFITS rows dequantise to those Stokes within one quantisation step;
 - the VDIF payload is the oracle's requantisation of the inverse-PFB floats,
   its headers the oracle's for that second.
@@ -25,7 +26,9 @@ NT = 8
 GPS = 1300000000
 
 
-def _metafits(path, east, north, height, cable, flagged, mode="MWAX_VCS"):
+def _metafits(path, east, north, height, cable, flagged, mode="MWAX_VCS", delays=None):
+    """delays: optional [NT][16] dipole delay steps per tile (32 = dead),
+    written as the TILEDATA column Delays."""
     n = 2 * NT
     ant = np.repeat(np.arange(NT), 2)
     pol = ["X", "Y"] * NT
@@ -44,6 +47,8 @@ def _metafits(path, east, north, height, cable, flagged, mode="MWAX_VCS"):
             ("Length", "14A", [f"EL_{float(cable[a]):.3f}".encode().ljust(14) for a in ant]),
             ("North", "E", f4(north[ant])), ("East", "E", f4(east[ant])),
             ("Height", "E", f4(height[ant]))]
+    if delays is not None:
+        cols.append(("Delays", "16I", np.ascontiguousarray(np.asarray(delays)[ant], ">i2")))
     path.write_bytes(_pad(b"".join(prim)) + _bintable("TILEDATA", cols, n))
 
 
@@ -96,13 +101,14 @@ def test_make_tied_array_beam_files(pkg, orc, gpu_lib, tmp_path):
     lib.vmLoadMetafits(str(tmp_path / "obs.metafits").encode(), C.byref(md), C.byref(vmd))
     D = sol.copy()
     D[..., 0, 1] = D[..., 1, 0] = 0           # cross terms zeroed (keep_cross_terms off)
-    D[flagged] = 0
+    # the metafits-flagged tile keeps its solution (it changes w only)
     runtime = pkg.RUNTIME_DIR
     fine = np.loadtxt(os.path.join(runtime, "FINEPFB.dat"))
     mirror = np.loadtxt(os.path.join(runtime, "MIRROR.dat"))
     print("same second in memory", flush=True)
-    ref = tied_array_beam(_Obs(md, vmd, data, fine, D), np.array([bg.az]),
-                          np.array([math.pi / 2 - bg.el]), pfb_flags=557, ipfb=True,
+    az0, el0 = out["bg_per_second"][0][0]      # the middle of the second
+    ref = tied_array_beam(_Obs(md, vmd, data, fine, D), np.array([az0]),
+                          np.array([math.pi / 2 - el0]), pfb_flags=557, ipfb=True,
                           synth_filter=mirror, ipfb_device=True, want_e=False)
     lib.vmFreeMetafits(C.byref(md))
     S, Sref = out["S"][0][0], ref["S"][0][0]            # [s][st][c]
@@ -274,3 +280,91 @@ def test_c_caller_matches_python_driver(pkg, gpu_lib, tmp_path):
         assert os.path.basename(cfiles[p]) == os.path.basename(py["psrfits"][p])
         assert sub_py["NAXIS2"] == sub_c["NAXIS2"] == 1
         assert tab_py == tab_c
+
+
+def test_tied_array_files_three_seconds_against_oracle(pkg, orc, gpu_lib, tmp_path):
+    """Three seconds through make_tied_array_beam_files against the FP64
+    oracle chain, each second's expected geometry computed here, not by the
+    driver:
+    - the beam geometry of second t is calc_beam_geom at
+      MJD = sched_start_mjd + (t + first GPS second - obs_id + 0.5) / 86400
+      (src/jones.c:170-188), so the phases follow the sky over the seconds;
+    - the metafits carries dipole delays with dead dipoles on three tiles
+      (two configurations besides all-live), so B per tile is the analytic
+      beam of the tile's configuration times the parallactic rotation, built
+      here from the FP64 oracle's beam;
+    - tile 5 is flagged in the metafits and its Offringa solution is finite:
+      it keeps D (metafits flags change w only, src/metadata.c:1554-1561);
+    - second t's PFB reads the last block of second t - 1 as its margin.
+    The FP32 Stokes must be within 1e-5 of the oracle in every second."""
+    from vcsbeam_amd import BeamGeom
+    from vcsbeam_amd.apps import make_tied_array_beam_files
+    from vcsbeam_amd.workloads import mwax_voltages, unit_vectors
+    lib = gpu_lib
+    rng = np.random.default_rng(21)
+    nsec = 3
+    east, north = rng.uniform(-700, 700, NT), rng.uniform(-700, 700, NT)
+    height = 377.0 + rng.uniform(-2, 2, NT)
+    cable = np.round(rng.uniform(50, 300, NT), 3)
+    flagged = 5
+    delays = np.tile(np.array([0, 1, 2, 3] * 4), (NT, 1))
+    delays[2, 6] = 32                     # one dead dipole
+    delays[4, 6] = 32                     # the same configuration
+    delays[6, [1, 14]] = 32               # two dead dipoles
+    _metafits(tmp_path / "obs.metafits", east, north, height, cable, flagged, delays=delays)
+    data = mwax_voltages(2 * NT, 20 * nsec, first_block=0, seed=33)
+    hdr = f"HDR_SIZE 4096\nNINPUTS {2 * NT}\n".encode().ljust(4096, b"\0")
+    (tmp_path / f"{GPS}_{GPS}_109.sub").write_bytes(hdr + bytes(data[0].nbytes) + data.tobytes())
+    sol = rng.standard_normal((NT, 128, 2, 2)) + 1j * rng.standard_normal((NT, 128, 2, 2))
+    _offringa(tmp_path / "cal.bin", sol)
+    ra_h, dec_d = 11.0, -20.0             # about 70 deg elevation at MJD 59000.5
+    print(f"files -> PSRFITS, {nsec} seconds", flush=True)
+    out = make_tied_array_beam_files(tmp_path / "obs.metafits", tmp_path, tmp_path / "out",
+                                     [ra_h], [dec_d], nseconds=nsec,
+                                     cal=("offringa", tmp_path / "cal.bin"), vdif=False)
+    D = sol.copy()
+    D[..., 0, 1] = D[..., 1, 0] = 0       # keep_cross_terms off; tile 5 keeps D
+    fine = np.loadtxt(os.path.join(pkg.RUNTIME_DIR, "FINEPFB.dat")).astype(np.int32)
+    centre = 109 * 1280000
+    freqs = centre - 640000 + 10000.0 * np.arange(128)
+    ninput = 2 * NT
+    out_idx = np.arange(ninput, dtype=np.int32)           # vcs_order = input for 16 inputs
+    polQ, polP = np.arange(0, ninput, 2, dtype=np.uint32), np.arange(1, ninput, 2, dtype=np.uint32)
+    invw = 1.0 / (2 * (NT - 1))
+    els = []
+    for t in range(nsec):
+        mjd = 59000.5 + (t + 0.5) / 86400.0
+        bg = BeamGeom()
+        lib.calc_beam_geom(ra_h, dec_d, mjd, C.byref(bg))
+        az, el = bg.az, bg.el
+        els.append(el)
+        assert abs(out["bg_per_second"][t][0][0] - az) < 1e-12
+        assert abs(out["bg_per_second"][t][0][1] - el) < 1e-12
+        za = math.pi / 2 - el
+        P = np.empty(4)
+        lib.parallactic_angle_correction(P.ctypes.data, C.c_double(-0.4660608448386394),
+                                         C.c_double(az), C.c_double(za))
+        B = np.empty((1, NT, 2, 2), complex)
+        for a in range(NT):
+            amps = (delays[a] != 32).astype(float)
+            cfg = np.zeros(96)
+            for q in range(2):
+                cfg[q * 32:(q + 1) * 32:2] = amps
+                cfg[64 + 16 * q:64 + 16 * (q + 1)] = np.where(amps != 0, delays[a] * 435e-12, 0)
+            B[0, a] = orc.dipole_beam(np.array([az]), np.array([za]), np.array([float(centre)]),
+                                      cfg)[0] @ P.reshape(2, 2)
+        margin = np.zeros_like(data[:1]) if t == 0 else data[20 * t - 1:20 * t]
+        htr = np.ascontiguousarray(np.concatenate([margin, data[20 * t:20 * (t + 1)]], axis=0))
+        v = orc.pfb_forward(htr, ninput, 10000, fine, out_idx, fpga=True)
+        # positions as the metafits stores them (float32)
+        f32 = lambda x: np.asarray(x, np.float32).astype(np.float64)
+        phi = orc.calc_phi(f32(east), f32(north), f32(height), cable, unit_vectors([az], [za]),
+                           freqs)
+        Sref, _ = orc.beamform(v, orc.calc_J(D, B), phi, polQ, polP, invw, 4)
+        err = orc.rel_err(out["S"][t][0], Sref[0])
+        print(f"second {t}: el {el:.9f} rad, Stokes rel err vs FP64 {err:.2e}", flush=True)
+        assert err <= 1e-5
+    # the sky moved: 1 s of Earth rotation is ~7.3e-5 rad
+    assert abs(els[2] - els[0]) > 1e-5
+    # the beams of the three configurations differ, so a wrong B would show
+    assert np.abs(B[0, 0] - B[0, 2]).max() > 1e-3 and np.abs(B[0, 2] - B[0, 6]).max() > 1e-3

@@ -4,8 +4,10 @@ Workload: 128 tiles x 2 pols, 0.1 s, PFB in full precision (VM_FLT), 40
 directions within 15 deg of boresight (an uneven count: 160 GEMM rows padded
 to 256).  The FP16 form (FP16 voltages with a per-channel power-of-two scale,
 split hi+lo FP16 weights, FP32 accumulation) must reach <= 1e-3 norm-wise
-relative error on Stokes I of every beam against the FP64 oracle chain; BF16
-(8-bit significand) is held to 2e-2.
+relative error on Stokes I of every beam against the FP64 oracle chain.  BF16
+(8-bit significand on the voltages) does not reach that bar: it measures
+2.4e-3 on every Stokes parameter and is held to 3.6e-3 (1.5x), so that a
+regression shows.
 """
 import numpy as np
 import pytest
@@ -42,7 +44,7 @@ def _errors(orc, S, Sref):
                      for p in range(S.shape[0])])
 
 
-@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 2e-2, 5e-2)])
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
 def test_mfma_matches_fp64_oracle(pkg, orc, gpu_lib, form, tol_i, tol_quv):
     from vcsbeam_amd.apps import tied_array_beam
     obs, az, za = _setup(pkg)
@@ -106,7 +108,7 @@ def test_every_beamformer_variant_and_tile_order(pkg, orc, gpu_lib, mfma_case, v
         sess.close()
 
 
-@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 2e-2, 5e-2)])
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
 def test_mfma_per_pointing_primary_beam(pkg, orc, gpu_lib, form, tol_i, tol_quv):
     """Every pointing with its own primary beam B[p][c] (the analytic
     dipole-array beam, the same for all tiles), so J^-1 = F B^-1 D^-1

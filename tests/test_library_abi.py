@@ -73,15 +73,13 @@ def test_host_vdif_matches_oracle(pkg, orc):
     np.testing.assert_array_equal(q.view(np.uint8), orc.vdif_requant(x))
 
 
-def test_read_buffer_margin_and_lock(pkg):
+@pytest.mark.gpu
+def test_read_buffer_margin_and_lock(pkg, gpu_lib):
     """vmReadNextSecondFrom copies the previous second's last block into the
-    margin and refuses a second read while locked (src/buffer.c:106-124,
-    src/metadata.c:941-950).  Pure host logic: only pinned host memory is
-    allocated, so this runs without a GPU when the runtime allows pinned
-    allocations; otherwise it is skipped."""
-    lib = pkg.load()
-    if lib.vmDeviceCount() < 1:
-        pytest.skip("pinned host memory needs a HIP device")
+    margin, refuses a second read while locked and reports the end of the
+    data (src/buffer.c:106-124, src/metadata.c:941-950).  Host logic over a
+    pinned host buffer, which needs a HIP device to allocate."""
+    lib = gpu_lib
     from vcsbeam_amd.workloads import Observation
     obs = Observation(ntiles=2, blocks_per_second=2, nseconds=3, nflagged=0)
     vm = lib.vmInit(False)

@@ -134,7 +134,9 @@ def test_demote_half_away_from_zero(orc):
 
 
 def test_unpack_int4_exhaustive(orc):
-    # pack then unpack: every byte decodes to the nibble pair that encodes it
+    # every byte decodes to the two's-complement nibble pair that encodes it:
+    # high nibble first (REAL_NIBBLE_TO_UINT8 = (X >> 4) & 0xf), low nibble
+    # second, include/vcsbeam.h.in:997-1003
     lib = orc.lib()
     import ctypes as C
     re, im = C.c_double(), C.c_double()
@@ -143,7 +145,70 @@ def test_unpack_int4_exhaustive(orc):
         hi, lo = byte >> 4, byte & 15
         assert re.value == (hi - 16 if hi >= 8 else hi)
         assert im.value == (lo - 16 if lo >= 8 else lo)
-        v = np.array([[im.value + 1j * 0]], dtype=np.complex128)  # noqa: F841
+
+
+def _nibble(n):
+    """UINT8_TO_INT of a 4-bit field, stated from two's complement."""
+    return n - 16 if n & 8 else n
+
+
+def test_pack_int4_round_trip_pins_nibble_order(orc):
+    """orc_pack_int4 packs the stored (first, second) component pair as
+    PACK_NIBBLES(first, second): first in the high nibble
+    (include/vcsbeam.h.in:453-456).  Unpacking with the reference's nibble
+    macros must give back (clip(round(first)), clip(round(second))), with
+    round half away from zero and clip to [-8, 7], over every grid pair that
+    covers the whole nibble range, every +-0.5 tie and both clips."""
+    grid = np.unique(np.concatenate([np.arange(-9.5, 9.51, 0.25), [-8.5, -8.49, 7.49, 7.5, 100.0, -100.0]]))
+
+    def expect(x):
+        r = int(Decimal(repr(float(x))).quantize(Decimal(1), rounding=ROUND_HALF_UP))
+        return min(max(r, -8), 7)
+
+    first, second = np.meshgrid(grid, grid, indexing="ij")
+    stored = (first + 1j * second).ravel()
+    packed = orc.pack_int4(stored)
+    hi = np.array([_nibble(int(b) >> 4) for b in packed])
+    lo = np.array([_nibble(int(b) & 0xF) for b in packed])
+    np.testing.assert_array_equal(hi, [expect(x) for x in first.ravel()])
+    np.testing.assert_array_equal(lo, [expect(x) for x in second.ravel()])
+    # the byte is exactly (hi << 4) | lo: no carry from the low nibble
+    assert all(int(b) == ((h & 0xF) << 4 | (l & 0xF)) for b, h, l in zip(packed, hi, lo))
+
+
+def test_pfb_int4_output_keeps_imag_in_high_nibble(orc):
+    """With PFB_IMAG_PART_FIRST unset the reference stores PACK_NIBBLES(im, re)
+    (src/pfb.cpp:325-331): the high nibble holds the imaginary part of the
+    fine channel.  A tone whose fine-channel value is known in closed form
+    fixes which nibble is which: an input x[t] = A e^{2 pi i t k0 / K} through
+    a one-tap-window filter gives, in channel k0, A h_sum (times a known
+    phase) over K, so putting all the energy in the imaginary part leaves the
+    low nibble (real part) zero and the high nibble non-zero."""
+    K, P = 128, 12
+    ninput, nspec = 2, 24
+    h = np.zeros(K * P, np.int32)
+    # only the p = 0 taps of the reversed filter, h[K P - n - 1] for n < K; a
+    # tap of 2^14 cancels the int2float scale of 2^-14 exactly
+    h[K * (P - 1): K * P] = 16384
+    # MWAX layout [block][input][64000][re, im]; two blocks cover spectra
+    # m + 500 - P .. for m < 24
+    htr = np.zeros((2, ninput, 64000, 2), np.int8)
+    htr[:, 0, :, 1] = 3                   # input 0: the constant 0 + 3i
+    out_idx = np.arange(ninput, dtype=np.int32)
+    v = orc.pfb_forward(htr, ninput, nspec, h, out_idx, fpga=False, imag_first=False,
+                        use_fft=False, K=K)
+    dc = v[:, K // 2, 0]                  # fftshift puts DC at channel 64
+    # every spectrum's DC bin is K * 3i before the /K; stored as (im, re)
+    np.testing.assert_allclose(dc.real, 3.0, atol=1e-12)
+    np.testing.assert_allclose(dc.imag, 0.0, atol=1e-12)
+    # every other channel is empty
+    assert np.abs(np.delete(v[:, :, 0], K // 2, axis=1)).max() < 1e-12
+    packed = orc.pack_int4(dc)
+    assert all(_nibble(int(b) >> 4) == 3 and (int(b) & 0xF) == 0 for b in packed)
+    # with PFB_IMAG_PART_FIRST the components come out as (re, im)
+    w = orc.pfb_forward(htr, ninput, nspec, h, out_idx, fpga=False, imag_first=True,
+                        use_fft=False, K=K)
+    np.testing.assert_allclose(w[:, K // 2, 0], 3j, atol=1e-12)
 
 
 def test_incoh_invariants(orc):
