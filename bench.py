@@ -408,7 +408,8 @@ def pipeline_stage(pkg, lib, nchan=24, steps=2):
 def aux_stage(pkg, lib, reps=5):
     """Row kernels outside the beamformer at one coarse channel-second of 128
     tiles: the incoherent beam (cu_form_incoh_beam on 4+4-bit legacy input
-    [s][c][i]: the 4+4 unpack and the power sum) and the 8-bit renormalise of
+    [s][c][i]: the 4+4 unpack, the power sum and its 8-bit renormalisation,
+    timed together) and the 8-bit renormalise of
     one beam's Stokes IQUV (vmSendSToFits), each with its HBM roofline."""
     import ctypes as C
     from vcsbeam_amd.apps import TiedArraySession, device_second
@@ -431,7 +432,9 @@ def aux_stage(pkg, lib, reps=5):
     incoh_ms, _ = kernel_ms_global(lib, "incoh")
     lib.vmSetKernelTiming(None, 0)
     lib.free_input_output_arrays(h, d)
-    incoh_bytes = data.nbytes + ns * nch * 4
+    # the whole device part of the call: 4+4 data read, I written, I read
+    # twice by the renormalisation (extremes, then quantisation), 8-bit out
+    incoh_bytes = data.nbytes + ns * nch * (4 + 4 + 4 + 1)
     obs = Observation(ntiles=128, blocks_per_second=20)
     ses = TiedArraySession(obs, np.array([0.7]), np.radians([20.0]),
                            pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA, nstokes=4,

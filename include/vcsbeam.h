@@ -1110,8 +1110,8 @@ void vmEventDestroy(void *ev);
  * "beamform_prep", "beamform_weights", "ipfb", "geometry", "renormalise") is
  * bracketed by events; vmKernelTimeMs returns the average milliseconds per
  * launch since timing was switched on.  With vm NULL they apply to the calls
- * that take no context: "incoh" (cu_form_incoh_beam's sum) and "vdif"
- * (vmRequantVDIFDevice). */
+ * that take no context: "incoh" (cu_form_incoh_beam's device part: the
+ * sum and the renormalisation) and "vdif" (vmRequantVDIFDevice). */
 void vmSetKernelTiming(vcsbeam_context *vm, int on);
 double vmKernelTimeMs(vcsbeam_context *vm, const char *kernel, long *count);
 

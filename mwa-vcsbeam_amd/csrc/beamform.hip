@@ -18,6 +18,7 @@
 //    [c][s/64][p][st][s%64], d_e [p][c][s][pol]) so stores are coalesced;
 //    vmPullS / vmPullE transpose to the reference host layouts.
 #include <hip/hip_runtime.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -169,84 +170,153 @@ __device__ __forceinline__ int incoh_sq4(uint32_t w)
                                   false);
 }
 
+// Rows are taken 64 at a time per workgroup: 4 per 16-lane group, each
+// lane with its 4 row loads in flight together.
 __global__ __launch_bounds__(256) void k_incoh(const uint8_t *__restrict__ data, int nsample,
                                                int nchan, int ninput, float *__restrict__ incoh)
 {
-    const int g = threadIdx.x & 15;
+    const int g = threadIdx.x & 15, rl = threadIdx.x >> 4;
     const size_t nrows = (size_t)nsample * nchan;
-    const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 4;
-    for (size_t row = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; row < nrows;
-         row += stride) {
-        const uint8_t *r = data + row * ninput;
-        int acc = 0; // 256 x the row's sum
+    const size_t stride = (size_t)gridDim.x * 64;
+    for (size_t r0 = (size_t)blockIdx.x * 64 + rl; r0 < nrows; r0 += stride) {
+        int acc[4] = {0, 0, 0, 0}; // 256 x each row's sum
         if ((ninput & 15) == 0) {
             for (int j = 16 * g; j < ninput; j += 256) {
-                const uint4 w = *(const uint4 *)(r + j);
-                acc += incoh_sq4(w.x) + incoh_sq4(w.y) + incoh_sq4(w.z) + incoh_sq4(w.w);
+                uint4 w[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const size_t row = r0 + 16 * k;
+                    w[k] = row < nrows ? *(const uint4 *)(data + row * ninput + j)
+                                       : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    acc[k] += incoh_sq4(w[k].x) + incoh_sq4(w[k].y) + incoh_sq4(w[k].z) +
+                              incoh_sq4(w[k].w);
             }
         } else {
-            for (int j = g; j < ninput; j += 16) acc += incoh_sq4((uint32_t)r[j]);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const size_t row = r0 + 16 * k;
+                if (row < nrows)
+                    for (int j = g; j < ninput; j += 16)
+                        acc[k] += incoh_sq4((uint32_t)data[row * ninput + j]);
+            }
         }
 #pragma unroll
-        for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
-        if (g == 0) incoh[row] = (float)(acc >> 8);
+        for (int k = 0; k < 4; k++) {
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, 16);
+            const size_t row = r0 + 16 * k;
+            if (g == 0 && row < nrows) incoh[row] = (float)(acc[k] >> 8);
+        }
     }
 }
 
 // --------------------------------------------------------- renormalise ----
-// Reference layout S[p][s][st][c] (used by cu_form_incoh_beam).
-// block (64 channels, 16 slices); grid (ceil(nchan/64), nstokes, npointing).
-__global__ __launch_bounds__(1024) void k_renorm_ref(const float *__restrict__ S, int nstep,
-                                                     int nstokes, int nchan,
-                                                     float *__restrict__ offsets,
-                                                     float *__restrict__ scales,
-                                                     uint8_t *__restrict__ Ss)
+// 8-bit renormalisation of each channel (src/form_beam.cpp:408-449): over
+// the samples of a channel, scale = (max - min) / 256, offset = min and
+// q = (uint8)((S - offset) / scale + 0.5), the + 0.5 in double as the
+// reference has it.  The reference's cast is undefined at S = max (256.5)
+// and for scale = 0; those saturate here (NaN -> 0).
+__device__ __forceinline__ uint8_t renorm_q(float S, float offset, float scale)
 {
-    __shared__ float smin[16][64], smax[16][64];
-    const int cl = threadIdx.x, sl = threadIdx.y;
-    const int c = blockIdx.x * 64 + cl, st = blockIdx.y, p = blockIdx.z;
-    const bool ok = c < nchan;
-    float mn = 0.f, mx = 0.f;
-    if (ok) {
-        size_t base = ((size_t)p * nstep * nstokes + st) * nchan + c;
-        size_t stride = (size_t)nstokes * nchan;
-        mn = mx = S[base];
-        for (int s = sl; s < nstep; s += 16) {
-            float val = S[base + s * stride];
-            mn = fminf(mn, val);
-            mx = fmaxf(mx, val);
+    const float val = (S - offset) / scale;
+    double r = (double)val + 0.5;
+    r = (r != r) ? 0.0 : fmin(fmax(r, 0.0), 255.0);
+    return (uint8_t)r;
+}
+
+// Incoherent beam I[s][c] (the reference layout, one pointing, Stokes I):
+// three short passes that each spread over the whole device.
+// 1. partial min/max per (sample split, channel): block 64 channels x 4
+//    sample lanes, grid (ceil(nchan / 64), nsplit); split q holds samples
+//    [q sps, min((q + 1) sps, nsample)) and writes part[q][c] = (min, max)
+//    for every c < nchan, so part needs no clearing.
+__global__ __launch_bounds__(256) void k_incoh_minmax(const float *__restrict__ I, int nsample,
+                                                      int nchan, int sps, float2 *__restrict__ part)
+{
+    __shared__ float smn[4][64], smx[4][64];
+    const int cl = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cl, q = blockIdx.y;
+    const int s0 = q * sps, s1 = min(s0 + sps, nsample);
+    float mn = INFINITY, mx = -INFINITY;
+    if (c < nchan)
+        for (int s = s0 + sl; s < s1; s += 4) {
+            const float v = I[(size_t)s * nchan + c];
+            mn = fminf(mn, v);
+            mx = fmaxf(mx, v);
+        }
+    smn[sl][cl] = mn;
+    smx[sl][cl] = mx;
+    __syncthreads();
+    if (sl == 0 && c < nchan) {
+#pragma unroll
+        for (int k = 1; k < 4; k++) {
+            mn = fminf(mn, smn[k][cl]);
+            mx = fmaxf(mx, smx[k][cl]);
+        }
+        part[(size_t)q * nchan + c] = make_float2(mn, mx);
+    }
+}
+
+// 2. per channel: the splits' extremes -> offset and scale; block 16
+//    channels x 16 lanes, each lane folding every 16th split
+__global__ __launch_bounds__(256) void k_incoh_scale(const float2 *__restrict__ part, int nsplit,
+                                                     int nchan, float *__restrict__ offsets,
+                                                     float *__restrict__ scales)
+{
+    __shared__ float smn[16][17], smx[16][17];
+    const int cl = threadIdx.x & 15, ql = threadIdx.x >> 4;
+    const int c = blockIdx.x * 16 + cl;
+    float mn = INFINITY, mx = -INFINITY;
+    if (c < nchan) {
+        int q = ql;
+        for (; q + 48 < nsplit; q += 64) { // four loads in flight
+            const float2 m0 = part[(size_t)q * nchan + c], m1 = part[(size_t)(q + 16) * nchan + c];
+            const float2 m2 = part[(size_t)(q + 32) * nchan + c];
+            const float2 m3 = part[(size_t)(q + 48) * nchan + c];
+            mn = fminf(mn, fminf(fminf(m0.x, m1.x), fminf(m2.x, m3.x)));
+            mx = fmaxf(mx, fmaxf(fmaxf(m0.y, m1.y), fmaxf(m2.y, m3.y)));
+        }
+        for (; q < nsplit; q += 16) {
+            const float2 m = part[(size_t)q * nchan + c];
+            mn = fminf(mn, m.x);
+            mx = fmaxf(mx, m.y);
         }
     }
-    smin[sl][cl] = mn;
-    smax[sl][cl] = mx;
+    smn[ql][cl] = mn;
+    smx[ql][cl] = mx;
     __syncthreads();
-    if (sl == 0 && ok) {
-        for (int q = 1; q < 16; q++) {
-            mn = fminf(mn, smin[q][cl]);
-            mx = fmaxf(mx, smax[q][cl]);
+    if (ql == 0 && c < nchan) {
+        for (int k = 1; k < 16; k++) {
+            mn = fminf(mn, smn[k][cl]);
+            mx = fmaxf(mx, smx[k][cl]);
         }
-        smin[0][cl] = mn;
-        smax[0][cl] = mx;
+        scales[c] = (mx - mn) / 256.0f;
+        offsets[c] = mn;
     }
-    __syncthreads();
-    if (!ok) return;
-    mn = smin[0][cl];
-    mx = smax[0][cl];
-    // src/form_beam.cpp:436-446
-    float scale = (mx - mn) / 256.0f, offset = mn;
-    size_t base = ((size_t)p * nstep * nstokes + st) * nchan + c;
-    size_t stride = (size_t)nstokes * nchan;
-    for (int s = sl; s < nstep; s += 16) {
-        float val = (S[base + s * stride] - offset) / scale;
-        float r = val + 0.5f;
-        // the reference's cast is undefined at r >= 256 (val == max) and for
-        // scale == 0; saturate instead
-        r = (r != r) ? 0.f : fminf(fmaxf(r, 0.f), 255.f);
-        Ss[base + s * stride] = (uint8_t)r;
-    }
-    if (sl == 0) {
-        scales[((size_t)p * nstokes + st) * nchan + c] = scale;
-        offsets[((size_t)p * nstokes + st) * nchan + c] = offset;
+}
+
+// 3. quantise, 4 consecutive elements per lane (one 4-byte store)
+__global__ __launch_bounds__(256) void k_incoh_quant(const float *__restrict__ I, size_t n,
+                                                     int nchan, const float *__restrict__ offsets,
+                                                     const float *__restrict__ scales,
+                                                     uint8_t *__restrict__ Iq)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x * 4;
+    for (size_t j = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; j < n; j += stride) {
+        uint8_t q[4] = {0, 0, 0, 0};
+        const int m = (int)min((size_t)4, n - j);
+        int c = (int)(j % (size_t)nchan); // one division per 4 elements
+        for (int k = 0; k < m; k++) {
+            q[k] = renorm_q(I[j + k], offsets[c], scales[c]);
+            if (++c == nchan) c = 0;
+        }
+        if (m == 4 && ((uintptr_t)(Iq + j) & 3) == 0)
+            *(uchar4 *)(Iq + j) = make_uchar4(q[0], q[1], q[2], q[3]);
+        else
+            for (int k = 0; k < m; k++) Iq[j + k] = q[k];
     }
 }
 
@@ -309,12 +379,8 @@ __global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, 
         uint8_t q = 0;
         if (c < nchan && s < nstep) {
             size_t row = (size_t)pst * ctot + coff + c;
-            float val = (S[VM_S_IDX(p, st, c, s, npointing, nstokes, VM_S_NBLK(nstep))] -
-                         offsets[row]) /
-                        scales[row];
-            float rr = val + 0.5f;
-            rr = (rr != rr) ? 0.f : fminf(fmaxf(rr, 0.f), 255.f);
-            q = (uint8_t)rr;
+            q = renorm_q(S[VM_S_IDX(p, st, c, s, npointing, nstokes, VM_S_NBLK(nstep))],
+                         offsets[row], scales[row]);
         }
         T[r][tx] = q;
     }
@@ -392,6 +458,35 @@ static void *stage_buffer(vcsbeam_context *vm, size_t bytes)
     return d->d_stage;
 }
 
+// The incoherent beam on device buffers, all on stream st: the detection
+// sum, then the three renormalisation passes with their (min, max)
+// partials in part (room for cap pairs; at most 256 sample splits, and at
+// least nchan pairs are needed).
+static void incoh_launch(const uint8_t *d_data, float *d_incoh, unsigned nsample, int nchan,
+                         int ninput, float *d_offsets, float *d_scales, uint8_t *d_Iscaled,
+                         float2 *part, size_t cap, hipStream_t st)
+{
+    const size_t rows = (size_t)nsample * nchan;
+    size_t nblk = (rows + 63) / 64; // 64 rows per workgroup and pass
+    if (nblk > 4096) nblk = 4096;
+    hipLaunchKernelGGL(k_incoh, dim3((unsigned)(nblk > 0 ? nblk : 1)), dim3(256), 0, st, d_data,
+                       (int)nsample, nchan, ninput, d_incoh);
+    if (nsample == 0) return;
+    size_t nsplit = cap / (size_t)nchan;
+    if (nsplit > 256) nsplit = 256;
+    if (nsplit > nsample) nsplit = nsample;
+    const int sps = (int)((nsample + nsplit - 1) / nsplit);
+    nsplit = (nsample + sps - 1) / sps; // no empty split
+    hipLaunchKernelGGL(k_incoh_minmax, dim3((unsigned)((nchan + 63) / 64), (unsigned)nsplit),
+                       dim3(256), 0, st, (const float *)d_incoh, (int)nsample, nchan, sps, part);
+    hipLaunchKernelGGL(k_incoh_scale, dim3((unsigned)((nchan + 15) / 16)), dim3(256), 0, st,
+                       (const float2 *)part, (int)nsplit, nchan, d_offsets, d_scales);
+    const size_t nq = (rows + 1023) / 1024;
+    hipLaunchKernelGGL(k_incoh_quant, dim3((unsigned)(nq < 8192 ? nq : 8192)), dim3(256), 0, st,
+                       (const float *)d_incoh, rows, nchan, (const float *)d_offsets,
+                       (const float *)d_scales, d_Iscaled);
+}
+
 void cu_form_incoh_beam(uint8_t *data, uint8_t *d_data, size_t data_size, float *d_incoh,
                         unsigned int nsample, int nchan, int ninput, float *offsets,
                         float *d_offsets, float *scales, float *d_scales, uint8_t *Iscaled,
@@ -402,18 +497,28 @@ void cu_form_incoh_beam(uint8_t *data, uint8_t *d_data, size_t data_size, float 
         VM_FATAL("cu_form_incoh_beam: %zu data bytes for %u x %d x %d samples, %zu scaled bytes",
                  data_size, nsample, nchan, ninput, Iscaled_size);
     HIPCHECK(hipMemcpy(d_data, data, data_size, hipMemcpyHostToDevice));
-    size_t rows = (size_t)nsample * nchan;
-    vm_timer_start_on(vm_dev_global(), VM_T_INCOH, NULL);
-    // 16 lanes per row; about 16 rows per lane group
-    const size_t want = (rows + 255) / 256;
-    const unsigned nblk = (unsigned)(want < 65536 ? want : 65536);
-    hipLaunchKernelGGL(k_incoh, dim3(nblk), dim3(256), 0, 0, d_data, (int)nsample, nchan, ninput,
-                       d_incoh);
+    if (nchan <= 0 || ninput <= 0) VM_FATAL("cu_form_incoh_beam: %d channels, %d inputs", nchan, ninput);
+    // the renormalisation's partial extremes: a library buffer per device,
+    // kept between calls and grown when a call needs more
+    static void *part[VM_MAX_DEVICES];
+    static size_t part_bytes[VM_MAX_DEVICES];
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= VM_MAX_DEVICES) VM_FATAL("cu_form_incoh_beam: device %d", dev);
+    const size_t need = (size_t)256 * (size_t)nchan * sizeof(float2);
+    if (part_bytes[dev] < need) {
+        if (part[dev]) HIPCHECK(hipFree(part[dev]));
+        HIPCHECK(hipMalloc(&part[dev], need));
+        part_bytes[dev] = need;
+    }
+    vm_device_state *g = vm_dev_global();
+    // the timer covers the whole device part: the sum and the
+    // renormalisation
+    vm_timer_start_on(g, VM_T_INCOH, NULL);
+    incoh_launch(d_data, d_incoh, nsample, nchan, ninput, d_offsets, d_scales, d_Iscaled,
+                 (float2 *)part[dev], part_bytes[dev] / sizeof(float2), 0);
     HIPCHECK(hipGetLastError());
-    vm_timer_stop_on(vm_dev_global(), VM_T_INCOH, NULL);
-    hipLaunchKernelGGL(k_renorm_ref, dim3((nchan + 63) / 64, 1, 1), dim3(64, 16), 0, 0, d_incoh,
-                       (int)nsample, 1, nchan, d_offsets, d_scales, d_Iscaled);
-    HIPCHECK(hipGetLastError());
+    vm_timer_stop_on(g, VM_T_INCOH, NULL);
     HIPCHECK(hipMemcpy(offsets, d_offsets, nchan * sizeof(float), hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(scales, d_scales, nchan * sizeof(float), hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(Iscaled, d_Iscaled, Iscaled_size, hipMemcpyDeviceToHost));

@@ -20,6 +20,9 @@ void vm_fatal(const char *file, int line, const char *fmt, ...);
  * file reader vmGetVoltageMetadata set up (obsfiles.c) */
 void vm_free_obs_contexts(vcsbeam_context *vm);
 
+/* Devices a process may use (per-device tables of library state). */
+#define VM_MAX_DEVICES 64
+
 /* Device Stokes layout d_S: [c][s / VM_SBLK][p][st][s % VM_SBLK] float, the
    second's samples cut into blocks of VM_SBLK, the last block padded.  A
    tile of pointings x VM_SBLK samples is then one contiguous run of memory

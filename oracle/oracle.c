@@ -386,9 +386,11 @@ void orc_renormalise(const float *S, int npointing, int nstep, int nstokes,
                 float scale = (float)((mx - mn) / 256.0);
                 float offset = mn;
                 for (int s = 0; s < nstep; s++) {
+                    /* src/form_beam.cpp:443-444: (uint8_t)(val + 0.5), the
+                     * sum in double */
                     float val = (S[CI(s)] - offset) / scale;
-                    float r = val + 0.5f;
-                    int ok = (scale != 0.0f) && (r < 256.0f) && (r >= 0.0f);
+                    double r = (double)val + 0.5;
+                    int ok = (scale != 0.0f) && (r < 256.0) && (r >= 0.0);
                     Sscaled[CI(s)] = ok ? (uint8_t)r : 0;
                     if (valid) valid[CI(s)] = (uint8_t)ok;
                 }

@@ -99,10 +99,16 @@ def test_stokes_I_only(pkg, orc, gpu_lib):
     _check(orc, out["S"][0], None, Sref, None)
 
 
-def test_incoherent_beam_exact_and_renormalised(pkg, orc, gpu_lib):
+@pytest.mark.parametrize("ns,nc,ni", [(1000, 128, 256), (37, 100, 24), (1, 3, 16), (513, 130, 48),
+                                      (64, 128, 512)])
+def test_incoherent_beam_exact_and_renormalised(pkg, orc, gpu_lib, ns, nc, ni):
+    """The sum over inputs is exact and the renormalisation (extremes per
+    channel, scale, offset, 8-bit values) equals the oracle's; ragged sizes
+    (inputs not a multiple of 16, channel counts not dividing the row
+    stride, one sample) take the general paths."""
     from vcsbeam_amd.apps import incoherent_beam
     from vcsbeam_amd.workloads import legacy_int4
-    ns, nc, ni = 1000, 128, 256
+    print(f"incoherent beam {ns} x {nc} x {ni}", flush=True)
     data = legacy_int4(ns * nc * ni, sigma=2.5)
     incoh, off, scl, Is = incoherent_beam(data, ns, nc, ni)
     ref = orc.incoh(data.reshape(ns, nc, ni), ns, nc, ni)
