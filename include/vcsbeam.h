@@ -862,6 +862,24 @@ void cu_form_incoh_beam(uint8_t *data, uint8_t *d_data, size_t data_size,
                         float *offsets, float *d_offsets, float *scales,
                         float *d_scales, uint8_t *Iscaled, uint8_t *d_Iscaled,
                         size_t Iscaled_size);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the device
+ * part of cu_form_incoh_beam on device buffers only, enqueued on `stream`
+ * (NULL: the null stream) with no synchronisation, no copies and no
+ * allocation.  d_data [nsample][nchan][ninput] 4+4-bit (data_size bytes);
+ * writes d_incoh [nsample][nchan], d_offsets and d_scales [nchan] and
+ * d_Iscaled [nsample][nchan] (Iscaled_size bytes, at least nsample x nchan).
+ * The renormalisation's partial extremes go to a fixed device array of
+ * VM_INCOH_SCRATCH_PAIRS (min, max) pairs, 512 KiB, which each GPU the
+ * library runs on holds until the process ends; it limits nchan to
+ * VM_INCOH_SCRATCH_PAIRS.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other threads) race.
+ * Returns 0, or -1 with a message on stderr when it rejects the sizes. */
+#define VM_INCOH_SCRATCH_PAIRS 65536
+int vmFormIncohBeamNoAlloc(const uint8_t *d_data, size_t data_size, float *d_incoh,
+                           unsigned int nsample, int nchan, int ninput, float *d_offsets,
+                           float *d_scales, uint8_t *d_Iscaled, size_t Iscaled_size,
+                           gpuStream_t stream);
 void vmApplyJChunk(vcsbeam_context *vm);
 void vmBeamformChunk(vcsbeam_context *vm);
 void vmBeamformSecond(vcsbeam_context *vm);

@@ -23,6 +23,7 @@ VCSLegacyRecombined = 4
 VCSMWAXv2 = 5
 VM_INT4, VM_DBL, VM_FLT = 0, 1, 2
 VM_SUCCESS, VM_END_OF_DATA, VM_READ_BUFFER_NOT_SET, VM_READ_BUFFER_LOCKED = 0, 1, 2, 3
+VM_INCOH_SCRATCH_PAIRS = 65536  # partial extremes the no-allocation incoherent beam holds
 ANALYSIS_FILTER, SYNTHESIS_FILTER = 0, 1
 PFB_MALLOC_HOST_INPUT = 0x01
 PFB_MALLOC_HOST_OUTPUT = 0x02
@@ -284,6 +285,9 @@ _SIGS = {
     "vmSetPolIdxLists": (None, [_CTX]),
     "vmPushPolIdxLists": (None, [_CTX]),
     "vmCalcJ": (None, [_CTX]),
+    "vmFormIncohBeamNoAlloc": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_uint, C.c_int,
+                                         C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                         C.c_void_p]),
     "vmCalcB": (None, [_CTX, C.POINTER(BeamGeom)]),
     "vmCalcJonesAndDelays": (None, [_CTX, C.c_void_p, C.c_void_p, C.POINTER(BeamGeom)]),
     "hash_dipole_config": (C.c_int, [C.c_void_p]),

@@ -524,6 +524,62 @@ void cu_form_incoh_beam(uint8_t *data, uint8_t *d_data, size_t data_size, float 
     HIPCHECK(hipMemcpy(Iscaled, d_Iscaled, Iscaled_size, hipMemcpyDeviceToHost));
 }
 
+// ------------------------------------------------ no-allocation form ----
+// cu_form_incoh_beam for callers that cannot allocate device memory: the
+// renormalisation's partial extremes live in this fixed array, one copy per
+// device.  Host code reaches it only through the address that
+// hipGetSymbolAddress gives on each device (kept in g_scratch), never
+// through the array's name.
+static __device__ __align__(256) float2 scratch[VM_INCOH_SCRATCH_PAIRS];
+static void *g_scratch[VM_MAX_DEVICES];
+
+// this device's address of scratch, looked up on the first call on the
+// device (with that device current); NULL on failure
+static float2 *incoh_scratch(int *dev_out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= VM_MAX_DEVICES) return NULL;
+    *dev_out = dev;
+    void *p = __atomic_load_n(&g_scratch[dev], __ATOMIC_ACQUIRE);
+    if (!p) {
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return NULL;
+        __atomic_store_n(&g_scratch[dev], p, __ATOMIC_RELEASE);
+    }
+    return (float2 *)p;
+}
+
+int vmFormIncohBeamNoAlloc(const uint8_t *d_data, size_t data_size, float *d_incoh,
+                           unsigned int nsample, int nchan, int ninput, float *d_offsets,
+                           float *d_scales, uint8_t *d_Iscaled, size_t Iscaled_size,
+                           gpuStream_t stream)
+{
+    if (nchan <= 0 || nchan > VM_INCOH_SCRATCH_PAIRS || ninput <= 0) {
+        fprintf(stderr, "vmFormIncohBeamNoAlloc: %d channels of %d inputs (at most %d channels)\n",
+                nchan, ninput, VM_INCOH_SCRATCH_PAIRS);
+        return -1;
+    }
+    const size_t rows = (size_t)nsample * (size_t)nchan;
+    if (data_size < rows * (size_t)ninput || Iscaled_size < rows) {
+        fprintf(stderr, "vmFormIncohBeamNoAlloc: %zu data bytes and %zu scaled bytes for %u x %d "
+                        "x %d samples\n", data_size, Iscaled_size, nsample, nchan, ninput);
+        return -1;
+    }
+    int dev = -1;
+    float2 *part = incoh_scratch(&dev);
+    if (!part) {
+        fprintf(stderr, "vmFormIncohBeamNoAlloc: no scratch address on device %d\n", dev);
+        return -1;
+    }
+    incoh_launch(d_data, d_incoh, nsample, nchan, ninput, d_offsets, d_scales, d_Iscaled, part,
+                 VM_INCOH_SCRATCH_PAIRS, (hipStream_t)stream);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "vmFormIncohBeamNoAlloc: %s\n", hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
 // Rebuild W = phi * J^-1 when J or phi changed (once per second).
 void vm_build_weights(vcsbeam_context *vm)
 {
