@@ -131,6 +131,8 @@ __device__ __forceinline__ double2 zmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
 
 // ||D B||_F^2 of row-major 2x2 complex matrices
 __device__ __forceinline__ double frob2_DB(const double2 *D, const double2 *B)
@@ -147,20 +149,29 @@ __device__ __forceinline__ double frob2_DB(const double2 *D, const double2 *B)
     return f;
 }
 
-// Per (c, a): D^-1 as float [c][a][4] (zero for a singular D, whose J^-1 is
-// zero) and g of P = D^H D as float4 [c][a].  D: [a][c][4] complex double.
-__global__ void k_pb_dinv(const double2 *__restrict__ D, int nant, int nchan, cf *__restrict__ Dinv,
+// Per (c, GEMM slot a): D^-1 as float [c][a][4] (zero for a singular D,
+// whose J^-1 is zero) and g of P = D^H D as float4 [c][a], for slots a <
+// nslot standing for antenna amap[a] (amap NULL: a itself; amap[a] < 0: a
+// padding slot, zero).  D: [antenna][c][4] complex double.
+__global__ void k_pb_dinv(const double2 *__restrict__ D, int nslot, int nchan,
+                          const int *__restrict__ amap, cf *__restrict__ Dinv,
                           float4 *__restrict__ g)
 {
     const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= (size_t)nant * nchan) return;
+    if (j >= (size_t)nslot * nchan) return;
     const int c = (int)(j % nchan), a = (int)(j / nchan);
-    const double2 *m = D + ((size_t)a * nchan + c) * 4;
+    const int as = amap ? amap[a] : a;
+    cf *o = Dinv + ((size_t)c * nslot + a) * 4;
+    if (as < 0) {
+        o[0] = o[1] = o[2] = o[3] = cf{0.f, 0.f};
+        g[(size_t)c * nslot + a] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const double2 *m = D + ((size_t)as * nchan + c) * 4;
     const double2 d00 = m[0], d01 = m[1], d10 = m[2], d11 = m[3];
     const double2 det = make_double2(d00.x * d11.x - d00.y * d11.y - (d01.x * d10.x - d01.y * d10.y),
                                      d00.x * d11.y + d00.y * d11.x - (d01.x * d10.y + d01.y * d10.x));
     const double dd = det.x * det.x + det.y * det.y;
-    cf *o = Dinv + ((size_t)c * nant + a) * 4;
     if (dd == 0.0) {
         o[0] = o[1] = o[2] = o[3] = cf{0.f, 0.f};
     } else {
@@ -179,7 +190,7 @@ __global__ void k_pb_dinv(const double2 *__restrict__ D, int nant, int nchan, cf
     const double p11 = d01.x * d01.x + d01.y * d01.y + d11.x * d11.x + d11.y * d11.y;
     const double p01r = d00.x * d01.x + d00.y * d01.y + d10.x * d11.x + d10.y * d11.y;
     const double p01i = d00.x * d01.y - d00.y * d01.x + d10.x * d11.y - d10.y * d11.x;
-    g[(size_t)c * nant + a] = make_float4((float)p00, (float)p11, (float)p01r, (float)p01i);
+    g[(size_t)c * nslot + a] = make_float4((float)p00, (float)p11, (float)p01r, (float)p01i);
 }
 
 // Per (c, p): h of Q = B B^H and the products of the entries of B^-1 that
@@ -232,6 +243,146 @@ __global__ void k_pb_beam(const double2 *__restrict__ D, const double2 *__restri
     esp[(size_t)c * np + p] = 13 - ex;
 }
 
+// ------------------------------------------- primary beam per tile group ---
+// With a primary beam per tile that takes a few distinct values (one per
+// dipole configuration; groups g of tiles sharing B_pg, at most
+// VM_MFMA_MAX_GROUPS), J^-1_pa = F_pa B_pg^-1 D_a^-1 for tile a of group g
+// (src/jones.c:97-157), so
+//   e_p = sum_g B_pg^-1 E_pg,   E_pg = sum_{a in g} (phi_pa F_pa) u_a,
+//   N_p = sum_g B_pg^-1 Y_pg B_pg^-H,   Y_pg = sum_k h_pg[k] T_gk,
+// with u_a = D_a^-1 v_a and T_gk the prep pass's T_k summed over the tiles
+// of group g alone.  The GEMM's K runs over the groups in turn (each group's
+// tiles padded to whole k-steps), and between groups the accumulators are
+// carried over by R_pg = B_pg B_p,g-1^-1 (a Horner scheme): after the last
+// group they hold B_p,G-1 sum_g B_pg^-1 E_pg, so one set of accumulators
+// serves every group and the epilogue applies M = B_p,G-1^-1 once.  Each
+// group's auto-correlation term goes through S_pg, the linear map
+// Y -> B_pg^-1 Y B_pg^-H of Hermitian Y written (Y00, Y11, Re Y01, Im Y01).
+#define VM_MFMA_MAX_GROUPS 4
+#define GX_G 28                                   // per group: h (4), S (16), R (8)
+#define GX_ROW (8 + GX_G * VM_MFMA_MAX_GROUPS)    // M (8) then the groups
+
+// GEMM slots [t[g], t[g + 1]) belong to group g (multiples of 8), n groups;
+// rep[g]: an antenna of the group (whose B stands for it)
+struct grp_bounds {
+    int t[VM_MFMA_MAX_GROUPS + 1];
+    int rep[VM_MFMA_MAX_GROUPS];
+    int n;
+};
+
+__device__ __forceinline__ int slot_group(const grp_bounds &gb, int a)
+{
+    int g = 0;
+#pragma unroll
+    for (int k = 1; k < VM_MFMA_MAX_GROUPS; k++) g += (k < gb.n && a >= gb.t[k]) ? 1 : 0;
+    return g;
+}
+
+// inverse of a 2x2 complex matrix (zero when singular)
+__device__ __forceinline__ void inv2x2(const double2 b[4], double2 o[4])
+{
+    const double2 det = make_double2(b[0].x * b[3].x - b[0].y * b[3].y - (b[1].x * b[2].x - b[1].y * b[2].y),
+                                     b[0].x * b[3].y + b[0].y * b[3].x - (b[1].x * b[2].y + b[1].y * b[2].x));
+    const double dd = det.x * det.x + det.y * det.y;
+    const double2 r = dd > 0.0 ? make_double2(det.x / dd, -det.y / dd) : make_double2(0.0, 0.0);
+    o[0] = zmul(b[3], r);
+    o[1] = zmul(make_double2(-b[1].x, -b[1].y), r);
+    o[2] = zmul(make_double2(-b[2].x, -b[2].y), r);
+    o[3] = zmul(b[0], r);
+}
+
+// Bg[p][g][c][4] = B[p][rep[g]][4]: the per-tile beam (frequency
+// independent) of each group's representative antenna
+__global__ void k_bg_build(const double2 *__restrict__ B, int np, int nant, int nchan,
+                           grp_bounds gb, double2 *__restrict__ Bg)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)np * gb.n * nchan) return;
+    const int p = (int)(j / ((size_t)gb.n * nchan)), g = (int)(j / nchan % gb.n);
+    const double2 *src = B + ((size_t)p * nant + gb.rep[g]) * 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) Bg[j * 4 + q] = src[q];
+}
+
+// Per (c, p): gx[c][p][GX_ROW] = M = B_p,G-1^-1 (4 complex), then per group
+// g: h of Q = B_pg B_pg^H (as k_pb_beam), S_pg (4 x 4, row-major, mapping
+// (Y00, Y11, Re Y01, Im Y01) to the same of M_g Y M_g^H, M_g = B_pg^-1) and
+// R_pg = B_pg M_p,g-1 (4 complex; g >= 1); and the weight exponent esp[c][p]
+// with max_a F_pa 2^esp in [2^12, 2^13).  Bg: [p][g][c][4]; D: [antenna]
+// [c][4]; amap: slot -> antenna (< 0: padding).
+__global__ void k_pbg_consts(const double2 *__restrict__ Bg, const double2 *__restrict__ D,
+                             const int *__restrict__ amap, grp_bounds gb, int np, int nchan,
+                             float *__restrict__ gx, int *__restrict__ esp)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)np * nchan) return;
+    const int c = (int)(j % nchan), p = (int)(j / nchan);
+    float *o = gx + ((size_t)c * np + p) * GX_ROW;
+    double2 Mprev[4];
+    for (int g = 0; g < gb.n; g++) {
+        double2 b[4], m[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) b[q] = Bg[(((size_t)p * gb.n + g) * nchan + c) * 4 + q];
+        inv2x2(b, m);
+        float *og = o + 8 + GX_G * g;
+        const double q00 = b[0].x * b[0].x + b[0].y * b[0].y + b[1].x * b[1].x + b[1].y * b[1].y;
+        const double q11 = b[2].x * b[2].x + b[2].y * b[2].y + b[3].x * b[3].x + b[3].y * b[3].y;
+        const double q01r = b[0].x * b[2].x + b[0].y * b[2].y + b[1].x * b[3].x + b[1].y * b[3].y;
+        const double q01i = b[0].y * b[2].x - b[0].x * b[2].y + b[1].y * b[3].x - b[1].x * b[3].y;
+        og[0] = (float)q00;
+        og[1] = (float)q11;
+        og[2] = (float)(2.0 * q01r);
+        og[3] = (float)(2.0 * q01i);
+        // S: X00 = |m00|^2 Y00 + |m01|^2 Y11 + 2 Re(m00 m01* Y01), likewise
+        // X11 with row 1; X01 = m00 m10* Y00 + m01 m11* Y11 + m00 m11* Y01 +
+        // m01 m10* Y01*
+        const double2 pp = zmul(m[0], zconj(m[1])), qq = zmul(m[2], zconj(m[3]));
+        const double2 aa = zmul(m[0], zconj(m[2])), bb = zmul(m[1], zconj(m[3]));
+        const double2 cc = zmul(m[0], zconj(m[3])), dd = zmul(m[1], zconj(m[2]));
+        const double S[16] = {
+            m[0].x * m[0].x + m[0].y * m[0].y, m[1].x * m[1].x + m[1].y * m[1].y, 2.0 * pp.x, -2.0 * pp.y,
+            m[2].x * m[2].x + m[2].y * m[2].y, m[3].x * m[3].x + m[3].y * m[3].y, 2.0 * qq.x, -2.0 * qq.y,
+            aa.x, bb.x, cc.x + dd.x, dd.y - cc.y,
+            aa.y, bb.y, cc.y + dd.y, cc.x - dd.x};
+#pragma unroll
+        for (int q = 0; q < 16; q++) og[4 + q] = (float)S[q];
+        if (g > 0) {
+            double2 r[4];
+            r[0] = zadd(zmul(b[0], Mprev[0]), zmul(b[1], Mprev[2]));
+            r[1] = zadd(zmul(b[0], Mprev[1]), zmul(b[1], Mprev[3]));
+            r[2] = zadd(zmul(b[2], Mprev[0]), zmul(b[3], Mprev[2]));
+            r[3] = zadd(zmul(b[2], Mprev[1]), zmul(b[3], Mprev[3]));
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                og[20 + 2 * q] = (float)r[q].x;
+                og[21 + 2 * q] = (float)r[q].y;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++) og[20 + q] = 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) Mprev[q] = m[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        o[2 * q] = (float)Mprev[q].x;
+        o[2 * q + 1] = (float)Mprev[q].y;
+    }
+    for (int q = 8 + GX_G * gb.n; q < GX_ROW; q++) o[q] = 0.f;
+    double f2 = 0.0;
+    for (int a = 0; a < gb.t[gb.n]; a++) {
+        const int as = amap[a];
+        if (as < 0) continue;
+        const int g = slot_group(gb, a);
+        f2 = fmax(f2, frob2_DB(D + ((size_t)as * nchan + c) * 4,
+                               Bg + (((size_t)p * gb.n + g) * nchan + c) * 4));
+    }
+    int ex = 0;
+    if (f2 > 0.0) frexp(sqrt(f2), &ex);
+    esp[(size_t)c * np + p] = 13 - ex;
+}
+
 // ------------------------------------------------------------ weights -----
 // Operand A in fragment order: Af[c][rb][ks][h][lane] (uint4), rb a block of
 // 32 GEMM rows (16 beams), ks a step of 16 k' (8 tiles), h = 0 hi part, 1 lo
@@ -241,9 +392,11 @@ __global__ void k_pb_beam(const double2 *__restrict__ D, const double2 *__restri
 // im rows (Im phi, Re phi); rows of beams >= npointing are zero.  Weights are
 // scaled by 2^PHASE_EXP.  phi: [p][a][c] complex double over nant_src
 // antennas; GEMM tile a is antenna amap[a] (amap NULL: a itself), nant tiles.
-// PB (per-pointing primary beam): the weight is phi F_pa 2^esp[c][p] with
-// F_pa = ||D_a B_p||_F (D [a][c][4], B [p][c][4]).
-template <bool BF16, bool PB>
+// PB 1 (per-pointing primary beam): the weight is phi F_pa 2^esp[c][p] with
+// F_pa = ||D_a B_p||_F (D [a][c][4], B [p][c][4]).  PB 2 (a beam per tile
+// group): the same with B = Bg[p][g][c][4] of the slot's group g (gb), and
+// slots whose amap is negative (group padding) are zero.
+template <bool BF16, int PB>
 __global__ __launch_bounds__(256) void k_phase_weights(const double2 *__restrict__ phi,
                                                        int npointing, int nant, int nchan,
                                                        int nrb, int nkstep,
@@ -251,7 +404,7 @@ __global__ __launch_bounds__(256) void k_phase_weights(const double2 *__restrict
                                                        const double2 *__restrict__ D,
                                                        const double2 *__restrict__ B,
                                                        const int *__restrict__ esp,
-                                                       uint4 *__restrict__ Af)
+                                                       uint4 *__restrict__ Af, grp_bounds gb)
 {
     typedef mtype<BF16> M;
     const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -271,13 +424,16 @@ __global__ __launch_bounds__(256) void k_phase_weights(const double2 *__restrict
     for (int q = 0; q < 4; q++) {
         uint16_t hb[2] = {0, 0}, lb[2] = {0, 0};
         const int a = 8 * ks + 4 * (lane >> 5) + q;
-        if (p < npointing && a < nant) {
-            const int as = amap ? amap[a] : a;
+        const int as = (a < nant) ? (amap ? amap[a] : a) : -1;
+        if (p < npointing && as >= 0) {
             const double2 f = phi[((size_t)p * nant_src + as) * nchan + c];
             double wf = ws;
-            if (PB)
+            if (PB == 1)
                 wf *= sqrt(frob2_DB(D + ((size_t)as * nchan + c) * 4,
                                     B + ((size_t)p * nchan + c) * 4));
+            if (PB == 2)
+                wf *= sqrt(frob2_DB(D + ((size_t)as * nchan + c) * 4,
+                                    B + (((size_t)p * gb.n + slot_group(gb, a)) * nchan + c) * 4));
             const double vv[2] = {(im ? f.y : f.x) * wf, (im ? f.x : -f.y) * wf};
 #pragma unroll
             for (int e2 = 0; e2 < 2; e2++) {
@@ -346,18 +502,22 @@ __global__ void k_jcheck(const double2 *__restrict__ J, uint32_t per_pointing,
 // tiles 8 ks + 4 (l/32) + {0..3} as (re, im) pairs of polarisation pol.
 // Samples >= ns are zero; tiles >= nant are zero.  sexp[c][sb] = es.
 // v: VM_FLT [c][i][s].  Needs nant <= NG APT.
-// PB (per-pointing primary beam): Jf holds D^-1 and, instead of the four N
+// PB 1 (per-pointing primary beam): Jf holds D^-1 and, instead of the four N
 // terms, N receives T_k = sum_a gw[c][a].k u_a u_a^H (k = 0..3, each as
-// (XX, YY, Re XY, Im XY)) as float4 [c][s][4].
-template <bool BF16, int APT, int NG, bool PB>
+// (XX, YY, Re XY, Im XY)) as float4 [c][s][4].  PB 2 (a beam per tile
+// group): "antennas" are GEMM slots (polQ, polP, Jf and gw given per slot,
+// padding slots with zero Jf), and the T_k are summed per group g of gb
+// into float4 [c][s][VM_MFMA_MAX_GROUPS][4].
+template <bool BF16, int APT, int NG, int PB>
 __global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, int ns, int ni, int nsb,
                                                  int nkstep, const cf *__restrict__ Jf,
                                                  const float4 *__restrict__ gw,
                                                  const uint32_t *__restrict__ polQ,
                                                  const uint32_t *__restrict__ polP, int nant,
                                                  uint4 *__restrict__ Bf, int *__restrict__ sexp,
-                                                 float4 *__restrict__ N)
+                                                 float4 *__restrict__ N, grp_bounds gb)
 {
+    constexpr int NGRP = PB == 2 ? VM_MFMA_MAX_GROUPS : 1;
     typedef mtype<BF16> M;
     constexpr int NT = 32 * NG; // lanes
     __shared__ float4 red[NG][SB];
@@ -392,26 +552,34 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, i
         }
     }
     float nxx = 0.f, nyy = 0.f, xyr = 0.f, xyi = 0.f, mx = 0.f;
-    float4 tk[4];
+    float4 tk[NGRP][4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) tk[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int q = 0; q < NGRP; q++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) tk[q][k] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int j = 0; j < APT; j++) {
         const cf x = ux[j], y = uy[j];
         mx = fmaxf(mx, fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(y.x), fabsf(y.y))));
         const float rxx = x.x * x.x + x.y * x.y, ryy = y.x * y.x + y.y * y.y;
         const float rxr = x.x * y.x + x.y * y.y, rxi = x.y * y.x - x.x * y.y;
-        if constexpr (PB) {
+        if constexpr (PB != 0) {
             const int a = g + NG * j;
             if (a < nant) {
                 const float4 ga = gw[(size_t)c * nant + a];
                 const float gk[4] = {ga.x, ga.y, ga.z, ga.w};
+                const int ag = PB == 2 ? slot_group(gb, a) : 0;
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    tk[k].x += gk[k] * rxx;
-                    tk[k].y += gk[k] * ryy;
-                    tk[k].z += gk[k] * rxr;
-                    tk[k].w += gk[k] * rxi;
+                for (int q = 0; q < NGRP; q++) {
+                    const float on = (q == ag) ? 1.f : 0.f; // the slot's group only
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const float w = on * gk[k];
+                        tk[q][k].x += w * rxx;
+                        tk[q][k].y += w * ryy;
+                        tk[q][k].z += w * rxr;
+                        tk[q][k].w += w * rxi;
+                    }
                 }
             }
         } else {
@@ -424,21 +592,26 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, i
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
     if ((t & 63) == 0) wmx[t >> 6] = mx;
-    if constexpr (PB) {
+    if constexpr (PB != 0) {
+        const int ngrp = PB == 2 ? gb.n : 1;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            red[g][sl] = tk[k];
-            __syncthreads();
-            if (g == 0 && ok) {
-                float4 acc = red[0][sl];
+        for (int q = 0; q < NGRP; q++) {
+            if (q >= ngrp) break; // uniform over the workgroup
 #pragma unroll
-                for (int q = 1; q < NG; q++) {
-                    const float4 r = red[q][sl];
-                    acc.x += r.x; acc.y += r.y; acc.z += r.z; acc.w += r.w;
+            for (int k = 0; k < 4; k++) {
+                red[g][sl] = tk[q][k];
+                __syncthreads();
+                if (g == 0 && ok) {
+                    float4 acc = red[0][sl];
+#pragma unroll
+                    for (int r2 = 1; r2 < NG; r2++) {
+                        const float4 r = red[r2][sl];
+                        acc.x += r.x; acc.y += r.y; acc.z += r.z; acc.w += r.w;
+                    }
+                    N[(((size_t)c * ns + s) * NGRP + q) * 4 + k] = acc;
                 }
-                N[((size_t)c * ns + s) * 4 + k] = acc;
+                __syncthreads();
             }
-            __syncthreads();
         }
     } else {
         red[g][sl] = make_float4(nxx, nyy, xyr, xyi);
@@ -577,6 +750,9 @@ __device__ __forceinline__ void stokes4(float xr, float xi, float yr, float yi, 
 // Per-pointing primary-beam form: N holds T_k as float4 [c][s][4], bx the
 // pointing's constants as float [c][p][PB_ROW] (k_pb_beam) and esp its
 // weight exponents.
+// Beam per tile group (PB 2): N holds the groups' T_k as float4
+// [c][s][VM_MFMA_MAX_GROUPS][4], gx the constants of k_pbg_consts, gb the
+// groups' slot ranges.
 struct bf_geom {
     const uint4 *Af;
     const uint4 *Bf;
@@ -585,10 +761,12 @@ struct bf_geom {
     float *S;
     const float *bx;
     const int *esp;
+    const float *gx;
     int nkstep, nrb, npointing, nstokes;
     int soffset, ns_total;
     float invw;
     int pad;
+    grp_bounds gb;
 };
 
 struct c2f { float x, y; };
@@ -628,17 +806,89 @@ __device__ __forceinline__ void stokes_pb(float xr, float xi, float yr, float yi
     st[3] = -2.0f * invw * Y01i;
 }
 
+// Stokes of one beam-sample with a beam per tile group (PB 2) from the GEMM
+// output A = (x, y) carried into the last group's frame: e = M A, then
+// X = e e^H u2 - sum_g S_g(Y_g), Y_g = sum_k h_g[k] T_gk, and the Stokes
+// parameters of X as in stokes4.  gxp: the pointing's constants (M, then
+// h, S per group); Tl: the sample's T_gk, 16 floats per group.
+__device__ __forceinline__ void stokes_pbg(float xr, float xi, float yr, float yi, float u2,
+                                           const float *Tl, const float *gxp, int ngroup,
+                                           float invw, float (&st)[4])
+{
+    const gmem f4v *gq = (const gmem f4v *)gxp;
+    const f4v m0 = gq[0], m1 = gq[1]; // (m00, m01), (m10, m11) as (re, im) pairs
+    const float exr = m0.x * xr - m0.y * xi + m0.z * yr - m0.w * yi;
+    const float exi = m0.x * xi + m0.y * xr + m0.z * yi + m0.w * yr;
+    const float eyr = m1.x * xr - m1.y * xi + m1.z * yr - m1.w * yi;
+    const float eyi = m1.x * xi + m1.y * xr + m1.z * yi + m1.w * yr;
+    float X00 = (exr * exr + exi * exi) * u2, X11 = (eyr * eyr + eyi * eyi) * u2;
+    float X01r = (exr * eyr + exi * eyi) * u2, X01i = (exi * eyr - exr * eyi) * u2;
+    for (int g = 0; g < ngroup; g++) {
+        const gmem f4v *G = gq + 2 + (GX_G / 4) * g;
+        const f4v h = G[0];
+        const float4 *T = (const float4 *)(Tl + 16 * g);
+        const float4 t0 = T[0], t1 = T[1], t2 = T[2], t3 = T[3];
+        const float y0 = h.x * t0.x + h.y * t1.x + h.z * t2.x + h.w * t3.x;
+        const float y1 = h.x * t0.y + h.y * t1.y + h.z * t2.y + h.w * t3.y;
+        const float y2 = h.x * t0.z + h.y * t1.z + h.z * t2.z + h.w * t3.z;
+        const float y3 = h.x * t0.w + h.y * t1.w + h.z * t2.w + h.w * t3.w;
+        const f4v s0 = G[1], s1 = G[2], s2 = G[3], s3 = G[4];
+        X00 -= s0.x * y0 + s0.y * y1 + s0.z * y2 + s0.w * y3;
+        X11 -= s1.x * y0 + s1.y * y1 + s1.z * y2 + s1.w * y3;
+        X01r -= s2.x * y0 + s2.y * y1 + s2.z * y2 + s2.w * y3;
+        X01i -= s3.x * y0 + s3.y * y1 + s3.z * y2 + s3.w * y3;
+    }
+    st[0] = invw * (X00 + X11);
+    st[1] = invw * (X00 - X11);
+    st[2] = 2.0f * invw * X01r;
+    st[3] = -2.0f * invw * X01i;
+}
+
+// Horner step between tile groups (PB 2): every beam's (Ex, Ey) of each
+// sample, A <- R_p,g1 A, R from the pointing's constants (clamped to the
+// last pointing for padding rows, whose accumulators are zero).  Lane
+// (lr, lk) of 32-row block m holds beams 16 (rb0 + m) + 4 jg + 2 lk + b.
+template <int MB, int NB>
+__device__ __forceinline__ void bf_horner(f16v (&acc)[MB][NB], const float *gxc, int np, int p0,
+                                          int lk, int g1)
+{
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int jg = 0; jg < 4; jg++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const int p = min(p0 + 16 * m + 4 * jg + 2 * lk + b, np - 1);
+                const gmem f4v *R =
+                    (const gmem f4v *)(gxc + (size_t)p * GX_ROW + 8 + GX_G * g1 + 20);
+                const f4v r0 = R[0], r1 = R[1];
+#pragma unroll
+                for (int j = 0; j < NB / 2; j++) {
+                    f16v &ax = acc[m][2 * j], &ay = acc[m][2 * j + 1];
+                    const float xr = ax[4 * jg + 2 * b], xi = ax[4 * jg + 2 * b + 1];
+                    const float yr = ay[4 * jg + 2 * b], yi = ay[4 * jg + 2 * b + 1];
+                    ax[4 * jg + 2 * b] = r0.x * xr - r0.y * xi + r0.z * yr - r0.w * yi;
+                    ax[4 * jg + 2 * b + 1] = r0.x * xi + r0.y * xr + r0.z * yi + r0.w * yr;
+                    ay[4 * jg + 2 * b] = r1.x * xr - r1.y * xi + r1.z * yr - r1.w * yi;
+                    ay[4 * jg + 2 * b + 1] = r1.x * xi + r1.y * xr + r1.z * yi + r1.w * yr;
+                }
+            }
+}
+
 // Each wavefront's staging tile holds 32 rows of 64 samples (68-float rows,
 // padded against bank conflicts).
 #define BF_WAVE_LDS (32 * 68)
+// Per-sample T rows of the tile-group form: 64 floats used, 68 apart
+#define BF_TROW 68
 // Fragments (1 KB each) of one k-step of a workgroup in the shared-operand
 // form: A (hi, lo) of its WR MB row blocks and B of its WC NB column blocks.
 template <int MB, int NB, int WR, int WC>
 constexpr int bf_wg_frags() { return 2 * WR * MB + WC * NB; }
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool PB, bool LDSOP = false>
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, int PB, bool LDSOP = false>
 constexpr int bf_lds_floats()
 {
-    return WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 32 : 0) +
+    return WR * WC * BF_WAVE_LDS + (PB == 1 ? WR * WC * MB * 16 * 32 : 0) +
+           (PB == 2 ? WR * WC * (NB / 2) * 32 * BF_TROW : 0) +
            (LDSOP ? 2 * bf_wg_frags<MB, NB, WR, WC>() * 256 : 0);
 }
 
@@ -646,13 +896,17 @@ constexpr int bf_lds_floats()
 // the per-pointing form reads the pointing's B^-1, h and weight exponent from
 // its LDS row pl of wpb and gives zeros for pointings past the last (valid
 // false).
-template <bool PB>
+template <int PB>
 __device__ __forceinline__ void bf_stokes(float xr, float xi, float yr, float yi, float unscale2,
                                           int esb, const float4 &nn, const float4 (&T4)[4],
                                           const float *wpb, int pl, bool valid, float invw,
-                                          float (&st4)[4])
+                                          const float *Tl, const float *gxp, int ngroup,
+                                          int espp, float (&st4)[4])
 {
-    if constexpr (PB) {
+    if constexpr (PB == 2) {
+        stokes_pbg(xr, xi, yr, yi, ldexpf(1.0f, -2 * (esb + espp)), Tl, gxp, ngroup, invw, st4);
+        if (!valid) st4[0] = st4[1] = st4[2] = st4[3] = 0.f;
+    } else if constexpr (PB == 1) {
         const float4 *bl = (const float4 *)(wpb + 32 * pl);
         const float4 R[5] = {bl[0], bl[1], bl[2], bl[3], bl[4]};
         const float u2 = ldexpf(1.0f, -2 * (esb + __float_as_int(wpb[32 * pl + PB_ROW])));
@@ -715,7 +969,7 @@ __device__ __forceinline__ void wg_lread(afrag<MB> &fa, bfrag<NB> &fb, const uin
     for (int n = 0; n < NB; n++) fb.b[n] = o[(FA + wc * NB + n) * 64];
 }
 
-template <bool BF16, int MB, int NB, int WR, int WC, bool PB>
+template <bool BF16, int MB, int NB, int WR, int WC, int PB>
 __device__ __forceinline__ void bf_kloop_lds(f16v (&acc)[MB][NB], rsrc_t rA, rsrc_t rB, int vl,
                                              int astr, int bstr, int nkstep, int nsb, int ty,
                                              int tx, int w, int lane, float *stg)
@@ -726,7 +980,8 @@ __device__ __forceinline__ void bf_kloop_lds(f16v (&acc)[MB][NB], rsrc_t rA, rsr
     // wavefront tiles only (a 128 x 128 tile's 256 accumulators leave no room)
     static_assert(MB * NB <= 8, "shared-operand k-loop is for 64 x 64 wavefront tiles");
     constexpr int PER = F / NW, NSB = NB / 2;
-    uint4 *ops = (uint4 *)(stg + WR * WC * BF_WAVE_LDS + (PB ? WR * WC * MB * 16 * 32 : 0));
+    static_assert(PB != 2, "the tile-group form uses the register-ring k-loop");
+    uint4 *ops = (uint4 *)(stg + WR * WC * BF_WAVE_LDS + (PB == 1 ? WR * WC * MB * 16 * 32 : 0));
     const int wr = w / WC, wc = w % WC;
     const int rbw0 = ty * WR * MB, sbw0 = tx * WC * NSB;
     int goff[PER]; // this wavefront's loader fragments: byte offsets at k-step 0
@@ -777,7 +1032,43 @@ __device__ __forceinline__ void bf_kloop_lds(f16v (&acc)[MB][NB], rsrc_t rA, rsr
     }
 }
 
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool VEC, bool PB, bool LDSOP>
+// Register-ring k-loop over k-steps [k0, k1).  Two register rings: A
+// (weights, small and L2-resident) two k-steps deep, B (voltage fragments,
+// streamed from HBM) NST deep, so each B fragment is requested NST-1
+// k-steps before its MFMAs.  The scheduling barriers pin each step's loads
+// ahead of its MFMAs; a super-step of U k-steps returns both rings to slot
+// 0, so ring slots are compile-time.  The loop runs whole super-steps with no
+// branches: past k1 - 1 the B loads re-read that step and the A loads fall
+// outside the weights buffer, which returns zeros, so those MFMAs add
+// nothing.
+#define VM_SB() __builtin_amdgcn_sched_barrier(0)
+template <bool BF16, int MB, int NB, int NST>
+__device__ __forceinline__ void bf_kloop_ring(f16v (&acc)[MB][NB], rsrc_t rA, rsrc_t rB, int vl,
+                                              int aoff, int astr, const int (&boff)[NB], int k0,
+                                              int k1)
+{
+    constexpr int U = (NST % 2 == 0) ? NST : 2 * NST;
+    afrag<MB> fa[2];
+    bfrag<NB> fb[NST];
+    const int kl = k1 - 1;
+    load_a<MB>(fa[0], rA, vl, aoff, astr, k0, true);
+#pragma unroll
+    for (int r = 0; r < NST - 1; r++) load_b<NB>(fb[r], rB, vl, boff, min(k0 + r, kl));
+    for (int ks = k0; ks < k1; ks += U) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            VM_SB();
+            load_a<MB>(fa[(u + 1) & 1], rA, vl, aoff, astr, ks + u + 1, ks + u + 1 <= kl);
+            load_b<NB>(fb[(u + NST - 1) % NST], rB, vl, boff, min(ks + u + NST - 1, kl));
+            mma_ab<BF16, MB, NB>(acc, fa[u & 1], fb[u % NST]);
+            interleave_loads<2 * MB + NB, 2 * MB * NB>();
+            VM_SB();
+        }
+    }
+}
+#undef VM_SB
+
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool VEC, int PB, bool LDSOP>
 __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int nchan, int c,
                                         int tx, int ty, float *stg)
 {
@@ -824,7 +1115,7 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
         const int sb = sb0 + j;
         const int s = SB * sb + lr;
         nnv[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (!PB && s < ns) {
+        if (PB == 0 && s < ns) {
             const f4v n4 = ((const gmem f4v *)G.N)[(size_t)c * ns + s];
             nnv[j] = make_float4(n4.x, n4.y, n4.z, n4.w);
         }
@@ -834,39 +1125,22 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                                                                           min(sb, nsb - 1)];
     }
 
-    // Two register rings: A (weights, small and L2-resident) two k-steps
-    // deep, B (voltage fragments, streamed from HBM) NST deep, so each B
-    // fragment is requested NST-1 k-steps before its MFMAs.  The scheduling
-    // barriers pin each step's loads ahead of its MFMAs; a super-step of U
-    // k-steps returns both rings to slot 0, so ring slots are compile-time.
-    // The loop runs whole super-steps with no branches: past the last k-step
-    // the B loads re-read it and the A loads fall outside the weights buffer,
-    // which returns zeros, so those MFMAs add nothing.
-#define VM_SB() __builtin_amdgcn_sched_barrier(0)
     if constexpr (LDSOP) {
         bf_kloop_lds<BF16, MB, NB, WR, WC, PB>(acc, rA, rB, vl, astr, bstr, nkstep, nsb, ty, tx, w,
                                                lane, stg);
-    } else {
-    constexpr int U = (NST % 2 == 0) ? NST : 2 * NST;
-    afrag<MB> fa[2];
-    bfrag<NB> fb[NST];
-    const int kl = nkstep - 1;
-    load_a<MB>(fa[0], rA, vl, aoff, astr, 0, true);
-#pragma unroll
-    for (int r = 0; r < NST - 1; r++) load_b<NB>(fb[r], rB, vl, boff, min(r, kl));
-    for (int ks = 0; ks < nkstep; ks += U) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            VM_SB();
-            load_a<MB>(fa[(u + 1) & 1], rA, vl, aoff, astr, ks + u + 1, ks + u + 1 <= kl);
-            load_b<NB>(fb[(u + NST - 1) % NST], rB, vl, boff, min(ks + u + NST - 1, kl));
-            mma_ab<BF16, MB, NB>(acc, fa[u & 1], fb[u % NST]);
-            interleave_loads<2 * MB + NB, 2 * MB * NB>();
-            VM_SB();
+    } else if constexpr (PB == 2) {
+        // the tile groups in turn, carried over by the Horner step
+        const int ng = G.gb.n;
+        for (int g = 0; g < ng; g++) {
+            bf_kloop_ring<BF16, MB, NB, NST>(acc, rA, rB, vl, aoff, astr, boff, G.gb.t[g] / 8,
+                                             G.gb.t[g + 1] / 8);
+            if (g + 1 < ng)
+                bf_horner<MB, NB>(acc, G.gx + (size_t)c * G.npointing * GX_ROW, G.npointing,
+                                  16 * rb0, lane >> 5, g + 1);
         }
+    } else {
+        bf_kloop_ring<BF16, MB, NB, NST>(acc, rA, rB, vl, aoff, astr, boff, 0, nkstep);
     }
-    }
-#undef VM_SB
 
     // epilogue: in a 32-row block lane (lr, lk) holds rows 8 jg + 4 lk + {0..3}
     // = beams 4 jg + 2 lk + {0, 1} as (re, im), column lr; block n = 2 j holds
@@ -880,9 +1154,27 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
     // Its pointings' constants (PB_ROW floats from k_pb_beam) and weight
     // exponents go to a 32-float LDS row per pointing of the wavefront (16 MB
     // pointings).
-    float4 T4v[PB ? NSB : 1][4];
-    float *wpb = stg + WR * WC * BF_WAVE_LDS + (PB ? w * MB * 16 * 32 : 0);
-    if constexpr (PB) {
+    float4 T4v[PB == 1 ? NSB : 1][4];
+    float *wpb = stg + WR * WC * BF_WAVE_LDS + (PB == 1 ? w * MB * 16 * 32 : 0);
+    // tile-group form: each sample's T_gk (VM_MFMA_MAX_GROUPS x 4 float4)
+    // staged in this wavefront's LDS rows, the lane halves taking half the
+    // groups each
+    float *wT = stg + WR * WC * BF_WAVE_LDS + (PB == 2 ? w * NSB * 32 * BF_TROW : 0);
+    if constexpr (PB == 2) {
+#pragma unroll
+        for (int j = 0; j < NSB; j++) {
+            const int s = SB * (sb0 + j) + lr;
+#pragma unroll
+            for (int q = 0; q < 2 * VM_MFMA_MAX_GROUPS; q++) {
+                const int gk = (lane >> 5) * 2 * VM_MFMA_MAX_GROUPS + q; // group * 4 + k
+                f4v t4 = {0.f, 0.f, 0.f, 0.f};
+                if (s < ns && (gk >> 2) < G.gb.n)
+                    t4 = ((const gmem f4v *)G.N)[((size_t)c * ns + s) * 4 * VM_MFMA_MAX_GROUPS + gk];
+                *(float4 *)(wT + (j * 32 + lr) * BF_TROW + 4 * gk) = make_float4(t4.x, t4.y, t4.z, t4.w);
+            }
+        }
+    }
+    if constexpr (PB == 1) {
         for (int pl = lane; pl < MB * 16; pl += 64) {
             const int cp = c * G.npointing + min(16 * rb0 + pl, G.npointing - 1);
             const gmem f4v *bp = (const gmem f4v *)(G.bx + (size_t)cp * PB_ROW);
@@ -940,10 +1232,15 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                             const f16v &ax = acc[m][2 * j], &ay = acc[m][2 * j + 1];
                             const int pl = 16 * m + 8 * h + 4 * jj + 2 * lk + b;
                             float st4[4];
+                            const int pc = min(16 * rb0 + pl, npointing - 1);
                             bf_stokes<PB>(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1],
                                           ay[4 * jg + 2 * b], ay[4 * jg + 2 * b + 1], us2[j],
-                                          esbv[j], nnv[j], T4v[PB ? j : 0], wpb, pl,
-                                          16 * rb0 + pl < npointing, invw, st4);
+                                          esbv[j], nnv[j], T4v[PB == 1 ? j : 0], wpb, pl,
+                                          16 * rb0 + pl < npointing, invw,
+                                          wT + (j * 32 + lr) * BF_TROW,
+                                          G.gx + ((size_t)c * npointing + pc) * GX_ROW, G.gb.n,
+                                          PB == 2 ? ((const gmem int *)G.esp)[(size_t)c * npointing + pc] : 0,
+                                          st4);
 #pragma unroll
                             for (int st = 0; st < 4; st++)
                                 wl[(st * 8 + 4 * jj + 2 * b + lk) * 68 + 32 * j + lr] = st4[st];
@@ -981,10 +1278,15 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
                         const f16v &ax = acc[m][2 * j], &ay = acc[m][2 * j + 1];
                         const int pl = 16 * m + 8 * h + 4 * jj + 2 * lk + b;
                         float st4[4];
+                        const int pc = min(16 * rb0 + pl, npointing - 1);
                         bf_stokes<PB>(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1],
                                       ay[4 * jg + 2 * b], ay[4 * jg + 2 * b + 1], us2[j], esbv[j],
-                                      nnv[j], T4v[PB ? j : 0], wpb, pl,
-                                      16 * rb0 + pl < npointing, invw, st4);
+                                      nnv[j], T4v[PB == 1 ? j : 0], wpb, pl,
+                                      16 * rb0 + pl < npointing, invw,
+                                      wT + (j * 32 + lr) * BF_TROW,
+                                      G.gx + ((size_t)c * npointing + pc) * GX_ROW, G.gb.n,
+                                      PB == 2 ? ((const gmem int *)G.esp)[(size_t)c * npointing + pc] : 0,
+                                      st4);
 #pragma unroll
                         for (int st = 0; st < 4; st++)
                             wl[(st * 8 + 4 * jj + 2 * b + lk) * 32 + lr] = st4[st];
@@ -1024,7 +1326,7 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
 // the same tile of 8 different channels, so each XCD walks its own channels;
 // within a channel tiles go in bands of gr row tiles, column by column (gr =
 // all row tiles for order 1, 1 for order 2, 2 for order 3, 4 for order 4).
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC, bool VEC, bool PB,
+template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC, bool VEC, int PB,
           bool LDSOP>
 __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, int ns, int nsb,
                                                                      int nchan, int ntx, int nty,
@@ -1064,7 +1366,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
     const bf_geom *__restrict__ geoms, const uint2 *__restrict__ items, int nitems,
     uint32_t *__restrict__ queue, int ns, int nsb, int nchan)
 {
-    __shared__ __attribute__((aligned(16))) float stg[bf_lds_floats<BF16, MB, NB, WR, WC, NST, false>()];
+    __shared__ __attribute__((aligned(16))) float stg[bf_lds_floats<BF16, MB, NB, WR, WC, NST, 0>()];
     __shared__ int slot[2];
     for (int it = 0;; it++) {
         // two slots: a slot is rewritten only after the barrier of the next
@@ -1081,8 +1383,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
         const int cnt = __builtin_amdgcn_readfirstlane((int)(e.y & 0xffu));
         const bf_geom G = geoms[g];
         for (int ty = ty0; ty < ty0 + cnt; ty++)
-            bf_tile<BF16, MB, NB, WR, WC, NST, true, false, false>(G, ns, nsb, nchan, c, tx, ty,
-                                                                   stg);
+            bf_tile<BF16, MB, NB, WR, WC, NST, true, 0, false>(G, ns, nsb, nchan, c, tx, ty, stg);
     }
 }
 
@@ -1101,7 +1402,11 @@ static const bf_variant kVariants[] = {
 #define N_VARIANTS ((int)(sizeof(kVariants) / sizeof(kVariants[0])))
 #define VM_N_ORDERS 5
 
-template <bool BF16, bool PB>
+// variants with the tile-group form (register-ring k-loop, at most four
+// wavefronts: its T rows take 17 KB of LDS per wavefront)
+static bool variant_has_groups(int vi) { return vi == 0 || vi == 3 || vi == 4 || vi == 5 || vi == 8; }
+
+template <bool BF16, int PB>
 static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, int ns, int nsb,
                            int nchan, int ntx, int nty, int order)
 {
@@ -1118,6 +1423,17 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
 #define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, OCC_, LDS_)                                       \
     hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, OCC_, true, PB, LDS_>),  \
                        grid, dim3(64 * WR_ * WC_), 0, st, G, ns, nsb, nchan, ntx, nty, order)
+    if constexpr (PB == 2) {
+        switch (vi) {
+        case 0: VM_LAUNCH(4, 4, 2, 2, 4, 1, false); break;
+        case 3: VM_LAUNCH(2, 4, 2, 2, 4, 2, false); break;
+        case 4: VM_LAUNCH(4, 2, 2, 2, 4, 2, false); break;
+        case 5: VM_LAUNCH(2, 4, 4, 1, 4, 2, false); break;
+        case 8: VM_LAUNCH(2, 4, 1, 4, 4, 2, false); break;
+        default: VM_FATAL("MFMA beamformer: variant %d has no tile-group form", vi);
+        }
+        return;
+    } else {
     switch (vi) {
     case 0: VM_LAUNCH(4, 4, 2, 2, 4, 1, false); break;
     case 1: VM_LAUNCH(4, 4, 1, 4, 4, 1, false); break;
@@ -1131,6 +1447,7 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
     case 9: VM_LAUNCH(2, 4, 4, 2, 4, 2, false); break;
     case 10: VM_LAUNCH(2, 4, 2, 2, 4, 2, true); break;
     default: VM_LAUNCH(2, 4, 8, 1, 4, 2, false); break;
+    }
     }
 #undef VM_LAUNCH
 }
@@ -1189,21 +1506,24 @@ static void ensure_buf(void **p, size_t *cap, size_t need)
 #define ROW_PAD 256 /* GEMM rows are padded to a multiple of the usual workgroup row tile */
 
 // k_bf_prep over (nsb sample blocks) x (nchan channels) for nant <= 256
-// antennas (the template's antennas-per-thread bound is chosen from nant)
-// gw non-NULL: the per-pointing primary-beam form (Jf = D^-1, N = T_k)
+// antennas or GEMM slots (the template's antennas-per-thread bound is chosen
+// from nant).  pbm 1: the per-pointing primary-beam form (Jf = D^-1, N =
+// T_k); pbm 2: the tile-group form (per-slot Jf = D^-1 and gw, N = T_gk of
+// the groups of gb).
 static void launch_prep(bool bf16, hipStream_t st, const cf *v, int ns, int ni, int nsb,
                         int nkstep, int nchan, const cf *Jf, const float4 *gw,
                         const uint32_t *polQ, const uint32_t *polP, int nant, uint4 *Bf, int *sexp,
-                        float4 *N)
+                        float4 *N, int pbm, const grp_bounds &gb)
 {
     if (nant > 256) VM_FATAL("MFMA beamformer: at most 256 antennas (%d)", nant);
     const dim3 pg((unsigned)nsb, (unsigned)nchan);
 #define VM_PREP(B_, A_, G_, P_)                                                               \
     hipLaunchKernelGGL((k_bf_prep<B_, A_, G_, P_>), pg, dim3(32 * G_), 0, st, v, ns, ni, nsb,  \
-                       nkstep, Jf, gw, polQ, polP, nant, Bf, sexp, N)
+                       nkstep, Jf, gw, polQ, polP, nant, Bf, sexp, N, gb)
 #define VM_PREP2(A_, G_)                                                                      \
-    if (gw) { if (bf16) VM_PREP(true, A_, G_, true); else VM_PREP(false, A_, G_, true); }     \
-    else { if (bf16) VM_PREP(true, A_, G_, false); else VM_PREP(false, A_, G_, false); }
+    if (pbm == 2) { if (bf16) VM_PREP(true, A_, G_, 2); else VM_PREP(false, A_, G_, 2); }     \
+    else if (pbm == 1) { if (bf16) VM_PREP(true, A_, G_, 1); else VM_PREP(false, A_, G_, 1); } \
+    else { if (bf16) VM_PREP(true, A_, G_, 0); else VM_PREP(false, A_, G_, 0); }
     if (nant <= 64) {
         VM_PREP2(8, 8);
     } else if (nant <= 128) {
@@ -1225,9 +1545,11 @@ struct mfma_plan {
     float4 *N;
     int *sexp;
     float invw;
-    int pb;      // per-pointing primary-beam form
-    float *bx;   // pb: B^-1 and h per (c, p)
-    int *esp;    // pb: weight exponents per (c, p)
+    int pb;      // 1: per-pointing primary beam; 2: primary beam per tile group
+    float *bx;   // pb 1: B^-1 and h per (c, p)
+    int *esp;    // pb 1, 2: weight exponents per (c, p)
+    float *gx;   // pb 2: the groups' constants per (c, p)
+    grp_bounds gb;
 };
 
 // Whether J^-1 (vm->d_J) differs between pointings, decided on the device
@@ -1246,9 +1568,9 @@ static int j_differs(vcsbeam_context *vm, hipStream_t st, uint32_t *flag)
     return h != 0;
 }
 
-// Whether J^-1 may differ between pointings.  When it was computed on the
-// device from D and the primary beam d_B (vmCalcJDevice*), J^-1[p] is a
-// fixed function of D and B[p], so it is the same for every pointing
+// Whether J^-1 may differ between pointings.  When it was computed from D
+// and the primary beam d_B (vmCalcJDevice*, or vmCalcJ + vmPushJ), J^-1[p]
+// is a fixed function of D and B[p], so it is the same for every pointing
 // wherever B is: the check reads B (at most a few MB) instead of J^-1
 // (hundreds of MB).  A beam that differs only on flagged tiles (D = 0) is
 // reported as differing, and a per-tile beam that differs is then checked
@@ -1273,6 +1595,46 @@ static int pointings_differ(vcsbeam_context *vm, hipStream_t st, uint32_t *flag)
     return h != 0;
 }
 
+// Tiles grouped by their primary beam pb.B[p][a] (equal in every pointing),
+// groups in order of first appearance, each group's antennas ascending and
+// padded to a multiple of 8 GEMM slots.  Fills slot_ant (-1: padding) and
+// gb; returns the number of groups (more than VM_MFMA_MAX_GROUPS: not
+// filled).
+static int build_tile_groups(const vcsbeam_context *vm, int *slot_ant, int *nslot, grp_bounds *gb)
+{
+    const int nant = (int)vm->obs_metadata->num_ants, np = (int)vm->npointing;
+    const gpuDoubleComplex *B = vm->pb.B;
+    int grp[256], rep[VM_MFMA_MAX_GROUPS + 1], ng = 0;
+    for (int a = 0; a < nant; a++) {
+        int g = 0;
+        for (; g < ng; g++) {
+            bool same = true;
+            for (int p = 0; p < np && same; p++)
+                same = memcmp(&B[PB_IDX(p, a, 0, nant, 4)], &B[PB_IDX(p, rep[g], 0, nant, 4)],
+                              4 * sizeof(gpuDoubleComplex)) == 0;
+            if (same) break;
+        }
+        if (g == ng) {
+            if (ng == VM_MFMA_MAX_GROUPS) return ng + 1;
+            rep[ng++] = a;
+        }
+        grp[a] = g;
+    }
+    int n = 0;
+    memset(gb, 0, sizeof(*gb));
+    for (int g = 0; g < ng; g++) {
+        gb->t[g] = n;
+        gb->rep[g] = rep[g];
+        for (int a = 0; a < nant; a++)
+            if (grp[a] == g) slot_ant[n++] = a;
+        while (n % 8) slot_ant[n++] = -1;
+    }
+    gb->t[ng] = n;
+    gb->n = ng;
+    *nslot = n;
+    return ng;
+}
+
 static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
 {
     vm_device_state *d = vm_dev(vm);
@@ -1285,39 +1647,64 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     const int nchan = vm->nfine_chan, np = (int)vm->npointing;
     if (ni != 2 * nant) VM_FATAL("MFMA beamformer: %d RF inputs for %d antennas", ni, nant);
     if (nant > 256) VM_FATAL("MFMA beamformer: at most 256 antennas (%d)", nant);
-    const int nkstep = (nant + 7) / 8; // 8 tiles (16 k') per step; tiles past nant are zero
     const int nrb = ((2 * np + ROW_PAD - 1) / ROW_PAD) * ROW_PAD / RBK;
     const int ns = vm->fine_sample_rate / vm->chunks_per_second;
     const int nsb = (ns + SB - 1) / SB;
 
     bool rebuild = d->Wm_gen != d->weights_gen || d->Wm_form != vm->beamform_form;
+    int slot_ant[256 + 8 * VM_MFMA_MAX_GROUPS];
+    int nslot = 0;
     if (rebuild) {
-        // the form: J^-1 shared by every pointing (u = J^-1 v), or a
-        // per-pointing primary beam B[p][c] shared by every tile
+        // the form: J^-1 shared by every pointing (u = J^-1 v), a primary
+        // beam B[p][c] per pointing shared by every tile, or a beam per tile
+        // that takes a few distinct values (tile groups)
         int pb = 0;
         if (np > 1) {
             ensure_buf(&d->d_flag, &d->flag_bytes, 256);
-            const int hflag = pointings_differ(vm, st, (uint32_t *)d->d_flag);
-            if (hflag) {
-                if (!d->B_chan || !d->d_B)
-                    VM_FATAL("MFMA beamformer: J^-1 differs between pointings with a primary beam "
-                             "per tile (vmCalcJDevice); this form takes a per-pointing beam shared "
-                             "by all tiles (vmCalcJDeviceChanB), or use the FP32 form");
-                pb = 1;
+            if (pointings_differ(vm, st, (uint32_t *)d->d_flag)) {
+                if (d->B_chan && d->d_B) {
+                    pb = 1;
+                } else if (d->J_fromB && d->d_B && d->B_np == np && vm->pb.B) {
+                    grp_bounds gb;
+                    const int ng = build_tile_groups(vm, slot_ant, &nslot, &gb);
+                    if (ng > VM_MFMA_MAX_GROUPS)
+                        VM_FATAL("MFMA beamformer: the tiles have more than %d distinct primary "
+                                 "beams; use the FP32 form", VM_MFMA_MAX_GROUPS);
+                    pb = 2;
+                    memcpy(d->grp_t, gb.t, sizeof(gb.t));
+                    memcpy(d->grp_rep, gb.rep, sizeof(gb.rep));
+                    d->grp_n = gb.n;
+                } else {
+                    VM_FATAL("MFMA beamformer: J^-1 differs between pointings and was not "
+                             "computed from D and a primary beam (vmCalcJ + vmPushJ, "
+                             "vmCalcJDevice, vmCalcJDeviceChanB); use the FP32 form");
+                }
             }
         }
         d->Wm_pb = pb;
     }
     const int pb = d->Wm_pb;
+    grp_bounds gb = {};
+    if (pb == 2) {
+        memcpy(gb.t, d->grp_t, sizeof(gb.t));
+        memcpy(gb.rep, d->grp_rep, sizeof(gb.rep));
+        gb.n = d->grp_n;
+    }
+    // GEMM slots: the antennas, or (tile groups) the groups' padded runs
+    const int nk = pb == 2 ? gb.t[gb.n] : nant;
+    const int nkstep = (nk + 7) / 8; // 8 tiles (16 k') per step; slots past nk are zero
 
-    // scratch: Af | Bf | N (T_k for pb) | sexp | Jf (D^-1 for pb) | g | bx | esp
+    // scratch: Af | Bf | N | sexp | Jf | g | bx (pb 1) or gx (pb 2) | esp
+    const int ngrp = pb == 2 ? VM_MFMA_MAX_GROUPS : 1;
     const size_t af_bytes = align256((size_t)nchan * nrb * nkstep * 128 * sizeof(uint4));
     const size_t bf_bytes = align256((size_t)nchan * nsb * nkstep * 128 * sizeof(uint4));
-    const size_t n_bytes = align256((size_t)nchan * ns * sizeof(float4) * (pb ? 4 : 1));
+    const size_t n_bytes = align256((size_t)nchan * ns * sizeof(float4) * (pb ? 4 * ngrp : 1));
     const size_t se_bytes = align256((size_t)nchan * nsb * sizeof(int));
-    const size_t jf_bytes = align256((size_t)nchan * nant * 4 * sizeof(cf));
-    const size_t g_bytes = pb ? align256((size_t)nchan * nant * sizeof(float4)) : 0;
-    const size_t bx_bytes = pb ? align256((size_t)nchan * np * PB_ROW * sizeof(float)) : 0;
+    const size_t jf_bytes = align256((size_t)nchan * nk * 4 * sizeof(cf));
+    const size_t g_bytes = pb ? align256((size_t)nchan * nk * sizeof(float4)) : 0;
+    const size_t bx_bytes = pb == 1   ? align256((size_t)nchan * np * PB_ROW * sizeof(float))
+                            : pb == 2 ? align256((size_t)nchan * np * GX_ROW * sizeof(float))
+                                      : 0;
     const size_t esp_bytes = pb ? align256((size_t)nchan * np * sizeof(int)) : 0;
     const size_t need = af_bytes + bf_bytes + n_bytes + se_bytes + jf_bytes + g_bytes + bx_bytes +
                         esp_bytes;
@@ -1326,14 +1713,24 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
         rebuild = true;
     }
     uint8_t *q = (uint8_t *)d->d_Wm;
-    uint4 *Af = (uint4 *)q;                 q += af_bytes;
-    uint4 *Bf = (uint4 *)q;                 q += bf_bytes;
-    float4 *N = (float4 *)q;                q += n_bytes;
-    int *sexp = (int *)q;                   q += se_bytes;
-    cf *Jf = (cf *)q;                       q += jf_bytes;
-    float4 *gw = pb ? (float4 *)q : NULL;   q += g_bytes;
-    float *bx = pb ? (float *)q : NULL;     q += bx_bytes;
+    uint4 *Af = (uint4 *)q;                      q += af_bytes;
+    uint4 *Bf = (uint4 *)q;                      q += bf_bytes;
+    float4 *N = (float4 *)q;                     q += n_bytes;
+    int *sexp = (int *)q;                        q += se_bytes;
+    cf *Jf = (cf *)q;                            q += jf_bytes;
+    float4 *gw = pb ? (float4 *)q : NULL;        q += g_bytes;
+    float *bx = pb == 1 ? (float *)q : NULL;
+    float *gx = pb == 2 ? (float *)q : NULL;     q += bx_bytes;
     int *esp = pb ? (int *)q : NULL;
+    // tile groups: slot arrays (antenna, polQ, polP) on the device
+    int *d_amap = NULL;
+    uint32_t *d_sq = NULL, *d_sp = NULL;
+    if (pb == 2) {
+        ensure_buf(&d->d_grp, &d->grp_bytes, (size_t)3 * nk * sizeof(int));
+        d_amap = (int *)d->d_grp;
+        d_sq = (uint32_t *)(d_amap + nk);
+        d_sp = d_sq + nk;
+    }
 
     if (rebuild) {
         vm_timer_start(d, VM_T_BF_WEIGHTS);
@@ -1341,32 +1738,64 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
         const dim3 g((unsigned)((total + 255) / 256));
         const double2 *dD = (const double2 *)vm->d_D, *dB = (const double2 *)d->d_B;
         const double2 *phi = (const double2 *)vm->gdelays.d_phi;
-        if (pb) {
+        const grp_bounds none = {};
+        if (pb == 2) {
+            if (nslot != nk) VM_FATAL("MFMA beamformer: tile groups changed without a rebuild");
+            uint32_t *hs = (uint32_t *)malloc((size_t)3 * nk * sizeof(uint32_t));
+            if (!hs) VM_FATAL("out of host memory");
+            for (int a = 0; a < nk; a++) {
+                const int as = slot_ant[a];
+                hs[a] = (uint32_t)as;
+                hs[nk + a] = as >= 0 ? vm->polQ_idxs[as] : 0u;
+                hs[2 * nk + a] = as >= 0 ? vm->polP_idxs[as] : 0u;
+            }
+            HIPCHECK(hipMemcpyAsync(d->d_grp, hs, (size_t)3 * nk * sizeof(uint32_t),
+                                    hipMemcpyHostToDevice, st));
+            ensure_buf(&d->d_Bg, &d->Bg_bytes, (size_t)np * gb.n * nchan * 4 * sizeof(double2));
+            const size_t nbg = (size_t)np * gb.n * nchan;
+            hipLaunchKernelGGL(k_bg_build, dim3((unsigned)((nbg + 255) / 256)), dim3(256), 0, st,
+                               dB, np, nant, nchan, gb, (double2 *)d->d_Bg);
+            hipLaunchKernelGGL(k_pb_dinv, dim3((unsigned)((nk * nchan + 255) / 256)), dim3(256),
+                               0, st, dD, nk, nchan, (const int *)d_amap, Jf, gw);
+            hipLaunchKernelGGL(k_pbg_consts, dim3((unsigned)((np * nchan + 255) / 256)), dim3(256),
+                               0, st, (const double2 *)d->d_Bg, dD, (const int *)d_amap, gb, np,
+                               nchan, gx, esp);
+            if (bf16)
+                hipLaunchKernelGGL((k_phase_weights<true, 2>), g, dim3(256), 0, st, phi, np, nk,
+                                   nchan, nrb, nkstep, (const int *)d_amap, nant, dD,
+                                   (const double2 *)d->d_Bg, (const int *)esp, Af, gb);
+            else
+                hipLaunchKernelGGL((k_phase_weights<false, 2>), g, dim3(256), 0, st, phi, np, nk,
+                                   nchan, nrb, nkstep, (const int *)d_amap, nant, dD,
+                                   (const double2 *)d->d_Bg, (const int *)esp, Af, gb);
+            HIPCHECK(hipStreamSynchronize(st)); // the host slot array is released
+            free(hs);
+        } else if (pb == 1) {
             hipLaunchKernelGGL(k_pb_dinv, dim3((unsigned)((nant * nchan + 255) / 256)), dim3(256),
-                               0, st, dD, nant, nchan, Jf, gw);
+                               0, st, dD, nant, nchan, (const int *)NULL, Jf, gw);
             hipLaunchKernelGGL(k_pb_beam, dim3((unsigned)((np * nchan + 255) / 256)), dim3(256), 0,
                                st, dD, dB, np, nant, nchan, bx, esp);
             if (bf16)
-                hipLaunchKernelGGL((k_phase_weights<true, true>), g, dim3(256), 0, st, phi, np,
+                hipLaunchKernelGGL((k_phase_weights<true, 1>), g, dim3(256), 0, st, phi, np,
                                    nant, nchan, nrb, nkstep, (const int *)NULL, nant, dD, dB,
-                                   (const int *)esp, Af);
+                                   (const int *)esp, Af, none);
             else
-                hipLaunchKernelGGL((k_phase_weights<false, true>), g, dim3(256), 0, st, phi, np,
+                hipLaunchKernelGGL((k_phase_weights<false, 1>), g, dim3(256), 0, st, phi, np,
                                    nant, nchan, nrb, nkstep, (const int *)NULL, nant, dD, dB,
-                                   (const int *)esp, Af);
+                                   (const int *)esp, Af, none);
         } else {
             hipLaunchKernelGGL(k_jf, dim3((unsigned)((nant * nchan + 255) / 256)), dim3(256), 0,
                                st, (const double2 *)vm->d_J, nant, nchan, (const int *)NULL, Jf);
             if (bf16)
-                hipLaunchKernelGGL((k_phase_weights<true, false>), g, dim3(256), 0, st, phi, np,
+                hipLaunchKernelGGL((k_phase_weights<true, 0>), g, dim3(256), 0, st, phi, np,
                                    nant, nchan, nrb, nkstep, (const int *)NULL, nant,
                                    (const double2 *)NULL, (const double2 *)NULL,
-                                   (const int *)NULL, Af);
+                                   (const int *)NULL, Af, none);
             else
-                hipLaunchKernelGGL((k_phase_weights<false, false>), g, dim3(256), 0, st, phi, np,
+                hipLaunchKernelGGL((k_phase_weights<false, 0>), g, dim3(256), 0, st, phi, np,
                                    nant, nchan, nrb, nkstep, (const int *)NULL, nant,
                                    (const double2 *)NULL, (const double2 *)NULL,
-                                   (const int *)NULL, Af);
+                                   (const int *)NULL, Af, none);
         }
         HIPCHECK(hipGetLastError());
         vm_timer_stop(d, VM_T_BF_WEIGHTS);
@@ -1375,14 +1804,18 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     }
     // u = J^-1 v (D^-1 v), auto-correlation terms, block scales and the B operand
     vm_timer_start(d, VM_T_BF_PREP);
-    launch_prep(bf16, st, (const cf *)vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw,
-                vm->d_polQ_idxs, vm->d_polP_idxs, nant, Bf, sexp, N);
+    if (pb == 2)
+        launch_prep(bf16, st, (const cf *)vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw, d_sq, d_sp,
+                    nk, Bf, sexp, N, 2, gb);
+    else
+        launch_prep(bf16, st, (const cf *)vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw,
+                    vm->d_polQ_idxs, vm->d_polP_idxs, nant, Bf, sexp, N, pb, gb);
     vm_timer_stop(d, VM_T_BF_PREP);
     pl->nant = nant; pl->ni = ni; pl->nchan = nchan; pl->np = np; pl->nkstep = nkstep;
     pl->nrb = nrb; pl->ns = ns; pl->nsb = nsb;
     pl->chunk = vm->chunk_to_load % vm->chunks_per_second;
     pl->Af = Af; pl->Bf = Bf; pl->N = N; pl->sexp = sexp;
-    pl->pb = pb; pl->bx = bx; pl->esp = esp;
+    pl->pb = pb; pl->bx = bx; pl->esp = esp; pl->gx = gx; pl->gb = gb;
     pl->invw = (float)(1.0 / (double)vm->num_not_flagged);
 }
 
@@ -1390,6 +1823,7 @@ static bool variant_ok(int vi, const mfma_plan *pl)
 {
     // sample counts that are not multiples of 4 have only variant 0
     if (pl->ns & 3 && vi != 0) return false;
+    if (pl->pb == 2 && !variant_has_groups(vi)) return false;
     return vi >= 0 && vi < N_VARIANTS && pl->nrb % (kVariants[vi].mb * kVariants[vi].wr) == 0;
 }
 
@@ -1405,16 +1839,19 @@ static void mfma_run(vcsbeam_context *vm, const mfma_plan *pl, int vi, int order
     G.Af = pl->Af; G.Bf = pl->Bf; G.sexp = pl->sexp; G.N = pl->N; G.S = (float *)vm->d_S;
     G.nkstep = pl->nkstep; G.nrb = pl->nrb; G.npointing = pl->np; G.nstokes = vm->out_nstokes;
     G.soffset = pl->chunk * pl->ns; G.ns_total = vm->fine_sample_rate; G.invw = pl->invw;
-    G.bx = pl->bx; G.esp = pl->esp; G.pad = 0;
+    G.bx = pl->bx; G.esp = pl->esp; G.gx = pl->gx; G.gb = pl->gb; G.pad = 0;
     const bool bf16 = vm->beamform_form == VM_BF_MFMA_BF16;
-    if (bf16 && pl->pb)
-        launch_variant<true, true>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
-    else if (bf16)
-        launch_variant<true, false>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
-    else if (pl->pb)
-        launch_variant<false, true>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
-    else
-        launch_variant<false, false>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order);
+#define VM_RUN(B_, P_) launch_variant<B_, P_>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order)
+    if (bf16) {
+        if (pl->pb == 2) VM_RUN(true, 2);
+        else if (pl->pb == 1) VM_RUN(true, 1);
+        else VM_RUN(true, 0);
+    } else {
+        if (pl->pb == 2) VM_RUN(false, 2);
+        else if (pl->pb == 1) VM_RUN(false, 1);
+        else VM_RUN(false, 0);
+    }
+#undef VM_RUN
     HIPCHECK(hipGetLastError());
 }
 
@@ -1690,16 +2127,17 @@ static void batch_refresh(vm_batch *b)
         }
         const size_t total = (size_t)nchan * nrb * nkstep * 64;
         const dim3 gw((unsigned)((total + 255) / 256));
+        const grp_bounds none = {};
         if (b->bf16)
-            hipLaunchKernelGGL((k_phase_weights<true, false>), gw, dim3(256), 0, st,
+            hipLaunchKernelGGL((k_phase_weights<true, 0>), gw, dim3(256), 0, st,
                                (const double2 *)vm->gdelays.d_phi, np, nact, nchan, nrb, nkstep,
                                (const int *)C.d_amap, nant, (const double2 *)NULL,
-                               (const double2 *)NULL, (const int *)NULL, C.Af);
+                               (const double2 *)NULL, (const int *)NULL, C.Af, none);
         else
-            hipLaunchKernelGGL((k_phase_weights<false, false>), gw, dim3(256), 0, st,
+            hipLaunchKernelGGL((k_phase_weights<false, 0>), gw, dim3(256), 0, st,
                                (const double2 *)vm->gdelays.d_phi, np, nact, nchan, nrb, nkstep,
                                (const int *)C.d_amap, nant, (const double2 *)NULL,
-                               (const double2 *)NULL, (const int *)NULL, C.Af);
+                               (const double2 *)NULL, (const int *)NULL, C.Af, none);
         int hflag = 0;
         if (np > 1) hflag = pointings_differ(vm, st, queue_flag);
         HIPCHECK(hipGetLastError());
@@ -1748,9 +2186,10 @@ static void batch_pfb(vm_batch *b, int g, hipStream_t st)
 static void batch_prep(vm_batch *b, int g, hipStream_t st)
 {
     const vm_batch_chan &C = b->ch[g];
+    const grp_bounds none = {};
     launch_prep(b->bf16, st, (const cf *)C.vm->fpfb->d_vcs_data, b->ns,
                 (int)C.vm->obs_metadata->num_rf_inputs, b->nsb, C.nkstep, b->nchan, C.Jf,
-                (const float4 *)NULL, C.d_polQ, C.d_polP, C.nact, C.Bf, C.sexp, C.N);
+                (const float4 *)NULL, C.d_polQ, C.d_polP, C.nact, C.Bf, C.sexp, C.N, 0, none);
 }
 
 static void batch_gemm(vm_batch *b, hipStream_t st)

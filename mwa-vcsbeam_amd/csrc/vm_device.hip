@@ -70,6 +70,9 @@ void vm_dev_free(vcsbeam_context *vm)
     dfree(d->d_pos);
     dfree(d->d_freqs);
     dfree(d->d_B);
+    dfree(d->d_grp);
+    dfree(d->d_Bg);
+    dfree(d->d_gx);
     dfree(d->d_unit);
     dfree(d->d_ipfb_out);
     dfree(d->d_nterm);
@@ -249,11 +252,30 @@ void vmPushChunk(vcsbeam_context *vm)
 
 void vmPushJ(vcsbeam_context *vm)
 {
-    hipStream_t s = (hipStream_t)vm_dev(vm)->stream;
+    vm_device_state *d = vm_dev(vm);
+    hipStream_t s = (hipStream_t)d->stream;
     HIPCHECK(hipMemcpyAsync(vm->d_J, vm->J, vm->J_size_bytes, hipMemcpyHostToDevice, s));
+    d->J_fromB = 0;
+    if (d->J_host_fromB && vm->pb.B && vm->D && vm->d_D &&
+        vm->pb.npointings == vm->npointing && vm->pb.nant == vm->obs_metadata->num_ants) {
+        /* J^-1 came from vmCalcJ, i.e. from D and the primary beam pb.B:
+         * hand both to the device as well, so that the precision form can
+         * factor J^-1 = F B^-1 D^-1 when the beam differs between tiles */
+        const size_t bbytes = (size_t)vm->npointing * vm->pb.nant * 4 * sizeof(gpuDoubleComplex);
+        if (d->B_bytes < bbytes) {
+            dfree(d->d_B);
+            d->d_B = dmalloc(bbytes);
+            d->B_bytes = bbytes;
+        }
+        HIPCHECK(hipMemcpyAsync(d->d_B, vm->pb.B, bbytes, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(vm->d_D, vm->D, vm->D_size_bytes, hipMemcpyHostToDevice, s));
+        d->B_chan = 0;
+        d->B_np = (int)vm->npointing;
+        d->J_fromB = 1;
+    }
+    d->J_host_fromB = 0;
     HIPCHECK(hipStreamSynchronize(s));
-    vm->dev->weights_gen++;
-    vm->dev->J_fromB = 0;
+    d->weights_gen++;
 }
 
 void vmPushD(vcsbeam_context *vm)

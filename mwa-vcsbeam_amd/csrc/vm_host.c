@@ -310,6 +310,8 @@ void vmCalcJ(vcsbeam_context *vm)
                 if (F != 0.0) inv2x2S(Ji, out);
                 else for (int q = 0; q < 4; q++) out[q] = C(0.0, 0.0);
             }
+    /* the next vmPushJ hands D and B to the device as well */
+    if (vm->dev) vm->dev->J_host_fromB = 1;
 }
 
 /* src/jones.c:170-188: geometry at the middle of the current second, then

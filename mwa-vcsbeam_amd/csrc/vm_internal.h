@@ -77,6 +77,22 @@ struct vm_device_state {
        1: [B_np][c][4]). */
     int J_fromB;
     int B_np;
+    /* J^-1 on the host was last computed by vmCalcJ from D and pb.B (set
+       there, taken by the next vmPushJ, which then also hands D and B to the
+       device) */
+    int J_host_fromB;
+    /* MFMA tile-group form: GEMM slot arrays (antenna, polQ, polP per
+       slot), the groups' beams [p][g][c][4] complex double and per-(c, p)
+       constants; the groups' slot bounds and representative antennas */
+    void *d_grp;
+    size_t grp_bytes;
+    void *d_Bg;
+    size_t Bg_bytes;
+    void *d_gx;
+    size_t gx_bytes;
+    int grp_t[5];
+    int grp_rep[4];
+    int grp_n;
     /* unit vectors per pointing (uE, uN, uH) */
     double *d_unit;
     size_t unit_bytes;

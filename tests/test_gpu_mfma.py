@@ -155,3 +155,112 @@ def test_mfma_per_pointing_primary_beam(pkg, orc, gpu_lib, form, tol_i, tol_quv)
     print("max per-beam rel err I,Q,U,V:", err.max(axis=0))
     assert err[:, 0].max() <= tol_i
     assert err[:, 1:].max() <= tol_quv
+
+
+def _tile_config_case(ntiles=128):
+    from vcsbeam_amd.workloads import Observation
+    dead = {a: [3] for a in range(5, 13)}
+    dead.update({a: [0, 9] for a in range(40, 46)})
+    return Observation(ntiles=ntiles, blocks_per_second=2, dipole_delays=[0, 1, 2, 3] * 4,
+                       dead_dipoles=dead)
+
+
+def _tile_config_oracle(orc, obs, az, za, B):
+    from vcsbeam_amd.workloads import unit_vectors
+    D = obs.calibration()
+    cur = obs.second(0)
+    htr = np.ascontiguousarray(np.concatenate([np.zeros_like(cur[:1]), cur], axis=0))
+    vv = orc.pfb_forward(htr, obs.ninput, obs.nspectra, obs.filter.astype(np.int32),
+                         obs.out_idx(), fpga=True)
+    E, N, H, L = obs.positions()
+    phi = orc.calc_phi(E, N, H, L, unit_vectors(az, za), obs.freqs)
+    polQ, polP = obs.pol_idx()
+    Sref, _ = orc.beamform(vv, orc.calc_J(D, B), phi, polQ, polP,
+                           1.0 / (2 * int((~obs.flag_mask).sum())), 4)
+    return Sref
+
+
+def _host_beams(lib, ses, npointing, ntiles):
+    import ctypes as C
+    B = np.ctypeslib.as_array(C.cast(ses.vm.contents.pb.B, C.POINTER(C.c_double)),
+                              (npointing, ntiles, 2, 2, 2)).copy()
+    return B[..., 0] + 1j * B[..., 1]
+
+
+def test_mfma_tile_groups_every_variant_and_order(pkg, orc, gpu_lib):
+    """The tile-group form in every GEMM variant that has it (register-ring
+    k-loop, at most four wavefronts) and every tile order, against the FP64
+    oracle chain, to the FP16 bar."""
+    from vcsbeam_amd.apps import TiedArraySession
+    from vcsbeam_amd.workloads import directions_in_cap
+    lib = gpu_lib
+    obs = _tile_config_case()
+    az, za = directions_in_cap(40, 15.0)
+    ses = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
+                           form=pkg.VM_BF_MFMA_FP16)
+    try:
+        ses.read(0)
+        ses.pfb()
+        lib.vmCalcB(ses.vm, ses.bg)
+        B = _host_beams(lib, ses, len(az), obs.ntiles)
+        Sref = _tile_config_oracle(orc, obs, az, za, B)
+        for variant in (0, 3, 4, 5, 8):
+            for order in range(5):
+                print(f"tile groups: variant {variant} order {order}", flush=True)
+                lib.vmSetBeamformerVariant(ses.vm, variant, order)
+                ses.geometry()
+                ses.beamform()
+                err = _errors(orc, ses.stokes(), Sref)
+                print("   max per-beam rel err I,Q,U,V:", err.max(axis=0))
+                assert err[:, 0].max() <= 1e-3
+                assert err[:, 1:].max() <= 3e-3
+    finally:
+        ses.close()
+
+
+@pytest.mark.parametrize("path", ["device", "host"])
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
+def test_mfma_primary_beam_per_tile_configuration(pkg, orc, gpu_lib, form, tol_i, tol_quv, path):
+    """A primary beam per tile from its dipole configuration (vmCalcB):
+    three configurations over 128 tiles (all live; dipole 3 dead on tiles
+    5-12; dipoles 0 and 9 dead on tiles 40-45), so J^-1 = F (D B)^-1 differs
+    between pointings and between tiles.  The MFMA form groups the tiles by
+    configuration, runs the GEMM over the groups in turn with the Horner
+    carry between them and forms each group's auto-correlation term; it must
+    match the FP64 oracle chain (J^-1 from D and the per-tile B) to the same
+    bars as the beam-independent form.  J^-1 reaches the device either from
+    D and B there (vmCalcJDevice) or as the reference's drivers compute it,
+    on the host (vmCalcJ + vmPushJ).  Workload: 128 tiles, 0.1 s, 40
+    directions within 15 deg."""
+    from vcsbeam_amd.apps import TiedArraySession
+    from vcsbeam_amd.workloads import directions_in_cap
+    lib = gpu_lib
+    obs = _tile_config_case()
+    az, za = directions_in_cap(40, 15.0)
+    print(f"MFMA form {form}, per-tile beams of 3 configurations, {path} J^-1", flush=True)
+    ses = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
+                           form=form)
+    try:
+        ses.read(0)
+        ses.pfb()
+        lib.vmCalcB(ses.vm, ses.bg)
+        B = _host_beams(lib, ses, len(az), obs.ntiles)
+        if path == "device":
+            lib.vmCalcPhiDevice(ses.vm, ses.bg)
+            lib.vmCalcJDevice(ses.vm)
+        else:
+            lib.vmCalcPhi(ses.vm, ses.bg)
+            lib.vmCalcJ(ses.vm)
+            lib.vmPushPhi(ses.vm)
+            lib.vmPushJ(ses.vm)
+        ses.beamform()
+        S = ses.stokes()
+    finally:
+        ses.close()
+    # three distinct beams per pointing, as configured
+    assert np.abs(B[:, 0] - B[:, 5]).max() > 1e-3 and np.abs(B[:, 5] - B[:, 40]).max() > 1e-3
+    assert np.array_equal(B[:, 5], B[:, 12]) and np.array_equal(B[:, 0], B[:, 127])
+    err = _errors(orc, S, _tile_config_oracle(orc, obs, az, za, B))
+    print("max per-beam rel err I,Q,U,V:", err.max(axis=0))
+    assert err[:, 0].max() <= tol_i
+    assert err[:, 1:].max() <= tol_quv
