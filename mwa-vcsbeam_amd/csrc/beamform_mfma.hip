@@ -508,8 +508,19 @@ __global__ void k_jcheck(const double2 *__restrict__ J, uint32_t per_pointing,
 // group): "antennas" are GEMM slots (polQ, polP, Jf and gw given per slot,
 // padding slots with zero Jf), and the T_k are summed per group g of gb
 // into float4 [c][s][VM_MFMA_MAX_GROUPS][4].
-template <bool BF16, int APT, int NG, int PB>
-__global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, int ns, int ni, int nsb,
+// I4: legacy 4+4-bit voltages [s][c][i] (one byte per input-sample, high
+// nibble re, low nibble im, two's complement; include/vcsbeam.h.in:997-1003)
+// instead of VM_FLT [c][i][s]: the block's 32 rows of ni bytes are staged in
+// LDS with coalesced loads and each lane decodes its inputs from there.
+__device__ __forceinline__ cf nibbles_to_cf(uint32_t b)
+{
+    const int re = (int)((b >> 4) & 0xfu), im = (int)(b & 0xfu);
+    return cf{(float)(re >= 8 ? re - 16 : re), (float)(im >= 8 ? im - 16 : im)};
+}
+
+#define PREP_I4_MAX_INPUTS 512
+template <bool BF16, int APT, int NG, int PB, bool I4>
+__global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv, int ns, int ni, int nsb,
                                                  int nkstep, const cf *__restrict__ Jf,
                                                  const float4 *__restrict__ gw,
                                                  const uint32_t *__restrict__ polQ,
@@ -518,6 +529,8 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, i
                                                  float4 *__restrict__ N, grp_bounds gb)
 {
     constexpr int NGRP = PB == 2 ? VM_MFMA_MAX_GROUPS : 1;
+    __shared__ uint32_t vrow[I4 ? SB * PREP_I4_MAX_INPUTS / 4 : 1];
+    const cf *v = (const cf *)vv;
     typedef mtype<BF16> M;
     constexpr int NT = 32 * NG; // lanes
     __shared__ float4 red[NG][SB];
@@ -531,6 +544,26 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, i
     const int nfrag = nkstep * 2 * 64;
 
     for (int k = t; k < nfrag; k += NT) frag[k] = make_uint4(0u, 0u, 0u, 0u);
+    if constexpr (I4) {
+        // rows (s, c) of ni bytes for the block's samples; rows past ns zero
+        const uint8_t *vb = (const uint8_t *)vv;
+        const int nchan = (int)gridDim.y;
+        uint8_t *lb = (uint8_t *)vrow;
+        if ((ni & 3) == 0) {
+            const int dw = ni >> 2;
+            for (int k = t; k < SB * dw; k += NT) {
+                const int r = k / dw, q = k % dw, sr = SB * sb + r;
+                vrow[r * dw + q] = sr < ns ? ((const uint32_t *)(vb + ((size_t)sr * nchan + c) * ni))[q]
+                                           : 0u;
+            }
+        } else {
+            for (int k = t; k < SB * ni; k += NT) {
+                const int r = k / ni, q = k % ni, sr = SB * sb + r;
+                lb[k] = sr < ns ? vb[((size_t)sr * nchan + c) * ni + q] : (uint8_t)0;
+            }
+        }
+        __syncthreads();
+    }
 
     cf ux[APT], uy[APT];
 #pragma unroll
@@ -538,8 +571,14 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const cf *__restrict__ v, i
         const int a = g + NG * j;
         cf q = {0.f, 0.f}, p = {0.f, 0.f};
         if (ok && a < nant) {
-            q = vc[(size_t)polQ[a] * ns + s];
-            p = vc[(size_t)polP[a] * ns + s];
+            if constexpr (I4) {
+                const uint8_t *lb = (const uint8_t *)vrow + sl * ni;
+                q = nibbles_to_cf(lb[polQ[a]]);
+                p = nibbles_to_cf(lb[polP[a]]);
+            } else {
+                q = vc[(size_t)polQ[a] * ns + s];
+                p = vc[(size_t)polP[a] * ns + s];
+            }
         }
         ux[j] = uy[j] = cf{0.f, 0.f};
         if (a < nant) {
@@ -1510,16 +1549,26 @@ static void ensure_buf(void **p, size_t *cap, size_t need)
 // from nant).  pbm 1: the per-pointing primary-beam form (Jf = D^-1, N =
 // T_k); pbm 2: the tile-group form (per-slot Jf = D^-1 and gw, N = T_gk of
 // the groups of gb).
-static void launch_prep(bool bf16, hipStream_t st, const cf *v, int ns, int ni, int nsb,
+// int4: legacy 4+4-bit voltages [s][c][i] in v (else VM_FLT [c][i][s]).
+static void launch_prep(bool bf16, hipStream_t st, const void *v, int ns, int ni, int nsb,
                         int nkstep, int nchan, const cf *Jf, const float4 *gw,
                         const uint32_t *polQ, const uint32_t *polP, int nant, uint4 *Bf, int *sexp,
-                        float4 *N, int pbm, const grp_bounds &gb)
+                        float4 *N, int pbm, const grp_bounds &gb, bool int4 = false)
 {
     if (nant > 256) VM_FATAL("MFMA beamformer: at most 256 antennas (%d)", nant);
+    if (int4 && ni > PREP_I4_MAX_INPUTS)
+        VM_FATAL("MFMA beamformer: at most %d RF inputs for 4+4-bit input (%d)",
+                 PREP_I4_MAX_INPUTS, ni);
     const dim3 pg((unsigned)nsb, (unsigned)nchan);
 #define VM_PREP(B_, A_, G_, P_)                                                               \
-    hipLaunchKernelGGL((k_bf_prep<B_, A_, G_, P_>), pg, dim3(32 * G_), 0, st, v, ns, ni, nsb,  \
-                       nkstep, Jf, gw, polQ, polP, nant, Bf, sexp, N, gb)
+    do {                                                                                       \
+        if (int4)                                                                              \
+            hipLaunchKernelGGL((k_bf_prep<B_, A_, G_, P_, true>), pg, dim3(32 * G_), 0, st, v,  \
+                               ns, ni, nsb, nkstep, Jf, gw, polQ, polP, nant, Bf, sexp, N, gb); \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_bf_prep<B_, A_, G_, P_, false>), pg, dim3(32 * G_), 0, st, v, \
+                               ns, ni, nsb, nkstep, Jf, gw, polQ, polP, nant, Bf, sexp, N, gb); \
+    } while (0)
 #define VM_PREP2(A_, G_)                                                                      \
     if (pbm == 2) { if (bf16) VM_PREP(true, A_, G_, 2); else VM_PREP(false, A_, G_, 2); }     \
     else if (pbm == 1) { if (bf16) VM_PREP(true, A_, G_, 1); else VM_PREP(false, A_, G_, 1); } \
@@ -1640,9 +1689,10 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     vm_device_state *d = vm_dev(vm);
     hipStream_t st = (hipStream_t)d->stream;
     const bool bf16 = vm->beamform_form == VM_BF_MFMA_BF16;
-    if (vm->datatype != VM_FLT)
-        VM_FATAL("MFMA beamformer needs VM_FLT voltages (forward PFB output); got %d",
-                 (int)vm->datatype);
+    if (vm->datatype != VM_FLT && vm->datatype != VM_INT4)
+        VM_FATAL("MFMA beamformer needs VM_FLT (forward PFB output) or VM_INT4 (legacy) "
+                 "voltages; got %d", (int)vm->datatype);
+    const bool int4 = vm->datatype == VM_INT4;
     const int nant = (int)vm->obs_metadata->num_ants, ni = (int)vm->obs_metadata->num_rf_inputs;
     const int nchan = vm->nfine_chan, np = (int)vm->npointing;
     if (ni != 2 * nant) VM_FATAL("MFMA beamformer: %d RF inputs for %d antennas", ni, nant);
@@ -1805,11 +1855,11 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     // u = J^-1 v (D^-1 v), auto-correlation terms, block scales and the B operand
     vm_timer_start(d, VM_T_BF_PREP);
     if (pb == 2)
-        launch_prep(bf16, st, (const cf *)vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw, d_sq, d_sp,
-                    nk, Bf, sexp, N, 2, gb);
+        launch_prep(bf16, st, vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw, d_sq, d_sp, nk, Bf,
+                    sexp, N, 2, gb, int4);
     else
-        launch_prep(bf16, st, (const cf *)vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw,
-                    vm->d_polQ_idxs, vm->d_polP_idxs, nant, Bf, sexp, N, pb, gb);
+        launch_prep(bf16, st, vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw, vm->d_polQ_idxs,
+                    vm->d_polP_idxs, nant, Bf, sexp, N, pb, gb, int4);
     vm_timer_stop(d, VM_T_BF_PREP);
     pl->nant = nant; pl->ni = ni; pl->nchan = nchan; pl->np = np; pl->nkstep = nkstep;
     pl->nrb = nrb; pl->ns = ns; pl->nsb = nsb;

@@ -264,3 +264,32 @@ def test_mfma_primary_beam_per_tile_configuration(pkg, orc, gpu_lib, form, tol_i
     print("max per-beam rel err I,Q,U,V:", err.max(axis=0))
     assert err[:, 0].max() <= tol_i
     assert err[:, 1:].max() <= tol_quv
+
+
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
+def test_mfma_legacy_int4_input(pkg, orc, gpu_lib, form, tol_i, tol_quv):
+    """Legacy recombined 4+4-bit input ([s][c][i], one byte per
+    input-sample): the prep pass decodes the nibbles itself (high = re,
+    low = im).  Against the FP64 oracle beamformer on the same bytes
+    (src/form_beam.cpp:122-204), to the MFMA bars.  Workload: 128 tiles,
+    1000 samples x 128 fine channels, 40 directions within 15 deg."""
+    from vcsbeam_amd.apps import tied_array_beam
+    from vcsbeam_amd.workloads import Observation, directions_in_cap, unit_vectors
+    obs = Observation(ntiles=128, blocks_per_second=2, legacy=True)
+    az, za = directions_in_cap(40, 15.0)
+    print(f"MFMA form {form} on legacy 4+4-bit input, {len(az)} beams", flush=True)
+    out = tied_array_beam(obs, az, za, nstokes=4, form=form, want_e=False, device_geometry=True)
+    ns = obs.vmd.num_samples_per_voltage_block
+    v = obs.legacy_second(0).reshape(ns, 128, obs.ninput)
+    D = obs.calibration()
+    B = np.zeros((len(az), obs.ntiles, 2, 2), complex)
+    B[..., 0, 0] = B[..., 1, 1] = 1
+    E, N, H, L = obs.positions()
+    phi = orc.calc_phi(E, N, H, L, unit_vectors(az, za), obs.freqs)
+    polQ, polP = obs.pol_idx()
+    Sref, _ = orc.beamform(v, orc.calc_J(D, B), phi, polQ, polP,
+                           1.0 / (2 * int((~obs.flag_mask).sum())), 4, int4=True)
+    err = _errors(orc, out["S"][0], Sref)
+    print("max per-beam rel err I,Q,U,V:", err.max(axis=0))
+    assert err[:, 0].max() <= tol_i
+    assert err[:, 1:].max() <= tol_quv
