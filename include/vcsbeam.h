@@ -1056,10 +1056,17 @@ void vmSynthMWAXDevice(uint64_t seed, uint64_t first_block, int nblocks, int nin
                        double sigma, int zero_blocks, void *d_out);
 
 /* Device-resident inverse PFB of pointing 0 straight from d_e (no host
- * round trip); output complex float [n][pol] on the device, copied to
- * data_buffer_vdif when it is not NULL. */
+ * round trip), one fused kernel that also keeps the last P fine samples for
+ * the next second; output complex float [n][pol] on the device (g->d_out),
+ * copied to data_buffer_vdif when it is not NULL.  Seconds are expected in
+ * order (file_no 0, 1, ...), as cu_invert_pfb expects them. */
 void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g,
                        int file_no, float *data_buffer_vdif);
+/* The same, and the output's 8-bit VDIF samples (as vmRequantVDIFDevice
+ * makes them from g->d_out) written by the same kernel into d_vdif
+ * (device, n x pol x 2 bytes). */
+void vmInvertPFBDeviceVDIF(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_no,
+                           uint8_t *d_vdif);
 
 /* Device VDIF requantisation of n floats (float2int8_trunc + offset binary). */
 void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n,

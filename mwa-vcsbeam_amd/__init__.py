@@ -351,6 +351,7 @@ _SIGS = {
     "cu_invert_pfb": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                              C.POINTER(GpuIpfbArrays), C.c_void_p]),
     "vmInvertPFBDevice": (None, [_CTX, C.POINTER(GpuIpfbArrays), C.c_int, C.c_void_p]),
+    "vmInvertPFBDeviceVDIF": (None, [_CTX, C.POINTER(GpuIpfbArrays), C.c_int, C.c_void_p]),
     "vmRequantVDIFDevice": (None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "create_data_buffer_fine": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "prepare_data_buffer_fine": (None, [C.c_void_p, _CTX, C.c_size_t]),

@@ -582,9 +582,8 @@ class TiedArraySession:
 
     def invert(self, t):
         """Inverse PFB of pointing 0 from the device e (second t) and its 8-bit
-        VDIF requantisation, both on the device."""
-        self.lib.vmInvertPFBDevice(self.vm, C.byref(self.gi), t, None)
-        self.lib.vmRequantVDIFDevice(self.gi.d_out, self.d_vdif, self.nvdif, self.stream)
+        VDIF requantisation, both on the device in one kernel."""
+        self.lib.vmInvertPFBDeviceVDIF(self.vm, C.byref(self.gi), t, self.d_vdif)
         self.lib.vmSynchronize(self.vm)
 
     def geometry(self):

@@ -60,36 +60,22 @@ static void ensure_twiddles_i()
     done = 1;
 }
 
-// in: complex [s_in][k][pol] for s_in in [0, ns + P), element type T
-// (double2 for the host-buffer path, float2 for the device path).
-// hK: polyphase filter f[qK + r] / K as float, length P*K.
-// out: float [n][pol][2], n in [0, ns*K).
-// grid (ceil(ns/TJ), npol), block 256.
-template <typename T>
-__global__ __launch_bounds__(256) void k_ipfb(const T *__restrict__ in, int ns, int P, int npol,
-                                              const float *__restrict__ hK,
-                                              float *__restrict__ out)
+// VDIF requantisation of one value: clamp to [-126, 127], round half to
+// even, int8, offset binary (src/beam_vdif.c:29-38, 254-260)
+__device__ __forceinline__ uint8_t vdif_q1(float v)
 {
-    __shared__ cf Z[(TJ + MAXP - 1) * ROW];
-    const int t = threadIdx.x;
-    const int j0 = blockIdx.x * TJ;
-    const int pol = blockIdx.y;
-    const int nrows = TJ + P - 1; // s_in = j0 + 1 + row
-    const int jmax = min(TJ, ns - j0);
+    const float x = fmaxf(fminf(v, 127.0f), -126.0f);
+    return (uint8_t)((int8_t)(int)rintf(x) ^ 0x80);
+}
 
-    // load conj(X) for rows whose s_in exists
-    for (int idx = t; idx < nrows * KCH; idx += 256) {
-        int row = idx >> 7, k = idx & 127;
-        int s_in = j0 + 1 + row;
-        cf v = {0.f, 0.f};
-        if (s_in < ns + P) {
-            T x = in[((size_t)s_in * KCH + k) * npol + pol];
-            v = cf{(float)x.x, -(float)x.y};
-        }
-        Z[row * ROW + k] = v;
-    }
-    __syncthreads();
-
+// The inverse DFTs of the nrows rows in Z (conj(X) in, Z out) and the
+// polyphase FIR into out (and, when out8 is not NULL, the same values
+// requantised for VDIF at the same [n][pol][re, im] positions).
+// The inverse DFTs of the nrows rows in Z, by the 256 threads of one
+// polarisation (every thread of the workgroup passes the same barriers).
+__device__ __forceinline__ void ipfb_idft(cf *Z, int nrows)
+{
+    const int t = threadIdx.x & 255;
     // inverse DFTs, 32 rows per pass
     const int f = t & 31, l = t >> 5;
     for (int base = 0; base < nrows; base += 32) {
@@ -112,56 +98,154 @@ __global__ __launch_bounds__(256) void k_ipfb(const T *__restrict__ in, int ns, 
         }
         __syncthreads();
     }
+}
 
+// PC: the number of taps when known at compile time (0: P at run time, up
+// to MAXP)
+template <int PC = 0>
+__device__ __forceinline__ void ipfb_rows(cf *Z, int nrows, int Prt, int jmax, int j0, int pol,
+                                          int npol, const float *__restrict__ hK,
+                                          float *__restrict__ out, uint8_t *__restrict__ out8)
+{
+    constexpr int NQ = PC ? PC : MAXP;
+    const int P = PC ? PC : Prt;
+    const int t = threadIdx.x & 255; // the 256 threads of this polarisation
+    ipfb_idft(Z, nrows);
     // polyphase FIR: out[(j0+jj)K + r] = sum_q hK[qK + r] Z[jj - q + P - 1][r]
     const int r = t & 127;
-    float hr[MAXP];
+    float hr[NQ];
 #pragma unroll
-    for (int q = 0; q < MAXP; q++) hr[q] = (q < P) ? hK[q * KCH + r] : 0.f;
+    for (int q = 0; q < NQ; q++) hr[q] = (q < P) ? hK[q * KCH + r] : 0.f;
     for (int jj = t >> 7; jj < jmax; jj += 2) {
         float ar = 0.f, ai = 0.f;
+        const cf *zc = Z + (jj + P - 1) * ROW + r;
 #pragma unroll
-        for (int q = 0; q < MAXP; q++) {
-            if (q < P) {
-                cf z = Z[(jj - q + P - 1) * ROW + r];
+        for (int q = 0; q < NQ; q++) {
+            if (PC || q < P) {
+                const cf z = zc[-q * ROW];
                 ar = fmaf(hr[q], z.x, ar);
                 ai = fmaf(hr[q], z.y, ai);
             }
         }
         size_t n = (size_t)(j0 + jj) * KCH + r;
         *(float2 *)&out[(n * npol + pol) * 2] = make_float2(ar, ai);
+        if (out8)
+            *(uchar2 *)&out8[(n * npol + pol) * 2] = make_uchar2(vdif_q1(ar), vdif_q1(ai));
     }
 }
 
-// Gather the device-resident IPFB input from the beamformer output d_e
-// ([c][s][pol] complex float for pointing 0) and the P-sample history
-// ([P][c][pol]); zero history when file_no == 0.
-__global__ void k_ipfb_gather(const cf *__restrict__ e, const cf *__restrict__ hist, int ns, int P,
-                              int npol, int file_no, cf *__restrict__ in)
+// in: complex [s_in][k][pol] for s_in in [0, ns + P), element type T
+// (double2 for the host-buffer path, float2 for the device path).
+// hK: polyphase filter f[qK + r] / K as float, length P*K.
+// out: float [n][pol][2], n in [0, ns*K).
+// grid (ceil(ns/TJ), npol), block 256.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ipfb(const T *__restrict__ in, int ns, int P, int npol,
+                                              const float *__restrict__ hK,
+                                              float *__restrict__ out)
 {
-    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t total = (size_t)(ns + P) * KCH * npol;
-    if (idx >= total) return;
-    int pol = (int)(idx % npol);
-    int k = (int)((idx / npol) % KCH);
-    int s_in = (int)(idx / ((size_t)npol * KCH));
-    cf v;
-    if (s_in < P) v = (file_no == 0) ? cf{0.f, 0.f} : hist[((size_t)s_in * KCH + k) * npol + pol];
-    else v = e[((size_t)k * ns + (s_in - P)) * npol + pol];
-    in[idx] = v;
+    __shared__ cf Z[(TJ + MAXP - 1) * ROW];
+    const int t = threadIdx.x;
+    const int j0 = blockIdx.x * TJ;
+    const int pol = blockIdx.y;
+    const int nrows = TJ + P - 1; // s_in = j0 + 1 + row
+    const int jmax = min(TJ, ns - j0);
+
+    // load conj(X) for rows whose s_in exists, eight loads in flight per lane
+    for (int i0 = t; i0 < nrows * KCH; i0 += 8 * 256) {
+        cf v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int idx = i0 + 256 * u;
+            const int row = idx >> 7, k = idx & 127;
+            const int s_in = j0 + 1 + row;
+            v[u] = cf{0.f, 0.f};
+            if (idx < nrows * KCH && s_in < ns + P) {
+                const T x = in[((size_t)s_in * KCH + k) * npol + pol];
+                v[u] = cf{(float)x.x, -(float)x.y};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int idx = i0 + 256 * u;
+            if (idx < nrows * KCH) Z[(idx >> 7) * ROW + (idx & 127)] = v[u];
+        }
+    }
+    __syncthreads();
+
+    ipfb_rows<0>(Z, nrows, P, jmax, j0, pol, npol, hK, out, (uint8_t *)nullptr);
 }
 
-// Keep the last P fine samples of this second as the next second's history.
-__global__ void k_ipfb_save_hist(const cf *__restrict__ e, int ns, int P, int npol,
-                                 cf *__restrict__ hist)
+// Device-resident form: the fine channels straight from the beamformer
+// output e ([k][s][pol] complex float, pointing 0), the rows before the
+// second's first sample from the previous second's last P samples in
+// hist_in ([P][k][pol]; zeros when first), and this second's last P samples
+// copied to hist_out for the next second (the workgroups that load them
+// write them; duplicates carry equal values).  A workgroup of 256 x npol
+// threads (npol <= 2) holds both polarisations, so each lane loads a
+// sample's pols together (16 bytes) and lanes walk the samples of a fine
+// channel; each 256-thread half then runs one polarisation's inverse DFTs
+// and FIR.  It takes 33 - P output fine samples, so that with their P - 1
+// earlier ones its rows are exactly one 32-row pass of inverse DFTs.
+// out8: optional VDIF bytes.  grid ceil(ns / (33 - P)), block 256 x npol.
+#define IPFB_DEV_ROWS 32
+template <int PC>
+__global__ __launch_bounds__(512) void k_ipfb_dev(const cf *__restrict__ e, int ns, int P, int npol,
+                                                  const cf *__restrict__ hist_in, int first,
+                                                  cf *__restrict__ hist_out,
+                                                  const float *__restrict__ hK,
+                                                  float *__restrict__ out, uint8_t *__restrict__ out8)
 {
-    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t total = (size_t)P * KCH * npol;
-    if (idx >= total) return;
-    int pol = (int)(idx % npol);
-    int k = (int)((idx / npol) % KCH);
-    int q = (int)(idx / ((size_t)npol * KCH));
-    hist[idx] = e[((size_t)k * ns + (ns - P + q)) * npol + pol];
+    __shared__ cf Z[2][IPFB_DEV_ROWS * ROW];
+    const int t = threadIdx.x;
+    const int tj = IPFB_DEV_ROWS - (P - 1);
+    const int j0 = blockIdx.x * tj;
+    const int nrows = IPFB_DEV_ROWS; // s_in = j0 + 1 + row
+    const int jmax = min(tj, ns - j0);
+    const int nel = nrows * KCH;      // (row, k) pairs, each with npol values
+    const int nthr = 256 * npol;
+    for (int i0 = t; i0 < nel; i0 += 8 * nthr) {
+        cf v[8][2];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int idx = i0 + nthr * u;
+            const int row = idx % nrows, k = idx / nrows; // lanes along samples
+            const int s_in = j0 + 1 + row;
+            v[u][0] = v[u][1] = cf{0.f, 0.f};
+            if (idx < nel && s_in < ns + P) {
+                const cf *src = s_in < P ? (first ? nullptr : hist_in + ((size_t)s_in * KCH + k) * npol)
+                                         : e + ((size_t)k * ns + (s_in - P)) * npol;
+                if (src) {
+                    if (npol == 2) {
+                        const float4 w = *(const float4 *)src;
+                        v[u][0] = cf{w.x, w.y};
+                        v[u][1] = cf{w.z, w.w};
+                    } else {
+                        v[u][0] = src[0];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int idx = i0 + nthr * u;
+            if (idx >= nel) continue;
+            const int row = idx % nrows, k = idx / nrows;
+            const int s_in = j0 + 1 + row;
+#pragma unroll
+            for (int pol = 0; pol < 2; pol++) {
+                if (pol >= npol) break;
+                if (s_in >= ns && s_in < ns + P) // e sample ns - P + (s_in - ns)
+                    hist_out[((size_t)(s_in - ns) * KCH + k) * npol + pol] = v[u][pol];
+                Z[pol][row * ROW + k] = cf{v[u][pol].x, -v[u][pol].y};
+            }
+        }
+    }
+    __syncthreads();
+    // each 256-thread half: one polarisation's inverse DFTs and FIR (both
+    // halves pass the same barriers inside ipfb_rows)
+    const int pol = t >> 8;
+    ipfb_rows<PC>(Z[pol], nrows, P, jmax, j0, pol, npol, hK, out, out8);
 }
 
 // VDIF requantisation: clamp to [-126, 127], round half to even, int8,
@@ -380,50 +464,64 @@ void cu_invert_pfb(gpuDoubleComplex *data_buffer_fine, int file_no, int npointin
     HIPCHECK(hipMemcpy(data_buffer_vdif, g->d_out, g->out_size, hipMemcpyDeviceToHost));
 }
 
-// Device-resident path: e straight from the beamformer output.
-void vm_ipfb_device_launch(const void *d_in_f32, int ns, int P, int npol, const float *d_hK,
-                           float *d_out, hipStream_t stream)
-{
-    dim3 grid((ns + TJ - 1) / TJ, npol);
-    hipLaunchKernelGGL(k_ipfb<float2>, grid, dim3(256), 0, stream, (const float2 *)d_in_f32, ns, P,
-                       npol, d_hK, d_out);
-    HIPCHECK(hipGetLastError());
-}
 
-void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_no,
-                       float *data_buffer_vdif)
+// The device-resident inverse PFB of pointing 0 (one fused kernel), with the
+// VDIF bytes too when d_vdif is not NULL.  The P-sample history lives in two
+// buffers used in turn: second file_no reads buffer (file_no + 1) % 2 (the
+// previous second's tail) and writes buffer file_no % 2.
+static void ipfb_device(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_no,
+                        float *data_buffer_vdif, uint8_t *d_vdif, const char *who)
 {
     ensure_twiddles_i();
     vm_device_state *d = vm_dev(vm);
     hipStream_t s = (hipStream_t)d->stream;
     int ns = vm->fine_sample_rate, P = g->ntaps, npol = (int)vm->obs_metadata->num_ant_pols;
-    if (vm->nfine_chan != KCH || !vm->d_e || ns <= P ||
+    if (vm->nfine_chan != KCH || !vm->d_e || ns <= P || P > MAXP ||
         (size_t)ns * KCH * npol * 2 * sizeof(float) > (size_t)g->out_size)
-        VM_FATAL("vmInvertPFBDevice: %d samples x %d pols of %d channels do not fit the arrays "
-                 "malloc_ipfb allocated", ns, npol, vm->nfine_chan);
-    size_t in_elems = (size_t)(ns + P) * KCH * npol;
-    size_t hist_elems = (size_t)P * KCH * npol;
-    size_t need = (in_elems + hist_elems) * sizeof(cf);
+        VM_FATAL("%s: %d samples x %d pols of %d channels do not fit the arrays malloc_ipfb "
+                 "allocated", who, ns, npol, vm->nfine_chan);
+    const size_t hist_elems = (size_t)P * KCH * npol;
+    const size_t need = 2 * hist_elems * sizeof(cf);
     if (d->ipfb_out_bytes < need) {
         if (d->d_ipfb_out) HIPCHECK(hipFree(d->d_ipfb_out));
         HIPCHECK(hipMalloc(&d->d_ipfb_out, need));
         HIPCHECK(hipMemsetAsync(d->d_ipfb_out, 0, need, s)); // same stream as its users
         d->ipfb_out_bytes = need;
     }
-    cf *din = (cf *)d->d_ipfb_out;
-    cf *hist = din + in_elems;
+    cf *hist = (cf *)d->d_ipfb_out;
+    const cf *hist_in = hist + (size_t)((file_no + 1) & 1) * hist_elems;
+    cf *hist_out = hist + (size_t)(file_no & 1) * hist_elems;
     vm_timer_start(d, VM_T_IPFB);
-    hipLaunchKernelGGL(k_ipfb_gather, dim3((unsigned)((in_elems + 255) / 256)), dim3(256), 0, s,
-                       (const cf *)vm->d_e, hist, ns, P, npol, file_no, din);
-    HIPCHECK(hipGetLastError());
-    vm_ipfb_device_launch(din, ns, P, npol, g->d_ft_real, g->d_out, s);
-    hipLaunchKernelGGL(k_ipfb_save_hist, dim3((unsigned)((hist_elems + 255) / 256)), dim3(256), 0,
-                       s, (const cf *)vm->d_e, ns, P, npol, hist);
+    if (npol < 1 || npol > 2) VM_FATAL("%s: %d polarisations", who, npol);
+    const int tj = IPFB_DEV_ROWS - (P - 1);
+    const dim3 grid((unsigned)((ns + tj - 1) / tj));
+    // the MWA synthesis filters have 12 taps: that count is compiled in
+    if (P == 12)
+        hipLaunchKernelGGL(k_ipfb_dev<12>, grid, dim3(256 * npol), 0, s, (const cf *)vm->d_e, ns,
+                           P, npol, hist_in, file_no == 0 ? 1 : 0, hist_out, g->d_ft_real,
+                           g->d_out, d_vdif);
+    else
+        hipLaunchKernelGGL(k_ipfb_dev<0>, grid, dim3(256 * npol), 0, s, (const cf *)vm->d_e, ns,
+                           P, npol, hist_in, file_no == 0 ? 1 : 0, hist_out, g->d_ft_real,
+                           g->d_out, d_vdif);
     HIPCHECK(hipGetLastError());
     vm_timer_stop(d, VM_T_IPFB);
     if (data_buffer_vdif)
         HIPCHECK(hipMemcpyAsync(data_buffer_vdif, g->d_out, g->out_size, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+}
+
+void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_no,
+                       float *data_buffer_vdif)
+{
+    ipfb_device(vm, g, file_no, data_buffer_vdif, NULL, "vmInvertPFBDevice");
+}
+
+void vmInvertPFBDeviceVDIF(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_no,
+                           uint8_t *d_vdif)
+{
+    if (!d_vdif) VM_FATAL("vmInvertPFBDeviceVDIF: no VDIF output buffer");
+    ipfb_device(vm, g, file_no, NULL, d_vdif, "vmInvertPFBDeviceVDIF");
 }
 
 void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n, gpuStream_t stream)

@@ -119,3 +119,48 @@ def test_vdif_write_second_file(pkg, orc, gpu_lib, tmp_path):
     np.testing.assert_array_equal(np.fromfile(tmp_path / "beam.vdif", np.uint8),
                                   np.concatenate(want))
     assert (tmp_path / "beam.hdr").stat().st_size == 4096
+
+
+def test_ipfb_device_vdif_fused(pkg, orc, gpu_lib):
+    """vmInvertPFBDeviceVDIF over three seconds in order: its floats are
+    vmInvertPFBDevice's (same kernel, the history carried across seconds in
+    two buffers used in turn) and within the tolerance of the direct-form
+    oracle on the ring of the seconds' e, and its bytes are the oracle's VDIF
+    requantisation of those floats, bit for bit."""
+    import ctypes as C
+    from vcsbeam_amd.apps import TiedArraySession
+    from vcsbeam_amd.workloads import Observation
+    lib = gpu_lib
+    obs = Observation(ntiles=32, blocks_per_second=2, nseconds=3)
+    az, za = np.array([0.9]), np.array([0.35])
+    ses = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, nstokes=1,
+                           ipfb=True, want_e=True)
+    ns = obs.nspectra
+    sf = obs.filter / obs.filter.max()
+    h = sf * (15.0 / 7.2)
+    ring = np.zeros((2 * ns, K, 2), complex)
+    try:
+        for t in range(3):
+            print(f"second {t}: PFB, beam, fused inverse PFB + VDIF", flush=True)
+            ses.read(t)
+            ses.pfb()
+            ses.geometry()
+            ses.beamform()
+            ses.invert(t)
+            n = ns * K * 2 * 2
+            got = np.empty(n, np.float32)
+            lib.vmMemcpyDtoH(got.ctypes.data, ses.gi.d_out, got.nbytes)
+            q = np.empty(n, np.uint8)
+            lib.vmMemcpyDtoH(q.ctypes.data, ses.d_vdif, q.nbytes)
+            np.testing.assert_array_equal(q, orc.vdif_requant(got))
+            lib.vmPullE(ses.vm)
+            v = ses.vm.contents
+            e = np.ctypeslib.as_array(C.cast(v.e, C.POINTER(C.c_double)), (1, ns, K, 2, 2))[0]
+            ring[(t % 2) * ns:(t % 2 + 1) * ns] = e[..., 0] + 1j * e[..., 1]
+            ref, mask = orc.ipfb(orc.ipfb_gather(ring, t, ns, len(h) // K), ns, h)
+            gz = got.reshape(ns * K, 2, 2)
+            err = orc.rel_err((gz[..., 0] + 1j * gz[..., 1])[mask], ref[mask])
+            print(f"   rel err vs direct form {err:.2e}", flush=True)
+            assert err <= TOL
+    finally:
+        ses.close()
