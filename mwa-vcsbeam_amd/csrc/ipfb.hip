@@ -242,10 +242,58 @@ __global__ __launch_bounds__(512) void k_ipfb_dev(const cf *__restrict__ e, int 
         }
     }
     __syncthreads();
-    // each 256-thread half: one polarisation's inverse DFTs and FIR (both
-    // halves pass the same barriers inside ipfb_rows)
     const int pol = t >> 8;
-    ipfb_rows<PC>(Z[pol], nrows, P, jmax, j0, pol, npol, hK, out, out8);
+    if (PC == 0 || npol != 2) {
+        // each 256-thread half: one polarisation's inverse DFTs and FIR (both
+        // halves pass the same barriers inside ipfb_rows)
+        ipfb_rows<PC>(Z[pol], nrows, P, jmax, j0, pol, npol, hK, out, out8);
+        return;
+    }
+    // each half: one polarisation's inverse DFTs (ipfb_idft ends on a barrier)
+    ipfb_idft(Z[pol], nrows);
+    // the FIR of both polarisations per thread: four groups of 128 lanes,
+    // each a contiguous run of PER outputs from a window of PER + P - 1 rows
+    // held in registers, so that each LDS row is read once per thread and
+    // each lane stores its n's two polarisations (16 B, 4 B VDIF) together
+    constexpr int PQ = PC ? PC : 1;
+    constexpr int TJC = IPFB_DEV_ROWS - (PQ - 1);
+    constexpr int PER = (TJC + 3) / 4;
+    constexpr int WL = PER + PQ - 1;
+    const int r = t & 127;
+    const int ja = (t >> 7) * PER;
+    float hr[PQ];
+#pragma unroll
+    for (int q = 0; q < PQ; q++) hr[q] = hK[q * KCH + r];
+    cf w0[WL], w1[WL];
+#pragma unroll
+    for (int i = 0; i < WL; i++) {
+        const int row = ja + i;
+        w0[i] = w1[i] = cf{0.f, 0.f};
+        if (row < nrows) {
+            w0[i] = Z[0][row * ROW + r];
+            w1[i] = Z[1][row * ROW + r];
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < PER; o++) {
+        const int jj = ja + o;
+        if (jj >= jmax) break;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int q = 0; q < PQ; q++) {
+            const cf z0 = w0[o + PQ - 1 - q], z1 = w1[o + PQ - 1 - q];
+            a.x = fmaf(hr[q], z0.x, a.x);
+            a.y = fmaf(hr[q], z0.y, a.y);
+            a.z = fmaf(hr[q], z1.x, a.z);
+            a.w = fmaf(hr[q], z1.y, a.w);
+        }
+        const size_t n = (size_t)(j0 + jj) * KCH + r;
+        *(float4 *)&out[n * 4] = a;
+        if (out8)
+            *(uint32_t *)&out8[n * 4] = (uint32_t)vdif_q1(a.x) | ((uint32_t)vdif_q1(a.y) << 8) |
+                                        ((uint32_t)vdif_q1(a.z) << 16) |
+                                        ((uint32_t)vdif_q1(a.w) << 24);
+    }
 }
 
 // VDIF requantisation: clamp to [-126, 127], round half to even, int8,
