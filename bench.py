@@ -55,33 +55,132 @@ def event_time(lib, stream, fn, reps):
     return ms
 
 
-def cpu_baseline(sess, obs, az, za, target_s=10.0):
-    """FP64 oracle beamformer on the first n samples of the same second."""
+def cpu_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def _sized(fn, n_probe, target_s, n_max, quantum=1):
+    """Run fn(n_probe), then fn(n) with n sized so the run takes about
+    target_s; returns (n, seconds, result of the last run)."""
+    t, _ = fn(n_probe)
+    n = int(max(n_probe, min(n_max, target_s / max(t / n_probe, 1e-9))))
+    n = max(quantum, n // quantum * quantum)
+    t, r = fn(n)
+    return n, t, r
+
+
+def cpu_baseline(sess, obs, az, za, budget_s=12.0):
+    """The reference arithmetic on the host cores, stage by stage, under the
+    reference's stopwatch names (src/metadata.c:127-139), each stage run on a
+    bounded sample of one coarse channel-second of the 128-tile workload and
+    its time scaled to the whole channel-second: the forward PFB (pfb-wola,
+    pfb-round, pfb-fft, pfb-pack), the FP64 geometric phases of every beam
+    (delay), the FP64 beamformer (calc), and for one beam the FP32
+    direct-form inverse PFB (ipfb) and its VDIF requantisation (vdif).  Run
+    with all OpenMP threads and again with one."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc  # the oracle, used here only as the CPU baseline
-    from vcsbeam_amd.workloads import unit_vectors
-    vfl = sess.voltages()  # [c][i][s]
+    from vcsbeam_amd.workloads import SEED_DATA, mwax_voltages, unit_vectors
+    K, P = 128, 12
+    nspc, nbeam = obs.nspectra, len(az)
+    vfl = sess.voltages()  # [c][i][s] complex64
     D = obs.calibration()
-    B = np.zeros((len(az), obs.ntiles, 2, 2), complex)
+    B = np.zeros((nbeam, obs.ntiles, 2, 2), complex)
     B[..., 0, 0] = B[..., 1, 1] = 1
     J = orc.calc_J(D, B)
     E, N, H, L = obs.positions()
-    phi = orc.calc_phi(E, N, H, L, unit_vectors(az, za), obs.freqs)
+    unit = unit_vectors(az, za)
     polQ, polP = obs.pol_idx()
     invw = 1.0 / (2 * int((~obs.flag_mask).sum()))
+    h16 = obs.filter.astype(np.int32)
+    out_idx = obs.out_idx()
+    # PFB input: a zero margin block and the first blocks of second 0
+    # (orc_wola reads (nspec + 500) K samples of the margin-first buffer)
+    nb = 3
+    htr = np.ascontiguousarray(np.concatenate(
+        [np.zeros((1, obs.ninput, 64000, 2), np.int8),
+         mwax_voltages(obs.ninput, nb - 1, first_block=0, seed=SEED_DATA + obs.seed_offset)]))
+    nspec_max = nb * 64000 // K - 500
+    rng = np.random.default_rng(7)
+    synth = np.ascontiguousarray(rng.standard_normal(K * P) / K)
+    table = orc.cpu_ipfb_table(synth)
+    vdif_in = (rng.standard_normal(nspc * K * 2 * 2) * 30).astype(np.float32)
+    phi_all = None
+    S = None
+    nthreads = int(orc.lib().orc_num_threads())
+    legs = {}
+    for threads in (nthreads, 1):
+        orc.set_threads(threads)
+        share = budget_s * (1.0 if threads == nthreads else 0.5)
+        stages, sample = {}, {}
 
-    def run(n):
-        v = np.ascontiguousarray(vfl[:, :, :n].transpose(2, 0, 1)).astype(np.complex128)
+        def pfb(n):
+            t0 = time.perf_counter()
+            _, st = orc.cpu_pfb(htr, obs.ninput, n, h16, out_idx)
+            return time.perf_counter() - t0, st
+        n, _, st = _sized(pfb, 2, 0.15 * share, nspec_max)
+        for k, v in st.items():
+            stages[k] = v * nspc / n
+        sample["pfb"] = f"{n} of {nspc} spectra x {obs.ninput} inputs"
+
+        def delay(nb_):
+            t0 = time.perf_counter()
+            ph = orc.cpu_delay(E, N, H, L, unit[:nb_], obs.freqs)
+            return time.perf_counter() - t0, ph
+        n, t, ph = _sized(delay, min(4, nbeam), 0.05 * share, nbeam)
+        stages["delay"] = t * nbeam / n
+        sample["delay"] = f"{n} of {nbeam} beams x {obs.ntiles} tiles x {K} channels"
+        if phi_all is None:
+            phi_all = ph if n == nbeam else orc.calc_phi(E, N, H, L, unit, obs.freqs)
+
+        def calc(ns):
+            v = np.ascontiguousarray(vfl[:, :, :ns].transpose(2, 0, 1)).astype(np.complex128)
+            t0 = time.perf_counter()
+            out, _ = orc.beamform(v, J, phi_all, polQ, polP, invw, nstokes=4)
+            return time.perf_counter() - t0, out
+        n, t, Sn = _sized(calc, 2, 0.6 * share, 400)
+        stages["calc"] = t * nspc / n
+        sample["calc"] = f"{nbeam} beams x {K} channels x {n} of {nspc} samples"
+        if threads == nthreads:
+            S, n_calc, t_calc = Sn, n, t
+
+        def ipfb(nrow):
+            nout = nrow * K
+            buf = (rng.standard_normal((nrow + 2 * P + 2, K, 2))
+                   + 1j * rng.standard_normal((nrow + 2 * P + 2, K, 2))).astype(np.complex64)
+            t0 = time.perf_counter()
+            out = orc.cpu_ipfb(buf, P * K, nout, table, P)
+            return time.perf_counter() - t0, out
+        n, t, _ = _sized(ipfb, 1, 0.15 * share, nspc)
+        stages["ipfb"] = t * nspc / n
+        sample["ipfb"] = f"{n * K} of {nspc * K} output samples x 2 pols, one beam"
         t0 = time.perf_counter()
-        S, _ = orc.beamform(v, J, phi, polQ, polP, invw, nstokes=4)
-        return time.perf_counter() - t0, S
-
-    t, _ = run(2)
-    n = int(max(2, min(400, round(target_s / max(t / 2, 1e-6)))))
-    t, S = run(n)
-    nbeam = len(az)
-    return {"seconds": t, "nsamp": n, "S": S, "rate": nbeam * 128 * n / t,
-            "cores": int(orc.lib().orc_num_threads())}
+        orc.cpu_vdif(vdif_in)
+        stages["vdif"] = time.perf_counter() - t0
+        sample["vdif"] = f"{vdif_in.size} values, one beam (whole channel-second)"
+        pfb_s = sum(stages[k] for k in ("pfb-wola", "pfb-round", "pfb-fft", "pfb-pack"))
+        legs[threads] = {
+            "threads": threads,
+            "seconds_per_channel_second": {k: round(v, 4) for k, v in stages.items()},
+            "beam_samples_per_s": round(nbeam * K * nspc / (stages["delay"] + stages["calc"]), 1),
+            "pfb_input_samples_per_s": round(obs.ninput * nspc * K / pfb_s, 1),
+            "end_to_end_s_per_channel_second_one_beam": round(
+                pfb_s + (stages["delay"] + stages["calc"]) / nbeam + stages["ipfb"]
+                + stages["vdif"], 3),
+            "sample": sample}
+    orc.set_threads(nthreads)
+    return {"rate": legs[nthreads]["beam_samples_per_s"], "cores": nthreads, "S": S,
+            "nsamp": n_calc, "seconds": t_calc, "legs": legs, "info": cpu_info()}
 
 
 def kernel_ms(lib, sess, name):
@@ -708,10 +807,21 @@ def main():
         n = cb["nsamp"]
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as orc
-        result["cpu_baseline"] = {"value": round(cb["rate"], 1), "unit": "beam-samples/s",
-                                  "cores": cb["cores"], "kind": "port",
-                                  "sample": f"{args.beams} beams x 128 ch x {n} samples of the same "
-                                            f"second, FP64 oracle beamformer ({cb['seconds']:.1f} s)"}
+        full, one = cb["legs"][cb["cores"]], cb["legs"][1]
+        result["cpu_baseline"] = {
+            "value": round(cb["rate"], 1), "unit": "beam-samples/s", "cores": cb["cores"],
+            "kind": "port",
+            "sample": f"{args.beams} beams x 128 ch x {n} samples of the same second (FP64 "
+                      f"delay + calc, the GPU step's work; {cb['seconds']:.1f} s), stages "
+                      f"scaled to the channel-second",
+            **cb["info"],
+            "single_thread_value": one["beam_samples_per_s"],
+            "stages_s_per_channel_second": full["seconds_per_channel_second"],
+            "stages_s_per_channel_second_1_thread": one["seconds_per_channel_second"],
+            "pfb_input_samples_per_s": full["pfb_input_samples_per_s"],
+            "end_to_end_s_per_channel_second_one_beam":
+                full["end_to_end_s_per_channel_second_one_beam"],
+            "samples": full["sample"], "samples_1_thread": one["sample"]}
         par = {}
         for name, S in (("mfma_fp16", S_mfma), ("fp32", S_fp32)):
             if S is None:

@@ -5,7 +5,9 @@ Produces, under ``mwa-vcsbeam_amd/_lib/``:
 * ``libvcsbeam_amd.so`` -- the C-ABI library (HIP kernels + host C module)
   declared in ``include/vcsbeam.h``;
 * ``libvmsynth.so`` -- the seeded synthetic-input generator;
-* ``tied_array_beam`` -- a C caller of the library (examples/tied_array_beam.c).
+* ``tied_array_beam``, ``incoh_beam``, ``fine_pfb_offline`` -- C callers of the
+  library (examples/*.c) following app/make_mwa_tied_array_beam.c,
+  app/make_mwa_incoh_beam.c and app/fine_pfb_offline.c.
 
 Run as ``python mwa-vcsbeam_amd/build.py`` or call :func:`build`.
 """
@@ -22,6 +24,7 @@ INC = os.path.join(ROOT, "include")
 HIP_SOURCES = ["vm_device.hip", "pfb.hip", "beamform.hip", "beamform_mfma.hip",
                "ipfb.hip", "geometry.hip", "beam_model.hip"]
 C_SOURCES = ["vm_host.c", "calibration.c", "astrometry.c", "obsfiles.c", "psrfits.c"]
+EXAMPLES = ["tied_array_beam", "incoh_beam", "fine_pfb_offline"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code (they have given
@@ -71,13 +74,14 @@ def build(verbose=False):
     lib = os.path.join(LIB, "libvcsbeam_amd.so")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", lib + ".tmp"] + objs)
     os.replace(lib + ".tmp", lib)
-    # a C caller of the library (examples/tied_array_beam.c), linked against
-    # the installed header and the .so next to it
-    example = os.path.join(LIB, "tied_array_beam")
-    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I", INC,
-          os.path.join(HERE, "examples", "tied_array_beam.c"), "-L", LIB, "-lvcsbeam_amd",
-          "-Wl,-rpath,$ORIGIN", "-lm", "-o", example + ".tmp"])
-    os.replace(example + ".tmp", example)
+    # C callers of the library (examples/*.c, one per reference application),
+    # linked against the installed header and the .so next to them
+    for name in EXAMPLES:
+        example = os.path.join(LIB, name)
+        _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I", INC,
+              os.path.join(HERE, "examples", name + ".c"), "-L", LIB, "-lvcsbeam_amd",
+              "-Wl,-rpath,$ORIGIN", "-lm", "-o", example + ".tmp"])
+        os.replace(example + ".tmp", example)
     synth = os.path.join(LIB, "libvmsynth.so")
     _run(["gcc", "-O3", "-fPIC", "-shared", "-fopenmp", "-o", synth + ".tmp",
           os.path.join(CSRC, "vmsynth.c"), "-lm"])

@@ -181,6 +181,18 @@ double orc_mlp_grad(const double *X, const double *T, int B, int nl, const int *
 /* Number of OpenMP threads the parallel loops use. */
 int orc_num_threads(void);
 
+/* cpu_pipeline.c: the CPU baseline, stage by stage (test infrastructure) */
+void orc_set_threads(int n);
+int orc_cpu_pfb(const int8_t *htr, int ninput, int nspectra, int K, int P, const int32_t *h,
+                const int32_t *out_idx, int fpga, int imag_first, float *vout, double *t);
+void orc_cpu_ipfb_table(const double *h, int K, int P, float *ft_re, float *ft_im);
+void orc_cpu_ipfb(const float *in, long n0, long nout, int K, int P, int npol,
+                  const float *ft_re, const float *ft_im, float *out);
+void orc_cpu_vdif(const float *f, size_t n, uint8_t *out);
+void orc_cpu_delay(const double *E, const double *N, const double *H, const double *L,
+                   int nant, const double *unit, int npointing, const double *freq_hz,
+                   int nchan, double Href, double *phi);
+
 /* FP64 norm-wise relative error ||x - ref|| / ||ref|| over n values. */
 double orc_rel_err(const double *x, const double *ref, size_t n);
 

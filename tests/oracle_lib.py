@@ -195,6 +195,64 @@ def vdif_frames(f, nframes, payload_bytes, hdr, frame_rate):
     return out, h
 
 
+# ------------------------------------------- CPU baseline (cpu_pipeline.c) --
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def cpu_pfb(htr, ninput, nspectra, h, out_idx, fpga=True, imag_first=False, K=128):
+    """Forward PFB stage by stage -> (complex64 [m][k'][vcs(i)], seconds per
+    stage {pfb-wola, pfb-round, pfb-fft, pfb-pack})."""
+    P = len(h) // K
+    h = np.ascontiguousarray(h, dtype=np.int32)
+    oi = np.ascontiguousarray(out_idx, dtype=np.int32)
+    v = np.empty((nspectra, K, ninput), dtype=np.complex64)
+    t = np.zeros(4)
+    rc = lib().orc_cpu_pfb(_p(htr), ninput, nspectra, K, P, _p(h), _p(oi), int(fpga),
+                           int(imag_first), _p(v), _p(t))
+    if rc != 0:
+        raise MemoryError("orc_cpu_pfb: out of memory")
+    return v, dict(zip(("pfb-wola", "pfb-round", "pfb-fft", "pfb-pack"), t.tolist()))
+
+
+def cpu_ipfb_table(h, K=128):
+    P = len(h) // K
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    re = np.empty(K * P * K, np.float32)
+    im = np.empty(K * P * K, np.float32)
+    lib().orc_cpu_ipfb_table(_p(h), K, P, _p(re), _p(im))
+    return re, im
+
+
+def cpu_ipfb(inbuf, n0, nout, table, P, K=128, npol=2):
+    """FP32 direct-form inverse PFB of outputs [n0, n0 + nout): inbuf complex64
+    [row][K][npol] (row = m + P) -> complex64 [nout][npol]."""
+    inbuf = np.ascontiguousarray(inbuf, dtype=np.complex64)
+    if n0 < P * K or (n0 + nout - 1) // K + P >= inbuf.shape[0]:
+        raise ValueError("cpu_ipfb: outputs outside the rows given")
+    out = np.empty((nout, npol), np.complex64)
+    lib().orc_cpu_ipfb(_p(inbuf), C.c_long(n0), C.c_long(nout), K, P, npol, _p(table[0]),
+                       _p(table[1]), _p(out))
+    return out
+
+
+def cpu_vdif(f):
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    out = np.empty(f.shape, np.uint8)
+    lib().orc_cpu_vdif(_p(f), C.c_size_t(f.size), _p(out))
+    return out
+
+
+def cpu_delay(E, N, H, L, unit, freqs, Href=377.827):
+    E, N, H, L = (np.ascontiguousarray(x, dtype=np.float64) for x in (E, N, H, L))
+    unit = np.ascontiguousarray(unit, dtype=np.float64).reshape(-1, 3)
+    freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+    phi = np.empty((unit.shape[0], len(E), len(freqs)), np.complex128)
+    lib().orc_cpu_delay(_p(E), _p(N), _p(H), _p(L), len(E), _p(unit), unit.shape[0],
+                        _p(freqs), len(freqs), C.c_double(Href), _p(phi))
+    return phi
+
+
 def rel_err(x, ref):
     x = np.asarray(x)
     ref = np.asarray(ref)

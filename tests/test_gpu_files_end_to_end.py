@@ -282,6 +282,61 @@ def test_c_caller_matches_python_driver(pkg, gpu_lib, tmp_path):
         assert tab_py == tab_c
 
 
+def test_c_callers_offline_pfb_and_incoh_match_python_drivers(pkg, gpu_lib, tmp_path):
+    """examples/fine_pfb_offline.c and examples/incoh_beam.c (include/vcsbeam.h
+    only, the reference applications' call sequences) against the Python file
+    drivers on the same two seconds: the offline PFB's legacy recombined
+    files are byte-equal (the C caller writes them into the data directory,
+    as vmWritePFBOutputToFile does), and the incoherent beam's PSRFITS from
+    those files has the same headers (the file's creation DATE aside) and the
+    same row bytes."""
+    import subprocess
+    from vcsbeam_amd.apps import fine_pfb_offline_files, make_incoh_beam_files
+    from vcsbeam_amd.workloads import mwax_voltages
+    rng = np.random.default_rng(15)
+    nsec = 2
+    geo = (rng.uniform(-700, 700, NT), rng.uniform(-700, 700, NT), np.full(NT, 377.0),
+           np.full(NT, 100.0))
+    _metafits(tmp_path / "mwax.metafits", *geo, flagged=-1)
+    _metafits(tmp_path / "legacy.metafits", *geo, flagged=2, mode="VOLTAGE_START")
+    data = mwax_voltages(2 * NT, 20 * nsec, first_block=0, seed=43)
+    hdr = f"HDR_SIZE 4096\nNINPUTS {2 * NT}\n".encode().ljust(4096, b"\0")
+    cdir = tmp_path / "c"
+    cdir.mkdir()
+    (cdir / f"{GPS}_{GPS}_109.sub").write_bytes(hdr + bytes(data[0].nbytes) + data.tobytes())
+    bindir = os.path.dirname(pkg.LIBPATH)
+    print("Python offline PFB driver, 2 s", flush=True)
+    paths = fine_pfb_offline_files(tmp_path / "mwax.metafits", cdir, tmp_path / "py",
+                                   nseconds=nsec)
+    print("C offline PFB caller, 2 s", flush=True)
+    r = subprocess.run([os.path.join(bindir, "fine_pfb_offline"), str(tmp_path / "mwax.metafits"),
+                        str(cdir), "109", "+0", str(nsec)],
+                       capture_output=True, text=True, timeout=120, env=dict(os.environ))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert len(paths) == nsec
+    for p in paths:
+        name = os.path.basename(p)
+        assert (cdir / name).read_bytes() == pathlib_of(p).read_bytes(), name
+    print("Python incoherent-beam driver on the C caller's files", flush=True)
+    py = make_incoh_beam_files(tmp_path / "legacy.metafits", cdir, tmp_path / "py_incoh",
+                               nseconds=nsec)
+    (tmp_path / "c_incoh").mkdir()
+    print("C incoherent-beam caller", flush=True)
+    r = subprocess.run([os.path.join(bindir, "incoh_beam"), str(tmp_path / "legacy.metafits"),
+                        str(cdir), str(tmp_path / "c_incoh"), "109", "+0", str(nsec)],
+                       capture_output=True, text=True, timeout=120, env=dict(os.environ))
+    assert r.returncode == 0, r.stderr[-2000:]
+    cbase = r.stdout.split()
+    assert len(cbase) == 1 and os.path.basename(cbase[0]) == os.path.basename(py["psrfits"])
+    (prim_py, _), (sub_py, tab_py) = read_fits(pathlib_of(py["psrfits"] + "_0001.fits"))
+    (prim_c, _), (sub_c, tab_c) = read_fits(pathlib_of(cbase[0] + "_0001.fits"))
+    prim_py.pop("DATE", None)
+    prim_c.pop("DATE", None)
+    assert prim_py == prim_c and sub_py == sub_c
+    assert sub_c["NAXIS2"] == nsec
+    assert tab_py == tab_c
+
+
 def test_tied_array_files_three_seconds_against_oracle(pkg, orc, gpu_lib, tmp_path):
     """Three seconds through make_tied_array_beam_files against the FP64
     oracle chain, each second's expected geometry computed here, not by the
