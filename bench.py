@@ -439,10 +439,10 @@ def pipeline_stage(pkg, lib, nchan=24, steps=2):
     flagged), fused FP32 forward PFB, FP64 device geometry, the plain FP32
     tied-array beam towards one pointing (za 20 deg), Stokes I (the PSRFITS
     product, 100 us samples), inverse PFB back to 1.28 MHz complex float32 with
-    its 8-bit VDIF requantisation in the same kernel, all device-resident; the
-    channels run in turn and share one forward-PFB output buffer owned by the
-    caller."""
-    from vcsbeam_amd.apps import TiedArraySession, device_second
+    its 8-bit VDIF requantisation in the same kernel (one launch over all
+    channels), all device-resident; the channels' PFB and beamformer run in
+    turn and share one forward-PFB output buffer owned by the caller."""
+    from vcsbeam_amd.apps import TiedArraySession, device_second, invert_batch
     from vcsbeam_amd.workloads import Observation
     obs = [Observation(ntiles=128, blocks_per_second=20, nflagged=4, rec_chan=109 + g,
                        seed_offset=1000 * (g + 1), geometry_seed_offset=0) for g in range(nchan)]
@@ -464,7 +464,7 @@ def pipeline_stage(pkg, lib, nchan=24, steps=2):
                 x.pfb()
                 x.geometry()
                 x.beamform()
-                x.invert(t)
+            invert_batch(sess, t)  # all channels' inverse PFB + VDIF in one launch
 
         second(0)
         for x in sess:

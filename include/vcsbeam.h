@@ -1067,6 +1067,16 @@ void vmInvertPFBDevice(vcsbeam_context *vm, struct gpu_ipfb_arrays *g,
  * (device, n x pol x 2 bytes). */
 void vmInvertPFBDeviceVDIF(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_no,
                            uint8_t *d_vdif);
+/* vmInvertPFBDeviceVDIF for the nctx contexts of several coarse channels of
+ * the same second (equal sample counts, taps and polarisations) in one
+ * launch, one grid row per context, on vm[0]'s stream after each context's
+ * own stream has drained.  Each context's g[k]->d_out, d_vdif[k] and history
+ * end as its own vmInvertPFBDeviceVDIF call would leave them; the kernel
+ * time is recorded on vm[0].  No counterpart in the reference, which
+ * inverts one coarse channel per process (src/pfb.cpp:940-1102): for
+ * callers that hold several channels on one device. */
+void vmInvertPFBDeviceVDIFBatch(vcsbeam_context **vm, struct gpu_ipfb_arrays **g, int nctx,
+                                int file_no, uint8_t **d_vdif);
 
 /* Device VDIF requantisation of n floats (float2int8_trunc + offset binary). */
 void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n,

@@ -352,6 +352,8 @@ _SIGS = {
                              C.POINTER(GpuIpfbArrays), C.c_void_p]),
     "vmInvertPFBDevice": (None, [_CTX, C.POINTER(GpuIpfbArrays), C.c_int, C.c_void_p]),
     "vmInvertPFBDeviceVDIF": (None, [_CTX, C.POINTER(GpuIpfbArrays), C.c_int, C.c_void_p]),
+    "vmInvertPFBDeviceVDIFBatch": (None, [C.POINTER(_CTX), C.POINTER(C.POINTER(GpuIpfbArrays)),
+                                          C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "vmRequantVDIFDevice": (None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "create_data_buffer_fine": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "prepare_data_buffer_fine": (None, [C.c_void_p, _CTX, C.c_size_t]),

@@ -651,6 +651,19 @@ class TiedArraySession:
         self.vm = None
 
 
+def invert_batch(sessions, t):
+    """Inverse PFB and 8-bit VDIF of second t for several sessions (coarse
+    channels of the same second) in one launch: vmInvertPFBDeviceVDIFBatch
+    leaves each session's outputs and history as its own invert(t) would."""
+    n = len(sessions)
+    lib = sessions[0].lib
+    ctx = type(sessions[0].vm)
+    vms = (ctx * n)(*[x.vm for x in sessions])
+    gis = (C.POINTER(GpuIpfbArrays) * n)(*[C.pointer(x.gi) for x in sessions])
+    outs = (C.c_void_p * n)(*[x.d_vdif for x in sessions])
+    lib.vmInvertPFBDeviceVDIFBatch(vms, gis, n, t, outs)
+
+
 def device_second(ses, t):
     """Generate second t of ses.obs in place in the session's forward-PFB
     input buffer (margin block + the second's blocks) with the device copy of

@@ -189,14 +189,31 @@ __global__ __launch_bounds__(256) void k_ipfb(const T *__restrict__ in, int ns, 
 // earlier ones its rows are exactly one 32-row pass of inverse DFTs.
 // out8: optional VDIF bytes.  grid ceil(ns / (33 - P)), block 256 x npol.
 #define IPFB_DEV_ROWS 32
-template <int PC>
-__global__ __launch_bounds__(512) void k_ipfb_dev(const cf *__restrict__ e, int ns, int P, int npol,
-                                                  const cf *__restrict__ hist_in, int first,
-                                                  cf *__restrict__ hist_out,
-                                                  const float *__restrict__ hK,
-                                                  float *__restrict__ out, uint8_t *__restrict__ out8)
+// One coarse channel's inverse PFB on the device: its beamformed voltages,
+// history in and out, synthesis filter and outputs (out8 may be NULL).
+struct ipfb_job {
+    const cf *e;
+    const cf *hist_in;
+    cf *hist_out;
+    const float *hK;
+    float *out;
+    uint8_t *out8;
+};
+
+// BATCH: the job is jobs[blockIdx.y] (one coarse channel per grid row),
+// otherwise job1.
+template <int PC, bool BATCH>
+__global__ __launch_bounds__(512) void k_ipfb_dev(ipfb_job job1, const ipfb_job *__restrict__ jobs,
+                                                  int ns, int P, int npol, int first)
 {
     __shared__ cf Z[2][IPFB_DEV_ROWS * ROW];
+    const ipfb_job job = BATCH ? jobs[blockIdx.y] : job1;
+    const cf *__restrict__ e = job.e;
+    const cf *__restrict__ hist_in = job.hist_in;
+    cf *__restrict__ hist_out = job.hist_out;
+    const float *__restrict__ hK = job.hK;
+    float *__restrict__ out = job.out;
+    uint8_t *__restrict__ out8 = job.out8;
     const int t = threadIdx.x;
     const int tj = IPFB_DEV_ROWS - (P - 1);
     const int j0 = blockIdx.x * tj;
@@ -517,10 +534,11 @@ void cu_invert_pfb(gpuDoubleComplex *data_buffer_fine, int file_no, int npointin
 // VDIF bytes too when d_vdif is not NULL.  The P-sample history lives in two
 // buffers used in turn: second file_no reads buffer (file_no + 1) % 2 (the
 // previous second's tail) and writes buffer file_no % 2.
-static void ipfb_device(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_no,
-                        float *data_buffer_vdif, uint8_t *d_vdif, const char *who)
+// Checks one context's arrays, makes its history buffers and fills its job
+// for second file_no.
+static void ipfb_job_for(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_no,
+                         uint8_t *d_vdif, const char *who, ipfb_job &job)
 {
-    ensure_twiddles_i();
     vm_device_state *d = vm_dev(vm);
     hipStream_t s = (hipStream_t)d->stream;
     int ns = vm->fine_sample_rate, P = g->ntaps, npol = (int)vm->obs_metadata->num_ant_pols;
@@ -528,6 +546,7 @@ static void ipfb_device(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file
         (size_t)ns * KCH * npol * 2 * sizeof(float) > (size_t)g->out_size)
         VM_FATAL("%s: %d samples x %d pols of %d channels do not fit the arrays malloc_ipfb "
                  "allocated", who, ns, npol, vm->nfine_chan);
+    if (npol < 1 || npol > 2) VM_FATAL("%s: %d polarisations", who, npol);
     const size_t hist_elems = (size_t)P * KCH * npol;
     const size_t need = 2 * hist_elems * sizeof(cf);
     if (d->ipfb_out_bytes < need) {
@@ -537,22 +556,48 @@ static void ipfb_device(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file
         d->ipfb_out_bytes = need;
     }
     cf *hist = (cf *)d->d_ipfb_out;
-    const cf *hist_in = hist + (size_t)((file_no + 1) & 1) * hist_elems;
-    cf *hist_out = hist + (size_t)(file_no & 1) * hist_elems;
-    vm_timer_start(d, VM_T_IPFB);
-    if (npol < 1 || npol > 2) VM_FATAL("%s: %d polarisations", who, npol);
+    job.e = (const cf *)vm->d_e;
+    job.hist_in = hist + (size_t)((file_no + 1) & 1) * hist_elems;
+    job.hist_out = hist + (size_t)(file_no & 1) * hist_elems;
+    job.hK = g->d_ft_real;
+    job.out = g->d_out;
+    job.out8 = d_vdif;
+}
+
+// Launch over njob jobs (njob 1: the job by value; more: jobs_dev[y]).
+static void ipfb_launch(const ipfb_job &job1, const ipfb_job *jobs_dev, int njob, int ns, int P,
+                        int npol, int file_no, hipStream_t s)
+{
     const int tj = IPFB_DEV_ROWS - (P - 1);
-    const dim3 grid((unsigned)((ns + tj - 1) / tj));
+    const dim3 grid((unsigned)((ns + tj - 1) / tj), (unsigned)njob);
+    const int first = file_no == 0 ? 1 : 0;
     // the MWA synthesis filters have 12 taps: that count is compiled in
-    if (P == 12)
-        hipLaunchKernelGGL(k_ipfb_dev<12>, grid, dim3(256 * npol), 0, s, (const cf *)vm->d_e, ns,
-                           P, npol, hist_in, file_no == 0 ? 1 : 0, hist_out, g->d_ft_real,
-                           g->d_out, d_vdif);
+    if (njob == 1 && P == 12)
+        hipLaunchKernelGGL((k_ipfb_dev<12, false>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
+                           ns, P, npol, first);
+    else if (njob == 1)
+        hipLaunchKernelGGL((k_ipfb_dev<0, false>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
+                           ns, P, npol, first);
+    else if (P == 12)
+        hipLaunchKernelGGL((k_ipfb_dev<12, true>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
+                           ns, P, npol, first);
     else
-        hipLaunchKernelGGL(k_ipfb_dev<0>, grid, dim3(256 * npol), 0, s, (const cf *)vm->d_e, ns,
-                           P, npol, hist_in, file_no == 0 ? 1 : 0, hist_out, g->d_ft_real,
-                           g->d_out, d_vdif);
+        hipLaunchKernelGGL((k_ipfb_dev<0, true>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
+                           ns, P, npol, first);
     HIPCHECK(hipGetLastError());
+}
+
+static void ipfb_device(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int file_no,
+                        float *data_buffer_vdif, uint8_t *d_vdif, const char *who)
+{
+    ensure_twiddles_i();
+    vm_device_state *d = vm_dev(vm);
+    hipStream_t s = (hipStream_t)d->stream;
+    ipfb_job job;
+    ipfb_job_for(vm, g, file_no, d_vdif, who, job);
+    vm_timer_start(d, VM_T_IPFB);
+    ipfb_launch(job, nullptr, 1, vm->fine_sample_rate, g->ntaps,
+                (int)vm->obs_metadata->num_ant_pols, file_no, s);
     vm_timer_stop(d, VM_T_IPFB);
     if (data_buffer_vdif)
         HIPCHECK(hipMemcpyAsync(data_buffer_vdif, g->d_out, g->out_size, hipMemcpyDeviceToHost, s));
@@ -570,6 +615,43 @@ void vmInvertPFBDeviceVDIF(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int f
 {
     if (!d_vdif) VM_FATAL("vmInvertPFBDeviceVDIF: no VDIF output buffer");
     ipfb_device(vm, g, file_no, NULL, d_vdif, "vmInvertPFBDeviceVDIF");
+}
+
+void vmInvertPFBDeviceVDIFBatch(vcsbeam_context **vm, struct gpu_ipfb_arrays **g, int nctx,
+                                int file_no, uint8_t **d_vdif)
+{
+    const char *who = "vmInvertPFBDeviceVDIFBatch";
+    if (nctx < 1 || !vm || !g || !d_vdif) VM_FATAL("%s: %d contexts", who, nctx);
+    ensure_twiddles_i();
+    const int ns = vm[0]->fine_sample_rate, P = g[0]->ntaps;
+    const int npol = (int)vm[0]->obs_metadata->num_ant_pols;
+    ipfb_job *jobs = (ipfb_job *)malloc(sizeof(ipfb_job) * nctx);
+    if (!jobs) VM_FATAL("%s: out of host memory", who);
+    for (int k = 0; k < nctx; k++) {
+        if (!d_vdif[k]) VM_FATAL("%s: no VDIF output buffer for context %d", who, k);
+        if (vm[k]->fine_sample_rate != ns || g[k]->ntaps != P ||
+            (int)vm[k]->obs_metadata->num_ant_pols != npol)
+            VM_FATAL("%s: context %d has %d samples, %d taps, %d pols; context 0 %d, %d, %d", who,
+                     k, vm[k]->fine_sample_rate, g[k]->ntaps,
+                     (int)vm[k]->obs_metadata->num_ant_pols, ns, P, npol);
+        ipfb_job_for(vm[k], g[k], file_no, d_vdif[k], who, jobs[k]);
+        // each context's e and history were written on its own stream
+        HIPCHECK(hipStreamSynchronize((hipStream_t)vm_dev(vm[k])->stream));
+    }
+    vm_device_state *d = vm_dev(vm[0]);
+    hipStream_t s = (hipStream_t)d->stream;
+    const size_t jbytes = sizeof(ipfb_job) * (size_t)nctx;
+    if (d->ipfb_jobs_bytes < jbytes) {
+        if (d->d_ipfb_jobs) HIPCHECK(hipFree(d->d_ipfb_jobs));
+        HIPCHECK(hipMalloc(&d->d_ipfb_jobs, jbytes));
+        d->ipfb_jobs_bytes = jbytes;
+    }
+    HIPCHECK(hipMemcpyAsync(d->d_ipfb_jobs, jobs, jbytes, hipMemcpyHostToDevice, s));
+    vm_timer_start(d, VM_T_IPFB);
+    ipfb_launch(jobs[0], (const ipfb_job *)d->d_ipfb_jobs, nctx, ns, P, npol, file_no, s);
+    vm_timer_stop(d, VM_T_IPFB);
+    HIPCHECK(hipStreamSynchronize(s));
+    free(jobs);
 }
 
 void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n, gpuStream_t stream)

@@ -75,6 +75,7 @@ void vm_dev_free(vcsbeam_context *vm)
     dfree(d->d_gx);
     dfree(d->d_unit);
     dfree(d->d_ipfb_out);
+    dfree(d->d_ipfb_jobs);
     dfree(d->d_nterm);
     dfree(d->d_flag);
     HIPCHECK(hipStreamDestroy((hipStream_t)d->stream));

@@ -99,6 +99,8 @@ struct vm_device_state {
     /* inverse-PFB device buffers for the device-resident path */
     void *d_ipfb_out;
     size_t ipfb_out_bytes;
+    void *d_ipfb_jobs; /* job descriptors of vmInvertPFBDeviceVDIFBatch */
+    size_t ipfb_jobs_bytes;
     /* per-channel N-term scratch for the MFMA form (floats) */
     void *d_nterm;
     size_t nterm_bytes;

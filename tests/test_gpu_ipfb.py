@@ -164,3 +164,53 @@ def test_ipfb_device_vdif_fused(pkg, orc, gpu_lib):
             assert err <= TOL
     finally:
         ses.close()
+
+
+def test_ipfb_device_vdif_batch_equals_per_channel(pkg, gpu_lib):
+    """vmInvertPFBDeviceVDIFBatch over three coarse channels of two seconds in
+    order: every channel's floats, VDIF bytes and carried history equal, bit
+    for bit, those of the same channel's own vmInvertPFBDeviceVDIF calls in
+    a separate session."""
+    from vcsbeam_amd.apps import TiedArraySession, invert_batch
+    from vcsbeam_amd.workloads import Observation
+    lib = gpu_lib
+    az, za = np.array([0.4]), np.array([0.3])
+    obs = [Observation(ntiles=16, blocks_per_second=2, nseconds=2, rec_chan=120 + g,
+                       seed_offset=100 * g) for g in range(3)]
+    one = [TiedArraySession(o, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, nstokes=1, ipfb=True)
+           for o in obs]
+    many = [TiedArraySession(o, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, nstokes=1, ipfb=True)
+            for o in obs]
+    ns = obs[0].nspectra
+    n = ns * K * 2 * 2
+    try:
+        for t in range(2):
+            print(f"second {t}: three channels, per-channel and batched inverse PFB", flush=True)
+            for x in one + many:
+                x.read(t)
+                x.pfb()
+                x.geometry()
+                x.beamform()
+            for x in one:
+                x.invert(t)
+            invert_batch(many, t)
+            for g in range(3):
+                outs = []
+                for x in (one[g], many[g]):
+                    f = np.empty(n, np.float32)
+                    lib.vmMemcpyDtoH(f.ctypes.data, x.gi.d_out, f.nbytes)
+                    q = np.empty(n, np.uint8)
+                    lib.vmMemcpyDtoH(q.ctypes.data, x.d_vdif, q.nbytes)
+                    outs.append((f, q))
+                np.testing.assert_array_equal(outs[0][0], outs[1][0])
+                np.testing.assert_array_equal(outs[0][1], outs[1][1])
+                assert np.abs(outs[0][0]).max() > 0
+            # the channels differ, so a batch row reading another's job would show
+            f0 = np.empty(n, np.float32)
+            f1 = np.empty(n, np.float32)
+            lib.vmMemcpyDtoH(f0.ctypes.data, many[0].gi.d_out, f0.nbytes)
+            lib.vmMemcpyDtoH(f1.ctypes.data, many[1].gi.d_out, f1.nbytes)
+            assert not np.array_equal(f0, f1)
+    finally:
+        for x in one + many:
+            x.close()
