@@ -635,6 +635,13 @@ def surrogate_feed(lib, s, pkg, sess, obs, az, za, steps=5, nbeam_check=16, nsam
             "check": f"{pk} beams x {n} samples x 128 ch"}
 
 
+def calibration_copy(pkg, lib):
+    """One plain copy of COPY_BYTES: the memory counters' calibration."""
+    a, b = pkg.DeviceArray(COPY_BYTES), pkg.DeviceArray(COPY_BYTES)
+    lib.vmDeviceCopy(b.ptr, a.ptr, COPY_BYTES)
+    del a, b
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -684,7 +691,10 @@ def main():
         print(json.dumps({"batch": batch_stage(pkg, lib)}))
         return
     if args.pipeline_only:
-        print(json.dumps({"pipeline": pipeline_stage(pkg, lib), "aux": aux_stage(pkg, lib)}))
+        out = {"pipeline": pipeline_stage(pkg, lib), "aux": aux_stage(pkg, lib)}
+        if args.calibrate_copy:
+            calibration_copy(pkg, lib)
+        print(json.dumps(out))
         return
     from vcsbeam_amd.apps import TiedArraySession
     from vcsbeam_amd.workloads import Observation, directions_in_cap
@@ -843,9 +853,7 @@ def main():
         print("[bench] incoherent beam and renormalise", file=sys.stderr, flush=True)
         result["aux"] = aux_stage(pkg, lib)
     if args.calibrate_copy:
-        a, b = pkg.DeviceArray(COPY_BYTES), pkg.DeviceArray(COPY_BYTES)
-        lib.vmDeviceCopy(b.ptr, a.ptr, COPY_BYTES)
-        del a, b
+        calibration_copy(pkg, lib)
     print(json.dumps(result))
 
 

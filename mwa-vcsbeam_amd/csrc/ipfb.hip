@@ -200,14 +200,28 @@ struct ipfb_job {
     uint8_t *out8;
 };
 
-// BATCH: the job is jobs[blockIdx.y] (one coarse channel per grid row),
-// otherwise job1.
+// Workgroups are dispatched to the 8 XCDs in turn (linear id L to XCD L % 8),
+// each XCD with its own L2.  Neighbouring tiles share P - 1 input rows, so
+// the tiles are renumbered to give each XCD one contiguous run of them: the
+// shared rows are then fetched from HBM once, into that XCD's L2.  A
+// bijection of [0, n): XCD x (of q + 1 tiles for x < n % 8, else q) takes
+// tiles [x q + min(x, n % 8), ...).
+__device__ __forceinline__ int xcd_contiguous(int L, int n)
+{
+    const int q = n >> 3, r = n & 7, x = L & 7, k = L >> 3;
+    return x * q + min(x, r) + k;
+}
+
+// BATCH: the job is jobs[y] (one coarse channel per grid row), otherwise
+// job1.  (x, y): the tile after the XCD renumbering.
 template <int PC, bool BATCH>
 __global__ __launch_bounds__(512) void k_ipfb_dev(ipfb_job job1, const ipfb_job *__restrict__ jobs,
                                                   int ns, int P, int npol, int first)
 {
     __shared__ cf Z[2][IPFB_DEV_ROWS * ROW];
-    const ipfb_job job = BATCH ? jobs[blockIdx.y] : job1;
+    const int tile = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int bx = tile % gridDim.x, by = tile / gridDim.x;
+    const ipfb_job job = BATCH ? jobs[by] : job1;
     const cf *__restrict__ e = job.e;
     const cf *__restrict__ hist_in = job.hist_in;
     cf *__restrict__ hist_out = job.hist_out;
@@ -216,7 +230,7 @@ __global__ __launch_bounds__(512) void k_ipfb_dev(ipfb_job job1, const ipfb_job 
     uint8_t *__restrict__ out8 = job.out8;
     const int t = threadIdx.x;
     const int tj = IPFB_DEV_ROWS - (P - 1);
-    const int j0 = blockIdx.x * tj;
+    const int j0 = bx * tj;
     const int nrows = IPFB_DEV_ROWS; // s_in = j0 + 1 + row
     const int jmax = min(tj, ns - j0);
     const int nel = nrows * KCH;      // (row, k) pairs, each with npol values
