@@ -730,6 +730,9 @@ def main():
         lib.vmDeviceCopy(cb.ptr, ca.ptr, COPY_BYTES)
     copy_GBps = 5 * 2 * COPY_BYTES / (time.perf_counter() - t0) / 1e9
     del ca, cb
+    # measured dense FP16 MFMA rate (register-only matrix instruction chains);
+    # the better of two calls, the first of which may still see the clock rise
+    mfma_peak_tf = max(lib.vmMFMAPeakTFLOPS(), lib.vmMFMAPeakTFLOPS())
     nsamp_in = obs.ninput * obs.nspectra * 128
     pfb_bytes = nsamp_in * 10  # 2 B in + 8 B out per input complex sample
     beam_samples = args.beams * 128 * obs.nspectra
@@ -773,6 +776,8 @@ def main():
                 "algorithmic_bytes": int(gemm_bytes), "algorithmic_flops": gemm_flops,
                 "mfma_tflops": round(mfma_tf, 1),
                 "mfma_frac_issued": round(2 * mfma_tf / FP16_PEAK_TFLOPS, 4),
+                "mfma_measured_peak_tflops": round(mfma_peak_tf, 1),
+                "mfma_frac_of_measured_issued": round(2 * mfma_tf / mfma_peak_tf, 4),
                 "kernel": "k_beamform_mfma<fp16>", "kernel_ms": round(gemm_ms, 4),
                 "launches": gemm_n, "autotuned_variant": variant, "tile_order": sel_o.value,
                 "workgroup_tile": f"{vr.value} rows x {vs.value} samples, {vw.value} waves",
@@ -793,6 +798,7 @@ def main():
                    "pfb_hbm_frac": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                    "measured_write_GBps": round(write_GBps, 1),
                    "measured_copy_GBps": round(copy_GBps, 1),
+                   "measured_mfma_fp16_tflops": round(mfma_peak_tf, 1),
                    "pfb_frac_of_measured_write": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9
                                                        / write_GBps, 4),
                    "beamform_prep_ms": round(prep_ms, 4),

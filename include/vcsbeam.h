@@ -1134,6 +1134,10 @@ void vmMemcpy2DDtoH(void *dst, size_t dpitch, const void *src, size_t spitch, si
 void vmMemsetDevice(void *dst, int value, size_t bytes);
 void vmDeviceSynchronize(void);
 void vmDeviceCopy(void *dst, const void *src, size_t bytes); /* plain copy kernel */
+/* The device's dense FP16 MFMA rate in TFLOP/s, measured with chains of
+ * 32x32x16 FP16 matrix instructions on operands held in registers (no memory
+ * traffic): a measured peak beside the published one.  Synchronous. */
+double vmMFMAPeakTFLOPS(void);
 int vmDeviceCount(void);
 void *vmEventCreate(void);
 void vmEventRecord(void *ev, gpuStream_t stream);

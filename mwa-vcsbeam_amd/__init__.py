@@ -432,6 +432,7 @@ _SIGS = {
     "vmMemsetDevice": (None, [C.c_void_p, C.c_int, C.c_size_t]),
     "vmDeviceSynchronize": (None, []),
     "vmDeviceCopy": (None, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "vmMFMAPeakTFLOPS": (C.c_double, []),
     "vmMemcpy2DDtoH": (None, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
                               C.c_size_t]),
     "vmSynthMWAXDevice": (None, [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double,

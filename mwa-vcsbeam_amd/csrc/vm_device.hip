@@ -620,6 +620,72 @@ extern "C" void vmDeviceCopy(void *dst, const void *src, size_t bytes)
     HIPCHECK(hipDeviceSynchronize());
 }
 
+/* ------------------------------------------------ dense MFMA issue rate ---- */
+typedef _Float16 mp_h8 __attribute__((ext_vector_type(8)));
+typedef float mp_f16 __attribute__((ext_vector_type(16)));
+#define MP_CHAINS 4 /* independent accumulations per wavefront */
+#define MP_ITERS 4096
+
+/* MP_CHAINS chains of v_mfma_f32_32x32x16_f16 per wavefront on operands held
+ * in registers: the dense FP16 MFMA rate with no memory traffic.  The sums
+ * are stored only where they equal a value they cannot reach (operands of
+ * at most 0.008, so |sum| < 1), which keeps the work from being removed. */
+__global__ __launch_bounds__(256) void k_mfma_peak(float *out)
+{
+    mp_h8 a, b;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        a[i] = (_Float16)(0.001f * (float)((threadIdx.x + i) & 7));
+        b[i] = (_Float16)(0.001f * (float)((threadIdx.x * 3 + i) & 7));
+    }
+    mp_f16 acc[MP_CHAINS];
+#pragma unroll
+    for (int c = 0; c < MP_CHAINS; c++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[c][e] = 0.f;
+    for (int it = 0; it < MP_ITERS; it++) {
+#pragma unroll
+        for (int c = 0; c < MP_CHAINS; c++)
+            acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[c], 0, 0, 0);
+    }
+    float sum = 0.f;
+#pragma unroll
+    for (int c = 0; c < MP_CHAINS; c++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) sum += acc[c][e];
+    if (sum == 1.0e6f) out[threadIdx.x] = sum;
+}
+
+extern "C" double vmMFMAPeakTFLOPS(void)
+{
+    /* dense FP16 MFMA rate of this device, measured: 8 workgroups of 4
+     * wavefronts per CU, ten timed launches after five warm-up launches (the
+     * clock rises under a sustained matrix load) */
+    int dev = 0, ncu = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    float *out = nullptr;
+    HIPCHECK(hipMalloc(&out, 256 * sizeof(float)));
+    const int nwg = 8 * ncu, reps = 10;
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    for (int r = 0; r < 5; r++) hipLaunchKernelGGL(k_mfma_peak, dim3(nwg), dim3(256), 0, 0, out);
+    HIPCHECK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_mfma_peak, dim3(nwg), dim3(256), 0, 0, out);
+    HIPCHECK(hipEventRecord(e1, 0));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    HIPCHECK(hipEventDestroy(e0));
+    HIPCHECK(hipEventDestroy(e1));
+    HIPCHECK(hipFree(out));
+    const double flops = (double)reps * nwg * 4 /* wavefronts */ * MP_ITERS * MP_CHAINS *
+                         (32.0 * 32.0 * 16.0 * 2.0);
+    return flops / (ms * 1e-3) / 1e12;
+}
+
 /* ---------------------------------------------- seeded synthetic voltages ---- */
 /* Counter-based generator: element idx of stream `seed` is the SplitMix64
  * finaliser of seed * 0xD1B54A32D192ED03 + (idx + 1) * 0x9E3779B97F4A7C15;
