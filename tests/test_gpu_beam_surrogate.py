@@ -182,3 +182,63 @@ def test_surrogate_jones_feed_the_beamformer(pkg, gpu_lib, orc):
     print(f"Stokes I per-beam error: surrogate {err[3]}, nominal beam {err[1]}")
     assert err[3].max() <= 1e-3
     assert np.median(err[1]) >= 10 * np.median(err[3])
+
+
+def test_surrogate_jones_feed_the_mfma_beamformer_512_beams(pkg, gpu_lib, orc):
+    """The surrogate-fed precision form as the benchmark runs it: Jones for
+    512 beams within 15 deg x 128 fine channels from the trained network with
+    fallback (mode 3) drive the FP16 MFMA per-pointing-beam form through
+    vmCalcJDeviceChanB.  For every beam, its Stokes I over the whole 0.1 s is
+    within 1e-3 of the FP32 form fed the same Jones, and 4 beams are within
+    1e-3 of the FP64 chain with the analytic beam.  Workload: 32 tiles, 0.1 s,
+    coarse channel 150 (191.36-192.64 MHz)."""
+    from vcsbeam_amd.apps import TiedArraySession
+    from vcsbeam_amd.workloads import Observation, unit_vectors
+    obs = Observation(ntiles=32, blocks_per_second=2, rec_chan=150)
+    az, za = _cap(512, 15.0, 43)
+    s = _new(gpu_lib, seed=7)
+    try:
+        print("training the surrogate (3000 steps)", flush=True)
+        gpu_lib.vmSurrogateTrain(s, 1 << 20, 3000, 2048, 2e-3, 20.0, 11, None)
+        Bs, _, nfb = _jones(gpu_lib, s, 3, az, za, obs.freqs)
+        Bt = _jones(gpu_lib, s, 0, az[:4], za[:4], obs.freqs)[0]
+    finally:
+        gpu_lib.vmSurrogateDestroy(s)
+    print(f"surrogate Jones: {nfb} analytic fallbacks of {Bs.shape[0] * Bs.shape[1]}", flush=True)
+    Bs = np.ascontiguousarray(Bs)
+    ses = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
+                           form=pkg.VM_BF_FP32)
+    S = {}
+    try:
+        ses.read(0)
+        ses.pfb()
+        for form in (pkg.VM_BF_FP32, pkg.VM_BF_MFMA_FP16):
+            print(f"512 surrogate-fed beams, form {form}", flush=True)
+            ses.v.beamform_form = form
+            gpu_lib.vmCalcPhiDevice(ses.vm, ses.bg)
+            gpu_lib.vmCalcJDeviceChanB(ses.vm, Bs.ctypes.data)
+            ses.beamform()
+            S[form] = ses.stokes()
+    finally:
+        ses.close()
+    m, f = S[pkg.VM_BF_MFMA_FP16], S[pkg.VM_BF_FP32]
+    e = np.array([orc.rel_err(m[p, :, 0, :], f[p, :, 0, :]) for p in range(len(az))])
+    print(f"MFMA vs FP32, Stokes I per beam: max {e.max():.2e} median {np.median(e):.2e}")
+    assert e.max() <= 1e-3
+    # FP64 chain with the analytic beam, 4 beams
+    D = obs.calibration()
+    J = np.empty((4, obs.ntiles, 128, 2, 2), complex)
+    for c in range(128):
+        Bc = np.repeat(Bt[:, c][:, None], obs.ntiles, axis=1)
+        J[:, :, c] = orc.calc_J(D[:, c:c + 1], Bc)[:, :, 0]
+    cur = obs.second(0)
+    htr = np.ascontiguousarray(np.concatenate([np.zeros_like(cur[:1]), cur], axis=0))
+    v = orc.pfb_forward(htr, obs.ninput, obs.nspectra, obs.filter.astype(np.int32), obs.out_idx(),
+                        fpga=True)
+    E, N, H, L = obs.positions()
+    phi = orc.calc_phi(E, N, H, L, unit_vectors(az[:4], za[:4]), obs.freqs)
+    polQ, polP = obs.pol_idx()
+    Sref, _ = orc.beamform(v, J, phi, polQ, polP, 1.0 / (2 * int((~obs.flag_mask).sum())), 4)
+    e4 = np.array([orc.rel_err(m[p, :, 0, :], Sref[p, :, 0, :]) for p in range(4)])
+    print(f"MFMA (surrogate Jones) vs FP64 (analytic beam), Stokes I: {e4}")
+    assert e4.max() <= 1e-3

@@ -141,3 +141,87 @@ def test_full_batched_workload(pkg, orc, gpu_lib):
                 assert err[0] <= 1e-3 and max(err[1:]) <= 3e-3
     finally:
         bs.close()
+
+
+def test_full_pipeline_two_channels_stokes_ipfb_vdif(pkg, orc, gpu_lib):
+    """Workload 2's device sequence at its full per-channel size, as the
+    benchmark runs it: two coarse channels of 128 tiles x 1 s (4 flagged),
+    seconds generated on the device, one shared forward-PFB output buffer,
+    the FP32 form towards one beam at za 20 deg (Stokes I), then the inverse
+    PFB and 8-bit VDIF of both channels in one launch.  Per channel:
+    - Stokes I on 64 samples x 128 fine channels within 1e-5 of the FP64
+      oracle fed that channel's own PFB output;
+    - the inverse PFB on 40 fine samples from mid-second (5120 outputs x 2
+      pols) within 1e-5 of the direct-form oracle on those rows of e;
+    - the whole second's VDIF bytes equal the oracle's requantisation of the
+      kernel's floats."""
+    import ctypes as C
+    from vcsbeam_amd.apps import TiedArraySession, device_second, invert_batch
+    from vcsbeam_amd.workloads import Observation, unit_vectors
+    K, P, n_s = 128, 12, 64
+    obs = [Observation(ntiles=128, blocks_per_second=20, nflagged=4, rec_chan=109 + g,
+                       seed_offset=1000 * (g + 1), geometry_seed_offset=0) for g in range(2)]
+    ns, ni = obs[0].nspectra, obs[0].ninput
+    az, za = np.array([0.7]), np.radians([20.0])
+    dv = pkg.DeviceArray(K * ni * ns * 8)
+    flags = (pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA) & ~pkg.PFB_MALLOC_DEVICE_OUTPUT
+    sess = []
+    try:
+        for o in obs:
+            sess.append(TiedArraySession(o, az, za, pfb_flags=flags, nstokes=1,
+                                         form=pkg.VM_BF_FP32, host_buffers=False, ipfb=True,
+                                         want_e=True, d_v=dv.ptr))
+        v_head, S_head = [], []
+        for g, x in enumerate(sess):
+            print(f"channel {g}: device second, PFB, geometry, FP32 beam", flush=True)
+            device_second(x, 0)
+            x.pfb()
+            x.geometry()
+            x.beamform()
+            # the first n_s samples of every [c][i] row of the shared PFB output
+            h = np.empty((K * ni, n_s), np.complex64)
+            gpu_lib.vmMemcpy2DDtoH(h.ctypes.data, n_s * 8, dv.ptr, ns * 8, n_s * 8, K * ni)
+            v_head.append(h.reshape(K, ni, n_s))
+            S_head.append(x.stokes_slice(0, 1, 0, n_s))
+        print("inverse PFB + VDIF of both channels in one launch", flush=True)
+        invert_batch(sess, 0)
+        sf = obs[0].filter / obs[0].filter.max()
+        hs = sf * (15.0 / 7.2)
+        for g, (x, o) in enumerate(zip(sess, obs)):
+            D = o.calibration()
+            B = np.zeros((1, o.ntiles, 2, 2), complex)
+            B[..., 0, 0] = B[..., 1, 1] = 1
+            E, N, H, L = o.positions()
+            phi = orc.calc_phi(E, N, H, L, unit_vectors(az, za), o.freqs)
+            polQ, polP = o.pol_idx()
+            v = np.ascontiguousarray(v_head[g].transpose(2, 0, 1)).astype(np.complex128)
+            ref, _ = orc.beamform(v, orc.calc_J(D, B), phi, polQ, polP,
+                                  1.0 / (2 * int((~o.flag_mask).sum())), nstokes=1)
+            e = orc.rel_err(S_head[g][0, :, 0, :], ref[0, :, 0, :])
+            print(f"channel {g}: Stokes I vs FP64 {e:.2e}", flush=True)
+            assert e <= 1e-5
+            n = ns * K * 2 * 2
+            got = np.empty(n, np.float32)
+            gpu_lib.vmMemcpyDtoH(got.ctypes.data, x.gi.d_out, got.nbytes)
+            q = np.empty(n, np.uint8)
+            gpu_lib.vmMemcpyDtoH(q.ctypes.data, x.d_vdif, q.nbytes)
+            np.testing.assert_array_equal(q, orc.vdif_requant(got))
+            gpu_lib.vmPullE(x.vm)
+            ev = np.ctypeslib.as_array(C.cast(x.v.e, C.POINTER(C.c_double)), (ns, K, 2, 2))
+            e_c = ev[..., 0] + 1j * ev[..., 1]
+            # rows r0 .. r0 + W + P of [P zero history rows | e] give outputs
+            # n = r0 K + n' for n' in [P K, W K) exactly as the whole buffer does
+            r0, W = 5000, 40
+            sub = e_c[r0 - P:r0 + W]
+            ref_i, mask = orc.ipfb(sub, W, hs)
+            gz = got.reshape(ns * K, 2, 2)
+            gz = (gz[..., 0] + 1j * gz[..., 1])[r0 * K:(r0 + W) * K]
+            ei = orc.rel_err(gz[mask], ref_i[mask])
+            print(f"channel {g}: inverse PFB vs direct form {ei:.2e} on {int(mask.sum())} "
+                  f"outputs x 2 pols", flush=True)
+            # valid: n' >= P K, and the P multiples of K below it
+            assert mask.sum() == (W - P) * K + P and ei <= 1e-5
+    finally:
+        for x in sess:
+            x.close()
+        del dv
