@@ -346,18 +346,19 @@ def surrogate_stage(lib, nbeams=4096, nchan=128, steps=4000, batch=2048, nsample
 def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_check=4):
     """Batched uneven workload: nchan coarse channels x 1 s of ntiles tiles,
     per channel a seeded flagged-tile count ~ U{0..40} and beam count ~
-    floor(exp U[0, ln 2049)) (log-uniform 1..2048) within 15 deg; FP32
-    forward PFB + FP16 MFMA beamformer, Stokes IQUV, every channel in one
-    persistent launch inside one replayed hipGraph; seconds generated on the
-    device.  Per channel, beams_check beams x nsamp_check samples x 128
-    channels are checked against the FP64 oracle beamformer fed with the
-    GPU's own PFB output of that channel."""
+    floor(exp U[0, ln 2049)) (log-uniform 1..2048) within 15 deg; forward
+    PFB to FP16 (PFB_COMPLEX_FLOAT16, the shared buffer) + FP16 MFMA
+    beamformer, Stokes IQUV, every channel in one persistent launch inside
+    one replayed hipGraph; seconds generated on the device.  Per channel,
+    beams_check beams x nsamp_check samples x 128 channels are checked
+    against the FP64 oracle chain (oracle PFB of the same seeded voltages,
+    then the FP64 beamformer), so the check covers the FP16 voltages too."""
     import ctypes as C
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc  # the checker (per-channel error only)
     from vcsbeam_amd.apps import BatchSession
-    from vcsbeam_amd.workloads import (SEED_BEAMS, SEED_FLAGS, Observation, directions_in_cap,
-                                       uniform, unit_vectors)
+    from vcsbeam_amd.workloads import (SEED_BEAMS, SEED_DATA, SEED_FLAGS, Observation,
+                                       directions_in_cap, mwax_voltages, uniform, unit_vectors)
     u = uniform(SEED_FLAGS, 2 * nchan, offset=10 ** 8)
     nflag = np.minimum((u[0::2] * 41).astype(int), 40)
     nbeam = np.floor(np.exp(u[1::2] * np.log(2049.0))).astype(int)
@@ -367,9 +368,10 @@ def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_c
     dirs = [directions_in_cap(int(nbeam[g]), 15.0, seed=SEED_BEAMS + 100 * (g + 1))
             for g in range(nchan)]
     t0 = time.perf_counter()
-    bs = BatchSession(obs, dirs, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
-                      nstokes=4, form=pkg.VM_BF_MFMA_FP16, share_pfb_output=True,
-                      device_data=True)
+    flags = (pkg.PFB_FULL_PRECISION & ~pkg.PFB_TYPE_MASK) | pkg.PFB_COMPLEX_FLOAT16 | \
+        pkg.PFB_EMULATE_FPGA
+    bs = BatchSession(obs, dirs, pfb_flags=flags, nstokes=4, form=pkg.VM_BF_MFMA_FP16,
+                      share_pfb_output=True, device_data=True)
     try:
         bs.read(0)
         t_setup = time.perf_counter() - t0
@@ -396,18 +398,20 @@ def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_c
         per = []
         for g, ses in enumerate(bs.sessions):
             o, (az, za) = obs[g], dirs[g]
-            ses.pfb()
-            v = np.empty((128, o.ninput, nsamp_check), np.complex64)
-            lib.vmMemcpy2DDtoH(v.ctypes.data, nsamp_check * 8, ses.v.d_v, nspc * 8,
-                               nsamp_check * 8, 128 * o.ninput)
+            # oracle PFB of the first nsamp_check spectra: margin block (zeros
+            # in the first second) and the second's first block
+            htr = np.ascontiguousarray(np.concatenate(
+                [np.zeros((1, o.ninput, 64000, 2), np.int8),
+                 mwax_voltages(o.ninput, 1, first_block=0, seed=SEED_DATA + o.seed_offset)]))
+            v = orc.pfb_forward(htr, o.ninput, nsamp_check, o.filter.astype(np.int32),
+                                o.out_idx(), fpga=True)  # [s][c][i]
             pk = min(beams_check, len(az))
             B = np.zeros((pk, o.ntiles, 2, 2), complex)
             B[..., 0, 0] = B[..., 1, 1] = 1
             E, N, H, L = o.positions()
             phi = orc.calc_phi(E, N, H, L, unit_vectors(az[:pk], za[:pk]), o.freqs)
             polQ, polP = o.pol_idx()
-            Sref, _ = orc.beamform(np.ascontiguousarray(v.transpose(2, 0, 1)).astype(np.complex128),
-                                   orc.calc_J(o.calibration(), B), phi, polQ, polP,
+            Sref, _ = orc.beamform(v, orc.calc_J(o.calibration(), B), phi, polQ, polP,
                                    1.0 / (2 * int((~o.flag_mask).sum())), nstokes=4)
             got = ses.stokes_slice(0, pk, 0, nsamp_check)
             err = np.array([[orc.rel_err(got[p, :, k, :], Sref[p, :, k, :]) for k in range(4)]
@@ -416,15 +420,16 @@ def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_c
                         "gemm_tiles": nact[g],
                         "max_beam_rel_err_IQUV": [float(f"{x:.3e}") for x in err]})
         return {"workload": f"{nchan} coarse channels x 1 s, {ntiles} tiles x 2 pols, flagged "
-                            f"tiles U{{0..40}} and beams log-uniform 1..2048 per channel, FP32 PFB "
-                            f"+ FP16 MFMA beamformer, Stokes IQUV, one persistent launch in one "
-                            f"hipGraph",
+                            f"tiles U{{0..40}} and beams log-uniform 1..2048 per channel, PFB to "
+                            f"FP16 + FP16 MFMA beamformer, Stokes IQUV, one persistent launch in "
+                            f"one hipGraph",
                 "value": round(bsamp / (ms * 1e-3), 1), "unit": "beam-samples/s",
                 "beams_total": int(nbeam.sum()), "ms_per_second_of_data": round(ms, 2),
                 "stages_eager_ms": {k: round(x, 2) for k, x in st.items()},
                 "gemm_tflops_algorithmic": round(flops / (st["gemm_ms"] * 1e-3) / 1e12, 1),
                 "gemm_GBps_algorithmic": round(gemm_bytes / (st["gemm_ms"] * 1e-3) / 1e9, 1),
-                "pfb_hbm_frac": round(nchan * 2 * ntiles * nspc * 128 * 10
+                # 2 B in, 4 B (complex FP16) out per input-sample
+                "pfb_hbm_frac": round(nchan * 2 * ntiles * nspc * 128 * 6
                                       / (st["pfb_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "schedule": {**info, "tuned_gemm_ms": round(tune_ms, 2)},
                 "setup_s": round(t_setup, 1),

@@ -258,7 +258,10 @@ gpuDoubleComplex *roots_of_unity(int N);
 typedef enum vcsbeam_datatype_t {
     VM_INT4,  /* legacy 4+4-bit [s][c][i] (reference) */
     VM_DBL,   /* complex double [s][c][i] (reference) */
-    VM_FLT    /* MI355X: complex float [c][i][s], i in VCS order */
+    VM_FLT,   /* MI355X: complex float [c][i][s], i in VCS order */
+    VM_HALF   /* MI355X: complex FP16 (re, im halves of a dword, each rounded
+                 to nearest even) in 32-sample blocks [c][s/32][i][s%32], the
+                 last block padded; the MFMA forms' input */
 } vcsbeam_datatype;
 
 typedef enum vm_error_t {
@@ -475,6 +478,8 @@ typedef enum pfb_flags_t {
     PFB_COMPLEX_INT4 = 0x10,
     PFB_COMPLEX_FLOAT64 = 0x20,
     PFB_COMPLEX_FLOAT32 = 0x40, /* MI355X: device output VM_FLT [c][i][s] */
+    PFB_COMPLEX_FLOAT16 = 0x80, /* MI355X: device output VM_HALF [c][s/32][i][s%32]
+                                   (fused kernel only), read by the MFMA forms */
     PFB_IMAG_PART_FIRST = 0x100,
     PFB_EMULATE_FPGA = 0x200,
     PFB_STAGED = 0x400,         /* MI355X: run WOLA/round/FFT/pack as the
@@ -487,7 +492,8 @@ typedef struct forward_pfb_t {
     vm_char2 *d_htr_data;
     void *vcs_data;              /* host output, reference layout/format */
     void *d_vcs_data;            /* device output: VM_FLT [c][i][s] complex
-                                    float, or VM_INT4 legacy [s][c][i] */
+                                    float, VM_HALF [c][s/32][i][s%32] complex
+                                    FP16, or VM_INT4 legacy [s][c][i] */
     size_t d_htr_size;
     size_t htr_stride;
     size_t vcs_size;

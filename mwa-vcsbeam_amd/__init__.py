@@ -21,7 +21,7 @@ RUNTIME_DIR = os.path.join(HERE, "pfb_filter")
 # ------------------------------------------------------------------ enums --
 VCSLegacyRecombined = 4
 VCSMWAXv2 = 5
-VM_INT4, VM_DBL, VM_FLT = 0, 1, 2
+VM_INT4, VM_DBL, VM_FLT, VM_HALF = 0, 1, 2, 3
 VM_SUCCESS, VM_END_OF_DATA, VM_READ_BUFFER_NOT_SET, VM_READ_BUFFER_LOCKED = 0, 1, 2, 3
 VM_INCOH_SCRATCH_PAIRS = 65536  # partial extremes the no-allocation incoherent beam holds
 ANALYSIS_FILTER, SYNTHESIS_FILTER = 0, 1
@@ -29,9 +29,11 @@ PFB_MALLOC_HOST_INPUT = 0x01
 PFB_MALLOC_HOST_OUTPUT = 0x02
 PFB_MALLOC_DEVICE_INPUT = 0x04
 PFB_MALLOC_DEVICE_OUTPUT = 0x08
+PFB_TYPE_MASK = 0xF0
 PFB_COMPLEX_INT4 = 0x10
 PFB_COMPLEX_FLOAT64 = 0x20
 PFB_COMPLEX_FLOAT32 = 0x40
+PFB_COMPLEX_FLOAT16 = 0x80
 PFB_IMAG_PART_FIRST = 0x100
 PFB_EMULATE_FPGA = 0x200
 PFB_STAGED = 0x400

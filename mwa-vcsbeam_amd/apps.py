@@ -16,7 +16,7 @@ import numpy as np
 
 from . import (ANALYSIS_FILTER, SYNTHESIS_FILTER, BeamGeom, GpuIpfbArrays, MpiPsrfits,
                PFB_EMULATE_FPGA, PFB_FULL_PRECISION, PFB_MALLOC_HOST_OUTPUT, PFB_SMART, VCSMWAXv2, VM_BF_FP32,
-               VM_SUCCESS, VcsbeamContext, Vdifinfo, load)
+               VM_HALF, VM_SUCCESS, VcsbeamContext, Vdifinfo, load)
 from .workloads import unit_vectors
 
 # samples per block of the device Stokes layout d_S [c][s / 64][p][st][s % 64]
@@ -140,11 +140,21 @@ def tied_array_beam(obs, az, za, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, nch
     return out
 
 
+def half_blocks_to_cis(h, nchan, ninput, ns):
+    """VM_HALF device voltages (float16 (re, im) pairs in 32-sample blocks
+    [c][s/32][i][s%32], the last block padded) -> float16 [c][i][s][2]."""
+    nsb = (ns + 31) // 32
+    a = np.asarray(h).reshape(nchan, nsb, ninput, 32, 2).transpose(0, 2, 1, 3, 4)
+    return np.ascontiguousarray(a.reshape(nchan, ninput, nsb * 32, 2)[:, :, :ns])
+
+
 def fine_pfb_offline(obs, *, nseconds=1, nchunks=1, pfb_flags=PFB_SMART | PFB_MALLOC_HOST_OUTPUT,
                      voltage_dir=None, first_gps=None):
     """Forward PFB of `nseconds` seconds to the legacy recombined format.
     Returns a list of per-second host outputs (uint8 [s][c][i] for SMART,
-    complex128 [s][c][i] for FULL_PRECISION + host output).  With
+    complex128 [s][c][i] for FULL_PRECISION + host output, complex64
+    [c][i][s] for PFB_COMPLEX_FLOAT32, float16 [c][i][s][re, im] for
+    PFB_COMPLEX_FLOAT16).  With
     `voltage_dir`, the seconds from `first_gps` on are read from MWAX
     subfiles there (vmReadSecondFromFiles) instead of generated."""
     from . import VoltageFiles
@@ -178,12 +188,18 @@ def fine_pfb_offline(obs, *, nseconds=1, nchunks=1, pfb_flags=PFB_SMART | PFB_MA
                                     shape=(fp.vcs_size,)).copy()
         if pfb_flags & 0x10:
             outs.append(raw.reshape(fp.nspectra, fp.K, fp.I))
-        elif pfb_flags & 0x40:
-            # device layout [c][i][s] per chunk, chunks concatenated
+        elif pfb_flags & 0xC0:
+            # device layout per chunk, chunks concatenated: complex64 [c][i][s],
+            # or (0x80) float16 (re, im) pairs [c][s/32][i][s%32]; returned as
+            # [c][i][s] (float16 with a trailing (re, im) axis)
             nch = fp.nspectra // fp.nspectra_per_chunk
-            a = raw.view(np.complex64).reshape(nch, fp.K, fp.I, fp.nspectra_per_chunk)
-            outs.append(np.ascontiguousarray(a.transpose(1, 2, 0, 3)).reshape(fp.K, fp.I,
-                                                                              fp.nspectra))
+            if pfb_flags & 0x80:
+                a = np.stack([half_blocks_to_cis(x, fp.K, fp.I, fp.nspectra_per_chunk)
+                              for x in raw.view(np.float16).reshape(nch, -1)])
+            else:
+                a = raw.view(np.complex64).reshape(nch, fp.K, fp.I, fp.nspectra_per_chunk)
+            a = np.ascontiguousarray(np.moveaxis(a, 0, 2))
+            outs.append(a.reshape((fp.K, fp.I, fp.nspectra) + a.shape[4:]))
         else:
             outs.append(raw.view(np.complex128).reshape(fp.nspectra, fp.K, fp.I))
     lib.destroy_vcsbeam_context(vm)
@@ -603,11 +619,19 @@ class TiedArraySession:
                                             v.nfine_chan)).copy()
 
     def voltages(self):
-        """The device voltages d_v (VM_FLT [c][i][s]) as complex64."""
+        """The device voltages d_v (VM_FLT [c][i][s], or VM_HALF widened) as
+        complex64."""
         v = self.v
         n = v.nfine_chan * self.obs.ninput * v.fine_sample_rate
-        out = np.empty(n, np.complex64)
-        self.lib.vmMemcpyDtoH(out.ctypes.data, v.d_v, out.nbytes)
+        if v.datatype == VM_HALF:
+            nsb = (v.fine_sample_rate + 31) // 32
+            h = np.empty(v.nfine_chan * nsb * 32 * self.obs.ninput * 2, np.float16)
+            self.lib.vmMemcpyDtoH(h.ctypes.data, v.d_v, h.nbytes)
+            h = half_blocks_to_cis(h, v.nfine_chan, self.obs.ninput, v.fine_sample_rate)
+            out = h.astype(np.float32).view(np.complex64)
+        else:
+            out = np.empty(n, np.complex64)
+            self.lib.vmMemcpyDtoH(out.ctypes.data, v.d_v, out.nbytes)
         return out.reshape(v.nfine_chan, self.obs.ninput, v.fine_sample_rate)
 
     def stokes_slice(self, p0, p1, s0, s1):

@@ -628,7 +628,8 @@ void vm_beamform_fp32_launch(const void *d_v, vcsbeam_datatype dt, int ns, int n
                            ni, (const cf *)d_W, d_polQ, d_polP, nant, npointing, invw, nstokes,
                            soffset, ns_total, d_S, (cf *)d_e);
     } else {
-        VM_FATAL("beamformer: unsupported voltage datatype %d", (int)dt);
+        VM_FATAL("FP32 beamformer: voltage datatype %d; it reads VM_FLT or VM_INT4 (VM_HALF, "
+                 "from PFB_COMPLEX_FLOAT16, is the MFMA forms' input)", (int)dt);
     }
     HIPCHECK(hipGetLastError());
 }

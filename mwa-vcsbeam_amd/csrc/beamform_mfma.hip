@@ -508,10 +508,16 @@ __global__ void k_jcheck(const double2 *__restrict__ J, uint32_t per_pointing,
 // group): "antennas" are GEMM slots (polQ, polP, Jf and gw given per slot,
 // padding slots with zero Jf), and the T_k are summed per group g of gb
 // into float4 [c][s][VM_MFMA_MAX_GROUPS][4].
-// I4: legacy 4+4-bit voltages [s][c][i] (one byte per input-sample, high
-// nibble re, low nibble im, two's complement; include/vcsbeam.h.in:997-1003)
-// instead of VM_FLT [c][i][s]: the block's 32 rows of ni bytes are staged in
-// LDS with coalesced loads and each lane decodes its inputs from there.
+// VIN, the voltages' kind: PREP_VIN_F32 VM_FLT [c][i][s] complex float;
+// PREP_VIN_F16 VM_HALF, complex FP16 in 32-sample blocks [c][s/32][i][s%32]
+// (a workgroup's block of every input is one contiguous run); PREP_VIN_I4 legacy
+// 4+4-bit [s][c][i] (one byte per input-sample, high nibble re, low nibble
+// im, two's complement; include/vcsbeam.h.in:997-1003), whose block's 32
+// rows of ni bytes are staged in LDS with coalesced loads and each lane
+// decodes its inputs from there.
+#define PREP_VIN_F32 0
+#define PREP_VIN_I4 1
+#define PREP_VIN_F16 2
 __device__ __forceinline__ cf nibbles_to_cf(uint32_t b)
 {
     const int re = (int)((b >> 4) & 0xfu), im = (int)(b & 0xfu);
@@ -519,7 +525,8 @@ __device__ __forceinline__ cf nibbles_to_cf(uint32_t b)
 }
 
 #define PREP_I4_MAX_INPUTS 512
-template <bool BF16, int APT, int NG, int PB, bool I4>
+#define PREP_F16_MAX_INPUTS 512
+template <bool BF16, int APT, int NG, int PB, int VIN>
 __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv, int ns, int ni, int nsb,
                                                  int nkstep, const cf *__restrict__ Jf,
                                                  const float4 *__restrict__ gw,
@@ -529,13 +536,19 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
                                                  float4 *__restrict__ N, grp_bounds gb)
 {
     constexpr int NGRP = PB == 2 ? VM_MFMA_MAX_GROUPS : 1;
+    constexpr bool I4 = VIN == PREP_VIN_I4;
+    constexpr bool H16 = VIN == PREP_VIN_F16;
     __shared__ uint32_t vrow[I4 ? SB * PREP_I4_MAX_INPUTS / 4 : 1];
     const cf *v = (const cf *)vv;
     typedef mtype<BF16> M;
     constexpr int NT = 32 * NG; // lanes
     __shared__ float4 red[NG][SB];
     __shared__ float wmx[NG / 2];
-    __shared__ uint4 frag[(NG * APT / 8) * 2 * 64]; // nkstep x 2 pols x 64 lanes
+    // nkstep x 2 pols x 64 lanes of fragments; with H16 first the block's
+    // staged FP16 input (SB x ni dwords), consumed before the fragments
+    constexpr int NFRAG = (NG * APT / 8) * 2 * 64;
+    constexpr int NSTG = H16 ? SB * PREP_F16_MAX_INPUTS / 4 : 0;
+    __shared__ uint4 frag[NFRAG > NSTG ? NFRAG : NSTG];
     const int sb = blockIdx.x, c = blockIdx.y;
     const int t = threadIdx.x, sl = t & 31, g = t >> 5;
     const int s = SB * sb + sl;
@@ -543,7 +556,17 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
     const cf *vc = v + (size_t)c * ni * ns;
     const int nfrag = nkstep * 2 * 64;
 
-    for (int k = t; k < nfrag; k += NT) frag[k] = make_uint4(0u, 0u, 0u, 0u);
+    if constexpr (!H16)
+        for (int k = t; k < nfrag; k += NT) frag[k] = make_uint4(0u, 0u, 0u, 0u);
+    if constexpr (H16) {
+        // this block of every input, [c][s/32][i][s%32]: one contiguous run
+        // of SB ni dwords, staged with 16-byte loads (samples past ns are the
+        // padding of the last block and are never used)
+        const uint4 *src = (const uint4 *)((const uint32_t *)vv +
+                                           ((size_t)c * ((ns + SB - 1) / SB) + sb) * ni * SB);
+        for (int k = t; k < SB * ni / 4; k += NT) frag[k] = src[k];
+        __syncthreads();
+    }
     if constexpr (I4) {
         // rows (s, c) of ni bytes for the block's samples; rows past ns zero
         const uint8_t *vb = (const uint8_t *)vv;
@@ -575,6 +598,13 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
                 const uint8_t *lb = (const uint8_t *)vrow + sl * ni;
                 q = nibbles_to_cf(lb[polQ[a]]);
                 p = nibbles_to_cf(lb[polP[a]]);
+            } else if constexpr (H16) {
+                const uint32_t *vh = (const uint32_t *)frag + sl; // the staged block
+                const uint32_t hq = vh[polQ[a] * SB], hp = vh[polP[a] * SB];
+                q = cf{(float)__builtin_bit_cast(_Float16, (uint16_t)(hq & 0xffffu)),
+                       (float)__builtin_bit_cast(_Float16, (uint16_t)(hq >> 16))};
+                p = cf{(float)__builtin_bit_cast(_Float16, (uint16_t)(hp & 0xffffu)),
+                       (float)__builtin_bit_cast(_Float16, (uint16_t)(hp >> 16))};
             } else {
                 q = vc[(size_t)polQ[a] * ns + s];
                 p = vc[(size_t)polP[a] * ns + s];
@@ -589,6 +619,10 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
             uy[j].x = w10.x * q.x - w10.y * q.y + w11.x * p.x - w11.y * p.y;
             uy[j].y = w10.x * q.y + w10.y * q.x + w11.x * p.y + w11.y * p.x;
         }
+    }
+    if constexpr (H16) {
+        __syncthreads(); // every lane has read its inputs from the staged block
+        for (int k = t; k < nfrag; k += NT) frag[k] = make_uint4(0u, 0u, 0u, 0u);
     }
     float nxx = 0.f, nyy = 0.f, xyr = 0.f, xyi = 0.f, mx = 0.f;
     float4 tk[NGRP][4];
@@ -1549,25 +1583,28 @@ static void ensure_buf(void **p, size_t *cap, size_t need)
 // from nant).  pbm 1: the per-pointing primary-beam form (Jf = D^-1, N =
 // T_k); pbm 2: the tile-group form (per-slot Jf = D^-1 and gw, N = T_gk of
 // the groups of gb).
-// int4: legacy 4+4-bit voltages [s][c][i] in v (else VM_FLT [c][i][s]).
+// vin: the voltages' kind in v (PREP_VIN_*).
 static void launch_prep(bool bf16, hipStream_t st, const void *v, int ns, int ni, int nsb,
                         int nkstep, int nchan, const cf *Jf, const float4 *gw,
                         const uint32_t *polQ, const uint32_t *polP, int nant, uint4 *Bf, int *sexp,
-                        float4 *N, int pbm, const grp_bounds &gb, bool int4 = false)
+                        float4 *N, int pbm, const grp_bounds &gb, int vin)
 {
     if (nant > 256) VM_FATAL("MFMA beamformer: at most 256 antennas (%d)", nant);
-    if (int4 && ni > PREP_I4_MAX_INPUTS)
+    if (vin == PREP_VIN_I4 && ni > PREP_I4_MAX_INPUTS)
         VM_FATAL("MFMA beamformer: at most %d RF inputs for 4+4-bit input (%d)",
                  PREP_I4_MAX_INPUTS, ni);
+    if (vin == PREP_VIN_F16 && ni > PREP_F16_MAX_INPUTS)
+        VM_FATAL("MFMA beamformer: at most %d RF inputs for FP16 input (%d)",
+                 PREP_F16_MAX_INPUTS, ni);
     const dim3 pg((unsigned)nsb, (unsigned)nchan);
+#define VM_PREP1(B_, A_, G_, P_, V_)                                                            \
+    hipLaunchKernelGGL((k_bf_prep<B_, A_, G_, P_, V_>), pg, dim3(32 * G_), 0, st, v, ns, ni, nsb, \
+                       nkstep, Jf, gw, polQ, polP, nant, Bf, sexp, N, gb)
 #define VM_PREP(B_, A_, G_, P_)                                                               \
     do {                                                                                       \
-        if (int4)                                                                              \
-            hipLaunchKernelGGL((k_bf_prep<B_, A_, G_, P_, true>), pg, dim3(32 * G_), 0, st, v,  \
-                               ns, ni, nsb, nkstep, Jf, gw, polQ, polP, nant, Bf, sexp, N, gb); \
-        else                                                                                   \
-            hipLaunchKernelGGL((k_bf_prep<B_, A_, G_, P_, false>), pg, dim3(32 * G_), 0, st, v, \
-                               ns, ni, nsb, nkstep, Jf, gw, polQ, polP, nant, Bf, sexp, N, gb); \
+        if (vin == PREP_VIN_I4) VM_PREP1(B_, A_, G_, P_, PREP_VIN_I4);                         \
+        else if (vin == PREP_VIN_F16) VM_PREP1(B_, A_, G_, P_, PREP_VIN_F16);                  \
+        else VM_PREP1(B_, A_, G_, P_, PREP_VIN_F32);                                           \
     } while (0)
 #define VM_PREP2(A_, G_)                                                                      \
     if (pbm == 2) { if (bf16) VM_PREP(true, A_, G_, 2); else VM_PREP(false, A_, G_, 2); }     \
@@ -1582,6 +1619,7 @@ static void launch_prep(bool bf16, hipStream_t st, const void *v, int ns, int ni
     }
 #undef VM_PREP2
 #undef VM_PREP
+#undef VM_PREP1
     HIPCHECK(hipGetLastError());
 }
 
@@ -1689,10 +1727,12 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     vm_device_state *d = vm_dev(vm);
     hipStream_t st = (hipStream_t)d->stream;
     const bool bf16 = vm->beamform_form == VM_BF_MFMA_BF16;
-    if (vm->datatype != VM_FLT && vm->datatype != VM_INT4)
-        VM_FATAL("MFMA beamformer needs VM_FLT (forward PFB output) or VM_INT4 (legacy) "
-                 "voltages; got %d", (int)vm->datatype);
-    const bool int4 = vm->datatype == VM_INT4;
+    if (vm->datatype != VM_FLT && vm->datatype != VM_HALF && vm->datatype != VM_INT4)
+        VM_FATAL("MFMA beamformer needs VM_FLT or VM_HALF (forward PFB output) or VM_INT4 "
+                 "(legacy) voltages; got %d", (int)vm->datatype);
+    const int vin = vm->datatype == VM_INT4   ? PREP_VIN_I4
+                    : vm->datatype == VM_HALF ? PREP_VIN_F16
+                                              : PREP_VIN_F32;
     const int nant = (int)vm->obs_metadata->num_ants, ni = (int)vm->obs_metadata->num_rf_inputs;
     const int nchan = vm->nfine_chan, np = (int)vm->npointing;
     if (ni != 2 * nant) VM_FATAL("MFMA beamformer: %d RF inputs for %d antennas", ni, nant);
@@ -1856,10 +1896,10 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     vm_timer_start(d, VM_T_BF_PREP);
     if (pb == 2)
         launch_prep(bf16, st, vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw, d_sq, d_sp, nk, Bf,
-                    sexp, N, 2, gb, int4);
+                    sexp, N, 2, gb, vin);
     else
         launch_prep(bf16, st, vm->d_v, ns, ni, nsb, nkstep, nchan, Jf, gw, vm->d_polQ_idxs,
-                    vm->d_polP_idxs, nant, Bf, sexp, N, pb, gb, int4);
+                    vm->d_polP_idxs, nant, Bf, sexp, N, pb, gb, vin);
     vm_timer_stop(d, VM_T_BF_PREP);
     pl->nant = nant; pl->ni = ni; pl->nchan = nchan; pl->np = np; pl->nkstep = nkstep;
     pl->nrb = nrb; pl->ns = ns; pl->nsb = nsb;
@@ -2004,7 +2044,7 @@ void vmBeamformChunkMFMA(vcsbeam_context *vm)
 
 void vm_pfb_fused_launch(const void *d_htr, size_t htr_bytes, const int *d_h,
                          const int *d_out_idx, int I, int nspc, int P, int fpga, int imag_first,
-                         int split, void *d_out, hipStream_t stream);
+                         int split, int half, void *d_out, hipStream_t stream);
 
 struct vm_batch_chan {
     vcsbeam_context *vm;
@@ -2230,7 +2270,7 @@ static void batch_pfb(vm_batch *b, int g, hipStream_t st)
     vm_pfb_fused_launch(f->d_htr_data, f->d_htr_size, f->d_filter_coeffs, f->d_i_output_idx, f->I,
                         f->nspectra_per_chunk, f->P, (f->flags & PFB_EMULATE_FPGA) != 0,
                         (f->flags & PFB_IMAG_PART_FIRST) != 0, !f->filter_fits_int16,
-                        f->d_vcs_data, st);
+                        b->ch[g].vm->datatype == VM_HALF, f->d_vcs_data, st);
 }
 
 static void batch_prep(vm_batch *b, int g, hipStream_t st)
@@ -2239,7 +2279,8 @@ static void batch_prep(vm_batch *b, int g, hipStream_t st)
     const grp_bounds none = {};
     launch_prep(b->bf16, st, (const cf *)C.vm->fpfb->d_vcs_data, b->ns,
                 (int)C.vm->obs_metadata->num_rf_inputs, b->nsb, C.nkstep, b->nchan, C.Jf,
-                (const float4 *)NULL, C.d_polQ, C.d_polP, C.nact, C.Bf, C.sexp, C.N, 0, none);
+                (const float4 *)NULL, C.d_polQ, C.d_polP, C.nact, C.Bf, C.sexp, C.N, 0, none,
+                C.vm->datatype == VM_HALF ? PREP_VIN_F16 : PREP_VIN_F32);
 }
 
 static void batch_gemm(vm_batch *b, hipStream_t st)
@@ -2286,8 +2327,10 @@ vm_batch *vmBatchCreate(vcsbeam_context **vms, int n)
         vcsbeam_context *vm = vms[g];
         forward_pfb *f = vm ? vm->fpfb : NULL;
         if (!f) VM_FATAL("vmBatchCreate: context %d has no forward PFB", g);
-        if (vm->datatype != VM_FLT || (f->flags & PFB_STAGED))
-            VM_FATAL("vmBatchCreate: context %d: the batch runs the fused PFB to VM_FLT", g);
+        if ((vm->datatype != VM_FLT && vm->datatype != VM_HALF) || (f->flags & PFB_STAGED) ||
+            vm->datatype != v0->datatype)
+            VM_FATAL("vmBatchCreate: context %d: the batch runs the fused PFB to VM_FLT or "
+                     "VM_HALF, the same in every context", g);
         if (vm->chunks_per_second != 1)
             VM_FATAL("vmBatchCreate: context %d: one chunk per second is required", g);
         if (vm->nfine_chan != b->nchan || (int)vm->fine_sample_rate != b->ns)

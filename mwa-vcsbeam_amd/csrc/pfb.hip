@@ -170,13 +170,31 @@ __device__ __forceinline__ void pfb_put_window(const uint4 (&wr)[PFB_WPL], uint3
     for (int r = 0; r < PFB_WPL; r++) ((uint4 *)win)[t + 256 * r] = wr[r];
 }
 
+// One output value: complex float, or (H16) complex FP16 packed in a dword
+// (each part rounded to nearest even).
+template <bool H16> struct pfb_out;
+template <> struct pfb_out<false> {
+    typedef cf T;
+    static __device__ __forceinline__ T make(float a, float b) { return cf{a, b}; }
+};
+template <> struct pfb_out<true> {
+    typedef uint32_t T;
+    static __device__ __forceinline__ T make(float a, float b)
+    {
+        return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)a) |
+               ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)b) << 16);
+    }
+};
+
 // WOLA + rounding + FFT + reorder of one tile whose window is in LDS (win
 // aliases F); ends with the tile's stores.  Contains workgroup barriers.
 // IMF: output as (im, re) instead of (re, im) (src/pfb.cpp:322-331).
-template <bool FPGA, bool SPLIT, bool IMF>
+// H16: complex FP16 output (VM_HALF) in 32-sample blocks [c][s/32][i][s%32]
+// (the last block padded), so that one block of every input is contiguous.
+template <bool FPGA, bool SPLIT, bool IMF, bool H16>
 __device__ __forceinline__ void pfb_tile(cf *F, const uint32_t *win, const cf *tw,
                                          const pfb_taps &T, int i, int I, int m0, int nspc, int oi,
-                                         cf *__restrict__ out, int t)
+                                         typename pfb_out<H16>::T *__restrict__ out, int t)
 {
     constexpr int P = PFB_P;
     constexpr int NW = 8 + P - 1; // window blocks per lane
@@ -253,8 +271,13 @@ __device__ __forceinline__ void pfb_tile(cf *F, const uint32_t *win, const cf *t
     // the channel stride I nspc from one per-tile base.
     const int fs = min(f, nvalid - 1);
     const int src = (t & 32) + fs; // lane of (fs, l) in this wavefront
-    const size_t cstride = (size_t)I * nspc;
-    cf *ob = out + ((size_t)(l + KCH / 2) * I + oi) * nspc + m0 + fs;
+    // channel stride and this lane's first output; a tile's 32 spectra are
+    // one block of the H16 layout (m0 is a multiple of TILE_M = 32)
+    const size_t nsbp = (size_t)(nspc + TILE_M - 1) / TILE_M;
+    const size_t cstride = H16 ? nsbp * I * TILE_M : (size_t)I * nspc;
+    typename pfb_out<H16>::T *ob =
+        H16 ? out + (((size_t)(l + KCH / 2) * nsbp + m0 / TILE_M) * I + oi) * TILE_M + fs
+            : out + ((size_t)(l + KCH / 2) * I + oi) * nspc + m0 + fs;
     if (__builtin_expect(nvalid < TILE_M, 0)) { // the last tile of an input only
 #pragma unroll
         for (int hh = 0; hh < 2; hh++)
@@ -270,7 +293,8 @@ __device__ __forceinline__ void pfb_tile(cf *F, const uint32_t *win, const cf *t
         for (int k2 = 0; k2 < 8; k2++) {
             const long coef = 8 * hh + 16 * k2 - (k2 >= 4 ? KCH : 0);
             const cf v = y[hh][k2];
-            ob[coef * (long)cstride] = IMF ? v : cf{v.y, v.x};
+            ob[coef * (long)cstride] = IMF ? pfb_out<H16>::make(v.x, v.y)
+                                           : pfb_out<H16>::make(v.y, v.x);
         }
 }
 
@@ -283,10 +307,11 @@ __device__ __forceinline__ void pfb_tile(cf *F, const uint32_t *win, const cf *t
 // instructions the wait for the window is vmcnt(#stores), not vmcnt(0).
 // Taps and twiddles are loaded once per workgroup.  htr_bytes: size of the
 // chunk's input buffer (loads past it return zeros).
-template <bool FPGA, bool SPLIT, bool IMF>
+template <bool FPGA, bool SPLIT, bool IMF, bool H16>
 __global__ __launch_bounds__(256) void k_pfb_fused(
     const uint32_t *__restrict__ htr, uint32_t htr_bytes, const int *__restrict__ h,
-    const int *__restrict__ out_idx, int I, int nspc, cf *__restrict__ out, int per)
+    const int *__restrict__ out_idx, int I, int nspc, typename pfb_out<H16>::T *__restrict__ out,
+    int per)
 {
     __shared__ cf F[TILE_M * ROW]; // FFT rows (33 KB); the window aliases it
     __shared__ cf tw[KCH];         // twiddles in LDS: no vector-memory loads in the loop
@@ -311,8 +336,8 @@ __global__ __launch_bounds__(256) void k_pfb_fused(
     const int t1 = min(T0 + 1, Tend - 1);
     pfb_fetch_window(wr, rs, t1 / nmb, I, (t1 % nmb) * TILE_M, t);
     __syncthreads();
-    pfb_tile<FPGA, SPLIT, IMF>(F, win, tw, T, T0 / nmb, I, (T0 % nmb) * TILE_M, nspc,
-                               out_idx[T0 / nmb], out, t);
+    pfb_tile<FPGA, SPLIT, IMF, H16>(F, win, tw, T, T0 / nmb, I, (T0 % nmb) * TILE_M, nspc,
+                                    out_idx[T0 / nmb], out, t);
     for (int tt = T0 + 1; tt < Tend; tt++) {
         const int i = tt / nmb, m0 = (tt % nmb) * TILE_M;
         __syncthreads(); // the previous tile's FFT is done with F
@@ -320,7 +345,7 @@ __global__ __launch_bounds__(256) void k_pfb_fused(
         const int tn = min(tt + 1, Tend - 1); // the last prefetch repeats this tile's
         pfb_fetch_window(wr, rs, tn / nmb, I, (tn % nmb) * TILE_M, t);
         __syncthreads();
-        pfb_tile<FPGA, SPLIT, IMF>(F, win, tw, T, i, I, m0, nspc, out_idx[i], out, t);
+        pfb_tile<FPGA, SPLIT, IMF, H16>(F, win, tw, T, i, I, m0, nspc, out_idx[i], out, t);
     }
 }
 
@@ -433,24 +458,25 @@ __global__ __launch_bounds__(256) void k_flt_to_legacy(const cf *__restrict__ in
 // resident at once (registers and LDS decide how many per CU, queried once
 // per kernel instantiation), so no workgroup starts after the first ones
 // finish.
-template <bool FPGA, bool SPLIT, bool IMF>
+template <bool FPGA, bool SPLIT, bool IMF, bool H16>
 static void pfb_fused_run(const uint32_t *x, size_t htr_bytes, const int *d_h, const int *d_out_idx,
-                          int I, int nspc, cf *o, hipStream_t stream)
+                          int I, int nspc, void *o, hipStream_t stream)
 {
     static int ncu = 0, bpc = 0;
     if (!ncu) {
         int dev = 0;
         HIPCHECK(hipGetDevice(&dev));
         HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_pfb_fused<FPGA, SPLIT, IMF>,
-                                                              256, 0));
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &bpc, k_pfb_fused<FPGA, SPLIT, IMF, H16>, 256, 0));
         if (bpc < 1) bpc = 1;
     }
     const int ntiles = ((nspc + TILE_M - 1) / TILE_M) * I;
     const int nwg = min(ntiles, bpc * ncu);
     const int per = (ntiles + nwg - 1) / nwg;
-    hipLaunchKernelGGL((k_pfb_fused<FPGA, SPLIT, IMF>), dim3((ntiles + per - 1) / per), dim3(256),
-                       0, stream, x, (uint32_t)htr_bytes, d_h, d_out_idx, I, nspc, o, per);
+    hipLaunchKernelGGL((k_pfb_fused<FPGA, SPLIT, IMF, H16>), dim3((ntiles + per - 1) / per),
+                       dim3(256), 0, stream, x, (uint32_t)htr_bytes, d_h, d_out_idx, I, nspc,
+                       (typename pfb_out<H16>::T *)o, per);
 }
 
 // =================================================================== host ==
@@ -514,6 +540,16 @@ void vmInitForwardPFB(vcsbeam_context *vm, int M, pfb_flags flags)
         fpfb->vcs_size = (size_t)fpfb->nspectra * fpfb->I * fpfb->K * sizeof(gpuFloatComplex);
         fpfb->d_vcs_size = nsamp_chunk * sizeof(gpuFloatComplex);
         vm->datatype = VM_FLT;
+    } else if (type & PFB_COMPLEX_FLOAT16) {
+        // the fused kernel's values rounded to FP16, VM_FLT's layout
+        if (flags & PFB_STAGED)
+            VM_FATAL("vmInitForwardPFB: PFB_COMPLEX_FLOAT16 output is written by the fused "
+                     "kernel only (not with PFB_STAGED)");
+        // 32-sample blocks, the last padded: [c][s/32][i][s%32] per chunk
+        const size_t nsbp = ((size_t)fpfb->nspectra_per_chunk + 31) / 32;
+        fpfb->d_vcs_size = (size_t)fpfb->K * nsbp * 32 * fpfb->I * 2 * sizeof(uint16_t);
+        fpfb->vcs_size = fpfb->d_vcs_size * vm->chunks_per_second;
+        vm->datatype = VM_HALF;
     } else {
         // default (PFB_COMPLEX_FLOAT64): host output is complex double legacy
         // layout; on the device the beamformer reads the VM_FLT form
@@ -611,24 +647,34 @@ static cf *flt_target(vcsbeam_context *vm)
 // API and by the benchmark / batched paths).
 void vm_pfb_fused_launch(const void *d_htr, size_t htr_bytes, const int *d_h,
                          const int *d_out_idx, int I, int nspc, int P, int fpga, int imag_first,
-                         int split, void *d_out, hipStream_t stream)
+                         int split, int half, void *d_out, hipStream_t stream)
 {
     if (P != PFB_P) VM_FATAL("fused PFB kernel is built for 12 taps (got %d); use PFB_STAGED", P);
     if (htr_bytes >= ((size_t)1 << 31))
         VM_FATAL("fused PFB: chunk input of %zu bytes exceeds 2 GiB; use more chunks", htr_bytes);
     const uint32_t *x = (const uint32_t *)d_htr;
-    cf *o = (cf *)d_out;
-    const int sel = (fpga ? 4 : 0) + (split ? 2 : 0) + (imag_first ? 1 : 0);
+    void *o = d_out;
+    const int sel = (half ? 8 : 0) + (fpga ? 4 : 0) + (split ? 2 : 0) + (imag_first ? 1 : 0);
+#define VM_PFB_RUN(F_, S_, I_, H_) pfb_fused_run<F_, S_, I_, H_>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream)
     switch (sel) {
-    case 0: pfb_fused_run<false, false, false>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
-    case 1: pfb_fused_run<false, false, true>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
-    case 2: pfb_fused_run<false, true, false>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
-    case 3: pfb_fused_run<false, true, true>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
-    case 4: pfb_fused_run<true, false, false>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
-    case 5: pfb_fused_run<true, false, true>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
-    case 6: pfb_fused_run<true, true, false>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
-    default: pfb_fused_run<true, true, true>(x, htr_bytes, d_h, d_out_idx, I, nspc, o, stream); break;
+    case 0: VM_PFB_RUN(false, false, false, false); break;
+    case 1: VM_PFB_RUN(false, false, true, false); break;
+    case 2: VM_PFB_RUN(false, true, false, false); break;
+    case 3: VM_PFB_RUN(false, true, true, false); break;
+    case 4: VM_PFB_RUN(true, false, false, false); break;
+    case 5: VM_PFB_RUN(true, false, true, false); break;
+    case 6: VM_PFB_RUN(true, true, false, false); break;
+    case 7: VM_PFB_RUN(true, true, true, false); break;
+    case 8: VM_PFB_RUN(false, false, false, true); break;
+    case 9: VM_PFB_RUN(false, false, true, true); break;
+    case 10: VM_PFB_RUN(false, true, false, true); break;
+    case 11: VM_PFB_RUN(false, true, true, true); break;
+    case 12: VM_PFB_RUN(true, false, false, true); break;
+    case 13: VM_PFB_RUN(true, false, true, true); break;
+    case 14: VM_PFB_RUN(true, true, false, true); break;
+    default: VM_PFB_RUN(true, true, true, true); break;
     }
+#undef VM_PFB_RUN
     HIPCHECK(hipGetLastError());
 }
 
@@ -655,7 +701,7 @@ void vmWOLAChunk(vcsbeam_context *vm)
                             fpfb->nspectra_per_chunk, fpfb->P,
                             (fpfb->flags & PFB_EMULATE_FPGA) != 0,
                             (fpfb->flags & PFB_IMAG_PART_FIRST) != 0, !fpfb->filter_fits_int16,
-                            flt_target(vm), s);
+                            vm->datatype == VM_HALF, flt_target(vm), s);
         vm_timer_stop(vm_dev(vm), VM_T_PFB);
     }
     HIPCHECK(hipStreamSynchronize(s));
@@ -727,7 +773,7 @@ void vmDownloadForwardPFBChunk(vcsbeam_context *vm)
     int chunk = vm->chunk_to_load % vm->chunks_per_second;
     char *dst = (char *)fpfb->vcs_data + (size_t)chunk * fpfb->vcs_stride;
     int type = fpfb->flags & PFB_TYPE_MASK;
-    if ((type & PFB_COMPLEX_INT4) || (type & PFB_COMPLEX_FLOAT32)) {
+    if ((type & PFB_COMPLEX_INT4) || (type & PFB_COMPLEX_FLOAT32) || (type & PFB_COMPLEX_FLOAT16)) {
         HIPCHECK(hipMemcpyAsync(dst, fpfb->d_vcs_data, fpfb->d_vcs_size, hipMemcpyDeviceToHost, s));
     } else {
         vm_device_state *d = vm_dev(vm);

@@ -149,3 +149,35 @@ def test_device_generator_matches_host_generator(pkg, gpu_lib):
     diff = np.abs(dev.astype(np.int16) - host.astype(np.int16))
     assert diff.max() <= 1 and int((diff != 0).sum()) <= 4
     assert not dev[0].any()
+
+
+def test_batch_fp16_pfb_output(pkg, orc, gpu_lib):
+    """The batch with the contexts' forward PFBs writing FP16 (VM_HALF) into
+    the shared buffer (half its bytes): each channel within 1e-5 of the
+    single-channel MFMA path reading the same FP16 voltages, and within the
+    MFMA bars of the FP64 oracle fed the exact PFB output."""
+    from vcsbeam_amd.apps import BatchSession
+    obs, dirs = _case()
+    flags = (pkg.PFB_FULL_PRECISION & ~pkg.PFB_TYPE_MASK) | pkg.PFB_COMPLEX_FLOAT16 | \
+        pkg.PFB_EMULATE_FPGA
+    print("batch with FP16 PFB output", flush=True)
+    bs = BatchSession(obs, dirs, pfb_flags=flags, form=1)
+    try:
+        bs.read(0)
+        bs.run(use_graph=False)
+        S = [ses.stokes() for ses in bs.sessions]
+        for g, (o, (az, za)) in enumerate(zip(obs, dirs)):
+            err = _errors(orc, S[g], _oracle(orc, o, az, za))
+            print(f"channel {g}: max per-beam rel err I,Q,U,V vs FP64:", err.max(axis=0),
+                  flush=True)
+            assert err[:, 0].max() <= 1e-3
+            assert err[:, 1:].max() <= 3e-3
+        for g, ses in enumerate(bs.sessions):
+            assert ses.v.datatype == pkg.VM_HALF
+            ses.pfb()
+            ses.beamform()
+            err = _errors(orc, S[g].astype(np.float64), ses.stokes().astype(np.float64))
+            print(f"channel {g}: batch vs single-channel path:", err.max(axis=0), flush=True)
+            assert err.max() <= 1e-5
+    finally:
+        bs.close()

@@ -293,3 +293,36 @@ def test_mfma_legacy_int4_input(pkg, orc, gpu_lib, form, tol_i, tol_quv):
     print("max per-beam rel err I,Q,U,V:", err.max(axis=0))
     assert err[:, 0].max() <= tol_i
     assert err[:, 1:].max() <= tol_quv
+
+
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
+def test_mfma_fp16_pfb_output(pkg, orc, gpu_lib, form, tol_i, tol_quv):
+    """The forward PFB's FP16 output (PFB_COMPLEX_FLOAT16, VM_HALF) read by
+    the precision form: against the FP64 oracle fed the exact PFB output, to
+    the MFMA bars (the voltages' FP16 rounding is inside them).  Workload: 128
+    tiles, 0.1 s, 40 directions within 15 deg."""
+    from vcsbeam_amd.apps import tied_array_beam
+    from vcsbeam_amd.workloads import Observation, directions_in_cap, unit_vectors
+    obs = Observation(ntiles=128, blocks_per_second=2)
+    az, za = directions_in_cap(40, 15.0)
+    flags = (pkg.PFB_FULL_PRECISION & ~pkg.PFB_TYPE_MASK) | pkg.PFB_COMPLEX_FLOAT16 | \
+        pkg.PFB_EMULATE_FPGA
+    print(f"MFMA form {form} on FP16 PFB output, {len(az)} beams", flush=True)
+    out = tied_array_beam(obs, az, za, pfb_flags=flags, nstokes=4, form=form, want_e=False,
+                          device_geometry=True)
+    cur = obs.second(0)
+    htr = np.ascontiguousarray(np.concatenate([np.zeros_like(cur[:1]), cur], axis=0))
+    v = orc.pfb_forward(htr, obs.ninput, obs.nspectra, obs.filter.astype(np.int32), obs.out_idx(),
+                        fpga=True)
+    D = obs.calibration()
+    B = np.zeros((len(az), obs.ntiles, 2, 2), complex)
+    B[..., 0, 0] = B[..., 1, 1] = 1
+    E, N, H, L = obs.positions()
+    phi = orc.calc_phi(E, N, H, L, unit_vectors(az, za), obs.freqs)
+    polQ, polP = obs.pol_idx()
+    Sref, _ = orc.beamform(v, orc.calc_J(D, B), phi, polQ, polP,
+                           1.0 / (2 * int((~obs.flag_mask).sum())), 4)
+    err = _errors(orc, out["S"][0], Sref)
+    print("max per-beam rel err I,Q,U,V:", err.max(axis=0))
+    assert err[:, 0].max() <= tol_i
+    assert err[:, 1:].max() <= tol_quv

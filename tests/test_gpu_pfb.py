@@ -113,3 +113,23 @@ def test_chunking_and_second_continuity(pkg, orc, gpu_lib):
     # VM_FLT is [c][i][s]; the oracle is [s][c][i]
     got = one[1].transpose(2, 0, 1)
     assert orc.rel_err(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("finepfb", [False, True])
+def test_fused_pfb_fp16_output_is_fp32_output_rounded(pkg, gpu_lib, finepfb):
+    """PFB_COMPLEX_FLOAT16 (VM_HALF, the MFMA forms' input): the fused kernel's
+    FP16 output is its FP32 output (PFB_COMPLEX_FLOAT32, held to the oracle at
+    1e-5 above) rounded to nearest even, bit for bit, in the same [c][i][s]
+    layout; with the int16 taps and with FINEPFB's split taps, over two
+    chunks."""
+    from vcsbeam_amd.apps import fine_pfb_offline
+    obs = _obs(pkg, ntiles=32, finepfb=finepfb)
+    base = (pkg.PFB_FULL_PRECISION & ~pkg.PFB_TYPE_MASK) | pkg.PFB_MALLOC_HOST_OUTPUT | \
+        pkg.PFB_EMULATE_FPGA
+    print(f"fused PFB to FP32 and FP16, finepfb={finepfb}", flush=True)
+    f32 = fine_pfb_offline(obs, nchunks=2, pfb_flags=base | pkg.PFB_COMPLEX_FLOAT32)[0]
+    f16 = fine_pfb_offline(obs, nchunks=2, pfb_flags=base | pkg.PFB_COMPLEX_FLOAT16)[0]
+    want = np.stack([f32.real, f32.imag], axis=-1).astype(np.float16)
+    assert f16.shape == want.shape
+    np.testing.assert_array_equal(f16.view(np.uint16), want.view(np.uint16))
+    assert np.abs(f32).max() < 6e4  # within FP16's range
