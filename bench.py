@@ -273,12 +273,15 @@ def rocprof_summary(dirs, title, command, out_md, out_json):
                   open(out_json, "w"), indent=1)
 
 
-def committed_traffic(kernel, variant, order):
+def committed_traffic(kernel, variant, order, form=1):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3
     summaries.  The beamformer GEMM's traffic depends on its tile shape and
-    order (how often voltage fragments are re-read), so its records are kept
-    per (variant, order): profiles/bench_traffic_v<variant>_o<order>.json."""
-    f = os.path.join(ROOT, "profiles", f"bench_traffic_v{variant}_o{order}.json")
+    order (how often voltage fragments are re-read) and on the weight parts
+    its form reads, so its records are kept per (form, variant, order):
+    profiles/bench_traffic_v<variant>_o<order>.json for the split FP16 form,
+    profiles/bench_traffic_f<form>_v<variant>_o<order>.json for the others."""
+    tag = "" if form == 1 else f"f{form}_"
+    f = os.path.join(ROOT, "profiles", f"bench_traffic_{tag}v{variant}_o{order}.json")
     if not os.path.exists(f):
         return None
     try:
@@ -671,6 +674,11 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="fix the MFMA beamformer variant instead of autotuning")
     ap.add_argument("--order", type=int, default=1, help="tile order with --variant")
+    ap.add_argument("--form", type=int, default=3, choices=(1, 2, 3),
+                    help="MFMA form of the headline: 1 FP16 voltages with split hi+lo FP16 "
+                         "weights, 2 BF16 split, 3 FP16 with single FP16 weights")
+    ap.add_argument("--no-side-form", action="store_true",
+                    help="skip timing the other FP16 form beside the headline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 1 GiB plain copy (memory-counter calibration)")
     ap.add_argument("--rocprof-summary", nargs="+", metavar="DIR",
@@ -742,61 +750,87 @@ def main():
     pfb_bytes = nsamp_in * 10  # 2 B in + 8 B out per input complex sample
     beam_samples = args.beams * 128 * obs.nspectra
 
-    print("[bench] MFMA FP16 form: autotune", file=sys.stderr, flush=True)
     import ctypes as C
-    sess.v.beamform_form = pkg.VM_BF_MFMA_FP16
-    sess.geometry()
-    best = C.c_float(0)
-    if args.variant >= 0:
-        variant = args.variant
-        lib.vmSetBeamformerVariant(sess.vm, args.variant, args.order)
-    else:
-        variant = lib.vmTuneBeamformer(sess.vm, 3, C.byref(best))
-    print("[bench] MFMA FP16 form: warmup + timed steps", file=sys.stderr, flush=True)
-    ms_step = time_form(lib, sess, pkg.VM_BF_MFMA_FP16, args.steps, args.warmup)
-    gemm_ms, gemm_n = kernel_ms(lib, sess, "beamform_mfma")
-    sel_v, sel_o = C.c_int(0), C.c_int(0)
-    lib.vmBeamformerSelection(sess.vm, C.byref(sel_v), C.byref(sel_o))
-    prep_ms, _ = kernel_ms(lib, sess, "beamform_prep")
-    weights_ms, _ = kernel_ms(lib, sess, "beamform_weights")
+    parts = {1: 2, 2: 2, 3: 1}  # weight parts (split hi + lo, or one)
+    form_name = {1: "mfma-fp16-split", 2: "mfma-bf16-split", 3: "mfma-fp16-single"}
+
+    def run_form(form):
+        """Autotune (or fix) the GEMM variant for one MFMA form, then time
+        the step; per-kernel times are live HIP events on the context stream."""
+        print(f"[bench] MFMA form {form} ({form_name[form]}): autotune", file=sys.stderr,
+              flush=True)
+        sess.v.beamform_form = form
+        sess.geometry()
+        best = C.c_float(0)
+        if args.variant >= 0:
+            variant = args.variant
+            lib.vmSetBeamformerVariant(sess.vm, args.variant, args.order)
+        else:
+            variant = lib.vmTuneBeamformer(sess.vm, 3, C.byref(best))
+        print(f"[bench] MFMA form {form}: warmup + timed steps", file=sys.stderr, flush=True)
+        ms = time_form(lib, sess, form, args.steps, args.warmup)
+        g_ms, g_n = kernel_ms(lib, sess, "beamform_mfma")
+        sv, so = C.c_int(0), C.c_int(0)
+        lib.vmBeamformerSelection(sess.vm, C.byref(sv), C.byref(so))
+        return {"form": form, "ms_step": ms, "gemm_ms": g_ms, "gemm_n": g_n, "variant": variant,
+                "order": so.value, "sel_variant": sv.value, "best": best.value,
+                "prep_ms": kernel_ms(lib, sess, "beamform_prep")[0],
+                "weights_ms": kernel_ms(lib, sess, "beamform_weights")[0],
+                "S": sess.stokes()}
+
+    head = run_form(args.form)
+    side_form = 1 if args.form == 3 else 3
+    side = None if args.no_side_form else run_form(side_form)
+    ms_step, gemm_ms, gemm_n, variant = head["ms_step"], head["gemm_ms"], head["gemm_n"], head["variant"]
+    prep_ms, weights_ms = head["prep_ms"], head["weights_ms"]
     vr, vs, vw = C.c_int(0), C.c_int(0), C.c_int(0)
     lib.vmBeamformerVariantInfo(variant, C.byref(vr), C.byref(vs), C.byref(vw))
     value = beam_samples / (ms_step * 1e-3)
-    S_mfma = sess.stokes()
+    S_mfma = head["S"]
     # Dominant kernel k_beamform_mfma, per launch (one channel-second, 512 beams):
     # algorithmic bytes = Stokes IQUV out (4 B x 4 per beam-sample) + J^-1-applied
-    # voltages in (FP16 complex, 4 B per tile-pol-sample) + hi/lo phase weights
-    # (2 x 2 B per beam-row x k') + N terms in (16 B per channel-sample);
-    # algorithmic flops = 8 x 2 pols x tiles per beam-sample (phase GEMM after
-    # J^-1 is applied per antenna), issued = 2 x that (split weights).
+    # voltages in (FP16 complex, 4 B per tile-pol-sample) + phase weights
+    # (2 B per beam-row x k' per weight part: hi + lo, or one) + N terms in
+    # (16 B per channel-sample); algorithmic flops = 8 x 2 pols x tiles per
+    # beam-sample (phase GEMM after J^-1 is applied per antenna), issued = that
+    # times the weight parts.
     ntile, nch, nsamp = obs.ntiles, 128, obs.nspectra
-    gemm_bytes = (beam_samples * 4 * 4 + nch * ntile * 2 * nsamp * 4
-                  + nch * (2 * args.beams) * (2 * ntile) * 2 * 2 + nch * nsamp * 16)
+
+    def gemm_algorithmic_bytes(form):
+        return (beam_samples * 4 * 4 + nch * ntile * 2 * nsamp * 4
+                + nch * (2 * args.beams) * (2 * ntile) * 2 * parts[form] + nch * nsamp * 16)
+    gemm_bytes = gemm_algorithmic_bytes(args.form)
     gemm_flops = 8.0 * 2 * ntile * beam_samples
     achieved = gemm_bytes / (gemm_ms * 1e-3) / 1e9
     mfma_tf = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    issued = parts[args.form]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": committed_traffic("k_beamform_mfma", sel_v.value, sel_o.value),
+                "traffic": committed_traffic("k_beamform_mfma", head["sel_variant"],
+                                             head["order"], args.form),
                 "algorithmic_bytes": int(gemm_bytes), "algorithmic_flops": gemm_flops,
                 "mfma_tflops": round(mfma_tf, 1),
-                "mfma_frac_issued": round(2 * mfma_tf / FP16_PEAK_TFLOPS, 4),
+                "mfma_frac_issued": round(issued * mfma_tf / FP16_PEAK_TFLOPS, 4),
                 "mfma_measured_peak_tflops": round(mfma_peak_tf, 1),
-                "mfma_frac_of_measured_issued": round(2 * mfma_tf / mfma_peak_tf, 4),
-                "kernel": "k_beamform_mfma<fp16>", "kernel_ms": round(gemm_ms, 4),
-                "launches": gemm_n, "autotuned_variant": variant, "tile_order": sel_o.value,
+                "mfma_frac_of_measured_issued": round(issued * mfma_tf / mfma_peak_tf, 4),
+                "kernel": f"k_beamform_mfma<{form_name[args.form]}>",
+                "kernel_ms": round(gemm_ms, 4),
+                "launches": gemm_n, "autotuned_variant": variant, "tile_order": head["order"],
                 "workgroup_tile": f"{vr.value} rows x {vs.value} samples, {vw.value} waves",
-                "autotune_best_ms": round(best.value, 4)}
+                "autotune_best_ms": round(head["best"], 4)}
+    dtype = {1: "f16 (split hi+lo f16 weights, f32 accumulate)",
+             2: "bf16 (split hi+lo bf16 weights, f32 accumulate)",
+             3: "f16 (f16 weights, f32 accumulate)"}[args.form]
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "beam-samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f16 (split hi+lo f16 weights, f32 accumulate)",
+        "dtype": dtype,
         "data": "synthetic (seeded Gaussian 8+8-bit voltages, sigma 16)",
         "config": {"workload": "128 tiles x 2 pols, 1 coarse channel x 1 s, 512 beams within "
-                               "15 deg of boresight, MFMA FP16 beamformer, Stokes IQUV",
+                               "15 deg of boresight, MFMA beamformer, Stokes IQUV",
                    "tiles": 128, "beams": args.beams, "fine_channels": 128,
-                   "samples": obs.nspectra, "form": "mfma-fp16"},
+                   "samples": obs.nspectra, "form": form_name[args.form]},
         "roofline": roofline,
         "stages": {"pfb_fused_ms_per_ch_s": round(pfb_ms, 4),
                    "pfb_GBps": round(pfb_bytes / (pfb_ms * 1e-3) / 1e9, 1),
@@ -809,6 +843,17 @@ def main():
                    "beamform_prep_ms": round(prep_ms, 4),
                    "beamform_weights_ms": round(weights_ms, 4)},
     }
+    S_side = None
+    if side is not None:
+        sb = gemm_algorithmic_bytes(side_form)
+        result["other_fp16_form"] = {
+            "form": form_name[side_form], "value": round(beam_samples / (side["ms_step"] * 1e-3), 1),
+            "ms_per_step": round(side["ms_step"], 4), "kernel_ms": round(side["gemm_ms"], 4),
+            "hbm_frac": round(sb / (side["gemm_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "autotuned_variant": side["variant"], "tile_order": side["order"],
+            "traffic": committed_traffic("k_beamform_mfma", side["sel_variant"], side["order"],
+                                         side_form)}
+        S_side = side["S"]
     S_fp32 = None
     if not args.no_fp32:
         print("[bench] plain FP32 form", file=sys.stderr, flush=True)
@@ -844,7 +889,8 @@ def main():
                 full["end_to_end_s_per_channel_second_one_beam"],
             "samples": full["sample"], "samples_1_thread": one["sample"]}
         par = {}
-        for name, S in (("mfma_fp16", S_mfma), ("fp32", S_fp32)):
+        for name, S in ((form_name[args.form], S_mfma),
+                        (form_name[side_form], S_side), ("fp32", S_fp32)):
             if S is None:
                 continue
             e = np.array([[orc.rel_err(S[p, :n, st, :], cb["S"][p, :n, st, :]) for st in range(4)]

@@ -557,7 +557,9 @@ typedef int (*vm_read_second_fn)(void *user, uint64_t gps_second,
 typedef enum vm_beamform_form_t {
     VM_BF_FP32 = 0,          /* plain FP32 drop-in (default) */
     VM_BF_MFMA_FP16 = 1,     /* FP16 voltages, split hi+lo FP16 weights, MFMA */
-    VM_BF_MFMA_BF16 = 2      /* BF16 voltages, split hi+lo BF16 weights, MFMA */
+    VM_BF_MFMA_BF16 = 2,     /* BF16 voltages, split hi+lo BF16 weights, MFMA */
+    VM_BF_MFMA_FP16_SINGLE = 3 /* FP16 voltages, one FP16 part per weight, MFMA:
+                                  half the MFMA work of VM_BF_MFMA_FP16 */
 } vm_beamform_form;
 
 typedef struct vcsbeam_context_t {

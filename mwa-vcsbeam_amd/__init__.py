@@ -39,7 +39,7 @@ PFB_EMULATE_FPGA = 0x200
 PFB_STAGED = 0x400
 PFB_SMART = 0x21D
 PFB_FULL_PRECISION = 0x2D
-VM_BF_FP32, VM_BF_MFMA_FP16, VM_BF_MFMA_BF16 = 0, 1, 2
+VM_BF_FP32, VM_BF_MFMA_FP16, VM_BF_MFMA_BF16, VM_BF_MFMA_FP16_SINGLE = 0, 1, 2, 3
 MWA_ALTITUDE_METRES = 377.827
 SPEED_OF_LIGHT = 299792458.0
 

@@ -23,8 +23,12 @@
 // Precision form: u is converted to FP16 (or BF16) with a power-of-two scale
 // per (channel, 32-sample block); the phase weights are split phi = phi_hi +
 // phi_lo into two FP16 (BF16) parts and both products accumulate in FP32.
+// The single-weight FP16 form uses phi_hi alone: half the MFMA work and
+// operand traffic, with the weights' FP16 rounding (relative 2^-11) adding
+// about as much Stokes error as the voltages' own FP16 rounding.
 // Phases and Jones inverses are produced in FP64 (geometry.hip) and demoted
 // only at the split (phases) or to FP32 for the prep pass (J^-1).
+// GEMM-side templates take PREC: 0 FP16 split, 1 BF16 split, 2 FP16 single.
 //
 // Data flow per chunk:
 //   k_bf_prep        FP32 voltages -> u = J^-1 v, N terms, block scales and
@@ -97,6 +101,14 @@ template <> struct mtype<true> {
                                                        __builtin_bit_cast(v8, b), c, 0, 0, 0);
     }
 };
+
+// GEMM precision PREC: operand type (BF16 for 1) and weight parts (1 for 2)
+template <int PREC> constexpr bool prec_bf16() { return PREC == 1; }
+template <int PREC> constexpr int prec_parts() { return PREC == 2 ? 1 : 2; }
+static int form_prec(int form)
+{
+    return form == VM_BF_MFMA_BF16 ? 1 : form == VM_BF_MFMA_FP16_SINGLE ? 2 : 0;
+}
 
 // exponent e with x * 2^e in [2^13, 2^14) for the largest component x > 0
 __device__ __forceinline__ int scale_exp(float mx)
@@ -741,8 +753,8 @@ __device__ __forceinline__ uint4 bload(rsrc_t r, int voff, int soff)
     return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
-template <int MB> struct afrag {
-    uint4 a[MB][2]; // hi, lo
+template <int MB, int WP> struct afrag {
+    uint4 a[MB][WP]; // hi, lo (WP 2) or hi alone (WP 1)
 };
 template <int NB> struct bfrag {
     uint4 b[NB];
@@ -753,16 +765,15 @@ template <int NB> struct bfrag {
 // k-step ks at + 2048 ks (the two polarisation fragments of a k-step are
 // adjacent).
 // (valid false: an offset past any weights buffer, so the loads return zeros)
-template <int MB>
-__device__ __forceinline__ void load_a(afrag<MB> &f, rsrc_t rA, int vl, int aoff, int astr, int ks,
-                                       bool valid)
+template <int MB, int WP>
+__device__ __forceinline__ void load_a(afrag<MB, WP> &f, rsrc_t rA, int vl, int aoff, int astr,
+                                       int ks, bool valid)
 {
     const int base = valid ? aoff + 2048 * ks : 0x7fff0000;
 #pragma unroll
-    for (int m = 0; m < MB; m++) {
-        f.a[m][0] = bload(rA, vl, base + m * astr);
-        f.a[m][1] = bload(rA, vl, base + m * astr + 1024);
-    }
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int h = 0; h < WP; h++) f.a[m][h] = bload(rA, vl, base + m * astr + 1024 * h);
 }
 
 template <int NB>
@@ -785,18 +796,18 @@ __device__ __forceinline__ void interleave_loads()
     if constexpr (NM % NL != 0) __builtin_amdgcn_sched_group_barrier(0x008, NM % NL, 0);
 }
 
-template <bool BF16, int MB, int NB>
-__device__ __forceinline__ void mma_ab(f16v (&acc)[MB][NB], const afrag<MB> &A, const bfrag<NB> &B)
+template <int PREC, int MB, int NB>
+__device__ __forceinline__ void mma_ab(f16v (&acc)[MB][NB], const afrag<MB, prec_parts<PREC>()> &A,
+                                       const bfrag<NB> &B)
 {
     // independent hi products first, then the lo products
 #pragma unroll
-    for (int m = 0; m < MB; m++)
+    for (int h = 0; h < prec_parts<PREC>(); h++)
 #pragma unroll
-        for (int n = 0; n < NB; n++) acc[m][n] = mtype<BF16>::mfma(A.a[m][0], B.b[n], acc[m][n]);
+        for (int m = 0; m < MB; m++)
 #pragma unroll
-    for (int m = 0; m < MB; m++)
-#pragma unroll
-        for (int n = 0; n < NB; n++) acc[m][n] = mtype<BF16>::mfma(A.a[m][1], B.b[n], acc[m][n]);
+            for (int n = 0; n < NB; n++)
+                acc[m][n] = mtype<prec_bf16<PREC>()>::mfma(A.a[m][h], B.b[n], acc[m][n]);
 }
 
 // Stokes (I, Q, U, V) from a beam's (Ex, Ey) = (xr + i xi, yr + i yi): |E|^2
@@ -954,15 +965,16 @@ __device__ __forceinline__ void bf_horner(f16v (&acc)[MB][NB], const float *gxc,
 // Per-sample T rows of the tile-group form: 64 floats used, 68 apart
 #define BF_TROW 68
 // Fragments (1 KB each) of one k-step of a workgroup in the shared-operand
-// form: A (hi, lo) of its WR MB row blocks and B of its WC NB column blocks.
-template <int MB, int NB, int WR, int WC>
-constexpr int bf_wg_frags() { return 2 * WR * MB + WC * NB; }
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, int PB, bool LDSOP = false>
+// form: A (WP weight parts) of its WR MB row blocks and B of its WC NB column
+// blocks.
+template <int MB, int NB, int WR, int WC, int WP>
+constexpr int bf_wg_frags() { return WP * WR * MB + WC * NB; }
+template <int PREC, int MB, int NB, int WR, int WC, int NST, int PB, bool LDSOP = false>
 constexpr int bf_lds_floats()
 {
     return WR * WC * BF_WAVE_LDS + (PB == 1 ? WR * WC * MB * 16 * 32 : 0) +
            (PB == 2 ? WR * WC * (NB / 2) * 32 * BF_TROW : 0) +
-           (LDSOP ? 2 * bf_wg_frags<MB, NB, WR, WC>() * 256 : 0);
+           (LDSOP ? 2 * bf_wg_frags<MB, NB, WR, WC, prec_parts<PREC>()>() * 256 : 0);
 }
 
 // Stokes of one beam-sample from the GEMM outputs (x, y), in either form;
@@ -1001,7 +1013,7 @@ __device__ __forceinline__ void lds_sync()
 }
 
 // k-loop with workgroup-shared operands (the LDSOP form of bf_tile).  The
-// fragments of one k-step of the whole workgroup (bf_wg_frags: A hi/lo of
+// fragments of one k-step of the whole workgroup (bf_wg_frags: A parts of
 // its WR MB row blocks, B of its WC NB column blocks) are fetched once per
 // workgroup -- wavefront w takes fragments w, w + NW, ... -- two k-steps
 // ahead into registers (R0, R1), written to one of two LDS stages, and read
@@ -1010,11 +1022,11 @@ __device__ __forceinline__ void lds_sync()
 // per workgroup instead of once per wavefront.  One barrier per k-step;
 // stage s is rewritten only after every wavefront's reads of it completed
 // before the previous barrier.  Prefetches past the last k-step re-read it.
-template <int MB, int NB, int WR, int WC, int PER>
+template <int MB, int NB, int WR, int WC, int WP, int PER>
 __device__ __forceinline__ void wg_gload(uint4 (&R)[PER], rsrc_t rA, rsrc_t rB, int vl,
                                          const int (&goff)[PER], int k)
 {
-    constexpr int NW = WR * WC, FA = 2 * WR * MB;
+    constexpr int NW = WR * WC, FA = WP * WR * MB;
 #pragma unroll
     for (int i = 0; i < PER; i++) R[i] = bload(NW * i < FA ? rA : rB, vl, goff[i] + 2048 * k);
 }
@@ -1028,26 +1040,26 @@ __device__ __forceinline__ void wg_lwrite(uint4 *ops, const uint4 (&R)[PER], int
     for (int i = 0; i < PER; i++) ops[(stage * F + w + NW * i) * 64 + lane] = R[i];
 }
 
-template <int MB, int NB, int F, int FA>
-__device__ __forceinline__ void wg_lread(afrag<MB> &fa, bfrag<NB> &fb, const uint4 *ops, int stage,
-                                         int wr, int wc, int lane)
+template <int MB, int NB, int WP, int F, int FA>
+__device__ __forceinline__ void wg_lread(afrag<MB, WP> &fa, bfrag<NB> &fb, const uint4 *ops,
+                                         int stage, int wr, int wc, int lane)
 {
     const uint4 *o = ops + stage * F * 64 + lane;
 #pragma unroll
-    for (int m = 0; m < MB; m++) {
-        fa.a[m][0] = o[(2 * (wr * MB + m)) * 64];
-        fa.a[m][1] = o[(2 * (wr * MB + m) + 1) * 64];
-    }
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int h = 0; h < WP; h++) fa.a[m][h] = o[(WP * (wr * MB + m) + h) * 64];
 #pragma unroll
     for (int n = 0; n < NB; n++) fb.b[n] = o[(FA + wc * NB + n) * 64];
 }
 
-template <bool BF16, int MB, int NB, int WR, int WC, int PB>
+template <int PREC, int MB, int NB, int WR, int WC, int PB>
 __device__ __forceinline__ void bf_kloop_lds(f16v (&acc)[MB][NB], rsrc_t rA, rsrc_t rB, int vl,
                                              int astr, int bstr, int nkstep, int nsb, int ty,
                                              int tx, int w, int lane, float *stg)
 {
-    constexpr int NW = WR * WC, FA = 2 * WR * MB, F = bf_wg_frags<MB, NB, WR, WC>();
+    constexpr int WP = prec_parts<PREC>();
+    constexpr int NW = WR * WC, FA = WP * WR * MB, F = bf_wg_frags<MB, NB, WR, WC, WP>();
     static_assert(F % NW == 0 && FA % NW == 0, "fragments divide among the wavefronts");
     // two fragment sets and two prefetch sets beside the accumulators: small
     // wavefront tiles only (a 128 x 128 tile's 256 accumulators leave no room)
@@ -1062,7 +1074,7 @@ __device__ __forceinline__ void bf_kloop_lds(f16v (&acc)[MB][NB], rsrc_t rA, rsr
     for (int i = 0; i < PER; i++) {
         const int f = w + NW * i;
         if (NW * i < FA) {
-            goff[i] = (rbw0 + (f >> 1)) * astr + 1024 * (f & 1);
+            goff[i] = (rbw0 + f / WP) * astr + 1024 * (f % WP);
         } else {
             const int g = f - FA;
             // blocks past the last sample block re-read it (in bounds); their
@@ -1072,34 +1084,34 @@ __device__ __forceinline__ void bf_kloop_lds(f16v (&acc)[MB][NB], rsrc_t rA, rsr
     }
     const int kl = nkstep - 1;
     uint4 R0[PER], R1[PER];
-    afrag<MB> fa0, fa1;
+    afrag<MB, WP> fa0, fa1;
     bfrag<NB> fb0, fb1;
-    wg_gload<MB, NB, WR, WC, PER>(R0, rA, rB, vl, goff, 0);
-    wg_gload<MB, NB, WR, WC, PER>(R1, rA, rB, vl, goff, min(1, kl));
+    wg_gload<MB, NB, WR, WC, WP, PER>(R0, rA, rB, vl, goff, 0);
+    wg_gload<MB, NB, WR, WC, WP, PER>(R1, rA, rB, vl, goff, min(1, kl));
     wg_lwrite<PER, F>(ops, R0, 0, w, lane);
-    wg_gload<MB, NB, WR, WC, PER>(R0, rA, rB, vl, goff, min(2, kl));
+    wg_gload<MB, NB, WR, WC, WP, PER>(R0, rA, rB, vl, goff, min(2, kl));
     wg_lwrite<PER, F>(ops, R1, 1, w, lane);
-    wg_gload<MB, NB, WR, WC, PER>(R1, rA, rB, vl, goff, min(3, kl));
+    wg_gload<MB, NB, WR, WC, WP, PER>(R1, rA, rB, vl, goff, min(3, kl));
     lds_sync();
-    wg_lread<MB, NB, F, FA>(fa0, fb0, ops, 0, wr, wc, lane);
+    wg_lread<MB, NB, WP, F, FA>(fa0, fb0, ops, 0, wr, wc, lane);
     lds_sync(); // every wavefront holds k-step 0 before stage 0 is rewritten
     for (int ks = 0; ks < nkstep; ks += 2) {
         // fa0/fb0: k-step ks; stage 1: ks + 1; R0: ks + 2; R1: ks + 3
-        wg_lread<MB, NB, F, FA>(fa1, fb1, ops, 1, wr, wc, lane);
+        wg_lread<MB, NB, WP, F, FA>(fa1, fb1, ops, 1, wr, wc, lane);
         __builtin_amdgcn_sched_barrier(0);
-        mma_ab<BF16, MB, NB>(acc, fa0, fb0);
+        mma_ab<PREC, MB, NB>(acc, fa0, fb0);
         __builtin_amdgcn_sched_barrier(0);
         wg_lwrite<PER, F>(ops, R0, 0, w, lane);
-        wg_gload<MB, NB, WR, WC, PER>(R0, rA, rB, vl, goff, min(ks + 4, kl));
+        wg_gload<MB, NB, WR, WC, WP, PER>(R0, rA, rB, vl, goff, min(ks + 4, kl));
         lds_sync();
         if (ks + 1 < nkstep) {
             // fa1/fb1: k-step ks + 1; stage 0: ks + 2; R1: ks + 3; R0: ks + 4
-            wg_lread<MB, NB, F, FA>(fa0, fb0, ops, 0, wr, wc, lane);
+            wg_lread<MB, NB, WP, F, FA>(fa0, fb0, ops, 0, wr, wc, lane);
             __builtin_amdgcn_sched_barrier(0);
-            mma_ab<BF16, MB, NB>(acc, fa1, fb1);
+            mma_ab<PREC, MB, NB>(acc, fa1, fb1);
             __builtin_amdgcn_sched_barrier(0);
             wg_lwrite<PER, F>(ops, R1, 1, w, lane);
-            wg_gload<MB, NB, WR, WC, PER>(R1, rA, rB, vl, goff, min(ks + 5, kl));
+            wg_gload<MB, NB, WR, WC, WP, PER>(R1, rA, rB, vl, goff, min(ks + 5, kl));
             lds_sync();
         }
     }
@@ -1115,33 +1127,34 @@ __device__ __forceinline__ void bf_kloop_lds(f16v (&acc)[MB][NB], rsrc_t rA, rsr
 // outside the weights buffer, which returns zeros, so those MFMAs add
 // nothing.
 #define VM_SB() __builtin_amdgcn_sched_barrier(0)
-template <bool BF16, int MB, int NB, int NST>
+template <int PREC, int MB, int NB, int NST>
 __device__ __forceinline__ void bf_kloop_ring(f16v (&acc)[MB][NB], rsrc_t rA, rsrc_t rB, int vl,
                                               int aoff, int astr, const int (&boff)[NB], int k0,
                                               int k1)
 {
     constexpr int U = (NST % 2 == 0) ? NST : 2 * NST;
-    afrag<MB> fa[2];
+    constexpr int WP = prec_parts<PREC>();
+    afrag<MB, WP> fa[2];
     bfrag<NB> fb[NST];
     const int kl = k1 - 1;
-    load_a<MB>(fa[0], rA, vl, aoff, astr, k0, true);
+    load_a<MB, WP>(fa[0], rA, vl, aoff, astr, k0, true);
 #pragma unroll
     for (int r = 0; r < NST - 1; r++) load_b<NB>(fb[r], rB, vl, boff, min(k0 + r, kl));
     for (int ks = k0; ks < k1; ks += U) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             VM_SB();
-            load_a<MB>(fa[(u + 1) & 1], rA, vl, aoff, astr, ks + u + 1, ks + u + 1 <= kl);
+            load_a<MB, WP>(fa[(u + 1) & 1], rA, vl, aoff, astr, ks + u + 1, ks + u + 1 <= kl);
             load_b<NB>(fb[(u + NST - 1) % NST], rB, vl, boff, min(ks + u + NST - 1, kl));
-            mma_ab<BF16, MB, NB>(acc, fa[u & 1], fb[u % NST]);
-            interleave_loads<2 * MB + NB, 2 * MB * NB>();
+            mma_ab<PREC, MB, NB>(acc, fa[u & 1], fb[u % NST]);
+            interleave_loads<WP * MB + NB, WP * MB * NB>();
             VM_SB();
         }
     }
 }
 #undef VM_SB
 
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, bool VEC, int PB, bool LDSOP>
+template <int PREC, int MB, int NB, int WR, int WC, int NST, bool VEC, int PB, bool LDSOP>
 __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int nchan, int c,
                                         int tx, int ty, float *stg)
 {
@@ -1199,20 +1212,20 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
     }
 
     if constexpr (LDSOP) {
-        bf_kloop_lds<BF16, MB, NB, WR, WC, PB>(acc, rA, rB, vl, astr, bstr, nkstep, nsb, ty, tx, w,
+        bf_kloop_lds<PREC, MB, NB, WR, WC, PB>(acc, rA, rB, vl, astr, bstr, nkstep, nsb, ty, tx, w,
                                                lane, stg);
     } else if constexpr (PB == 2) {
         // the tile groups in turn, carried over by the Horner step
         const int ng = G.gb.n;
         for (int g = 0; g < ng; g++) {
-            bf_kloop_ring<BF16, MB, NB, NST>(acc, rA, rB, vl, aoff, astr, boff, G.gb.t[g] / 8,
+            bf_kloop_ring<PREC, MB, NB, NST>(acc, rA, rB, vl, aoff, astr, boff, G.gb.t[g] / 8,
                                              G.gb.t[g + 1] / 8);
             if (g + 1 < ng)
                 bf_horner<MB, NB>(acc, G.gx + (size_t)c * G.npointing * GX_ROW, G.npointing,
                                   16 * rb0, lane >> 5, g + 1);
         }
     } else {
-        bf_kloop_ring<BF16, MB, NB, NST>(acc, rA, rB, vl, aoff, astr, boff, 0, nkstep);
+        bf_kloop_ring<PREC, MB, NB, NST>(acc, rA, rB, vl, aoff, astr, boff, 0, nkstep);
     }
 
     // epilogue: in a 32-row block lane (lr, lk) holds rows 8 jg + 4 lk + {0..3}
@@ -1399,14 +1412,14 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
 // the same tile of 8 different channels, so each XCD walks its own channels;
 // within a channel tiles go in bands of gr row tiles, column by column (gr =
 // all row tiles for order 1, 1 for order 2, 2 for order 3, 4 for order 4).
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC, bool VEC, int PB,
+template <int PREC, int MB, int NB, int WR, int WC, int NST, int OCC, bool VEC, int PB,
           bool LDSOP>
 __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, int ns, int nsb,
                                                                      int nchan, int ntx, int nty,
                                                                      int order)
 {
     __shared__ __attribute__((aligned(16))) float
-        stg[bf_lds_floats<BF16, MB, NB, WR, WC, NST, PB, LDSOP>()];
+        stg[bf_lds_floats<PREC, MB, NB, WR, WC, NST, PB, LDSOP>()];
     const int ntile = ntx * nty;
     const int L = blockIdx.x;
     int c, tile;
@@ -1422,7 +1435,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, 
     if (nty % gr) gr = 1;
     const int band = tile / (gr * ntx), rem = tile % (gr * ntx);
     const int tx = rem / gr, ty = band * gr + rem % gr;
-    bf_tile<BF16, MB, NB, WR, WC, NST, VEC, PB, LDSOP>(G, ns, nsb, nchan, c, tx, ty, stg);
+    bf_tile<PREC, MB, NB, WR, WC, NST, VEC, PB, LDSOP>(G, ns, nsb, nchan, c, tx, ty, stg);
 }
 
 // Batched form: one persistent launch over the GEMM problems of several
@@ -1434,12 +1447,12 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_mfma(bf_geom G, 
 // must be zero at launch; every workgroup leaves the loop when the counter
 // passes nitems, so the grid always drains.  Every channel's ns, ns_total
 // and sample offset are multiples of 4 (16-byte Stokes stores).
-template <bool BF16, int MB, int NB, int WR, int WC, int NST, int OCC>
+template <int PREC, int MB, int NB, int WR, int WC, int NST, int OCC>
 __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
     const bf_geom *__restrict__ geoms, const uint2 *__restrict__ items, int nitems,
     uint32_t *__restrict__ queue, int ns, int nsb, int nchan)
 {
-    __shared__ __attribute__((aligned(16))) float stg[bf_lds_floats<BF16, MB, NB, WR, WC, NST, 0>()];
+    __shared__ __attribute__((aligned(16))) float stg[bf_lds_floats<PREC, MB, NB, WR, WC, NST, 0>()];
     __shared__ int slot[2];
     for (int it = 0;; it++) {
         // two slots: a slot is rewritten only after the barrier of the next
@@ -1456,7 +1469,7 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
         const int cnt = __builtin_amdgcn_readfirstlane((int)(e.y & 0xffu));
         const bf_geom G = geoms[g];
         for (int ty = ty0; ty < ty0 + cnt; ty++)
-            bf_tile<BF16, MB, NB, WR, WC, NST, true, 0, false>(G, ns, nsb, nchan, c, tx, ty, stg);
+            bf_tile<PREC, MB, NB, WR, WC, NST, true, 0, false>(G, ns, nsb, nchan, c, tx, ty, stg);
     }
 }
 
@@ -1479,7 +1492,7 @@ static const bf_variant kVariants[] = {
 // wavefronts: its T rows take 17 KB of LDS per wavefront)
 static bool variant_has_groups(int vi) { return vi == 0 || vi == 3 || vi == 4 || vi == 5 || vi == 8; }
 
-template <bool BF16, int PB>
+template <int PREC, int PB>
 static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, int ns, int nsb,
                            int nchan, int ntx, int nty, int order)
 {
@@ -1488,13 +1501,13 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
         // built for the first variant's shape only (its grid is the caller's
         // when vi = 0)
         if (vi != 0) VM_FATAL("MFMA beamformer: variant %d needs ns %% 4 == 0", vi);
-        hipLaunchKernelGGL((k_beamform_mfma<BF16, 4, 4, 2, 2, 4, 1, false, PB, false>), grid,
+        hipLaunchKernelGGL((k_beamform_mfma<PREC, 4, 4, 2, 2, 4, 1, false, PB, false>), grid,
                            dim3(256),
                            0, st, G, ns, nsb, nchan, ntx, nty, order);
         return;
     }
 #define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, OCC_, LDS_)                                       \
-    hipLaunchKernelGGL((k_beamform_mfma<BF16, MB_, NB_, WR_, WC_, NST_, OCC_, true, PB, LDS_>),  \
+    hipLaunchKernelGGL((k_beamform_mfma<PREC, MB_, NB_, WR_, WC_, NST_, OCC_, true, PB, LDS_>),  \
                        grid, dim3(64 * WR_ * WC_), 0, st, G, ns, nsb, nchan, ntx, nty, order)
     if constexpr (PB == 2) {
         switch (vi) {
@@ -1525,13 +1538,13 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
 #undef VM_LAUNCH
 }
 
-template <bool BF16>
+template <int PREC>
 static void launch_batch_variant(int vi, unsigned nwg, hipStream_t st, const bf_geom *geoms,
                                  const uint2 *items, int nitems, uint32_t *queue, int ns, int nsb,
                                  int nchan)
 {
 #define VM_LAUNCH(MB_, NB_, WR_, WC_, NST_, OCC_)                                             \
-    hipLaunchKernelGGL((k_beamform_batch<BF16, MB_, NB_, WR_, WC_, NST_, OCC_>), dim3(nwg),  \
+    hipLaunchKernelGGL((k_beamform_batch<PREC, MB_, NB_, WR_, WC_, NST_, OCC_>), dim3(nwg),  \
                        dim3(64 * WR_ * WC_), 0, st, geoms, items, nitems, queue, ns, nsb, nchan)
     switch (vi) {
     case 0: VM_LAUNCH(4, 4, 2, 2, 4, 1); break;
@@ -1930,17 +1943,18 @@ static void mfma_run(vcsbeam_context *vm, const mfma_plan *pl, int vi, int order
     G.nkstep = pl->nkstep; G.nrb = pl->nrb; G.npointing = pl->np; G.nstokes = vm->out_nstokes;
     G.soffset = pl->chunk * pl->ns; G.ns_total = vm->fine_sample_rate; G.invw = pl->invw;
     G.bx = pl->bx; G.esp = pl->esp; G.gx = pl->gx; G.gb = pl->gb; G.pad = 0;
-    const bool bf16 = vm->beamform_form == VM_BF_MFMA_BF16;
+    const int prec = form_prec(vm->beamform_form);
 #define VM_RUN(B_, P_) launch_variant<B_, P_>(vi, grid, st, G, pl->ns, pl->nsb, pl->nchan, ntx, nty, order)
-    if (bf16) {
-        if (pl->pb == 2) VM_RUN(true, 2);
-        else if (pl->pb == 1) VM_RUN(true, 1);
-        else VM_RUN(true, 0);
-    } else {
-        if (pl->pb == 2) VM_RUN(false, 2);
-        else if (pl->pb == 1) VM_RUN(false, 1);
-        else VM_RUN(false, 0);
-    }
+#define VM_RUN_PB(B_)                       \
+    do {                                    \
+        if (pl->pb == 2) VM_RUN(B_, 2);      \
+        else if (pl->pb == 1) VM_RUN(B_, 1); \
+        else VM_RUN(B_, 0);                  \
+    } while (0)
+    if (prec == 1) VM_RUN_PB(1);
+    else if (prec == 2) VM_RUN_PB(2);
+    else VM_RUN_PB(0);
+#undef VM_RUN_PB
 #undef VM_RUN
     HIPCHECK(hipGetLastError());
 }
@@ -2063,7 +2077,7 @@ struct vm_batch_chan {
 };
 
 struct vm_batch {
-    int n, nchan, ns, nsb, bf16, ncu;
+    int n, nchan, ns, nsb, bf16, prec, ncu;
     vm_batch_chan *ch;
     void *d_v;                    // forward-PFB output shared by contexts without their own
     bf_geom *d_geoms;
@@ -2290,12 +2304,15 @@ static void batch_gemm(vm_batch *b, hipStream_t st)
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, b->d_small, (size_t)1, 0u);
     const int per_cu = max(1, 4 * V.occ / (V.wr * V.wc));
     const unsigned nwg = (unsigned)min(b->nitems, b->ncu * per_cu);
-    if (b->bf16)
-        launch_batch_variant<true>(b->variant, nwg, st, b->d_geoms, b->d_items, b->nitems,
-                                   b->d_small, b->ns, b->nsb, b->nchan);
+    if (b->prec == 1)
+        launch_batch_variant<1>(b->variant, nwg, st, b->d_geoms, b->d_items, b->nitems,
+                                b->d_small, b->ns, b->nsb, b->nchan);
+    else if (b->prec == 2)
+        launch_batch_variant<2>(b->variant, nwg, st, b->d_geoms, b->d_items, b->nitems,
+                                b->d_small, b->ns, b->nsb, b->nchan);
     else
-        launch_batch_variant<false>(b->variant, nwg, st, b->d_geoms, b->d_items, b->nitems,
-                                    b->d_small, b->ns, b->nsb, b->nchan);
+        launch_batch_variant<0>(b->variant, nwg, st, b->d_geoms, b->d_items, b->nitems,
+                                b->d_small, b->ns, b->nsb, b->nchan);
     HIPCHECK(hipGetLastError());
 }
 
@@ -2322,6 +2339,7 @@ vm_batch *vmBatchCreate(vcsbeam_context **vms, int n)
     if (b->ns & 3) VM_FATAL("vmBatchCreate: %d samples per second (a multiple of 4 is needed)", b->ns);
     b->nsb = (b->ns + SB - 1) / SB;
     b->bf16 = v0->beamform_form == VM_BF_MFMA_BF16;
+    b->prec = form_prec(v0->beamform_form);
     size_t vneed = 0;
     for (int g = 0; g < n; g++) {
         vcsbeam_context *vm = vms[g];
@@ -2336,7 +2354,8 @@ vm_batch *vmBatchCreate(vcsbeam_context **vms, int n)
         if (vm->nfine_chan != b->nchan || (int)vm->fine_sample_rate != b->ns)
             VM_FATAL("vmBatchCreate: context %d differs in channels or samples", g);
         if (vm->beamform_form != v0->beamform_form ||
-            (vm->beamform_form != VM_BF_MFMA_FP16 && vm->beamform_form != VM_BF_MFMA_BF16))
+            (vm->beamform_form != VM_BF_MFMA_FP16 && vm->beamform_form != VM_BF_MFMA_BF16 &&
+             vm->beamform_form != VM_BF_MFMA_FP16_SINGLE))
             VM_FATAL("vmBatchCreate: context %d: all contexts need the same MFMA form", g);
         if (!vm->d_S || !vm->d_J || !vm->gdelays.d_phi || !vm->polQ_idxs || !f->d_htr_data)
             VM_FATAL("vmBatchCreate: context %d lacks S, J, phi, pol lists or PFB input", g);

@@ -151,7 +151,8 @@ def test_device_generator_matches_host_generator(pkg, gpu_lib):
     assert not dev[0].any()
 
 
-def test_batch_fp16_pfb_output(pkg, orc, gpu_lib):
+@pytest.mark.parametrize("form", [1, 3])
+def test_batch_fp16_pfb_output(pkg, orc, gpu_lib, form):
     """The batch with the contexts' forward PFBs writing FP16 (VM_HALF) into
     the shared buffer (half its bytes): each channel within 1e-5 of the
     single-channel MFMA path reading the same FP16 voltages, and within the
@@ -160,8 +161,8 @@ def test_batch_fp16_pfb_output(pkg, orc, gpu_lib):
     obs, dirs = _case()
     flags = (pkg.PFB_FULL_PRECISION & ~pkg.PFB_TYPE_MASK) | pkg.PFB_COMPLEX_FLOAT16 | \
         pkg.PFB_EMULATE_FPGA
-    print("batch with FP16 PFB output", flush=True)
-    bs = BatchSession(obs, dirs, pfb_flags=flags, form=1)
+    print(f"batch with FP16 PFB output, MFMA form {form}", flush=True)
+    bs = BatchSession(obs, dirs, pfb_flags=flags, form=form)
     try:
         bs.read(0)
         bs.run(use_graph=False)

@@ -44,17 +44,18 @@ def _rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
 
 
-def test_full_second_512_beams_mfma(pkg, orc, gpu_lib, full_obs):
+@pytest.mark.parametrize("form", [1, 3])
+def test_full_second_512_beams_mfma(pkg, orc, gpu_lib, full_obs, form):
     from vcsbeam_amd.apps import TiedArraySession
     from vcsbeam_amd.workloads import directions_in_cap, unit_vectors
     obs = full_obs
     az, za = directions_in_cap(512, 15.0)
     sess = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
-                            nstokes=4, form=pkg.VM_BF_MFMA_FP16)
+                            nstokes=4, form=form)
     try:
         sess.read(0)
         sess.pfb()
-        print("512 beams x 1 s, MFMA FP16", flush=True)
+        print(f"512 beams x 1 s, MFMA form {form}", flush=True)
         sess.geometry()
         sess.beamform()
         beams = [(0, 4), (508, 512)]

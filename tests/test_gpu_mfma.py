@@ -3,8 +3,9 @@
 Workload: 128 tiles x 2 pols, 0.1 s, PFB in full precision (VM_FLT), 40
 directions within 15 deg of boresight (an uneven count: 160 GEMM rows padded
 to 256).  The FP16 form (FP16 voltages with a per-channel power-of-two scale,
-split hi+lo FP16 weights, FP32 accumulation) must reach <= 1e-3 norm-wise
-relative error on Stokes I of every beam against the FP64 oracle chain.  BF16
+split hi+lo FP16 weights, FP32 accumulation) and the single-weight FP16 form
+(form 3: one FP16 part per weight) must reach <= 1e-3 norm-wise relative error
+on Stokes I of every beam against the FP64 oracle chain.  BF16
 (8-bit significand on the voltages) does not reach that bar: it measures
 2.4e-3 on every Stokes parameter and is held to 3.6e-3 (1.5x), so that a
 regression shows.
@@ -44,7 +45,7 @@ def _errors(orc, S, Sref):
                      for p in range(S.shape[0])])
 
 
-@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3), (3, 1e-3, 3e-3)])
 def test_mfma_matches_fp64_oracle(pkg, orc, gpu_lib, form, tol_i, tol_quv):
     from vcsbeam_amd.apps import tied_array_beam
     obs, az, za = _setup(pkg)
@@ -76,11 +77,12 @@ def mfma_case(pkg, orc):
     return obs, az, za, _oracle(orc, obs, az, za)
 
 
+@pytest.mark.parametrize("form", [1, 3])
 @pytest.mark.parametrize("variant", range(12))
-def test_every_beamformer_variant_and_tile_order(pkg, orc, gpu_lib, mfma_case, variant):
+def test_every_beamformer_variant_and_tile_order(pkg, orc, gpu_lib, mfma_case, variant, form):
     """Every GEMM variant (per-tile and persistent kernels, all tile shapes
     and epilogues) in every tile order gives the same Stokes as the FP64
-    oracle chain, to the FP16 bar."""
+    oracle chain, to the FP16 bar, with split and with single weights."""
     import ctypes as C
     from vcsbeam_amd.apps import TiedArraySession
     nv = gpu_lib.vmBeamformerVariantInfo(0, None, None, None)
@@ -90,13 +92,13 @@ def test_every_beamformer_variant_and_tile_order(pkg, orc, gpu_lib, mfma_case, v
     gpu_lib.vmBeamformerVariantInfo(variant, C.byref(rows), C.byref(samples), C.byref(waves))
     obs, az, za, Sref = mfma_case
     sess = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
-                            nstokes=4, form=pkg.VM_BF_MFMA_FP16)
+                            nstokes=4, form=form)
     try:
         sess.read(0)
         sess.pfb()
         for order in range(5):
-            print(f"variant {variant} ({rows.value}x{samples.value}, {waves.value} waves) "
-                  f"order {order}", flush=True)
+            print(f"form {form} variant {variant} ({rows.value}x{samples.value}, "
+                  f"{waves.value} waves) order {order}", flush=True)
             gpu_lib.vmSetBeamformerVariant(sess.vm, variant, order)
             sess.geometry()
             sess.beamform()
@@ -108,7 +110,7 @@ def test_every_beamformer_variant_and_tile_order(pkg, orc, gpu_lib, mfma_case, v
         sess.close()
 
 
-@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3), (3, 1e-3, 3e-3)])
 def test_mfma_per_pointing_primary_beam(pkg, orc, gpu_lib, form, tol_i, tol_quv):
     """Every pointing with its own primary beam B[p][c] (the analytic
     dipole-array beam, the same for all tiles), so J^-1 = F B^-1 D^-1
@@ -219,7 +221,7 @@ def test_mfma_tile_groups_every_variant_and_order(pkg, orc, gpu_lib):
 
 
 @pytest.mark.parametrize("path", ["device", "host"])
-@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3), (3, 1e-3, 3e-3)])
 def test_mfma_primary_beam_per_tile_configuration(pkg, orc, gpu_lib, form, tol_i, tol_quv, path):
     """A primary beam per tile from its dipole configuration (vmCalcB):
     three configurations over 128 tiles (all live; dipole 3 dead on tiles
@@ -266,7 +268,7 @@ def test_mfma_primary_beam_per_tile_configuration(pkg, orc, gpu_lib, form, tol_i
     assert err[:, 1:].max() <= tol_quv
 
 
-@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3), (3, 1e-3, 3e-3)])
 def test_mfma_legacy_int4_input(pkg, orc, gpu_lib, form, tol_i, tol_quv):
     """Legacy recombined 4+4-bit input ([s][c][i], one byte per
     input-sample): the prep pass decodes the nibbles itself (high = re,
@@ -295,7 +297,7 @@ def test_mfma_legacy_int4_input(pkg, orc, gpu_lib, form, tol_i, tol_quv):
     assert err[:, 1:].max() <= tol_quv
 
 
-@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3)])
+@pytest.mark.parametrize("form,tol_i,tol_quv", [(1, 1e-3, 3e-3), (2, 3.6e-3, 3.6e-3), (3, 1e-3, 3e-3)])
 def test_mfma_fp16_pfb_output(pkg, orc, gpu_lib, form, tol_i, tol_quv):
     """The forward PFB's FP16 output (PFB_COMPLEX_FLOAT16, VM_HALF) read by
     the precision form: against the FP64 oracle fed the exact PFB output, to
