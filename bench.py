@@ -346,12 +346,12 @@ def surrogate_stage(lib, nbeams=4096, nchan=128, steps=4000, batch=2048, nsample
         lib.vmSurrogateDestroy(s)
 
 
-def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_check=4):
+def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_check=4, form=3):
     """Batched uneven workload: nchan coarse channels x 1 s of ntiles tiles,
     per channel a seeded flagged-tile count ~ U{0..40} and beam count ~
     floor(exp U[0, ln 2049)) (log-uniform 1..2048) within 15 deg; forward
-    PFB to FP16 (PFB_COMPLEX_FLOAT16, the shared buffer) + FP16 MFMA
-    beamformer, Stokes IQUV, every channel in one persistent launch inside
+    PFB to FP16 (PFB_COMPLEX_FLOAT16, the shared buffer) + the MFMA
+    beamformer in `form` (FP16 voltages), Stokes IQUV, every channel in one persistent launch inside
     one replayed hipGraph; seconds generated on the device.  Per channel,
     beams_check beams x nsamp_check samples x 128 channels are checked
     against the FP64 oracle chain (oracle PFB of the same seeded voltages,
@@ -373,7 +373,7 @@ def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_c
     t0 = time.perf_counter()
     flags = (pkg.PFB_FULL_PRECISION & ~pkg.PFB_TYPE_MASK) | pkg.PFB_COMPLEX_FLOAT16 | \
         pkg.PFB_EMULATE_FPGA
-    bs = BatchSession(obs, dirs, pfb_flags=flags, nstokes=4, form=pkg.VM_BF_MFMA_FP16,
+    bs = BatchSession(obs, dirs, pfb_flags=flags, nstokes=4, form=form,
                       share_pfb_output=True, device_data=True)
     try:
         bs.read(0)
@@ -424,8 +424,9 @@ def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_c
                         "max_beam_rel_err_IQUV": [float(f"{x:.3e}") for x in err]})
         return {"workload": f"{nchan} coarse channels x 1 s, {ntiles} tiles x 2 pols, flagged "
                             f"tiles U{{0..40}} and beams log-uniform 1..2048 per channel, PFB to "
-                            f"FP16 + FP16 MFMA beamformer, Stokes IQUV, one persistent launch in "
-                            f"one hipGraph",
+                            f"FP16 + MFMA beamformer form {form} (FP16 voltages"
+                            f"{', single FP16 weights' if form == 3 else ''}), Stokes IQUV, one "
+                            f"persistent launch in one hipGraph",
                 "value": round(bsamp / (ms * 1e-3), 1), "unit": "beam-samples/s",
                 "beams_total": int(nbeam.sum()), "ms_per_second_of_data": round(ms, 2),
                 "stages_eager_ms": {k: round(x, 2) for k, x in st.items()},
@@ -701,7 +702,7 @@ def main():
     pkg = load_pkg()
     lib = pkg.load()
     if args.batch_only:
-        print(json.dumps({"batch": batch_stage(pkg, lib)}))
+        print(json.dumps({"batch": batch_stage(pkg, lib, form=args.form)}))
         return
     if args.pipeline_only:
         out = {"pipeline": pipeline_stage(pkg, lib), "aux": aux_stage(pkg, lib)}
@@ -901,7 +902,7 @@ def main():
     if not args.no_batch:
         print("[bench] batched uneven workload (24 channels x 256 tiles)", file=sys.stderr,
               flush=True)
-        result["batch"] = batch_stage(pkg, lib)
+        result["batch"] = batch_stage(pkg, lib, form=args.form)
     if not args.no_pipeline:
         print("[bench] plain pipeline with inverse PFB (24 channels x 128 tiles)",
               file=sys.stderr, flush=True)

@@ -505,13 +505,14 @@ __global__ void k_jcheck(const double2 *__restrict__ J, uint32_t per_pointing,
 // ------------------------------------------------------- voltage prep -----
 // One workgroup of 32 NG lanes per (32-sample block sb, channel c), one pass
 // over the block: lane (sl, g) loads sample 32 sb + sl of both inputs of
-// tiles g, g+NG, ... (APT tiles at most), forms u = J^-1 v with Jf = J^-1
-// (float, [c][a][4]), the auto-correlation sums N = sum_a (|u_x|^2, |u_y|^2,
-// Re u_x u_y^*, Im u_x u_y^*) and the block's largest |component| of u.  The
-// NG partial sums per sample are reduced in LDS and written as N[c][s]; u, scaled by
-// 2^es, is placed in LDS in fragment order and written out as B fragments
-// Bf[c][sb][ks][pol][lane] (uint4): lane l holds sample 32 sb + l%32 and
-// tiles 8 ks + 4 (l/32) + {0..3} as (re, im) pairs of polarisation pol.
+// its APT tiles (quads of consecutive tiles, prep_ant), forms u = J^-1 v with
+// Jf = J^-1 (float, [c][a][4]), the auto-correlation sums N = sum_a
+// (|u_x|^2, |u_y|^2, Re u_x u_y^*, Im u_x u_y^*) and the block's largest
+// |component| of u.  The NG partial sums per sample are reduced in LDS and
+// written as N[c][s]; u, scaled by 2^es, is written straight from registers
+// as B fragments Bf[c][sb][ks][pol][lane] (uint4): lane l holds sample
+// 32 sb + l%32 and tiles 8 ks + 4 (l/32) + {0..3} as (re, im) pairs of
+// polarisation pol, so each thread's quad is one 16-byte slot.
 // Samples >= ns are zero; tiles >= nant are zero.  sexp[c][sb] = es.
 // v: VM_FLT [c][i][s].  Needs nant <= NG APT.
 // PB 1 (per-pointing primary beam): Jf holds D^-1 and, instead of the four N
@@ -522,7 +523,8 @@ __global__ void k_jcheck(const double2 *__restrict__ J, uint32_t per_pointing,
 // into float4 [c][s][VM_MFMA_MAX_GROUPS][4].
 // VIN, the voltages' kind: PREP_VIN_F32 VM_FLT [c][i][s] complex float;
 // PREP_VIN_F16 VM_HALF, complex FP16 in 32-sample blocks [c][s/32][i][s%32]
-// (a workgroup's block of every input is one contiguous run); PREP_VIN_I4 legacy
+// (a workgroup's block of every input is one contiguous run, staged in LDS:
+// read directly in VCS order, its 128-byte runs are slower); PREP_VIN_I4 legacy
 // 4+4-bit [s][c][i] (one byte per input-sample, high nibble re, low nibble
 // im, two's complement; include/vcsbeam.h.in:997-1003), whose block's 32
 // rows of ni bytes are staged in LDS with coalesced loads and each lane
@@ -538,6 +540,15 @@ __device__ __forceinline__ cf nibbles_to_cf(uint32_t b)
 
 #define PREP_I4_MAX_INPUTS 512
 #define PREP_F16_MAX_INPUTS 512
+// Antenna j of thread group g: groups own quads of consecutive antennas
+// (quad g + NG (j / 4), antenna j % 4 in it), so one thread holds the four
+// tiles of one 16-byte B-fragment slot and writes it with one store.
+template <int NG>
+__device__ __forceinline__ int prep_ant(int g, int j)
+{
+    return 4 * (g + NG * (j >> 2)) + (j & 3);
+}
+
 template <bool BF16, int APT, int NG, int PB, int VIN>
 __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv, int ns, int ni, int nsb,
                                                  int nkstep, const cf *__restrict__ Jf,
@@ -547,36 +558,30 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
                                                  uint4 *__restrict__ Bf, int *__restrict__ sexp,
                                                  float4 *__restrict__ N, grp_bounds gb)
 {
+    static_assert(APT % 4 == 0, "antennas per thread come in quads");
     constexpr int NGRP = PB == 2 ? VM_MFMA_MAX_GROUPS : 1;
     constexpr bool I4 = VIN == PREP_VIN_I4;
     constexpr bool H16 = VIN == PREP_VIN_F16;
     __shared__ uint32_t vrow[I4 ? SB * PREP_I4_MAX_INPUTS / 4 : 1];
     const cf *v = (const cf *)vv;
     typedef mtype<BF16> M;
-    constexpr int NT = 32 * NG; // lanes
     __shared__ float4 red[NG][SB];
     __shared__ float wmx[NG / 2];
-    // nkstep x 2 pols x 64 lanes of fragments; with H16 first the block's
-    // staged FP16 input (SB x ni dwords), consumed before the fragments
-    constexpr int NFRAG = (NG * APT / 8) * 2 * 64;
-    constexpr int NSTG = H16 ? SB * PREP_F16_MAX_INPUTS / 4 : 0;
-    __shared__ uint4 frag[NFRAG > NSTG ? NFRAG : NSTG];
     const int sb = blockIdx.x, c = blockIdx.y;
     const int t = threadIdx.x, sl = t & 31, g = t >> 5;
     const int s = SB * sb + sl;
     const bool ok = s < ns;
     const cf *vc = v + (size_t)c * ni * ns;
-    const int nfrag = nkstep * 2 * 64;
-
-    if constexpr (!H16)
-        for (int k = t; k < nfrag; k += NT) frag[k] = make_uint4(0u, 0u, 0u, 0u);
+    constexpr int NT = 32 * NG; // lanes
+    // H16: this block of every input, [c][s/32][i][s%32] complex FP16, is one
+    // contiguous run of SB ni dwords, staged in LDS with 16-byte loads (the
+    // inputs are then read in VCS order from there; samples past ns are the
+    // padding of the last block and are never used)
+    __shared__ uint4 stg16[H16 ? SB * PREP_F16_MAX_INPUTS / 4 : 1];
+    const uint32_t *vh = (const uint32_t *)stg16 + sl;
     if constexpr (H16) {
-        // this block of every input, [c][s/32][i][s%32]: one contiguous run
-        // of SB ni dwords, staged with 16-byte loads (samples past ns are the
-        // padding of the last block and are never used)
-        const uint4 *src = (const uint4 *)((const uint32_t *)vv +
-                                           ((size_t)c * ((ns + SB - 1) / SB) + sb) * ni * SB);
-        for (int k = t; k < SB * ni / 4; k += NT) frag[k] = src[k];
+        const uint4 *src = (const uint4 *)((const uint32_t *)vv + ((size_t)c * nsb + sb) * ni * SB);
+        for (int k = t; k < SB * ni / 4; k += NT) stg16[k] = src[k];
         __syncthreads();
     }
     if constexpr (I4) {
@@ -603,7 +608,7 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
     cf ux[APT], uy[APT];
 #pragma unroll
     for (int j = 0; j < APT; j++) {
-        const int a = g + NG * j;
+        const int a = prep_ant<NG>(g, j);
         cf q = {0.f, 0.f}, p = {0.f, 0.f};
         if (ok && a < nant) {
             if constexpr (I4) {
@@ -611,7 +616,6 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
                 q = nibbles_to_cf(lb[polQ[a]]);
                 p = nibbles_to_cf(lb[polP[a]]);
             } else if constexpr (H16) {
-                const uint32_t *vh = (const uint32_t *)frag + sl; // the staged block
                 const uint32_t hq = vh[polQ[a] * SB], hp = vh[polP[a] * SB];
                 q = cf{(float)__builtin_bit_cast(_Float16, (uint16_t)(hq & 0xffffu)),
                        (float)__builtin_bit_cast(_Float16, (uint16_t)(hq >> 16))};
@@ -632,10 +636,6 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
             uy[j].y = w10.x * q.y + w10.y * q.x + w11.x * p.y + w11.y * p.x;
         }
     }
-    if constexpr (H16) {
-        __syncthreads(); // every lane has read its inputs from the staged block
-        for (int k = t; k < nfrag; k += NT) frag[k] = make_uint4(0u, 0u, 0u, 0u);
-    }
     float nxx = 0.f, nyy = 0.f, xyr = 0.f, xyi = 0.f, mx = 0.f;
     float4 tk[NGRP][4];
 #pragma unroll
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
         const float rxx = x.x * x.x + x.y * x.y, ryy = y.x * y.x + y.y * y.y;
         const float rxr = x.x * y.x + x.y * y.y, rxi = x.y * y.x - x.x * y.y;
         if constexpr (PB != 0) {
-            const int a = g + NG * j;
+            const int a = prep_ant<NG>(g, j);
             if (a < nant) {
                 const float4 ga = gw[(size_t)c * nant + a];
                 const float gk[4] = {ga.x, ga.y, ga.z, ga.w};
@@ -718,21 +718,25 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
     if (t == 0) sexp[(size_t)c * nsb + sb] = es;
     const float scale = ldexpf(1.0f, es);
 
-    // tile a, polarisation P, sample sl -> dword (a%4) of lane sl + 32 ((a%8)/4)
-    // of fragment (ks = a/8, P)
-    uint32_t *fd = (uint32_t *)frag;
+    // quad Q = tiles 4Q..4Q+3 of polarisation P at sample sl is the 16-byte
+    // slot of lane sl + 32 (Q % 2) in fragment (ks = Q / 2, P); quads past the
+    // last k-step are not written (their tiles are past nant and zero)
+    uint4 *dst = Bf + ((size_t)c * nsb + sb) * nkstep * 128 + sl;
 #pragma unroll
-    for (int j = 0; j < APT; j++) {
-        const int a = g + NG * j;
-        if (a >= nant) break;
-        const int base = ((a >> 3) * 2) * 256 + (sl + 32 * ((a >> 2) & 1)) * 4 + (a & 3);
-        fd[base] = (uint32_t)M::bits(ux[j].x * scale) | ((uint32_t)M::bits(ux[j].y * scale) << 16);
-        fd[base + 256] =
-            (uint32_t)M::bits(uy[j].x * scale) | ((uint32_t)M::bits(uy[j].y * scale) << 16);
+    for (int j4 = 0; j4 < APT / 4; j4++) {
+        const int Q = g + NG * j4;
+        if ((Q >> 1) >= nkstep) break;
+        uint32_t wx[4], wy[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const cf x = ux[4 * j4 + k], y = uy[4 * j4 + k];
+            wx[k] = (uint32_t)M::bits(x.x * scale) | ((uint32_t)M::bits(x.y * scale) << 16);
+            wy[k] = (uint32_t)M::bits(y.x * scale) | ((uint32_t)M::bits(y.y * scale) << 16);
+        }
+        uint4 *f = dst + (Q >> 1) * 128 + 32 * (Q & 1);
+        f[0] = make_uint4(wx[0], wx[1], wx[2], wx[3]);
+        f[64] = make_uint4(wy[0], wy[1], wy[2], wy[3]);
     }
-    __syncthreads();
-    uint4 *dst = Bf + ((size_t)c * nsb + sb) * nfrag;
-    for (int k = t; k < nfrag; k += NT) dst[k] = frag[k];
 }
 
 // ------------------------------------------------------------- GEMM -------

@@ -95,7 +95,8 @@ def test_full_second_512_beams_mfma(pkg, orc, gpu_lib, full_obs, form):
         sess.close()
 
 
-def test_full_batched_workload(pkg, orc, gpu_lib):
+@pytest.mark.parametrize("form,half", [(1, False), (3, True)])
+def test_full_batched_workload(pkg, orc, gpu_lib, form, half):
     from vcsbeam_amd.apps import BatchSession
     from vcsbeam_amd.workloads import (SEED_BEAMS, SEED_FLAGS, Observation, directions_in_cap,
                                        uniform, unit_vectors)
@@ -109,9 +110,12 @@ def test_full_batched_workload(pkg, orc, gpu_lib):
            for g in range(nchan)]
     dirs = [directions_in_cap(int(nbeam[g]), 15.0, seed=SEED_BEAMS + 100 * (g + 1))
             for g in range(nchan)]
-    print(f"batch: {nchan} channels x 1 s, {ntiles} tiles, {int(nbeam.sum())} beams", flush=True)
-    bs = BatchSession(obs, dirs, pfb_flags=pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA,
-                      nstokes=4, form=pkg.VM_BF_MFMA_FP16, share_pfb_output=True,
+    print(f"batch: {nchan} channels x 1 s, {ntiles} tiles, {int(nbeam.sum())} beams, MFMA form "
+          f"{form}, {'FP16' if half else 'FP32'} PFB output", flush=True)
+    flags = pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA
+    if half:
+        flags = (flags & ~pkg.PFB_TYPE_MASK) | pkg.PFB_COMPLEX_FLOAT16
+    bs = BatchSession(obs, dirs, pfb_flags=flags, nstokes=4, form=form, share_pfb_output=True,
                       device_data=True)
     try:
         bs.read(0)
@@ -121,9 +125,19 @@ def test_full_batched_workload(pkg, orc, gpu_lib):
         for g, ses in enumerate(bs.sessions):
             o, (az, za) = obs[g], dirs[g]
             ses.pfb()  # the shared PFB output buffer holds the last channel's
-            v = np.empty((128, o.ninput, ns), np.complex64)
-            lib.vmMemcpy2DDtoH(v.ctypes.data, ns * 8, ses.v.d_v, nspc * 8, ns * 8,
-                               128 * o.ninput)
+            if half:
+                # VM_HALF [c][s/32][i][s%32][re, im]: block 0 of each channel is
+                # one row of ninput x 32 complex halves
+                assert ns == 32
+                nsb = (nspc + 31) // 32
+                h = np.empty((128, o.ninput, 32, 2), np.float16)
+                lib.vmMemcpy2DDtoH(h.ctypes.data, o.ninput * 128, ses.v.d_v,
+                                   nsb * o.ninput * 128, o.ninput * 128, 128)
+                v = h.astype(np.float32).view(np.complex64)[..., 0]
+            else:
+                v = np.empty((128, o.ninput, ns), np.complex64)
+                lib.vmMemcpy2DDtoH(v.ctypes.data, ns * 8, ses.v.d_v, nspc * 8, ns * 8,
+                                   128 * o.ninput)
             npt = len(az)
             sel = sorted(set([0, 1, npt - 2, npt - 1]) & set(range(npt)))
             B = np.zeros((len(sel), o.ntiles, 2, 2), complex)
