@@ -346,12 +346,13 @@ def surrogate_stage(lib, nbeams=4096, nchan=128, steps=4000, batch=2048, nsample
         lib.vmSurrogateDestroy(s)
 
 
-def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_check=4, form=3):
+def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_check=4, form=1,
+                half=False):
     """Batched uneven workload: nchan coarse channels x 1 s of ntiles tiles,
     per channel a seeded flagged-tile count ~ U{0..40} and beam count ~
     floor(exp U[0, ln 2049)) (log-uniform 1..2048) within 15 deg; forward
-    PFB to FP16 (PFB_COMPLEX_FLOAT16, the shared buffer) + the MFMA
-    beamformer in `form` (FP16 voltages), Stokes IQUV, every channel in one persistent launch inside
+    PFB to FP32 (or to FP16, PFB_COMPLEX_FLOAT16, with `half`) into the
+    shared buffer + the MFMA beamformer in `form` (FP16 voltages), Stokes IQUV, every channel in one persistent launch inside
     one replayed hipGraph; seconds generated on the device.  Per channel,
     beams_check beams x nsamp_check samples x 128 channels are checked
     against the FP64 oracle chain (oracle PFB of the same seeded voltages,
@@ -371,8 +372,9 @@ def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_c
     dirs = [directions_in_cap(int(nbeam[g]), 15.0, seed=SEED_BEAMS + 100 * (g + 1))
             for g in range(nchan)]
     t0 = time.perf_counter()
-    flags = (pkg.PFB_FULL_PRECISION & ~pkg.PFB_TYPE_MASK) | pkg.PFB_COMPLEX_FLOAT16 | \
-        pkg.PFB_EMULATE_FPGA
+    flags = pkg.PFB_FULL_PRECISION | pkg.PFB_EMULATE_FPGA
+    if half:
+        flags = (flags & ~pkg.PFB_TYPE_MASK) | pkg.PFB_COMPLEX_FLOAT16
     bs = BatchSession(obs, dirs, pfb_flags=flags, nstokes=4, form=form,
                       share_pfb_output=True, device_data=True)
     try:
@@ -393,8 +395,9 @@ def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_c
         nspc = obs[0].nspectra
         bsamp = int(nbeam.sum()) * 128 * nspc
         flops = sum(16.0 * nact[g] * nbeam[g] * 128 * nspc for g in range(nchan))
+        wbytes = 2 if form == 3 else 4  # per weight: one FP16 part, or hi + lo
         gemm_bytes = sum(nbeam[g] * 128 * nspc * 16 + 128 * nact[g] * 2 * nspc * 4
-                         + 128 * (2 * nbeam[g]) * (2 * nact[g]) * 4 + 128 * nspc * 16
+                         + 128 * (2 * nbeam[g]) * (2 * nact[g]) * wbytes + 128 * nspc * 16
                          for g in range(nchan))
         print("[bench] batch: per-channel check against the FP64 oracle", file=sys.stderr,
               flush=True)
@@ -422,18 +425,19 @@ def batch_stage(pkg, lib, nchan=24, ntiles=256, steps=3, nsamp_check=64, beams_c
             per.append({"beams": int(nbeam[g]), "flagged_tiles": int(nflag[g]),
                         "gemm_tiles": nact[g],
                         "max_beam_rel_err_IQUV": [float(f"{x:.3e}") for x in err]})
+        wname = {1: "split hi+lo FP16 weights", 2: "BF16 voltages, split BF16 weights",
+                 3: "one FP16 part per weight"}[form]
         return {"workload": f"{nchan} coarse channels x 1 s, {ntiles} tiles x 2 pols, flagged "
                             f"tiles U{{0..40}} and beams log-uniform 1..2048 per channel, PFB to "
-                            f"FP16 + MFMA beamformer form {form} (FP16 voltages"
-                            f"{', single FP16 weights' if form == 3 else ''}), Stokes IQUV, one "
-                            f"persistent launch in one hipGraph",
+                            f"{'FP16' if half else 'FP32'} + MFMA beamformer form {form} "
+                            f"({wname}), Stokes IQUV, one persistent launch in one hipGraph",
                 "value": round(bsamp / (ms * 1e-3), 1), "unit": "beam-samples/s",
                 "beams_total": int(nbeam.sum()), "ms_per_second_of_data": round(ms, 2),
                 "stages_eager_ms": {k: round(x, 2) for k, x in st.items()},
                 "gemm_tflops_algorithmic": round(flops / (st["gemm_ms"] * 1e-3) / 1e12, 1),
                 "gemm_GBps_algorithmic": round(gemm_bytes / (st["gemm_ms"] * 1e-3) / 1e9, 1),
-                # 2 B in, 4 B (complex FP16) out per input-sample
-                "pfb_hbm_frac": round(nchan * 2 * ntiles * nspc * 128 * 6
+                # 2 B in, 8 B (complex FP32) or 4 B (complex FP16) out per input-sample
+                "pfb_hbm_frac": round(nchan * 2 * ntiles * nspc * 128 * (6 if half else 10)
                                       / (st["pfb_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "schedule": {**info, "tuned_gemm_ms": round(tune_ms, 2)},
                 "setup_s": round(t_setup, 1),
@@ -675,7 +679,7 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="fix the MFMA beamformer variant instead of autotuning")
     ap.add_argument("--order", type=int, default=1, help="tile order with --variant")
-    ap.add_argument("--form", type=int, default=3, choices=(1, 2, 3),
+    ap.add_argument("--form", type=int, default=1, choices=(1, 2, 3),
                     help="MFMA form of the headline: 1 FP16 voltages with split hi+lo FP16 "
                          "weights, 2 BF16 split, 3 FP16 with single FP16 weights")
     ap.add_argument("--no-side-form", action="store_true",
@@ -702,7 +706,9 @@ def main():
     pkg = load_pkg()
     lib = pkg.load()
     if args.batch_only:
-        print(json.dumps({"batch": batch_stage(pkg, lib, form=args.form)}))
+        print(json.dumps({"batch": batch_stage(pkg, lib),
+                          "batch_fp16_pfb_single_weights": batch_stage(pkg, lib, form=3,
+                                                                       half=True)}))
         return
     if args.pipeline_only:
         out = {"pipeline": pipeline_stage(pkg, lib), "aux": aux_stage(pkg, lib)}
@@ -902,7 +908,10 @@ def main():
     if not args.no_batch:
         print("[bench] batched uneven workload (24 channels x 256 tiles)", file=sys.stderr,
               flush=True)
-        result["batch"] = batch_stage(pkg, lib, form=args.form)
+        result["batch"] = batch_stage(pkg, lib)
+        print("[bench] batched workload with FP16 PFB output and single weights",
+              file=sys.stderr, flush=True)
+        result["batch_fp16_pfb_single_weights"] = batch_stage(pkg, lib, form=3, half=True)
     if not args.no_pipeline:
         print("[bench] plain pipeline with inverse PFB (24 channels x 128 tiles)",
               file=sys.stderr, flush=True)

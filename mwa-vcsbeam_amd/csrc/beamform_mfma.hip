@@ -1627,12 +1627,14 @@ static void launch_prep(bool bf16, hipStream_t st, const void *v, int ns, int ni
     if (pbm == 2) { if (bf16) VM_PREP(true, A_, G_, 2); else VM_PREP(false, A_, G_, 2); }     \
     else if (pbm == 1) { if (bf16) VM_PREP(true, A_, G_, 1); else VM_PREP(false, A_, G_, 1); } \
     else { if (bf16) VM_PREP(true, A_, G_, 0); else VM_PREP(false, A_, G_, 0); }
+    // 8 antennas per thread; more antennas take more lanes, not more
+    // registers per lane (occupancy hides the loads' latency)
     if (nant <= 64) {
         VM_PREP2(8, 8);
     } else if (nant <= 128) {
         VM_PREP2(8, 16);
     } else {
-        VM_PREP2(16, 16);
+        VM_PREP2(8, 32);
     }
 #undef VM_PREP2
 #undef VM_PREP
