@@ -177,3 +177,20 @@ def test_vdif_write_data_files(pkg, tmp_path):
     assert float(kv["SEC_OFFSET"]) == 2.0 and float(kv["FREQ"]) == 139.52
     assert float(kv["BW"]) == 1.28 and abs(float(kv["TSAMP"]) - 1 / 1.28) < 1e-6
     assert (kv["NBIT"], kv["NDIM"], kv["NPOL"]) == ("8", "2", "2")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbytes", [16, 4096 * 16 + 48, 1 << 24])
+def test_device_copy_kernel(pkg, gpu_lib, nbytes):
+    """vmDeviceCopy (the plain copy kernel the bench's copy rate and the
+    memory-counter calibration use) copies every 16-byte element, including
+    sizes that do not fill its last workgroup, and nothing past the end."""
+    rng = np.random.default_rng(11)
+    src = rng.integers(0, 2 ** 32, size=nbytes // 4, dtype=np.uint32)
+    a = pkg.DeviceArray.from_numpy(src)
+    b = pkg.DeviceArray(nbytes + 64)
+    gpu_lib.vmMemsetDevice(b.ptr, 0xAB, nbytes + 64)
+    gpu_lib.vmDeviceCopy(b.ptr, a.ptr, nbytes)
+    out = b.to_numpy(np.uint8, (nbytes + 64,))
+    np.testing.assert_array_equal(out[:nbytes].view(np.uint32), src)
+    assert (out[nbytes:] == 0xAB).all()
