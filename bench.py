@@ -820,6 +820,11 @@ def main():
                 "mfma_frac_issued": round(issued * mfma_tf / FP16_PEAK_TFLOPS, 4),
                 "mfma_measured_peak_tflops": round(mfma_peak_tf, 1),
                 "mfma_frac_of_measured_issued": round(issued * mfma_tf / mfma_peak_tf, 4),
+                # the kernel's bytes are 88 % writes: beside the spec peak, the
+                # write and copy rates measured on this device in this run
+                "measured_write_GBps": round(write_GBps, 1),
+                "frac_of_measured_write": round(achieved / write_GBps, 4),
+                "measured_copy_GBps": round(copy_GBps, 1),
                 "kernel": f"k_beamform_mfma<{form_name[args.form]}>",
                 "kernel_ms": round(gemm_ms, 4),
                 "launches": gemm_n, "autotuned_variant": variant, "tile_order": head["order"],
