@@ -959,8 +959,15 @@ void vmSetFilter(vcsbeam_context *vm, const double *coeffs, int ncoeffs,
 void vmCalcPhiDevice(vcsbeam_context *vm, beam_geom *beam_geom_vals);
 /* Device FP64: J^-1 = F (D B)^-1 from d_D and pb.B; result in d_J.
  * d_J is written by this call, vmCalcJDeviceChanB or vmPushJ only: the MFMA
- * form learns from them whether J^-1 can differ between pointings. */
+ * form learns from them whether J^-1 can differ between pointings.  When
+ * every pointing has the same beam, J^-1 is the same for every pointing and
+ * only pointing 0's is written; the library writes the others before it
+ * reads J^-1 per pointing (the FP32 form), and a caller that reads d_J
+ * itself calls vmExpandJDevice first. */
 void vmCalcJDevice(vcsbeam_context *vm);
+/* Extension (no counterpart in the reference): every pointing's J^-1 in
+ * d_J after vmCalcJDevice shared pointing 0's; synchronous. */
+void vmExpandJDevice(vcsbeam_context *vm);
 /* Upload vm->D to d_D. */
 void vmPushD(vcsbeam_context *vm);
 /* Device FP64 J^-1 = F (D B)^-1 with a per-fine-channel primary beam shared

@@ -302,6 +302,7 @@ _SIGS = {
     "vmCalcPhi": (None, [_CTX, C.POINTER(BeamGeom)]),
     "vmCalcPhiDevice": (None, [_CTX, C.POINTER(BeamGeom)]),
     "vmCalcJDevice": (None, [_CTX]),
+    "vmExpandJDevice": (None, [_CTX]),
     "vmCalcJDeviceChanB": (None, [_CTX, C.c_void_p]),
     "vmSurrogateCreate": (C.c_void_p, [C.c_uint64, C.c_double, C.c_double]),
     "vmSurrogateDestroy": (None, [C.c_void_p]),

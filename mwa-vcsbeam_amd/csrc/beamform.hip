@@ -592,6 +592,7 @@ void vm_build_weights(vcsbeam_context *vm)
         HIPCHECK(hipMalloc(&d->d_W, need));
         d->W_bytes = need;
     }
+    vm_expand_J(vm); // the weights read J^-1 per pointing
     size_t total = (size_t)vm->npointing * nant * nchan;
     hipLaunchKernelGGL(k_build_weights, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                        vm_stream(vm), (const double2 *)vm->d_J, (const double2 *)vm->gdelays.d_phi,

@@ -1685,6 +1685,7 @@ static int pointings_differ(vcsbeam_context *vm, hipStream_t st, uint32_t *flag)
 {
     vm_device_state *d = vm_dev(vm);
     const int np = (int)vm->npointing;
+    if (d->J_shared) return 0; // one J^-1 for every pointing (vmCalcJDevice)
     // d_B must hold the current pointings (a changed pointing count without
     // a new J^-1 falls back to comparing J^-1 itself)
     if (!d->J_fromB || !d->d_B || d->B_np != np) return j_differs(vm, st, flag);

@@ -182,6 +182,46 @@ void vmCalcPhiDevice(vcsbeam_context *vm, beam_geom *bg)
     d->weights_gen++;
 }
 
+// *flag |= 1 where block p (p = 1 + blockIdx.y) of `per` values differs
+// from block 0 (an exact comparison; NaN counts as differing)
+__global__ void k_differs_from_first(const double2 *__restrict__ x, uint32_t per,
+                                     uint32_t *__restrict__ flag)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool dif = false;
+    if (j < per) {
+        const double2 a = x[j], b = x[(size_t)(blockIdx.y + 1) * per + j];
+        dif = !(a.x == b.x && a.y == b.y);
+    }
+    if (__any(dif) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+// blocks 1..n-1 of `per` values = block 0
+__global__ void k_copy_first_block(double2 *__restrict__ x, uint32_t per)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < per) x[(size_t)(blockIdx.y + 1) * per + j] = x[j];
+}
+
+void vm_expand_J(vcsbeam_context *vm)
+{
+    vm_device_state *d = vm_dev(vm);
+    if (!d->J_shared) return;
+    const int np = (int)vm->npointing;
+    const uint32_t per = (uint32_t)vm->obs_metadata->num_ants * vm->nfine_chan * 4;
+    if (np > 1)
+        hipLaunchKernelGGL(k_copy_first_block, dim3((per + 255) / 256, (unsigned)(np - 1)),
+                           dim3(256), 0, (hipStream_t)d->stream, (double2 *)vm->d_J, per);
+    HIPCHECK(hipGetLastError());
+    d->J_shared = 0;
+}
+
+void vmExpandJDevice(vcsbeam_context *vm)
+{
+    vm_expand_J(vm);
+    HIPCHECK(hipStreamSynchronize((hipStream_t)vm_dev(vm)->stream));
+}
+
 void vmCalcJDevice(vcsbeam_context *vm)
 {
     vm_device_state *d = vm_dev(vm);
@@ -190,9 +230,25 @@ void vmCalcJDevice(vcsbeam_context *vm)
     size_t bbytes = (size_t)np * nant * 4 * sizeof(gpuDoubleComplex);
     ensure(&d->d_B, &d->B_bytes, bbytes);
     HIPCHECK(hipMemcpyAsync(d->d_B, vm->pb.B, bbytes, hipMemcpyHostToDevice, s));
+    // every pointing with the same beam (B = I, or one beam for all) gives
+    // every pointing the same J^-1: compute pointing 0's only and share it
+    // (vm_expand_J writes the others when something reads them)
+    uint32_t differs = 1;
+    if (np > 1) {
+        ensure(&d->d_flag, &d->flag_bytes, 256);
+        HIPCHECK(hipMemsetAsync(d->d_flag, 0, 4, s));
+        const uint32_t per = (uint32_t)nant * 4;
+        hipLaunchKernelGGL(k_differs_from_first, dim3((per + 255) / 256, (unsigned)(np - 1)),
+                           dim3(256), 0, s, (const double2 *)d->d_B, per, (uint32_t *)d->d_flag);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(&differs, d->d_flag, 4, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+    const int shared = np > 1 && !differs;
     vm_timer_start(d, VM_T_GEOM);
-    vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 0, vm->d_J, s);
+    vm_calc_J_launch(vm->d_D, d->d_B, shared ? 1 : np, nant, nchan, 0, vm->d_J, s);
     vm_timer_stop(d, VM_T_GEOM);
+    d->J_shared = shared;
     d->B_chan = 0;
     d->J_fromB = 1;
     d->B_np = np;
@@ -209,6 +265,7 @@ void vmCalcJDeviceChanB(vcsbeam_context *vm, const gpuDoubleComplex *B)
     ensure(&d->d_B, &d->B_bytes, bbytes);
     HIPCHECK(hipMemcpyAsync(d->d_B, B, bbytes, hipMemcpyHostToDevice, s));
     vm_calc_J_launch(vm->d_D, d->d_B, np, nant, nchan, 1, vm->d_J, s);
+    d->J_shared = 0;
     d->B_chan = 1;
     d->J_fromB = 1;
     d->B_np = np;

@@ -257,6 +257,7 @@ void vmPushJ(vcsbeam_context *vm)
     hipStream_t s = (hipStream_t)d->stream;
     HIPCHECK(hipMemcpyAsync(vm->d_J, vm->J, vm->J_size_bytes, hipMemcpyHostToDevice, s));
     d->J_fromB = 0;
+    d->J_shared = 0;
     if (d->J_host_fromB && vm->pb.B && vm->D && vm->d_D &&
         vm->pb.npointings == vm->npointing && vm->pb.nant == vm->obs_metadata->num_ants) {
         /* J^-1 came from vmCalcJ, i.e. from D and the primary beam pb.B:

@@ -77,6 +77,10 @@ struct vm_device_state {
        1: [B_np][c][4]). */
     int J_fromB;
     int B_np;
+    /* d_J holds pointing 0's J^-1 only, which every pointing shares (every
+       pointing had the same beam when vmCalcJDevice computed it); expanded
+       to all pointings by vm_expand_J before anything reads d_J per pointing */
+    int J_shared;
     /* J^-1 on the host was last computed by vmCalcJ from D and pb.B (set
        there, taken by the next vmPushJ, which then also hands D and B to the
        device) */
@@ -141,6 +145,9 @@ vm_device_state *vm_dev_global(void);
 /* Host helpers implemented in vm_host.c */
 vm_device_state *vm_dev(vcsbeam_context *vm); /* create on first use */
 void vm_dev_free(vcsbeam_context *vm);
+/* d_J for every pointing (expands a shared J^-1, see J_shared); enqueued on
+   the context stream */
+void vm_expand_J(vcsbeam_context *vm);
 
 /* Pinned host memory (hipHostMalloc), implemented in vm_device.hip */
 void *vm_host_alloc_pinned(size_t bytes);

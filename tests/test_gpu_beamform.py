@@ -139,3 +139,59 @@ def test_send_S_to_fits_matches_oracle_renormalisation(pkg, orc, gpu_lib):
     np.testing.assert_array_equal(q[valid], rq[valid])
     lib.vmDestroyStatistics(vm)
     lib.destroy_vcsbeam_context(vm)
+
+
+def test_device_jones_shared_between_pointings(pkg, orc, gpu_lib):
+    """vmCalcJDevice with the same beam for every pointing writes pointing 0's
+    J^-1 only: it equals the FP64 oracle's, vmExpandJDevice gives every
+    pointing the same, and the FP32 form (which expands it itself) still
+    matches the oracle.  A beam that differs for one pointing gives J^-1 per
+    pointing."""
+    import ctypes as C
+    from vcsbeam_amd.apps import TiedArraySession
+    from vcsbeam_amd.workloads import directions_in_cap
+    obs = _obs(ntiles=16)
+    az, za = directions_in_cap(6, 15.0, centre_za_deg=20.0)
+    ses = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, nstokes=4)
+    try:
+        v = ses.v
+        npt, nant, nch = len(az), obs.ntiles, 128
+        D = obs.calibration()
+
+        def device_J():
+            J = np.empty((npt, nant, nch, 2, 2), np.complex128)
+            gpu_lib.vmMemcpyDtoH(J.ctypes.data, v.d_J, J.nbytes)
+            return J
+
+        ses.read(0)
+        ses.pfb()
+        ses.geometry()  # B = I for every pointing
+        B = np.zeros((npt, nant, 2, 2), complex)
+        B[..., 0, 0] = B[..., 1, 1] = 1
+        Jref = orc.calc_J(D, B)
+        J = device_J()
+        assert np.abs(J[0] - Jref[0]).max() <= 1e-12 * np.abs(Jref[0]).max()
+        gpu_lib.vmExpandJDevice(ses.vm)
+        J = device_J()
+        assert np.abs(J - Jref).max() <= 1e-12 * np.abs(Jref).max()
+        # the FP32 form reads J^-1 per pointing
+        ses.geometry()
+        ses.beamform()
+        S = ses.stokes()
+        vv = orc.pfb_forward(_htr(obs, 0), obs.ninput, obs.nspectra, obs.filter.astype(np.int32),
+                             obs.out_idx(), fpga=False)
+        Sref, _ = _oracle_tab(orc, obs, vv, az, za)
+        for p in range(npt):
+            assert orc.rel_err(S[p, :, 0, :], Sref[p, :, 0, :]) <= 1e-5
+        # a beam that differs for pointing 2: J^-1 per pointing
+        Bh = np.ctypeslib.as_array(C.cast(v.pb.B, C.POINTER(C.c_double)),
+                                   shape=(npt, nant, 4, 2))
+        Bh[2, :, 0, 0] = 1.5
+        B[2, :, 0, 0] = 1.5
+        gpu_lib.vmCalcJDevice(ses.vm)
+        Jref = orc.calc_J(D, B)
+        J = device_J()
+        assert np.abs(J - Jref).max() <= 1e-12 * np.abs(Jref).max()
+        assert np.abs(J[2] - J[0]).max() > 1e-3 * np.abs(J[0]).max()
+    finally:
+        ses.close()
