@@ -794,7 +794,11 @@ def main():
     lib.vmBeamformerVariantInfo(variant, C.byref(vr), C.byref(vs), C.byref(vw))
     value = beam_samples / (ms_step * 1e-3)
     S_mfma = head["S"]
-    # Dominant kernel k_beamform_mfma, per launch (one channel-second, 512 beams):
+
+    def gemm_kernel(v):
+        return lib.vmBeamformerVariantKernel(v).decode()
+    # Dominant kernel (the GEMM variant's kernel, k_beamform_mfma or
+    # k_beamform_res), per launch (one channel-second, 512 beams):
     # algorithmic bytes = Stokes IQUV out (4 B x 4 per beam-sample) + J^-1-applied
     # voltages in (FP16 complex, 4 B per tile-pol-sample) + phase weights
     # (2 B per beam-row x k' per weight part: hi + lo, or one) + N terms in
@@ -813,7 +817,7 @@ def main():
     issued = parts[args.form]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": committed_traffic("k_beamform_mfma", head["sel_variant"],
+                "traffic": committed_traffic(gemm_kernel(variant), head["sel_variant"],
                                              head["order"], args.form),
                 "algorithmic_bytes": int(gemm_bytes), "algorithmic_flops": gemm_flops,
                 "mfma_tflops": round(mfma_tf, 1),
@@ -825,7 +829,7 @@ def main():
                 "measured_write_GBps": round(write_GBps, 1),
                 "frac_of_measured_write": round(achieved / write_GBps, 4),
                 "measured_copy_GBps": round(copy_GBps, 1),
-                "kernel": f"k_beamform_mfma<{form_name[args.form]}>",
+                "kernel": f"{gemm_kernel(variant)}<{form_name[args.form]}>",
                 "kernel_ms": round(gemm_ms, 4),
                 "launches": gemm_n, "autotuned_variant": variant, "tile_order": head["order"],
                 "workgroup_tile": f"{vr.value} rows x {vs.value} samples, {vw.value} waves",
@@ -863,8 +867,9 @@ def main():
             "ms_per_step": round(side["ms_step"], 4), "kernel_ms": round(side["gemm_ms"], 4),
             "hbm_frac": round(sb / (side["gemm_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "autotuned_variant": side["variant"], "tile_order": side["order"],
-            "traffic": committed_traffic("k_beamform_mfma", side["sel_variant"], side["order"],
-                                         side_form)}
+            "kernel": gemm_kernel(side["variant"]),
+            "traffic": committed_traffic(gemm_kernel(side["variant"]), side["sel_variant"],
+                                         side["order"], side_form)}
         S_side = side["S"]
     S_fp32 = None
     if not args.no_fp32:

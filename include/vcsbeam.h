@@ -1031,6 +1031,10 @@ int vmBeamformerSelection(vcsbeam_context *vm, int *variant, int *order);
 /* Number of MFMA beamformer variants and the tile shape of one: rows and
  * samples per workgroup, wavefronts per workgroup. Returns 0 if no such. */
 int vmBeamformerVariantInfo(int variant, int *rows, int *samples, int *waves);
+/* Name of the GPU kernel a variant launches ("k_beamform_mfma", or
+ * "k_beamform_res" for the resident-weights form), for profiles; NULL if no
+ * such variant. */
+const char *vmBeamformerVariantKernel(int variant);
 
 /* Batched beamforming of several coarse channels (no counterpart in the
  * reference, which runs one process per coarse channel, app/

@@ -427,6 +427,7 @@ _SIGS = {
     "vmBeamformerSelection": (C.c_int, [_CTX, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vmBeamformerVariantInfo": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int)]),
+    "vmBeamformerVariantKernel": (C.c_char_p, [C.c_int]),
     "vmKernelTimeMs": (C.c_double, [_CTX, C.c_char_p, C.POINTER(C.c_long)]),
     "vmDeviceMalloc": (C.c_void_p, [C.c_size_t]),
     "vmDeviceFree": (None, [C.c_void_p]),

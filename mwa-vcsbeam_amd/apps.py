@@ -39,11 +39,13 @@ def _beam_geoms(az, za):
 
 def tied_array_beam(obs, az, za, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, nchunks=1,
                     form=VM_BF_FP32, nseconds=1, device_geometry=False, want_e=True,
-                    ipfb=False, synth_filter=None, ipfb_device=False, return_vm=False):
+                    ipfb=False, synth_filter=None, ipfb_device=False, return_vm=False,
+                    variant=None):
     """Tied-array beamforming of `nseconds` seconds of `obs` towards the
     directions (az, za) [radians].  Returns dict with per-second lists:
     'S' [p][s][st][c] float32, 'e' [p][s][c][pol] complex128 (if want_e),
-    'vdif' float32 [n][pol][2] (if ipfb)."""
+    'vdif' float32 [n][pol][2] (if ipfb).  variant: the MFMA form's GEMM
+    variant (vmSetBeamformerVariant, tile order 0) instead of the default."""
     lib = load()
     az = np.atleast_1d(np.asarray(az, dtype=np.float64))
     za = np.atleast_1d(np.asarray(za, dtype=np.float64))
@@ -91,6 +93,8 @@ def tied_array_beam(obs, az, za, *, pfb_flags=PFB_FULL_PRECISION, nstokes=4, nch
     lib.vmCreateCudaStreams(vm)
     if device_geometry:
         lib.vmPushD(vm)
+    if variant is not None:
+        lib.vmSetBeamformerVariant(vm, variant, 0)
     out = {"S": [], "e": [], "vdif": []}
     for t in range(nseconds):
         sec = obs.second(t) if not obs.legacy else obs.legacy_second(t)

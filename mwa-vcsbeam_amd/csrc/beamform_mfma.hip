@@ -744,6 +744,7 @@ __global__ __launch_bounds__(32 * NG) void k_bf_prep(const void *__restrict__ vv
 // every wave-uniform offset live in scalar registers, the only vector
 // address is the lane's 16-byte slot.
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 #define RSRC_DWORD3 0x00020000 /* raw buffer, 32-bit data format (gfx9 family) */
 
 __device__ __forceinline__ rsrc_t make_rsrc(const void *base, uint32_t bytes)
@@ -1004,6 +1005,97 @@ __device__ __forceinline__ void bf_stokes(float xr, float xi, float yr, float yi
     } else {
         stokes4(xr, xi, yr, yi, unscale2, nn, invw, st4);
     }
+}
+
+// Epilogue of a wavefront tile of MB row blocks x two sample blocks (column
+// blocks Ex, Ey of each), Stokes of its 16 MB pointings x 64 samples.
+// ALL: every one of the 16 MB store instructions is issued, as raw-buffer
+// stores over the channel's part of S; rows and lanes with nothing to store
+// (Stokes past nstokes, pointings past the last, samples past the chunk) get
+// an offset past the buffer, and the hardware drops them.  The count of
+// vector-memory operations a tile leaves is then fixed (waits can count on
+// it); the channel's part of S must be under 2 GB.
+template <int PB, int MB, bool ALL = false>
+__device__ __forceinline__ void bf_store_runs(const f16v (&acc)[MB][4], const bf_geom &G, int ns,
+                                              int c, int rb0, int sb0, int lane,
+                                              const float (&us2)[2], const int (&esbv)[2],
+                                              const float4 (&nnv)[2],
+                                              const float4 (&T4v)[PB == 1 ? 2 : 1][4],
+                                              const float *wpb, const float *wT, float *wl)
+{
+    const int lk = lane >> 5, lr = lane & 31;
+    const int nstokes = G.nstokes, npointing = G.npointing;
+    const float invw = G.invw;
+    const int nblk = VM_S_NBLK(G.ns_total);
+    // Both sample blocks of a half block go through one 32-row x 64-sample
+    // LDS tile, and 16 lanes store one row: every store instruction writes
+    // 4 rows x 256 contiguous bytes (HBM takes such runs markedly faster
+    // than 128-byte ones).
+    // With the blocked layout of S a wavefront's 64 pointings x nstokes x
+    // 64 samples are one contiguous run when the chunk offset is a
+    // multiple of 64 (a lane's 4 samples never straddle a block).
+    const int q16 = lane & 15;
+    const int s4 = SB * sb0 + 4 * q16;
+    gmem float *Sc =
+        (gmem float *)G.S + VM_S_IDX(0, 0, c, G.soffset + s4, npointing, nstokes, nblk);
+    // ALL: the channel's part of S as a buffer, and the lane's byte offset in
+    // it of (pointing 0, Stokes 0) of its 4 samples
+    const uint32_t chan_bytes = (uint32_t)nblk * npointing * nstokes * VM_SBLK * 4;
+    const rsrc_t rS = make_rsrc((const float *)G.S + (size_t)c * nblk * npointing * nstokes * VM_SBLK,
+                                chan_bytes);
+    const int so = G.soffset + s4;
+    const uint32_t lane_off = 4u * ((uint32_t)(so / VM_SBLK) * npointing * nstokes * VM_SBLK +
+                                    (uint32_t)(so % VM_SBLK));
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        const int jg = 2 * h + jj;
+                        const f16v &ax = acc[m][2 * j], &ay = acc[m][2 * j + 1];
+                        const int pl = 16 * m + 8 * h + 4 * jj + 2 * lk + b;
+                        float st4[4];
+                        const int pc = min(16 * rb0 + pl, npointing - 1);
+                        bf_stokes<PB>(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1],
+                                      ay[4 * jg + 2 * b], ay[4 * jg + 2 * b + 1], us2[j],
+                                      esbv[j], nnv[j], T4v[PB == 1 ? j : 0], wpb, pl,
+                                      16 * rb0 + pl < npointing, invw,
+                                      wT + (j * 32 + lr) * BF_TROW,
+                                      G.gx + ((size_t)c * npointing + pc) * GX_ROW, G.gb.n,
+                                      PB == 2 ? ((const gmem int *)G.esp)[(size_t)c * npointing + pc] : 0,
+                                      st4);
+#pragma unroll
+                        for (int st = 0; st < 4; st++)
+                            wl[(st * 8 + 4 * jj + 2 * b + lk) * 68 + 32 * j + lr] = st4[st];
+                    }
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int r = 4 * i + (lane >> 4);
+                const int st = r >> 3, bbp = r & 7;
+                // bbp = 4 jj + 2 b + lk  ->  beam in the half = 4 jj + 2 lk + b
+                const int bb = 4 * (bbp >> 2) + 2 * (bbp & 1) + ((bbp >> 1) & 1);
+                const int p = 16 * (rb0 + m) + 8 * h + bb;
+                const float4 val = *(const float4 *)(wl + r * 68 + 4 * q16);
+                if constexpr (ALL) {
+                    const bool ok = st < nstokes && p < npointing && s4 < ns;
+                    const uint32_t off =
+                        ok ? lane_off + 4u * (uint32_t)(p * nstokes + st) * VM_SBLK : 0x7ffffff0u;
+                    // streamed output (nt), never read back by this kernel
+                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, val), rS,
+                                                           (int)off, 0, 2);
+                    continue;
+                }
+                if (st >= nstokes || p >= npointing || s4 >= ns) continue;
+                // streamed output, never read back by this kernel
+                __builtin_nontemporal_store(f4v{val.x, val.y, val.z, val.w},
+                                            (gmem f4v *)(Sc + ((size_t)p * nstokes + st) * VM_SBLK));
+            }
+        }
 }
 
 // Workgroup barrier that orders LDS only: the fences wait for this
@@ -1297,58 +1389,7 @@ __device__ __forceinline__ void bf_tile(const bf_geom &G, int ns, int nsb, int n
 #pragma unroll
     for (int j = 0; j < NSB; j++) us2[j] = ldexpf(1.0f, -2 * (esbv[j] + PHASE_EXP));
     if constexpr (VEC && NSB == 2) {
-        // Both sample blocks of a half block go through one 32-row x 64-sample
-        // LDS tile, and 16 lanes store one row: every store instruction writes
-        // 4 rows x 256 contiguous bytes (HBM takes such runs markedly faster
-        // than 128-byte ones).
-        // With the blocked layout of S a wavefront's 64 pointings x nstokes x
-        // 64 samples are one contiguous run when the chunk offset is a
-        // multiple of 64 (a lane's 4 samples never straddle a block).
-        const int q16 = lane & 15;
-        const int s4 = SB * sb0 + 4 * q16;
-        gmem float *Sc =
-            (gmem float *)G.S + VM_S_IDX(0, 0, c, G.soffset + s4, npointing, nstokes, nblk);
-#pragma unroll
-        for (int m = 0; m < MB; m++)
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-#pragma unroll
-                for (int j = 0; j < 2; j++)
-#pragma unroll
-                    for (int jj = 0; jj < 2; jj++)
-#pragma unroll
-                        for (int b = 0; b < 2; b++) {
-                            const int jg = 2 * h + jj;
-                            const f16v &ax = acc[m][2 * j], &ay = acc[m][2 * j + 1];
-                            const int pl = 16 * m + 8 * h + 4 * jj + 2 * lk + b;
-                            float st4[4];
-                            const int pc = min(16 * rb0 + pl, npointing - 1);
-                            bf_stokes<PB>(ax[4 * jg + 2 * b], ax[4 * jg + 2 * b + 1],
-                                          ay[4 * jg + 2 * b], ay[4 * jg + 2 * b + 1], us2[j],
-                                          esbv[j], nnv[j], T4v[PB == 1 ? j : 0], wpb, pl,
-                                          16 * rb0 + pl < npointing, invw,
-                                          wT + (j * 32 + lr) * BF_TROW,
-                                          G.gx + ((size_t)c * npointing + pc) * GX_ROW, G.gb.n,
-                                          PB == 2 ? ((const gmem int *)G.esp)[(size_t)c * npointing + pc] : 0,
-                                          st4);
-#pragma unroll
-                            for (int st = 0; st < 4; st++)
-                                wl[(st * 8 + 4 * jj + 2 * b + lk) * 68 + 32 * j + lr] = st4[st];
-                        }
-#pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    const int r = 4 * i + (lane >> 4);
-                    const int st = r >> 3, bbp = r & 7;
-                    // bbp = 4 jj + 2 b + lk  ->  beam in the half = 4 jj + 2 lk + b
-                    const int bb = 4 * (bbp >> 2) + 2 * (bbp & 1) + ((bbp >> 1) & 1);
-                    const int p = 16 * (rb0 + m) + 8 * h + bb;
-                    const float4 val = *(const float4 *)(wl + r * 68 + 4 * q16);
-                    if (st >= nstokes || p >= npointing || s4 >= ns) continue;
-                    // streamed output, never read back by this kernel
-                    __builtin_nontemporal_store(f4v{val.x, val.y, val.z, val.w},
-                                                (gmem f4v *)(Sc + ((size_t)p * nstokes + st) * VM_SBLK));
-                }
-            }
+        bf_store_runs<PB, MB>(acc, G, ns, c, rb0, sb0, lane, us2, esbv, nnv, T4v, wpb, wT, wl);
         return;
     }
     const int qq = lane & 7;
@@ -1477,17 +1518,209 @@ __global__ __launch_bounds__(64 * WR * WC, OCC) void k_beamform_batch(
     }
 }
 
+// Resident-weights form (pointing-independent J^-1, at most RES_NK k-steps,
+// i.e. 128 antennas).  A workgroup of WR wavefronts owns one (fine channel,
+// row tile) for the whole chunk: each wavefront holds its MB row blocks of A
+// for every k-step in registers, and the workgroup walks the column tiles (64
+// samples) with each tile's B fragments and N terms in LDS, double-buffered.
+// Tile tx + 1 is copied HBM -> LDS by direct (LDS-DMA) loads issued before
+// tile tx's k-loop, so the k-loop reads only LDS and registers.
+// Why: a CU's vector-memory path serves loads and stores in order, and while
+// the CU's Stokes stores are held up by HBM, a load issued behind them waits
+// as well; the register-ring k-loop then stalls under every store phase.
+// Here a wavefront's only loads are the DMA of the next tile, issued before
+// its stores of the current one, and the MFMAs run under the store stream.
+// The waits are explicit: before tile tx the wavefront waits until at most
+// as many operations are outstanding as it issued after tile tx's DMA (the
+// stores of tile tx - 1, always 16 MB: bf_store_runs with ALL), and the
+// barriers order LDS only, so no wait covers the stores.
+#define RES_NK 16      // k-steps held in registers
+#define RES_BSB 32768  // LDS bytes of one sample block's B fragments (RES_NK k-steps)
+template <int WR> constexpr int res_buf_bytes()
+{
+    // the B fragments or the Stokes staging, whichever is larger
+    return WR * BF_WAVE_LDS * 4 > 2 * RES_BSB ? WR * BF_WAVE_LDS * 4 : 2 * RES_BSB;
+}
+
+// LDS byte address of a __shared__ array
+#define LDS_ADDR(x) ((uint32_t)(uintptr_t)(__attribute__((address_space(3))) const void *)(x))
+
+// One LDS-DMA load: 64 lanes x 16 bytes from the lanes' global addresses g
+// to LDS bytes [lds, lds + 1024), lane i at lds + 16 i (M0 holds lds while
+// it issues; M0 is restored after).
+// Written as asm so that the compiler's wait insertion does not see it: it
+// cannot always tell an LDS-DMA load from the LDS accesses around it and
+// would then wait for the load (and for every store issued before it) at
+// unrelated LDS reads.  The kernel waits for it explicitly.  The instruction
+// writes no registers; its address registers and M0 are read when it issues
+// (one wait state after the M0 write).
+__device__ __forceinline__ void lds_dma16(const void *g, uint32_t lds)
+{
+    uint32_t m0save;
+    asm volatile("s_mov_b32 %0, m0\n\t"
+                 "s_mov_b32 m0, %2\n\t"
+                 "s_nop 0\n\t"
+                 "global_load_lds_dwordx4 %1, off\n\t"
+                 "s_mov_b32 m0, %0"
+                 : "=&s"(m0save)
+                 : "v"(g), "s"(lds)
+                 : "memory");
+}
+
+// s_waitcnt with vmcnt n alone (expcnt and lgkmcnt at their maxima)
+#define VM_WAIT_VMCNT(n) \
+    __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
+
+// Workgroup barrier with no memory fence: this wavefront's LDS operations
+// complete (lgkmcnt 0), then s_barrier; the empty asm statements keep the
+// compiler from moving memory operations across it.  LDS-DMA completion is
+// waited for explicitly before it (res_wait).
+__device__ __forceinline__ void res_barrier()
+{
+    __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 << 14));
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// This wavefront's share of tile tx's DMA into buf: sample block j's B
+// fragments (2 nkstep chunks of 1 KB, in their HBM order) at j RES_BSB, the
+// chunks taken round-robin by the wavefronts; the last wavefront also copies
+// the N terms of the tile's 64 samples to nbuf (samples past the last: the
+// last).
+// Blocks past the last sample block re-read it; their columns are never
+// stored.
+template <int WR>
+__device__ __forceinline__ void res_dma(uint32_t buf, uint32_t nbuf, const char *Bc,
+                                        const float4 *Nc, int tx, int nsb, int bstr, int nkstep,
+                                        int ns, int w, int lane)
+{
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const char *src = Bc + (size_t)min(2 * tx + j, nsb - 1) * bstr + 16 * lane;
+        for (int r = w; r < 2 * nkstep; r += WR)
+            lds_dma16(src + 1024 * r, buf + j * RES_BSB + 1024 * r);
+    }
+    if (w == WR - 1) lds_dma16(Nc + min(64 * tx + lane, ns - 1), nbuf);
+}
+
+// One column tile tx from buffer cur (and N terms ncur); starts the DMA of
+// tile tx + 1 into the buffers at LDS addresses nxt, nnxt.
+template <int PREC, int MB, int WR>
+__device__ __forceinline__ void res_tile(char *cur, const float4 *ncur, uint32_t nxt, uint32_t nnxt,
+                                         const bf_geom &G,
+                                         const afrag<MB, prec_parts<PREC>()> (&A)[RES_NK],
+                                         const char *Bc, const float4 *Nc, int c, int tx, int ntx,
+                                         int rb0, int ns, int nsb, int bstr, int w, int lane)
+{
+    const int nkstep = G.nkstep;
+    // tile tx's DMA was followed by this wavefront's 16 MB stores of tile
+    // tx - 1 and nothing else (none at tile 0, whose DMA the prologue waited for)
+    VM_WAIT_VMCNT(16 * MB);
+    res_barrier(); // tile tx is in cur; every wavefront is done with nxt
+    if (tx + 1 < ntx) res_dma<WR>(nxt, nnxt, Bc, Nc, tx + 1, nsb, bstr, nkstep, ns, w, lane);
+    const int lr = lane & 31;
+    float4 nnv[2];
+    int esbv[2];
+    float us2[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        nnv[j] = ncur[32 * j + lr];
+        esbv[j] = ((const __attribute__((address_space(4))) int *)G.sexp)[(size_t)c * nsb +
+                                                                          min(2 * tx + j, nsb - 1)];
+        us2[j] = ldexpf(1.0f, -2 * (esbv[j] + PHASE_EXP));
+    }
+    f16v acc[MB][4];
+#pragma unroll
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[m][n][q] = 0.f;
+    const uint4 *cb = (const uint4 *)cur + lane;
+#pragma unroll
+    for (int ks = 0; ks < RES_NK; ks++) {
+        if (ks < nkstep) {
+            bfrag<4> fb;
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+                fb.b[n] = cb[((n >> 1) * RES_BSB + 2048 * ks + 1024 * (n & 1)) / 16];
+            mma_ab<PREC, MB, 4>(acc, A[ks], fb);
+        }
+    }
+    res_barrier(); // every wavefront's k-loop reads of cur are done: staging may rewrite it
+    const float4 T4v[1][4] = {};
+    bf_store_runs<0, MB, true>(acc, G, ns, c, rb0, 2 * tx, lane, us2, esbv, nnv, T4v, nullptr,
+                               nullptr, (float *)(cur + w * BF_WAVE_LDS * 4));
+}
+
+template <int PREC, int MB, int WR>
+__global__ __launch_bounds__(64 * WR, 1) void k_beamform_res(bf_geom G, int ns, int nsb, int nchan,
+                                                             int nty)
+{
+    __shared__ __attribute__((aligned(16))) char bufA[res_buf_bytes<WR>()];
+    __shared__ __attribute__((aligned(16))) char bufB[res_buf_bytes<WR>()];
+    __shared__ float4 nA[64], nB[64];
+    constexpr int WP = prec_parts<PREC>();
+    // XCD-aware: the nty row tiles of a channel go to one XCD (workgroups
+    // L, L + 8, ... share an XCD), so their common B tiles meet in its L2
+    const int L = blockIdx.x;
+    int c, ty;
+    if ((nchan & 7) == 0) {
+        const int g = L >> 3;
+        c = (g / nty) * 8 + (L & 7);
+        ty = g % nty;
+    } else {
+        c = L / nty;
+        ty = L % nty;
+    }
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nkstep = G.nkstep, nrb = G.nrb;
+    const int rb0 = (ty * WR + w) * MB;
+    const int astr = nkstep * 2048, bstr = nkstep * 2048;
+    const rsrc_t rA = make_rsrc(G.Af + (size_t)c * nrb * nkstep * 128, (uint32_t)nrb * astr);
+    const char *Bc = (const char *)(G.Bf + (size_t)c * nsb * nkstep * 128);
+    const float4 *Nc = G.N + (size_t)c * ns;
+    const int ntx = (nsb + 1) / 2;
+    const uint32_t lA = LDS_ADDR(bufA), lB = LDS_ADDR(bufB), lnA = LDS_ADDR(nA), lnB = LDS_ADDR(nB);
+    res_dma<WR>(lA, lnA, Bc, Nc, 0, nsb, bstr, nkstep, ns, w, lane);
+    afrag<MB, WP> A[RES_NK];
+#pragma unroll
+    for (int ks = 0; ks < RES_NK; ks++) {
+        if (ks < nkstep) {
+            load_a<MB, WP>(A[ks], rA, lane * 16, rb0 * astr, astr, ks, true);
+        } else {
+#pragma unroll
+            for (int m = 0; m < MB; m++)
+#pragma unroll
+                for (int h = 0; h < WP; h++) A[ks].a[m][h] = make_uint4(0, 0, 0, 0);
+        }
+    }
+    VM_WAIT_VMCNT(0); // tile 0's DMA and A
+    for (int tx = 0; tx < ntx; tx += 2) {
+        res_tile<PREC, MB, WR>(bufA, nA, lB, lnB, G, A, Bc, Nc, c, tx, ntx, rb0, ns, nsb, bstr, w,
+                               lane);
+        if (tx + 1 < ntx)
+            res_tile<PREC, MB, WR>(bufB, nB, lA, lnA, G, A, Bc, Nc, c, tx + 1, ntx, rb0, ns, nsb,
+                                   bstr, w, lane);
+    }
+}
+
 // Kernel variants the autotuner chooses from: register tile per wavefront
 // (MB row blocks of 32 x NB column blocks of 32, NB = 2 x sample blocks),
 // wavefront grid per workgroup (WR x WC), depth of the B register ring (the
 // A ring is two deep) and the minimum wavefronts per SIMD the registers must
 // allow.
-// lds: the k-loop with workgroup-shared operands (bf_kloop_lds)
-struct bf_variant { int mb, nb, wr, wc, nst, occ, lds; };
+// lds: the k-loop with workgroup-shared operands (bf_kloop_lds); res: the
+// resident-weights form (k_beamform_res; one workgroup per channel and row
+// tile, tile orders do not apply)
+struct bf_variant { int mb, nb, wr, wc, nst, occ, lds, res; };
 static const bf_variant kVariants[] = {
     {4, 4, 2, 2, 4, 1}, {4, 4, 1, 4, 4, 1}, {4, 4, 1, 4, 6, 1}, {2, 4, 2, 2, 4, 2},
     {4, 2, 2, 2, 4, 2}, {2, 4, 4, 1, 4, 2}, {4, 4, 4, 1, 4, 1}, {4, 4, 2, 2, 6, 1},
     {2, 4, 1, 4, 4, 2}, {2, 4, 4, 2, 4, 2}, {2, 4, 2, 2, 4, 2, 1}, {2, 4, 8, 1, 4, 2},
+    {1, 4, 8, 1, 0, 1, 0, 1}, {2, 4, 4, 1, 0, 1, 0, 1},
 };
 #define N_VARIANTS ((int)(sizeof(kVariants) / sizeof(kVariants[0])))
 #define VM_N_ORDERS 5
@@ -1536,7 +1769,18 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
     case 8: VM_LAUNCH(2, 4, 1, 4, 4, 2, false); break;
     case 9: VM_LAUNCH(2, 4, 4, 2, 4, 2, false); break;
     case 10: VM_LAUNCH(2, 4, 2, 2, 4, 2, true); break;
-    default: VM_LAUNCH(2, 4, 8, 1, 4, 2, false); break;
+    case 11: VM_LAUNCH(2, 4, 8, 1, 4, 2, false); break;
+    case 12:
+        if constexpr (PB == 0)
+            hipLaunchKernelGGL((k_beamform_res<PREC, 1, 8>), dim3((unsigned)(nchan * nty)),
+                               dim3(512), 0, st, G, ns, nsb, nchan, nty);
+        break;
+    case 13:
+        if constexpr (PB == 0 && PREC == 2)
+            hipLaunchKernelGGL((k_beamform_res<PREC, 2, 4>), dim3((unsigned)(nchan * nty)),
+                               dim3(256), 0, st, G, ns, nsb, nchan, nty);
+        break;
+    default: VM_FATAL("MFMA beamformer: no variant %d", vi);
     }
     }
 #undef VM_LAUNCH
@@ -1656,6 +1900,8 @@ struct mfma_plan {
     int *esp;    // pb 1, 2: weight exponents per (c, p)
     float *gx;   // pb 2: the groups' constants per (c, p)
     grp_bounds gb;
+    int prec;    // GEMM precision (form_prec)
+    int ns_total; // samples per second (the blocked layout of S)
 };
 
 // Whether J^-1 (vm->d_J) differs between pointings, decided on the device
@@ -1927,6 +2173,8 @@ static void mfma_prepare(vcsbeam_context *vm, mfma_plan *pl)
     pl->Af = Af; pl->Bf = Bf; pl->N = N; pl->sexp = sexp;
     pl->pb = pb; pl->bx = bx; pl->esp = esp; pl->gx = gx; pl->gb = gb;
     pl->invw = (float)(1.0 / (double)vm->num_not_flagged);
+    pl->prec = form_prec(vm->beamform_form);
+    pl->ns_total = (int)vm->fine_sample_rate;
 }
 
 static bool variant_ok(int vi, const mfma_plan *pl)
@@ -1934,6 +2182,15 @@ static bool variant_ok(int vi, const mfma_plan *pl)
     // sample counts that are not multiples of 4 have only variant 0
     if (pl->ns & 3 && vi != 0) return false;
     if (pl->pb == 2 && !variant_has_groups(vi)) return false;
+    // resident weights: pointing-independent J^-1 and A within the registers
+    // (two row blocks per wavefront hold A in registers with one weight part
+    // only); each channel's part of S under 2 GB
+    if (vi >= 0 && vi < N_VARIANTS && kVariants[vi].res) {
+        const size_t chan_bytes = (size_t)VM_S_NBLK(pl->ns_total) * pl->np * 4 * VM_SBLK * 4;
+        if (pl->pb != 0 || pl->nkstep > RES_NK || (kVariants[vi].mb > 1 && pl->prec != 2) ||
+            chan_bytes >= ((size_t)1 << 31))
+            return false;
+    }
     return vi >= 0 && vi < N_VARIANTS && pl->nrb % (kVariants[vi].mb * kVariants[vi].wr) == 0;
 }
 
@@ -1976,6 +2233,12 @@ int vmBeamformerVariantInfo(int variant, int *rows, int *samples, int *waves)
     return N_VARIANTS;
 }
 
+const char *vmBeamformerVariantKernel(int variant)
+{
+    if (variant < 0 || variant >= N_VARIANTS) return NULL;
+    return kVariants[variant].res ? "k_beamform_res" : "k_beamform_mfma";
+}
+
 int vmBeamformerSelection(vcsbeam_context *vm, int *variant, int *order)
 {
     vm_device_state *d = vm_dev(vm);
@@ -2010,7 +2273,8 @@ int vmTuneBeamformer(vcsbeam_context *vm, int reps, float *best_ms)
     int bv = 0, bo = 1;
     for (int vi = 0; vi < N_VARIANTS; vi++) {
         if (!variant_ok(vi, &pl)) continue;
-        for (int order = 0; order < VM_N_ORDERS; order++) {
+        // (the resident-weights form has one workgroup order)
+        for (int order = 0; order < (kVariants[vi].res ? 1 : VM_N_ORDERS); order++) {
             mfma_run(vm, &pl, vi, order); // warm-up
             HIPCHECK(hipEventRecord(e0, st));
             for (int r = 0; r < reps; r++) mfma_run(vm, &pl, vi, order);
@@ -2101,7 +2365,7 @@ struct vm_batch {
 static bool batch_variant_ok(int vi)
 {
     // the persistent batch kernel has the register-ring k-loop only
-    return vi >= 0 && vi < N_VARIANTS && !kVariants[vi].lds &&
+    return vi >= 0 && vi < N_VARIANTS && !kVariants[vi].lds && !kVariants[vi].res &&
            ROW_PAD % (RBK * kVariants[vi].mb * kVariants[vi].wr) == 0;
 }
 

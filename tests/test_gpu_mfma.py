@@ -71,6 +71,33 @@ def test_mfma_chunked_and_fp32_agree(pkg, orc, gpu_lib):
     assert err[:, 0].max() <= 1e-3
 
 
+@pytest.mark.parametrize("variant,form", [(12, 1), (12, 3), (13, 3)])
+def test_resident_weights_chunks_and_stokes_i(pkg, orc, gpu_lib, variant, form):
+    """The resident-weights GEMM (variants 12 and 13: A held in registers, B
+    tiles copied to LDS ahead of the k-loop, a fixed count of stores per
+    tile) with two chunks per second -- the second chunk starts at sample
+    500, inside a 64-sample block of S, and ends in a partial tile -- and
+    with Stokes I alone (the other Stokes rows' stores dropped), against the
+    FP32 form on the same inputs.  40 beams: the last wavefront's pointings
+    run past the end.  128 tiles, 0.1 s."""
+    from vcsbeam_amd.apps import tied_array_beam
+    obs, az, za = _setup(pkg, nbeams=40, ntiles=128)
+    for nstokes in (1, 4):
+        print(f"variant {variant} form {form} nstokes {nstokes}, 2 chunks", flush=True)
+        m = tied_array_beam(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, form=form, nchunks=2,
+                            nstokes=nstokes, want_e=False, variant=variant)
+        f = tied_array_beam(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, form=0, nchunks=2,
+                            nstokes=nstokes, want_e=False)
+        Sm, Sf = m["S"][0], f["S"][0].astype(np.float64)
+        err = np.array([[orc.rel_err(Sm[p, :, st, :], Sf[p, :, st, :]) for st in range(nstokes)]
+                        for p in range(Sm.shape[0])])
+        print("   max per-beam rel err vs FP32:", err.max(axis=0))
+        assert np.isfinite(Sm).all()
+        assert err[:, 0].max() <= 1e-3
+        if nstokes == 4:
+            assert err[:, 1:].max() <= 3e-3
+
+
 @pytest.fixture(scope="module")
 def mfma_case(pkg, orc):
     obs, az, za = _setup(pkg)
@@ -78,7 +105,7 @@ def mfma_case(pkg, orc):
 
 
 @pytest.mark.parametrize("form", [1, 3])
-@pytest.mark.parametrize("variant", range(12))
+@pytest.mark.parametrize("variant", range(14))
 def test_every_beamformer_variant_and_tile_order(pkg, orc, gpu_lib, mfma_case, variant, form):
     """Every GEMM variant (per-tile and persistent kernels, all tile shapes
     and epilogues) in every tile order gives the same Stokes as the FP64
