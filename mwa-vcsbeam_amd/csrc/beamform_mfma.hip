@@ -1720,7 +1720,7 @@ static const bf_variant kVariants[] = {
     {4, 4, 2, 2, 4, 1}, {4, 4, 1, 4, 4, 1}, {4, 4, 1, 4, 6, 1}, {2, 4, 2, 2, 4, 2},
     {4, 2, 2, 2, 4, 2}, {2, 4, 4, 1, 4, 2}, {4, 4, 4, 1, 4, 1}, {4, 4, 2, 2, 6, 1},
     {2, 4, 1, 4, 4, 2}, {2, 4, 4, 2, 4, 2}, {2, 4, 2, 2, 4, 2, 1}, {2, 4, 8, 1, 4, 2},
-    {1, 4, 8, 1, 0, 1, 0, 1}, {2, 4, 4, 1, 0, 1, 0, 1},
+    {1, 4, 8, 1, 0, 1, 0, 1},
 };
 #define N_VARIANTS ((int)(sizeof(kVariants) / sizeof(kVariants[0])))
 #define VM_N_ORDERS 5
@@ -1774,11 +1774,6 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
         if constexpr (PB == 0)
             hipLaunchKernelGGL((k_beamform_res<PREC, 1, 8>), dim3((unsigned)(nchan * nty)),
                                dim3(512), 0, st, G, ns, nsb, nchan, nty);
-        break;
-    case 13:
-        if constexpr (PB == 0 && PREC == 2)
-            hipLaunchKernelGGL((k_beamform_res<PREC, 2, 4>), dim3((unsigned)(nchan * nty)),
-                               dim3(256), 0, st, G, ns, nsb, nchan, nty);
         break;
     default: VM_FATAL("MFMA beamformer: no variant %d", vi);
     }
@@ -2182,14 +2177,11 @@ static bool variant_ok(int vi, const mfma_plan *pl)
     // sample counts that are not multiples of 4 have only variant 0
     if (pl->ns & 3 && vi != 0) return false;
     if (pl->pb == 2 && !variant_has_groups(vi)) return false;
-    // resident weights: pointing-independent J^-1 and A within the registers
-    // (two row blocks per wavefront hold A in registers with one weight part
-    // only); each channel's part of S under 2 GB
+    // resident weights: pointing-independent J^-1 and A within the registers;
+    // each channel's part of S under 2 GB
     if (vi >= 0 && vi < N_VARIANTS && kVariants[vi].res) {
         const size_t chan_bytes = (size_t)VM_S_NBLK(pl->ns_total) * pl->np * 4 * VM_SBLK * 4;
-        if (pl->pb != 0 || pl->nkstep > RES_NK || (kVariants[vi].mb > 1 && pl->prec != 2) ||
-            chan_bytes >= ((size_t)1 << 31))
-            return false;
+        if (pl->pb != 0 || pl->nkstep > RES_NK || chan_bytes >= ((size_t)1 << 31)) return false;
     }
     return vi >= 0 && vi < N_VARIANTS && pl->nrb % (kVariants[vi].mb * kVariants[vi].wr) == 0;
 }
