@@ -1605,8 +1605,10 @@ __device__ __forceinline__ void res_dma(uint32_t buf, uint32_t nbuf, const char 
 }
 
 // One column tile tx from buffer cur (and N terms ncur); starts the DMA of
-// tile tx + 1 into the buffers at LDS addresses nxt, nnxt.
-template <int PREC, int MB, int WR>
+// tile tx + 1 into the buffers at LDS addresses nxt, nnxt.  FULLK: exactly
+// RES_NK k-steps, so the k-loop needs no per-step test and each step's B
+// fragments are read one step ahead of its MFMAs.
+template <int PREC, int MB, int WR, bool FULLK>
 __device__ __forceinline__ void res_tile(char *cur, const float4 *ncur, uint32_t nxt, uint32_t nnxt,
                                          const bf_geom &G,
                                          const afrag<MB, prec_parts<PREC>()> (&A)[RES_NK],
@@ -1638,14 +1640,32 @@ __device__ __forceinline__ void res_tile(char *cur, const float4 *ncur, uint32_t
 #pragma unroll
             for (int q = 0; q < 16; q++) acc[m][n][q] = 0.f;
     const uint4 *cb = (const uint4 *)cur + lane;
+    if constexpr (FULLK) {
+        bfrag<4> fb[2];
 #pragma unroll
-    for (int ks = 0; ks < RES_NK; ks++) {
-        if (ks < nkstep) {
-            bfrag<4> fb;
+        for (int n = 0; n < 4; n++) fb[0].b[n] = cb[((n >> 1) * RES_BSB + 1024 * (n & 1)) / 16];
 #pragma unroll
-            for (int n = 0; n < 4; n++)
-                fb.b[n] = cb[((n >> 1) * RES_BSB + 2048 * ks + 1024 * (n & 1)) / 16];
-            mma_ab<PREC, MB, 4>(acc, A[ks], fb);
+        for (int ks = 0; ks < RES_NK; ks++) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (ks + 1 < RES_NK) {
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    fb[(ks + 1) & 1].b[n] =
+                        cb[((n >> 1) * RES_BSB + 2048 * (ks + 1) + 1024 * (n & 1)) / 16];
+            }
+            mma_ab<PREC, MB, 4>(acc, A[ks], fb[ks & 1]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+#pragma unroll
+        for (int ks = 0; ks < RES_NK; ks++) {
+            if (ks < nkstep) {
+                bfrag<4> fb;
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    fb.b[n] = cb[((n >> 1) * RES_BSB + 2048 * ks + 1024 * (n & 1)) / 16];
+                mma_ab<PREC, MB, 4>(acc, A[ks], fb);
+            }
         }
     }
     res_barrier(); // every wavefront's k-loop reads of cur are done: staging may rewrite it
@@ -1654,7 +1674,7 @@ __device__ __forceinline__ void res_tile(char *cur, const float4 *ncur, uint32_t
                                nullptr, (float *)(cur + w * BF_WAVE_LDS * 4));
 }
 
-template <int PREC, int MB, int WR>
+template <int PREC, int MB, int WR, bool FULLK>
 __global__ __launch_bounds__(64 * WR, 1) void k_beamform_res(bf_geom G, int ns, int nsb, int nchan,
                                                              int nty)
 {
@@ -1699,11 +1719,11 @@ __global__ __launch_bounds__(64 * WR, 1) void k_beamform_res(bf_geom G, int ns, 
     }
     VM_WAIT_VMCNT(0); // tile 0's DMA and A
     for (int tx = 0; tx < ntx; tx += 2) {
-        res_tile<PREC, MB, WR>(bufA, nA, lB, lnB, G, A, Bc, Nc, c, tx, ntx, rb0, ns, nsb, bstr, w,
-                               lane);
+        res_tile<PREC, MB, WR, FULLK>(bufA, nA, lB, lnB, G, A, Bc, Nc, c, tx, ntx, rb0, ns, nsb,
+                                      bstr, w, lane);
         if (tx + 1 < ntx)
-            res_tile<PREC, MB, WR>(bufB, nB, lA, lnA, G, A, Bc, Nc, c, tx + 1, ntx, rb0, ns, nsb,
-                                   bstr, w, lane);
+            res_tile<PREC, MB, WR, FULLK>(bufB, nB, lA, lnA, G, A, Bc, Nc, c, tx + 1, ntx, rb0, ns,
+                                          nsb, bstr, w, lane);
     }
 }
 
@@ -1771,9 +1791,14 @@ static void launch_variant(int vi, dim3 grid, hipStream_t st, const bf_geom &G, 
     case 10: VM_LAUNCH(2, 4, 2, 2, 4, 2, true); break;
     case 11: VM_LAUNCH(2, 4, 8, 1, 4, 2, false); break;
     case 12:
-        if constexpr (PB == 0)
-            hipLaunchKernelGGL((k_beamform_res<PREC, 1, 8>), dim3((unsigned)(nchan * nty)),
-                               dim3(512), 0, st, G, ns, nsb, nchan, nty);
+        if constexpr (PB == 0) {
+            if (G.nkstep == RES_NK)
+                hipLaunchKernelGGL((k_beamform_res<PREC, 1, 8, true>), dim3((unsigned)(nchan * nty)),
+                                   dim3(512), 0, st, G, ns, nsb, nchan, nty);
+            else
+                hipLaunchKernelGGL((k_beamform_res<PREC, 1, 8, false>), dim3((unsigned)(nchan * nty)),
+                                   dim3(512), 0, st, G, ns, nsb, nchan, nty);
+        }
         break;
     default: VM_FATAL("MFMA beamformer: no variant %d", vi);
     }
