@@ -78,6 +78,7 @@ void vm_dev_free(vcsbeam_context *vm)
     dfree(d->d_ipfb_jobs);
     dfree(d->d_nterm);
     dfree(d->d_flag);
+    vm_timer_free(d);
     HIPCHECK(hipStreamDestroy((hipStream_t)d->stream));
     free(d);
     vm->dev = NULL;
@@ -551,14 +552,50 @@ static const char *vm_timer_names[VM_T_NSLOT] = {
 static vm_device_state g_dev_global;
 vm_device_state *vm_dev_global(void) { return &g_dev_global; }
 
+// The timers record events and do not wait for them, so timing adds no
+// host synchronisation to the timed calls: each slot keeps a ring of
+// VM_T_RING (start, stop) pairs, resolved when the times are read (or when
+// the ring is full).
+static void vm_timer_flush_slot(vm_device_state *d, int slot)
+{
+    for (int k = 0; k < d->t_pend[slot]; k++) {
+        HIPCHECK(hipEventSynchronize((hipEvent_t)d->ev[slot][k][1]));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, (hipEvent_t)d->ev[slot][k][0],
+                                     (hipEvent_t)d->ev[slot][k][1]));
+        d->t_ms[slot] += ms;
+        d->t_n[slot] += 1;
+    }
+    d->t_pend[slot] = 0;
+}
+
+void vm_timer_flush(vm_device_state *d)
+{
+    for (int i = 0; i < VM_T_NSLOT; i++) vm_timer_flush_slot(d, i);
+}
+
+void vm_timer_free(vm_device_state *d)
+{
+    for (int i = 0; i < VM_T_NSLOT; i++)
+        for (int k = 0; k < VM_T_RING; k++)
+            for (int e = 0; e < 2; e++)
+                if (d->ev[i][k][e]) {
+                    HIPCHECK(hipEventDestroy((hipEvent_t)d->ev[i][k][e]));
+                    d->ev[i][k][e] = NULL;
+                }
+}
+
 void vm_timer_start_on(vm_device_state *d, int slot, void *stream)
 {
     if (!d->timing) return;
-    if (!d->ev[slot][0]) {
-        HIPCHECK(hipEventCreate((hipEvent_t *)&d->ev[slot][0]));
-        HIPCHECK(hipEventCreate((hipEvent_t *)&d->ev[slot][1]));
+    if (d->t_pend[slot] == VM_T_RING) vm_timer_flush_slot(d, slot);
+    void **e = d->ev[slot][d->t_pend[slot]];
+    if (!e[0]) {
+        HIPCHECK(hipEventCreate((hipEvent_t *)&e[0]));
+        HIPCHECK(hipEventCreate((hipEvent_t *)&e[1]));
     }
-    HIPCHECK(hipEventRecord((hipEvent_t)d->ev[slot][0], (hipStream_t)stream));
+    HIPCHECK(hipEventRecord((hipEvent_t)e[0], (hipStream_t)stream));
+    d->t_open[slot] = 1;
 }
 
 void vm_timer_start(vm_device_state *d, int slot) { vm_timer_start_on(d, slot, d->stream); }
@@ -567,18 +604,17 @@ void vm_timer_stop(vm_device_state *d, int slot) { vm_timer_stop_on(d, slot, d->
 
 void vm_timer_stop_on(vm_device_state *d, int slot, void *stream)
 {
-    if (!d->timing || !d->ev[slot][0]) return;
-    HIPCHECK(hipEventRecord((hipEvent_t)d->ev[slot][1], (hipStream_t)stream));
-    HIPCHECK(hipEventSynchronize((hipEvent_t)d->ev[slot][1]));
-    float ms = 0.f;
-    HIPCHECK(hipEventElapsedTime(&ms, (hipEvent_t)d->ev[slot][0], (hipEvent_t)d->ev[slot][1]));
-    d->t_ms[slot] += ms;
-    d->t_n[slot] += 1;
+    if (!d->timing || !d->t_open[slot]) return;
+    HIPCHECK(hipEventRecord((hipEvent_t)d->ev[slot][d->t_pend[slot]][1], (hipStream_t)stream));
+    d->t_pend[slot]++;
+    d->t_open[slot] = 0;
 }
 
 void vmSetKernelTiming(vcsbeam_context *vm, int on)
 {
     vm_device_state *d = vm ? vm_dev(vm) : vm_dev_global();
+    vm_timer_flush(d);
+    for (int i = 0; i < VM_T_NSLOT; i++) d->t_open[i] = 0;
     d->timing = on;
     if (!on) return; /* keep the counts for vmKernelTimeMs */
     for (int i = 0; i < VM_T_NSLOT; i++) {
@@ -590,6 +626,7 @@ void vmSetKernelTiming(vcsbeam_context *vm, int on)
 double vmKernelTimeMs(vcsbeam_context *vm, const char *kernel, long *count)
 {
     vm_device_state *d = vm ? vm_dev(vm) : vm_dev_global();
+    vm_timer_flush(d);
     for (int i = 0; i < VM_T_NSLOT; i++)
         if (strcmp(kernel, vm_timer_names[i]) == 0) {
             if (count) *count = d->t_n[i];

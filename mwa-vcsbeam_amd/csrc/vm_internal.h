@@ -39,6 +39,7 @@ void vm_free_obs_contexts(vcsbeam_context *vm);
      (size_t)((s) % VM_SBLK))
 
 /* Device-side state hung off vcsbeam_context.dev (created lazily). */
+#define VM_T_RING 32 /* timed launches per slot between two reads */
 struct vm_device_state {
     void *stream;             /* hipStream_t on which all context work runs */
     /* FP32 beamformer weights W = phi * J^-1, complex float
@@ -113,9 +114,13 @@ struct vm_device_state {
     int bf_order;
     /* compute units of the device (persistent launches) */
     int ncu;
-    /* per-kernel event timing (vmSetKernelTiming) */
+    /* per-kernel event timing (vmSetKernelTiming): per slot a ring of
+       VM_T_RING (start, stop) event pairs, recorded without waiting and
+       resolved into t_ms / t_n when read (vm_timer_flush) */
     int timing;
-    void *ev[16][2];
+    void *ev[16][VM_T_RING][2];
+    int t_pend[16];  /* pairs recorded and not yet resolved */
+    int t_open[16];  /* a start without its stop yet */
     double t_ms[16];
     long t_n[16];
 };
@@ -140,6 +145,10 @@ void vm_timer_stop(vm_device_state *d, int slot);
 /* the same on another stream (calls without a context use vm_dev_global()) */
 void vm_timer_start_on(vm_device_state *d, int slot, void *stream);
 void vm_timer_stop_on(vm_device_state *d, int slot, void *stream);
+/* resolve the recorded pairs into t_ms / t_n (waits for their stops) */
+void vm_timer_flush(vm_device_state *d);
+/* destroy the timing events */
+void vm_timer_free(vm_device_state *d);
 vm_device_state *vm_dev_global(void);
 
 /* Host helpers implemented in vm_host.c */
