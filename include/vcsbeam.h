@@ -1169,7 +1169,10 @@ void vmEventDestroy(void *ev);
  * bracketed by events; vmKernelTimeMs returns the average milliseconds per
  * launch since timing was switched on.  With vm NULL they apply to the calls
  * that take no context: "incoh" (cu_form_incoh_beam's device part: the
- * sum and the renormalisation) and "vdif" (vmRequantVDIFDevice). */
+ * sum and the renormalisation) and "vdif" (vmRequantVDIFDevice).  The
+ * events are recorded without waiting for them (timing adds no host
+ * synchronisation to the timed calls); vmKernelTimeMs and vmSetKernelTiming
+ * wait for the recorded ones. */
 void vmSetKernelTiming(vcsbeam_context *vm, int on);
 double vmKernelTimeMs(vcsbeam_context *vm, const char *kernel, long *count);
 

@@ -195,3 +195,37 @@ def test_device_jones_shared_between_pointings(pkg, orc, gpu_lib):
         assert np.abs(J[2] - J[0]).max() > 1e-3 * np.abs(J[0]).max()
     finally:
         ses.close()
+
+
+def test_kernel_timing_counts_every_launch(pkg, gpu_lib):
+    """The kernel timers record each timed launch without waiting; 40 steps
+    (more than the 32 pairs a slot holds between two reads, so the ring is
+    resolved once while timing) are all counted, with positive averages, and
+    the Stokes are the same as untimed."""
+    import ctypes as C
+    from vcsbeam_amd.apps import TiedArraySession
+    from vcsbeam_amd.workloads import directions_in_cap
+    obs = _obs(ntiles=32)
+    az, za = directions_in_cap(8, 15.0, centre_za_deg=20.0)
+    ses = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, nstokes=4,
+                           form=pkg.VM_BF_MFMA_FP16)
+    try:
+        ses.read(0)
+        ses.pfb()
+        ses.geometry()
+        ses.beamform()
+        S0 = ses.stokes()
+        gpu_lib.vmSetKernelTiming(ses.vm, 1)
+        for _ in range(40):
+            ses.geometry()
+            ses.beamform()
+        gpu_lib.vmSetKernelTiming(ses.vm, 0)
+        n = C.c_long(0)
+        for name, per_step in (("beamform_mfma", 1), ("beamform_prep", 1), ("geometry", 2)):
+            ms = gpu_lib.vmKernelTimeMs(ses.vm, name.encode(), C.byref(n))
+            print(f"{name}: {n.value} launches, {ms:.4f} ms each", flush=True)
+            assert n.value == 40 * per_step
+            assert ms > 0
+        assert np.array_equal(ses.stokes(), S0)
+    finally:
+        ses.close()
