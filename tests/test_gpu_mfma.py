@@ -71,17 +71,18 @@ def test_mfma_chunked_and_fp32_agree(pkg, orc, gpu_lib):
     assert err[:, 0].max() <= 1e-3
 
 
-@pytest.mark.parametrize("variant,form", [(12, 1), (12, 3)])
-def test_resident_weights_chunks_and_stokes_i(pkg, orc, gpu_lib, variant, form):
+@pytest.mark.parametrize("variant,form,ntiles", [(12, 1, 128), (12, 3, 128), (12, 1, 64)])
+def test_resident_weights_chunks_and_stokes_i(pkg, orc, gpu_lib, variant, form, ntiles):
     """The resident-weights GEMM (variant 12: A held in registers, B tiles
     copied to LDS ahead of the k-loop, a fixed count of stores per tile) with
     two chunks per second -- the second chunk starts at sample
     500, inside a 64-sample block of S, and ends in a partial tile -- and
     with Stokes I alone (the other Stokes rows' stores dropped), against the
     FP32 form on the same inputs.  40 beams: the last wavefront's pointings
-    run past the end.  128 tiles, 0.1 s."""
+    run past the end.  128 tiles (all 16 k-steps: the full-K k-loop) or 64
+    (8 k-steps: the tested k-loop), 0.1 s."""
     from vcsbeam_amd.apps import tied_array_beam
-    obs, az, za = _setup(pkg, nbeams=40, ntiles=128)
+    obs, az, za = _setup(pkg, nbeams=40, ntiles=ntiles)
     for nstokes in (1, 4):
         print(f"variant {variant} form {form} nstokes {nstokes}, 2 chunks", flush=True)
         m = tied_array_beam(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, form=form, nchunks=2,
