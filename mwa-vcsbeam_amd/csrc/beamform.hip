@@ -338,8 +338,31 @@ __global__ __launch_bounds__(256) void k_minmax_dev(const float *__restrict__ S,
     const float *r0 = S + VM_S_IDX(p, st, c, 0, npointing, nstokes, nblk);
     const size_t bstride = (size_t)npointing * nstokes * VM_SBLK; // next sample block
     float mn = r0[0], mx = r0[0];
-    for (int s = threadIdx.x; s < nstep; s += 256) {
-        float val = r0[(size_t)(s / VM_SBLK) * bstride + s % VM_SBLK];
+    // lane: 4 consecutive samples (q = lane % 16) of block b = lane / 16 +
+    // 16 k; four blocks' loads in flight per lane (the row is 40 KB at most
+    // 10 000 samples, so one workgroup has it all in flight in a few rounds)
+    const int q = threadIdx.x & 15, b0 = threadIdx.x >> 4;
+    const int nfull = nstep / VM_SBLK; // blocks with 64 samples
+    int b = b0;
+    for (; b + 48 < nfull; b += 64) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            v[u] = *(const float4 *)(r0 + (size_t)(b + 16 * u) * bstride + 4 * q);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            mn = fminf(mn, fminf(fminf(v[u].x, v[u].y), fminf(v[u].z, v[u].w)));
+            mx = fmaxf(mx, fmaxf(fmaxf(v[u].x, v[u].y), fmaxf(v[u].z, v[u].w)));
+        }
+    }
+    for (; b < nfull; b += 16) {
+        const float4 v = *(const float4 *)(r0 + (size_t)b * bstride + 4 * q);
+        mn = fminf(mn, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
+        mx = fmaxf(mx, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
+    }
+    // the last, partial block sample by sample
+    for (int s = nfull * VM_SBLK + threadIdx.x; s < nstep; s += 256) {
+        const float val = r0[(size_t)nfull * bstride + s % VM_SBLK];
         mn = fminf(mn, val);
         mx = fmaxf(mx, val);
     }
@@ -362,7 +385,10 @@ __global__ __launch_bounds__(256) void k_minmax_dev(const float *__restrict__ S,
 
 // Quantise + transpose: d_S (VM_S_IDX) -> Ss[p][s][st][ctot] (uint8), this
 // channel's nchan at coff (offsets/scales rows as in k_minmax_dev).
-// block 256, tile 64 channels x 64 samples; grid (ceil(ns/64), ceil(nc/64), p*nst)
+// block 256, tile 64 channels x 64 samples (one sample block of S, so each
+// channel's 64 samples are one contiguous 256-byte row); grid
+// (ceil(ns/64), ceil(nc/64), p*nst).  Lanes read 4 samples of a channel row
+// (16 lanes a row) and write 4 channels of an output row (16 lanes a row).
 __global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, int nstep,
                                                    int npointing, int nstokes, int nchan,
                                                    int ctot, int coff,
@@ -370,25 +396,42 @@ __global__ __launch_bounds__(256) void k_quant_dev(const float *__restrict__ S, 
                                                    const float *__restrict__ scales,
                                                    uint8_t *__restrict__ Ss)
 {
-    __shared__ uint8_t T[64][65];
+    __shared__ __attribute__((aligned(16))) uint8_t T[64][68]; // [channel][sample], rows 4-byte aligned
     const int pst = blockIdx.z, p = pst / nstokes, st = pst % nstokes;
     const int c0 = blockIdx.y * 64, s0 = blockIdx.x * 64;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int r = ty; r < 64; r += 4) {
-        int c = c0 + r, s = s0 + tx;
-        uint8_t q = 0;
-        if (c < nchan && s < nstep) {
-            size_t row = (size_t)pst * ctot + coff + c;
-            q = renorm_q(S[VM_S_IDX(p, st, c, s, npointing, nstokes, VM_S_NBLK(nstep))],
-                         offsets[row], scales[row]);
+    const int l4 = threadIdx.x & 15, ry = threadIdx.x >> 4;
+    // row c of the tile: S + VM_S_IDX(p, st, c, s0) + (c - c0) * cstr
+    const size_t cstr = (size_t)VM_S_NBLK(nstep) * npointing * nstokes * VM_SBLK;
+    const float *base = S + VM_S_IDX(p, st, c0, s0, npointing, nstokes, VM_S_NBLK(nstep));
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int r = ry + 16 * k, c = c0 + r;
+        uint32_t w = 0;
+        if (c < nchan) {
+            // samples past nstep are the padding of the last block: read, not stored
+            const float4 v = *(const float4 *)(base + (size_t)r * cstr + 4 * l4);
+            const size_t row = (size_t)pst * ctot + coff + c;
+            const float off = offsets[row], sc = scales[row];
+            w = (uint32_t)renorm_q(v.x, off, sc) | ((uint32_t)renorm_q(v.y, off, sc) << 8) |
+                ((uint32_t)renorm_q(v.z, off, sc) << 16) | ((uint32_t)renorm_q(v.w, off, sc) << 24);
         }
-        T[r][tx] = q;
+        *(uint32_t *)&T[r][4 * l4] = w;
     }
     __syncthreads();
-    for (int r = ty; r < 64; r += 4) {
-        int s = s0 + r, c = c0 + tx;
-        if (s < nstep && c < nchan)
-            Ss[(((size_t)p * nstep + s) * nstokes + st) * ctot + coff + c] = T[tx][r];
+    const bool vec = ((ctot | coff) & 3) == 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int r = ry + 16 * k, s = s0 + r; // output row: sample s, channels c0 + 4 l4 ..
+        if (s >= nstep) continue;
+        const int c = c0 + 4 * l4;
+        uint8_t *dst = Ss + (((size_t)p * nstep + s) * nstokes + st) * ctot + coff + c;
+        if (vec && c + 3 < nchan) {
+            *(uchar4 *)dst = make_uchar4(T[c - c0][r], T[c - c0 + 1][r], T[c - c0 + 2][r],
+                                         T[c - c0 + 3][r]);
+        } else {
+            for (int e = 0; e < 4; e++)
+                if (c + e < nchan) dst[e] = T[c - c0 + e][r];
+        }
     }
 }
 
