@@ -1574,7 +1574,7 @@ __device__ __forceinline__ void lds_dma16(const void *g, uint32_t lds)
 // Workgroup barrier with no memory fence: this wavefront's LDS operations
 // complete (lgkmcnt 0), then s_barrier; the empty asm statements keep the
 // compiler from moving memory operations across it.  LDS-DMA completion is
-// waited for explicitly before it (res_wait).
+// waited for explicitly before it (VM_WAIT_VMCNT).
 __device__ __forceinline__ void res_barrier()
 {
     __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 << 14));
