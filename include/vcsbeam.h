@@ -1096,6 +1096,14 @@ void vmInvertPFBDeviceVDIF(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int f
  * callers that hold several channels on one device. */
 void vmInvertPFBDeviceVDIFBatch(vcsbeam_context **vm, struct gpu_ipfb_arrays **g, int nctx,
                                 int file_no, uint8_t **d_vdif);
+/* Passes per workgroup of the device inverse PFB with 12 taps and two
+ * polarisations (1 to 64; 0, the default, chooses from the channel and
+ * sample counts).  A workgroup walks a run of passes along one channel's
+ * samples: the first gives 21 output samples, every later one 32.  The
+ * outputs do not depend on it.  Process-wide.  No counterpart in the
+ * reference (its inverse PFB has no such tiling, src/pfb.cpp:842-907): for
+ * tests and tuning. */
+void vmSetInversePFBPasses(int npass);
 
 /* Device VDIF requantisation of n floats (float2int8_trunc + offset binary). */
 void vmRequantVDIFDevice(const float *d_in, uint8_t *d_out, size_t n,

@@ -424,6 +424,7 @@ _SIGS = {
     "vmSetKernelTiming": (None, [_CTX, C.c_int]),
     "vmTuneBeamformer": (C.c_int, [_CTX, C.c_int, C.POINTER(C.c_float)]),
     "vmSetBeamformerVariant": (None, [_CTX, C.c_int, C.c_int]),
+    "vmSetInversePFBPasses": (None, [C.c_int]),
     "vmBeamformerSelection": (C.c_int, [_CTX, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vmBeamformerVariantInfo": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int)]),

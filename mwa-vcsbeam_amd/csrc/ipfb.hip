@@ -73,7 +73,8 @@ __device__ __forceinline__ uint8_t vdif_q1(float v)
 // requantised for VDIF at the same [n][pol][re, im] positions).
 // The inverse DFTs of the nrows rows in Z, by the 256 threads of one
 // polarisation (every thread of the workgroup passes the same barriers).
-__device__ __forceinline__ void ipfb_idft(cf *Z, int nrows)
+// tw: the twiddle table (g_tw128i, or a copy of it in LDS).
+__device__ __forceinline__ void ipfb_idft(cf *Z, int nrows, const cf *tw = g_tw128i)
 {
     const int t = threadIdx.x & 255;
     // inverse DFTs, 32 rows per pass
@@ -82,7 +83,7 @@ __device__ __forceinline__ void ipfb_idft(cf *Z, int nrows)
         cf *row = &Z[min(base + f, nrows - 1) * ROW];
         // rows past nrows duplicate the last row's work but never store
         cf keep[2][8];
-        vmfft::fft_stage1(row, l, g_tw128i);
+        vmfft::fft_stage1(row, l, tw);
         vmfft::fft_stage2(row, l, keep);
         __syncthreads();
         if (base + f < nrows) {
@@ -213,8 +214,10 @@ __device__ __forceinline__ int xcd_contiguous(int L, int n)
 }
 
 // BATCH: the job is jobs[y] (one coarse channel per grid row), otherwise
-// job1.  (x, y): the tile after the XCD renumbering.
-template <int PC, bool BATCH>
+// job1.  (x, y): the tile after the XCD renumbering.  The general form (any
+// tap count up to MAXP, one or two polarisations); k_ipfb_run takes the
+// 12-tap, two-polarisation case.
+template <bool BATCH>
 __global__ __launch_bounds__(512) void k_ipfb_dev(ipfb_job job1, const ipfb_job *__restrict__ jobs,
                                                   int ns, int P, int npol, int first)
 {
@@ -274,56 +277,152 @@ __global__ __launch_bounds__(512) void k_ipfb_dev(ipfb_job job1, const ipfb_job 
     }
     __syncthreads();
     const int pol = t >> 8;
-    if (PC == 0 || npol != 2) {
-        // each 256-thread half: one polarisation's inverse DFTs and FIR (both
-        // halves pass the same barriers inside ipfb_rows)
-        ipfb_rows<PC>(Z[pol], nrows, P, jmax, j0, pol, npol, hK, out, out8);
-        return;
+    // each 256-thread half: one polarisation's inverse DFTs and FIR (both
+    // halves pass the same barriers inside ipfb_rows)
+    ipfb_rows<0>(Z[pol], nrows, P, jmax, j0, pol, npol, hK, out, out8);
+}
+
+// Streaming form for 12 taps and two polarisations (the MWA synthesis
+// filter).  A workgroup walks a run of npass consecutive passes of one coarse
+// channel.  Its first pass is the tile of k_ipfb_dev: 32 rows, the 11 rows
+// before its first output and 21 outputs.  Every later pass loads the next 32
+// rows, inverse-transforms them and gives 32 outputs, the 11 rows before them
+// carried in registers from the pass before.  So every row after a run's
+// first 11 is transformed once, where k_ipfb_dev transforms 11 of every 32
+// twice.  The inverse DFTs and the FIR sums are those of k_ipfb_dev, term for
+// term, so both give the same bits.
+#define IPFB_RUN_P 12
+#define IPFB_RUN_T0 (IPFB_DEV_ROWS - (IPFB_RUN_P - 1)) // outputs of a run's first pass
+#define IPFB_MAX_PASSES 64
+
+// The job's pointers are generic; accesses through GLOBAL(T, p) are global
+// memory instructions, which (unlike flat ones) do not count against the LDS
+// wait counter, so waiting for LDS reads does not wait for them.
+#define GLOBAL(T, p) ((__attribute__((address_space(1))) T *)(p))
+typedef float vf4 __attribute__((ext_vector_type(4))); // native vectors: no
+typedef float vf2 __attribute__((ext_vector_type(2))); // generic-reference
+typedef uint8_t vu2 __attribute__((ext_vector_type(2))); // copy operators
+
+// Rows s_in = s0 + row (row < 32) of both polarisations into v: one
+// (row, k) pair per lane and u, lanes along the samples of a fine channel
+// (16-byte loads of e[k][s][pol]); history rows from hist_in (zeros when
+// first), rows past ns + P - 1 zeros.
+__device__ __forceinline__ void ipfb_run_load(float4 v[8], const cf *__restrict__ e,
+                                              const cf *__restrict__ hist_in, int first, int s0,
+                                              int ns, int t)
+{
+    constexpr int P = IPFB_RUN_P;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int idx = t + 512 * u;
+        const int row = idx & (IPFB_DEV_ROWS - 1), k = idx / IPFB_DEV_ROWS;
+        const int s_in = s0 + row;
+        const cf *src = s_in >= ns + P ? nullptr
+                        : s_in < P     ? (first ? nullptr : hist_in + ((size_t)s_in * KCH + k) * 2)
+                                       : e + ((size_t)k * ns + (s_in - P)) * 2;
+        vf4 x = {0.f, 0.f, 0.f, 0.f};
+        if (src) x = *GLOBAL(const vf4, src);
+        v[u] = make_float4(x.x, x.y, x.z, x.w);
     }
-    // each half: one polarisation's inverse DFTs (ipfb_idft ends on a barrier)
-    ipfb_idft(Z[pol], nrows);
-    // the FIR of both polarisations per thread: four groups of 128 lanes,
-    // each a contiguous run of PER outputs from a window of PER + P - 1 rows
-    // held in registers, so that each LDS row is read once per thread and
-    // each lane stores its n's two polarisations (16 B, 4 B VDIF) together
-    constexpr int PQ = PC ? PC : 1;
-    constexpr int TJC = IPFB_DEV_ROWS - (PQ - 1);
-    constexpr int PER = (TJC + 3) / 4;
-    constexpr int WL = PER + PQ - 1;
-    const int r = t & 127;
-    const int ja = (t >> 7) * PER;
-    float hr[PQ];
+}
+
+// v (loaded by ipfb_run_load for the same s0 and t) as conj(X) into Z, and
+// the second's last P rows into hist_out ([P][k][pol]) for the next second.
+__device__ __forceinline__ void ipfb_run_put(const float4 v[8], cf (*Z)[IPFB_DEV_ROWS * ROW],
+                                             cf *__restrict__ hist_out, int s0, int ns, int t)
+{
+    constexpr int P = IPFB_RUN_P;
 #pragma unroll
-    for (int q = 0; q < PQ; q++) hr[q] = hK[q * KCH + r];
-    cf w0[WL], w1[WL];
-#pragma unroll
-    for (int i = 0; i < WL; i++) {
-        const int row = ja + i;
-        w0[i] = w1[i] = cf{0.f, 0.f};
-        if (row < nrows) {
-            w0[i] = Z[0][row * ROW + r];
-            w1[i] = Z[1][row * ROW + r];
-        }
+    for (int u = 0; u < 8; u++) {
+        const int idx = t + 512 * u;
+        const int row = idx & (IPFB_DEV_ROWS - 1), k = idx / IPFB_DEV_ROWS;
+        const int s_in = s0 + row;
+        if (s_in >= ns && s_in < ns + P)
+            *GLOBAL(vf4, &hist_out[((size_t)(s_in - ns) * KCH + k) * 2]) =
+                vf4{v[u].x, v[u].y, v[u].z, v[u].w};
+        Z[0][row * ROW + k] = cf{v[u].x, -v[u].y};
+        Z[1][row * ROW + k] = cf{v[u].z, -v[u].w};
     }
-#pragma unroll
-    for (int o = 0; o < PER; o++) {
-        const int jj = ja + o;
-        if (jj >= jmax) break;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int q = 0; q < PQ; q++) {
-            const cf z0 = w0[o + PQ - 1 - q], z1 = w1[o + PQ - 1 - q];
-            a.x = fmaf(hr[q], z0.x, a.x);
-            a.y = fmaf(hr[q], z0.y, a.y);
-            a.z = fmaf(hr[q], z1.x, a.z);
-            a.w = fmaf(hr[q], z1.y, a.w);
+}
+
+// grid (runs, jobs), block 512; a run is IPFB_RUN_T0 + 32 (npass - 1)
+// outputs.  The FIR lanes are (r, pol) pairs, pol fastest, so a wavefront's
+// stores (8-byte complex floats, 2 VDIF bytes) are contiguous runs; the
+// first four wavefronts take a pass's outputs 0-15, the other four 16-31.
+template <bool BATCH>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_ipfb_run(ipfb_job job1, const ipfb_job *__restrict__ jobs,
+                                                  int ns, int npass, int first)
+{
+    constexpr int P = IPFB_RUN_P, H = P - 1, NR = IPFB_DEV_ROWS, T0 = IPFB_RUN_T0;
+    __shared__ cf Z[2][NR * ROW];
+    __shared__ cf TW[KCH];
+    __shared__ float HK[P * KCH];
+    const int tile = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int bx = tile % gridDim.x, by = tile / gridDim.x;
+    const ipfb_job job = BATCH ? jobs[by] : job1;
+    const int t = threadIdx.x;
+    // twiddles and taps in LDS (read after the first pass's first barrier):
+    // the taps are held in registers during the FIR only
+    if (t < KCH) TW[t] = g_tw128i[t];
+    for (int i = t; i < P * KCH; i += 512) HK[i] = *GLOBAL(const float, job.hK + i);
+    const int fpol = t & 1, r = (t >> 1) & (KCH - 1), g = t >> 8;
+    const int J0 = bx * (T0 + NR * (npass - 1)); // the run's first output
+    cf carry[H];
+    for (int k = 0; k < npass; k++) {
+        const int jb = k == 0 ? J0 : J0 + T0 + NR * (k - 1); // the pass's first output
+        if (jb >= ns) break;
+        // Z row 0 is s_in = s0; a pass's outputs jb + o take s_in jb + o + 1 .. jb + o + P
+        const int s0 = k == 0 ? jb + 1 : jb + P;
+        {
+            float4 v[8];
+            ipfb_run_load(v, job.e, job.hist_in, first, s0, ns, t);
+            ipfb_run_put(v, Z, job.hist_out, s0, ns, t);
         }
-        const size_t n = (size_t)(j0 + jj) * KCH + r;
-        *(float4 *)&out[n * 4] = a;
-        if (out8)
-            *(uint32_t *)&out8[n * 4] = (uint32_t)vdif_q1(a.x) | ((uint32_t)vdif_q1(a.y) << 8) |
-                                        ((uint32_t)vdif_q1(a.z) << 16) |
-                                        ((uint32_t)vdif_q1(a.w) << 24);
+        __syncthreads();
+        ipfb_idft(Z[t >> 8], NR, TW); // ends on a barrier
+        // output o takes rows o .. o + H of the pass's combined rows: the H
+        // rows before its first output, then its new rows; combined row c is
+        // Z row c + zb (the first pass: all 32 rows are in Z)
+        const int nnew = k == 0 ? T0 : NR;
+        const int zb = NR - nnew - H;
+        const int jlim = min(nnew, ns - jb);
+        const cf *Zp = Z[fpol] + r;
+        if (16 * g < jlim) {
+            float hr[P];
+#pragma unroll
+            for (int q = 0; q < P; q++) hr[q] = HK[q * KCH + r];
+            cf w[P]; // w[i]: combined row o + i
+            if (g == 0 && k > 0) {
+#pragma unroll
+                for (int i = 0; i < H; i++) w[i] = carry[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < H; i++) w[i] = Zp[(16 * g + i + zb) * ROW];
+            }
+#pragma unroll
+            for (int oo = 0; oo < 16; oo++) {
+                const int o = 16 * g + oo;
+                if (o >= jlim) break;
+                w[H] = Zp[(o + H + zb) * ROW];
+                float ar = 0.f, ai = 0.f;
+#pragma unroll
+                for (int q = 0; q < P; q++) {
+                    ar = fmaf(hr[q], w[H - q].x, ar);
+                    ai = fmaf(hr[q], w[H - q].y, ai);
+                }
+                const size_t n = ((size_t)(jb + o) * KCH + r) * 2 + fpol;
+                *GLOBAL(vf2, &job.out[n * 2]) = vf2{ar, ai};
+                if (job.out8) *GLOBAL(vu2, &job.out8[n * 2]) = vu2{vdif_q1(ar), vdif_q1(ai)};
+#pragma unroll
+                for (int i = 0; i < H; i++) w[i] = w[i + 1];
+            }
+        }
+        // the next pass's first H combined rows: this pass's last H rows
+        if (g == 0) {
+#pragma unroll
+            for (int i = 0; i < H; i++) carry[i] = Zp[(NR - H + i) * ROW];
+        }
+        __syncthreads();
     }
 }
 
@@ -578,26 +677,73 @@ static void ipfb_job_for(vcsbeam_context *vm, struct gpu_ipfb_arrays *g, int fil
     job.out8 = d_vdif;
 }
 
+// Passes per k_ipfb_run workgroup: 0 chooses them (ipfb_run_passes), else
+// vmSetInversePFBPasses's value.
+static int g_ipfb_npass = 0;
+
+void vmSetInversePFBPasses(int npass)
+{
+    if (npass < 0 || npass > IPFB_MAX_PASSES)
+        VM_FATAL("vmSetInversePFBPasses: %d passes (0 to %d)", npass, IPFB_MAX_PASSES);
+    g_ipfb_npass = npass;
+}
+
+// Passes per run for njob channels of ns outputs when slots workgroups are
+// resident at once: the fewest that minimise (rounds of resident
+// workgroups) x (passes per workgroup), the longest chain of passes one slot
+// runs.  A run's first pass gives 21 outputs, every later one 32, so long
+// runs do less work per output, while short runs balance better over the
+// slots (24 channels of 10000 samples on 512 slots: 8 passes, 41 runs per
+// channel, 984 workgroups in two rounds).
+static int ipfb_run_passes(int ns, int njob, int slots)
+{
+    int best = 1;
+    long bestc = -1;
+    for (int np = 1; np <= IPFB_MAX_PASSES; np++) {
+        const long span = IPFB_RUN_T0 + (long)IPFB_DEV_ROWS * (np - 1);
+        const long nwg = (long)njob * ((ns + span - 1) / span);
+        const long c = ((nwg + slots - 1) / slots) * np;
+        if (bestc < 0 || c < bestc) {
+            bestc = c;
+            best = np;
+        }
+    }
+    return best;
+}
+
 // Launch over njob jobs (njob 1: the job by value; more: jobs_dev[y]).
 static void ipfb_launch(const ipfb_job &job1, const ipfb_job *jobs_dev, int njob, int ns, int P,
                         int npol, int file_no, hipStream_t s)
 {
-    const int tj = IPFB_DEV_ROWS - (P - 1);
-    const dim3 grid((unsigned)((ns + tj - 1) / tj), (unsigned)njob);
     const int first = file_no == 0 ? 1 : 0;
-    // the MWA synthesis filters have 12 taps: that count is compiled in
-    if (njob == 1 && P == 12)
-        hipLaunchKernelGGL((k_ipfb_dev<12, false>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
-                           ns, P, npol, first);
-    else if (njob == 1)
-        hipLaunchKernelGGL((k_ipfb_dev<0, false>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
-                           ns, P, npol, first);
-    else if (P == 12)
-        hipLaunchKernelGGL((k_ipfb_dev<12, true>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
-                           ns, P, npol, first);
-    else
-        hipLaunchKernelGGL((k_ipfb_dev<0, true>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
-                           ns, P, npol, first);
+    if (P == IPFB_RUN_P && npol == 2) {
+        static int slots = 0;
+        if (!slots) {
+            int dev = 0, ncu = 0, bpc = 0;
+            HIPCHECK(hipGetDevice(&dev));
+            HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_ipfb_run<true>, 512, 0));
+            slots = ncu * (bpc < 1 ? 1 : bpc);
+        }
+        const int npass = g_ipfb_npass ? g_ipfb_npass : ipfb_run_passes(ns, njob, slots);
+        const int span = IPFB_RUN_T0 + IPFB_DEV_ROWS * (npass - 1);
+        const dim3 grid((unsigned)((ns + span - 1) / span), (unsigned)njob);
+        if (njob == 1)
+            hipLaunchKernelGGL((k_ipfb_run<false>), grid, dim3(512), 0, s, job1, jobs_dev, ns,
+                               npass, first);
+        else
+            hipLaunchKernelGGL((k_ipfb_run<true>), grid, dim3(512), 0, s, job1, jobs_dev, ns,
+                               npass, first);
+    } else {
+        const int tj = IPFB_DEV_ROWS - (P - 1);
+        const dim3 grid((unsigned)((ns + tj - 1) / tj), (unsigned)njob);
+        if (njob == 1)
+            hipLaunchKernelGGL((k_ipfb_dev<false>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
+                               ns, P, npol, first);
+        else
+            hipLaunchKernelGGL((k_ipfb_dev<true>), grid, dim3(256 * npol), 0, s, job1, jobs_dev,
+                               ns, P, npol, first);
+    }
     HIPCHECK(hipGetLastError());
 }
 

@@ -121,16 +121,21 @@ def test_vdif_write_second_file(pkg, orc, gpu_lib, tmp_path):
     assert (tmp_path / "beam.hdr").stat().st_size == 4096
 
 
-def test_ipfb_device_vdif_fused(pkg, orc, gpu_lib):
+@pytest.mark.parametrize("npass", [0, 3])
+def test_ipfb_device_vdif_fused(pkg, orc, gpu_lib, npass):
     """vmInvertPFBDeviceVDIF over three seconds in order: its floats are
     vmInvertPFBDevice's (same kernel, the history carried across seconds in
     two buffers used in turn) and within the tolerance of the direct-form
     oracle on the ring of the seconds' e, and its bytes are the oracle's VDIF
-    requantisation of those floats, bit for bit."""
+    requantisation of those floats, bit for bit.  npass 0: the passes per
+    workgroup the library chooses (one, for one channel of 1000 samples);
+    3: runs of three passes, so most rows come after a carried history."""
     import ctypes as C
     from vcsbeam_amd.apps import TiedArraySession
     from vcsbeam_amd.workloads import Observation
     lib = gpu_lib
+    print(f"passes per workgroup: {npass}", flush=True)
+    lib.vmSetInversePFBPasses(npass)
     obs = Observation(ntiles=32, blocks_per_second=2, nseconds=3)
     az, za = np.array([0.9]), np.array([0.35])
     ses = TiedArraySession(obs, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, nstokes=1,
@@ -164,6 +169,7 @@ def test_ipfb_device_vdif_fused(pkg, orc, gpu_lib):
             assert err <= TOL
     finally:
         ses.close()
+        lib.vmSetInversePFBPasses(0)
 
 
 def test_ipfb_device_vdif_batch_equals_per_channel(pkg, gpu_lib):
@@ -213,4 +219,55 @@ def test_ipfb_device_vdif_batch_equals_per_channel(pkg, gpu_lib):
             assert not np.array_equal(f0, f1)
     finally:
         for x in one + many:
+            x.close()
+
+
+def test_ipfb_run_length_does_not_change_bits(pkg, gpu_lib):
+    """The batched inverse PFB of three coarse channels over two seconds with
+    1, 2, 5 and 16 passes per workgroup (runs of 21, 53, 149 and 501 output
+    samples against 1000 per second, the last run of each channel partial):
+    floats and VDIF bytes equal bit for bit.  A row carried wrongly from one
+    pass to the next would change the outputs inside runs but not at their
+    starts, so the run lengths would disagree."""
+    from vcsbeam_amd.apps import TiedArraySession, invert_batch
+    from vcsbeam_amd.workloads import Observation
+    lib = gpu_lib
+    az, za = np.array([0.6]), np.array([0.25])
+    obs = [Observation(ntiles=16, blocks_per_second=2, nseconds=2, rec_chan=110 + g,
+                       seed_offset=50 * g) for g in range(3)]
+    sess = [TiedArraySession(o, az, za, pfb_flags=pkg.PFB_FULL_PRECISION, nstokes=1, ipfb=True)
+            for o in obs]
+    ns = obs[0].nspectra
+    n = ns * K * 2 * 2
+    try:
+        for t in range(2):
+            for x in sess:
+                x.read(t)
+                x.pfb()
+                x.geometry()
+                x.beamform()
+            first = None
+            for npass in (1, 2, 5, 16):
+                print(f"second {t}: {npass} passes per workgroup", flush=True)
+                lib.vmSetInversePFBPasses(npass)
+                # second t reads the history buffer second t - 1 wrote and
+                # writes the other, so repeating it gives the same inputs
+                invert_batch(sess, t)
+                outs = []
+                for x in sess:
+                    f = np.empty(n, np.float32)
+                    lib.vmMemcpyDtoH(f.ctypes.data, x.gi.d_out, f.nbytes)
+                    q = np.empty(n, np.uint8)
+                    lib.vmMemcpyDtoH(q.ctypes.data, x.d_vdif, q.nbytes)
+                    outs.append((f, q))
+                if first is None:
+                    first = outs
+                    assert all(np.abs(f).max() > 0 for f, _ in outs)
+                    continue
+                for (f0, q0), (f1, q1) in zip(first, outs):
+                    np.testing.assert_array_equal(f0, f1)
+                    np.testing.assert_array_equal(q0, q1)
+    finally:
+        lib.vmSetInversePFBPasses(0)
+        for x in sess:
             x.close()
