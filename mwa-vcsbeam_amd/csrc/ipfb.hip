@@ -368,18 +368,22 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int fpol = t & 1, r = (t >> 1) & (KCH - 1), g = t >> 8;
     const int J0 = bx * (T0 + NR * (npass - 1)); // the run's first output
     cf carry[H];
+    float4 v[8];
+    ipfb_run_load(v, job.e, job.hist_in, first, J0 + 1, ns, t);
     for (int k = 0; k < npass; k++) {
         const int jb = k == 0 ? J0 : J0 + T0 + NR * (k - 1); // the pass's first output
         if (jb >= ns) break;
         // Z row 0 is s_in = s0; a pass's outputs jb + o take s_in jb + o + 1 .. jb + o + P
         const int s0 = k == 0 ? jb + 1 : jb + P;
-        {
-            float4 v[8];
-            ipfb_run_load(v, job.e, job.hist_in, first, s0, ns, t);
-            ipfb_run_put(v, Z, job.hist_out, s0, ns, t);
-        }
+        ipfb_run_put(v, Z, job.hist_out, s0, ns, t);
         __syncthreads();
         ipfb_idft(Z[t >> 8], NR, TW); // ends on a barrier
+        // the next pass's rows are requested after the FIR (while the FIR's
+        // registers are free) and are in flight across the pass's last
+        // barrier; rows from ns + P on read as zeros without a memory
+        // access, so after a run's last pass nothing is requested
+        const int jn = J0 + T0 + NR * k; // the next pass's first output
+        const int sn = k + 1 < npass && jn < ns ? jn + P : ns + P;
         // output o takes rows o .. o + H of the pass's combined rows: the H
         // rows before its first output, then its new rows; combined row c is
         // Z row c + zb (the first pass: all 32 rows are in Z)
@@ -422,6 +426,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
             for (int i = 0; i < H; i++) carry[i] = Zp[(NR - H + i) * ROW];
         }
+        ipfb_run_load(v, job.e, job.hist_in, first, sn, ns, t);
         __syncthreads();
     }
 }
