@@ -348,7 +348,8 @@ __device__ __forceinline__ void ipfb_run_put(const float4 v[8], cf (*Z)[IPFB_DEV
 // grid (runs, jobs), block 512; a run is IPFB_RUN_T0 + 32 (npass - 1)
 // outputs.  The FIR lanes are (r, pol) pairs, pol fastest, so a wavefront's
 // stores (8-byte complex floats, 2 VDIF bytes) are contiguous runs; the
-// first four wavefronts take a pass's outputs 0-15, the other four 16-31.
+// first four wavefronts take the first half of a pass's outputs, the other
+// four the second half.
 template <bool BATCH>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_ipfb_run(ipfb_job job1, const ipfb_job *__restrict__ jobs,
                                                   int ns, int npass, int first)
@@ -391,7 +392,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int zb = NR - nnew - H;
         const int jlim = min(nnew, ns - jb);
         const cf *Zp = Z[fpol] + r;
-        if (16 * g < jlim) {
+        // the first four wavefronts take outputs [0, oh), the others
+        // [oh, nnew): 16 and 16, or 11 and 10 in a run's first pass
+        const int oh = (nnew + 1) >> 1;
+        const int o0 = g ? oh : 0, o1 = min(g ? nnew : oh, jlim);
+        if (o0 < o1) {
             float hr[P];
 #pragma unroll
             for (int q = 0; q < P; q++) hr[q] = HK[q * KCH + r];
@@ -401,12 +406,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
                 for (int i = 0; i < H; i++) w[i] = carry[i];
             } else {
 #pragma unroll
-                for (int i = 0; i < H; i++) w[i] = Zp[(16 * g + i + zb) * ROW];
+                for (int i = 0; i < H; i++) w[i] = Zp[(o0 + i + zb) * ROW];
             }
 #pragma unroll
             for (int oo = 0; oo < 16; oo++) {
-                const int o = 16 * g + oo;
-                if (o >= jlim) break;
+                const int o = o0 + oo;
+                if (o >= o1) break;
                 w[H] = Zp[(o + H + zb) * ROW];
                 float ar = 0.f, ai = 0.f;
 #pragma unroll
