@@ -303,13 +303,28 @@ typedef float vf4 __attribute__((ext_vector_type(4))); // native vectors: no
 typedef float vf2 __attribute__((ext_vector_type(2))); // generic-reference
 typedef uint8_t vu2 __attribute__((ext_vector_type(2))); // copy operators
 
+// Raw buffer accesses: 32-bit offsets from a base held in scalar registers
+// (fewer vector registers and no 64-bit address arithmetic per access), and
+// an offset past the buffer's size reads zeros.
+typedef __amdgpu_buffer_rsrc_t ipfb_rsrc_t;
+typedef uint32_t ipfb_v4u __attribute__((ext_vector_type(4)));
+typedef uint32_t ipfb_v2u __attribute__((ext_vector_type(2)));
+#define IPFB_NO_ROW 0x7ffffff0u // an offset past every buffer here
+__device__ __forceinline__ ipfb_rsrc_t ipfb_rsrc(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000 /* raw, 32-bit format */);
+}
+
 // Rows s_in = s0 + row (row < 32) of both polarisations into v: one
 // (row, k) pair per lane and u, lanes along the samples of a fine channel
-// (16-byte loads of e[k][s][pol]); history rows from hist_in (zeros when
-// first), rows past ns + P - 1 zeros.
-__device__ __forceinline__ void ipfb_run_load(float4 v[8], const cf *__restrict__ e,
-                                              const cf *__restrict__ hist_in, int first, int s0,
-                                              int ns, int t)
+// (16-byte loads of e[k][s][pol] through re, the channel's e); history rows
+// s_in < P from hist_in ([P][k][pol], through rh; zeros when first), rows
+// past ns + P - 1 zeros.  Only a pass with s0 < P (a run at the second's
+// start) reads history rows: it loads from both buffers, each lane's other
+// offset out of range (zeros), and ORs the bits.
+__device__ __forceinline__ void ipfb_run_load(float4 v[8], ipfb_rsrc_t re, ipfb_rsrc_t rh,
+                                              int first, int s0, int ns, int t)
 {
     constexpr int P = IPFB_RUN_P;
 #pragma unroll
@@ -317,12 +332,17 @@ __device__ __forceinline__ void ipfb_run_load(float4 v[8], const cf *__restrict_
         const int idx = t + 512 * u;
         const int row = idx & (IPFB_DEV_ROWS - 1), k = idx / IPFB_DEV_ROWS;
         const int s_in = s0 + row;
-        const cf *src = s_in >= ns + P ? nullptr
-                        : s_in < P     ? (first ? nullptr : hist_in + ((size_t)s_in * KCH + k) * 2)
-                                       : e + ((size_t)k * ns + (s_in - P)) * 2;
-        vf4 x = {0.f, 0.f, 0.f, 0.f};
-        if (src) x = *GLOBAL(const vf4, src);
-        v[u] = make_float4(x.x, x.y, x.z, x.w);
+        const uint32_t oe = s_in >= P && s_in < ns + P
+                                ? ((uint32_t)k * (uint32_t)ns + (uint32_t)(s_in - P)) * 16u
+                                : IPFB_NO_ROW;
+        ipfb_v4u x = __builtin_amdgcn_raw_buffer_load_b128(re, (int)oe, 0, 0);
+        if (s0 < P) {
+            const uint32_t oh =
+                s_in < P && !first ? ((uint32_t)s_in * KCH + (uint32_t)k) * 16u : IPFB_NO_ROW;
+            x |= __builtin_amdgcn_raw_buffer_load_b128(rh, (int)oh, 0, 0);
+        }
+        const vf4 f = __builtin_bit_cast(vf4, x);
+        v[u] = make_float4(f.x, f.y, f.z, f.w);
     }
 }
 
@@ -368,9 +388,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int i = t; i < P * KCH; i += 512) HK[i] = *GLOBAL(const float, job.hK + i);
     const int fpol = t & 1, r = (t >> 1) & (KCH - 1), g = t >> 8;
     const int J0 = bx * (T0 + NR * (npass - 1)); // the run's first output
+    // e, out and out8 of one channel are under 2 GB (checked at launch)
+    const ipfb_rsrc_t re = ipfb_rsrc(job.e, (uint32_t)ns * KCH * 16u);
+    const ipfb_rsrc_t rh = ipfb_rsrc(job.hist_in, (uint32_t)P * KCH * 16u);
+    const ipfb_rsrc_t ro = ipfb_rsrc(job.out, (uint32_t)ns * KCH * 16u);
+    const ipfb_rsrc_t ro8 = ipfb_rsrc(job.out8, (uint32_t)ns * KCH * 4u);
     cf carry[H];
     float4 v[8];
-    ipfb_run_load(v, job.e, job.hist_in, first, J0 + 1, ns, t);
+    ipfb_run_load(v, re, rh, first, J0 + 1, ns, t);
     for (int k = 0; k < npass; k++) {
         const int jb = k == 0 ? J0 : J0 + T0 + NR * (k - 1); // the pass's first output
         if (jb >= ns) break;
@@ -419,9 +444,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
                     ar = fmaf(hr[q], w[H - q].x, ar);
                     ai = fmaf(hr[q], w[H - q].y, ai);
                 }
-                const size_t n = ((size_t)(jb + o) * KCH + r) * 2 + fpol;
-                *GLOBAL(vf2, &job.out[n * 2]) = vf2{ar, ai};
-                if (job.out8) *GLOBAL(vu2, &job.out8[n * 2]) = vu2{vdif_q1(ar), vdif_q1(ai)};
+                const uint32_t n = ((uint32_t)(jb + o) * KCH + (uint32_t)r) * 2u + (uint32_t)fpol;
+                __builtin_amdgcn_raw_buffer_store_b64(
+                    __builtin_bit_cast(ipfb_v2u, vf2{ar, ai}), ro, (int)(n * 8u), 0, 0);
+                if (job.out8)
+                    __builtin_amdgcn_raw_buffer_store_b16(
+                        (uint16_t)((uint16_t)vdif_q1(ar) | ((uint16_t)vdif_q1(ai) << 8)), ro8,
+                        (int)(n * 2u), 0, 0);
 #pragma unroll
                 for (int i = 0; i < H; i++) w[i] = w[i + 1];
             }
@@ -431,7 +460,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
             for (int i = 0; i < H; i++) carry[i] = Zp[(NR - H + i) * ROW];
         }
-        ipfb_run_load(v, job.e, job.hist_in, first, sn, ns, t);
+        ipfb_run_load(v, re, rh, first, sn, ns, t);
         __syncthreads();
     }
 }
@@ -727,6 +756,9 @@ static void ipfb_launch(const ipfb_job &job1, const ipfb_job *jobs_dev, int njob
 {
     const int first = file_no == 0 ? 1 : 0;
     if (P == IPFB_RUN_P && npol == 2) {
+        // k_ipfb_run addresses one channel's e and outputs with 32-bit offsets
+        if ((size_t)ns * KCH * 16 >= (size_t)IPFB_NO_ROW)
+            VM_FATAL("inverse PFB: %d samples per second exceed the 2 GB a channel may take", ns);
         static int slots = 0;
         if (!slots) {
             int dev = 0, ncu = 0, bpc = 0;
