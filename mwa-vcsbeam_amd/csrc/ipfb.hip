@@ -300,8 +300,7 @@ __global__ __launch_bounds__(512) void k_ipfb_dev(ipfb_job job1, const ipfb_job 
 // wait counter, so waiting for LDS reads does not wait for them.
 #define GLOBAL(T, p) ((__attribute__((address_space(1))) T *)(p))
 typedef float vf4 __attribute__((ext_vector_type(4))); // native vectors: no
-typedef float vf2 __attribute__((ext_vector_type(2))); // generic-reference
-typedef uint8_t vu2 __attribute__((ext_vector_type(2))); // copy operators
+typedef float vf2 __attribute__((ext_vector_type(2))); // generic-reference copy operators
 
 // Raw buffer accesses: 32-bit offsets from a base held in scalar registers
 // (fewer vector registers and no 64-bit address arithmetic per access), and
