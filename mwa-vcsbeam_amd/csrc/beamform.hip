@@ -654,12 +654,13 @@ void vmApplyJChunk(vcsbeam_context *vm)
     HIPCHECK(hipStreamSynchronize(vm_stream(vm)));
 }
 
-void vm_beamform_fp32_launch(const void *d_v, vcsbeam_datatype dt, int ns, int nchan, int ni,
-                             const void *d_W, const uint32_t *d_polQ, const uint32_t *d_polP,
-                             int nant, int npointing, float invw, int nstokes, int soffset,
-                             int ns_total, float *d_S, void *d_e, hipStream_t stream)
+extern "C++" template <int PB>
+static void beamform_fp32_launch_pb(const void *d_v, vcsbeam_datatype dt, int ns, int nchan,
+                                    int ni, const void *d_W, const uint32_t *d_polQ,
+                                    const uint32_t *d_polP, int nant, int npointing, float invw,
+                                    int nstokes, int soffset, int ns_total, float *d_S, void *d_e,
+                                    hipStream_t stream)
 {
-    const int PB = 4;
     dim3 grid((ns + 255) / 256, nchan, (npointing + PB - 1) / PB), block(256);
     if (dt == VM_INT4) {
         size_t lds = (size_t)256 * ni;
@@ -676,6 +677,23 @@ void vm_beamform_fp32_launch(const void *d_v, vcsbeam_datatype dt, int ns, int n
                  "from PFB_COMPLEX_FLOAT16, is the MFMA forms' input)", (int)dt);
     }
     HIPCHECK(hipGetLastError());
+}
+
+// Four pointings per thread share each voltage load; a single pointing takes
+// the one-pointing form, which would otherwise compute the same pointing four
+// times (the kernel clamps pointings past the last).  Each pointing's
+// arithmetic is the same in both forms.
+void vm_beamform_fp32_launch(const void *d_v, vcsbeam_datatype dt, int ns, int nchan, int ni,
+                             const void *d_W, const uint32_t *d_polQ, const uint32_t *d_polP,
+                             int nant, int npointing, float invw, int nstokes, int soffset,
+                             int ns_total, float *d_S, void *d_e, hipStream_t stream)
+{
+    if (npointing == 1)
+        beamform_fp32_launch_pb<1>(d_v, dt, ns, nchan, ni, d_W, d_polQ, d_polP, nant, npointing,
+                                   invw, nstokes, soffset, ns_total, d_S, d_e, stream);
+    else
+        beamform_fp32_launch_pb<4>(d_v, dt, ns, nchan, ni, d_W, d_polQ, d_polP, nant, npointing,
+                                   invw, nstokes, soffset, ns_total, d_S, d_e, stream);
 }
 
 void vmBeamformChunk(vcsbeam_context *vm)
